@@ -1,0 +1,220 @@
+"""Benchmark of the two-site DMRG effective-Hamiltonian hot path on one GPU.
+
+Workload: the batched H*psi microbenchmark -- one middle-site effective
+Hamiltonian of the n=40-orbital, M=3000, SU(2) singlet problem with its
+sector-size profile reproduced from a seed (about 170 wavefunction blocks,
+block dimensions 1..600, heavy tailed), 4 right-hand sides (state-averaged
+roots) per H*psi.  A step is one H*psi over the 4 vectors (FP64 MFMA),
+with every input already resident in HBM.
+
+Printed JSON (one line): whole-job FP64 H*psi throughput in TFLOP/s
+(FLOPs = 2 * the reference's multiply-add count of the recorded pair list,
+src/core/batch_gemm.hpp:307), the roofline of the dominant kernel measured
+with HIP events on the library's stream, the FP32 throughput, a Davidson
+solve in FP64 and in mixed precision (FP32 matvecs + FP64 refinement) with
+|dE| and 1-|<psi_mixed|psi_fp64>|, and the CPU baseline (the oracle's
+OpenMP restatement timed on a bounded sample of the same pair list).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "block2-preview_amd"))
+
+METRIC = ("H·ψ TFLOP/s + s/sweep; FP32-mixed vs FP64 GPU at "
+          "|ΔE|≤1e-6 Eh")
+FP64_MFMA_PEAK = 78.6   # TFLOP/s, MI355X dense FP64 matrix
+FP32_MFMA_PEAK = 157.3  # TFLOP/s, MI355X dense FP32 matrix
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(pb, ldata, rdata, seconds, threads):
+    """Oracle restatement timed on a bounded prefix of the recorded pairs:
+    a 64-pair calibration run gives the cost per pair, then a prefix sized
+    for about `seconds` of CPU work is timed."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import numpy as np
+    import b2g_oracle
+    pb.lops.data = ldata
+    pb.rops.data = rdata
+    orc = b2g_oracle.Oracle(pb)
+    x = np.random.default_rng(11).standard_normal(pb.ket_len)
+    t0 = time.perf_counter()
+    _, m0 = orc.hpsi_sample(x, 64, threads)
+    dt0 = max(time.perf_counter() - t0, 1e-3)
+    n = int(max(64, min(seconds * (m0 / dt0) / max(m0 / 64.0, 1.0), 1 << 30)))
+    t0 = time.perf_counter()
+    _, macs = orc.hpsi_sample(x, n, threads)
+    dt = time.perf_counter() - t0
+    return dict(value=round(2.0 * macs / dt / 1e12, 6), unit="TFLOP/s",
+                cores=threads, kind="port",
+                sample="first %d recorded pairs of one H*psi (1 RHS, %.3g of "
+                       "%.3g multiply-adds), %.1f s" % (
+                           n, macs, orc.macs, dt))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n-orb", type=int, default=40)
+    ap.add_argument("--m", type=int, default=3000)
+    ap.add_argument("--max-dim", type=int, default=600)
+    ap.add_argument("--pair-scale", type=int, default=4)
+    ap.add_argument("--rhs", type=int, default=4)
+    ap.add_argument("--seed", type=int, default=1234)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-davidson", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: the hot path is a single-GPU "
+              "kernel here; run with --gpus 1", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+    from b2g import synth
+    from b2g.effham import Context, EffectiveHamiltonian
+
+    t0 = time.time()
+    pb = synth.middle_site(n_orb=args.n_orb, m=args.m, max_dim=args.max_dim,
+                           seed=args.seed, pair_scale=args.pair_scale,
+                           data="device")
+    ctx = Context(0)
+    dev = ctx.device
+    ldev, rdev = synth.fill_device(pb, dev)
+    torch.cuda.synchronize()
+    h = EffectiveHamiltonian(pb, ctx, fp32=True, device_data=(ldev, rdev))
+    log("setup %.1fs: |psi| %d, pairs %d, MACs/H*psi %.4g, terms %d" % (
+        time.time() - t0, h.ket_len, h.n_pairs, h.macs, len(pb.term_conj)))
+    pairs = h.pairs()
+    c_m = pairs["c_m"].astype(np.float64)
+    c_n = pairs["c_n"].astype(np.float64)
+    v_m = pairs["v_m"].astype(np.float64)
+    v_n = pairs["v_n"].astype(np.float64)
+    macs1 = float(np.sum(c_m * c_n * v_n))  # work = C op(B)
+    macs2 = float(np.sum(v_m * c_m * v_n))  # V += alpha op(A) work
+    R = args.rhs
+    g = torch.Generator(device=dev).manual_seed(args.seed)
+    x = torch.randn((R, h.ket_len), generator=g, dtype=torch.float64,
+                    device=dev)
+    y = torch.empty_like(x)
+    stream = torch.cuda.ExternalStream(ctx.lib.b2g_ctx_stream(ctx.h))
+
+    def run(prec, steps, timed):
+        for _ in range(args.warmup):
+            y.copy_(h.multiply(x, precision=prec))
+        torch.cuda.synchronize()
+        ctx.synchronize()
+        if timed:
+            h.set_timing(True)
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(steps):
+            code = h.lib.b2g_effham_hpsi(h.h, x.data_ptr(), y.data_ptr(), R,
+                                         1.0, 0, {"f64": 0, "f32": 1}[prec])
+            if code != 0:
+                raise RuntimeError("b2g_effham_hpsi failed: %d" % code)
+        e1.record(stream)
+        e1.synchronize()
+        ms = e0.elapsed_time(e1) / steps
+        ph = h.timing() if timed else None
+        if timed:
+            h.set_timing(False)
+        return ms, ph
+
+    ms64, ph = run("f64", args.steps, True)
+    tflops = 2.0 * h.macs * R / (ms64 * 1e-3) / 1e12
+    ms1, ms2, nl = ph
+    d1, d2 = ms1 / max(nl, 1), ms2 / max(nl, 1)
+    if d2 >= d1:
+        dom, dflop, dur = "phase2 V+=alpha*op(A)*W", 2.0 * macs2 * R, d2
+    else:
+        dom, dflop, dur = "phase1 W=C*op(B)", 2.0 * macs1 * R, d1
+    achieved = dflop / (dur * 1e-3) / 1e12
+    log("FP64: %.3f ms/step, %.2f TFLOP/s; phase1 %.3f ms (%.2f TF) phase2 "
+        "%.3f ms (%.2f TF)" % (ms64, tflops, d1, 2 * macs1 * R / d1 / 1e9, d2,
+                               2 * macs2 * R / d2 / 1e9))
+    ms32, _ = run("f32", max(3, args.steps // 2), False)
+    tflops32 = 2.0 * h.macs * R / (ms32 * 1e-3) / 1e12
+    log("FP32: %.3f ms/step, %.2f TFLOP/s" % (ms32, tflops32))
+
+    extra = {}
+    if not args.no_davidson:
+        # DMRG-like initial guess: the lowest diagonal state plus noise
+        d = h.diag
+        guess = 0.3 * torch.randn(h.ket_len, generator=g, dtype=torch.float64,
+                                  device=dev) / float(np.sqrt(h.ket_len))
+        guess[int(torch.argmin(d))] += 1.0
+        guess /= torch.linalg.norm(guess)
+        x64 = guess.clone()
+        e64, nd64, nf64, t64 = h.eigs(x64, conv_thrd=1e-9)
+        xmx = guess.clone()
+        emx, ndmx, nfmx, tmx = h.eigs(xmx, conv_thrd=1e-9, precision="f32",
+                                      refine=True)
+        ov = float(torch.dot(x64, xmx).abs())
+        extra["davidson"] = dict(
+            conv_thrd=1e-9, e_fp64=e64, ndav_fp64=nd64, s_fp64=round(t64, 4),
+            e_mixed=emx, ndav_mixed=ndmx, s_mixed=round(tmx, 4),
+            abs_dE=abs(emx - e64), infidelity=max(0.0, 1.0 - ov),
+            speedup_mixed=round(t64 / tmx, 3) if tmx > 0 else None)
+        log("Davidson FP64: E=%.12f ndav=%d %.3fs | mixed: E=%.12f ndav=%d "
+            "%.3fs | |dE|=%.2e" % (e64, nd64, t64, emx, ndmx, tmx,
+                                   abs(emx - e64)))
+
+    cpu = None
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline(pb, ldev.cpu().numpy(), rdev.cpu().numpy(),
+                           args.cpu_seconds, threads)
+        log("CPU baseline:", cpu)
+
+    out = {
+        "metric": METRIC,
+        "value": round(tflops, 4),
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms64, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "batched H*psi, middle-site two-site effective "
+                        "Hamiltonian, n=%d orbitals, M=%d, SU(2), %d RHS, "
+                        "seed %d" % (args.n_orb, args.m, R, args.seed),
+            "psi_len": h.ket_len, "psi_blocks": int(len(pb.ket.quanta)),
+            "max_block_dim": args.max_dim, "n_terms": int(len(pb.term_conj)),
+            "n_pairs": h.n_pairs, "macs_per_rhs": h.macs,
+            "parallelism": "single GPU"},
+        "roofline": {"bound": "mfma", "kernel": dom,
+                     "achieved": round(achieved, 4), "peak": FP64_MFMA_PEAK,
+                     "unit": "TFLOP/s",
+                     "frac": round(achieved / FP64_MFMA_PEAK, 4),
+                     "traffic": None},
+        "fp32": {"value": round(tflops32, 4), "unit": "TFLOP/s",
+                 "ms_per_step": round(ms32, 4),
+                 "frac_of_fp32_peak": round(tflops32 / FP32_MFMA_PEAK, 4)},
+        "cpu_baseline": cpu,
+    }
+    out.update(extra)
+    print(json.dumps(out), flush=True)
+    h.deallocate()
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,248 @@
+"""EffectiveHamiltonian on the GPU, mirroring block2's interface.
+
+block2 (src/dmrg/effective_hamiltonian.hpp:99-1162, bound to Python at
+src/pybind/pybind_dmrg.hpp:588-647) exposes precompute(), __call__ /
+operator()(b, c, idx, factor, all_reduce), eigs(...) -> (energy, ndav, nflop,
+tdav) and deallocate().  This class keeps those names and argument meanings;
+vectors are torch float64 tensors on the context's device (PyTorch is only
+the device-memory plumbing: every operation runs in libb2g.so).
+"""
+import ctypes as C
+
+import torch
+
+from . import _lib as L
+
+
+class Context:
+    """A device context (one HIP stream) of libb2g."""
+
+    def __init__(self, device=0):
+        self.lib = L.lib()
+        h = C.c_void_p()
+        L.check(self.lib.b2g_ctx_create(int(device), 0, C.byref(h)))
+        self.h = h
+        self.device = torch.device("cuda", int(device))
+
+    def synchronize(self):
+        L.check(self.lib.b2g_ctx_synchronize(self.h))
+
+    @property
+    def num_cu(self):
+        n = C.c_int()
+        L.check(self.lib.b2g_ctx_num_cu(self.h, C.byref(n)))
+        return n.value
+
+    def close(self):
+        if self.h:
+            self.lib.b2g_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _dptr(t):
+    if not (t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()):
+        raise ValueError("expected a contiguous float64 CUDA tensor")
+    return C.c_void_p(t.data_ptr())
+
+
+class EffectiveHamiltonian:
+    """Two-site effective Hamiltonian of one DMRG site on the GPU.
+
+    The constructor does what block2's constructor and precompute() do:
+    connection lists (ConnectionInfo::initialize_wfn / initialize_diag),
+    the diagonal, and the recorded pair list, uploaded once.
+    """
+
+    def __init__(self, problem, ctx, fp32=True, device_data=None):
+        """device_data: optional (left, right) float64 CUDA tensors holding
+        the operator buffers (e.g. from synth.fill_device)."""
+        self.lib = L.lib()
+        self.ctx = ctx
+        self.problem = problem
+        h = C.c_void_p()
+        flags = L.FLAG_F32 if fp32 else 0
+        if device_data is not None:
+            ld, rd = device_data
+            flags |= L.FLAG_DEVICE_DATA
+            cp = problem.to_c((ld.data_ptr(), ld.numel(), rd.data_ptr(),
+                               rd.numel()))
+        else:
+            cp = problem.to_c()
+        L.check(self.lib.b2g_effham_create(ctx.h, C.byref(cp), flags,
+                                           C.byref(h)))
+        self.h = h
+        kl, bl, npairs, macs = (C.c_uint64(), C.c_uint64(), C.c_uint64(),
+                                C.c_uint64())
+        L.check(self.lib.b2g_effham_info(h, C.byref(kl), C.byref(bl),
+                                         C.byref(npairs), C.byref(macs)))
+        self.ket_len, self.bra_len = kl.value, bl.value
+        self.n_pairs, self.macs = npairs.value, macs.value
+        self._diag = None
+
+    # block2: precompute() records the GEMM list; here it is built and
+    # uploaded by the constructor already.
+    def precompute(self):
+        return None
+
+    def post_precompute(self):
+        return None
+
+    def __call__(self, b, c, idx=0, factor=1.0, all_reduce=True,
+                 precision="f64"):
+        """c += factor * H b (operator()(b, c, idx, factor, all_reduce),
+        effective_hamiltonian.hpp:459-477).  b, c: (n,) or (n_rhs, n)."""
+        if idx != 0:
+            raise ValueError("only the ground operator (idx 0) is held")
+        nrhs = self._check(b, c)
+        torch.cuda.current_stream(self.ctx.device).synchronize()
+        L.check(self.lib.b2g_effham_hpsi(self.h, _dptr(b), _dptr(c), nrhs,
+                                         float(factor), 1, L.PREC[precision]))
+        self.ctx.synchronize()
+
+    def multiply(self, b, precision="f64", scale=1.0):
+        """Return H b (overwrite semantics)."""
+        nrhs = 1 if b.dim() == 1 else b.shape[0]
+        c = torch.empty((nrhs, self.bra_len) if b.dim() > 1 else
+                        (self.bra_len,), dtype=torch.float64, device=b.device)
+        self._check(b, c)
+        torch.cuda.current_stream(self.ctx.device).synchronize()
+        L.check(self.lib.b2g_effham_hpsi(self.h, _dptr(b), _dptr(c), nrhs,
+                                         float(scale), 0, L.PREC[precision]))
+        self.ctx.synchronize()
+        return c
+
+    def _check(self, b, c):
+        """Number of right-hand sides; rejects tensors whose sizes do not
+        match the plan (the C ABI cannot see device allocation sizes)."""
+        nrhs = 1 if b.dim() == 1 else b.shape[0]
+        if b.dim() > 2 or b.numel() != nrhs * self.ket_len:
+            raise ValueError("b must hold n_rhs x %d values" % self.ket_len)
+        if c.numel() != nrhs * self.bra_len:
+            raise ValueError("c must hold n_rhs x %d values" % self.bra_len)
+        if b.device != self.ctx.device or c.device != self.ctx.device:
+            raise ValueError("tensors must live on the context's device")
+        return nrhs
+
+    @property
+    def plan(self):
+        return self.lib.b2g_effham_plan(self.h)
+
+    def set_timing(self, enable=True):
+        L.check(self.lib.b2g_plan_set_timing(self.plan, 1 if enable else 0))
+
+    def timing(self):
+        """(ms phase 1, ms phase 2, launches) recorded since the last call."""
+        a, b, n = C.c_double(), C.c_double(), C.c_uint64()
+        L.check(self.lib.b2g_plan_timing(self.plan, C.byref(a), C.byref(b),
+                                         C.byref(n)))
+        return a.value, b.value, n.value
+
+    def pairs(self):
+        """Recorded pair list as a numpy structured array."""
+        import numpy as np
+        n = C.c_uint64()
+        L.check(self.lib.b2g_effham_pairs(self.h, C.byref(n), None))
+        arr = (L.b2g_pair * max(n.value, 1))()
+        L.check(self.lib.b2g_effham_pairs(self.h, C.byref(n), arr))
+        return np.ctypeslib.as_array(arr)[:n.value]
+
+    @property
+    def diag(self):
+        if self._diag is None:
+            d = torch.empty(self.ket_len, dtype=torch.float64,
+                            device=self.ctx.device)
+            torch.cuda.current_stream(self.ctx.device).synchronize()
+            L.check(self.lib.b2g_effham_diag(self.h, _dptr(d)))
+            self.ctx.synchronize()
+            self._diag = d
+        return self._diag
+
+    def eigs(self, ket, iprint=False, conv_thrd=5E-6, rel_conv_thrd=0.0,
+             max_iter=5000, soft_max_iter=-1, deflation_min_size=2,
+             deflation_max_size=50, precision="f64", refine=True):
+        """Davidson for the lowest roots (eigs,
+        effective_hamiltonian.hpp:480-568).  ket: (n,) or (k, n) CUDA
+        tensor, overwritten with the Ritz vector(s).  Returns
+        (energy or energies, ndav, nflop, tdav) with nflop in the reference's
+        multiply-add convention."""
+        if rel_conv_thrd != 0.0 or soft_max_iter != -1:
+            raise ValueError("rel_conv_thrd / soft_max_iter are not supported")
+        k = 1 if ket.dim() == 1 else ket.shape[0]
+        if ket.numel() != k * self.ket_len:
+            raise ValueError("ket must hold k x %d values" % self.ket_len)
+        en = (C.c_double * k)()
+        nd, nfl, td = C.c_int(), C.c_uint64(), C.c_double()
+        torch.cuda.current_stream(self.ctx.device).synchronize()
+        code = self.lib.b2g_effham_eigs(
+            self.h, _dptr(ket), k, float(conv_thrd), int(max_iter),
+            int(deflation_min_size), int(deflation_max_size),
+            L.PREC[precision], 1 if refine else 0, en, C.byref(nd),
+            C.byref(nfl), C.byref(td))
+        self.ctx.synchronize()
+        L.check(code)
+        e = [en[i] for i in range(k)]
+        return (e[0] if ket.dim() == 1 else e), nd.value, nfl.value, td.value
+
+    def connections(self, which=0):
+        """Host copy of the connection list (which 0: wfn, 1: diag)."""
+        return connections(self.h, which)
+
+    def deallocate(self):
+        if self.h:
+            self.lib.b2g_effham_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.deallocate()
+        except Exception:
+            pass
+
+
+def connections(h, which):
+    import numpy as np
+    lib = L.lib()
+    n = C.c_uint64()
+    L.check(lib.b2g_effham_connections(h, which, C.byref(n), None, None, None,
+                                       None, None))
+    nc = n.value
+    ia = np.zeros(nc, np.uint32)
+    ib = np.zeros(nc, np.uint32)
+    ic = np.zeros(nc, np.uint32)
+    st = np.zeros(nc, np.uint64)
+    f = np.zeros(nc, np.float64)
+    L.check(lib.b2g_effham_connections(
+        h, which, C.byref(n), L.ptr(ia, C.c_uint32), L.ptr(ib, C.c_uint32),
+        L.ptr(ic, C.c_uint32), L.ptr(st, C.c_uint64), L.ptr(f, C.c_double)))
+    return dict(ia=ia, ib=ib, ic=ic, stride=st, factor=f)
+
+
+def host_plan(problem):
+    """Build only the host-side plan (no device): returns a handle whose
+    connection lists and pairs can be queried."""
+    lib = L.lib()
+    h = C.c_void_p()
+    cp = problem.to_c()
+    L.check(lib.b2g_effham_create(None, C.byref(cp), L.FLAG_HOST_ONLY,
+                                  C.byref(h)))
+    return h
+
+
+def host_plan_info(h):
+    lib = L.lib()
+    kl, bl, npairs, macs = (C.c_uint64(), C.c_uint64(), C.c_uint64(),
+                            C.c_uint64())
+    L.check(lib.b2g_effham_info(h, C.byref(kl), C.byref(bl), C.byref(npairs),
+                                C.byref(macs)))
+    return dict(ket_len=kl.value, bra_len=bl.value, n_pairs=npairs.value,
+                macs=macs.value)
+
+
+def destroy(h):
+    L.lib().b2g_effham_destroy(h)

@@ -1,0 +1,51 @@
+// Shared host-side definitions of libb2g: error state, context, buffers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/b2g.h"
+
+namespace b2g {
+
+void set_error(const std::string &msg);
+
+#define B2G_HIP(call)                                                          \
+    do {                                                                       \
+        hipError_t _e = (call);                                                \
+        if (_e != hipSuccess) {                                                \
+            ::b2g::set_error(std::string(#call) + ": " +                       \
+                             hipGetErrorString(_e));                           \
+            return B2G_ERR_DEVICE;                                             \
+        }                                                                      \
+    } while (0)
+
+#define B2G_REQUIRE(cond, code, msg)                                           \
+    do {                                                                       \
+        if (!(cond)) {                                                         \
+            ::b2g::set_error(msg);                                             \
+            return code;                                                       \
+        }                                                                      \
+    } while (0)
+
+} // namespace b2g
+
+struct b2g_ctx_s {
+    int device = 0;
+    int num_cu = 0;
+    hipStream_t stream = nullptr;
+    // reusable device scratch for reductions (deterministic two-stage)
+    double *red = nullptr;
+    size_t red_len = 0;
+    double *red_host = nullptr; // pinned
+};
+
+struct b2g_buf_s {
+    b2g_ctx ctx = nullptr;
+    void *ptr = nullptr;
+    size_t n = 0;
+    int dtype = B2G_F64;
+};

@@ -1,0 +1,565 @@
+// Effective-Hamiltonian layer: problem intake, plan building, diagonal and
+// the Davidson solve, on top of the thin device layer.
+//
+// Mirrors EffectiveHamiltonian<S, FL, MPS> (src/dmrg/effective_hamiltonian.hpp):
+//   ctor        -> b2g_effham_create: sub-labels, ConnectionInfo for the
+//                  diagonal and the wavefunction, diagonal (:126-224)
+//   precompute  -> pair recording + upload (:226-246)
+//   operator()  -> b2g_effham_hpsi (:459-477)
+//   eigs        -> b2g_effham_eigs (:480-568)
+//   deallocate  -> b2g_effham_destroy (:1136-1162)
+#include "common.hpp"
+#include "launch.hpp"
+#include "plan.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <numeric>
+
+using namespace b2g;
+
+struct b2g_effham_s {
+    b2g_ctx ctx = nullptr;
+    Problem prob;
+    Connections wfn, dg;
+    std::vector<PairRec> pairs;
+    std::vector<DiagRec> drecs;
+    b2g_buf lbuf = nullptr, rbuf = nullptr;
+    b2g_plan plan = nullptr;
+    double *diag = nullptr; // device, bra length
+    uint64_t macs = 0;
+};
+
+namespace b2g {
+
+static BlockInfo to_info(const b2g_sparse_info &s) {
+    BlockInfo b;
+    b.dq = s.delta_quantum;
+    b.is_fermion = s.is_fermion != 0;
+    b.quanta.assign(s.quanta, s.quanta + s.n);
+    b.nbra.assign(s.n_states_bra, s.n_states_bra + s.n);
+    b.nket.assign(s.n_states_ket, s.n_states_ket + s.n);
+    b.shift.assign(s.block_shifts, s.block_shifts + s.n);
+    return b;
+}
+
+static int check_info(const b2g_sparse_info &s, const char *what) {
+    B2G_REQUIRE(s.n >= 0, B2G_ERR_ARG, std::string(what) + ": negative n");
+    B2G_REQUIRE(s.n == 0 || (s.quanta && s.n_states_bra && s.n_states_ket &&
+                             s.block_shifts),
+                B2G_ERR_ARG, std::string(what) + ": NULL arrays");
+    for (int i = 1; i < s.n; i++)
+        B2G_REQUIRE(s.quanta[i - 1] < s.quanta[i], B2G_ERR_ARG,
+                    std::string(what) + ": quanta not strictly ascending");
+    return B2G_OK;
+}
+
+static std::vector<b2g_block> blocks_of(const BlockInfo &b) {
+    std::vector<b2g_block> r(b.n());
+    for (int i = 0; i < b.n(); i++) {
+        r[i].off = b.shift[i];
+        r[i].m = b.nbra[i];
+        r[i].n = b.nket[i];
+    }
+    return r;
+}
+
+// Symmetric eigen-decomposition of an m x m matrix (cyclic Jacobi,
+// FP64).  On return w holds ascending eigenvalues and row j of v the
+// eigenvector of w[j] (the layout GMatrixFunctions::eigs leaves in its
+// input, src/core/matrix_functions.hpp:1672-1688).
+void jacobi_eigs(int m, std::vector<double> a, std::vector<double> &w,
+                 std::vector<double> &v) {
+    std::vector<double> z(m * m, 0.0);
+    for (int i = 0; i < m; i++)
+        z[i * m + i] = 1.0;
+    for (int sweep = 0; sweep < 100; sweep++) {
+        double off = 0.0, tot = 0.0;
+        for (int i = 0; i < m; i++)
+            for (int j = 0; j < m; j++) {
+                tot += a[i * m + j] * a[i * m + j];
+                if (i != j)
+                    off += a[i * m + j] * a[i * m + j];
+            }
+        if (off <= 1E-30 * tot || off == 0.0)
+            break;
+        for (int p = 0; p < m - 1; p++)
+            for (int q = p + 1; q < m; q++) {
+                const double apq = a[p * m + q];
+                if (std::abs(apq) < 1E-300)
+                    continue;
+                const double app = a[p * m + p], aqq = a[q * m + q];
+                const double theta = (aqq - app) / (2.0 * apq);
+                const double t =
+                    (theta >= 0 ? 1.0 : -1.0) /
+                    (std::abs(theta) + std::sqrt(theta * theta + 1.0));
+                const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+                for (int k = 0; k < m; k++) {
+                    const double akp = a[k * m + p], akq = a[k * m + q];
+                    a[k * m + p] = c * akp - s * akq;
+                    a[k * m + q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < m; k++) {
+                    const double apk = a[p * m + k], aqk = a[q * m + k];
+                    a[p * m + k] = c * apk - s * aqk;
+                    a[q * m + k] = s * apk + c * aqk;
+                }
+                for (int k = 0; k < m; k++) {
+                    const double zkp = z[k * m + p], zkq = z[k * m + q];
+                    z[k * m + p] = c * zkp - s * zkq;
+                    z[k * m + q] = s * zkp + c * zkq;
+                }
+            }
+    }
+    std::vector<int> idx(m);
+    std::iota(idx.begin(), idx.end(), 0);
+    std::stable_sort(idx.begin(), idx.end(), [&a, m](int x, int y) {
+        return a[x * m + x] < a[y * m + y];
+    });
+    w.resize(m);
+    v.assign(m * m, 0.0);
+    for (int j = 0; j < m; j++) {
+        w[j] = a[idx[j] * m + idx[j]];
+        for (int i = 0; i < m; i++)
+            v[j * m + i] = z[i * m + idx[j]];
+    }
+}
+
+} // namespace b2g
+
+extern "C" int b2g_effham_create(b2g_ctx ctx, const b2g_problem *pb,
+                                 unsigned flags, b2g_effham *out) {
+    B2G_REQUIRE((ctx || (flags & B2G_EFFHAM_HOST_ONLY)) && pb && out,
+                B2G_ERR_ARG, "NULL arg");
+    B2G_REQUIRE(pb->symmetry == B2G_SZ || pb->symmetry == B2G_SU2,
+                B2G_ERR_ARG, "unknown symmetry");
+    int r;
+    if ((r = check_info(pb->ket, "ket")) != B2G_OK ||
+        (r = check_info(pb->bra, "bra")) != B2G_OK)
+        return r;
+    B2G_REQUIRE(pb->n_left_infos >= 0 && pb->n_right_infos >= 0 &&
+                    pb->n_left_ops >= 0 && pb->n_right_ops >= 0 &&
+                    pb->n_terms >= 0,
+                B2G_ERR_ARG, "negative counts");
+    std::unique_ptr<b2g_effham_s> h(new b2g_effham_s());
+    h->ctx = ctx;
+    Problem &p = h->prob;
+    p.symm = pb->symmetry == B2G_SZ ? Symm::SZ : Symm::SU2;
+    p.opdq = pb->opdq;
+    p.hop_left_vacuum = pb->hop_left_vacuum;
+    p.ket = to_info(pb->ket);
+    p.bra = to_info(pb->bra);
+    for (int i = 0; i < pb->n_left_infos; i++) {
+        if ((r = check_info(pb->left_infos[i], "left info")) != B2G_OK)
+            return r;
+        B2G_REQUIRE(i == 0 || pb->left_info_dq[i - 1] < pb->left_info_dq[i],
+                    B2G_ERR_ARG, "left infos not sorted by delta quantum");
+        p.linfos.push_back(to_info(pb->left_infos[i]));
+        p.linfos.back().dq = pb->left_info_dq[i];
+    }
+    for (int i = 0; i < pb->n_right_infos; i++) {
+        if ((r = check_info(pb->right_infos[i], "right info")) != B2G_OK)
+            return r;
+        B2G_REQUIRE(i == 0 || pb->right_info_dq[i - 1] < pb->right_info_dq[i],
+                    B2G_ERR_ARG, "right infos not sorted by delta quantum");
+        p.rinfos.push_back(to_info(pb->right_infos[i]));
+        p.rinfos.back().dq = pb->right_info_dq[i];
+    }
+    auto take_ops = [](const b2g_operator *ops, int n,
+                       const std::vector<BlockInfo> &infos, size_t len,
+                       std::vector<Operator> &dst, const char *what) -> int {
+        for (int i = 0; i < n; i++) {
+            const b2g_operator &o = ops[i];
+            B2G_REQUIRE(o.info >= 0 && o.info < (int)infos.size(), B2G_ERR_ARG,
+                        std::string(what) + " operator: bad info index");
+            B2G_REQUIRE(infos[o.info].dq == o.delta_quantum, B2G_ERR_ARG,
+                        std::string(what) +
+                            " operator: delta quantum differs from its info");
+            B2G_REQUIRE(o.data_off + infos[o.info].total() <= len, B2G_ERR_ARG,
+                        std::string(what) + " operator: data out of range");
+            dst.push_back(Operator{o.delta_quantum, o.factor, o.info,
+                                   o.data_off});
+        }
+        return B2G_OK;
+    };
+    if ((r = take_ops(pb->left_ops, pb->n_left_ops, p.linfos, pb->left_len,
+                      p.lops, "left")) != B2G_OK ||
+        (r = take_ops(pb->right_ops, pb->n_right_ops, p.rinfos, pb->right_len,
+                      p.rops, "right")) != B2G_OK)
+        return r;
+    for (int i = 0; i < pb->n_terms; i++) {
+        const b2g_term &t = pb->terms[i];
+        B2G_REQUIRE(t.a >= 0 && t.a < pb->n_left_ops && t.b >= 0 &&
+                        t.b < pb->n_right_ops && t.conj < 4,
+                    B2G_ERR_ARG, "term references a missing operator");
+        p.terms.push_back(Term{t.a, t.b, (uint8_t)t.conj, t.factor});
+    }
+    // connection lists (EffectiveHamiltonian ctor)
+    SU2Coupling cg;
+    std::vector<SubLabel> subl = uniq_sub_labels(p);
+    std::vector<InfoRef> ai, bi;
+    for (auto &x : p.linfos)
+        ai.emplace_back(x.dq, &x);
+    for (auto &x : p.rinfos)
+        bi.emplace_back(x.dq, &x);
+    build_diag_connections(h->dg, p.symm, cg, p.ket.dq, p.opdq, subl, ai, bi,
+                           p.ket);
+    build_wfn_connections(h->wfn, p.symm, cg, p.ket.dq, p.bra.dq, p.opdq, subl,
+                          ai, bi, p.ket, p.bra);
+    std::string err;
+    h->pairs = record_pairs(p, h->wfn, err);
+    B2G_REQUIRE(err.empty(), B2G_ERR_ARG, err);
+    h->drecs = record_diag(p, h->dg, err);
+    B2G_REQUIRE(err.empty(), B2G_ERR_ARG, err);
+    h->macs = 0;
+    for (auto &q : h->pairs)
+        h->macs += pair_macs(q);
+    if (flags & B2G_EFFHAM_HOST_ONLY) {
+        // plan only: connection lists and pairs, no device state
+        *out = h.release();
+        return B2G_OK;
+    }
+    // upload operators and plan
+    if (flags & B2G_EFFHAM_DEVICE_DATA) {
+        if ((r = b2g_buf_create(ctx, pb->left_len, B2G_F64, &h->lbuf)) !=
+            B2G_OK)
+            return r;
+        if ((r = b2g_buf_create(ctx, pb->right_len, B2G_F64, &h->rbuf)) !=
+            B2G_OK) {
+            b2g_buf_destroy(h->lbuf);
+            return r;
+        }
+        if (hipMemcpyAsync(h->lbuf->ptr, pb->left_data,
+                           pb->left_len * sizeof(double),
+                           hipMemcpyDeviceToDevice, ctx->stream) !=
+                hipSuccess ||
+            hipMemcpyAsync(h->rbuf->ptr, pb->right_data,
+                           pb->right_len * sizeof(double),
+                           hipMemcpyDeviceToDevice, ctx->stream) !=
+                hipSuccess ||
+            hipStreamSynchronize(ctx->stream) != hipSuccess) {
+            b2g_buf_destroy(h->lbuf);
+            b2g_buf_destroy(h->rbuf);
+            set_error("copy of device operator data failed");
+            return B2G_ERR_DEVICE;
+        }
+    } else {
+        if ((r = b2g_upload_blocks(ctx, pb->left_data, pb->left_len, B2G_F64,
+                                   &h->lbuf)) != B2G_OK)
+            return r;
+        if ((r = b2g_upload_blocks(ctx, pb->right_data, pb->right_len,
+                                   B2G_F64, &h->rbuf)) != B2G_OK) {
+            b2g_buf_destroy(h->lbuf);
+            return r;
+        }
+    }
+    std::vector<b2g_pair> dp(h->pairs.size());
+    for (size_t i = 0; i < dp.size(); i++) {
+        const PairRec &q = h->pairs[i];
+        dp[i] = b2g_pair{q.a_off, q.b_off, q.c_off, q.v_off, q.c_m, q.c_n,
+                         q.v_m, q.v_n, q.conj, 0, q.alpha};
+    }
+    std::vector<b2g_block> yb = blocks_of(p.bra);
+    r = b2g_upload_plan(ctx, h->lbuf, h->rbuf, dp.data(), dp.size(), yb.data(),
+                        yb.size(), p.ket.total(), p.bra.total(), flags,
+                        &h->plan);
+    if (r == B2G_OK) {
+        std::vector<b2g_diag_term> dt(h->drecs.size());
+        for (size_t i = 0; i < dt.size(); i++) {
+            const DiagRec &q = h->drecs[i];
+            dt[i] = b2g_diag_term{q.a_off, q.b_off, q.d_off,
+                                  q.a_n,   q.b_n,   q.alpha};
+        }
+        std::vector<b2g_block> db = blocks_of(p.ket);
+        const size_t dl = std::max<uint64_t>(p.ket.total(), 1);
+        if (hipMalloc(&h->diag, dl * sizeof(double)) != hipSuccess) {
+            set_error("diagonal allocation failed");
+            r = B2G_ERR_MEMORY;
+        } else
+            r = b2g_diag(ctx, h->lbuf, h->rbuf, dt.data(), dt.size(),
+                         db.data(), db.size(), h->diag, p.ket.total());
+    }
+    if (r != B2G_OK) {
+        b2g_effham_destroy(h.release());
+        return r;
+    }
+    *out = h.release();
+    return B2G_OK;
+}
+
+extern "C" int b2g_effham_destroy(b2g_effham h) {
+    if (h == nullptr)
+        return B2G_OK;
+    b2g_plan_destroy(h->plan);
+    b2g_buf_destroy(h->lbuf);
+    b2g_buf_destroy(h->rbuf);
+    if (h->diag)
+        hipFree(h->diag);
+    delete h;
+    return B2G_OK;
+}
+
+extern "C" int b2g_effham_info(b2g_effham h, uint64_t *ket_len,
+                               uint64_t *bra_len, uint64_t *n_pairs,
+                               uint64_t *macs) {
+    B2G_REQUIRE(h, B2G_ERR_ARG, "NULL effham");
+    if (ket_len)
+        *ket_len = h->prob.ket.total();
+    if (bra_len)
+        *bra_len = h->prob.bra.total();
+    if (n_pairs)
+        *n_pairs = h->pairs.size();
+    if (macs)
+        *macs = h->macs;
+    return B2G_OK;
+}
+
+extern "C" int b2g_effham_connections(b2g_effham h, int which,
+                                      uint64_t *n_conn, uint32_t *ia,
+                                      uint32_t *ib, uint32_t *ic,
+                                      uint64_t *stride, double *factor) {
+    B2G_REQUIRE(h && n_conn && (which == 0 || which == 1), B2G_ERR_ARG,
+                "bad arg");
+    const Connections &c = which == 0 ? h->wfn : h->dg;
+    *n_conn = c.ia.size();
+    if (ia)
+        std::copy(c.ia.begin(), c.ia.end(), ia);
+    if (ib)
+        std::copy(c.ib.begin(), c.ib.end(), ib);
+    if (ic)
+        std::copy(c.ic.begin(), c.ic.end(), ic);
+    if (stride)
+        std::copy(c.stride.begin(), c.stride.end(), stride);
+    if (factor)
+        std::copy(c.factor.begin(), c.factor.end(), factor);
+    return B2G_OK;
+}
+
+extern "C" int b2g_effham_pairs(b2g_effham h, uint64_t *n_pairs,
+                                b2g_pair *pairs) {
+    B2G_REQUIRE(h && n_pairs, B2G_ERR_ARG, "NULL arg");
+    *n_pairs = h->pairs.size();
+    if (pairs)
+        for (size_t i = 0; i < h->pairs.size(); i++) {
+            const PairRec &q = h->pairs[i];
+            pairs[i] = b2g_pair{q.a_off, q.b_off, q.c_off, q.v_off, q.c_m,
+                                q.c_n, q.v_m, q.v_n, q.conj, 0, q.alpha};
+        }
+    return B2G_OK;
+}
+
+extern "C" b2g_plan b2g_effham_plan(b2g_effham h) {
+    return h ? h->plan : nullptr;
+}
+
+extern "C" int b2g_effham_hpsi(b2g_effham h, const double *x, double *y,
+                               int n_rhs, double scale, int accumulate,
+                               int precision) {
+    B2G_REQUIRE(h, B2G_ERR_ARG, "NULL effham");
+    B2G_REQUIRE(h->plan, B2G_ERR_STATE, "effham was built host-only");
+    return b2g_hpsi(h->ctx, h->plan, x, y, n_rhs, scale, accumulate,
+                    precision);
+}
+
+extern "C" int b2g_effham_diag(b2g_effham h, double *d) {
+    B2G_REQUIRE(h && d, B2G_ERR_ARG, "NULL arg");
+    B2G_REQUIRE(h->diag, B2G_ERR_STATE, "effham was built host-only");
+    B2G_HIP(hipMemcpyAsync(d, h->diag, h->prob.ket.total() * sizeof(double),
+                           hipMemcpyDeviceToDevice, h->ctx->stream));
+    return B2G_OK;
+}
+
+// --------------------------------------------------------------- Davidson
+namespace b2g {
+
+struct DavWork {
+    double *B = nullptr, *S = nullptr, *q = nullptr, *t = nullptr;
+    double *scal = nullptr; // device scalars (>= 64*64 + 16)
+    ~DavWork() {
+        if (B)
+            hipFree(B);
+        if (S)
+            hipFree(S);
+        if (q)
+            hipFree(q);
+        if (t)
+            hipFree(t);
+        if (scal)
+            hipFree(scal);
+    }
+};
+
+static int dev_dot(b2g_ctx ctx, const double *x, const double *y, size_t n,
+                   double &out) {
+    return b2g_vec_dot(ctx, x, y, n, &out);
+}
+
+// One Davidson solve (IterativeMatrixFunctions::davidson,
+// src/core/iterative_matrix_functions.hpp:866-1173; DavidsonTypes::Normal,
+// Olsen preconditioner, no orthogonal-state projection).
+static int davidson(b2g_effham h, double *x, int k, double conv_thrd,
+                    int max_iter, int def_min, int def_max, int precision,
+                    std::vector<double> &eigvals, int &ndav, uint64_t &nmv) {
+    b2g_ctx ctx = h->ctx;
+    hipStream_t st = ctx->stream;
+    const size_t n = h->prob.ket.total();
+    B2G_REQUIRE(h->prob.bra.total() == n, B2G_ERR_ARG,
+                "Davidson needs bra and ket of equal size");
+    B2G_REQUIRE(k >= 1 && k <= 32, B2G_ERR_ARG, "bad number of roots");
+    if (def_min < k)
+        def_min = k;
+    if (def_max < k + k / 2)
+        def_max = k + k / 2;
+    B2G_REQUIRE(def_max <= 64 && def_min < def_max, B2G_ERR_ARG,
+                "deflation sizes must satisfy k <= def_min < def_max <= 64");
+    DavWork w;
+    B2G_HIP(hipMalloc(&w.B, (size_t)def_max * n * sizeof(double)));
+    B2G_HIP(hipMalloc(&w.S, (size_t)def_max * n * sizeof(double)));
+    B2G_HIP(hipMalloc(&w.q, n * sizeof(double)));
+    B2G_HIP(hipMalloc(&w.t, n * sizeof(double)));
+    B2G_HIP(hipMalloc(&w.scal, (64 * 64 + 64) * sizeof(double)));
+    double *Bv = w.B, *Sv = w.S;
+    auto bv = [&](int i) { return Bv + (size_t)i * n; };
+    auto sv = [&](int i) { return Sv + (size_t)i * n; };
+    for (int i = 0; i < k; i++)
+        B2G_HIP(launch_copy(x + (size_t)i * n, bv(i), n, st));
+    int m = k;
+    for (int i = 0; i < k; i++) {
+        for (int j = 0; j < i; j++) {
+            double d;
+            int r = dev_dot(ctx, bv(j), bv(i), n, d);
+            if (r)
+                return r;
+            B2G_HIP(launch_axpy(-d, bv(j), bv(i), n, st));
+        }
+        double nrm2;
+        int r = dev_dot(ctx, bv(i), bv(i), n, nrm2);
+        if (r)
+            return r;
+        if (std::abs(nrm2) < 1E-14 && i > 0) {
+            m = i;
+            break;
+        }
+        B2G_REQUIRE(std::abs(nrm2) >= 1E-14, B2G_ERR_ARG,
+                    "Davidson initial guess has zero norm");
+        B2G_HIP(launch_scale(1.0 / std::sqrt(nrm2), bv(i), n, st));
+    }
+    eigvals.assign(k, 0.0);
+    std::vector<double> ld, V, G(64 * 64), A;
+    int ck = 0, msig = 0, xiter = 0;
+    double qq = 0.0;
+    nmv = 0;
+    bool converged = false;
+    while (xiter < max_iter) {
+        xiter++;
+        for (int i = msig; i < m; i++, msig++) {
+            int r = b2g_hpsi(ctx, h->plan, bv(i), sv(i), 1, 1.0, 0, precision);
+            if (r)
+                return r;
+            nmv++;
+        }
+        // subspace matrix alpha(i, j) = <b_i | sigma_j>, lower triangle
+        B2G_HIP(launch_gram(Bv, m, Sv, m, n, n, ctx->red, w.scal, st));
+        B2G_HIP(hipMemcpyAsync(G.data(), w.scal, (size_t)m * m * sizeof(double),
+                               hipMemcpyDeviceToHost, st));
+        B2G_HIP(hipStreamSynchronize(st));
+        A.assign((size_t)m * m, 0.0);
+        for (int i = 0; i < m; i++)
+            for (int j = 0; j <= i; j++)
+                A[i * m + j] = A[j * m + i] = G[i * m + j];
+        jacobi_eigs(m, A, ld, V);
+        // rotate b and sigma into the Ritz basis
+        B2G_HIP(hipMemcpyAsync(w.scal, V.data(), (size_t)m * m * sizeof(double),
+                               hipMemcpyHostToDevice, st));
+        B2G_HIP(launch_rotate(Sv, n, n, m, w.scal, st));
+        B2G_HIP(launch_rotate(Bv, n, n, m, w.scal, st));
+        // re-check roots already converged
+        for (int i = 0; i < ck; i++) {
+            B2G_HIP(launch_copy(sv(i), w.q, n, st));
+            B2G_HIP(launch_axpy(-ld[i], bv(i), w.q, n, st));
+            double d;
+            int r = dev_dot(ctx, w.q, w.q, n, d);
+            if (r)
+                return r;
+            if (std::abs(d) >= conv_thrd) {
+                ck = i;
+                break;
+            }
+        }
+        B2G_HIP(launch_copy(sv(ck), w.q, n, st));
+        B2G_HIP(launch_axpy(-ld[ck], bv(ck), w.q, n, st));
+        int r = dev_dot(ctx, w.q, w.q, n, qq);
+        if (r)
+            return r;
+        B2G_HIP(launch_olsen(w.q, bv(ck), h->diag, ld[ck], w.t, n, ctx->red,
+                             w.scal + 64 * 64, st));
+        for (int i = 0; i <= ck; i++)
+            eigvals[i] = ld[i];
+        if (std::abs(qq) < conv_thrd) {
+            ck++;
+            if (ck == k) {
+                converged = true;
+                break;
+            }
+        } else {
+            if (m >= def_max)
+                m = msig = def_min;
+            // one Gram-Schmidt pass of q against the subspace
+            for (int j = 0; j < m; j++) {
+                B2G_HIP(launch_dot(bv(j), w.q, n, ctx->red, w.scal, st));
+                B2G_HIP(launch_axpy_dev(w.scal, -1.0, bv(j), w.q, n, st));
+            }
+            double nq;
+            r = dev_dot(ctx, w.q, w.q, n, nq);
+            if (r)
+                return r;
+            B2G_HIP(launch_scale(1.0 / std::sqrt(nq), w.q, n, st));
+            B2G_HIP(launch_copy(w.q, bv(m), n, st));
+            m++;
+        }
+    }
+    for (int i = 0; i < k; i++)
+        B2G_HIP(launch_copy(bv(i), x + (size_t)i * n, n, st));
+    B2G_HIP(hipStreamSynchronize(st));
+    ndav = xiter;
+    if (!converged) {
+        set_error("Davidson: only " + std::to_string(ck) + " converged");
+        return B2G_ERR_NOCONV;
+    }
+    return B2G_OK;
+}
+
+} // namespace b2g
+
+extern "C" int b2g_effham_eigs(b2g_effham h, double *x, int k,
+                               double conv_thrd, int max_iter, int def_min,
+                               int def_max, int precision, int refine,
+                               double *energies, int *ndav, uint64_t *nflop,
+                               double *tdav) {
+    B2G_REQUIRE(h && x && energies, B2G_ERR_ARG, "NULL arg");
+    B2G_REQUIRE(h->plan, B2G_ERR_STATE, "effham was built host-only");
+    B2G_REQUIRE(precision == B2G_PREC_F64 || precision == B2G_PREC_F32,
+                B2G_ERR_ARG, "unsupported precision");
+    auto t0 = std::chrono::steady_clock::now();
+    std::vector<double> ev;
+    int nd = 0, nd2 = 0;
+    uint64_t nmv = 0, nmv2 = 0;
+    int r = davidson(h, x, k, conv_thrd, max_iter, def_min, def_max, precision,
+                     ev, nd, nmv);
+    if (r == B2G_OK && precision != B2G_PREC_F64 && refine)
+        r = davidson(h, x, k, conv_thrd, max_iter, def_min, def_max,
+                     B2G_PREC_F64, ev, nd2, nmv2);
+    auto t1 = std::chrono::steady_clock::now();
+    for (int i = 0; i < k && i < (int)ev.size(); i++)
+        energies[i] = ev[i];
+    if (ndav)
+        *ndav = nd + nd2;
+    if (nflop)
+        *nflop = h->macs * (nmv + nmv2);
+    if (tdav)
+        *tdav = std::chrono::duration<double>(t1 - t0).count();
+    return r;
+}

@@ -1,0 +1,498 @@
+// Device kernels: segmented grouped GEMM on MFMA (FP64 / FP32), the
+// diagonal, and the FP64 vector algebra of the Davidson loop.
+#include "kernels.hpp"
+#include "launch.hpp"
+
+namespace b2g {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+template <typename TC> struct Mfma;
+// 16x16x4 single-block MFMA.  Lane l holds A[l%16][l/16] and B[l/16][l%16];
+// the 16x16 accumulator holds D[4*(l/16)+r][l%16], r = 0..3.
+template <> struct Mfma<double> {
+    typedef v4d acc_t;
+    static __device__ __forceinline__ acc_t op(double a, double b, acc_t c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+};
+template <> struct Mfma<float> {
+    typedef v4f acc_t;
+    static __device__ __forceinline__ acc_t op(float a, float b, acc_t c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+};
+
+// One workgroup = 4 waves in a 2x2 arrangement; the workgroup tile is
+// TM x TN, each wave owns (TM/2) x (TN/2) as (TM/32) x (TN/32) MFMA tiles.
+// K is staged through LDS in chunks of TK.  Tasks are strided over the grid
+// (persistent launch); blockIdx.y selects the right-hand side.
+template <typename TA, typename TB, typename TD, typename TC, int TM, int TN>
+__global__ __launch_bounds__(256) void seg_gemm_kernel(
+    const TA *__restrict__ A, const TB *__restrict__ B, TD *__restrict__ D,
+    const Task *__restrict__ tasks, int ntasks, const Seg *__restrict__ segs,
+    uint64_t a_rs, uint64_t b_rs, uint64_t d_rs, double scale, int beta0) {
+    constexpr int TK = 16;
+    constexpr int WM = TM / 2, WN = TN / 2;
+    constexpr int FM = WM / 16, FN = WN / 16;
+    typedef typename Mfma<TC>::acc_t acc_t;
+    __shared__ TC As[TK][TM + 1];
+    __shared__ TC Bs[TK][TN + 1];
+    const int rhs = blockIdx.y;
+    A += rhs * a_rs;
+    B += rhs * b_rs;
+    D += rhs * d_rs;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int lr = lane & 15, lk = lane >> 4;
+    for (int t = blockIdx.x; t < ntasks; t += gridDim.x) {
+        const Task task = tasks[t];
+        acc_t acc[FM][FN];
+#pragma unroll
+        for (int i = 0; i < FM; i++)
+#pragma unroll
+            for (int j = 0; j < FN; j++)
+                acc[i][j] = acc_t{0, 0, 0, 0};
+        for (uint32_t s = task.seg_begin; s < task.seg_end; s++) {
+            const Seg sg = segs[s];
+            const TC alpha = (TC)sg.alpha;
+            const bool ta = sg.flags & 1, tb = (sg.flags >> 1) & 1;
+            for (uint32_t k0 = 0; k0 < sg.K; k0 += TK) {
+#pragma unroll
+                for (int q = 0; q < TM * TK / 256; q++) {
+                    const int e = tid + q * 256;
+                    int m, k;
+                    uint64_t idx;
+                    if (!ta) {
+                        m = e / TK, k = e % TK;
+                        idx = sg.a_off + (uint64_t)(task.row0 + m) * sg.lda +
+                              (k0 + k);
+                    } else {
+                        k = e / TM, m = e % TM;
+                        idx = sg.a_off + (uint64_t)(k0 + k) * sg.lda +
+                              (task.row0 + m);
+                    }
+                    TC v = (TC)0;
+                    if (task.row0 + m < task.M && k0 + k < sg.K)
+                        v = (TC)A[idx] * alpha;
+                    As[k][m] = v;
+                }
+#pragma unroll
+                for (int q = 0; q < TN * TK / 256; q++) {
+                    const int e = tid + q * 256;
+                    int n, k;
+                    uint64_t idx;
+                    if (!tb) {
+                        k = e / TN, n = e % TN;
+                        idx = sg.b_off + (uint64_t)(k0 + k) * sg.ldb +
+                              (task.col0 + n);
+                    } else {
+                        n = e / TK, k = e % TK;
+                        idx = sg.b_off + (uint64_t)(task.col0 + n) * sg.ldb +
+                              (k0 + k);
+                    }
+                    TC v = (TC)0;
+                    if (task.col0 + n < task.N && k0 + k < sg.K)
+                        v = (TC)B[idx];
+                    Bs[k][n] = v;
+                }
+                __syncthreads();
+#pragma unroll
+                for (int kk = 0; kk < TK; kk += 4) {
+                    TC a[FM], b[FN];
+#pragma unroll
+                    for (int i = 0; i < FM; i++)
+                        a[i] = As[kk + lk][wm * WM + i * 16 + lr];
+#pragma unroll
+                    for (int j = 0; j < FN; j++)
+                        b[j] = Bs[kk + lk][wn * WN + j * 16 + lr];
+#pragma unroll
+                    for (int i = 0; i < FM; i++)
+#pragma unroll
+                        for (int j = 0; j < FN; j++)
+                            acc[i][j] = Mfma<TC>::op(a[i], b[j], acc[i][j]);
+                }
+                __syncthreads();
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < FM; i++)
+#pragma unroll
+            for (int j = 0; j < FN; j++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const uint32_t row = task.row0 + wm * WM + i * 16 + lk * 4 + r;
+                    const uint32_t col = task.col0 + wn * WN + j * 16 + lr;
+                    if (row < task.M && col < task.N) {
+                        const uint64_t idx =
+                            task.d_off + (uint64_t)row * task.ldd + col;
+                        const double v = scale * (double)acc[i][j][r];
+                        if (task.beta | beta0)
+                            D[idx] = (TD)((double)D[idx] + v);
+                        else
+                            D[idx] = (TD)v;
+                    }
+                }
+    }
+}
+
+template <typename TA, typename TB, typename TD, typename TC>
+hipError_t launch_seg_gemm(const TA *A, const TB *B, TD *D, const Task *tasks,
+                           int ntasks, const Seg *segs, uint64_t a_rs,
+                           uint64_t b_rs, uint64_t d_rs, double scale,
+                           int beta0, int n_rhs, int grid, hipStream_t st) {
+    if (ntasks <= 0)
+        return hipSuccess;
+    dim3 g((unsigned)(grid < ntasks ? grid : ntasks), (unsigned)n_rhs);
+    hipLaunchKernelGGL((seg_gemm_kernel<TA, TB, TD, TC, 64, 64>), g, dim3(256),
+                       0, st, A, B, D, tasks, ntasks, segs, a_rs, b_rs, d_rs,
+                       scale, beta0);
+    return hipGetLastError();
+}
+
+template hipError_t launch_seg_gemm<double, double, double, double>(
+    const double *, const double *, double *, const Task *, int, const Seg *,
+    uint64_t, uint64_t, uint64_t, double, int, int, int, hipStream_t);
+template hipError_t launch_seg_gemm<double, float, float, float>(
+    const double *, const float *, float *, const Task *, int, const Seg *,
+    uint64_t, uint64_t, uint64_t, double, int, int, int, hipStream_t);
+template hipError_t launch_seg_gemm<float, float, double, float>(
+    const float *, const float *, double *, const Task *, int, const Seg *,
+    uint64_t, uint64_t, uint64_t, double, int, int, int, hipStream_t);
+
+// ---------------------------------------------------------------- diagonal
+__global__ __launch_bounds__(256) void diag_kernel(
+    const double *__restrict__ L, const double *__restrict__ R,
+    double *__restrict__ Dv, const DiagBlock *__restrict__ blocks, int nblocks,
+    const DiagTermDev *__restrict__ terms) {
+    for (int bi = blockIdx.x; bi < nblocks; bi += gridDim.x) {
+        const DiagBlock blk = blocks[bi];
+        const uint32_t mn = blk.m * blk.n;
+        for (uint32_t e = threadIdx.x; e < mn; e += blockDim.x) {
+            const uint32_t i = e / blk.n, j = e % blk.n;
+            double s = 0.0;
+            for (uint32_t t = blk.t_begin; t < blk.t_end; t++) {
+                const DiagTermDev tm = terms[t];
+                s += tm.alpha * L[tm.a_off + (uint64_t)i * (tm.a_n + 1)] *
+                     R[tm.b_off + (uint64_t)j * (tm.b_n + 1)];
+            }
+            Dv[blk.d_off + e] = s;
+        }
+    }
+}
+
+hipError_t launch_diag(const double *L, const double *R, double *Dv,
+                       const DiagBlock *blocks, int nblocks,
+                       const DiagTermDev *terms, int grid, hipStream_t st) {
+    if (nblocks <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(diag_kernel, dim3(grid < nblocks ? grid : nblocks),
+                       dim3(256), 0, st, L, R, Dv, blocks, nblocks, terms);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------ conversions
+template <typename TS, typename TT>
+__global__ void convert_kernel(const TS *__restrict__ s, TT *__restrict__ t,
+                               size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        t[i] = (TT)s[i];
+}
+hipError_t launch_f64_to_f32(const double *s, float *t, size_t n,
+                             hipStream_t st) {
+    if (n == 0)
+        return hipSuccess;
+    size_t g = (n + 255) / 256;
+    if (g > 65536)
+        g = 65536;
+    hipLaunchKernelGGL((convert_kernel<double, float>), dim3((unsigned)g),
+                       dim3(256), 0, st, s, t, n);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------ vector ops
+// Two-stage deterministic reductions: stage 1 writes one partial per block
+// (fixed grid), stage 2 sums the partials in order in a single block.
+constexpr int kRedGrid = 512;
+
+__device__ __forceinline__ double block_sum(double v, double *sh) {
+    const int tid = threadIdx.x;
+    sh[tid] = v;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (tid < s)
+            sh[tid] += sh[tid + s];
+        __syncthreads();
+    }
+    double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+// partial[b*k + i] = sum over the block's rows of X_i . Y_i  (k pairs)
+__global__ __launch_bounds__(256) void dot_stage1(
+    const double *__restrict__ X, const double *__restrict__ Y, size_t n,
+    double *__restrict__ partial) {
+    __shared__ double sh[256];
+    double s = 0.0;
+    for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * 256)
+        s += X[i] * Y[i];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0)
+        partial[blockIdx.x] = s;
+}
+
+// out[i] = sum_b partial[b * stride + i] for i < k
+__global__ __launch_bounds__(256) void reduce_stage2(
+    const double *__restrict__ partial, int nblk, int k, double *__restrict__ out) {
+    __shared__ double sh[256];
+    for (int i = 0; i < k; i++) {
+        double s = 0.0;
+        for (int b = threadIdx.x; b < nblk; b += 256)
+            s += partial[(size_t)b * k + i];
+        s = block_sum(s, sh);
+        if (threadIdx.x == 0)
+            out[i] = s;
+    }
+}
+
+hipError_t launch_dot(const double *x, const double *y, size_t n,
+                      double *partial, double *out, hipStream_t st) {
+    hipLaunchKernelGGL(dot_stage1, dim3(kRedGrid), dim3(256), 0, st, x, y, n,
+                       partial);
+    hipLaunchKernelGGL(reduce_stage2, dim3(1), dim3(256), 0, st, partial,
+                       kRedGrid, 1, out);
+    return hipGetLastError();
+}
+
+// Gram block  G[i*ky + j] = X_i . Y_j  for i < kx, j < ky <= 64 (vectors
+// stored with stride ld).  Rows are processed in tiles of 32 staged through
+// LDS.
+__global__ __launch_bounds__(256) void gram_stage1(
+    const double *__restrict__ X, int kx, const double *__restrict__ Y, int ky,
+    size_t ld, size_t n, double *__restrict__ partial) {
+    constexpr int RT = 32;
+    __shared__ double xs[64][RT + 1];
+    __shared__ double ys[64][RT + 1];
+    const int npair = kx * ky;
+    double acc[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++)
+        acc[q] = 0.0;
+    for (size_t r0 = (size_t)blockIdx.x * RT; r0 < n;
+         r0 += (size_t)gridDim.x * RT) {
+        for (int e = threadIdx.x; e < kx * RT; e += 256) {
+            const int i = e / RT, r = e % RT;
+            xs[i][r] = r0 + r < n ? X[(size_t)i * ld + r0 + r] : 0.0;
+        }
+        for (int e = threadIdx.x; e < ky * RT; e += 256) {
+            const int j = e / RT, r = e % RT;
+            ys[j][r] = r0 + r < n ? Y[(size_t)j * ld + r0 + r] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            const int p = threadIdx.x + q * 256;
+            if (p < npair) {
+                const int i = p / ky, j = p % ky;
+                double s = 0.0;
+                for (int r = 0; r < RT; r++)
+                    s += xs[i][r] * ys[j][r];
+                acc[q] += s;
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+        const int p = threadIdx.x + q * 256;
+        if (p < npair)
+            partial[(size_t)blockIdx.x * npair + p] = acc[q];
+    }
+}
+
+hipError_t launch_gram(const double *X, int kx, const double *Y, int ky,
+                       size_t ld, size_t n, double *partial, double *out,
+                       hipStream_t st) {
+    if (kx <= 0 || ky <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(gram_stage1, dim3(kRedGrid), dim3(256), 0, st, X, kx,
+                       Y, ky, ld, n, partial);
+    hipLaunchKernelGGL(reduce_stage2, dim3(1), dim3(256), 0, st, partial,
+                       kRedGrid, kx * ky, out);
+    return hipGetLastError();
+}
+
+__global__ void axpy_kernel(double a, const double *__restrict__ x,
+                            double *__restrict__ y, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        y[i] += a * x[i];
+}
+// y += coef[0] * x with the coefficient read on the device
+__global__ void axpy_dev_kernel(const double *__restrict__ coef, double sign,
+                                const double *__restrict__ x,
+                                double *__restrict__ y, size_t n) {
+    const double a = sign * coef[0];
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        y[i] += a * x[i];
+}
+__global__ void scale_kernel(double a, double *__restrict__ x, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        x[i] *= a;
+}
+__global__ void copy_kernel(const double *__restrict__ x,
+                            double *__restrict__ y, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        y[i] = x[i];
+}
+__global__ void fill_kernel(double v, double *__restrict__ x, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        x[i] = v;
+}
+
+static dim3 grid_for(size_t n) {
+    size_t g = (n + 255) / 256;
+    if (g > 8192)
+        g = 8192;
+    if (g == 0)
+        g = 1;
+    return dim3((unsigned)g);
+}
+
+hipError_t launch_axpy(double a, const double *x, double *y, size_t n,
+                       hipStream_t st) {
+    hipLaunchKernelGGL(axpy_kernel, grid_for(n), dim3(256), 0, st, a, x, y, n);
+    return hipGetLastError();
+}
+hipError_t launch_axpy_dev(const double *coef, double sign, const double *x,
+                           double *y, size_t n, hipStream_t st) {
+    hipLaunchKernelGGL(axpy_dev_kernel, grid_for(n), dim3(256), 0, st, coef,
+                       sign, x, y, n);
+    return hipGetLastError();
+}
+hipError_t launch_scale(double a, double *x, size_t n, hipStream_t st) {
+    hipLaunchKernelGGL(scale_kernel, grid_for(n), dim3(256), 0, st, a, x, n);
+    return hipGetLastError();
+}
+hipError_t launch_copy(const double *x, double *y, size_t n, hipStream_t st) {
+    hipLaunchKernelGGL(copy_kernel, grid_for(n), dim3(256), 0, st, x, y, n);
+    return hipGetLastError();
+}
+hipError_t launch_fill(double v, double *x, size_t n, hipStream_t st) {
+    hipLaunchKernelGGL(fill_kernel, grid_for(n), dim3(256), 0, st, v, x, n);
+    return hipGetLastError();
+}
+
+// X_j <- sum_i V[j*m + i] X_i for j < m, in place; vectors with stride ld.
+// Each thread owns one row across all m vectors.
+template <int MAXM>
+__global__ __launch_bounds__(256) void rotate_kernel(
+    double *__restrict__ X, size_t ld, size_t n, int m,
+    const double *__restrict__ V) {
+    __shared__ double vs[MAXM * MAXM];
+    for (int e = threadIdx.x; e < m * m; e += 256)
+        vs[e] = V[e];
+    __syncthreads();
+    for (size_t r = blockIdx.x * (size_t)256 + threadIdx.x; r < n;
+         r += (size_t)gridDim.x * 256) {
+        double x[MAXM];
+#pragma unroll
+        for (int i = 0; i < MAXM; i++)
+            x[i] = i < m ? X[(size_t)i * ld + r] : 0.0;
+#pragma unroll
+        for (int j = 0; j < MAXM; j++) {
+            if (j < m) {
+                double s = 0.0;
+#pragma unroll
+                for (int i = 0; i < MAXM; i++)
+                    if (i < m)
+                        s += vs[j * m + i] * x[i];
+                X[(size_t)j * ld + r] = s;
+            }
+        }
+    }
+}
+
+hipError_t launch_rotate(double *X, size_t ld, size_t n, int m,
+                         const double *V, hipStream_t st) {
+    dim3 g = grid_for(n);
+    if (m <= 8)
+        hipLaunchKernelGGL((rotate_kernel<8>), g, dim3(256), 0, st, X, ld, n,
+                           m, V);
+    else if (m <= 16)
+        hipLaunchKernelGGL((rotate_kernel<16>), g, dim3(256), 0, st, X, ld, n,
+                           m, V);
+    else if (m <= 32)
+        hipLaunchKernelGGL((rotate_kernel<32>), g, dim3(256), 0, st, X, ld, n,
+                           m, V);
+    else if (m <= 64)
+        hipLaunchKernelGGL((rotate_kernel<64>), g, dim3(256), 0, st, X, ld, n,
+                           m, V);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+// Olsen preconditioner (IterativeMatrixFunctions::olsen_precondition,
+// src/core/iterative_matrix_functions.hpp:93-108), first pass:
+//   t = c / (ld - aa), q = q / (ld - aa) where |ld - aa| > 1e-12,
+// plus partial sums of c.q and c.t (after scaling).
+__global__ __launch_bounds__(256) void olsen_stage1(
+    double *__restrict__ q, const double *__restrict__ c,
+    const double *__restrict__ aa, double ld, double *__restrict__ t, size_t n,
+    double *__restrict__ partial) {
+    __shared__ double sh[256];
+    double s_cq = 0.0, s_ct = 0.0;
+    for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * 256) {
+        double ci = c[i], qi = q[i], ti = ci;
+        const double dd = ld - aa[i];
+        if (fabs(dd) > 1E-12) {
+            ti = ci / dd;
+            qi = qi / dd;
+            q[i] = qi;
+        }
+        t[i] = ti;
+        s_cq += ci * qi;
+        s_ct += ci * ti;
+    }
+    s_cq = block_sum(s_cq, sh);
+    s_ct = block_sum(s_ct, sh);
+    if (threadIdx.x == 0) {
+        partial[(size_t)blockIdx.x * 2 + 0] = s_cq;
+        partial[(size_t)blockIdx.x * 2 + 1] = s_ct;
+    }
+}
+// q += t * (-(c.q) / (c.t))
+__global__ void olsen_stage2(double *__restrict__ q,
+                             const double *__restrict__ t,
+                             const double *__restrict__ sums, size_t n) {
+    const double a = -sums[0] / sums[1];
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        q[i] += a * t[i];
+}
+
+hipError_t launch_olsen(double *q, const double *c, const double *aa,
+                        double ld, double *t, size_t n, double *partial,
+                        double *sums, hipStream_t st) {
+    hipLaunchKernelGGL(olsen_stage1, dim3(kRedGrid), dim3(256), 0, st, q, c,
+                       aa, ld, t, n, partial);
+    hipLaunchKernelGGL(reduce_stage2, dim3(1), dim3(256), 0, st, partial,
+                       kRedGrid, 2, sums);
+    hipLaunchKernelGGL(olsen_stage2, grid_for(n), dim3(256), 0, st, q, t, sums,
+                       n);
+    return hipGetLastError();
+}
+
+int red_grid() { return kRedGrid; }
+
+} // namespace b2g

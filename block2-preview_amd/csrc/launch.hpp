@@ -1,0 +1,40 @@
+// Host-callable launchers of the device kernels (all enqueue on `st`).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+
+namespace b2g {
+
+template <typename TA, typename TB, typename TD, typename TC>
+hipError_t launch_seg_gemm(const TA *A, const TB *B, TD *D, const Task *tasks,
+                           int ntasks, const Seg *segs, uint64_t a_rs,
+                           uint64_t b_rs, uint64_t d_rs, double scale,
+                           int beta0, int n_rhs, int grid, hipStream_t st);
+
+hipError_t launch_diag(const double *L, const double *R, double *Dv,
+                       const DiagBlock *blocks, int nblocks,
+                       const DiagTermDev *terms, int grid, hipStream_t st);
+hipError_t launch_f64_to_f32(const double *s, float *t, size_t n,
+                             hipStream_t st);
+hipError_t launch_dot(const double *x, const double *y, size_t n,
+                      double *partial, double *out, hipStream_t st);
+hipError_t launch_gram(const double *X, int kx, const double *Y, int ky,
+                       size_t ld, size_t n, double *partial, double *out,
+                       hipStream_t st);
+hipError_t launch_axpy(double a, const double *x, double *y, size_t n,
+                       hipStream_t st);
+hipError_t launch_axpy_dev(const double *coef, double sign, const double *x,
+                           double *y, size_t n, hipStream_t st);
+hipError_t launch_scale(double a, double *x, size_t n, hipStream_t st);
+hipError_t launch_copy(const double *x, double *y, size_t n, hipStream_t st);
+hipError_t launch_fill(double v, double *x, size_t n, hipStream_t st);
+hipError_t launch_rotate(double *X, size_t ld, size_t n, int m,
+                         const double *V, hipStream_t st);
+hipError_t launch_olsen(double *q, const double *c, const double *aa,
+                        double ld, double *t, size_t n, double *partial,
+                        double *sums, hipStream_t st);
+int red_grid();
+
+} // namespace b2g

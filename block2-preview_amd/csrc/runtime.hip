@@ -1,0 +1,679 @@
+// C ABI of the thin device layer: contexts, buffers, replay plans, H*psi,
+// diagonal and vector kernels.
+#include "common.hpp"
+#include "launch.hpp"
+
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <sstream>
+#include <tuple>
+#include <unordered_map>
+
+namespace b2g {
+
+static thread_local std::string g_err;
+void set_error(const std::string &msg) { g_err = msg; }
+
+} // namespace b2g
+
+using namespace b2g;
+
+extern "C" const char *b2g_last_error(void) { return g_err.c_str(); }
+
+// ------------------------------------------------------------------ context
+extern "C" int b2g_ctx_create(int device, unsigned flags, b2g_ctx *out) {
+    (void)flags;
+    B2G_REQUIRE(out != nullptr, B2G_ERR_ARG, "b2g_ctx_create: out is NULL");
+    int ndev = 0;
+    B2G_HIP(hipGetDeviceCount(&ndev));
+    B2G_REQUIRE(device >= 0 && device < ndev, B2G_ERR_ARG,
+                "b2g_ctx_create: no such device");
+    B2G_HIP(hipSetDevice(device));
+    b2g_ctx c = new b2g_ctx_s();
+    c->device = device;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) {
+        delete c;
+        set_error("hipGetDeviceProperties failed");
+        return B2G_ERR_DEVICE;
+    }
+    c->num_cu = prop.multiProcessorCount;
+    // a blocking stream (default flags and priority): work on the legacy
+    // default stream, where callers such as PyTorch enqueue by default, is
+    // ordered against it
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamDefault) != hipSuccess) {
+        delete c;
+        set_error("hipStreamCreate failed");
+        return B2G_ERR_DEVICE;
+    }
+    // reduction partials: red_grid() blocks x up to 64*64 values, plus room
+    c->red_len = (size_t)red_grid() * 64 * 64 + 8192;
+    if (hipMalloc(&c->red, c->red_len * sizeof(double)) != hipSuccess ||
+        hipHostMalloc(&c->red_host, 8192 * sizeof(double)) != hipSuccess) {
+        hipStreamDestroy(c->stream);
+        delete c;
+        set_error("allocation of reduction scratch failed");
+        return B2G_ERR_MEMORY;
+    }
+    *out = c;
+    return B2G_OK;
+}
+
+extern "C" int b2g_ctx_destroy(b2g_ctx ctx) {
+    if (ctx == nullptr)
+        return B2G_OK;
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+    hipFree(ctx->red);
+    hipHostFree(ctx->red_host);
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return B2G_OK;
+}
+
+extern "C" void *b2g_ctx_stream(b2g_ctx ctx) {
+    return ctx ? (void *)ctx->stream : nullptr;
+}
+
+extern "C" int b2g_ctx_synchronize(b2g_ctx ctx) {
+    B2G_REQUIRE(ctx != nullptr, B2G_ERR_ARG, "ctx is NULL");
+    B2G_HIP(hipStreamSynchronize(ctx->stream));
+    return B2G_OK;
+}
+
+extern "C" int b2g_ctx_num_cu(b2g_ctx ctx, int *out) {
+    B2G_REQUIRE(ctx != nullptr && out != nullptr, B2G_ERR_ARG, "NULL arg");
+    *out = ctx->num_cu;
+    return B2G_OK;
+}
+
+// ------------------------------------------------------------------ buffers
+static size_t dtype_size(int dtype) {
+    return dtype == B2G_F64 ? 8 : dtype == B2G_F32 ? 4 : 0;
+}
+
+extern "C" int b2g_buf_create(b2g_ctx ctx, size_t n, int dtype, b2g_buf *out) {
+    B2G_REQUIRE(ctx != nullptr && out != nullptr, B2G_ERR_ARG, "NULL arg");
+    B2G_REQUIRE(dtype_size(dtype) != 0, B2G_ERR_ARG, "bad dtype");
+    B2G_HIP(hipSetDevice(ctx->device));
+    b2g_buf b = new b2g_buf_s();
+    b->ctx = ctx;
+    b->n = n;
+    b->dtype = dtype;
+    if (n != 0 && hipMalloc(&b->ptr, n * dtype_size(dtype)) != hipSuccess) {
+        delete b;
+        set_error("b2g_buf_create: hipMalloc failed");
+        return B2G_ERR_MEMORY;
+    }
+    *out = b;
+    return B2G_OK;
+}
+
+extern "C" int b2g_buf_destroy(b2g_buf buf) {
+    if (buf == nullptr)
+        return B2G_OK;
+    hipSetDevice(buf->ctx->device);
+    hipStreamSynchronize(buf->ctx->stream);
+    if (buf->ptr)
+        hipFree(buf->ptr);
+    delete buf;
+    return B2G_OK;
+}
+
+extern "C" int b2g_buf_write(b2g_buf buf, const void *host, size_t offset,
+                             size_t n) {
+    B2G_REQUIRE(buf != nullptr && (host != nullptr || n == 0), B2G_ERR_ARG,
+                "NULL arg");
+    B2G_REQUIRE(offset + n <= buf->n, B2G_ERR_ARG, "b2g_buf_write: range");
+    if (n == 0)
+        return B2G_OK;
+    const size_t es = dtype_size(buf->dtype);
+    B2G_HIP(hipMemcpyAsync((char *)buf->ptr + offset * es, host, n * es,
+                           hipMemcpyHostToDevice, buf->ctx->stream));
+    B2G_HIP(hipStreamSynchronize(buf->ctx->stream));
+    return B2G_OK;
+}
+
+extern "C" int b2g_buf_read(b2g_buf buf, void *host, size_t offset, size_t n) {
+    B2G_REQUIRE(buf != nullptr && (host != nullptr || n == 0), B2G_ERR_ARG,
+                "NULL arg");
+    B2G_REQUIRE(offset + n <= buf->n, B2G_ERR_ARG, "b2g_buf_read: range");
+    if (n == 0)
+        return B2G_OK;
+    const size_t es = dtype_size(buf->dtype);
+    B2G_HIP(hipMemcpyAsync(host, (char *)buf->ptr + offset * es, n * es,
+                           hipMemcpyDeviceToHost, buf->ctx->stream));
+    B2G_HIP(hipStreamSynchronize(buf->ctx->stream));
+    return B2G_OK;
+}
+
+extern "C" int b2g_upload_blocks(b2g_ctx ctx, const void *host, size_t n,
+                                 int dtype, b2g_buf *out) {
+    int r = b2g_buf_create(ctx, n, dtype, out);
+    if (r != B2G_OK)
+        return r;
+    r = b2g_buf_write(*out, host, 0, n);
+    if (r != B2G_OK) {
+        b2g_buf_destroy(*out);
+        *out = nullptr;
+    }
+    return r;
+}
+
+extern "C" void *b2g_buf_device_ptr(b2g_buf buf) {
+    return buf ? buf->ptr : nullptr;
+}
+extern "C" size_t b2g_buf_size(b2g_buf buf) { return buf ? buf->n : 0; }
+
+// -------------------------------------------------------------------- plans
+namespace b2g {
+
+struct Chunk {
+    // host images
+    std::vector<Task> t1, t2;
+    std::vector<Seg> s1, s2;
+    uint64_t w_len = 0; // W elements per right-hand side
+    // device copies (inside plan->dev)
+    Task *d_t1 = nullptr, *d_t2 = nullptr;
+    Seg *d_s1 = nullptr, *d_s2 = nullptr;
+};
+
+} // namespace b2g
+
+struct b2g_plan_s {
+    b2g_ctx ctx = nullptr;
+    b2g_buf lbuf = nullptr, rbuf = nullptr;
+    size_t x_len = 0, y_len = 0;
+    uint64_t macs = 0;
+    std::vector<Chunk> chunks;
+    uint64_t w_max = 0;
+    void *dev = nullptr; // one allocation for all task / segment arrays
+    float *l32 = nullptr, *r32 = nullptr;
+    void *w = nullptr; // W scratch
+    size_t w_bytes = 0;
+    unsigned flags = 0;
+    // optional per-phase timing: event triples (before phase 1, between the
+    // phases, after phase 2) resolved when queried
+    bool timing = false;
+    std::vector<hipEvent_t> events;
+};
+
+namespace b2g {
+
+static uint64_t tile_cost(const Task &t, const std::vector<Seg> &segs) {
+    uint64_t k = 0;
+    for (uint32_t s = t.seg_begin; s < t.seg_end; s++)
+        k += segs[s].K;
+    return k + 1;
+}
+
+// tiles of an M x N output block
+static void add_tiles(std::vector<Task> &tasks, uint64_t d_off, uint32_t M,
+                      uint32_t N, uint32_t sb, uint32_t se, uint32_t beta) {
+    const uint32_t T = 64;
+    for (uint32_t r = 0; r < M; r += T)
+        for (uint32_t c = 0; c < N; c += T) {
+            Task t;
+            t.d_off = d_off;
+            t.ldd = N;
+            t.M = M;
+            t.N = N;
+            t.row0 = r;
+            t.col0 = c;
+            t.seg_begin = sb;
+            t.seg_end = se;
+            t.beta = beta;
+            tasks.push_back(t);
+        }
+}
+
+static void sort_tasks(std::vector<Task> &tasks, const std::vector<Seg> &segs) {
+    std::stable_sort(tasks.begin(), tasks.end(),
+                     [&segs](const Task &a, const Task &b) {
+                         return tile_cost(a, segs) > tile_cost(b, segs);
+                     });
+}
+
+static uint64_t w_cap_elems() {
+    const char *e = getenv("B2G_W_CAP");
+    if (e != nullptr) {
+        long long v = atoll(e);
+        if (v > 0)
+            return (uint64_t)v;
+    }
+    return (uint64_t)1 << 28; // 256 Mi elements per right-hand side
+}
+
+static int compile_plan(b2g_plan p, const b2g_pair *pairs, size_t n_pairs,
+                        const b2g_block *yb, size_t n_yb) {
+    const size_t llen = p->lbuf->n, rlen = p->rbuf->n;
+    std::unordered_map<uint64_t, size_t> yidx;
+    for (size_t i = 0; i < n_yb; i++) {
+        B2G_REQUIRE(yb[i].off + (uint64_t)yb[i].m * yb[i].n <= p->y_len,
+                    B2G_ERR_ARG, "output block outside the output vector");
+        yidx[yb[i].off] = i;
+    }
+    p->macs = 0;
+    for (size_t i = 0; i < n_pairs; i++) {
+        const b2g_pair &q = pairs[i];
+        std::ostringstream ss;
+        ss << "pair " << i << ": ";
+        B2G_REQUIRE(q.kind == 0, B2G_ERR_ARG, ss.str() + "unsupported kind");
+        B2G_REQUIRE(q.a_off + (uint64_t)q.c_m * q.v_m <= llen, B2G_ERR_ARG,
+                    ss.str() + "A block outside the L buffer");
+        B2G_REQUIRE(q.b_off + (uint64_t)q.c_n * q.v_n <= rlen, B2G_ERR_ARG,
+                    ss.str() + "B block outside the R buffer");
+        B2G_REQUIRE(q.c_off + (uint64_t)q.c_m * q.c_n <= p->x_len, B2G_ERR_ARG,
+                    ss.str() + "C block outside the input vector");
+        auto it = yidx.find(q.v_off);
+        B2G_REQUIRE(it != yidx.end(), B2G_ERR_ARG,
+                    ss.str() + "V block is not an output block");
+        B2G_REQUIRE(yb[it->second].m == q.v_m && yb[it->second].n == q.v_n,
+                    B2G_ERR_ARG, ss.str() + "V block shape mismatch");
+        p->macs += (uint64_t)q.c_m * q.c_n * q.v_n +
+                   (uint64_t)q.v_m * q.c_m * q.v_n;
+    }
+    // split the pair list into chunks whose W scratch fits the cap
+    const uint64_t cap = w_cap_elems();
+    size_t i0 = 0;
+    bool first = true;
+    while (i0 < n_pairs || first) {
+        Chunk ch;
+        typedef std::tuple<uint64_t, uint64_t, uint32_t> WKey;
+        std::map<WKey, uint64_t> wmap; // -> W offset
+        std::vector<uint64_t> pw;      // W offset per pair of this chunk
+        size_t i1 = i0;
+        for (; i1 < n_pairs; i1++) {
+            const b2g_pair &q = pairs[i1];
+            WKey key(q.b_off, q.c_off, q.conj & 2);
+            auto it = wmap.find(key);
+            if (it == wmap.end()) {
+                const uint64_t sz = (uint64_t)q.c_m * q.v_n;
+                if (ch.w_len + sz > cap && i1 > i0)
+                    break;
+                it = wmap.emplace(key, ch.w_len).first;
+                // phase 1: W = C * op(B)
+                Seg s;
+                s.a_off = q.c_off;
+                s.lda = q.c_n;
+                s.b_off = q.b_off;
+                const bool tb = !(q.conj & 2);
+                s.ldb = tb ? q.c_n : q.v_n;
+                s.K = q.c_n;
+                s.flags = tb ? 2u : 0u;
+                s.alpha = 1.0;
+                const uint32_t si = (uint32_t)ch.s1.size();
+                ch.s1.push_back(s);
+                add_tiles(ch.t1, ch.w_len, q.c_m, q.v_n, si, si + 1, 0);
+                ch.w_len += sz;
+            }
+            pw.push_back(it->second);
+        }
+        // phase 2: group this chunk's pairs by output block (record order)
+        std::map<uint64_t, std::vector<size_t>> byv;
+        for (size_t i = i0; i < i1; i++)
+            byv[pairs[i].v_off].push_back(i);
+        auto emit = [&](const b2g_block &blk, const std::vector<size_t> *lst,
+                        uint32_t beta) {
+            const uint32_t sb = (uint32_t)ch.s2.size();
+            if (lst != nullptr)
+                for (size_t i : *lst) {
+                    const b2g_pair &q = pairs[i];
+                    Seg s;
+                    s.a_off = q.a_off;
+                    const bool ta = q.conj & 1;
+                    s.lda = ta ? q.v_m : q.c_m;
+                    s.b_off = pw[i - i0];
+                    s.ldb = q.v_n;
+                    s.K = q.c_m;
+                    s.flags = ta ? 1u : 0u;
+                    s.alpha = q.alpha;
+                    ch.s2.push_back(s);
+                }
+            add_tiles(ch.t2, blk.off, blk.m, blk.n, sb,
+                      (uint32_t)ch.s2.size(), beta);
+        };
+        if (first) {
+            for (size_t b = 0; b < n_yb; b++) {
+                auto it = byv.find(yb[b].off);
+                emit(yb[b], it == byv.end() ? nullptr : &it->second, 0);
+            }
+        } else {
+            for (auto &kv : byv)
+                emit(yb[yidx[kv.first]], &kv.second, 1);
+        }
+        sort_tasks(ch.t1, ch.s1);
+        sort_tasks(ch.t2, ch.s2);
+        p->w_max = std::max(p->w_max, ch.w_len);
+        p->chunks.push_back(std::move(ch));
+        first = false;
+        i0 = i1;
+        if (n_pairs == 0)
+            break;
+    }
+    // upload all task/segment arrays in one allocation
+    size_t bytes = 0;
+    for (auto &ch : p->chunks)
+        bytes += (ch.t1.size() + ch.t2.size()) * sizeof(Task) +
+                 (ch.s1.size() + ch.s2.size()) * sizeof(Seg);
+    std::vector<char> host(bytes + 16);
+    B2G_HIP(hipMalloc(&p->dev, bytes + 16));
+    size_t off = 0;
+    auto put = [&](const void *src, size_t nb) -> size_t {
+        size_t o = off;
+        if (nb)
+            memcpy(host.data() + o, src, nb);
+        off += nb;
+        return o;
+    };
+    std::vector<std::tuple<size_t, size_t, size_t, size_t>> offs;
+    for (auto &ch : p->chunks) {
+        size_t a = put(ch.t1.data(), ch.t1.size() * sizeof(Task));
+        size_t b = put(ch.t2.data(), ch.t2.size() * sizeof(Task));
+        size_t c = put(ch.s1.data(), ch.s1.size() * sizeof(Seg));
+        size_t d = put(ch.s2.data(), ch.s2.size() * sizeof(Seg));
+        offs.emplace_back(a, b, c, d);
+    }
+    B2G_HIP(hipMemcpyAsync(p->dev, host.data(), bytes, hipMemcpyHostToDevice,
+                           p->ctx->stream));
+    B2G_HIP(hipStreamSynchronize(p->ctx->stream));
+    char *base = (char *)p->dev;
+    for (size_t i = 0; i < p->chunks.size(); i++) {
+        Chunk &ch = p->chunks[i];
+        ch.d_t1 = (Task *)(base + std::get<0>(offs[i]));
+        ch.d_t2 = (Task *)(base + std::get<1>(offs[i]));
+        ch.d_s1 = (Seg *)(base + std::get<2>(offs[i]));
+        ch.d_s2 = (Seg *)(base + std::get<3>(offs[i]));
+    }
+    return B2G_OK;
+}
+
+} // namespace b2g
+
+extern "C" int b2g_upload_plan(b2g_ctx ctx, b2g_buf lbuf, b2g_buf rbuf,
+                               const b2g_pair *pairs, size_t n_pairs,
+                               const b2g_block *yblocks, size_t n_yblocks,
+                               size_t x_len, size_t y_len, unsigned flags,
+                               b2g_plan *out) {
+    B2G_REQUIRE(ctx && lbuf && rbuf && out, B2G_ERR_ARG, "NULL arg");
+    B2G_REQUIRE(pairs != nullptr || n_pairs == 0, B2G_ERR_ARG, "NULL pairs");
+    B2G_REQUIRE(yblocks != nullptr || n_yblocks == 0, B2G_ERR_ARG,
+                "NULL yblocks");
+    B2G_REQUIRE(lbuf->dtype == B2G_F64 && rbuf->dtype == B2G_F64, B2G_ERR_ARG,
+                "operator buffers must be FP64");
+    B2G_HIP(hipSetDevice(ctx->device));
+    b2g_plan p = new b2g_plan_s();
+    p->ctx = ctx;
+    p->lbuf = lbuf;
+    p->rbuf = rbuf;
+    p->x_len = x_len;
+    p->y_len = y_len;
+    p->flags = flags;
+    int r = compile_plan(p, pairs, n_pairs, yblocks, n_yblocks);
+    if (r == B2G_OK && (flags & 1)) {
+        if (hipMalloc(&p->l32, std::max<size_t>(lbuf->n, 1) * 4) !=
+                hipSuccess ||
+            hipMalloc(&p->r32, std::max<size_t>(rbuf->n, 1) * 4) !=
+                hipSuccess) {
+            set_error("b2g_upload_plan: FP32 shadow allocation failed");
+            r = B2G_ERR_MEMORY;
+        } else if (launch_f64_to_f32((const double *)lbuf->ptr, p->l32,
+                                     lbuf->n, ctx->stream) != hipSuccess ||
+                   launch_f64_to_f32((const double *)rbuf->ptr, p->r32,
+                                     rbuf->n, ctx->stream) != hipSuccess ||
+                   hipStreamSynchronize(ctx->stream) != hipSuccess) {
+            set_error("b2g_upload_plan: FP32 conversion failed");
+            r = B2G_ERR_DEVICE;
+        }
+    }
+    if (r != B2G_OK) {
+        b2g_plan_destroy(p);
+        return r;
+    }
+    *out = p;
+    return B2G_OK;
+}
+
+extern "C" int b2g_plan_destroy(b2g_plan p) {
+    if (p == nullptr)
+        return B2G_OK;
+    hipSetDevice(p->ctx->device);
+    hipStreamSynchronize(p->ctx->stream);
+    for (hipEvent_t e : p->events)
+        hipEventDestroy(e);
+    if (p->dev)
+        hipFree(p->dev);
+    if (p->l32)
+        hipFree(p->l32);
+    if (p->r32)
+        hipFree(p->r32);
+    if (p->w)
+        hipFree(p->w);
+    delete p;
+    return B2G_OK;
+}
+
+extern "C" int b2g_plan_set_timing(b2g_plan p, int enable) {
+    B2G_REQUIRE(p, B2G_ERR_ARG, "NULL plan");
+    p->timing = enable != 0;
+    return B2G_OK;
+}
+
+extern "C" int b2g_plan_timing(b2g_plan p, double *ms1, double *ms2,
+                               uint64_t *n_launch) {
+    B2G_REQUIRE(p, B2G_ERR_ARG, "NULL plan");
+    double a = 0, b = 0;
+    uint64_t n = 0;
+    for (size_t i = 0; i + 2 < p->events.size(); i += 3) {
+        B2G_HIP(hipEventSynchronize(p->events[i + 2]));
+        float t1 = 0, t2 = 0;
+        B2G_HIP(hipEventElapsedTime(&t1, p->events[i], p->events[i + 1]));
+        B2G_HIP(hipEventElapsedTime(&t2, p->events[i + 1], p->events[i + 2]));
+        a += t1, b += t2, n++;
+    }
+    for (hipEvent_t e : p->events)
+        B2G_HIP(hipEventDestroy(e));
+    p->events.clear();
+    if (ms1)
+        *ms1 = a;
+    if (ms2)
+        *ms2 = b;
+    if (n_launch)
+        *n_launch = n;
+    return B2G_OK;
+}
+
+extern "C" int b2g_plan_macs(b2g_plan p, uint64_t *out) {
+    B2G_REQUIRE(p && out, B2G_ERR_ARG, "NULL arg");
+    *out = p->macs;
+    return B2G_OK;
+}
+
+extern "C" int b2g_plan_stats(b2g_plan p, uint64_t *n1, uint64_t *n2,
+                              uint64_t *scratch) {
+    B2G_REQUIRE(p, B2G_ERR_ARG, "NULL plan");
+    uint64_t a = 0, b = 0;
+    for (auto &ch : p->chunks)
+        a += ch.t1.size(), b += ch.t2.size();
+    if (n1)
+        *n1 = a;
+    if (n2)
+        *n2 = b;
+    if (scratch)
+        *scratch = p->w_max;
+    return B2G_OK;
+}
+
+extern "C" int b2g_hpsi(b2g_ctx ctx, b2g_plan p, const double *x, double *y,
+                        int n_rhs, double scale, int accumulate,
+                        int precision) {
+    B2G_REQUIRE(ctx && p && x && y, B2G_ERR_ARG, "NULL arg");
+    B2G_REQUIRE(n_rhs >= 1 && n_rhs <= 65535, B2G_ERR_ARG, "bad n_rhs");
+    B2G_REQUIRE(precision == B2G_PREC_F64 || precision == B2G_PREC_F32,
+                B2G_ERR_ARG, "unsupported precision for this plan");
+    B2G_REQUIRE(precision != B2G_PREC_F32 || p->l32 != nullptr, B2G_ERR_STATE,
+                "plan was uploaded without FP32 shadows");
+    B2G_HIP(hipSetDevice(ctx->device));
+    const size_t es = precision == B2G_PREC_F64 ? 8 : 4;
+    const size_t need = std::max<uint64_t>(p->w_max, 1) * n_rhs * es;
+    if (need > p->w_bytes) {
+        if (p->w) {
+            B2G_HIP(hipStreamSynchronize(ctx->stream));
+            B2G_HIP(hipFree(p->w));
+            p->w = nullptr;
+            p->w_bytes = 0;
+        }
+        B2G_HIP(hipMalloc(&p->w, need));
+        p->w_bytes = need;
+    }
+    const int grid = ctx->num_cu * 8;
+    for (size_t ic = 0; ic < p->chunks.size(); ic++) {
+        const Chunk &ch = p->chunks[ic];
+        // chunk 0 overwrites unless accumulating; later chunks' tasks add
+        const int beta0 = ic == 0 ? (accumulate ? 1 : 0) : 1;
+        hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+        if (p->timing) {
+            for (int k = 0; k < 3; k++) {
+                B2G_HIP(hipEventCreate(&ev[k]));
+                p->events.push_back(ev[k]);
+            }
+        }
+        auto mark = [&](int k) -> hipError_t {
+            return p->timing ? hipEventRecord(ev[k], ctx->stream) : hipSuccess;
+        };
+        B2G_HIP(mark(0));
+        if (precision == B2G_PREC_F64) {
+            double *W = (double *)p->w;
+            B2G_HIP((launch_seg_gemm<double, double, double, double>(
+                x, (const double *)p->rbuf->ptr, W, ch.d_t1, (int)ch.t1.size(),
+                ch.d_s1, p->x_len, 0, p->w_max, 1.0, 0, n_rhs, grid,
+                ctx->stream)));
+            B2G_HIP(mark(1));
+            B2G_HIP((launch_seg_gemm<double, double, double, double>(
+                (const double *)p->lbuf->ptr, W, y, ch.d_t2, (int)ch.t2.size(),
+                ch.d_s2, 0, p->w_max, p->y_len, scale, beta0, n_rhs, grid,
+                ctx->stream)));
+            B2G_HIP(mark(2));
+        } else {
+            float *W = (float *)p->w;
+            B2G_HIP((launch_seg_gemm<double, float, float, float>(
+                x, p->r32, W, ch.d_t1, (int)ch.t1.size(), ch.d_s1, p->x_len, 0,
+                p->w_max, 1.0, 0, n_rhs, grid, ctx->stream)));
+            B2G_HIP(mark(1));
+            B2G_HIP((launch_seg_gemm<float, float, double, float>(
+                p->l32, W, y, ch.d_t2, (int)ch.t2.size(), ch.d_s2, 0, p->w_max,
+                p->y_len, scale, beta0, n_rhs, grid, ctx->stream)));
+            B2G_HIP(mark(2));
+        }
+    }
+    return B2G_OK;
+}
+
+// ----------------------------------------------------------------- diagonal
+extern "C" int b2g_diag(b2g_ctx ctx, b2g_buf lbuf, b2g_buf rbuf,
+                        const b2g_diag_term *terms, size_t n_terms,
+                        const b2g_block *dblocks, size_t n_dblocks, double *d,
+                        size_t d_len) {
+    B2G_REQUIRE(ctx && lbuf && rbuf && d, B2G_ERR_ARG, "NULL arg");
+    B2G_REQUIRE(terms != nullptr || n_terms == 0, B2G_ERR_ARG, "NULL terms");
+    B2G_REQUIRE(lbuf->dtype == B2G_F64 && rbuf->dtype == B2G_F64, B2G_ERR_ARG,
+                "operator buffers must be FP64");
+    std::unordered_map<uint64_t, size_t> didx;
+    for (size_t i = 0; i < n_dblocks; i++) {
+        B2G_REQUIRE(dblocks[i].off + (uint64_t)dblocks[i].m * dblocks[i].n <=
+                        d_len,
+                    B2G_ERR_ARG, "diagonal block outside the output");
+        didx[dblocks[i].off] = i;
+    }
+    std::vector<std::vector<size_t>> by(n_dblocks);
+    for (size_t i = 0; i < n_terms; i++) {
+        const b2g_diag_term &t = terms[i];
+        auto it = didx.find(t.d_off);
+        B2G_REQUIRE(it != didx.end(), B2G_ERR_ARG,
+                    "diagonal term targets no output block");
+        const b2g_block &blk = dblocks[it->second];
+        B2G_REQUIRE(blk.m == t.a_n && blk.n == t.b_n, B2G_ERR_ARG,
+                    "diagonal term shape mismatch");
+        B2G_REQUIRE(t.a_off + (uint64_t)t.a_n * t.a_n <= lbuf->n &&
+                        t.b_off + (uint64_t)t.b_n * t.b_n <= rbuf->n,
+                    B2G_ERR_ARG, "diagonal term outside operator buffers");
+        by[it->second].push_back(i);
+    }
+    std::vector<DiagBlock> blocks(n_dblocks);
+    std::vector<DiagTermDev> dterms;
+    dterms.reserve(n_terms);
+    for (size_t b = 0; b < n_dblocks; b++) {
+        blocks[b].d_off = dblocks[b].off;
+        blocks[b].m = dblocks[b].m;
+        blocks[b].n = dblocks[b].n;
+        blocks[b].t_begin = (uint32_t)dterms.size();
+        for (size_t i : by[b]) {
+            DiagTermDev t;
+            t.a_off = terms[i].a_off;
+            t.b_off = terms[i].b_off;
+            t.a_n = terms[i].a_n;
+            t.b_n = terms[i].b_n;
+            t.alpha = terms[i].alpha;
+            dterms.push_back(t);
+        }
+        blocks[b].t_end = (uint32_t)dterms.size();
+    }
+    B2G_HIP(hipSetDevice(ctx->device));
+    const size_t nb1 = blocks.size() * sizeof(DiagBlock),
+                 nb2 = dterms.size() * sizeof(DiagTermDev);
+    char *dev = nullptr;
+    B2G_HIP(hipMalloc(&dev, nb1 + nb2 + 16));
+    if (nb1)
+        B2G_HIP(hipMemcpyAsync(dev, blocks.data(), nb1, hipMemcpyHostToDevice,
+                               ctx->stream));
+    if (nb2)
+        B2G_HIP(hipMemcpyAsync(dev + nb1, dterms.data(), nb2,
+                               hipMemcpyHostToDevice, ctx->stream));
+    hipError_t e = launch_fill(0.0, d, d_len, ctx->stream);
+    if (e == hipSuccess)
+        e = launch_diag((const double *)lbuf->ptr, (const double *)rbuf->ptr,
+                        d, (const DiagBlock *)dev, (int)blocks.size(),
+                        (const DiagTermDev *)(dev + nb1), ctx->num_cu * 4,
+                        ctx->stream);
+    hipError_t e2 = hipStreamSynchronize(ctx->stream);
+    hipFree(dev);
+    B2G_HIP(e);
+    B2G_HIP(e2);
+    return B2G_OK;
+}
+
+// ------------------------------------------------------------- vector ops
+extern "C" int b2g_vec_dot(b2g_ctx ctx, const double *x, const double *y,
+                           size_t n, double *out_host) {
+    B2G_REQUIRE(ctx && x && y && out_host, B2G_ERR_ARG, "NULL arg");
+    B2G_HIP(launch_dot(x, y, n, ctx->red, ctx->red + ctx->red_len - 8,
+                       ctx->stream));
+    B2G_HIP(hipMemcpyAsync(ctx->red_host, ctx->red + ctx->red_len - 8,
+                           sizeof(double), hipMemcpyDeviceToHost,
+                           ctx->stream));
+    B2G_HIP(hipStreamSynchronize(ctx->stream));
+    *out_host = ctx->red_host[0];
+    return B2G_OK;
+}
+
+extern "C" int b2g_vec_axpy(b2g_ctx ctx, double a, const double *x, double *y,
+                            size_t n) {
+    B2G_REQUIRE(ctx && x && y, B2G_ERR_ARG, "NULL arg");
+    B2G_HIP(launch_axpy(a, x, y, n, ctx->stream));
+    return B2G_OK;
+}
+
+extern "C" int b2g_vec_scale(b2g_ctx ctx, double a, double *x, size_t n) {
+    B2G_REQUIRE(ctx && x, B2G_ERR_ARG, "NULL arg");
+    B2G_HIP(launch_scale(a, x, n, ctx->stream));
+    return B2G_OK;
+}
+
+extern "C" int b2g_vec_copy(b2g_ctx ctx, const double *x, double *y,
+                            size_t n) {
+    B2G_REQUIRE(ctx && x && y, B2G_ERR_ARG, "NULL arg");
+    B2G_HIP(launch_copy(x, y, n, ctx->stream));
+    return B2G_OK;
+}

@@ -1,0 +1,275 @@
+/*
+ * b2g: MI355X (gfx950) two-site DMRG effective-Hamiltonian hot path.
+ *
+ * C ABI of libb2g.so.  Plain pointers and sizes, int status codes
+ * (B2G_OK == 0), no exceptions across the boundary, error text through
+ * b2g_last_error().  All device memory is owned by a context.
+ *
+ * Two layers are exported:
+ *
+ *  (1) The thin device layer (b2g_ctx_*, b2g_buf_*, b2g_upload_plan,
+ *      b2g_hpsi, b2g_diag, b2g_vec_*): a recorded list of GEMM-chain pairs
+ *      (the device image of BatchGEMMSeq's recorded batches) replayed on the
+ *      GPU.  It replaces the replay side of the reference:
+ *        TensorFunctions::operator()(b, c, scale)      src/core/tensor_functions.hpp:59-62
+ *        BatchGEMMSeq::operator()(c, v)                 src/core/batch_gemm.hpp:1570-1691
+ *        AdvancedGEMM::tensor_product_diagonal          src/core/batch_gemm.hpp:507-513
+ *
+ *  (2) The effective-Hamiltonian layer (b2g_effham_*): takes the block-sparse
+ *      problem exactly as block2's EffectiveHamiltonian constructor receives it
+ *      (operator infos, operator blocks, the H_eff expression, bra/ket infos),
+ *      builds the connection lists and the pair list on the host, uploads them
+ *      and offers H*psi, the diagonal and the Davidson solve.  It replaces
+ *        EffectiveHamiltonian<S,FL,MPS>  src/dmrg/effective_hamiltonian.hpp:99-1162
+ *          ctor (diag + wfn_infos)        :126-224
+ *          precompute / post_precompute   :226-253
+ *          operator()(b, c, idx, factor)  :459-477
+ *          eigs(...) -> (E, ndav, nflop, tdav)  :480-568
+ *          deallocate()                   :1136-1162
+ *        EffectiveHamiltonian<S,FL,MultiMPS>::eigs (k roots)  :1708-1800
+ *        tensor_left/right_partial_expectation (backward)
+ *                                         src/core/operator_functions.hpp:354-472
+ */
+#ifndef B2G_H
+#define B2G_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ----------------------------------------------------------------- status */
+#define B2G_OK 0
+#define B2G_ERR_ARG 1     /* invalid argument / inconsistent shapes        */
+#define B2G_ERR_DEVICE 2  /* HIP runtime error                             */
+#define B2G_ERR_MEMORY 3  /* allocation failed                             */
+#define B2G_ERR_NOCONV 4  /* Davidson hit max_iter before convergence      */
+#define B2G_ERR_STATE 5   /* object used in the wrong state                */
+
+/* element types */
+#define B2G_F64 0
+#define B2G_F32 1
+
+/* precision of the H*psi contraction (b2g_hpsi / b2g_effham_*) */
+#define B2G_PREC_F64 0    /* FP64 MFMA, FP64 accumulate                    */
+#define B2G_PREC_F32 1    /* FP32 MFMA, FP32 accumulate                    */
+#define B2G_PREC_BF16X3 2 /* split-BF16 three-term product, FP32 accumulate */
+
+/* symmetry of the packed quantum labels (block2 SZLongLong / SU2LongLong
+ * bit layouts, src/core/symmetry.hpp:656-732, 1184-1290)                  */
+#define B2G_SZ 0
+#define B2G_SU2 1
+
+typedef struct b2g_ctx_s *b2g_ctx;
+typedef struct b2g_buf_s *b2g_buf;
+typedef struct b2g_plan_s *b2g_plan;
+typedef struct b2g_effham_s *b2g_effham;
+
+/* Last error message of the calling thread ("" if none). */
+const char *b2g_last_error(void);
+
+/* ------------------------------------------------------------ context */
+/* One context per host thread; it owns a HIP stream on `device`.        */
+int b2g_ctx_create(int device, unsigned flags, b2g_ctx *out);
+int b2g_ctx_destroy(b2g_ctx ctx);
+int b2g_ctx_synchronize(b2g_ctx ctx);
+/* the context's hipStream_t (for callers that order their own work or
+ * record events against it)                                              */
+void *b2g_ctx_stream(b2g_ctx ctx);
+/* number of compute units of the context's device */
+int b2g_ctx_num_cu(b2g_ctx ctx, int *out);
+
+/* -------------------------------------------------------------- buffers */
+/* Device buffer of n elements of dtype (B2G_F64 / B2G_F32).              */
+int b2g_buf_create(b2g_ctx ctx, size_t n, int dtype, b2g_buf *out);
+int b2g_buf_destroy(b2g_buf buf);
+/* Device buffer initialised from host memory.  Replaces the per-site
+ * upload of L/R operator blocks (the data behind lopt/ropt->ops).        */
+int b2g_upload_blocks(b2g_ctx ctx, const void *host, size_t n, int dtype,
+                      b2g_buf *out);
+int b2g_buf_write(b2g_buf buf, const void *host, size_t offset, size_t n);
+int b2g_buf_read(b2g_buf buf, void *host, size_t offset, size_t n);
+/* raw device address (for callers that hand device pointers in directly) */
+void *b2g_buf_device_ptr(b2g_buf buf);
+size_t b2g_buf_size(b2g_buf buf);
+
+/* ------------------------------------------------------------ pair list */
+/* One GEMM chain  V[v] += alpha * op(A) * C[c] * op(B)^T  as recorded by
+ * OperatorFunctions::tensor_product_multiply -> BatchGEMMSeq::rotate
+ * (src/core/operator_functions.hpp:474-542, src/core/batch_gemm.hpp:893-902,
+ * src/core/matrix_functions.hpp:998-1009).  All blocks are row-major.
+ *   conj bit 0 set:   op(A) = A^T  (A stored c_m x v_m) else A (v_m x c_m)
+ *   conj bit 1 set:   C*B         (B stored c_n x v_n) else C*B^T (v_n x c_n)
+ * Offsets are element offsets into the L buffer (A), R buffer (B), the input
+ * vector (C) and the output vector (V).                                  */
+typedef struct {
+    uint64_t a_off, b_off, c_off, v_off;
+    uint32_t c_m, c_n, v_m, v_n;
+    uint32_t conj;
+    uint32_t kind; /* 0: two-tensor rotate */
+    double alpha;
+} b2g_pair;
+
+/* Output-vector block table: every bra block (offset, rows, cols), so that
+ * blocks no pair touches are written too.                                */
+typedef struct {
+    uint64_t off;
+    uint32_t m, n;
+} b2g_block;
+
+/* Diagonal contribution  D[d](i,j) += alpha * A(i,i) * B(j,j)
+ * (AdvancedGEMM::tensor_product_diagonal, src/core/batch_gemm.hpp:507-513;
+ * GMatrixFunctions::tensor_product_diagonal, matrix_functions.hpp:1204-1212).
+ * A is a square block (a_n x a_n) at a_off in the L buffer, B square
+ * (b_n x b_n) at b_off in the R buffer, D (a_n x b_n) at d_off.          */
+typedef struct {
+    uint64_t a_off, b_off, d_off;
+    uint32_t a_n, b_n;
+    double alpha;
+} b2g_diag_term;
+
+/* Compile and upload a replay plan.  `lbuf`/`rbuf` hold the operator blocks
+ * (FP64); x_len / y_len are the input/output vector lengths.  `flags` bit 0
+ * also builds FP32 shadows of the operator buffers for B2G_PREC_F32, bit 1
+ * BF16 hi/lo shadows for B2G_PREC_BF16X3.                                 */
+int b2g_upload_plan(b2g_ctx ctx, b2g_buf lbuf, b2g_buf rbuf,
+                    const b2g_pair *pairs, size_t n_pairs,
+                    const b2g_block *yblocks, size_t n_yblocks, size_t x_len,
+                    size_t y_len, unsigned flags, b2g_plan *out);
+int b2g_plan_destroy(b2g_plan plan);
+/* Reference-order work of one H*psi, in multiply-adds
+ * (sum over pairs of c_m*c_n*v_n + v_m*c_m*v_n, the reference's nflop,
+ * src/core/batch_gemm.hpp:307).  FLOPs = 2 * this.                        */
+int b2g_plan_macs(b2g_plan plan, uint64_t *out);
+/* Per-phase kernel timing with HIP events on the context stream: while
+ * enabled, every b2g_hpsi records events around its two launches per
+ * chunk; b2g_plan_timing waits for them and returns the summed durations
+ * (ms) of phase 1 (work = C op(B)) and phase 2 (V += alpha op(A) work) and
+ * the number of chunk launches, then clears the record.                  */
+int b2g_plan_set_timing(b2g_plan plan, int enable);
+int b2g_plan_timing(b2g_plan plan, double *ms_phase1, double *ms_phase2,
+                    uint64_t *n_launch);
+/* Number of kernel tasks per phase and scratch size, for reporting.      */
+int b2g_plan_stats(b2g_plan plan, uint64_t *n_tasks1, uint64_t *n_tasks2,
+                   uint64_t *scratch_elems);
+
+/* y[r] = (accumulate ? y[r] : 0) + scale * H * x[r]  for r < n_rhs.
+ * x and y are device arrays of n_rhs stacked vectors (x_len, y_len each),
+ * FP64.  precision: B2G_PREC_*.  Enqueued on the context stream.
+ * Replaces TensorFunctions::operator()(b, c, scale).                     */
+int b2g_hpsi(b2g_ctx ctx, b2g_plan plan, const double *x, double *y,
+             int n_rhs, double scale, int accumulate, int precision);
+
+/* Diagonal of H into d (device, y_len doubles, overwritten).            */
+int b2g_diag(b2g_ctx ctx, b2g_buf lbuf, b2g_buf rbuf,
+             const b2g_diag_term *terms, size_t n_terms,
+             const b2g_block *dblocks, size_t n_dblocks, double *d,
+             size_t d_len);
+
+/* ------------------------------------------------- device vector kernels */
+/* FP64 BLAS-1 style kernels used by the Davidson loop; n-length device
+ * vectors, deterministic reductions (fixed block order).                 */
+int b2g_vec_dot(b2g_ctx ctx, const double *x, const double *y, size_t n,
+                double *out_host);
+int b2g_vec_axpy(b2g_ctx ctx, double a, const double *x, double *y,
+                 size_t n);
+int b2g_vec_scale(b2g_ctx ctx, double a, double *x, size_t n);
+int b2g_vec_copy(b2g_ctx ctx, const double *x, double *y, size_t n);
+
+/* ------------------------------------------------ effective Hamiltonian */
+/* Block-sparse matrix info (SparseMatrixInfo, sparse_matrix.hpp:47-877):
+ * n blocks with composite quanta (sorted ascending), row/col dims and
+ * element offsets of each block.                                          */
+typedef struct {
+    uint64_t delta_quantum;
+    int32_t is_fermion;
+    int32_t n;
+    const uint64_t *quanta;
+    const uint32_t *n_states_bra;
+    const uint32_t *n_states_ket;
+    const uint64_t *block_shifts;
+} b2g_sparse_info;
+
+/* One operator block set (an entry of lopt/ropt->ops): delta quantum,
+ * scalar factor, the index of its info in left/right_op_infos and the
+ * element offset of its data in the operator buffer.                      */
+typedef struct {
+    uint64_t delta_quantum;
+    double factor;
+    int32_t info;
+    int32_t pad;
+    uint64_t data_off;
+} b2g_operator;
+
+/* One OpProduct string of the H_eff OpSum (expr.hpp:503, 811):
+ * H += factor * op(L[a]) (x) op(R[b]) with block2's conj bits.            */
+typedef struct {
+    int32_t a, b;
+    uint32_t conj;
+    uint32_t pad;
+    double factor;
+} b2g_term;
+
+typedef struct {
+    int32_t symmetry;          /* B2G_SZ / B2G_SU2                         */
+    int32_t n_left_infos, n_right_infos;
+    int32_t n_left_ops, n_right_ops, n_terms;
+    uint64_t opdq;             /* H's delta quantum (hop->q_label)         */
+    uint64_t hop_left_vacuum;  /* left vacuum of the MPO                   */
+    b2g_sparse_info ket, bra;  /* psi (ket) and sigma (bra) layouts        */
+    const uint64_t *left_info_dq, *right_info_dq; /* sorted keys           */
+    const b2g_sparse_info *left_infos, *right_infos;
+    const b2g_operator *left_ops, *right_ops;
+    const b2g_term *terms;
+    const double *left_data;   /* host FP64 operator blocks                */
+    const double *right_data;
+    size_t left_len, right_len;
+} b2g_problem;
+
+/* Build connection lists, pair list, diagonal; upload.  flags as in
+ * b2g_upload_plan; with B2G_EFFHAM_HOST_ONLY only the host-side plan is
+ * built (connection lists and pairs can be queried, nothing touches a
+ * device).                                                                 */
+#define B2G_EFFHAM_HOST_ONLY 0x100
+/* with B2G_EFFHAM_DEVICE_DATA, left_data / right_data are device addresses
+ * (FP64, on the context's device); they are copied into the context.      */
+#define B2G_EFFHAM_DEVICE_DATA 0x200
+int b2g_effham_create(b2g_ctx ctx, const b2g_problem *prob, unsigned flags,
+                      b2g_effham *out);
+int b2g_effham_destroy(b2g_effham h);
+/* sizes: |psi| (ket total), |sigma| (bra total), pairs, reference MACs    */
+int b2g_effham_info(b2g_effham h, uint64_t *ket_len, uint64_t *bra_len,
+                    uint64_t *n_pairs, uint64_t *macs);
+/* Host copies of the connection lists the builder produced (for checking
+ * against ConnectionInfo): which = 0 wfn list, 1 diagonal list.  Arrays of
+ * length *n_conn (queried with NULL pointers first).                      */
+int b2g_effham_connections(b2g_effham h, int which, uint64_t *n_conn,
+                           uint32_t *ia, uint32_t *ib, uint32_t *ic,
+                           uint64_t *stride, double *factor);
+int b2g_effham_pairs(b2g_effham h, uint64_t *n_pairs, b2g_pair *pairs);
+/* the replay plan behind an effective Hamiltonian (NULL if host-only)    */
+b2g_plan b2g_effham_plan(b2g_effham h);
+/* H*psi on device arrays (see b2g_hpsi).                                  */
+int b2g_effham_hpsi(b2g_effham h, const double *x, double *y, int n_rhs,
+                    double scale, int accumulate, int precision);
+/* Copy the diagonal (device, bra_len doubles).                            */
+int b2g_effham_diag(b2g_effham h, double *d);
+/* Davidson (IterativeMatrixFunctions::davidson,
+ * src/core/iterative_matrix_functions.hpp:866-1173, Olsen preconditioner):
+ * k roots from k initial device vectors `x` (overwritten with the Ritz
+ * vectors).  energies: host array of k.  Returns B2G_ERR_NOCONV when
+ * max_iter is reached.  precision B2G_PREC_F64, or a mixed mode where the
+ * matvecs run in FP32 / split-BF16 and a final FP64 stage refines
+ * (refine != 0).                                                           */
+int b2g_effham_eigs(b2g_effham h, double *x, int k, double conv_thrd,
+                    int max_iter, int def_min, int def_max, int precision,
+                    int refine, double *energies, int *ndav,
+                    uint64_t *nflop, double *tdav);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* B2G_H */

@@ -1,0 +1,446 @@
+// Reference harness: TEST INFRASTRUCTURE ONLY.
+//
+// Compiled from the reference's own header-only sources (src/block2_core.hpp,
+// src/block2_dmrg.hpp) by oracle/Makefile into oracle/_ref/b2ref.  It runs a
+// plain two-site DMRG with the reference (MPOQC conventional MPO, as in
+// unit_test/test_dmrg_n2_sto3g.cpp:53-70) and, at chosen (sweep, site) points,
+// dumps the effective-Hamiltonian problem handed to
+// EffectiveHamiltonian::eigs (src/dmrg/sweep_algorithm.hpp:1214-1250):
+//   * ket/bra SparseMatrixInfo (src/core/sparse_matrix.hpp:47-877),
+//   * left/right operator infos and operator data (lopt/ropt ops),
+//   * the H_eff expression (OpSum of OpProduct, src/core/expr.hpp:503,811),
+//   * the reference's ConnectionInfo lists for H*psi and for the diagonal
+//     (sparse_matrix.hpp:81-292),
+//   * psi, H*psi computed by the reference in SeqTypes::None mode
+//     (operator_functions.hpp:474-542 -> matrix_functions.hpp:998-1009),
+//   * the diagonal (effective_hamiltonian.hpp:190-202),
+//   * a Davidson solve on that site (effective_hamiltonian.hpp:480-568).
+// Output goes to a directory of .npy files plus meta.txt; tests/golden/
+// make_fixtures.py packs them into the committed .npz fixtures.
+// Nothing in the product links or loads this program.
+
+#include "block2_core.hpp"
+#include "block2_dmrg.hpp"
+
+#include <cstdio>
+#include <fstream>
+#include <sstream>
+#include <sys/stat.h>
+
+using namespace block2;
+
+// ---------------------------------------------------------------- npy writer
+template <typename T> static const char *npy_descr();
+template <> const char *npy_descr<double>() { return "<f8"; }
+template <> const char *npy_descr<uint64_t>() { return "<u8"; }
+template <> const char *npy_descr<uint32_t>() { return "<u4"; }
+template <> const char *npy_descr<int32_t>() { return "<i4"; }
+template <> const char *npy_descr<uint8_t>() { return "|u1"; }
+
+template <typename T>
+static void write_npy(const string &path, const T *data, size_t n) {
+    stringstream hs;
+    hs << "{'descr': '" << npy_descr<T>() << "', 'fortran_order': False, "
+       << "'shape': (" << n << ",), }";
+    string h = hs.str();
+    size_t total = 10 + h.size() + 1;
+    size_t pad = (64 - total % 64) % 64;
+    h += string(pad, ' ');
+    h += '\n';
+    ofstream ofs(path, ios::binary);
+    const char magic[] = "\x93NUMPY";
+    ofs.write(magic, 6);
+    char ver[2] = {1, 0};
+    ofs.write(ver, 2);
+    uint16_t hl = (uint16_t)h.size();
+    ofs.write((const char *)&hl, 2);
+    ofs.write(h.data(), h.size());
+    ofs.write((const char *)data, sizeof(T) * n);
+}
+template <typename T>
+static void write_npy(const string &path, const vector<T> &v) {
+    write_npy(path, v.data(), v.size());
+}
+
+// --------------------------------------------------------------- settings
+struct Settings {
+    string fcidump, sym = "su2", outdir = "ref_out";
+    vector<int> bdims{50};
+    vector<double> noises{1E-5, 0.0};
+    int nsweeps = 6, threads = 4, dump_sweep = -1;
+    vector<int> dump_sites;
+    double dav_conv = 1E-8, dump_conv = 1E-10;
+    bool delayed = false;
+    string noise_type = "dm";
+};
+static Settings G;
+
+template <typename S> static uint64_t qdata(S q) { return (uint64_t)q.data; }
+
+// ------------------------------------------------------------ info packing
+template <typename S> struct InfoPack {
+    vector<uint64_t> dq, quanta;
+    vector<uint32_t> nbra, nket, shift, n, off, fermion;
+    void add(const shared_ptr<SparseMatrixInfo<S>> &info) {
+        dq.push_back(qdata(info->delta_quantum));
+        fermion.push_back(info->is_fermion);
+        n.push_back((uint32_t)info->n);
+        off.push_back((uint32_t)quanta.size());
+        for (int i = 0; i < info->n; i++) {
+            quanta.push_back(qdata(info->quanta[i]));
+            nbra.push_back((uint32_t)info->n_states_bra[i]);
+            nket.push_back((uint32_t)info->n_states_ket[i]);
+            shift.push_back(info->block_shifts[i]);
+        }
+    }
+    void write(const string &d, const string &p) const {
+        write_npy(d + "/" + p + "_dq.npy", dq);
+        write_npy(d + "/" + p + "_fermion.npy", fermion);
+        write_npy(d + "/" + p + "_n.npy", n);
+        write_npy(d + "/" + p + "_off.npy", off);
+        write_npy(d + "/" + p + "_quanta.npy", quanta);
+        write_npy(d + "/" + p + "_nbra.npy", nbra);
+        write_npy(d + "/" + p + "_nket.npy", nket);
+        write_npy(d + "/" + p + "_shift.npy", shift);
+    }
+};
+
+template <typename S, typename CI>
+static void write_cinfo(const string &d, const string &p, const CI &ci) {
+    vector<int32_t> n(ci->n, ci->n + 5);
+    write_npy(d + "/" + p + "_n.npy", n);
+    int nq = ci->n[4], nc = ci->nc;
+    vector<uint64_t> q(nq);
+    for (int i = 0; i < nq; i++)
+        q[i] = qdata(ci->quanta[i]);
+    write_npy(d + "/" + p + "_quanta.npy", q);
+    write_npy(d + "/" + p + "_idx.npy", ci->idx, (size_t)nq);
+    write_npy(d + "/" + p + "_stride.npy", ci->stride, (size_t)nc);
+    write_npy(d + "/" + p + "_factor.npy", ci->factor, (size_t)nc);
+    write_npy(d + "/" + p + "_ia.npy", ci->ia, (size_t)nc);
+    write_npy(d + "/" + p + "_ib.npy", ci->ib, (size_t)nc);
+    write_npy(d + "/" + p + "_ic.npy", ci->ic, (size_t)nc);
+}
+
+// operator set: index per referenced OpExpr, data concatenated
+template <typename S> struct OpSet {
+    map<OpExpr<S> *, int> ids;
+    vector<shared_ptr<SparseMatrix<S, double>>> mats;
+    int get(const shared_ptr<OperatorTensor<S, double>> &opt,
+            const shared_ptr<OpExpr<S>> &x) {
+        auto it = ids.find(x.get());
+        if (it != ids.end())
+            return it->second;
+        assert(opt->ops.count(x));
+        int id = (int)mats.size();
+        ids[x.get()] = id;
+        mats.push_back(opt->ops.at(x));
+        return id;
+    }
+    void write(const string &d, const string &p,
+               const vector<pair<S, shared_ptr<SparseMatrixInfo<S>>>> &infos)
+        const {
+        vector<uint64_t> dq, off;
+        vector<double> factor, data;
+        vector<int32_t> info_idx;
+        for (auto &m : mats) {
+            dq.push_back(qdata(m->info->delta_quantum));
+            factor.push_back(m->factor);
+            int ii = -1;
+            for (size_t k = 0; k < infos.size(); k++)
+                if (infos[k].first == m->info->delta_quantum)
+                    ii = (int)k;
+            assert(ii != -1 && infos[ii].second->n == m->info->n);
+            info_idx.push_back(ii);
+            off.push_back(data.size());
+            data.insert(data.end(), m->data, m->data + m->total_memory);
+        }
+        write_npy(d + "/" + p + "op_dq.npy", dq);
+        write_npy(d + "/" + p + "op_factor.npy", factor);
+        write_npy(d + "/" + p + "op_info.npy", info_idx);
+        write_npy(d + "/" + p + "op_off.npy", off);
+        write_npy(d + "/" + p + "op_data.npy", data);
+    }
+};
+
+template <typename S> struct Dumper : CallbackKernel {
+    DMRG<S, double, double> *dmrg = nullptr;
+    mutable int ndumped = 0;
+    void compute(const string &name, int iprint) const override {
+        if (name != "DMRG::sweep::iter.eff_ham" || dmrg == nullptr)
+            return;
+        auto h_eff = dmrg->current_eff_ham;
+        int isite = dmrg->me->center;
+        if (G.dump_sweep != dmrg->isweep ||
+            find(G.dump_sites.begin(), G.dump_sites.end(), isite) ==
+                G.dump_sites.end())
+            return;
+        stringstream ss;
+        ss << G.outdir << "/site" << isite << "_sw" << dmrg->isweep;
+        string d = ss.str();
+        mkdir(d.c_str(), 0755);
+        dump(h_eff, d, isite);
+        ndumped++;
+    }
+    void dump(const shared_ptr<EffectiveHamiltonian<S, double>> &h, string d,
+              int isite) const {
+        auto op = h->op;
+        InfoPack<S> kp, bp, lp, rp;
+        kp.add(h->ket->info);
+        bp.add(h->bra->info);
+        kp.write(d, "ket");
+        bp.write(d, "bra");
+        for (auto &x : h->left_op_infos)
+            lp.add(x.second);
+        for (auto &x : h->right_op_infos)
+            rp.add(x.second);
+        lp.write(d, "linfo");
+        rp.write(d, "rinfo");
+        // expression
+        shared_ptr<OpExpr<S>> expr = op->mat->data[0];
+        if (expr->get_type() == OpTypes::ExprRef)
+            expr = dynamic_pointer_cast<OpExprRef<S>>(expr)->op;
+        vector<shared_ptr<OpProduct<S, double>>> strings;
+        if (expr->get_type() == OpTypes::Sum)
+            strings = dynamic_pointer_cast<OpSum<S, double>>(expr)->strings;
+        else if (expr->get_type() == OpTypes::Prod)
+            strings.push_back(dynamic_pointer_cast<OpProduct<S, double>>(expr));
+        else
+            assert(false);
+        OpSet<S> lset, rset;
+        vector<uint8_t> tconj;
+        vector<int32_t> ta, tb;
+        vector<double> tf;
+        for (auto &s : strings) {
+            assert(s->get_type() == OpTypes::Prod); // no delayed terms
+            tconj.push_back(s->conj);
+            ta.push_back(lset.get(op->lopt, s->a));
+            tb.push_back(rset.get(op->ropt, s->b));
+            tf.push_back(s->factor);
+        }
+        write_npy(d + "/term_conj.npy", tconj);
+        write_npy(d + "/term_a.npy", ta);
+        write_npy(d + "/term_b.npy", tb);
+        write_npy(d + "/term_factor.npy", tf);
+        lset.write(d, "l", h->left_op_infos);
+        rset.write(d, "r", h->right_op_infos);
+        // reference ConnectionInfo (wfn) and recomputed diag ConnectionInfo
+        assert(h->wfn_infos.size() >= 1 && h->wfn_infos[0] != nullptr);
+        write_cinfo<S>(d, "cwfn", h->wfn_infos[0]);
+        {
+            vector<S> msl = Partition<S, double>::get_uniq_labels({h->hop_mat});
+            auto msubsl = Partition<S, double>::get_uniq_sub_labels(
+                op->mat, h->hop_mat, msl, h->hop_left_vacuum);
+            vector<uint8_t> sconj;
+            vector<uint64_t> sq;
+            for (auto &x : msubsl[0])
+                sconj.push_back(x.first), sq.push_back(qdata(x.second));
+            write_npy(d + "/subl_conj.npy", sconj);
+            write_npy(d + "/subl_q.npy", sq);
+            auto di = make_shared<
+                typename SparseMatrixInfo<S>::ConnectionInfo>();
+            di->initialize_diag(h->ket->info->delta_quantum, h->opdq,
+                                msubsl[0], h->left_op_infos,
+                                h->right_op_infos, h->diag->info,
+                                h->tf->opf->cg);
+            write_cinfo<S>(d, "cdiag", di);
+            di->deallocate();
+        }
+        size_t nw = h->ket->total_memory;
+        vector<double> psi(h->ket->data, h->ket->data + nw);
+        write_npy(d + "/psi.npy", psi);
+        write_npy(d + "/diag.npy", h->diag->data, h->diag->total_memory);
+        // H*psi by the reference in SeqTypes::None mode (scalar GEMM chain)
+        vector<double> hpsi(h->bra->total_memory, 0.0);
+        SeqTypes mode = h->tf->opf->seq->mode;
+        h->tf->opf->seq->mode = SeqTypes::None;
+        h->tf->opf->seq->cumulative_nflop = 0;
+        Timer t;
+        t.get_time();
+        (*h)(GMatrix<double>(psi.data(), (MKL_INT)nw, 1),
+             GMatrix<double>(hpsi.data(), (MKL_INT)hpsi.size(), 1));
+        double thpsi = t.get_time();
+        size_t nflop = h->tf->opf->seq->cumulative_nflop;
+        h->tf->opf->seq->cumulative_nflop = 0;
+        h->tf->opf->seq->mode = mode;
+        write_npy(d + "/hpsi.npy", hpsi);
+        // a second, random input vector (seeded) and its H*x
+        vector<double> x2(nw), hx2(h->bra->total_memory, 0.0);
+        {
+            mt19937 gen(1234 + isite);
+            uniform_real_distribution<double> u(-1.0, 1.0);
+            for (auto &v : x2)
+                v = u(gen);
+            h->tf->opf->seq->mode = SeqTypes::None;
+            (*h)(GMatrix<double>(x2.data(), (MKL_INT)nw, 1),
+                 GMatrix<double>(hx2.data(), (MKL_INT)hx2.size(), 1));
+            h->tf->opf->seq->mode = mode;
+        }
+        write_npy(d + "/x2.npy", x2);
+        write_npy(d + "/hx2.npy", hx2);
+        // Davidson on this site (psi restored afterwards)
+        auto dres = h->eigs(nullptr, false, G.dump_conv, 0.0, 5000, -1, 2, 50,
+                            DavidsonTypes::Normal, 0.0);
+        vector<double> psi_dav(h->ket->data, h->ket->data + nw);
+        memcpy(h->ket->data, psi.data(), nw * sizeof(double));
+        write_npy(d + "/psi_dav.npy", psi_dav);
+        ofstream m(d + "/meta.txt");
+        m << setprecision(17);
+        m << "sym " << G.sym << "\n";
+        m << "site " << isite << "\n";
+        m << "sweep " << dmrg->isweep << "\n";
+        m << "forward " << (int)dmrg->forward << "\n";
+        m << "opdq " << qdata(h->opdq) << "\n";
+        m << "cdq " << qdata(h->ket->info->delta_quantum) << "\n";
+        m << "vdq " << qdata(h->bra->info->delta_quantum) << "\n";
+        m << "hop_left_vacuum " << qdata(h->hop_left_vacuum) << "\n";
+        m << "ket_fermion " << (int)h->ket->info->is_fermion << "\n";
+        m << "nflop_mac " << nflop << "\n";
+        m << "t_hpsi_s " << thpsi << "\n";
+        m << "dav_conv " << G.dump_conv << "\n";
+        m << "dav_e " << get<0>(dres) << "\n";
+        m << "dav_ndav " << get<1>(dres) << "\n";
+        m << "dav_nflop_mac " << get<2>(dres) << "\n";
+        m << "dav_t_s " << get<3>(dres) << "\n";
+        m << "const_e " << dmrg->me->mpo->const_e << "\n";
+        m << "threads " << G.threads << "\n";
+    }
+};
+
+template <typename S> static S make_target(int n, int twos, int pg);
+template <> SZ make_target<SZ>(int n, int twos, int pg) {
+    return SZ(n, twos, pg);
+}
+template <> SU2 make_target<SU2>(int n, int twos, int pg) {
+    return SU2(n, twos, pg);
+}
+
+template <typename S> static int run() {
+    shared_ptr<FCIDUMP<double>> fcidump = make_shared<FCIDUMP<double>>();
+    fcidump->read(G.fcidump);
+    vector<uint8_t> orbsym = fcidump->template orb_sym<uint8_t>();
+    PGTypes pg = PGTypes::D2H;
+    transform(orbsym.begin(), orbsym.end(), orbsym.begin(),
+              [pg](uint8_t x) { return (uint8_t)PointGroup::swap_pg(pg)(x); });
+    S vacuum(0);
+    int norb = fcidump->n_sites();
+    S target = make_target<S>(fcidump->n_elec(), fcidump->twos(),
+                              PointGroup::swap_pg(pg)(fcidump->isym()));
+    auto hamil =
+        make_shared<HamiltonianQC<S, double>>(vacuum, norb, orbsym, fcidump);
+    Timer t;
+    t.get_time();
+    shared_ptr<MPO<S, double>> mpo = make_shared<MPOQC<S, double>>(
+        hamil, QCTypes::Conventional, "HQC", norb / 2 / 2 * 2);
+    mpo->basis = hamil->basis;
+    mpo = make_shared<SimplifiedMPO<S, double>>(
+        mpo, make_shared<RuleQC<S, double>>(), true, true,
+        OpNamesSet({OpNames::R, OpNames::RD}));
+    cerr << "MPO done T = " << t.get_time() << endl;
+    Random::rand_seed(0);
+    auto mps_info =
+        make_shared<MPSInfo<S>>(mpo->n_sites, vacuum, target, mpo->basis);
+    mps_info->set_bond_dimension(G.bdims[0]);
+    auto mps = make_shared<MPS<S, double>>(mpo->n_sites, 0, 2);
+    mps->initialize(mps_info);
+    mps->random_canonicalize();
+    mps->save_mutable();
+    mps->deallocate();
+    mps_info->save_mutable();
+    mps_info->deallocate_mutable();
+    auto me =
+        make_shared<MovingEnvironment<S, double, double>>(mpo, mps, mps, "DMRG");
+    me->init_environments(false);
+    if (G.delayed)
+        me->delayed_contraction = OpNamesSet::normal_ops();
+    me->cached_contraction = true;
+    vector<ubond_t> bdims(G.bdims.begin(), G.bdims.end());
+    vector<double> noises(G.noises.begin(), G.noises.end());
+    auto dmrg = make_shared<DMRG<S, double, double>>(me, bdims, noises);
+    dmrg->iprint = 2;
+    dmrg->noise_type = G.noise_type == "pert"
+                           ? NoiseTypes::ReducedPerturbativeCollected
+                           : NoiseTypes::DensityMatrix;
+    dmrg->decomp_type = DecompositionTypes::DensityMatrix;
+    dmrg->davidson_soft_max_iter = 4000;
+    auto dumper = make_shared<Dumper<S>>();
+    dumper->dmrg = dmrg.get();
+    callback_() = dumper;
+    double energy = dmrg->solve(G.nsweeps, mps->center == 0, G.dav_conv * 0.1);
+    callback_() = make_shared<CallbackKernel>();
+    ofstream ofs(G.outdir + "/energies.txt");
+    ofs << setprecision(17);
+    ofs << "final " << energy << "\n";
+    for (size_t i = 0; i < dmrg->sweep_energies.size(); i++)
+        if (dmrg->sweep_energies[i].size() != 0)
+            ofs << "sweep " << i << " " << dmrg->sweep_energies[i][0] << " "
+                << (i < dmrg->sweep_time.size() ? dmrg->sweep_time[i] : 0.0)
+                << "\n";
+    ofs << "dumped " << dumper->ndumped << "\n";
+    cerr << "E = " << setprecision(15) << energy << endl;
+    mps_info->deallocate();
+    me->remove_partition_files();
+    mpo->deallocate();
+    hamil->deallocate();
+    fcidump->deallocate();
+    return 0;
+}
+
+static vector<double> parse_dlist(const string &s) {
+    vector<double> r;
+    stringstream ss(s);
+    string x;
+    while (getline(ss, x, ','))
+        r.push_back(stod(x));
+    return r;
+}
+
+int main(int argc, char **argv) {
+    for (int i = 1; i + 1 < argc; i += 2) {
+        string k = argv[i], v = argv[i + 1];
+        if (k == "--fcidump")
+            G.fcidump = v;
+        else if (k == "--sym")
+            G.sym = v;
+        else if (k == "--out")
+            G.outdir = v;
+        else if (k == "--bdims") {
+            G.bdims.clear();
+            for (double x : parse_dlist(v))
+                G.bdims.push_back((int)x);
+        } else if (k == "--noises")
+            G.noises = parse_dlist(v);
+        else if (k == "--nsweeps")
+            G.nsweeps = stoi(v);
+        else if (k == "--threads")
+            G.threads = stoi(v);
+        else if (k == "--dump-sweep")
+            G.dump_sweep = stoi(v);
+        else if (k == "--dump-sites") {
+            for (double x : parse_dlist(v))
+                G.dump_sites.push_back((int)x);
+        } else if (k == "--dav-conv")
+            G.dav_conv = stod(v);
+        else if (k == "--dump-conv")
+            G.dump_conv = stod(v);
+        else if (k == "--noise-type")
+            G.noise_type = v;
+        else {
+            cerr << "unknown option " << k << endl;
+            return 2;
+        }
+    }
+    mkdir(G.outdir.c_str(), 0755);
+    frame_<double>() = make_shared<DataFrame<double>>(
+        (size_t)1 << 26, (size_t)1 << 33, G.outdir + "/scratch");
+    frame_<double>()->use_main_stack = false;
+    frame_<double>()->minimal_disk_usage = true;
+    threading_() = make_shared<Threading>(
+        ThreadingTypes::OperatorBatchedGEMM | ThreadingTypes::Global,
+        G.threads, G.threads, 1);
+    threading_()->seq_type = SeqTypes::Tasked;
+    int r = G.sym == "sz" ? run<SZ>() : run<SU2>();
+    frame_<double>()->activate(0);
+    frame_<double>() = nullptr;
+    return r;
+}

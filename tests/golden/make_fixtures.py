@@ -1,0 +1,106 @@
+"""Regenerate the golden fixtures under tests/golden/ (this container only).
+
+Runs the reference harness oracle/_ref/b2ref (built by `make -C oracle ref`
+from the reference's own sources) on small systems and packs each dumped
+effective-Hamiltonian problem into a compressed .npz:
+
+  <name>.npz   ket/bra infos, left/right operator infos and data, the H_eff
+               strings, the reference ConnectionInfo lists (wfn, diag), psi,
+               the reference H*psi of psi and of a seeded random vector, the
+               diagonal, a reference Davidson energy (+ vector) and metadata.
+  energies.json  reference DMRG energies of the runs (known answers).
+
+Inputs: FCIDUMPs shipped in the reference's data/ directory, plus the 1D
+Hubbard chain L=16, U=4, t=1 written here (the shipped HUBBARD-L16 has U=2).
+"""
+import json
+import os
+import re
+import shutil
+import subprocess
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+REF = "/root/reference"
+B2REF = os.path.join(REPO, "oracle", "_ref", "b2ref")
+
+
+def write_hubbard(path, n, u, t):
+    with open(path, "w") as f:
+        f.write(" &FCI NORB=%3d,NELEC=%3d,MS2=  0,\n  ORBSYM=%s\n  ISYM=1,\n /\n"
+                % (n, n, "1," * n))
+        for i in range(1, n + 1):
+            f.write(" %22.16f %3d %3d %3d %3d\n" % (u, i, i, i, i))
+        for i in range(1, n):
+            f.write(" %22.16f %3d %3d %3d %3d\n" % (-t, i + 1, i, 0, 0))
+        f.write(" %22.16f %3d %3d %3d %3d\n" % (0.0, 0, 0, 0, 0))
+
+
+def run(name, fcidump, sym, bdims, noises, nsweeps, dump_sweep, dump_site,
+        work, dav_conv="1e-9"):
+    out = os.path.join(work, name)
+    if os.path.exists(out):
+        shutil.rmtree(out)
+    cmd = [B2REF, "--fcidump", fcidump, "--sym", sym, "--out", out,
+           "--bdims", ",".join(map(str, bdims)),
+           "--noises", ",".join(map(str, noises)),
+           "--nsweeps", str(nsweeps), "--threads", "8",
+           "--dump-sweep", str(dump_sweep), "--dump-sites", str(dump_site),
+           "--dav-conv", dav_conv]
+    print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True, stdout=subprocess.DEVNULL)
+    energies = {}
+    with open(os.path.join(out, "energies.txt")) as f:
+        for line in f:
+            p = line.split()
+            if p[0] == "final":
+                energies["final"] = float(p[1])
+    site_dir = os.path.join(out, "site%d_sw%d" % (dump_site, dump_sweep))
+    if os.path.isdir(site_dir):
+        arrs = {}
+        for fn in sorted(os.listdir(site_dir)):
+            if fn.endswith(".npy"):
+                arrs[fn[:-4]] = np.load(os.path.join(site_dir, fn))
+        meta = {}
+        with open(os.path.join(site_dir, "meta.txt")) as f:
+            for line in f:
+                k, v = line.split()
+                meta[k] = v
+        arrs["meta_json"] = np.frombuffer(json.dumps(meta).encode(), np.uint8)
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), **arrs)
+        print("  wrote", name + ".npz", flush=True)
+    return energies
+
+
+def main():
+    work = os.path.join(REPO, "_work", "fixtures")
+    os.makedirs(work, exist_ok=True)
+    hub16 = os.path.join(work, "HUBBARD-L16-U4.FCIDUMP")
+    write_hubbard(hub16, 16, 4.0, 1.0)
+    data = os.path.join(REF, "data")
+    res = {}
+    res["hub8_sz"] = run("hub8_sz", os.path.join(data, "HUBBARD-L8.FCIDUMP"),
+                         "sz", [200], [1e-6, 0], 8, 1, 3, work)
+    res["hub8_su2"] = run("hub8_su2", os.path.join(data, "HUBBARD-L8.FCIDUMP"),
+                          "su2", [200], [1e-6, 0], 8, 1, 3, work)
+    res["n2_su2"] = run("n2_su2", os.path.join(data, "N2.STO3G.FCIDUMP"),
+                        "su2", [100], [1e-6, 1e-7, 0], 8, 2, 4, work)
+    res["n2_sz"] = run("n2_sz", os.path.join(data, "N2.STO3G.FCIDUMP"),
+                       "sz", [60], [1e-6, 1e-7, 0], 8, 2, 4, work)
+    res["hub16_sz_m200"] = run("hub16_sz_m200", hub16, "sz", [200],
+                               [1e-5, 1e-5, 1e-5, 1e-5, 0], 10, 4, 7, work,
+                               dav_conv="1e-8")
+    # energy-only runs (known answers)
+    res["n2_su2_m500"] = run("n2_su2_m500",
+                             os.path.join(data, "N2.STO3G.FCIDUMP"), "su2",
+                             [500], [1e-6, 1e-7, 0], 10, -1, -1, work)
+    with open(os.path.join(HERE, "energies.json"), "w") as f:
+        json.dump(res, f, indent=1, sort_keys=True)
+    print(json.dumps(res, indent=1))
+
+
+if __name__ == "__main__":
+    sys.exit(main())
