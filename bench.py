@@ -134,16 +134,22 @@ def main():
 
     ms64, ph = run("f64", args.steps, True)
     tflops = 2.0 * h.macs * R / (ms64 * 1e-3) / 1e12
+    # per-launch figures: a step launches each phase once per scratch chunk;
+    # algorithmic FLOPs per launch = the phase's reference-order FLOPs of one
+    # step divided by the launches per step
     ms1, ms2, nl = ph
+    per_step = max(nl // args.steps, 1)
     d1, d2 = ms1 / max(nl, 1), ms2 / max(nl, 1)
+    f1 = 2.0 * macs1 * R / per_step
+    f2 = 2.0 * macs2 * R / per_step
     if d2 >= d1:
-        dom, dflop, dur = "phase2 V+=alpha*op(A)*W", 2.0 * macs2 * R, d2
+        dom, dflop, dur = "phase2 V+=alpha*op(A)*W", f2, d2
     else:
-        dom, dflop, dur = "phase1 W=C*op(B)", 2.0 * macs1 * R, d1
+        dom, dflop, dur = "phase1 W=C*op(B)", f1, d1
     achieved = dflop / (dur * 1e-3) / 1e12
-    log("FP64: %.3f ms/step, %.2f TFLOP/s; phase1 %.3f ms (%.2f TF) phase2 "
-        "%.3f ms (%.2f TF)" % (ms64, tflops, d1, 2 * macs1 * R / d1 / 1e9, d2,
-                               2 * macs2 * R / d2 / 1e9))
+    log("FP64: %.3f ms/step, %.2f TFLOP/s; %d launches/phase/step; phase1 "
+        "%.3f ms/launch (%.2f TF) phase2 %.3f ms/launch (%.2f TF)" % (
+            ms64, tflops, per_step, d1, f1 / d1 / 1e9, d2, f2 / d2 / 1e9))
     ms32, _ = run("f32", max(3, args.steps // 2), False)
     tflops32 = 2.0 * h.macs * R / (ms32 * 1e-3) / 1e12
     log("FP32: %.3f ms/step, %.2f TFLOP/s" % (ms32, tflops32))
@@ -200,6 +206,8 @@ def main():
             "n_pairs": h.n_pairs, "macs_per_rhs": h.macs,
             "parallelism": "single GPU"},
         "roofline": {"bound": "mfma", "kernel": dom,
+                     "launches_per_step": per_step,
+                     "avg_launch_ms": round(dur, 4),
                      "achieved": round(achieved, 4), "peak": FP64_MFMA_PEAK,
                      "unit": "TFLOP/s",
                      "frac": round(achieved / FP64_MFMA_PEAK, 4),

@@ -9,18 +9,26 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
 template <typename TC> struct Mfma;
-// 16x16x4 single-block MFMA.  Lane l holds A[l%16][l/16] and B[l/16][l%16];
-// the 16x16 accumulator holds D[4*(l/16)+r][l%16], r = 0..3.
+// 16x16x4 single-block MFMA.  Lane l holds A[l%16][l/16] and B[l/16][l%16]
+// for both types; accumulator element r of lane l is D[row][l%16] with
+// row = 4*r + l/16 for FP64 and row = 4*(l/16) + r for FP32 (measured with
+// an in-bounds probe of the builtins on gfx950).
 template <> struct Mfma<double> {
     typedef v4d acc_t;
     static __device__ __forceinline__ acc_t op(double a, double b, acc_t c) {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lk, int r) {
+        return 4 * r + lk;
     }
 };
 template <> struct Mfma<float> {
     typedef v4f acc_t;
     static __device__ __forceinline__ acc_t op(float a, float b, acc_t c) {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lk, int r) {
+        return 4 * lk + r;
     }
 };
 
@@ -122,7 +130,8 @@ __global__ __launch_bounds__(256) void seg_gemm_kernel(
             for (int j = 0; j < FN; j++)
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
-                    const uint32_t row = task.row0 + wm * WM + i * 16 + lk * 4 + r;
+                    const uint32_t row =
+                        task.row0 + wm * WM + i * 16 + Mfma<TC>::row(lk, r);
                     const uint32_t col = task.col0 + wn * WN + j * 16 + lr;
                     if (row < task.M && col < task.N) {
                         const uint64_t idx =
