@@ -161,7 +161,12 @@ class Problem:
         p.right_info_dq = L.ptr(self.rinfo.dq, C.c_uint64)
         p.left_infos, p.right_infos = lin, rin
         p.left_ops, p.right_ops, p.terms = lo, ro, tm
-        if device_data is None:
+        if device_data is None and self.lops.data is None:
+            # structure only (host-only plans): no data, declared lengths
+            dl = getattr(self, "data_len", {0: 0, 1: 0})
+            p.left_data = p.right_data = None
+            p.left_len, p.right_len = dl[0], dl[1]
+        elif device_data is None:
             p.left_data = L.ptr(self.lops.data, C.c_double)
             p.right_data = L.ptr(self.rops.data, C.c_double)
             p.left_len, p.right_len = len(self.lops.data), len(self.rops.data)

@@ -20,6 +20,9 @@ FLAGS = [
     "-Wno-unused-result", "-Wno-unused-value",
     # no packed-FP32 instructions anywhere in the device code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # MFMA accumulators in VGPRs: without it the register allocator copies
+    # every accumulator between VGPRs and AGPRs around each k-loop iteration
+    "-mllvm", "-amdgpu-mfma-vgpr-form=1",
     "-I" + os.path.join(REPO, "include"),
 ]
 

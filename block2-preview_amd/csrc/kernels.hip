@@ -34,19 +34,89 @@ template <> struct Mfma<float> {
 
 // One workgroup = 4 waves in a 2x2 arrangement; the workgroup tile is
 // TM x TN, each wave owns (TM/2) x (TN/2) as (TM/32) x (TN/32) MFMA tiles.
-// K is staged through LDS in chunks of TK.  Tasks are strided over the grid
-// (persistent launch); blockIdx.y selects the right-hand side.
-template <typename TA, typename TB, typename TD, typename TC, int TM, int TN>
+// K is staged through two LDS buffers in chunks of TK: while the MFMAs run on
+// one buffer, the next chunk's global loads are in flight in registers and
+// are then written to the other buffer, so one barrier per chunk suffices.
+// Loads are branch-free (out-of-range elements read the block's first
+// element; a bit mask zeroes them when the chunk is written to LDS, where
+// the per-segment alpha is applied too), so the loaded registers are not
+// consumed before the MFMAs of the current chunk have been issued.  Tasks are strided
+// over the grid (persistent launch); blockIdx.y selects the right-hand side.
+template <typename TA, typename TB, typename TC, int TM, int TN, int TK>
+struct TileLoader {
+    static constexpr int NA = TM * TK / 256, NB = TN * TK / 256;
+    static_assert(NA + NB <= 32, "mask bits");
+    TA ra[NA];
+    TB rb[NB];
+    uint32_t okmask; // bit q: element q in range (applied at store time)
+    __device__ __forceinline__ void load(const TA *__restrict__ A,
+                                         const TB *__restrict__ B,
+                                         const Seg &sg, uint32_t k0,
+                                         const Task &task, int tid) {
+        const bool ta = sg.flags & 1, tb = (sg.flags >> 1) & 1;
+        uint32_t mask = 0;
+#pragma unroll
+        for (int q = 0; q < NA; q++) {
+            const int e = tid + q * 256;
+            // (m, k) of this element for either storage order of A
+            const int m = ta ? e % TM : e / TK, k = ta ? e / TM : e % TK;
+            const uint32_t r = task.row0 + m, kk = k0 + k;
+            const bool ok = r < task.M && kk < sg.K;
+            const uint64_t idx =
+                ok ? sg.a_off + (ta ? (uint64_t)kk * sg.lda + r
+                                    : (uint64_t)r * sg.lda + kk)
+                   : sg.a_off;
+            ra[q] = A[idx];
+            mask |= (uint32_t)ok << q;
+        }
+#pragma unroll
+        for (int q = 0; q < NB; q++) {
+            const int e = tid + q * 256;
+            const int n = tb ? e / TK : e % TN, k = tb ? e % TK : e / TN;
+            const uint32_t c = task.col0 + n, kk = k0 + k;
+            const bool ok = c < task.N && kk < sg.K;
+            const uint64_t idx =
+                ok ? sg.b_off + (tb ? (uint64_t)c * sg.ldb + kk
+                                    : (uint64_t)kk * sg.ldb + c)
+                   : sg.b_off;
+            rb[q] = B[idx];
+            mask |= (uint32_t)ok << (NA + q);
+        }
+        okmask = mask;
+    }
+    // registers -> LDS buffer ([k][m] and [k][n] layouts), alpha applied
+    __device__ __forceinline__ void store(TC *As, TC *Bs, const Seg &sg,
+                                          int tid) {
+        const bool ta = sg.flags & 1, tb = (sg.flags >> 1) & 1;
+        const TC alpha = (TC)sg.alpha;
+#pragma unroll
+        for (int q = 0; q < NA; q++) {
+            const int e = tid + q * 256;
+            const int m = ta ? e % TM : e / TK, k = ta ? e / TM : e % TK;
+            As[k * (TM + 1) + m] =
+                ((okmask >> q) & 1) ? (TC)ra[q] * alpha : (TC)0;
+        }
+#pragma unroll
+        for (int q = 0; q < NB; q++) {
+            const int e = tid + q * 256;
+            const int n = tb ? e / TK : e % TN, k = tb ? e % TK : e / TN;
+            Bs[k * (TN + 1) + n] =
+                ((okmask >> (NA + q)) & 1) ? (TC)rb[q] : (TC)0;
+        }
+    }
+};
+
+template <typename TA, typename TB, typename TD, typename TC, int TM, int TN,
+          int TK>
 __global__ __launch_bounds__(256) void seg_gemm_kernel(
     const TA *__restrict__ A, const TB *__restrict__ B, TD *__restrict__ D,
     const Task *__restrict__ tasks, int ntasks, const Seg *__restrict__ segs,
     uint64_t a_rs, uint64_t b_rs, uint64_t d_rs, double scale, int beta0) {
-    constexpr int TK = 16;
     constexpr int WM = TM / 2, WN = TN / 2;
     constexpr int FM = WM / 16, FN = WN / 16;
+    constexpr int ASZ = TK * (TM + 1), BSZ = TK * (TN + 1);
     typedef typename Mfma<TC>::acc_t acc_t;
-    __shared__ TC As[TK][TM + 1];
-    __shared__ TC Bs[TK][TN + 1];
+    __shared__ TC smem[2 * (ASZ + BSZ)];
     const int rhs = blockIdx.y;
     A += rhs * a_rs;
     B += rhs * b_rs;
@@ -54,6 +124,7 @@ __global__ __launch_bounds__(256) void seg_gemm_kernel(
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
     const int lr = lane & 15, lk = lane >> 4;
+    TileLoader<TA, TB, TC, TM, TN, TK> ld;
     for (int t = blockIdx.x; t < ntasks; t += gridDim.x) {
         const Task task = tasks[t];
         acc_t acc[FM][FN];
@@ -62,67 +133,70 @@ __global__ __launch_bounds__(256) void seg_gemm_kernel(
 #pragma unroll
             for (int j = 0; j < FN; j++)
                 acc[i][j] = acc_t{0, 0, 0, 0};
-        for (uint32_t s = task.seg_begin; s < task.seg_end; s++) {
-            const Seg sg = segs[s];
-            const TC alpha = (TC)sg.alpha;
-            const bool ta = sg.flags & 1, tb = (sg.flags >> 1) & 1;
-            for (uint32_t k0 = 0; k0 < sg.K; k0 += TK) {
-#pragma unroll
-                for (int q = 0; q < TM * TK / 256; q++) {
-                    const int e = tid + q * 256;
-                    int m, k;
-                    uint64_t idx;
-                    if (!ta) {
-                        m = e / TK, k = e % TK;
-                        idx = sg.a_off + (uint64_t)(task.row0 + m) * sg.lda +
-                              (k0 + k);
-                    } else {
-                        k = e / TM, m = e % TM;
-                        idx = sg.a_off + (uint64_t)(k0 + k) * sg.lda +
-                              (task.row0 + m);
-                    }
-                    TC v = (TC)0;
-                    if (task.row0 + m < task.M && k0 + k < sg.K)
-                        v = (TC)A[idx] * alpha;
-                    As[k][m] = v;
+        // segments have K > 0 (the plan drops empty ones); descriptors are
+        // wave-uniform and read one segment ahead
+        uint32_t s = task.seg_begin, k0 = 0;
+        if (s < task.seg_end) {
+            Seg cur = segs[s];
+            Seg nxt = cur;
+            if (s + 1 < task.seg_end)
+                nxt = segs[s + 1];
+            int buf = 0;
+            ld.load(A, B, cur, k0, task, tid);
+            ld.store(smem, smem + ASZ, cur, tid);
+            __syncthreads();
+            while (true) {
+                // advance the loader to the next chunk and start its loads
+                k0 += TK;
+                if (k0 >= cur.K) {
+                    k0 = 0;
+                    s++;
+                    cur = nxt;
+                    if (s + 1 < task.seg_end)
+                        nxt = segs[s + 1];
                 }
+                const bool more = s < task.seg_end;
+                if (more)
+                    ld.load(A, B, cur, k0, task, tid);
+                const TC *As = smem + buf * (ASZ + BSZ);
+                const TC *Bs = As + ASZ;
+                // MFMAs on this buffer, fragment reads one k-step ahead
+                TC a[2][FM], b[2][FN];
 #pragma unroll
-                for (int q = 0; q < TN * TK / 256; q++) {
-                    const int e = tid + q * 256;
-                    int n, k;
-                    uint64_t idx;
-                    if (!tb) {
-                        k = e / TN, n = e % TN;
-                        idx = sg.b_off + (uint64_t)(k0 + k) * sg.ldb +
-                              (task.col0 + n);
-                    } else {
-                        n = e / TK, k = e % TK;
-                        idx = sg.b_off + (uint64_t)(task.col0 + n) * sg.ldb +
-                              (k0 + k);
+                for (int i = 0; i < FM; i++)
+                    a[0][i] = As[lk * (TM + 1) + wm * WM + i * 16 + lr];
+#pragma unroll
+                for (int j = 0; j < FN; j++)
+                    b[0][j] = Bs[lk * (TN + 1) + wn * WN + j * 16 + lr];
+#pragma unroll
+                for (int ks = 0; ks < TK / 4; ks++) {
+                    const int cb = ks & 1, nb = cb ^ 1;
+                    if (ks + 1 < TK / 4) {
+                        const int kr = (ks + 1) * 4 + lk;
+#pragma unroll
+                        for (int i = 0; i < FM; i++)
+                            a[nb][i] = As[kr * (TM + 1) + wm * WM + i * 16 + lr];
+#pragma unroll
+                        for (int j = 0; j < FN; j++)
+                            b[nb][j] = Bs[kr * (TN + 1) + wn * WN + j * 16 + lr];
                     }
-                    TC v = (TC)0;
-                    if (task.col0 + n < task.N && k0 + k < sg.K)
-                        v = (TC)B[idx];
-                    Bs[k][n] = v;
-                }
-                __syncthreads();
-#pragma unroll
-                for (int kk = 0; kk < TK; kk += 4) {
-                    TC a[FM], b[FN];
-#pragma unroll
-                    for (int i = 0; i < FM; i++)
-                        a[i] = As[kk + lk][wm * WM + i * 16 + lr];
-#pragma unroll
-                    for (int j = 0; j < FN; j++)
-                        b[j] = Bs[kk + lk][wn * WN + j * 16 + lr];
 #pragma unroll
                     for (int i = 0; i < FM; i++)
 #pragma unroll
                         for (int j = 0; j < FN; j++)
-                            acc[i][j] = Mfma<TC>::op(a[i], b[j], acc[i][j]);
+                            acc[i][j] = Mfma<TC>::op(a[cb][i], b[cb][j],
+                                                     acc[i][j]);
                 }
+                if (!more)
+                    break;
+                // the other buffer was released by the barrier that ended
+                // the previous chunk
+                buf ^= 1;
+                TC *Ns = smem + buf * (ASZ + BSZ);
+                ld.store(Ns, Ns + ASZ, cur, tid);
                 __syncthreads();
             }
+            __syncthreads(); // all waves done with LDS before the next task
         }
 #pragma unroll
         for (int i = 0; i < FM; i++)
@@ -146,28 +220,46 @@ __global__ __launch_bounds__(256) void seg_gemm_kernel(
     }
 }
 
+// tile configurations (index = Task bucket id): workgroup tiles
+// 64x64 (K chunk 16), 128x128 (K chunk 16), 32x32 (K chunk 32)
+int tile_m(int cfg) {
+    static const int t[] = {64, 128, 32};
+    return t[cfg];
+}
+int tile_n(int cfg) { return tile_m(cfg); }
+
 template <typename TA, typename TB, typename TD, typename TC>
-hipError_t launch_seg_gemm(const TA *A, const TB *B, TD *D, const Task *tasks,
-                           int ntasks, const Seg *segs, uint64_t a_rs,
-                           uint64_t b_rs, uint64_t d_rs, double scale,
-                           int beta0, int n_rhs, int grid, hipStream_t st) {
+hipError_t launch_seg_gemm(int cfg, const TA *A, const TB *B, TD *D,
+                           const Task *tasks, int ntasks, const Seg *segs,
+                           uint64_t a_rs, uint64_t b_rs, uint64_t d_rs,
+                           double scale, int beta0, int n_rhs, int grid,
+                           hipStream_t st) {
     if (ntasks <= 0)
         return hipSuccess;
     dim3 g((unsigned)(grid < ntasks ? grid : ntasks), (unsigned)n_rhs);
-    hipLaunchKernelGGL((seg_gemm_kernel<TA, TB, TD, TC, 64, 64>), g, dim3(256),
-                       0, st, A, B, D, tasks, ntasks, segs, a_rs, b_rs, d_rs,
-                       scale, beta0);
+    if (cfg == 1)
+        hipLaunchKernelGGL((seg_gemm_kernel<TA, TB, TD, TC, 128, 128, 16>), g,
+                           dim3(256), 0, st, A, B, D, tasks, ntasks, segs,
+                           a_rs, b_rs, d_rs, scale, beta0);
+    else if (cfg == 2)
+        hipLaunchKernelGGL((seg_gemm_kernel<TA, TB, TD, TC, 32, 32, 32>), g,
+                           dim3(256), 0, st, A, B, D, tasks, ntasks, segs,
+                           a_rs, b_rs, d_rs, scale, beta0);
+    else
+        hipLaunchKernelGGL((seg_gemm_kernel<TA, TB, TD, TC, 64, 64, 16>), g,
+                           dim3(256), 0, st, A, B, D, tasks, ntasks, segs,
+                           a_rs, b_rs, d_rs, scale, beta0);
     return hipGetLastError();
 }
 
 template hipError_t launch_seg_gemm<double, double, double, double>(
-    const double *, const double *, double *, const Task *, int, const Seg *,
+    int, const double *, const double *, double *, const Task *, int, const Seg *,
     uint64_t, uint64_t, uint64_t, double, int, int, int, hipStream_t);
 template hipError_t launch_seg_gemm<double, float, float, float>(
-    const double *, const float *, float *, const Task *, int, const Seg *,
+    int, const double *, const float *, float *, const Task *, int, const Seg *,
     uint64_t, uint64_t, uint64_t, double, int, int, int, hipStream_t);
 template hipError_t launch_seg_gemm<float, float, double, float>(
-    const float *, const float *, double *, const Task *, int, const Seg *,
+    int, const float *, const float *, double *, const Task *, int, const Seg *,
     uint64_t, uint64_t, uint64_t, double, int, int, int, hipStream_t);
 
 // ---------------------------------------------------------------- diagonal

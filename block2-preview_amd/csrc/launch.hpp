@@ -7,11 +7,17 @@
 
 namespace b2g {
 
+// tile configuration cfg: 0 = 64x64, 1 = 128x128, 2 = 32x32 workgroup tiles
+int tile_m(int cfg);
+int tile_n(int cfg);
+constexpr int kTileConfigs = 3;
+
 template <typename TA, typename TB, typename TD, typename TC>
-hipError_t launch_seg_gemm(const TA *A, const TB *B, TD *D, const Task *tasks,
-                           int ntasks, const Seg *segs, uint64_t a_rs,
-                           uint64_t b_rs, uint64_t d_rs, double scale,
-                           int beta0, int n_rhs, int grid, hipStream_t st);
+hipError_t launch_seg_gemm(int cfg, const TA *A, const TB *B, TD *D,
+                           const Task *tasks, int ntasks, const Seg *segs,
+                           uint64_t a_rs, uint64_t b_rs, uint64_t d_rs,
+                           double scale, int beta0, int n_rhs, int grid,
+                           hipStream_t st);
 
 hipError_t launch_diag(const double *L, const double *R, double *Dv,
                        const DiagBlock *blocks, int nblocks,

@@ -171,12 +171,12 @@ extern "C" size_t b2g_buf_size(b2g_buf buf) { return buf ? buf->n : 0; }
 namespace b2g {
 
 struct Chunk {
-    // host images
-    std::vector<Task> t1, t2;
+    // host images; tasks are bucketed by tile configuration
+    std::vector<Task> t1[kTileConfigs], t2[kTileConfigs];
     std::vector<Seg> s1, s2;
     uint64_t w_len = 0; // W elements per right-hand side
     // device copies (inside plan->dev)
-    Task *d_t1 = nullptr, *d_t2 = nullptr;
+    Task *d_t1[kTileConfigs] = {}, *d_t2[kTileConfigs] = {};
     Seg *d_s1 = nullptr, *d_s2 = nullptr;
 };
 
@@ -209,12 +209,31 @@ static uint64_t tile_cost(const Task &t, const std::vector<Seg> &segs) {
     return k + 1;
 }
 
+// tile configuration of an M x N output block: B2G_TILE=32|64|128 forces
+// one shape, otherwise large blocks take 128x128, small ones 32x32
+static int pick_tile(uint32_t M, uint32_t N) {
+    const char *e = getenv("B2G_TILE");
+    int forced = -1;
+    if (e != nullptr) {
+        const std::string v(e);
+        forced = v == "64" ? 0 : v == "128" ? 1 : v == "32" ? 2 : -1;
+    }
+    if (forced >= 0)
+        return forced;
+    if (M >= 128 && N >= 128)
+        return 1;
+    if (M <= 32 && N <= 32)
+        return 2;
+    return 0;
+}
+
 // tiles of an M x N output block
-static void add_tiles(std::vector<Task> &tasks, uint64_t d_off, uint32_t M,
+static void add_tiles(std::vector<Task> *tasks, uint64_t d_off, uint32_t M,
                       uint32_t N, uint32_t sb, uint32_t se, uint32_t beta) {
-    const uint32_t T = 64;
-    for (uint32_t r = 0; r < M; r += T)
-        for (uint32_t c = 0; c < N; c += T) {
+    const int cfg = pick_tile(M, N);
+    const uint32_t TMc = tile_m(cfg), TNc = tile_n(cfg);
+    for (uint32_t r = 0; r < M; r += TMc)
+        for (uint32_t c = 0; c < N; c += TNc) {
             Task t;
             t.d_off = d_off;
             t.ldd = N;
@@ -225,7 +244,7 @@ static void add_tiles(std::vector<Task> &tasks, uint64_t d_off, uint32_t M,
             t.seg_begin = sb;
             t.seg_end = se;
             t.beta = beta;
-            tasks.push_back(t);
+            tasks[cfg].push_back(t);
         }
 }
 
@@ -287,6 +306,10 @@ static int compile_plan(b2g_plan p, const b2g_pair *pairs, size_t n_pairs,
         size_t i1 = i0;
         for (; i1 < n_pairs; i1++) {
             const b2g_pair &q = pairs[i1];
+            if (q.c_m == 0 || q.c_n == 0 || q.v_m == 0 || q.v_n == 0) {
+                pw.push_back(0); // contributes nothing; never referenced
+                continue;
+            }
             WKey key(q.b_off, q.c_off, q.conj & 2);
             auto it = wmap.find(key);
             if (it == wmap.end()) {
@@ -321,6 +344,8 @@ static int compile_plan(b2g_plan p, const b2g_pair *pairs, size_t n_pairs,
             if (lst != nullptr)
                 for (size_t i : *lst) {
                     const b2g_pair &q = pairs[i];
+                    if (q.c_m == 0 || q.c_n == 0)
+                        continue; // K = 0: nothing to add
                     Seg s;
                     s.a_off = q.a_off;
                     const bool ta = q.conj & 1;
@@ -344,8 +369,10 @@ static int compile_plan(b2g_plan p, const b2g_pair *pairs, size_t n_pairs,
             for (auto &kv : byv)
                 emit(yb[yidx[kv.first]], &kv.second, 1);
         }
-        sort_tasks(ch.t1, ch.s1);
-        sort_tasks(ch.t2, ch.s2);
+        for (int c = 0; c < kTileConfigs; c++) {
+            sort_tasks(ch.t1[c], ch.s1);
+            sort_tasks(ch.t2[c], ch.s2);
+        }
         p->w_max = std::max(p->w_max, ch.w_len);
         p->chunks.push_back(std::move(ch));
         first = false;
@@ -355,9 +382,11 @@ static int compile_plan(b2g_plan p, const b2g_pair *pairs, size_t n_pairs,
     }
     // upload all task/segment arrays in one allocation
     size_t bytes = 0;
-    for (auto &ch : p->chunks)
-        bytes += (ch.t1.size() + ch.t2.size()) * sizeof(Task) +
-                 (ch.s1.size() + ch.s2.size()) * sizeof(Seg);
+    for (auto &ch : p->chunks) {
+        for (int c = 0; c < kTileConfigs; c++)
+            bytes += (ch.t1[c].size() + ch.t2[c].size()) * sizeof(Task);
+        bytes += (ch.s1.size() + ch.s2.size()) * sizeof(Seg);
+    }
     std::vector<char> host(bytes + 16);
     B2G_HIP(hipMalloc(&p->dev, bytes + 16));
     size_t off = 0;
@@ -368,13 +397,19 @@ static int compile_plan(b2g_plan p, const b2g_pair *pairs, size_t n_pairs,
         off += nb;
         return o;
     };
-    std::vector<std::tuple<size_t, size_t, size_t, size_t>> offs;
+    struct Offs {
+        size_t t1[kTileConfigs], t2[kTileConfigs], s1, s2;
+    };
+    std::vector<Offs> offs;
     for (auto &ch : p->chunks) {
-        size_t a = put(ch.t1.data(), ch.t1.size() * sizeof(Task));
-        size_t b = put(ch.t2.data(), ch.t2.size() * sizeof(Task));
-        size_t c = put(ch.s1.data(), ch.s1.size() * sizeof(Seg));
-        size_t d = put(ch.s2.data(), ch.s2.size() * sizeof(Seg));
-        offs.emplace_back(a, b, c, d);
+        Offs o;
+        for (int c = 0; c < kTileConfigs; c++) {
+            o.t1[c] = put(ch.t1[c].data(), ch.t1[c].size() * sizeof(Task));
+            o.t2[c] = put(ch.t2[c].data(), ch.t2[c].size() * sizeof(Task));
+        }
+        o.s1 = put(ch.s1.data(), ch.s1.size() * sizeof(Seg));
+        o.s2 = put(ch.s2.data(), ch.s2.size() * sizeof(Seg));
+        offs.push_back(o);
     }
     B2G_HIP(hipMemcpyAsync(p->dev, host.data(), bytes, hipMemcpyHostToDevice,
                            p->ctx->stream));
@@ -382,10 +417,12 @@ static int compile_plan(b2g_plan p, const b2g_pair *pairs, size_t n_pairs,
     char *base = (char *)p->dev;
     for (size_t i = 0; i < p->chunks.size(); i++) {
         Chunk &ch = p->chunks[i];
-        ch.d_t1 = (Task *)(base + std::get<0>(offs[i]));
-        ch.d_t2 = (Task *)(base + std::get<1>(offs[i]));
-        ch.d_s1 = (Seg *)(base + std::get<2>(offs[i]));
-        ch.d_s2 = (Seg *)(base + std::get<3>(offs[i]));
+        for (int c = 0; c < kTileConfigs; c++) {
+            ch.d_t1[c] = (Task *)(base + offs[i].t1[c]);
+            ch.d_t2[c] = (Task *)(base + offs[i].t2[c]);
+        }
+        ch.d_s1 = (Seg *)(base + offs[i].s1);
+        ch.d_s2 = (Seg *)(base + offs[i].s2);
     }
     return B2G_OK;
 }
@@ -496,7 +533,8 @@ extern "C" int b2g_plan_stats(b2g_plan p, uint64_t *n1, uint64_t *n2,
     B2G_REQUIRE(p, B2G_ERR_ARG, "NULL plan");
     uint64_t a = 0, b = 0;
     for (auto &ch : p->chunks)
-        a += ch.t1.size(), b += ch.t2.size();
+        for (int c = 0; c < kTileConfigs; c++)
+            a += ch.t1[c].size(), b += ch.t2[c].size();
     if (n1)
         *n1 = a;
     if (n2)
@@ -544,27 +582,34 @@ extern "C" int b2g_hpsi(b2g_ctx ctx, b2g_plan p, const double *x, double *y,
             return p->timing ? hipEventRecord(ev[k], ctx->stream) : hipSuccess;
         };
         B2G_HIP(mark(0));
+        // phase 1 then phase 2, one launch per tile configuration present
         if (precision == B2G_PREC_F64) {
             double *W = (double *)p->w;
-            B2G_HIP((launch_seg_gemm<double, double, double, double>(
-                x, (const double *)p->rbuf->ptr, W, ch.d_t1, (int)ch.t1.size(),
-                ch.d_s1, p->x_len, 0, p->w_max, 1.0, 0, n_rhs, grid,
-                ctx->stream)));
+            for (int c = 0; c < kTileConfigs; c++)
+                B2G_HIP((launch_seg_gemm<double, double, double, double>(
+                    c, x, (const double *)p->rbuf->ptr, W, ch.d_t1[c],
+                    (int)ch.t1[c].size(), ch.d_s1, p->x_len, 0, p->w_max, 1.0,
+                    0, n_rhs, grid, ctx->stream)));
             B2G_HIP(mark(1));
-            B2G_HIP((launch_seg_gemm<double, double, double, double>(
-                (const double *)p->lbuf->ptr, W, y, ch.d_t2, (int)ch.t2.size(),
-                ch.d_s2, 0, p->w_max, p->y_len, scale, beta0, n_rhs, grid,
-                ctx->stream)));
+            for (int c = 0; c < kTileConfigs; c++)
+                B2G_HIP((launch_seg_gemm<double, double, double, double>(
+                    c, (const double *)p->lbuf->ptr, W, y, ch.d_t2[c],
+                    (int)ch.t2[c].size(), ch.d_s2, 0, p->w_max, p->y_len,
+                    scale, beta0, n_rhs, grid, ctx->stream)));
             B2G_HIP(mark(2));
         } else {
             float *W = (float *)p->w;
-            B2G_HIP((launch_seg_gemm<double, float, float, float>(
-                x, p->r32, W, ch.d_t1, (int)ch.t1.size(), ch.d_s1, p->x_len, 0,
-                p->w_max, 1.0, 0, n_rhs, grid, ctx->stream)));
+            for (int c = 0; c < kTileConfigs; c++)
+                B2G_HIP((launch_seg_gemm<double, float, float, float>(
+                    c, x, p->r32, W, ch.d_t1[c], (int)ch.t1[c].size(), ch.d_s1,
+                    p->x_len, 0, p->w_max, 1.0, 0, n_rhs, grid,
+                    ctx->stream)));
             B2G_HIP(mark(1));
-            B2G_HIP((launch_seg_gemm<float, float, double, float>(
-                p->l32, W, y, ch.d_t2, (int)ch.t2.size(), ch.d_s2, 0, p->w_max,
-                p->y_len, scale, beta0, n_rhs, grid, ctx->stream)));
+            for (int c = 0; c < kTileConfigs; c++)
+                B2G_HIP((launch_seg_gemm<float, float, double, float>(
+                    c, p->l32, W, y, ch.d_t2[c], (int)ch.t2[c].size(),
+                    ch.d_s2, 0, p->w_max, p->y_len, scale, beta0, n_rhs, grid,
+                    ctx->stream)));
             B2G_HIP(mark(2));
         }
     }
