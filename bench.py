@@ -81,7 +81,8 @@ def main():
 
     import numpy as np
     import torch
-    from b2g import synth
+    from b2g import autotune, synth
+    from b2g import effham as eh
     from b2g.effham import Context, EffectiveHamiltonian
 
     t0 = time.time()
@@ -90,6 +91,20 @@ def main():
                            data="device")
     ctx = Context(0)
     dev = ctx.device
+    # autotune the tile shapes on the problem's own block-size histogram
+    hp = eh.host_plan(pb)
+    import ctypes as C
+    from b2g import _lib as L
+    npair = C.c_uint64()
+    L.check(L.lib().b2g_effham_pairs(hp, C.byref(npair), None))
+    parr = (L.b2g_pair * npair.value)()
+    L.check(L.lib().b2g_effham_pairs(hp, C.byref(npair), parr))
+    eh.destroy(hp)
+    rep = autotune.representative_dim(np.ctypeslib.as_array(parr))
+    t_tune = time.time()
+    rates = autotune.tune(ctx, dim=min(max(rep, 64), 512))
+    log("autotune at block dim %d (%.1fs): %s" % (
+        rep, time.time() - t_tune, " ".join("%.1f" % r for r in rates)))
     ldev, rdev = synth.fill_device(pb, dev)
     torch.cuda.synchronize()
     h = EffectiveHamiltonian(pb, ctx, fp32=True, device_data=(ldev, rdev))
@@ -154,6 +169,18 @@ def main():
     tflops32 = 2.0 * h.macs * R / (ms32 * 1e-3) / 1e12
     log("FP32: %.3f ms/step, %.2f TFLOP/s" % (ms32, tflops32))
 
+    # fixed-tile baseline (64x64 everywhere) on the same problem
+    os.environ["B2G_TILE"] = "64"
+    hf = EffectiveHamiltonian(pb, ctx, fp32=False, device_data=(ldev, rdev))
+    os.environ.pop("B2G_TILE", None)
+    hh, h = h, hf
+    msf, _ = run("f64", max(3, args.steps // 2), False)
+    h = hh
+    hf.deallocate()
+    tflops_fixed = 2.0 * h.macs * R / (msf * 1e-3) / 1e12
+    log("FP64 fixed 64x64 tiles: %.3f ms/step, %.2f TFLOP/s" % (
+        msf, tflops_fixed))
+
     extra = {}
     if not args.no_davidson:
         # DMRG-like initial guess: the lowest diagonal state plus noise
@@ -212,6 +239,10 @@ def main():
                      "unit": "TFLOP/s",
                      "frac": round(achieved / FP64_MFMA_PEAK, 4),
                      "traffic": None},
+        "autotune": {"block_dim": rep, "tile_rates_tflops":
+                     [round(r, 2) for r in rates],
+                     "fixed_64x64_tflops": round(tflops_fixed, 4),
+                     "speedup_vs_fixed": round(tflops / tflops_fixed, 3)},
         "fp32": {"value": round(tflops32, 4), "unit": "TFLOP/s",
                  "ms_per_step": round(ms32, 4),
                  "frac_of_fp32_peak": round(tflops32 / FP32_MFMA_PEAK, 4)},

@@ -110,8 +110,9 @@ template <typename TA, typename TB, typename TD, typename TC, int TM, int TN,
           int TK>
 __global__ __launch_bounds__(256) void seg_gemm_kernel(
     const TA *__restrict__ A, const TB *__restrict__ B, TD *__restrict__ D,
-    const Task *__restrict__ tasks, int ntasks, const Seg *__restrict__ segs,
-    uint64_t a_rs, uint64_t b_rs, uint64_t d_rs, double scale, int beta0) {
+    TD *__restrict__ P, const Task *__restrict__ tasks, int ntasks,
+    const Seg *__restrict__ segs, uint64_t a_rs, uint64_t b_rs, uint64_t d_rs,
+    uint64_t p_rs, double scale, int beta0) {
     constexpr int WM = TM / 2, WN = TN / 2;
     constexpr int FM = WM / 16, FN = WN / 16;
     constexpr int ASZ = TK * (TM + 1), BSZ = TK * (TN + 1);
@@ -121,6 +122,7 @@ __global__ __launch_bounds__(256) void seg_gemm_kernel(
     A += rhs * a_rs;
     B += rhs * b_rs;
     D += rhs * d_rs;
+    P += rhs * p_rs;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
     const int lr = lane & 15, lk = lane >> 4;
@@ -211,7 +213,9 @@ __global__ __launch_bounds__(256) void seg_gemm_kernel(
                         const uint64_t idx =
                             task.d_off + (uint64_t)row * task.ldd + col;
                         const double v = scale * (double)acc[i][j][r];
-                        if (task.beta | beta0)
+                        if (task.beta & 2)
+                            P[idx] = (TD)v; // partial tile, reduced later
+                        else if ((task.beta | beta0) & 1)
                             D[idx] = (TD)((double)D[idx] + v);
                         else
                             D[idx] = (TD)v;
@@ -220,47 +224,98 @@ __global__ __launch_bounds__(256) void seg_gemm_kernel(
     }
 }
 
-// tile configurations (index = Task bucket id): workgroup tiles
-// 64x64 (K chunk 16), 128x128 (K chunk 16), 32x32 (K chunk 32)
-int tile_m(int cfg) {
-    static const int t[] = {64, 128, 32};
-    return t[cfg];
+template <typename TD>
+__global__ __launch_bounds__(256) void splitk_reduce_kernel(
+    TD *__restrict__ D, const TD *__restrict__ P,
+    const RedTask *__restrict__ tasks, int ntasks, uint64_t d_rs,
+    uint64_t p_rs, int beta0) {
+    const int rhs = blockIdx.y;
+    D += rhs * d_rs;
+    P += rhs * p_rs;
+    for (int t = blockIdx.x; t < ntasks; t += gridDim.x) {
+        const RedTask rt = tasks[t];
+        const uint32_t tsz = rt.tm * rt.tn;
+        for (uint32_t e = threadIdx.x; e < tsz; e += 256) {
+            const uint32_t r = e / rt.tn, c = e % rt.tn;
+            const uint32_t row = rt.row0 + r, col = rt.col0 + c;
+            if (row >= rt.M || col >= rt.N)
+                continue;
+            double sum = 0.0;
+            for (uint32_t p = 0; p < rt.nparts; p++)
+                sum += (double)P[rt.p_off + (uint64_t)p * tsz + e];
+            const uint64_t idx = rt.d_off + (uint64_t)row * rt.ldd + col;
+            D[idx] = ((rt.beta | beta0) & 1) ? (TD)((double)D[idx] + sum)
+                                             : (TD)sum;
+        }
+    }
 }
-int tile_n(int cfg) { return tile_m(cfg); }
 
-template <typename TA, typename TB, typename TD, typename TC>
-hipError_t launch_seg_gemm(int cfg, const TA *A, const TB *B, TD *D,
-                           const Task *tasks, int ntasks, const Seg *segs,
-                           uint64_t a_rs, uint64_t b_rs, uint64_t d_rs,
-                           double scale, int beta0, int n_rhs, int grid,
-                           hipStream_t st) {
+template <typename TD>
+hipError_t launch_splitk_reduce(TD *D, const TD *P, const RedTask *tasks,
+                                int ntasks, uint64_t d_rs, uint64_t p_rs,
+                                int beta0, int n_rhs, int grid,
+                                hipStream_t st) {
     if (ntasks <= 0)
         return hipSuccess;
     dim3 g((unsigned)(grid < ntasks ? grid : ntasks), (unsigned)n_rhs);
-    if (cfg == 1)
-        hipLaunchKernelGGL((seg_gemm_kernel<TA, TB, TD, TC, 128, 128, 16>), g,
-                           dim3(256), 0, st, A, B, D, tasks, ntasks, segs,
-                           a_rs, b_rs, d_rs, scale, beta0);
-    else if (cfg == 2)
-        hipLaunchKernelGGL((seg_gemm_kernel<TA, TB, TD, TC, 32, 32, 32>), g,
-                           dim3(256), 0, st, A, B, D, tasks, ntasks, segs,
-                           a_rs, b_rs, d_rs, scale, beta0);
-    else
-        hipLaunchKernelGGL((seg_gemm_kernel<TA, TB, TD, TC, 64, 64, 16>), g,
-                           dim3(256), 0, st, A, B, D, tasks, ntasks, segs,
-                           a_rs, b_rs, d_rs, scale, beta0);
+    hipLaunchKernelGGL(splitk_reduce_kernel<TD>, g, dim3(256), 0, st, D, P,
+                       tasks, ntasks, d_rs, p_rs, beta0);
+    return hipGetLastError();
+}
+template hipError_t launch_splitk_reduce<double>(double *, const double *,
+                                                 const RedTask *, int,
+                                                 uint64_t, uint64_t, int, int,
+                                                 int, hipStream_t);
+template hipError_t launch_splitk_reduce<float>(float *, const float *,
+                                                const RedTask *, int, uint64_t,
+                                                uint64_t, int, int, int,
+                                                hipStream_t);
+
+// tile configurations (index = Task bucket id): cfg = 3 * i + j with
+// TM = 32 << i and TN = 32 << j; K chunk 16 (32 for the 32x32 tile)
+int tile_m(int cfg) { return 32 << (cfg / 3); }
+int tile_n(int cfg) { return 32 << (cfg % 3); }
+
+template <typename TA, typename TB, typename TD, typename TC>
+hipError_t launch_seg_gemm(int cfg, const TA *A, const TB *B, TD *D, TD *P,
+                           const Task *tasks, int ntasks, const Seg *segs,
+                           uint64_t a_rs, uint64_t b_rs, uint64_t d_rs,
+                           uint64_t p_rs, double scale, int beta0, int n_rhs,
+                           int grid, hipStream_t st) {
+    if (ntasks <= 0)
+        return hipSuccess;
+    dim3 g((unsigned)(grid < ntasks ? grid : ntasks), (unsigned)n_rhs);
+#define B2G_LAUNCH(TM_, TN_, TK_)                                              \
+    hipLaunchKernelGGL((seg_gemm_kernel<TA, TB, TD, TC, TM_, TN_, TK_>), g,    \
+                       dim3(256), 0, st, A, B, D, P, tasks, ntasks, segs,      \
+                       a_rs, b_rs, d_rs, p_rs, scale, beta0)
+    switch (cfg) {
+    case 0: B2G_LAUNCH(32, 32, 32); break;
+    case 1: B2G_LAUNCH(32, 64, 16); break;
+    case 2: B2G_LAUNCH(32, 128, 16); break;
+    case 3: B2G_LAUNCH(64, 32, 16); break;
+    case 4: B2G_LAUNCH(64, 64, 16); break;
+    case 5: B2G_LAUNCH(64, 128, 16); break;
+    case 6: B2G_LAUNCH(128, 32, 16); break;
+    case 7: B2G_LAUNCH(128, 64, 16); break;
+    default: B2G_LAUNCH(128, 128, 16); break;
+    }
+#undef B2G_LAUNCH
     return hipGetLastError();
 }
 
 template hipError_t launch_seg_gemm<double, double, double, double>(
-    int, const double *, const double *, double *, const Task *, int, const Seg *,
-    uint64_t, uint64_t, uint64_t, double, int, int, int, hipStream_t);
+    int, const double *, const double *, double *, double *, const Task *, int, const Seg *,
+    uint64_t, uint64_t, uint64_t, uint64_t, double, int, int, int,
+    hipStream_t);
 template hipError_t launch_seg_gemm<double, float, float, float>(
-    int, const double *, const float *, float *, const Task *, int, const Seg *,
-    uint64_t, uint64_t, uint64_t, double, int, int, int, hipStream_t);
+    int, const double *, const float *, float *, float *, const Task *, int, const Seg *,
+    uint64_t, uint64_t, uint64_t, uint64_t, double, int, int, int,
+    hipStream_t);
 template hipError_t launch_seg_gemm<float, float, double, float>(
-    int, const float *, const float *, double *, const Task *, int, const Seg *,
-    uint64_t, uint64_t, uint64_t, double, int, int, int, hipStream_t);
+    int, const float *, const float *, double *, double *, const Task *, int, const Seg *,
+    uint64_t, uint64_t, uint64_t, uint64_t, double, int, int, int,
+    hipStream_t);
 
 // ---------------------------------------------------------------- diagonal
 __global__ __launch_bounds__(256) void diag_kernel(

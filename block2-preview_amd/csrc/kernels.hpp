@@ -32,7 +32,16 @@ struct Task {
     uint32_t ldd, M, N; // block dims (for masking)
     uint32_t row0, col0;
     uint32_t seg_begin, seg_end;
-    uint32_t beta; // 1: accumulate into D
+    // bit 0: accumulate into D; bit 1: partial tile (split K) written to the
+    // partial buffer P with d_off / ldd addressing the tile's dense slot
+    uint32_t beta;
+};
+
+// Deterministic split-K reduction of one output tile:
+//   D[tile] = (beta ? D[tile] : 0) + sum_{p < nparts} P[p_off + p*tm*tn + ...]
+struct RedTask {
+    uint64_t d_off, p_off;
+    uint32_t ldd, M, N, row0, col0, tm, tn, nparts, beta, pad;
 };
 
 // diagonal: output block with its contribution range

@@ -7,17 +7,22 @@
 
 namespace b2g {
 
-// tile configuration cfg: 0 = 64x64, 1 = 128x128, 2 = 32x32 workgroup tiles
+// workgroup tile configuration cfg = 3 * i + j: TM = 32 << i, TN = 32 << j
 int tile_m(int cfg);
 int tile_n(int cfg);
-constexpr int kTileConfigs = 3;
+constexpr int kTileConfigs = 9;
 
 template <typename TA, typename TB, typename TD, typename TC>
-hipError_t launch_seg_gemm(int cfg, const TA *A, const TB *B, TD *D,
+hipError_t launch_seg_gemm(int cfg, const TA *A, const TB *B, TD *D, TD *P,
                            const Task *tasks, int ntasks, const Seg *segs,
                            uint64_t a_rs, uint64_t b_rs, uint64_t d_rs,
-                           double scale, int beta0, int n_rhs, int grid,
-                           hipStream_t st);
+                           uint64_t p_rs, double scale, int beta0, int n_rhs,
+                           int grid, hipStream_t st);
+template <typename TD>
+hipError_t launch_splitk_reduce(TD *D, const TD *P, const RedTask *tasks,
+                                int ntasks, uint64_t d_rs, uint64_t p_rs,
+                                int beta0, int n_rhs, int grid,
+                                hipStream_t st);
 
 hipError_t launch_diag(const double *L, const double *R, double *Dv,
                        const DiagBlock *blocks, int nblocks,

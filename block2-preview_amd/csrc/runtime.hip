@@ -4,8 +4,10 @@
 #include "launch.hpp"
 
 #include <algorithm>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <string>
 #include <map>
 #include <sstream>
 #include <tuple>
@@ -174,10 +176,13 @@ struct Chunk {
     // host images; tasks are bucketed by tile configuration
     std::vector<Task> t1[kTileConfigs], t2[kTileConfigs];
     std::vector<Seg> s1, s2;
-    uint64_t w_len = 0; // W elements per right-hand side
+    std::vector<RedTask> red; // split-K reductions of phase 2
+    uint64_t w_len = 0;       // W elements per right-hand side
+    uint64_t p_len = 0;       // partial-tile elements per right-hand side
     // device copies (inside plan->dev)
     Task *d_t1[kTileConfigs] = {}, *d_t2[kTileConfigs] = {};
     Seg *d_s1 = nullptr, *d_s2 = nullptr;
+    RedTask *d_red = nullptr;
 };
 
 } // namespace b2g
@@ -193,6 +198,9 @@ struct b2g_plan_s {
     float *l32 = nullptr, *r32 = nullptr;
     void *w = nullptr; // W scratch
     size_t w_bytes = 0;
+    uint64_t p_max = 0;   // partial elements per RHS (max over chunks)
+    void *pbuf = nullptr; // split-K partial tiles (FP64)
+    size_t p_bytes = 0;
     unsigned flags = 0;
     // optional per-phase timing: event triples (before phase 1, between the
     // phases, after phase 2) resolved when queried
@@ -209,43 +217,178 @@ static uint64_t tile_cost(const Task &t, const std::vector<Seg> &segs) {
     return k + 1;
 }
 
-// tile configuration of an M x N output block: B2G_TILE=32|64|128 forces
-// one shape, otherwise large blocks take 128x128, small ones 32x32
-static int pick_tile(uint32_t M, uint32_t N) {
-    const char *e = getenv("B2G_TILE");
-    int forced = -1;
-    if (e != nullptr) {
-        const std::string v(e);
-        forced = v == "64" ? 0 : v == "128" ? 1 : v == "32" ? 2 : -1;
+// Tiling policy.  Every output block is covered by workgroup tiles whose
+// row and column sizes are chosen from {32, 64, 128}: full 128 tiles for the
+// bulk and one remainder tile per dimension, picked jointly to minimise the
+// estimated time  sum(TM * TN * K / rate(TM, TN))  with measured per-config
+// rates (b2g_set_tile_rates, filled by the autotuner).  B2G_TILE=32|64|128
+// forces square tiles of one size.
+static double g_rates[kTileConfigs] = {30, 34, 36, 34, 42, 46, 36, 46, 50};
+
+static int size_index(uint32_t t) { return t == 32 ? 0 : t == 64 ? 1 : 2; }
+
+// candidate row (or column) covers of a dimension of length d
+static std::vector<std::vector<uint32_t>> covers(uint32_t d) {
+    std::vector<uint32_t> base(d / 128, 128u);
+    const uint32_t r = d % 128;
+    std::vector<std::vector<uint32_t>> out;
+    if (r == 0) {
+        out.push_back(base);
+        return out;
     }
-    if (forced >= 0)
-        return forced;
-    if (M >= 128 && N >= 128)
-        return 1;
-    if (M <= 32 && N <= 32)
-        return 2;
-    return 0;
+    for (const std::vector<uint32_t> &tail :
+         std::vector<std::vector<uint32_t>>{{128}, {64}, {64, 32}, {32},
+                                            {32, 32}}) {
+        uint32_t cov = 0;
+        for (uint32_t t : tail)
+            cov += t;
+        if (cov < r)
+            continue;
+        std::vector<uint32_t> v = base;
+        v.insert(v.end(), tail.begin(), tail.end());
+        out.push_back(v);
+    }
+    return out;
 }
 
-// tiles of an M x N output block
-static void add_tiles(std::vector<Task> *tasks, uint64_t d_off, uint32_t M,
-                      uint32_t N, uint32_t sb, uint32_t se, uint32_t beta) {
-    const int cfg = pick_tile(M, N);
-    const uint32_t TMc = tile_m(cfg), TNc = tile_n(cfg);
-    for (uint32_t r = 0; r < M; r += TMc)
-        for (uint32_t c = 0; c < N; c += TNc) {
-            Task t;
-            t.d_off = d_off;
-            t.ldd = N;
-            t.M = M;
-            t.N = N;
-            t.row0 = r;
-            t.col0 = c;
-            t.seg_begin = sb;
-            t.seg_end = se;
-            t.beta = beta;
-            tasks[cfg].push_back(t);
+struct Tile {
+    uint32_t row0, col0, tm, tn;
+};
+
+// tiles covering an M x N output block whose tasks have depth K
+static std::vector<Tile> tile_cover(uint32_t M, uint32_t N, uint64_t K) {
+    const char *e = getenv("B2G_TILE");
+    std::vector<uint32_t> rows, cols;
+    uint32_t fm = 0, fn = 0;
+    if (e != nullptr) {
+        // "T" (square) or "TMxTN", sizes from {32, 64, 128}
+        unsigned a = 0, b = 0;
+        const int nr = sscanf(e, "%ux%u", &a, &b);
+        if (nr == 1)
+            b = a;
+        auto okv = [](unsigned t) { return t == 32 || t == 64 || t == 128; };
+        if (nr >= 1 && okv(a) && okv(b))
+            fm = a, fn = b;
+    }
+    if (fm != 0) {
+        rows.assign((M + fm - 1) / fm, fm);
+        cols.assign((N + fn - 1) / fn, fn);
+    } else {
+        double best = 1e300;
+        for (const auto &rc : covers(M))
+            for (const auto &cc : covers(N)) {
+                double c = 0;
+                for (uint32_t a : rc)
+                    for (uint32_t b : cc)
+                        c += (double)a * b * (double)(K + 1) /
+                             g_rates[3 * size_index(a) + size_index(b)];
+                if (c < best)
+                    best = c, rows = rc, cols = cc;
+            }
+    }
+    std::vector<Tile> out;
+    uint32_t r0 = 0;
+    for (uint32_t a : rows) {
+        uint32_t c0 = 0;
+        for (uint32_t b : cols) {
+            out.push_back(Tile{r0, c0, a, b});
+            c0 += b;
         }
+        r0 += a;
+    }
+    return out;
+}
+
+static Task make_task(uint64_t d_off, uint32_t ldd, uint32_t M, uint32_t N,
+                      const Tile &t, uint32_t sb, uint32_t se, uint32_t beta) {
+    Task k;
+    k.d_off = d_off;
+    k.ldd = ldd;
+    k.M = M;
+    k.N = N;
+    k.row0 = t.row0;
+    k.col0 = t.col0;
+    k.seg_begin = sb;
+    k.seg_end = se;
+    k.beta = beta;
+    return k;
+}
+
+static int cfg_of(const Tile &t) {
+    return 3 * size_index(t.tm) + size_index(t.tn);
+}
+
+// one task per tile (phase 1 and unsplit phase-2 blocks)
+static void add_tiles(std::vector<Task> *tasks, uint64_t d_off, uint32_t M,
+                      uint32_t N, uint64_t K, uint32_t sb, uint32_t se,
+                      uint32_t beta) {
+    for (const Tile &t : tile_cover(M, N, K))
+        tasks[cfg_of(t)].push_back(make_task(d_off, N, M, N, t, sb, se, beta));
+}
+
+// maximum depth of one phase-2 task before its segment list is split
+static uint64_t ksplit() {
+    const char *e = getenv("B2G_KSPLIT");
+    if (e != nullptr && atoll(e) > 0)
+        return (uint64_t)atoll(e);
+    return 2048;
+}
+
+// Phase-2 tiles of an output block whose segments [sb, se) are deep: the
+// segment list is cut at segment boundaries into groups of depth about
+// ksplit(); each (tile, group) becomes a partial task writing a dense
+// TM x TN slot of the partial buffer, and a reduction task sums the slots in
+// group order (deterministic) into the output.
+static void add_split_tiles(Chunk &ch, uint64_t d_off, uint32_t M, uint32_t N,
+                            uint32_t sb, uint32_t se, uint32_t beta) {
+    std::vector<std::pair<uint32_t, uint32_t>> groups;
+    const uint64_t kmax = ksplit();
+    uint64_t K = 0, acc = 0;
+    uint32_t g0 = sb;
+    for (uint32_t k = sb; k < se; k++) {
+        K += ch.s2[k].K;
+        acc += ch.s2[k].K;
+        if (acc >= kmax && k + 1 < se) {
+            groups.emplace_back(g0, k + 1);
+            g0 = k + 1, acc = 0;
+        }
+    }
+    groups.emplace_back(g0, se);
+    const std::vector<Tile> tiles = tile_cover(M, N, K / groups.size());
+    if (groups.size() == 1) {
+        for (const Tile &t : tiles)
+            ch.t2[cfg_of(t)].push_back(make_task(d_off, N, M, N, t, sb, se,
+                                                 beta));
+        return;
+    }
+    for (const Tile &t : tiles) {
+        const uint64_t tsz = (uint64_t)t.tm * t.tn;
+        const uint64_t p0 = ch.p_len;
+        for (size_t g = 0; g < groups.size(); g++) {
+            // slot of group g: element (row, col) of the block lands at
+            // p0 + g*tsz + (row - row0)*tn + (col - col0)  (mod 2^64)
+            const uint64_t doff = p0 + g * tsz - (uint64_t)t.row0 * t.tn -
+                                  (uint64_t)t.col0;
+            ch.t2[cfg_of(t)].push_back(make_task(doff, t.tn, M, N, t,
+                                                 groups[g].first,
+                                                 groups[g].second, 2u));
+        }
+        RedTask rt;
+        rt.d_off = d_off;
+        rt.p_off = p0;
+        rt.ldd = N;
+        rt.M = M;
+        rt.N = N;
+        rt.row0 = t.row0;
+        rt.col0 = t.col0;
+        rt.tm = t.tm;
+        rt.tn = t.tn;
+        rt.nparts = (uint32_t)groups.size();
+        rt.beta = beta;
+        rt.pad = 0;
+        ch.red.push_back(rt);
+        ch.p_len += groups.size() * tsz;
+    }
 }
 
 static void sort_tasks(std::vector<Task> &tasks, const std::vector<Seg> &segs) {
@@ -329,7 +472,7 @@ static int compile_plan(b2g_plan p, const b2g_pair *pairs, size_t n_pairs,
                 s.alpha = 1.0;
                 const uint32_t si = (uint32_t)ch.s1.size();
                 ch.s1.push_back(s);
-                add_tiles(ch.t1, ch.w_len, q.c_m, q.v_n, si, si + 1, 0);
+                add_tiles(ch.t1, ch.w_len, q.c_m, q.v_n, q.c_n, si, si + 1, 0);
                 ch.w_len += sz;
             }
             pw.push_back(it->second);
@@ -357,8 +500,8 @@ static int compile_plan(b2g_plan p, const b2g_pair *pairs, size_t n_pairs,
                     s.alpha = q.alpha;
                     ch.s2.push_back(s);
                 }
-            add_tiles(ch.t2, blk.off, blk.m, blk.n, sb,
-                      (uint32_t)ch.s2.size(), beta);
+            add_split_tiles(ch, blk.off, blk.m, blk.n, sb,
+                            (uint32_t)ch.s2.size(), beta);
         };
         if (first) {
             for (size_t b = 0; b < n_yb; b++) {
@@ -374,6 +517,7 @@ static int compile_plan(b2g_plan p, const b2g_pair *pairs, size_t n_pairs,
             sort_tasks(ch.t2[c], ch.s2);
         }
         p->w_max = std::max(p->w_max, ch.w_len);
+        p->p_max = std::max(p->p_max, ch.p_len);
         p->chunks.push_back(std::move(ch));
         first = false;
         i0 = i1;
@@ -386,6 +530,7 @@ static int compile_plan(b2g_plan p, const b2g_pair *pairs, size_t n_pairs,
         for (int c = 0; c < kTileConfigs; c++)
             bytes += (ch.t1[c].size() + ch.t2[c].size()) * sizeof(Task);
         bytes += (ch.s1.size() + ch.s2.size()) * sizeof(Seg);
+        bytes += ch.red.size() * sizeof(RedTask);
     }
     std::vector<char> host(bytes + 16);
     B2G_HIP(hipMalloc(&p->dev, bytes + 16));
@@ -398,7 +543,7 @@ static int compile_plan(b2g_plan p, const b2g_pair *pairs, size_t n_pairs,
         return o;
     };
     struct Offs {
-        size_t t1[kTileConfigs], t2[kTileConfigs], s1, s2;
+        size_t t1[kTileConfigs], t2[kTileConfigs], s1, s2, red;
     };
     std::vector<Offs> offs;
     for (auto &ch : p->chunks) {
@@ -409,6 +554,7 @@ static int compile_plan(b2g_plan p, const b2g_pair *pairs, size_t n_pairs,
         }
         o.s1 = put(ch.s1.data(), ch.s1.size() * sizeof(Seg));
         o.s2 = put(ch.s2.data(), ch.s2.size() * sizeof(Seg));
+        o.red = put(ch.red.data(), ch.red.size() * sizeof(RedTask));
         offs.push_back(o);
     }
     B2G_HIP(hipMemcpyAsync(p->dev, host.data(), bytes, hipMemcpyHostToDevice,
@@ -423,6 +569,7 @@ static int compile_plan(b2g_plan p, const b2g_pair *pairs, size_t n_pairs,
         }
         ch.d_s1 = (Seg *)(base + offs[i].s1);
         ch.d_s2 = (Seg *)(base + offs[i].s2);
+        ch.d_red = (RedTask *)(base + offs[i].red);
     }
     return B2G_OK;
 }
@@ -488,7 +635,19 @@ extern "C" int b2g_plan_destroy(b2g_plan p) {
         hipFree(p->r32);
     if (p->w)
         hipFree(p->w);
+    if (p->pbuf)
+        hipFree(p->pbuf);
     delete p;
+    return B2G_OK;
+}
+
+extern "C" int b2g_set_tile_rates(const double *rates, int n) {
+    B2G_REQUIRE(rates != nullptr && n == kTileConfigs, B2G_ERR_ARG,
+                "b2g_set_tile_rates: expects 9 rates");
+    for (int i = 0; i < n; i++)
+        B2G_REQUIRE(rates[i] > 0, B2G_ERR_ARG, "rates must be positive");
+    for (int i = 0; i < n; i++)
+        g_rates[i] = rates[i];
     return B2G_OK;
 }
 
@@ -566,6 +725,18 @@ extern "C" int b2g_hpsi(b2g_ctx ctx, b2g_plan p, const double *x, double *y,
         B2G_HIP(hipMalloc(&p->w, need));
         p->w_bytes = need;
     }
+    const size_t pneed = std::max<uint64_t>(p->p_max, 1) * n_rhs * 8;
+    if (pneed > p->p_bytes) {
+        if (p->pbuf) {
+            B2G_HIP(hipStreamSynchronize(ctx->stream));
+            B2G_HIP(hipFree(p->pbuf));
+            p->pbuf = nullptr;
+            p->p_bytes = 0;
+        }
+        B2G_HIP(hipMalloc(&p->pbuf, pneed));
+        p->p_bytes = pneed;
+    }
+    double *P = (double *)p->pbuf;
     const int grid = ctx->num_cu * 8;
     for (size_t ic = 0; ic < p->chunks.size(); ic++) {
         const Chunk &ch = p->chunks[ic];
@@ -587,29 +758,35 @@ extern "C" int b2g_hpsi(b2g_ctx ctx, b2g_plan p, const double *x, double *y,
             double *W = (double *)p->w;
             for (int c = 0; c < kTileConfigs; c++)
                 B2G_HIP((launch_seg_gemm<double, double, double, double>(
-                    c, x, (const double *)p->rbuf->ptr, W, ch.d_t1[c],
-                    (int)ch.t1[c].size(), ch.d_s1, p->x_len, 0, p->w_max, 1.0,
-                    0, n_rhs, grid, ctx->stream)));
+                    c, x, (const double *)p->rbuf->ptr, W, W, ch.d_t1[c],
+                    (int)ch.t1[c].size(), ch.d_s1, p->x_len, 0, p->w_max, 0,
+                    1.0, 0, n_rhs, grid, ctx->stream)));
             B2G_HIP(mark(1));
             for (int c = 0; c < kTileConfigs; c++)
                 B2G_HIP((launch_seg_gemm<double, double, double, double>(
-                    c, (const double *)p->lbuf->ptr, W, y, ch.d_t2[c],
+                    c, (const double *)p->lbuf->ptr, W, y, P, ch.d_t2[c],
                     (int)ch.t2[c].size(), ch.d_s2, 0, p->w_max, p->y_len,
-                    scale, beta0, n_rhs, grid, ctx->stream)));
+                    p->p_max, scale, beta0, n_rhs, grid, ctx->stream)));
+            B2G_HIP(launch_splitk_reduce<double>(
+                y, P, ch.d_red, (int)ch.red.size(), p->y_len, p->p_max, beta0,
+                n_rhs, grid, ctx->stream));
             B2G_HIP(mark(2));
         } else {
             float *W = (float *)p->w;
             for (int c = 0; c < kTileConfigs; c++)
                 B2G_HIP((launch_seg_gemm<double, float, float, float>(
-                    c, x, p->r32, W, ch.d_t1[c], (int)ch.t1[c].size(), ch.d_s1,
-                    p->x_len, 0, p->w_max, 1.0, 0, n_rhs, grid,
+                    c, x, p->r32, W, W, ch.d_t1[c], (int)ch.t1[c].size(),
+                    ch.d_s1, p->x_len, 0, p->w_max, 0, 1.0, 0, n_rhs, grid,
                     ctx->stream)));
             B2G_HIP(mark(1));
             for (int c = 0; c < kTileConfigs; c++)
                 B2G_HIP((launch_seg_gemm<float, float, double, float>(
-                    c, p->l32, W, y, ch.d_t2[c], (int)ch.t2[c].size(),
-                    ch.d_s2, 0, p->w_max, p->y_len, scale, beta0, n_rhs, grid,
-                    ctx->stream)));
+                    c, p->l32, W, y, P, ch.d_t2[c], (int)ch.t2[c].size(),
+                    ch.d_s2, 0, p->w_max, p->y_len, p->p_max, scale, beta0,
+                    n_rhs, grid, ctx->stream)));
+            B2G_HIP(launch_splitk_reduce<double>(
+                y, P, ch.d_red, (int)ch.red.size(), p->y_len, p->p_max, beta0,
+                n_rhs, grid, ctx->stream));
             B2G_HIP(mark(2));
         }
     }

@@ -151,6 +151,12 @@ int b2g_plan_macs(b2g_plan plan, uint64_t *out);
 int b2g_plan_set_timing(b2g_plan plan, int enable);
 int b2g_plan_timing(b2g_plan plan, double *ms_phase1, double *ms_phase2,
                     uint64_t *n_launch);
+/* Relative throughput (any unit) of the 9 workgroup tile shapes
+ * TM x TN, TM, TN in {32, 64, 128}, index 3*i + j with TM = 32 << i,
+ * TN = 32 << j.  The tiling of plans uploaded afterwards minimises the
+ * estimated time with these rates (the autotuner measures them).  Process
+ * wide.                                                                   */
+int b2g_set_tile_rates(const double *rates, int n);
 /* Number of kernel tasks per phase and scratch size, for reporting.      */
 int b2g_plan_stats(b2g_plan plan, uint64_t *n_tasks1, uint64_t *n_tasks2,
                    uint64_t *scratch_elems);

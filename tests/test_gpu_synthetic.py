@@ -1,7 +1,7 @@
 """GPU H*psi on seeded synthetic middle-site problems against the oracle:
 every tile configuration of the grouped GEMM (32x32, 64x64, 128x128 forced
 through B2G_TILE, and the automatic choice), the scratch split into several
-chunks, 4 right-hand sides, FP32.  Blocks range from 1x1 up to 200x200, so
+chunks, split-K partial tiles (B2G_KSPLIT), 4 right-hand sides, FP32.  Blocks range from 1x1 up to 200x200, so
 edge tiles, K remainders and single-element blocks all occur.
 Tolerances: FP64 1e-12 relative (2-norm), FP32 2e-6."""
 import os
@@ -48,7 +48,7 @@ def run(gpu_ctx, pb, xs, env, precision="f64"):
     return y
 
 
-@pytest.mark.parametrize("tile", ["32", "64", "128", "auto"])
+@pytest.mark.parametrize("tile", ["32", "64", "128", "32x128", "128x32", "64x128", "128x64", "auto"])
 def test_tile_configurations(gpu_ctx, problem, tile):
     pb, xs, ys = problem
     print("tile", tile, flush=True)
@@ -64,6 +64,22 @@ def test_scratch_chunks(gpu_ctx, problem):
     y = run(gpu_ctx, pb, xs, {"B2G_W_CAP": "200000"})
     for r in range(4):
         assert rel(y[r], ys[r]) < 1e-12, r
+
+
+@pytest.mark.parametrize("ksplit", ["1", "64", "1000"])
+def test_split_k(gpu_ctx, problem, ksplit):
+    # deep output blocks cut into partial tiles plus a fixed-order reduction;
+    # with several scratch chunks the reductions accumulate into y
+    pb, xs, ys = problem
+    print("ksplit", ksplit, flush=True)
+    for env in ({"B2G_KSPLIT": ksplit},
+                {"B2G_KSPLIT": ksplit, "B2G_W_CAP": "200000"}):
+        y = run(gpu_ctx, pb, xs, env)
+        for r in range(4):
+            assert rel(y[r], ys[r]) < 1e-12, (env, r)
+    y = run(gpu_ctx, pb, xs, {"B2G_KSPLIT": ksplit}, precision="f32")
+    for r in range(4):
+        assert rel(y[r], ys[r]) < 2e-6, r
 
 
 def test_fp32(gpu_ctx, problem):
