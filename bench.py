@@ -73,7 +73,26 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-davidson", action="store_true")
+    ap.add_argument("--rates", default=None,
+                    help="9 comma-separated tile rates instead of autotuning "
+                         "(profiling runs: keeps the tuner's launches out of "
+                         "the per-kernel statistics)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run 5 device-to-device copies of MIB MiB "
+                         "(reference kernel of known traffic for the HBM "
+                         "byte counters) and exit")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        n = args.calibrate_copy * (1 << 20) // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written":
+                          n * 8, "launches": 5}))
+        return 0
     if args.gpus != 1:
         print("bench.py measures one GPU only: the hot path is a single-GPU "
               "kernel here; run with --gpus 1", file=sys.stderr)
@@ -102,7 +121,11 @@ def main():
     eh.destroy(hp)
     rep = autotune.representative_dim(np.ctypeslib.as_array(parr))
     t_tune = time.time()
-    rates = autotune.tune(ctx, dim=min(max(rep, 64), 512))
+    if args.rates:
+        rates = [float(r) for r in args.rates.split(",")]
+        L.check(L.lib().b2g_set_tile_rates((C.c_double * 9)(*rates), 9))
+    else:
+        rates = autotune.tune(ctx, dim=min(max(rep, 64), 512))
     log("autotune at block dim %d (%.1fs): %s" % (
         rep, time.time() - t_tune, " ".join("%.1f" % r for r in rates)))
     ldev, rdev = synth.fill_device(pb, dev)
@@ -136,7 +159,7 @@ def main():
         e0.record(stream)
         for _ in range(steps):
             code = h.lib.b2g_effham_hpsi(h.h, x.data_ptr(), y.data_ptr(), R,
-                                         1.0, 0, {"f64": 0, "f32": 1}[prec])
+                                         1.0, 0, L.PREC[prec])
             if code != 0:
                 raise RuntimeError("b2g_effham_hpsi failed: %d" % code)
         e1.record(stream)
@@ -149,25 +172,42 @@ def main():
 
     ms64, ph = run("f64", args.steps, True)
     tflops = 2.0 * h.macs * R / (ms64 * 1e-3) / 1e12
-    # per-launch figures: a step launches each phase once per scratch chunk;
-    # algorithmic FLOPs per launch = the phase's reference-order FLOPs of one
-    # step divided by the launches per step
-    ms1, ms2, nl = ph
-    per_step = max(nl // args.steps, 1)
-    d1, d2 = ms1 / max(nl, 1), ms2 / max(nl, 1)
-    f1 = 2.0 * macs1 * R / per_step
-    f2 = 2.0 * macs2 * R / per_step
-    if d2 >= d1:
-        dom, dflop, dur = "phase2 V+=alpha*op(A)*W", f2, d2
-    else:
-        dom, dflop, dur = "phase1 W=C*op(B)", f1, d1
-    achieved = dflop / (dur * 1e-3) / 1e12
-    log("FP64: %.3f ms/step, %.2f TFLOP/s; %d launches/phase/step; phase1 "
-        "%.3f ms/launch (%.2f TF) phase2 %.3f ms/launch (%.2f TF)" % (
-            ms64, tflops, per_step, d1, f1 / d1 / 1e9, d2, f2 / d2 / 1e9))
+    # per-kernel figures from one event pair around every launch; the
+    # dominant kernel is the one with the largest summed time, and its
+    # algorithmic FLOPs per launch are the plan's per-kernel count (tiles of
+    # that launch, all right-hand sides) divided by its launches per step
+    kms, kn = ph
+    kfl = h.kernel_flops()
+    kernels = {}
+    for k in range(L.NKERNELS):
+        if kn[k] == 0:
+            continue
+        per_step = kn[k] // args.steps
+        avg = kms[k] / kn[k]
+        fl = kfl[k] * R / max(per_step, 1)
+        kernels[L.kernel_name(k)] = dict(
+            launches_per_step=per_step, avg_launch_ms=round(avg, 4),
+            tflops=round(fl / (avg * 1e-3) / 1e12, 3) if fl > 0 else None)
+    dom_k = max((k for k in range(L.NKERNELS) if kn[k] and kfl[k] > 0),
+                key=lambda k: kms[k])
+    dom = L.kernel_name(dom_k)
+    dur = kms[dom_k] / kn[dom_k]
+    per_step = kn[dom_k] // args.steps
+    achieved = kfl[dom_k] * R / max(per_step, 1) / (dur * 1e-3) / 1e12
+    t_gemm = sum(kms[k] for k in range(L.NKERNELS)) / args.steps
+    log("FP64: %.3f ms/step (%.3f ms in kernels), %.2f TFLOP/s" % (
+        ms64, t_gemm, tflops))
+    for name, kd in sorted(kernels.items()):
+        log("  %-16s %2d launch/step %8.3f ms/launch %s TF" % (
+            name, kd["launches_per_step"], kd["avg_launch_ms"],
+            kd["tflops"]))
     ms32, _ = run("f32", max(3, args.steps // 2), False)
     tflops32 = 2.0 * h.macs * R / (ms32 * 1e-3) / 1e12
     log("FP32: %.3f ms/step, %.2f TFLOP/s" % (ms32, tflops32))
+    msb, _ = run("bf16x3", max(3, args.steps // 2), False)
+    tflopsb = 2.0 * h.macs * R / (msb * 1e-3) / 1e12
+    log("split-BF16 x3: %.3f ms/step, %.2f TFLOP/s (FP64-equivalent work)" % (
+        msb, tflopsb))
 
     # fixed-tile baseline (64x64 everywhere) on the same problem
     os.environ["B2G_TILE"] = "64"
@@ -191,18 +231,24 @@ def main():
         guess /= torch.linalg.norm(guess)
         x64 = guess.clone()
         e64, nd64, nf64, t64 = h.eigs(x64, conv_thrd=1e-9)
-        xmx = guess.clone()
-        emx, ndmx, nfmx, tmx = h.eigs(xmx, conv_thrd=1e-9, precision="f32",
-                                      refine=True)
-        ov = float(torch.dot(x64, xmx).abs())
-        extra["davidson"] = dict(
-            conv_thrd=1e-9, e_fp64=e64, ndav_fp64=nd64, s_fp64=round(t64, 4),
-            e_mixed=emx, ndav_mixed=ndmx, s_mixed=round(tmx, 4),
-            abs_dE=abs(emx - e64), infidelity=max(0.0, 1.0 - ov),
-            speedup_mixed=round(t64 / tmx, 3) if tmx > 0 else None)
-        log("Davidson FP64: E=%.12f ndav=%d %.3fs | mixed: E=%.12f ndav=%d "
-            "%.3fs | |dE|=%.2e" % (e64, nd64, t64, emx, ndmx, tmx,
-                                   abs(emx - e64)))
+        dav = dict(conv_thrd=1e-9, e_fp64=e64, ndav_fp64=nd64,
+                   s_fp64=round(t64, 4))
+        log("Davidson FP64: E=%.12f ndav=%d %.3fs" % (e64, nd64, t64))
+        # mixed precision: reduced-precision matvecs to convergence, then an
+        # FP64 refinement from the converged vector
+        for tag, prec in (("f32", "f32"), ("bf16x3", "bf16x3")):
+            xmx = guess.clone()
+            emx, ndmx, nfmx, tmx = h.eigs(xmx, conv_thrd=1e-9,
+                                          precision=prec, refine=True)
+            ov = float(torch.dot(x64, xmx).abs())
+            dav["mixed_" + tag] = dict(
+                e=emx, ndav=ndmx, s=round(tmx, 4), abs_dE=abs(emx - e64),
+                infidelity=max(0.0, 1.0 - ov),
+                speedup_vs_fp64=round(t64 / tmx, 3) if tmx > 0 else None)
+            log("Davidson %s+FP64 refine: E=%.12f ndav=%d %.3fs |dE|=%.2e "
+                "1-|<a|b>|=%.2e" % (tag, emx, ndmx, tmx, abs(emx - e64),
+                                    max(0.0, 1.0 - ov)))
+        extra["davidson"] = dav
 
     cpu = None
     if not args.no_cpu:
@@ -239,6 +285,7 @@ def main():
                      "unit": "TFLOP/s",
                      "frac": round(achieved / FP64_MFMA_PEAK, 4),
                      "traffic": None},
+        "kernels_fp64": kernels,
         "autotune": {"block_dim": rep, "tile_rates_tflops":
                      [round(r, 2) for r in rates],
                      "fixed_64x64_tflops": round(tflops_fixed, 4),
@@ -246,6 +293,10 @@ def main():
         "fp32": {"value": round(tflops32, 4), "unit": "TFLOP/s",
                  "ms_per_step": round(ms32, 4),
                  "frac_of_fp32_peak": round(tflops32 / FP32_MFMA_PEAK, 4)},
+        "bf16x3": {"value": round(tflopsb, 4), "unit": "TFLOP/s",
+                   "ms_per_step": round(msb, 4),
+                   "note": "FP64-equivalent FLOPs; 3 BF16 MFMA products "
+                           "per multiply-add, FP32 accumulation"},
         "cpu_baseline": cpu,
     }
     out.update(extra)

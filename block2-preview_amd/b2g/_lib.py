@@ -78,6 +78,19 @@ class b2g_problem(C.Structure):
 
 # every symbol declared in include/b2g.h, with (restype, argtypes)
 _vp = C.c_void_p
+
+# kernel ids of one H*psi (b2g.h): phase-1 / phase-2 grouped GEMM per tile
+# configuration c = 3*i + j (TM = 32 << i, TN = 32 << j), split-K reduction
+K_PHASE1, K_PHASE2, K_SPLITK_REDUCE, NKERNELS = 0, 9, 18, 19
+
+
+def kernel_name(k):
+    if k == K_SPLITK_REDUCE:
+        return "splitk_reduce"
+    c = k % 9
+    return "phase%d_%dx%d" % (1 if k < K_PHASE2 else 2, 32 << (c // 3),
+                              32 << (c % 3))
+
 _SIGS = {
     "b2g_last_error": (C.c_char_p, []),
     "b2g_ctx_create": (C.c_int, [C.c_int, C.c_uint, C.POINTER(_vp)]),
@@ -85,7 +98,8 @@ _SIGS = {
     "b2g_ctx_synchronize": (C.c_int, [_vp]),
     "b2g_ctx_stream": (_vp, [_vp]),
     "b2g_plan_set_timing": (C.c_int, [_vp, C.c_int]),
-    "b2g_plan_timing": (C.c_int, [_vp, f64p, f64p, u64p]),
+    "b2g_plan_timing": (C.c_int, [_vp, f64p, u64p]),
+    "b2g_plan_kernel_flops": (C.c_int, [_vp, f64p]),
     "b2g_effham_plan": (_vp, [_vp]),
     "b2g_set_tile_rates": (C.c_int, [f64p, C.c_int]),
     "b2g_ctx_num_cu": (C.c_int, [_vp, C.POINTER(C.c_int)]),

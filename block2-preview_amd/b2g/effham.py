@@ -137,11 +137,19 @@ class EffectiveHamiltonian:
         L.check(self.lib.b2g_plan_set_timing(self.plan, 1 if enable else 0))
 
     def timing(self):
-        """(ms phase 1, ms phase 2, launches) recorded since the last call."""
-        a, b, n = C.c_double(), C.c_double(), C.c_uint64()
-        L.check(self.lib.b2g_plan_timing(self.plan, C.byref(a), C.byref(b),
-                                         C.byref(n)))
-        return a.value, b.value, n.value
+        """Per kernel id (b2g.h, L.kernel_name): (summed ms, launches)
+        recorded since the last call, as two lists of L.NKERNELS."""
+        ms = (C.c_double * L.NKERNELS)()
+        n = (C.c_uint64 * L.NKERNELS)()
+        L.check(self.lib.b2g_plan_timing(self.plan, ms, n))
+        return list(ms), list(n)
+
+    def kernel_flops(self):
+        """Algorithmic FLOPs of one H*psi per right-hand side, per kernel
+        id (list of L.NKERNELS)."""
+        f = (C.c_double * L.NKERNELS)()
+        L.check(self.lib.b2g_plan_kernel_flops(self.plan, f))
+        return list(f)
 
     def pairs(self):
         """Recorded pair list as a numpy structured array."""

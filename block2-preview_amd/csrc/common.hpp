@@ -37,6 +37,11 @@ struct b2g_ctx_s {
     int device = 0;
     int num_cu = 0;
     hipStream_t stream = nullptr;
+    // auxiliary streams (default priority) for the independent launches of
+    // one H*psi phase, forked from and joined back to `stream` with events
+    std::vector<hipStream_t> aux;
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> join;
     // reusable device scratch for reductions (deterministic two-stage)
     double *red = nullptr;
     size_t red_len = 0;

@@ -541,7 +541,8 @@ extern "C" int b2g_effham_eigs(b2g_effham h, double *x, int k,
                                double *tdav) {
     B2G_REQUIRE(h && x && energies, B2G_ERR_ARG, "NULL arg");
     B2G_REQUIRE(h->plan, B2G_ERR_STATE, "effham was built host-only");
-    B2G_REQUIRE(precision == B2G_PREC_F64 || precision == B2G_PREC_F32,
+    B2G_REQUIRE(precision == B2G_PREC_F64 || precision == B2G_PREC_F32 ||
+                    precision == B2G_PREC_BF16X3,
                 B2G_ERR_ARG, "unsupported precision");
     auto t0 = std::chrono::steady_clock::now();
     std::vector<double> ev;

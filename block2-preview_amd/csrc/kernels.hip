@@ -107,7 +107,7 @@ struct TileLoader {
 };
 
 template <typename TA, typename TB, typename TD, typename TC, int TM, int TN,
-          int TK>
+          int TK, int PHASE>
 __global__ __launch_bounds__(256) void seg_gemm_kernel(
     const TA *__restrict__ A, const TB *__restrict__ B, TD *__restrict__ D,
     TD *__restrict__ P, const Task *__restrict__ tasks, int ntasks,
@@ -276,7 +276,12 @@ template hipError_t launch_splitk_reduce<float>(float *, const float *,
 int tile_m(int cfg) { return 32 << (cfg / 3); }
 int tile_n(int cfg) { return 32 << (cfg % 3); }
 
-template <typename TA, typename TB, typename TD, typename TC>
+// persistent grid: min(grid, ntasks) workgroups
+unsigned task_grid(int grid, int ntasks) {
+    return (unsigned)(grid < ntasks ? grid : ntasks);
+}
+
+template <typename TA, typename TB, typename TD, typename TC, int PHASE>
 hipError_t launch_seg_gemm(int cfg, const TA *A, const TB *B, TD *D, TD *P,
                            const Task *tasks, int ntasks, const Seg *segs,
                            uint64_t a_rs, uint64_t b_rs, uint64_t d_rs,
@@ -284,9 +289,10 @@ hipError_t launch_seg_gemm(int cfg, const TA *A, const TB *B, TD *D, TD *P,
                            int grid, hipStream_t st) {
     if (ntasks <= 0)
         return hipSuccess;
-    dim3 g((unsigned)(grid < ntasks ? grid : ntasks), (unsigned)n_rhs);
+    dim3 g(task_grid(grid, ntasks), (unsigned)n_rhs);
 #define B2G_LAUNCH(TM_, TN_, TK_)                                              \
-    hipLaunchKernelGGL((seg_gemm_kernel<TA, TB, TD, TC, TM_, TN_, TK_>), g,    \
+    hipLaunchKernelGGL((seg_gemm_kernel<TA, TB, TD, TC, TM_, TN_, TK_, PHASE>), \
+                       g,                                                      \
                        dim3(256), 0, st, A, B, D, P, tasks, ntasks, segs,      \
                        a_rs, b_rs, d_rs, p_rs, scale, beta0)
     switch (cfg) {
@@ -304,15 +310,19 @@ hipError_t launch_seg_gemm(int cfg, const TA *A, const TB *B, TD *D, TD *P,
     return hipGetLastError();
 }
 
-template hipError_t launch_seg_gemm<double, double, double, double>(
+template hipError_t launch_seg_gemm<double, double, double, double, 1>(
     int, const double *, const double *, double *, double *, const Task *, int, const Seg *,
     uint64_t, uint64_t, uint64_t, uint64_t, double, int, int, int,
     hipStream_t);
-template hipError_t launch_seg_gemm<double, float, float, float>(
+template hipError_t launch_seg_gemm<double, double, double, double, 2>(
+    int, const double *, const double *, double *, double *, const Task *, int, const Seg *,
+    uint64_t, uint64_t, uint64_t, uint64_t, double, int, int, int,
+    hipStream_t);
+template hipError_t launch_seg_gemm<double, float, float, float, 1>(
     int, const double *, const float *, float *, float *, const Task *, int, const Seg *,
     uint64_t, uint64_t, uint64_t, uint64_t, double, int, int, int,
     hipStream_t);
-template hipError_t launch_seg_gemm<float, float, double, float>(
+template hipError_t launch_seg_gemm<float, float, double, float, 2>(
     int, const float *, const float *, double *, double *, const Task *, int, const Seg *,
     uint64_t, uint64_t, uint64_t, uint64_t, double, int, int, int,
     hipStream_t);

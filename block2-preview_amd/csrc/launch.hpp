@@ -12,7 +12,12 @@ int tile_m(int cfg);
 int tile_n(int cfg);
 constexpr int kTileConfigs = 9;
 
-template <typename TA, typename TB, typename TD, typename TC>
+// persistent grid size of a grouped-GEMM launch
+unsigned task_grid(int grid, int ntasks);
+
+// PHASE (1: work = C op(B), 2: V += alpha op(A) work) only tags the kernel
+// name, so that profiles tell the phases apart
+template <typename TA, typename TB, typename TD, typename TC, int PHASE>
 hipError_t launch_seg_gemm(int cfg, const TA *A, const TB *B, TD *D, TD *P,
                            const Task *tasks, int ntasks, const Seg *segs,
                            uint64_t a_rs, uint64_t b_rs, uint64_t d_rs,
@@ -23,6 +28,15 @@ hipError_t launch_splitk_reduce(TD *D, const TD *P, const RedTask *tasks,
                                 int ntasks, uint64_t d_rs, uint64_t p_rs,
                                 int beta0, int n_rhs, int grid,
                                 hipStream_t st);
+
+// split-BF16 three-term variant (gemm_bf16x3.hip), FP32 accumulation
+template <typename TA, typename TB, typename TD>
+hipError_t launch_seg_gemm_bf3(int cfg, const TA *A, const TB *B, TD *D,
+                               TD *P, const Task *tasks, int ntasks,
+                               const Seg *segs, uint64_t a_rs, uint64_t b_rs,
+                               uint64_t d_rs, uint64_t p_rs, double scale,
+                               int beta0, int n_rhs, int grid,
+                               hipStream_t st);
 
 hipError_t launch_diag(const double *L, const double *R, double *Dv,
                        const DiagBlock *blocks, int nblocks,

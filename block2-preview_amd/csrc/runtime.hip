@@ -50,6 +50,35 @@ extern "C" int b2g_ctx_create(int device, unsigned flags, b2g_ctx *out) {
         set_error("hipStreamCreate failed");
         return B2G_ERR_DEVICE;
     }
+    // B2G_STREAMS: streams per H*psi phase, default 1 (serial launches)
+    int nstreams = 1;
+    if (const char *e = getenv("B2G_STREAMS"))
+        nstreams = std::max(1, std::min(4, atoi(e)));
+    bool ok = hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) ==
+              hipSuccess;
+    for (int i = 1; ok && i < nstreams; i++) {
+        hipStream_t s = nullptr;
+        hipEvent_t e = nullptr;
+        ok = hipStreamCreateWithFlags(&s, hipStreamNonBlocking) == hipSuccess;
+        if (ok)
+            c->aux.push_back(s);
+        ok = ok && hipEventCreateWithFlags(&e, hipEventDisableTiming) ==
+                       hipSuccess;
+        if (ok)
+            c->join.push_back(e);
+    }
+    if (!ok) {
+        for (hipStream_t s : c->aux)
+            hipStreamDestroy(s);
+        for (hipEvent_t e : c->join)
+            hipEventDestroy(e);
+        if (c->fork)
+            hipEventDestroy(c->fork);
+        hipStreamDestroy(c->stream);
+        delete c;
+        set_error("creation of auxiliary streams failed");
+        return B2G_ERR_DEVICE;
+    }
     // reduction partials: red_grid() blocks x up to 64*64 values, plus room
     c->red_len = (size_t)red_grid() * 64 * 64 + 8192;
     if (hipMalloc(&c->red, c->red_len * sizeof(double)) != hipSuccess ||
@@ -68,6 +97,13 @@ extern "C" int b2g_ctx_destroy(b2g_ctx ctx) {
         return B2G_OK;
     hipSetDevice(ctx->device);
     hipStreamSynchronize(ctx->stream);
+    for (hipStream_t s : ctx->aux) {
+        hipStreamSynchronize(s);
+        hipStreamDestroy(s);
+    }
+    for (hipEvent_t e : ctx->join)
+        hipEventDestroy(e);
+    hipEventDestroy(ctx->fork);
     hipFree(ctx->red);
     hipHostFree(ctx->red_host);
     hipStreamDestroy(ctx->stream);
@@ -183,6 +219,8 @@ struct Chunk {
     Task *d_t1[kTileConfigs] = {}, *d_t2[kTileConfigs] = {};
     Seg *d_s1 = nullptr, *d_s2 = nullptr;
     RedTask *d_red = nullptr;
+    // algorithmic FLOPs per RHS of each tile configuration's launch
+    double f1[kTileConfigs] = {}, f2[kTileConfigs] = {};
 };
 
 } // namespace b2g
@@ -202,10 +240,16 @@ struct b2g_plan_s {
     void *pbuf = nullptr; // split-K partial tiles (FP64)
     size_t p_bytes = 0;
     unsigned flags = 0;
-    // optional per-phase timing: event triples (before phase 1, between the
-    // phases, after phase 2) resolved when queried
+    // optional per-launch timing: (kernel id, start, stop) event records
+    // resolved when queried; kernel ids as in b2g.h (B2G_NKERNELS)
     bool timing = false;
-    std::vector<hipEvent_t> events;
+    struct Stamp {
+        int kernel;
+        hipEvent_t start, stop;
+    };
+    std::vector<Stamp> stamps;
+    // algorithmic FLOPs of one H*psi per right-hand side, per kernel id
+    double kflops[B2G_NKERNELS] = {};
 };
 
 namespace b2g {
@@ -391,6 +435,23 @@ static void add_split_tiles(Chunk &ch, uint64_t d_off, uint32_t M, uint32_t N,
     }
 }
 
+// multiply-adds x 2 of the tasks restricted to their tiles (the edge tiles'
+// masked rows and columns do not count)
+static double task_flops(const std::vector<Task> &tasks,
+                         const std::vector<Seg> &segs, int cfg) {
+    double f = 0;
+    const uint32_t tm = 32u << (cfg / 3), tn = 32u << (cfg % 3);
+    for (const Task &t : tasks) {
+        uint64_t k = 0;
+        for (uint32_t s = t.seg_begin; s < t.seg_end; s++)
+            k += segs[s].K;
+        const uint32_t m = std::min(tm, t.M - t.row0);
+        const uint32_t n = std::min(tn, t.N - t.col0);
+        f += 2.0 * m * n * (double)k;
+    }
+    return f;
+}
+
 static void sort_tasks(std::vector<Task> &tasks, const std::vector<Seg> &segs) {
     std::stable_sort(tasks.begin(), tasks.end(),
                      [&segs](const Task &a, const Task &b) {
@@ -405,7 +466,7 @@ static uint64_t w_cap_elems() {
         if (v > 0)
             return (uint64_t)v;
     }
-    return (uint64_t)1 << 28; // 256 Mi elements per right-hand side
+    return (uint64_t)1 << 30; // 1 Gi elements (8 GiB FP64) per right-hand side
 }
 
 static int compile_plan(b2g_plan p, const b2g_pair *pairs, size_t n_pairs,
@@ -515,6 +576,10 @@ static int compile_plan(b2g_plan p, const b2g_pair *pairs, size_t n_pairs,
         for (int c = 0; c < kTileConfigs; c++) {
             sort_tasks(ch.t1[c], ch.s1);
             sort_tasks(ch.t2[c], ch.s2);
+            ch.f1[c] = task_flops(ch.t1[c], ch.s1, c);
+            ch.f2[c] = task_flops(ch.t2[c], ch.s2, c);
+            p->kflops[B2G_K_PHASE1 + c] += ch.f1[c];
+            p->kflops[B2G_K_PHASE2 + c] += ch.f2[c];
         }
         p->w_max = std::max(p->w_max, ch.w_len);
         p->p_max = std::max(p->p_max, ch.p_len);
@@ -596,7 +661,7 @@ extern "C" int b2g_upload_plan(b2g_ctx ctx, b2g_buf lbuf, b2g_buf rbuf,
     p->y_len = y_len;
     p->flags = flags;
     int r = compile_plan(p, pairs, n_pairs, yblocks, n_yblocks);
-    if (r == B2G_OK && (flags & 1)) {
+    if (r == B2G_OK && (flags & 3)) {
         if (hipMalloc(&p->l32, std::max<size_t>(lbuf->n, 1) * 4) !=
                 hipSuccess ||
             hipMalloc(&p->r32, std::max<size_t>(rbuf->n, 1) * 4) !=
@@ -625,8 +690,10 @@ extern "C" int b2g_plan_destroy(b2g_plan p) {
         return B2G_OK;
     hipSetDevice(p->ctx->device);
     hipStreamSynchronize(p->ctx->stream);
-    for (hipEvent_t e : p->events)
-        hipEventDestroy(e);
+    for (auto &st : p->stamps) {
+        hipEventDestroy(st.start);
+        hipEventDestroy(st.stop);
+    }
     if (p->dev)
         hipFree(p->dev);
     if (p->l32)
@@ -657,27 +724,36 @@ extern "C" int b2g_plan_set_timing(b2g_plan p, int enable) {
     return B2G_OK;
 }
 
-extern "C" int b2g_plan_timing(b2g_plan p, double *ms1, double *ms2,
-                               uint64_t *n_launch) {
-    B2G_REQUIRE(p, B2G_ERR_ARG, "NULL plan");
-    double a = 0, b = 0;
-    uint64_t n = 0;
-    for (size_t i = 0; i + 2 < p->events.size(); i += 3) {
-        B2G_HIP(hipEventSynchronize(p->events[i + 2]));
-        float t1 = 0, t2 = 0;
-        B2G_HIP(hipEventElapsedTime(&t1, p->events[i], p->events[i + 1]));
-        B2G_HIP(hipEventElapsedTime(&t2, p->events[i + 1], p->events[i + 2]));
-        a += t1, b += t2, n++;
+extern "C" int b2g_plan_timing(b2g_plan p, double *ms, uint64_t *n_launch) {
+    B2G_REQUIRE(p && ms && n_launch, B2G_ERR_ARG, "NULL arg");
+    for (int k = 0; k < B2G_NKERNELS; k++)
+        ms[k] = 0, n_launch[k] = 0;
+    int rc = B2G_OK;
+    for (auto &st : p->stamps) {
+        float t = 0;
+        if (rc == B2G_OK) {
+            hipError_t e = hipEventSynchronize(st.stop);
+            if (e == hipSuccess)
+                e = hipEventElapsedTime(&t, st.start, st.stop);
+            if (e != hipSuccess) {
+                b2g::set_error(std::string("b2g_plan_timing: ") +
+                               hipGetErrorString(e));
+                rc = B2G_ERR_DEVICE;
+            }
+        }
+        ms[st.kernel] += t;
+        n_launch[st.kernel]++;
+        hipEventDestroy(st.start);
+        hipEventDestroy(st.stop);
     }
-    for (hipEvent_t e : p->events)
-        B2G_HIP(hipEventDestroy(e));
-    p->events.clear();
-    if (ms1)
-        *ms1 = a;
-    if (ms2)
-        *ms2 = b;
-    if (n_launch)
-        *n_launch = n;
+    p->stamps.clear();
+    return rc;
+}
+
+extern "C" int b2g_plan_kernel_flops(b2g_plan p, double *flops) {
+    B2G_REQUIRE(p && flops, B2G_ERR_ARG, "NULL arg");
+    for (int k = 0; k < B2G_NKERNELS; k++)
+        flops[k] = p->kflops[k];
     return B2G_OK;
 }
 
@@ -708,10 +784,12 @@ extern "C" int b2g_hpsi(b2g_ctx ctx, b2g_plan p, const double *x, double *y,
                         int precision) {
     B2G_REQUIRE(ctx && p && x && y, B2G_ERR_ARG, "NULL arg");
     B2G_REQUIRE(n_rhs >= 1 && n_rhs <= 65535, B2G_ERR_ARG, "bad n_rhs");
-    B2G_REQUIRE(precision == B2G_PREC_F64 || precision == B2G_PREC_F32,
-                B2G_ERR_ARG, "unsupported precision for this plan");
-    B2G_REQUIRE(precision != B2G_PREC_F32 || p->l32 != nullptr, B2G_ERR_STATE,
-                "plan was uploaded without FP32 shadows");
+    B2G_REQUIRE(precision == B2G_PREC_F64 || precision == B2G_PREC_F32 ||
+                    precision == B2G_PREC_BF16X3,
+                B2G_ERR_ARG, "unsupported precision");
+    B2G_REQUIRE(precision == B2G_PREC_F64 || p->l32 != nullptr, B2G_ERR_STATE,
+                "plan was uploaded without reduced-precision shadows "
+                "(flags bit 0 or 1)");
     B2G_HIP(hipSetDevice(ctx->device));
     const size_t es = precision == B2G_PREC_F64 ? 8 : 4;
     const size_t need = std::max<uint64_t>(p->w_max, 1) * n_rhs * es;
@@ -738,57 +816,111 @@ extern "C" int b2g_hpsi(b2g_ctx ctx, b2g_plan p, const double *x, double *y,
     }
     double *P = (double *)p->pbuf;
     const int grid = ctx->num_cu * 8;
+    // brackets one launch on stream `st` with timing events when enabled
+    auto timed = [&](int kernel, hipStream_t st,
+                     auto &&launch) -> hipError_t {
+        if (!p->timing)
+            return launch(st);
+        b2g_plan_s::Stamp sp{kernel, nullptr, nullptr};
+        hipError_t e = hipEventCreate(&sp.start);
+        if (e == hipSuccess)
+            e = hipEventCreate(&sp.stop);
+        if (e == hipSuccess)
+            e = hipEventRecord(sp.start, st);
+        if (e == hipSuccess)
+            e = launch(st);
+        if (e == hipSuccess)
+            e = hipEventRecord(sp.stop, st);
+        if (e == hipSuccess)
+            p->stamps.push_back(sp);
+        else {
+            if (sp.start)
+                hipEventDestroy(sp.start);
+            if (sp.stop)
+                hipEventDestroy(sp.stop);
+        }
+        return e;
+    };
+    // One phase: a launch per tile configuration with tasks.  The tasks of
+    // a phase write disjoint tiles, so the launches are independent: the
+    // largest runs on the context stream, the others on the auxiliary
+    // streams (forked from and joined back to the context stream), where
+    // they fill the large launch's idle compute units.
+    auto phase = [&](const std::vector<Task> *tl, const double *fl, int kbase,
+                     auto &&launch) -> hipError_t {
+        int order[kTileConfigs], n = 0;
+        for (int c = 0; c < kTileConfigs; c++)
+            if (!tl[c].empty())
+                order[n++] = c;
+        std::sort(order, order + n,
+                  [fl](int a, int b) { return fl[a] > fl[b]; });
+        const int naux = std::min<int>((int)ctx->aux.size(), n - 1);
+        hipError_t e = hipSuccess;
+        if (naux > 0) {
+            e = hipEventRecord(ctx->fork, ctx->stream);
+            for (int i = 0; e == hipSuccess && i < naux; i++)
+                e = hipStreamWaitEvent(ctx->aux[i], ctx->fork, 0);
+        }
+        for (int i = 0; e == hipSuccess && i < n; i++) {
+            const int c = order[i];
+            hipStream_t st = (i == 0 || naux == 0)
+                                 ? ctx->stream
+                                 : ctx->aux[(i - 1) % naux];
+            e = timed(kbase + c, st,
+                      [&](hipStream_t s) { return launch(c, s); });
+        }
+        for (int i = 0; e == hipSuccess && i < naux; i++) {
+            e = hipEventRecord(ctx->join[i], ctx->aux[i]);
+            if (e == hipSuccess)
+                e = hipStreamWaitEvent(ctx->stream, ctx->join[i], 0);
+        }
+        return e;
+    };
     for (size_t ic = 0; ic < p->chunks.size(); ic++) {
         const Chunk &ch = p->chunks[ic];
         // chunk 0 overwrites unless accumulating; later chunks' tasks add
         const int beta0 = ic == 0 ? (accumulate ? 1 : 0) : 1;
-        hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
-        if (p->timing) {
-            for (int k = 0; k < 3; k++) {
-                B2G_HIP(hipEventCreate(&ev[k]));
-                p->events.push_back(ev[k]);
-            }
-        }
-        auto mark = [&](int k) -> hipError_t {
-            return p->timing ? hipEventRecord(ev[k], ctx->stream) : hipSuccess;
-        };
-        B2G_HIP(mark(0));
-        // phase 1 then phase 2, one launch per tile configuration present
-        if (precision == B2G_PREC_F64) {
-            double *W = (double *)p->w;
-            for (int c = 0; c < kTileConfigs; c++)
-                B2G_HIP((launch_seg_gemm<double, double, double, double>(
-                    c, x, (const double *)p->rbuf->ptr, W, W, ch.d_t1[c],
-                    (int)ch.t1[c].size(), ch.d_s1, p->x_len, 0, p->w_max, 0,
-                    1.0, 0, n_rhs, grid, ctx->stream)));
-            B2G_HIP(mark(1));
-            for (int c = 0; c < kTileConfigs; c++)
-                B2G_HIP((launch_seg_gemm<double, double, double, double>(
-                    c, (const double *)p->lbuf->ptr, W, y, P, ch.d_t2[c],
-                    (int)ch.t2[c].size(), ch.d_s2, 0, p->w_max, p->y_len,
-                    p->p_max, scale, beta0, n_rhs, grid, ctx->stream)));
-            B2G_HIP(launch_splitk_reduce<double>(
-                y, P, ch.d_red, (int)ch.red.size(), p->y_len, p->p_max, beta0,
-                n_rhs, grid, ctx->stream));
-            B2G_HIP(mark(2));
-        } else {
-            float *W = (float *)p->w;
-            for (int c = 0; c < kTileConfigs; c++)
-                B2G_HIP((launch_seg_gemm<double, float, float, float>(
-                    c, x, p->r32, W, W, ch.d_t1[c], (int)ch.t1[c].size(),
+        // phase 1, phase 2, then the split-K reduction
+        B2G_HIP(phase(ch.t1, ch.f1, B2G_K_PHASE1, [&](int c, hipStream_t st) {
+            const int nt = (int)ch.t1[c].size();
+            if (precision == B2G_PREC_F64)
+                return launch_seg_gemm<double, double, double, double, 1>(
+                    c, x, (const double *)p->rbuf->ptr, (double *)p->w,
+                    (double *)p->w, ch.d_t1[c], nt, ch.d_s1, p->x_len, 0,
+                    p->w_max, 0, 1.0, 0, n_rhs, grid, st);
+            if (precision == B2G_PREC_BF16X3)
+                return launch_seg_gemm_bf3<double, float, float>(
+                    c, x, p->r32, (float *)p->w, (float *)p->w, ch.d_t1[c], nt,
                     ch.d_s1, p->x_len, 0, p->w_max, 0, 1.0, 0, n_rhs, grid,
-                    ctx->stream)));
-            B2G_HIP(mark(1));
-            for (int c = 0; c < kTileConfigs; c++)
-                B2G_HIP((launch_seg_gemm<float, float, double, float>(
-                    c, p->l32, W, y, P, ch.d_t2[c], (int)ch.t2[c].size(),
+                    st);
+            return launch_seg_gemm<double, float, float, float, 1>(
+                c, x, p->r32, (float *)p->w, (float *)p->w, ch.d_t1[c], nt,
+                ch.d_s1, p->x_len, 0, p->w_max, 0, 1.0, 0, n_rhs, grid, st);
+        }));
+        B2G_HIP(phase(ch.t2, ch.f2, B2G_K_PHASE2, [&](int c, hipStream_t st) {
+            const int nt = (int)ch.t2[c].size();
+            if (precision == B2G_PREC_F64)
+                return launch_seg_gemm<double, double, double, double, 2>(
+                    c, (const double *)p->lbuf->ptr, (const double *)p->w, y,
+                    P, ch.d_t2[c], nt, ch.d_s2, 0, p->w_max, p->y_len,
+                    p->p_max, scale, beta0, n_rhs, grid, st);
+            if (precision == B2G_PREC_BF16X3)
+                return launch_seg_gemm_bf3<float, float, double>(
+                    c, p->l32, (const float *)p->w, y, P, ch.d_t2[c], nt,
                     ch.d_s2, 0, p->w_max, p->y_len, p->p_max, scale, beta0,
-                    n_rhs, grid, ctx->stream)));
-            B2G_HIP(launch_splitk_reduce<double>(
-                y, P, ch.d_red, (int)ch.red.size(), p->y_len, p->p_max, beta0,
-                n_rhs, grid, ctx->stream));
-            B2G_HIP(mark(2));
-        }
+                    n_rhs, grid, st);
+            return launch_seg_gemm<float, float, double, float, 2>(
+                c, p->l32, (const float *)p->w, y, P, ch.d_t2[c], nt,
+                ch.d_s2, 0, p->w_max, p->y_len, p->p_max, scale, beta0, n_rhs,
+                grid, st);
+        }));
+        const int nr = (int)ch.red.size();
+        if (nr > 0)
+            B2G_HIP(timed(B2G_K_SPLITK_REDUCE, ctx->stream, [&](hipStream_t st) {
+                return launch_splitk_reduce<double>(y, P, ch.d_red, nr,
+                                                    p->y_len, p->p_max, beta0,
+                                                    n_rhs, grid, st);
+            }));
     }
     return B2G_OK;
 }

@@ -132,8 +132,9 @@ typedef struct {
 
 /* Compile and upload a replay plan.  `lbuf`/`rbuf` hold the operator blocks
  * (FP64); x_len / y_len are the input/output vector lengths.  `flags` bit 0
- * also builds FP32 shadows of the operator buffers for B2G_PREC_F32, bit 1
- * BF16 hi/lo shadows for B2G_PREC_BF16X3.                                 */
+ * or bit 1 also builds FP32 shadows of the operator buffers, which
+ * B2G_PREC_F32 and B2G_PREC_BF16X3 read (the split-BF16 kernels split each
+ * FP32 value into BF16 hi + lo when staging it into LDS).                 */
 int b2g_upload_plan(b2g_ctx ctx, b2g_buf lbuf, b2g_buf rbuf,
                     const b2g_pair *pairs, size_t n_pairs,
                     const b2g_block *yblocks, size_t n_yblocks, size_t x_len,
@@ -143,14 +144,26 @@ int b2g_plan_destroy(b2g_plan plan);
  * (sum over pairs of c_m*c_n*v_n + v_m*c_m*v_n, the reference's nflop,
  * src/core/batch_gemm.hpp:307).  FLOPs = 2 * this.                        */
 int b2g_plan_macs(b2g_plan plan, uint64_t *out);
-/* Per-phase kernel timing with HIP events on the context stream: while
- * enabled, every b2g_hpsi records events around its two launches per
- * chunk; b2g_plan_timing waits for them and returns the summed durations
- * (ms) of phase 1 (work = C op(B)) and phase 2 (V += alpha op(A) work) and
- * the number of chunk launches, then clears the record.                  */
+/* Kernel ids of one H*psi for timing and accounting: the grouped GEMM of
+ * phase 1 (work = C op(B)) in tile configuration c is B2G_K_PHASE1 + c,
+ * phase 2 (V += alpha op(A) work) is B2G_K_PHASE2 + c (c = 3*i + j,
+ * TM = 32 << i, TN = 32 << j), and the split-K reduction B2G_K_SPLITK_REDUCE. */
+#define B2G_K_PHASE1 0
+#define B2G_K_PHASE2 9
+#define B2G_K_SPLITK_REDUCE 18
+#define B2G_NKERNELS 19
+/* Per-launch kernel timing with HIP events on the context stream: while
+ * enabled, every b2g_hpsi brackets each kernel launch with two events;
+ * b2g_plan_timing waits for them, writes per kernel id the summed duration
+ * (ms) and the number of launches into ms[B2G_NKERNELS] and
+ * n_launch[B2G_NKERNELS], then clears the record.                        */
 int b2g_plan_set_timing(b2g_plan plan, int enable);
-int b2g_plan_timing(b2g_plan plan, double *ms_phase1, double *ms_phase2,
-                    uint64_t *n_launch);
+int b2g_plan_timing(b2g_plan plan, double *ms, uint64_t *n_launch);
+/* Algorithmic FLOPs of one H*psi per right-hand side, per kernel id: 2 x the
+ * multiply-adds of the kernel's tiles (phase 2 sums to exactly the
+ * reference's second-GEMM count; phase 1 computes each distinct work matrix
+ * once).  flops[B2G_NKERNELS].                                            */
+int b2g_plan_kernel_flops(b2g_plan plan, double *flops);
 /* Relative throughput (any unit) of the 9 workgroup tile shapes
  * TM x TN, TM, TN in {32, 64, 128}, index 3*i + j with TM = 32 << i,
  * TN = 32 << j.  The tiling of plans uploaded afterwards minimises the

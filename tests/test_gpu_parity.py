@@ -1,7 +1,10 @@
 """GPU parity: libb2g.so on an MI355X against the reference's golden vectors
 and the oracle.  Tolerances: FP64 H*psi / diagonal 1e-12 relative (2-norm),
-Davidson energy 1e-8 Eh; FP32 H*psi 2e-6 relative; mixed-precision Davidson
-with FP64 refinement 1e-8 Eh."""
+Davidson energy 1e-8 Eh; FP32 H*psi 2e-6 relative; split-BF16 three-term
+H*psi 1e-4 relative (each product carries about 3 * 2^-18 relative error from
+the dropped lo*lo term and the hi/lo representation, FP32 accumulation);
+mixed-precision Davidson (FP32 or split-BF16 matvecs) with FP64 refinement
+1e-8 Eh."""
 import numpy as np
 import pytest
 import torch
@@ -67,6 +70,15 @@ def test_hpsi_fp32(heff):
     assert rel(y, g["hpsi"]) < 2e-6, name
 
 
+def test_hpsi_bf16x3(heff):
+    name, pb, h = heff
+    g = pb.golden
+    y = h.multiply(dev(g["psi"]), precision="bf16x3").cpu().numpy()
+    err = rel(y, g["hpsi"])
+    print(name, "bf16x3 rel err %.3g" % err)
+    assert err < 1e-4, name
+
+
 def test_davidson_fp64_matches_reference(heff):
     name, pb, h = heff
     x = dev(pb.golden["psi"])
@@ -84,5 +96,13 @@ def test_davidson_mixed_precision_with_refinement(heff):
     name, pb, h = heff
     x = dev(pb.golden["psi"])
     e, nd, nflop, td = h.eigs(x, conv_thrd=1e-10, precision="f32",
+                              refine=True)
+    assert abs(e - float(pb.meta["dav_e"])) < 1e-8, (name, e)
+
+
+def test_davidson_bf16x3_with_refinement(heff):
+    name, pb, h = heff
+    x = dev(pb.golden["psi"])
+    e, nd, nflop, td = h.eigs(x, conv_thrd=1e-10, precision="bf16x3",
                               refine=True)
     assert abs(e - float(pb.meta["dav_e"])) < 1e-8, (name, e)

@@ -1,9 +1,10 @@
 """GPU H*psi on seeded synthetic middle-site problems against the oracle:
 every tile configuration of the grouped GEMM (32x32, 64x64, 128x128 forced
 through B2G_TILE, and the automatic choice), the scratch split into several
-chunks, split-K partial tiles (B2G_KSPLIT), 4 right-hand sides, FP32.  Blocks range from 1x1 up to 200x200, so
+chunks, split-K partial tiles (B2G_KSPLIT), 4 right-hand sides, FP32 and
+split-BF16 three-term (every tile shape).  Blocks range from 1x1 up to 200x200, so
 edge tiles, K remainders and single-element blocks all occur.
-Tolerances: FP64 1e-12 relative (2-norm), FP32 2e-6."""
+Tolerances: FP64 1e-12 relative (2-norm), FP32 2e-6, split-BF16 1e-4."""
 import os
 
 import numpy as np
@@ -48,7 +49,11 @@ def run(gpu_ctx, pb, xs, env, precision="f64"):
     return y
 
 
-@pytest.mark.parametrize("tile", ["32", "64", "128", "32x128", "128x32", "64x128", "128x64", "auto"])
+TILES = ["32", "64", "128", "32x64", "64x32", "32x128", "128x32", "64x128",
+         "128x64", "auto"]
+
+
+@pytest.mark.parametrize("tile", TILES)
 def test_tile_configurations(gpu_ctx, problem, tile):
     pb, xs, ys = problem
     print("tile", tile, flush=True)
@@ -80,6 +85,17 @@ def test_split_k(gpu_ctx, problem, ksplit):
     y = run(gpu_ctx, pb, xs, {"B2G_KSPLIT": ksplit}, precision="f32")
     for r in range(4):
         assert rel(y[r], ys[r]) < 2e-6, r
+
+
+@pytest.mark.parametrize("tile", TILES)
+def test_bf16x3_tile_configurations(gpu_ctx, problem, tile):
+    # split-BF16 three-term kernels, every tile shape; 1e-4 relative
+    pb, xs, ys = problem
+    print("bf16x3 tile", tile, flush=True)
+    env = {} if tile == "auto" else {"B2G_TILE": tile}
+    y = run(gpu_ctx, pb, xs, env, precision="bf16x3")
+    for r in range(4):
+        assert rel(y[r], ys[r]) < 1e-4, (tile, r, rel(y[r], ys[r]))
 
 
 def test_fp32(gpu_ctx, problem):
