@@ -137,6 +137,16 @@ _SIGS = {
     "b2g_effham_hpsi": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_double,
                                   C.c_int, C.c_int]),
     "b2g_effham_diag": (C.c_int, [_vp, _vp]),
+    "b2g_effham_set_data": (C.c_int, [_vp, _vp, _vp]),
+    "b2g_effham_data": (C.c_int, [_vp, C.POINTER(_vp), C.POINTER(C.c_size_t),
+                                  C.POINTER(_vp), C.POINTER(C.c_size_t)]),
+    "b2g_effham_grad_ops": (C.c_int, [_vp, _vp, _vp, C.c_int, _vp, _vp,
+                                      C.c_int]),
+    "b2g_grad_ops": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, _vp, _vp,
+                               C.c_int]),
+    "b2g_grad_flops": (C.c_int, [_vp, f64p]),
+    "b2g_plan_refresh": (C.c_int, [_vp]),
+    "b2g_effham_set_davidson_mode": (C.c_int, [_vp, C.c_int]),
     "b2g_effham_eigs": (C.c_int, [_vp, _vp, C.c_int, C.c_double, C.c_int,
                                   C.c_int, C.c_int, C.c_int, C.c_int, f64p,
                                   C.POINTER(C.c_int), u64p, f64p]),

@@ -201,6 +201,68 @@ class EffectiveHamiltonian:
         """Host copy of the connection list (which 0: wfn, 1: diag)."""
         return connections(self.h, which)
 
+    def set_davidson_mode(self, mode):
+        """1: graph-captured device Davidson iteration (default, one root);
+        0: host-driven loop."""
+        L.check(self.lib.b2g_effham_set_davidson_mode(self.h, int(mode)))
+
+    # ------------------------------------------------------------ backward
+    def operator_data(self):
+        """Copies of the left and right operator buffers (float64 device
+        tensors; element offsets as in the problem's operator sets)."""
+        lp, rp = C.c_void_p(), C.c_void_p()
+        ln, rn = C.c_size_t(), C.c_size_t()
+        L.check(self.lib.b2g_effham_data(self.h, C.byref(lp), C.byref(ln),
+                                         C.byref(rp), C.byref(rn)))
+        lt = torch.empty(ln.value, dtype=torch.float64, device=self.ctx.device)
+        rt = torch.empty(rn.value, dtype=torch.float64, device=self.ctx.device)
+        torch.cuda.current_stream(self.ctx.device).synchronize()
+        for src, dst in ((lp, lt), (rp, rt)):
+            if dst.numel():
+                L.check(self.lib.b2g_vec_copy(self.ctx.h, src, _dptr(dst),
+                                              dst.numel()))
+        self.ctx.synchronize()
+        return lt, rt
+
+    def set_operator_data(self, left, right):
+        """Replace the operator numbers (same structure): copies `left` and
+        `right` in, refreshes the FP32 shadows and the diagonal."""
+        lp, rp = C.c_void_p(), C.c_void_p()
+        ln, rn = C.c_size_t(), C.c_size_t()
+        L.check(self.lib.b2g_effham_data(self.h, C.byref(lp), C.byref(ln),
+                                         C.byref(rp), C.byref(rn)))
+        if left.numel() != ln.value or right.numel() != rn.value:
+            raise ValueError("operator buffers hold %d / %d values" % (
+                ln.value, rn.value))
+        torch.cuda.current_stream(self.ctx.device).synchronize()
+        L.check(self.lib.b2g_effham_set_data(self.h, _dptr(left),
+                                             _dptr(right)))
+        self._diag = None
+        self.ctx.synchronize()
+
+    def grad_ops(self, v, c):
+        """Partial expectations: (dE/dL, dE/dR) of E = sum_r <v_r|H|c_r>
+        with respect to every element of the left and right operator
+        buffers (operator_functions.hpp:354-472 contract the same)."""
+        nrhs = self._check(c, v)
+        lp, rp = C.c_void_p(), C.c_void_p()
+        ln, rn = C.c_size_t(), C.c_size_t()
+        L.check(self.lib.b2g_effham_data(self.h, C.byref(lp), C.byref(ln),
+                                         C.byref(rp), C.byref(rn)))
+        gl = torch.empty(ln.value, dtype=torch.float64, device=self.ctx.device)
+        gr = torch.empty(rn.value, dtype=torch.float64, device=self.ctx.device)
+        torch.cuda.current_stream(self.ctx.device).synchronize()
+        L.check(self.lib.b2g_effham_grad_ops(self.h, _dptr(v), _dptr(c), nrhs,
+                                             _dptr(gl), _dptr(gr), 0))
+        self.ctx.synchronize()
+        return gl, gr
+
+    def grad_flops(self):
+        """Algorithmic FLOPs of grad_ops per right-hand side."""
+        f = C.c_double()
+        L.check(self.lib.b2g_grad_flops(self.plan, C.byref(f)))
+        return f.value
+
     def deallocate(self):
         if self.h:
             self.lib.b2g_effham_destroy(self.h)
@@ -211,6 +273,34 @@ class EffectiveHamiltonian:
             self.deallocate()
         except Exception:
             pass
+
+
+class Energy(torch.autograd.Function):
+    """E(psi, L, R) = <psi| H(L, R) |psi> with a hand-written backward:
+    dE/dpsi = 2 H psi (H_eff is real symmetric in DMRG), dE/dL and dE/dR
+    from the partial expectations (EffectiveHamiltonian.grad_ops).
+    Usage: Energy.apply(psi, left_data, right_data, heff)."""
+
+    @staticmethod
+    def forward(ctx, psi, left, right, heff):
+        heff.set_operator_data(left.detach().contiguous(),
+                               right.detach().contiguous())
+        p = psi.detach().contiguous()
+        sigma = heff.multiply(p)
+        ctx.heff = heff
+        ctx.save_for_backward(p, sigma)
+        return torch.dot(p, sigma)
+
+    @staticmethod
+    def backward(ctx, g):
+        p, sigma = ctx.saved_tensors
+        gpsi = gl = gr = None
+        if ctx.needs_input_grad[0]:
+            gpsi = 2.0 * g * sigma
+        if ctx.needs_input_grad[1] or ctx.needs_input_grad[2]:
+            gl, gr = ctx.heff.grad_ops(p, p)
+            gl, gr = g * gl, g * gr
+        return gpsi, gl, gr, None
 
 
 def connections(h, which):

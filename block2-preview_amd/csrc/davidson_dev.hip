@@ -1,0 +1,476 @@
+// Device-resident Davidson iteration (one root), built to be captured once
+// into a hipGraph and replayed: every step reads the subspace size and the
+// Ritz data from a DavState in device memory, so one graph serves all
+// iterations and the host only reads a small status header per iteration.
+//
+// Steps follow IterativeMatrixFunctions::davidson
+// (src/core/iterative_matrix_functions.hpp:866-1173, DavidsonTypes::Normal):
+// sigma of the newest vector; subspace matrix alpha = <b_i|sigma_j>;
+// Ritz pairs; rotation of b and sigma; residual q = sigma_0 - lambda b_0 and
+// the squared-norm test; Olsen preconditioner (:93-108); restart to
+// def_min vectors at def_max; orthogonalisation of q against the subspace
+// and append.  The Ritz problem is solved on the device by cyclic Jacobi in
+// round-robin order (m/2 disjoint rotations at a time); the orthogonalisation
+// is classical Gram-Schmidt applied twice (as orthogonal as one modified
+// pass, with fixed launches).
+#include "kernels.hpp"
+#include "launch.hpp"
+
+namespace b2g {
+
+namespace {
+
+constexpr int kGrid = 512; // partial-sum blocks of the reductions
+
+__device__ __forceinline__ double bsum(double v, double *sh) {
+    const int tid = threadIdx.x;
+    sh[tid] = v;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (tid < s)
+            sh[tid] += sh[tid + s];
+        __syncthreads();
+    }
+    const double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+// S[m-1] = sn
+__global__ void put_sigma_kernel(DavState *st, double *__restrict__ S,
+                                 const double *__restrict__ sn, size_t n) {
+    const int m = st->m;
+    double *dst = S + (size_t)(m - 1) * n;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = sn[i];
+}
+
+// partial Gram sums of <B_i|S_j>, i, j < m, over row tiles of 32
+__global__ __launch_bounds__(256) void gram_kernel(
+    const DavState *st, const double *__restrict__ X,
+    const double *__restrict__ Y, size_t n, double *__restrict__ partial) {
+    constexpr int RT = 32;
+    __shared__ double xs[64][RT + 1];
+    __shared__ double ys[64][RT + 1];
+    const int m = st->m, npair = m * m;
+    double acc[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++)
+        acc[q] = 0.0;
+    for (size_t r0 = (size_t)blockIdx.x * RT; r0 < n;
+         r0 += (size_t)gridDim.x * RT) {
+        for (int e = threadIdx.x; e < m * RT; e += 256) {
+            const int i = e / RT, r = e % RT;
+            const bool ok = r0 + r < n;
+            xs[i][r] = ok ? X[(size_t)i * n + r0 + r] : 0.0;
+            ys[i][r] = ok ? Y[(size_t)i * n + r0 + r] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            const int p = threadIdx.x + q * 256;
+            if (p < npair) {
+                const int i = p / m, j = p % m;
+                double s = 0.0;
+                for (int r = 0; r < RT; r++)
+                    s += xs[i][r] * ys[j][r];
+                acc[q] += s;
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+        const int p = threadIdx.x + q * 256;
+        if (p < npair)
+            partial[(size_t)blockIdx.x * npair + p] = acc[q];
+    }
+}
+
+// G = sum of the Gram partials, symmetrised from its lower triangle
+__global__ __launch_bounds__(256) void gram_reduce_kernel(
+    DavState *st, const double *__restrict__ partial, int nblk) {
+    const int m = st->m, npair = m * m;
+    for (int p = threadIdx.x; p < npair; p += 256) {
+        double s = 0.0;
+        for (int b = 0; b < nblk; b++)
+            s += partial[(size_t)b * npair + p];
+        st->G[p] = s;
+    }
+    __syncthreads();
+    for (int p = threadIdx.x; p < npair; p += 256) {
+        const int i = p / m, j = p % m;
+        if (j > i)
+            st->G[p] = st->G[j * m + i];
+    }
+}
+
+// Ritz pairs of the m x m subspace matrix: cyclic Jacobi, round-robin pair
+// order, rotation angle and stopping rule as the host solver; eigenvalues
+// ascending into ld, eigenvector j into V[j*m + i].
+__global__ __launch_bounds__(256) void ritz_kernel(DavState *st) {
+    __shared__ double a[64][65];
+    __shared__ double z[64][65];
+    __shared__ int pp[32], qq[32];
+    __shared__ double cc[32], ss[32];
+    __shared__ double red[256];
+    __shared__ int order[64];
+    const int m = st->m, tid = threadIdx.x;
+    const int np = (m + 1) & ~1; // players (a dummy when m is odd)
+    for (int e = tid; e < 64 * 64; e += 256) {
+        const int i = e / 64, j = e % 64;
+        a[i][j] = (i < m && j < m) ? st->G[i * m + j] : 0.0;
+        z[i][j] = i == j ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    for (int sweep = 0; sweep < 100 && m > 1; sweep++) {
+        double off = 0.0, tot = 0.0;
+        for (int e = tid; e < m * m; e += 256) {
+            const int i = e / m, j = e % m;
+            const double v = a[i][j] * a[i][j];
+            tot += v;
+            if (i != j)
+                off += v;
+        }
+        off = bsum(off, red);
+        tot = bsum(tot, red);
+        if (off <= 1E-30 * tot || off == 0.0)
+            break;
+        for (int r = 0; r < np - 1; r++) {
+            if (tid < np / 2) {
+                int p, q;
+                if (tid == 0)
+                    p = np - 1, q = r;
+                else
+                    p = (r + tid) % (np - 1),
+                    q = (r - tid + (np - 1)) % (np - 1);
+                if (p > q) {
+                    const int t = p;
+                    p = q, q = t;
+                }
+                double c = 1.0, s = 0.0;
+                if (q < m) {
+                    const double apq = a[p][q];
+                    if (fabs(apq) >= 1E-300) {
+                        const double theta = (a[q][q] - a[p][p]) / (2.0 * apq);
+                        const double t = (theta >= 0 ? 1.0 : -1.0) /
+                                         (fabs(theta) + sqrt(theta * theta + 1.0));
+                        c = 1.0 / sqrt(t * t + 1.0);
+                        s = t * c;
+                    }
+                }
+                pp[tid] = p, qq[tid] = q, cc[tid] = c, ss[tid] = s;
+            }
+            __syncthreads();
+            // columns: A <- A J, Z <- Z J
+            for (int e = tid; e < (np / 2) * m; e += 256) {
+                const int k = e % m, w = e / m;
+                const int p = pp[w], q = qq[w];
+                if (q < m) {
+                    const double c = cc[w], s = ss[w];
+                    const double akp = a[k][p], akq = a[k][q];
+                    a[k][p] = c * akp - s * akq;
+                    a[k][q] = s * akp + c * akq;
+                    const double zkp = z[k][p], zkq = z[k][q];
+                    z[k][p] = c * zkp - s * zkq;
+                    z[k][q] = s * zkp + c * zkq;
+                }
+            }
+            __syncthreads();
+            // rows: A <- J^T A
+            for (int e = tid; e < (np / 2) * m; e += 256) {
+                const int k = e % m, w = e / m;
+                const int p = pp[w], q = qq[w];
+                if (q < m) {
+                    const double c = cc[w], s = ss[w];
+                    const double apk = a[p][k], aqk = a[q][k];
+                    a[p][k] = c * apk - s * aqk;
+                    a[q][k] = s * apk + c * aqk;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    if (tid == 0) {
+        // stable ascending order of the diagonal
+        for (int i = 0; i < m; i++)
+            order[i] = i;
+        for (int i = 1; i < m; i++) {
+            const int x = order[i];
+            int j = i - 1;
+            while (j >= 0 && a[order[j]][order[j]] > a[x][x]) {
+                order[j + 1] = order[j];
+                j--;
+            }
+            order[j + 1] = x;
+        }
+    }
+    __syncthreads();
+    for (int j = tid; j < m; j += 256)
+        st->ld[j] = a[order[j]][order[j]];
+    for (int e = tid; e < m * m; e += 256) {
+        const int j = e / m, i = e % m;
+        st->V[j * m + i] = z[i][order[j]];
+    }
+}
+
+// X_j <- sum_i V[j*m + i] X_i, j < m (in place, rows of n)
+__global__ __launch_bounds__(256) void rotate_dev_kernel(
+    const DavState *st, double *__restrict__ X, size_t n) {
+    __shared__ double vs[64 * 64];
+    const int m = st->m;
+    for (int e = threadIdx.x; e < m * m; e += 256)
+        vs[e] = st->V[e];
+    __syncthreads();
+    for (size_t r = blockIdx.x * (size_t)256 + threadIdx.x; r < n;
+         r += (size_t)gridDim.x * 256) {
+        double x[64];
+#pragma unroll
+        for (int i = 0; i < 64; i++)
+            x[i] = i < m ? X[(size_t)i * n + r] : 0.0;
+        for (int j = 0; j < m; j++) {
+            double s = 0.0;
+#pragma unroll
+            for (int i = 0; i < 64; i++)
+                if (i < m)
+                    s += vs[j * m + i] * x[i];
+            X[(size_t)j * n + r] = s;
+        }
+    }
+}
+
+// q = S_0 - ld_0 B_0 and partial sums of q.q
+__global__ __launch_bounds__(256) void residual_kernel(
+    const DavState *st, const double *__restrict__ B,
+    const double *__restrict__ S, double *__restrict__ q, size_t n,
+    double *__restrict__ partial) {
+    __shared__ double sh[256];
+    const double l = st->ld[0];
+    double s = 0.0;
+    for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * 256) {
+        const double v = S[i] - l * B[i];
+        q[i] = v;
+        s += v * v;
+    }
+    s = bsum(s, sh);
+    if (threadIdx.x == 0)
+        partial[blockIdx.x] = s;
+}
+
+// out[i] = sum_b partial[b*k + i], i < k (one block)
+__global__ __launch_bounds__(256) void reduce_kernel(
+    const double *__restrict__ partial, int nblk, int k,
+    double *__restrict__ out) {
+    __shared__ double sh[256];
+    for (int i = 0; i < k; i++) {
+        double s = 0.0;
+        for (int b = threadIdx.x; b < nblk; b += 256)
+            s += partial[(size_t)b * k + i];
+        s = bsum(s, sh);
+        if (threadIdx.x == 0)
+            out[i] = s;
+    }
+}
+
+// Olsen, first pass (ld from the state)
+__global__ __launch_bounds__(256) void olsen1_kernel(
+    const DavState *st, double *__restrict__ q, const double *__restrict__ c,
+    const double *__restrict__ aa, double *__restrict__ t, size_t n,
+    double *__restrict__ partial) {
+    __shared__ double sh[256];
+    const double ld = st->ld[0];
+    double s_cq = 0.0, s_ct = 0.0;
+    for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * 256) {
+        const double ci = c[i];
+        double qi = q[i], ti = ci;
+        const double dd = ld - aa[i];
+        if (fabs(dd) > 1E-12) {
+            ti = ci / dd;
+            qi = qi / dd;
+            q[i] = qi;
+        }
+        t[i] = ti;
+        s_cq += ci * qi;
+        s_ct += ci * ti;
+    }
+    s_cq = bsum(s_cq, sh);
+    s_ct = bsum(s_ct, sh);
+    if (threadIdx.x == 0) {
+        partial[(size_t)blockIdx.x * 2 + 0] = s_cq;
+        partial[(size_t)blockIdx.x * 2 + 1] = s_ct;
+    }
+}
+
+__global__ void olsen2_kernel(const DavState *st, double *__restrict__ q,
+                              const double *__restrict__ t, size_t n) {
+    const double a = -st->scal[1] / st->scal[2];
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        q[i] += a * t[i];
+}
+
+// convergence test and restart (one thread)
+__global__ void control_kernel(DavState *st) {
+    st->iter++;
+    st->qq = st->scal[0];
+    st->energy = st->ld[0];
+    if (fabs(st->qq) < st->conv) {
+        st->status = 1;
+        return;
+    }
+    if (st->m >= st->def_max)
+        st->m = st->def_min;
+    if (st->iter >= st->max_iter)
+        st->status = 2;
+}
+
+// partial dots <B_j|q>, j < m
+__global__ __launch_bounds__(256) void gs_dots_kernel(
+    const DavState *st, const double *__restrict__ B,
+    const double *__restrict__ q, size_t n, double *__restrict__ partial) {
+    __shared__ double sh[256];
+    if (st->status != 0)
+        return;
+    const int m = st->m;
+    for (int j = 0; j < m; j++) {
+        double s = 0.0;
+        for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < n;
+             i += (size_t)gridDim.x * 256)
+            s += B[(size_t)j * n + i] * q[i];
+        s = bsum(s, sh);
+        if (threadIdx.x == 0)
+            partial[(size_t)blockIdx.x * 64 + j] = s;
+    }
+}
+
+__global__ __launch_bounds__(256) void gs_reduce_kernel(
+    DavState *st, const double *__restrict__ partial, int nblk) {
+    __shared__ double sh[256];
+    if (st->status != 0)
+        return;
+    const int m = st->m;
+    for (int j = 0; j < m; j++) {
+        double s = 0.0;
+        for (int b = threadIdx.x; b < nblk; b += 256)
+            s += partial[(size_t)b * 64 + j];
+        s = bsum(s, sh);
+        if (threadIdx.x == 0)
+            st->dots[j] = s;
+    }
+}
+
+// q -= sum_j dots_j B_j
+__global__ __launch_bounds__(256) void gs_update_kernel(
+    const DavState *st, const double *__restrict__ B, double *__restrict__ q,
+    size_t n) {
+    __shared__ double d[64];
+    if (st->status != 0)
+        return;
+    const int m = st->m;
+    for (int j = threadIdx.x; j < m; j += 256)
+        d[j] = st->dots[j];
+    __syncthreads();
+    for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * 256) {
+        double s = q[i];
+        for (int j = 0; j < m; j++)
+            s -= d[j] * B[(size_t)j * n + i];
+        q[i] = s;
+    }
+}
+
+__global__ __launch_bounds__(256) void norm_kernel(
+    const DavState *st, const double *__restrict__ q, size_t n,
+    double *__restrict__ partial) {
+    __shared__ double sh[256];
+    if (st->status != 0)
+        return;
+    double s = 0.0;
+    for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * 256)
+        s += q[i] * q[i];
+    s = bsum(s, sh);
+    if (threadIdx.x == 0)
+        partial[blockIdx.x] = s;
+}
+
+// q /= |q|; qn = q; B[m] = q
+__global__ void append_kernel(const DavState *st, const double *__restrict__ q,
+                              double *__restrict__ qn, double *__restrict__ B,
+                              size_t n) {
+    if (st->status != 0)
+        return;
+    const double f = 1.0 / sqrt(st->scal[3]);
+    double *dst = B + (size_t)st->m * n;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const double v = q[i] * f;
+        qn[i] = v;
+        dst[i] = v;
+    }
+}
+
+__global__ void bump_kernel(DavState *st) {
+    if (st->status == 0)
+        st->m++;
+}
+
+dim3 vgrid(size_t n) {
+    size_t g = (n + 255) / 256;
+    if (g > 8192)
+        g = 8192;
+    return dim3((unsigned)(g ? g : 1));
+}
+
+} // namespace
+
+// The Davidson step after the matvec of the newest vector (sn = H qn):
+// everything up to and including the append of the next vector.  All
+// launches have fixed arguments; sizes come from the state.
+hipError_t launch_davidson_step(DavState *st, double *B, double *S,
+                                const double *sn, double *q, double *t,
+                                double *qn, const double *diag, size_t n,
+                                double *partial, hipStream_t s) {
+    hipLaunchKernelGGL(put_sigma_kernel, vgrid(n), dim3(256), 0, s, st, S, sn,
+                       n);
+    hipLaunchKernelGGL(gram_kernel, dim3(kGrid), dim3(256), 0, s, st, B, S, n,
+                       partial);
+    hipLaunchKernelGGL(gram_reduce_kernel, dim3(1), dim3(256), 0, s, st,
+                       partial, kGrid);
+    hipLaunchKernelGGL(ritz_kernel, dim3(1), dim3(256), 0, s, st);
+    hipLaunchKernelGGL(rotate_dev_kernel, vgrid(n), dim3(256), 0, s, st, S, n);
+    hipLaunchKernelGGL(rotate_dev_kernel, vgrid(n), dim3(256), 0, s, st, B, n);
+    hipLaunchKernelGGL(residual_kernel, dim3(kGrid), dim3(256), 0, s, st, B,
+                       S, q, n, partial);
+    hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(256), 0, s, partial, kGrid,
+                       1, st->scal + 0);
+    hipLaunchKernelGGL(olsen1_kernel, dim3(kGrid), dim3(256), 0, s, st, q, B,
+                       diag, t, n, partial);
+    hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(256), 0, s, partial, kGrid,
+                       2, st->scal + 1);
+    hipLaunchKernelGGL(olsen2_kernel, vgrid(n), dim3(256), 0, s, st, q, t, n);
+    hipLaunchKernelGGL(control_kernel, dim3(1), dim3(1), 0, s, st);
+    for (int pass = 0; pass < 2; pass++) {
+        hipLaunchKernelGGL(gs_dots_kernel, dim3(kGrid), dim3(256), 0, s, st, B,
+                           q, n, partial);
+        hipLaunchKernelGGL(gs_reduce_kernel, dim3(1), dim3(256), 0, s, st,
+                           partial, kGrid);
+        hipLaunchKernelGGL(gs_update_kernel, vgrid(n), dim3(256), 0, s, st, B,
+                           q, n);
+    }
+    hipLaunchKernelGGL(norm_kernel, dim3(kGrid), dim3(256), 0, s, st, q, n,
+                       partial);
+    hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(256), 0, s, partial, kGrid,
+                       1, st->scal + 3);
+    hipLaunchKernelGGL(append_kernel, vgrid(n), dim3(256), 0, s, st, q, qn, B,
+                       n);
+    hipLaunchKernelGGL(bump_kernel, dim3(1), dim3(1), 0, s, st);
+    return hipGetLastError();
+}
+
+int davidson_partial_len() { return kGrid * 64 * 64; }
+
+} // namespace b2g

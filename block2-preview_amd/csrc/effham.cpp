@@ -11,6 +11,7 @@
 #include "common.hpp"
 #include "launch.hpp"
 #include "plan.hpp"
+#include "replay.hpp"
 
 #include <algorithm>
 #include <chrono>
@@ -31,6 +32,9 @@ struct b2g_effham_s {
     b2g_plan plan = nullptr;
     double *diag = nullptr; // device, bra length
     uint64_t macs = 0;
+    // Davidson driver: 1 graph-captured device iteration (one root),
+    // 0 host-driven loop
+    int dav_mode = 1;
 };
 
 namespace b2g {
@@ -126,6 +130,22 @@ void jacobi_eigs(int m, std::vector<double> a, std::vector<double> &w,
         for (int i = 0; i < m; i++)
             v[j * m + i] = z[i * m + idx[j]];
     }
+}
+
+} // namespace b2g
+
+namespace b2g {
+
+// diagonal of H from the current operator buffers
+static int update_diag(b2g_effham h) {
+    std::vector<b2g_diag_term> dt(h->drecs.size());
+    for (size_t i = 0; i < dt.size(); i++) {
+        const DiagRec &q = h->drecs[i];
+        dt[i] = b2g_diag_term{q.a_off, q.b_off, q.d_off, q.a_n, q.b_n, q.alpha};
+    }
+    std::vector<b2g_block> db = blocks_of(h->prob.ket);
+    return b2g_diag(h->ctx, h->lbuf, h->rbuf, dt.data(), dt.size(), db.data(),
+                    db.size(), h->diag, h->prob.ket.total());
 }
 
 } // namespace b2g
@@ -267,20 +287,12 @@ extern "C" int b2g_effham_create(b2g_ctx ctx, const b2g_problem *pb,
                         yb.size(), p.ket.total(), p.bra.total(), flags,
                         &h->plan);
     if (r == B2G_OK) {
-        std::vector<b2g_diag_term> dt(h->drecs.size());
-        for (size_t i = 0; i < dt.size(); i++) {
-            const DiagRec &q = h->drecs[i];
-            dt[i] = b2g_diag_term{q.a_off, q.b_off, q.d_off,
-                                  q.a_n,   q.b_n,   q.alpha};
-        }
-        std::vector<b2g_block> db = blocks_of(p.ket);
         const size_t dl = std::max<uint64_t>(p.ket.total(), 1);
         if (hipMalloc(&h->diag, dl * sizeof(double)) != hipSuccess) {
             set_error("diagonal allocation failed");
             r = B2G_ERR_MEMORY;
         } else
-            r = b2g_diag(ctx, h->lbuf, h->rbuf, dt.data(), dt.size(),
-                         db.data(), db.size(), h->diag, p.ket.total());
+            r = update_diag(h.get());
     }
     if (r != B2G_OK) {
         b2g_effham_destroy(h.release());
@@ -370,6 +382,50 @@ extern "C" int b2g_effham_diag(b2g_effham h, double *d) {
     B2G_HIP(hipMemcpyAsync(d, h->diag, h->prob.ket.total() * sizeof(double),
                            hipMemcpyDeviceToDevice, h->ctx->stream));
     return B2G_OK;
+}
+
+extern "C" int b2g_effham_set_data(b2g_effham h, const double *left,
+                                   const double *right) {
+    B2G_REQUIRE(h && left && right, B2G_ERR_ARG, "NULL arg");
+    B2G_REQUIRE(h->plan, B2G_ERR_STATE, "effham was built host-only");
+    B2G_HIP(hipSetDevice(h->ctx->device));
+    if (h->lbuf->n)
+        B2G_HIP(hipMemcpyAsync(h->lbuf->ptr, left, h->lbuf->n * sizeof(double),
+                               hipMemcpyDeviceToDevice, h->ctx->stream));
+    if (h->rbuf->n)
+        B2G_HIP(hipMemcpyAsync(h->rbuf->ptr, right,
+                               h->rbuf->n * sizeof(double),
+                               hipMemcpyDeviceToDevice, h->ctx->stream));
+    int r = b2g_plan_refresh(h->plan);
+    if (r == B2G_OK)
+        r = update_diag(h);
+    return r;
+}
+
+extern "C" int b2g_effham_data(b2g_effham h, double **left, size_t *left_len,
+                               double **right, size_t *right_len) {
+    B2G_REQUIRE(h && left && left_len && right && right_len, B2G_ERR_ARG,
+                "NULL arg");
+    B2G_REQUIRE(h->plan, B2G_ERR_STATE, "effham was built host-only");
+    *left = (double *)h->lbuf->ptr;
+    *left_len = h->lbuf->n;
+    *right = (double *)h->rbuf->ptr;
+    *right_len = h->rbuf->n;
+    return B2G_OK;
+}
+
+extern "C" int b2g_effham_set_davidson_mode(b2g_effham h, int mode) {
+    B2G_REQUIRE(h && (mode == 0 || mode == 1), B2G_ERR_ARG, "bad mode");
+    h->dav_mode = mode;
+    return B2G_OK;
+}
+
+extern "C" int b2g_effham_grad_ops(b2g_effham h, const double *v,
+                                   const double *c, int n_rhs, double *gl,
+                                   double *gr, int accumulate) {
+    B2G_REQUIRE(h, B2G_ERR_ARG, "NULL effham");
+    B2G_REQUIRE(h->plan, B2G_ERR_STATE, "effham was built host-only");
+    return b2g_grad_ops(h->ctx, h->plan, v, c, n_rhs, gl, gr, accumulate);
 }
 
 // --------------------------------------------------------------- Davidson
@@ -532,6 +588,120 @@ static int davidson(b2g_effham h, double *x, int k, double conv_thrd,
     return B2G_OK;
 }
 
+// Graph-captured Davidson for one root: the first iteration runs eagerly
+// into the capture, every later iteration replays the same hipGraph (the
+// H*psi of the newest vector plus launch_davidson_step), and the host reads
+// a 64-byte status header from pinned memory after each replay.
+static int davidson_graph(b2g_effham h, double *x, double conv_thrd,
+                          int max_iter, int def_min, int def_max,
+                          int precision, std::vector<double> &eigvals,
+                          int &ndav, uint64_t &nmv) {
+    b2g_ctx ctx = h->ctx;
+    hipStream_t st = ctx->stream;
+    const size_t n = h->prob.ket.total();
+    B2G_REQUIRE(h->prob.bra.total() == n, B2G_ERR_ARG,
+                "Davidson needs bra and ket of equal size");
+    if (def_min < 1)
+        def_min = 1;
+    if (def_max < 2)
+        def_max = 2;
+    B2G_REQUIRE(def_max <= 63 && def_min < def_max, B2G_ERR_ARG,
+                "graph Davidson: need 1 <= def_min < def_max <= 63");
+    B2G_REQUIRE(!h->plan->timing, B2G_ERR_STATE,
+                "graph Davidson: disable plan timing first");
+    // the subspace holds def_max vectors plus the one appended before the
+    // restart check of the next iteration
+    const size_t nv = (size_t)def_max + 1;
+    DavWork w;
+    double *qn = nullptr, *sn = nullptr, *partial = nullptr;
+    DavState *dst = nullptr;
+    DavState *hdr = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    struct Cleanup {
+        double **a, **b, **c;
+        DavState **d, **e;
+        hipGraph_t *g;
+        hipGraphExec_t *x;
+        ~Cleanup() {
+            if (*x)
+                hipGraphExecDestroy(*x);
+            if (*g)
+                hipGraphDestroy(*g);
+            for (double **p : {a, b, c})
+                if (*p)
+                    hipFree(*p);
+            if (*d)
+                hipFree(*d);
+            if (*e)
+                hipHostFree(*e);
+        }
+    } cleanup{&qn, &sn, &partial, &dst, &hdr, &graph, &exec};
+    B2G_HIP(hipMalloc(&w.B, nv * n * sizeof(double)));
+    B2G_HIP(hipMalloc(&w.S, nv * n * sizeof(double)));
+    B2G_HIP(hipMalloc(&w.q, n * sizeof(double)));
+    B2G_HIP(hipMalloc(&w.t, n * sizeof(double)));
+    B2G_HIP(hipMalloc(&qn, n * sizeof(double)));
+    B2G_HIP(hipMalloc(&sn, n * sizeof(double)));
+    B2G_HIP(hipMalloc(&partial, davidson_partial_len() * sizeof(double)));
+    B2G_HIP(hipMalloc(&dst, sizeof(DavState)));
+    B2G_HIP(hipHostMalloc(&hdr, sizeof(DavState)));
+    // normalised start vector -> B[0] and qn
+    double nrm2 = 0;
+    int r = b2g_vec_dot(ctx, x, x, n, &nrm2);
+    if (r)
+        return r;
+    B2G_REQUIRE(nrm2 > 1E-28, B2G_ERR_ARG, "Davidson initial guess has zero norm");
+    B2G_HIP(launch_copy(x, w.B, n, st));
+    B2G_HIP(launch_scale(1.0 / std::sqrt(nrm2), w.B, n, st));
+    B2G_HIP(launch_copy(w.B, qn, n, st));
+    DavState init;
+    memset(&init, 0, sizeof init);
+    init.m = 1;
+    init.max_iter = max_iter;
+    init.def_min = def_min;
+    init.def_max = def_max;
+    init.conv = conv_thrd;
+    B2G_HIP(hipMemcpyAsync(dst, &init, sizeof init, hipMemcpyHostToDevice, st));
+    if ((r = plan_reserve(h->plan, 1, precision)) != B2G_OK)
+        return r;
+    B2G_HIP(hipStreamSynchronize(st));
+    // capture one iteration
+    B2G_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    int rc = b2g_hpsi(ctx, h->plan, qn, sn, 1, 1.0, 0, precision);
+    hipError_t e = rc == B2G_OK
+                       ? launch_davidson_step(dst, w.B, w.S, sn, w.q, w.t, qn,
+                                              h->diag, n, partial, st)
+                       : hipSuccess;
+    if (e == hipSuccess && rc == B2G_OK)
+        e = hipMemcpyAsync(hdr, dst, 64, hipMemcpyDeviceToHost, st);
+    const hipError_t e2 = hipStreamEndCapture(st, &graph);
+    if (rc != B2G_OK)
+        return rc;
+    B2G_HIP(e);
+    B2G_HIP(e2);
+    B2G_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    nmv = 0;
+    ndav = 0;
+    int status = 0;
+    while (status == 0) {
+        B2G_HIP(hipGraphLaunch(exec, st));
+        B2G_HIP(hipStreamSynchronize(st));
+        nmv++;
+        status = hdr->status;
+        ndav = hdr->iter;
+    }
+    eigvals.assign(1, hdr->energy);
+    B2G_HIP(launch_copy(w.B, x, n, st));
+    B2G_HIP(hipStreamSynchronize(st));
+    if (status != 1) {
+        set_error("Davidson (graph): not converged after " +
+                  std::to_string(ndav) + " iterations");
+        return B2G_ERR_NOCONV;
+    }
+    return B2G_OK;
+}
+
 } // namespace b2g
 
 extern "C" int b2g_effham_eigs(b2g_effham h, double *x, int k,
@@ -548,11 +718,16 @@ extern "C" int b2g_effham_eigs(b2g_effham h, double *x, int k,
     std::vector<double> ev;
     int nd = 0, nd2 = 0;
     uint64_t nmv = 0, nmv2 = 0;
-    int r = davidson(h, x, k, conv_thrd, max_iter, def_min, def_max, precision,
-                     ev, nd, nmv);
+    const bool graph = k == 1 && h->dav_mode == 1 && !h->plan->timing;
+    auto solve = [&](int prec, int &ndv, uint64_t &nm) {
+        return graph ? davidson_graph(h, x, conv_thrd, max_iter, def_min,
+                                      def_max, prec, ev, ndv, nm)
+                     : davidson(h, x, k, conv_thrd, max_iter, def_min,
+                                def_max, prec, ev, ndv, nm);
+    };
+    int r = solve(precision, nd, nmv);
     if (r == B2G_OK && precision != B2G_PREC_F64 && refine)
-        r = davidson(h, x, k, conv_thrd, max_iter, def_min, def_max,
-                     B2G_PREC_F64, ev, nd2, nmv2);
+        r = solve(B2G_PREC_F64, nd2, nmv2);
     auto t1 = std::chrono::steady_clock::now();
     for (int i = 0; i < k && i < (int)ev.size(); i++)
         energies[i] = ev[i];

@@ -155,7 +155,7 @@ __global__ __launch_bounds__(256) void seg_gemm_bf3_kernel(
     const TA *__restrict__ A, const TB *__restrict__ B, TD *__restrict__ D,
     TD *__restrict__ P, const Task *__restrict__ tasks, int ntasks,
     const Seg *__restrict__ segs, uint64_t a_rs, uint64_t b_rs, uint64_t d_rs,
-    uint64_t p_rs, double scale, int beta0) {
+    uint64_t p_rs, double scale, int beta0, XcdSplit xs) {
     typedef Bf3Tile<TA, TB, TM, TN> T;
     constexpr int WM = TM / 2, WN = TN / 2;
     constexpr int FM = WM / 16, FN = WN / 16;
@@ -169,7 +169,14 @@ __global__ __launch_bounds__(256) void seg_gemm_bf3_kernel(
     const int wm = wave >> 1, wn = wave & 1;
     const int lr = lane & 15, lk = lane >> 4;
     T ld;
-    for (int t = blockIdx.x; t < ntasks; t += gridDim.x) {
+    int t0 = blockIdx.x, tend = ntasks, tstep = gridDim.x;
+    if (xs.off[8] != 0) { // per-XCD task ranges, as in seg_gemm_kernel
+        const int x = blockIdx.x & 7;
+        t0 = (int)xs.off[x] + (int)(blockIdx.x >> 3);
+        tend = (int)xs.off[x + 1];
+        tstep = gridDim.x >> 3;
+    }
+    for (int t = t0; t < tend; t += tstep) {
         const Task task = tasks[t];
         f32x4 acc[FM][FN];
 #pragma unroll
@@ -262,14 +269,14 @@ hipError_t launch_seg_gemm_bf3(int cfg, const TA *A, const TB *B, TD *D,
                                const Seg *segs, uint64_t a_rs, uint64_t b_rs,
                                uint64_t d_rs, uint64_t p_rs, double scale,
                                int beta0, int n_rhs, int grid,
-                               hipStream_t st) {
+                               hipStream_t st, XcdSplit xs) {
     if (ntasks <= 0)
         return hipSuccess;
-    dim3 g(task_grid(grid, ntasks), (unsigned)n_rhs);
+    dim3 g(task_grid(grid, ntasks, xs), (unsigned)n_rhs);
 #define B2G_LAUNCH(TM_, TN_)                                                   \
     hipLaunchKernelGGL((seg_gemm_bf3_kernel<TA, TB, TD, TM_, TN_>), g,         \
                        dim3(256), 0, st, A, B, D, P, tasks, ntasks, segs,      \
-                       a_rs, b_rs, d_rs, p_rs, scale, beta0)
+                       a_rs, b_rs, d_rs, p_rs, scale, beta0, xs)
     switch (cfg) {
     case 0: B2G_LAUNCH(32, 32); break;
     case 1: B2G_LAUNCH(32, 64); break;
@@ -288,10 +295,10 @@ hipError_t launch_seg_gemm_bf3(int cfg, const TA *A, const TB *B, TD *D,
 template hipError_t launch_seg_gemm_bf3<double, float, float>(
     int, const double *, const float *, float *, float *, const Task *, int,
     const Seg *, uint64_t, uint64_t, uint64_t, uint64_t, double, int, int,
-    int, hipStream_t);
+    int, hipStream_t, XcdSplit);
 template hipError_t launch_seg_gemm_bf3<float, float, double>(
     int, const float *, const float *, double *, double *, const Task *, int,
     const Seg *, uint64_t, uint64_t, uint64_t, uint64_t, double, int, int,
-    int, hipStream_t);
+    int, hipStream_t, XcdSplit);
 
 } // namespace b2g

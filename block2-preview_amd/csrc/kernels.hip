@@ -112,7 +112,7 @@ __global__ __launch_bounds__(256) void seg_gemm_kernel(
     const TA *__restrict__ A, const TB *__restrict__ B, TD *__restrict__ D,
     TD *__restrict__ P, const Task *__restrict__ tasks, int ntasks,
     const Seg *__restrict__ segs, uint64_t a_rs, uint64_t b_rs, uint64_t d_rs,
-    uint64_t p_rs, double scale, int beta0) {
+    uint64_t p_rs, double scale, int beta0, XcdSplit xs) {
     constexpr int WM = TM / 2, WN = TN / 2;
     constexpr int FM = WM / 16, FN = WN / 16;
     constexpr int ASZ = TK * (TM + 1), BSZ = TK * (TN + 1);
@@ -127,7 +127,14 @@ __global__ __launch_bounds__(256) void seg_gemm_kernel(
     const int wm = wave >> 1, wn = wave & 1;
     const int lr = lane & 15, lk = lane >> 4;
     TileLoader<TA, TB, TC, TM, TN, TK> ld;
-    for (int t = blockIdx.x; t < ntasks; t += gridDim.x) {
+    int t0 = blockIdx.x, tend = ntasks, tstep = gridDim.x;
+    if (xs.off[8] != 0) {
+        const int x = blockIdx.x & 7;
+        t0 = (int)xs.off[x] + (int)(blockIdx.x >> 3);
+        tend = (int)xs.off[x + 1];
+        tstep = gridDim.x >> 3;
+    }
+    for (int t = t0; t < tend; t += tstep) {
         const Task task = tasks[t];
         acc_t acc[FM][FN];
 #pragma unroll
@@ -277,8 +284,9 @@ int tile_m(int cfg) { return 32 << (cfg / 3); }
 int tile_n(int cfg) { return 32 << (cfg % 3); }
 
 // persistent grid: min(grid, ntasks) workgroups
-unsigned task_grid(int grid, int ntasks) {
-    return (unsigned)(grid < ntasks ? grid : ntasks);
+unsigned task_grid(int grid, int ntasks, const XcdSplit &xs) {
+    const int g = grid < ntasks ? grid : ntasks;
+    return (unsigned)(xs.off[8] != 0 ? (g + 7) & ~7 : g);
 }
 
 template <typename TA, typename TB, typename TD, typename TC, int PHASE>
@@ -286,15 +294,15 @@ hipError_t launch_seg_gemm(int cfg, const TA *A, const TB *B, TD *D, TD *P,
                            const Task *tasks, int ntasks, const Seg *segs,
                            uint64_t a_rs, uint64_t b_rs, uint64_t d_rs,
                            uint64_t p_rs, double scale, int beta0, int n_rhs,
-                           int grid, hipStream_t st) {
+                           int grid, hipStream_t st, XcdSplit xs) {
     if (ntasks <= 0)
         return hipSuccess;
-    dim3 g(task_grid(grid, ntasks), (unsigned)n_rhs);
+    dim3 g(task_grid(grid, ntasks, xs), (unsigned)n_rhs);
 #define B2G_LAUNCH(TM_, TN_, TK_)                                              \
     hipLaunchKernelGGL((seg_gemm_kernel<TA, TB, TD, TC, TM_, TN_, TK_, PHASE>), \
                        g,                                                      \
                        dim3(256), 0, st, A, B, D, P, tasks, ntasks, segs,      \
-                       a_rs, b_rs, d_rs, p_rs, scale, beta0)
+                       a_rs, b_rs, d_rs, p_rs, scale, beta0, xs)
     switch (cfg) {
     case 0: B2G_LAUNCH(32, 32, 32); break;
     case 1: B2G_LAUNCH(32, 64, 16); break;
@@ -313,19 +321,27 @@ hipError_t launch_seg_gemm(int cfg, const TA *A, const TB *B, TD *D, TD *P,
 template hipError_t launch_seg_gemm<double, double, double, double, 1>(
     int, const double *, const double *, double *, double *, const Task *, int, const Seg *,
     uint64_t, uint64_t, uint64_t, uint64_t, double, int, int, int,
-    hipStream_t);
+    hipStream_t, XcdSplit);
 template hipError_t launch_seg_gemm<double, double, double, double, 2>(
     int, const double *, const double *, double *, double *, const Task *, int, const Seg *,
     uint64_t, uint64_t, uint64_t, uint64_t, double, int, int, int,
-    hipStream_t);
+    hipStream_t, XcdSplit);
+template hipError_t launch_seg_gemm<double, double, double, double, 3>(
+    int, const double *, const double *, double *, double *, const Task *, int, const Seg *,
+    uint64_t, uint64_t, uint64_t, uint64_t, double, int, int, int,
+    hipStream_t, XcdSplit);
+template hipError_t launch_seg_gemm<double, double, double, double, 4>(
+    int, const double *, const double *, double *, double *, const Task *, int, const Seg *,
+    uint64_t, uint64_t, uint64_t, uint64_t, double, int, int, int,
+    hipStream_t, XcdSplit);
 template hipError_t launch_seg_gemm<double, float, float, float, 1>(
     int, const double *, const float *, float *, float *, const Task *, int, const Seg *,
     uint64_t, uint64_t, uint64_t, uint64_t, double, int, int, int,
-    hipStream_t);
+    hipStream_t, XcdSplit);
 template hipError_t launch_seg_gemm<float, float, double, float, 2>(
     int, const float *, const float *, double *, double *, const Task *, int, const Seg *,
     uint64_t, uint64_t, uint64_t, uint64_t, double, int, int, int,
-    hipStream_t);
+    hipStream_t, XcdSplit);
 
 // ---------------------------------------------------------------- diagonal
 __global__ __launch_bounds__(256) void diag_kernel(

@@ -37,6 +37,13 @@ struct Task {
     uint32_t beta;
 };
 
+// Per-XCD task ranges of one launch: when off[8] != 0, workgroup b (dealt
+// to XCD b % 8 by the dispatcher) walks tasks [off[b % 8], off[b % 8 + 1])
+// with stride grid / 8; otherwise all workgroups stride the whole list.
+struct XcdSplit {
+    uint32_t off[9];
+};
+
 // Deterministic split-K reduction of one output tile:
 //   D[tile] = (beta ? D[tile] : 0) + sum_{p < nparts} P[p_off + p*tm*tn + ...]
 struct RedTask {
@@ -54,6 +61,24 @@ struct DiagTermDev {
     uint64_t a_off, b_off;
     uint32_t a_n, b_n;
     double alpha;
+};
+
+// Device-resident state of the graph-captured Davidson (davidson_dev.hip).
+// The first 64 bytes are the header the host reads after each iteration.
+struct DavState {
+    int m;        // vectors in the subspace
+    int status;   // 0 running, 1 converged, 2 iteration limit
+    int iter;     // iterations done
+    int max_iter;
+    int def_min, def_max;
+    int pad0, pad1;
+    double conv;   // threshold on the squared residual norm
+    double qq;     // squared residual norm of the last iteration
+    double energy; // lowest Ritz value of the last iteration
+    double pad2;
+    double scal[8]; // reduction results: |q|^2, Olsen c.q and c.t, |q|^2
+    double ld[64], dots[64];
+    double G[64 * 64], V[64 * 64];
 };
 
 } // namespace b2g

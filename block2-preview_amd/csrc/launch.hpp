@@ -12,17 +12,19 @@ int tile_m(int cfg);
 int tile_n(int cfg);
 constexpr int kTileConfigs = 9;
 
-// persistent grid size of a grouped-GEMM launch
-unsigned task_grid(int grid, int ntasks);
+// persistent grid size of a grouped-GEMM launch (a multiple of 8 when the
+// launch has per-XCD task ranges)
+unsigned task_grid(int grid, int ntasks, const XcdSplit &xs);
 
-// PHASE (1: work = C op(B), 2: V += alpha op(A) work) only tags the kernel
-// name, so that profiles tell the phases apart
+// PHASE (1: work = C op(B), 2: V += alpha op(A) work, 3 / 4: the two stages
+// of the operator gradients) only tags the kernel name, so that profiles tell
+// the phases apart
 template <typename TA, typename TB, typename TD, typename TC, int PHASE>
 hipError_t launch_seg_gemm(int cfg, const TA *A, const TB *B, TD *D, TD *P,
                            const Task *tasks, int ntasks, const Seg *segs,
                            uint64_t a_rs, uint64_t b_rs, uint64_t d_rs,
                            uint64_t p_rs, double scale, int beta0, int n_rhs,
-                           int grid, hipStream_t st);
+                           int grid, hipStream_t st, XcdSplit xs = XcdSplit{});
 template <typename TD>
 hipError_t launch_splitk_reduce(TD *D, const TD *P, const RedTask *tasks,
                                 int ntasks, uint64_t d_rs, uint64_t p_rs,
@@ -36,7 +38,7 @@ hipError_t launch_seg_gemm_bf3(int cfg, const TA *A, const TB *B, TD *D,
                                const Seg *segs, uint64_t a_rs, uint64_t b_rs,
                                uint64_t d_rs, uint64_t p_rs, double scale,
                                int beta0, int n_rhs, int grid,
-                               hipStream_t st);
+                               hipStream_t st, XcdSplit xs = XcdSplit{});
 
 hipError_t launch_diag(const double *L, const double *R, double *Dv,
                        const DiagBlock *blocks, int nblocks,
@@ -61,5 +63,13 @@ hipError_t launch_olsen(double *q, const double *c, const double *aa,
                         double ld, double *t, size_t n, double *partial,
                         double *sums, hipStream_t st);
 int red_grid();
+
+// One device Davidson step after sn = H qn (davidson_dev.hip); `partial`
+// holds davidson_partial_len() doubles.
+hipError_t launch_davidson_step(DavState *st, double *B, double *S,
+                                const double *sn, double *q, double *t,
+                                double *qn, const double *diag, size_t n,
+                                double *partial, hipStream_t s);
+int davidson_partial_len();
 
 } // namespace b2g

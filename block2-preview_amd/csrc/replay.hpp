@@ -1,0 +1,90 @@
+// Internal: the device replay plan shared by the H*psi and gradient
+// launchers (runtime.hip, grad.hip).
+#pragma once
+
+#include "common.hpp"
+#include "launch.hpp"
+
+#include <vector>
+
+namespace b2g {
+
+struct GradPlan; // grad.hip
+void grad_plan_destroy(GradPlan *g);
+
+struct Chunk {
+    // host images; tasks are bucketed by tile configuration
+    std::vector<Task> t1[kTileConfigs], t2[kTileConfigs];
+    std::vector<Seg> s1, s2;
+    std::vector<RedTask> red; // split-K reductions of phase 2
+    uint64_t w_len = 0;       // W elements per right-hand side
+    uint64_t p_len = 0;       // partial-tile elements per right-hand side
+    // device copies (inside plan->dev)
+    Task *d_t1[kTileConfigs] = {}, *d_t2[kTileConfigs] = {};
+    Seg *d_s1 = nullptr, *d_s2 = nullptr;
+    RedTask *d_red = nullptr;
+    // algorithmic FLOPs per RHS of each tile configuration's launch
+    double f1[kTileConfigs] = {}, f2[kTileConfigs] = {};
+    // per-XCD task ranges of each launch
+    XcdSplit x1[kTileConfigs] = {}, x2[kTileConfigs] = {};
+};
+
+} // namespace b2g
+
+struct b2g_plan_s {
+    b2g_ctx ctx = nullptr;
+    b2g_buf lbuf = nullptr, rbuf = nullptr;
+    size_t x_len = 0, y_len = 0;
+    uint64_t macs = 0;
+    std::vector<b2g::Chunk> chunks;
+    uint64_t w_max = 0;
+    void *dev = nullptr; // one allocation for all task / segment arrays
+    float *l32 = nullptr, *r32 = nullptr;
+    void *w = nullptr; // W scratch
+    size_t w_bytes = 0;
+    uint64_t p_max = 0;   // partial elements per RHS (max over chunks)
+    void *pbuf = nullptr; // split-K partial tiles (FP64)
+    size_t p_bytes = 0;
+    unsigned flags = 0;
+    // optional per-launch timing: (kernel id, start, stop) event records
+    // resolved when queried; kernel ids as in b2g.h (B2G_NKERNELS)
+    bool timing = false;
+    struct Stamp {
+        int kernel;
+        hipEvent_t start, stop;
+    };
+    std::vector<Stamp> stamps;
+    // algorithmic FLOPs of one H*psi per right-hand side, per kernel id
+    double kflops[B2G_NKERNELS] = {};
+    // the uploaded pair list (the gradient plan is built from it on first
+    // use) and that plan
+    std::vector<b2g_pair> pairs;
+    b2g::GradPlan *grad = nullptr;
+};
+
+
+namespace b2g {
+
+// one TM x TN tile of an output block
+struct Tile {
+    uint32_t row0, col0, tm, tn;
+};
+// tiling of an M x N output block for tasks of depth K (measured tile
+// rates, or B2G_TILE)
+std::vector<Tile> tile_cover(uint32_t M, uint32_t N, uint64_t K);
+Task make_task(uint64_t d_off, uint32_t ldd, uint32_t M, uint32_t N,
+               const Tile &t, uint32_t sb, uint32_t se, uint32_t beta);
+int cfg_of(const Tile &t);
+// one task per tile of the block, all over segments [sb, se)
+void add_tiles(std::vector<Task> *tasks, uint64_t d_off, uint32_t M,
+               uint32_t N, uint64_t K, uint32_t sb, uint32_t se,
+               uint32_t beta);
+double task_flops(const std::vector<Task> &tasks, const std::vector<Seg> &segs,
+                  int cfg);
+void sort_tasks(std::vector<Task> &tasks, const std::vector<Seg> &segs);
+// scratch elements per chunk and right-hand side (B2G_W_CAP)
+uint64_t w_cap_elems();
+// allocate the H*psi scratch for n_rhs at precision ahead of a capture
+int plan_reserve(b2g_plan p, int n_rhs, int precision);
+
+} // namespace b2g

@@ -2,6 +2,7 @@
 // diagonal and vector kernels.
 #include "common.hpp"
 #include "launch.hpp"
+#include "replay.hpp"
 
 #include <algorithm>
 #include <cstdio>
@@ -83,6 +84,13 @@ extern "C" int b2g_ctx_create(int device, unsigned flags, b2g_ctx *out) {
     c->red_len = (size_t)red_grid() * 64 * 64 + 8192;
     if (hipMalloc(&c->red, c->red_len * sizeof(double)) != hipSuccess ||
         hipHostMalloc(&c->red_host, 8192 * sizeof(double)) != hipSuccess) {
+        if (c->red)
+            hipFree(c->red);
+        for (hipStream_t s : c->aux)
+            hipStreamDestroy(s);
+        for (hipEvent_t e : c->join)
+            hipEventDestroy(e);
+        hipEventDestroy(c->fork);
         hipStreamDestroy(c->stream);
         delete c;
         set_error("allocation of reduction scratch failed");
@@ -208,52 +216,6 @@ extern "C" size_t b2g_buf_size(b2g_buf buf) { return buf ? buf->n : 0; }
 // -------------------------------------------------------------------- plans
 namespace b2g {
 
-struct Chunk {
-    // host images; tasks are bucketed by tile configuration
-    std::vector<Task> t1[kTileConfigs], t2[kTileConfigs];
-    std::vector<Seg> s1, s2;
-    std::vector<RedTask> red; // split-K reductions of phase 2
-    uint64_t w_len = 0;       // W elements per right-hand side
-    uint64_t p_len = 0;       // partial-tile elements per right-hand side
-    // device copies (inside plan->dev)
-    Task *d_t1[kTileConfigs] = {}, *d_t2[kTileConfigs] = {};
-    Seg *d_s1 = nullptr, *d_s2 = nullptr;
-    RedTask *d_red = nullptr;
-    // algorithmic FLOPs per RHS of each tile configuration's launch
-    double f1[kTileConfigs] = {}, f2[kTileConfigs] = {};
-};
-
-} // namespace b2g
-
-struct b2g_plan_s {
-    b2g_ctx ctx = nullptr;
-    b2g_buf lbuf = nullptr, rbuf = nullptr;
-    size_t x_len = 0, y_len = 0;
-    uint64_t macs = 0;
-    std::vector<Chunk> chunks;
-    uint64_t w_max = 0;
-    void *dev = nullptr; // one allocation for all task / segment arrays
-    float *l32 = nullptr, *r32 = nullptr;
-    void *w = nullptr; // W scratch
-    size_t w_bytes = 0;
-    uint64_t p_max = 0;   // partial elements per RHS (max over chunks)
-    void *pbuf = nullptr; // split-K partial tiles (FP64)
-    size_t p_bytes = 0;
-    unsigned flags = 0;
-    // optional per-launch timing: (kernel id, start, stop) event records
-    // resolved when queried; kernel ids as in b2g.h (B2G_NKERNELS)
-    bool timing = false;
-    struct Stamp {
-        int kernel;
-        hipEvent_t start, stop;
-    };
-    std::vector<Stamp> stamps;
-    // algorithmic FLOPs of one H*psi per right-hand side, per kernel id
-    double kflops[B2G_NKERNELS] = {};
-};
-
-namespace b2g {
-
 static uint64_t tile_cost(const Task &t, const std::vector<Seg> &segs) {
     uint64_t k = 0;
     for (uint32_t s = t.seg_begin; s < t.seg_end; s++)
@@ -295,12 +257,8 @@ static std::vector<std::vector<uint32_t>> covers(uint32_t d) {
     return out;
 }
 
-struct Tile {
-    uint32_t row0, col0, tm, tn;
-};
-
 // tiles covering an M x N output block whose tasks have depth K
-static std::vector<Tile> tile_cover(uint32_t M, uint32_t N, uint64_t K) {
+std::vector<Tile> tile_cover(uint32_t M, uint32_t N, uint64_t K) {
     const char *e = getenv("B2G_TILE");
     std::vector<uint32_t> rows, cols;
     uint32_t fm = 0, fn = 0;
@@ -343,7 +301,7 @@ static std::vector<Tile> tile_cover(uint32_t M, uint32_t N, uint64_t K) {
     return out;
 }
 
-static Task make_task(uint64_t d_off, uint32_t ldd, uint32_t M, uint32_t N,
+Task make_task(uint64_t d_off, uint32_t ldd, uint32_t M, uint32_t N,
                       const Tile &t, uint32_t sb, uint32_t se, uint32_t beta) {
     Task k;
     k.d_off = d_off;
@@ -358,12 +316,12 @@ static Task make_task(uint64_t d_off, uint32_t ldd, uint32_t M, uint32_t N,
     return k;
 }
 
-static int cfg_of(const Tile &t) {
+int cfg_of(const Tile &t) {
     return 3 * size_index(t.tm) + size_index(t.tn);
 }
 
 // one task per tile (phase 1 and unsplit phase-2 blocks)
-static void add_tiles(std::vector<Task> *tasks, uint64_t d_off, uint32_t M,
+void add_tiles(std::vector<Task> *tasks, uint64_t d_off, uint32_t M,
                       uint32_t N, uint64_t K, uint32_t sb, uint32_t se,
                       uint32_t beta) {
     for (const Tile &t : tile_cover(M, N, K))
@@ -437,7 +395,7 @@ static void add_split_tiles(Chunk &ch, uint64_t d_off, uint32_t M, uint32_t N,
 
 // multiply-adds x 2 of the tasks restricted to their tiles (the edge tiles'
 // masked rows and columns do not count)
-static double task_flops(const std::vector<Task> &tasks,
+double task_flops(const std::vector<Task> &tasks,
                          const std::vector<Seg> &segs, int cfg) {
     double f = 0;
     const uint32_t tm = 32u << (cfg / 3), tn = 32u << (cfg % 3);
@@ -452,14 +410,77 @@ static double task_flops(const std::vector<Task> &tasks,
     return f;
 }
 
-static void sort_tasks(std::vector<Task> &tasks, const std::vector<Seg> &segs) {
+void sort_tasks(std::vector<Task> &tasks, const std::vector<Seg> &segs) {
     std::stable_sort(tasks.begin(), tasks.end(),
                      [&segs](const Task &a, const Task &b) {
                          return tile_cost(a, segs) > tile_cost(b, segs);
                      });
 }
 
-static uint64_t w_cap_elems() {
+// B2G_XCD=0 turns the per-XCD task ranges off
+static bool xcd_enabled() {
+    const char *e = getenv("B2G_XCD");
+    return e == nullptr || atoi(e) != 0;
+}
+
+// Orders one launch's tasks for the 8 XCDs and returns their ranges.  Tasks
+// with the same segment list (the tiles of one output block, or of one
+// split-K group) re-read the same operator and work panels, so they form a
+// group that stays on one XCD and runs back to back there (shared L2).
+// Groups are dealt longest first to the least-loaded XCD; each XCD runs its
+// groups in that order, tiles of a group in creation (row-major) order.
+static XcdSplit xcd_order(std::vector<Task> &tasks,
+                          const std::vector<Seg> &segs) {
+    XcdSplit xs{};
+    if (tasks.empty())
+        return xs;
+    if (!xcd_enabled()) {
+        sort_tasks(tasks, segs);
+        return xs;
+    }
+    std::vector<std::pair<uint32_t, std::vector<size_t>>> groups;
+    std::unordered_map<uint32_t, size_t> gid;
+    for (size_t i = 0; i < tasks.size(); i++) {
+        auto it = gid.find(tasks[i].seg_begin);
+        if (it == gid.end()) {
+            it = gid.emplace(tasks[i].seg_begin, groups.size()).first;
+            groups.push_back({tasks[i].seg_begin, {}});
+        }
+        groups[it->second].second.push_back(i);
+    }
+    std::vector<double> gcost(groups.size(), 0.0);
+    for (size_t g = 0; g < groups.size(); g++)
+        for (size_t i : groups[g].second)
+            gcost[g] += (double)tile_cost(tasks[i], segs);
+    std::vector<size_t> order(groups.size());
+    for (size_t g = 0; g < order.size(); g++)
+        order[g] = g;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](size_t a, size_t b) { return gcost[a] > gcost[b]; });
+    double load[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    std::vector<size_t> per[8];
+    for (size_t g : order) {
+        int x = 0;
+        for (int y = 1; y < 8; y++)
+            if (load[y] < load[x])
+                x = y;
+        load[x] += gcost[g];
+        for (size_t i : groups[g].second)
+            per[x].push_back(i);
+    }
+    std::vector<Task> out;
+    out.reserve(tasks.size());
+    for (int x = 0; x < 8; x++) {
+        xs.off[x] = (uint32_t)out.size();
+        for (size_t i : per[x])
+            out.push_back(tasks[i]);
+    }
+    xs.off[8] = (uint32_t)out.size();
+    tasks.swap(out);
+    return xs;
+}
+
+uint64_t w_cap_elems() {
     const char *e = getenv("B2G_W_CAP");
     if (e != nullptr) {
         long long v = atoll(e);
@@ -574,8 +595,8 @@ static int compile_plan(b2g_plan p, const b2g_pair *pairs, size_t n_pairs,
                 emit(yb[yidx[kv.first]], &kv.second, 1);
         }
         for (int c = 0; c < kTileConfigs; c++) {
-            sort_tasks(ch.t1[c], ch.s1);
-            sort_tasks(ch.t2[c], ch.s2);
+            ch.x1[c] = xcd_order(ch.t1[c], ch.s1);
+            ch.x2[c] = xcd_order(ch.t2[c], ch.s2);
             ch.f1[c] = task_flops(ch.t1[c], ch.s1, c);
             ch.f2[c] = task_flops(ch.t2[c], ch.s2, c);
             p->kflops[B2G_K_PHASE1 + c] += ch.f1[c];
@@ -661,6 +682,8 @@ extern "C" int b2g_upload_plan(b2g_ctx ctx, b2g_buf lbuf, b2g_buf rbuf,
     p->y_len = y_len;
     p->flags = flags;
     int r = compile_plan(p, pairs, n_pairs, yblocks, n_yblocks);
+    if (r == B2G_OK)
+        p->pairs.assign(pairs, pairs + n_pairs);
     if (r == B2G_OK && (flags & 3)) {
         if (hipMalloc(&p->l32, std::max<size_t>(lbuf->n, 1) * 4) !=
                 hipSuccess ||
@@ -685,6 +708,18 @@ extern "C" int b2g_upload_plan(b2g_ctx ctx, b2g_buf lbuf, b2g_buf rbuf,
     return B2G_OK;
 }
 
+extern "C" int b2g_plan_refresh(b2g_plan p) {
+    B2G_REQUIRE(p, B2G_ERR_ARG, "NULL plan");
+    B2G_HIP(hipSetDevice(p->ctx->device));
+    if (p->l32) {
+        B2G_HIP(launch_f64_to_f32((const double *)p->lbuf->ptr, p->l32,
+                                  p->lbuf->n, p->ctx->stream));
+        B2G_HIP(launch_f64_to_f32((const double *)p->rbuf->ptr, p->r32,
+                                  p->rbuf->n, p->ctx->stream));
+    }
+    return B2G_OK;
+}
+
 extern "C" int b2g_plan_destroy(b2g_plan p) {
     if (p == nullptr)
         return B2G_OK;
@@ -704,6 +739,7 @@ extern "C" int b2g_plan_destroy(b2g_plan p) {
         hipFree(p->w);
     if (p->pbuf)
         hipFree(p->pbuf);
+    grad_plan_destroy(p->grad);
     delete p;
     return B2G_OK;
 }
@@ -779,18 +815,12 @@ extern "C" int b2g_plan_stats(b2g_plan p, uint64_t *n1, uint64_t *n2,
     return B2G_OK;
 }
 
-extern "C" int b2g_hpsi(b2g_ctx ctx, b2g_plan p, const double *x, double *y,
-                        int n_rhs, double scale, int accumulate,
-                        int precision) {
-    B2G_REQUIRE(ctx && p && x && y, B2G_ERR_ARG, "NULL arg");
-    B2G_REQUIRE(n_rhs >= 1 && n_rhs <= 65535, B2G_ERR_ARG, "bad n_rhs");
-    B2G_REQUIRE(precision == B2G_PREC_F64 || precision == B2G_PREC_F32 ||
-                    precision == B2G_PREC_BF16X3,
-                B2G_ERR_ARG, "unsupported precision");
-    B2G_REQUIRE(precision == B2G_PREC_F64 || p->l32 != nullptr, B2G_ERR_STATE,
-                "plan was uploaded without reduced-precision shadows "
-                "(flags bit 0 or 1)");
-    B2G_HIP(hipSetDevice(ctx->device));
+namespace b2g {
+
+// scratch (W, split-K partials) for n_rhs right-hand sides at `precision`;
+// reallocates only when growing (not allowed while a stream is captured)
+int plan_reserve(b2g_plan p, int n_rhs, int precision) {
+    b2g_ctx ctx = p->ctx;
     const size_t es = precision == B2G_PREC_F64 ? 8 : 4;
     const size_t need = std::max<uint64_t>(p->w_max, 1) * n_rhs * es;
     if (need > p->w_bytes) {
@@ -813,6 +843,28 @@ extern "C" int b2g_hpsi(b2g_ctx ctx, b2g_plan p, const double *x, double *y,
         }
         B2G_HIP(hipMalloc(&p->pbuf, pneed));
         p->p_bytes = pneed;
+    }
+    return B2G_OK;
+}
+
+} // namespace b2g
+
+extern "C" int b2g_hpsi(b2g_ctx ctx, b2g_plan p, const double *x, double *y,
+                        int n_rhs, double scale, int accumulate,
+                        int precision) {
+    B2G_REQUIRE(ctx && p && x && y, B2G_ERR_ARG, "NULL arg");
+    B2G_REQUIRE(n_rhs >= 1 && n_rhs <= 65535, B2G_ERR_ARG, "bad n_rhs");
+    B2G_REQUIRE(precision == B2G_PREC_F64 || precision == B2G_PREC_F32 ||
+                    precision == B2G_PREC_BF16X3,
+                B2G_ERR_ARG, "unsupported precision");
+    B2G_REQUIRE(precision == B2G_PREC_F64 || p->l32 != nullptr, B2G_ERR_STATE,
+                "plan was uploaded without reduced-precision shadows "
+                "(flags bit 0 or 1)");
+    B2G_HIP(hipSetDevice(ctx->device));
+    {
+        const int r = plan_reserve(p, n_rhs, precision);
+        if (r != B2G_OK)
+            return r;
     }
     double *P = (double *)p->pbuf;
     const int grid = ctx->num_cu * 8;
@@ -887,15 +939,15 @@ extern "C" int b2g_hpsi(b2g_ctx ctx, b2g_plan p, const double *x, double *y,
                 return launch_seg_gemm<double, double, double, double, 1>(
                     c, x, (const double *)p->rbuf->ptr, (double *)p->w,
                     (double *)p->w, ch.d_t1[c], nt, ch.d_s1, p->x_len, 0,
-                    p->w_max, 0, 1.0, 0, n_rhs, grid, st);
+                    p->w_max, 0, 1.0, 0, n_rhs, grid, st, ch.x1[c]);
             if (precision == B2G_PREC_BF16X3)
                 return launch_seg_gemm_bf3<double, float, float>(
                     c, x, p->r32, (float *)p->w, (float *)p->w, ch.d_t1[c], nt,
                     ch.d_s1, p->x_len, 0, p->w_max, 0, 1.0, 0, n_rhs, grid,
-                    st);
+                    st, ch.x1[c]);
             return launch_seg_gemm<double, float, float, float, 1>(
                 c, x, p->r32, (float *)p->w, (float *)p->w, ch.d_t1[c], nt,
-                ch.d_s1, p->x_len, 0, p->w_max, 0, 1.0, 0, n_rhs, grid, st);
+                ch.d_s1, p->x_len, 0, p->w_max, 0, 1.0, 0, n_rhs, grid, st, ch.x1[c]);
         }));
         B2G_HIP(phase(ch.t2, ch.f2, B2G_K_PHASE2, [&](int c, hipStream_t st) {
             const int nt = (int)ch.t2[c].size();
@@ -903,16 +955,16 @@ extern "C" int b2g_hpsi(b2g_ctx ctx, b2g_plan p, const double *x, double *y,
                 return launch_seg_gemm<double, double, double, double, 2>(
                     c, (const double *)p->lbuf->ptr, (const double *)p->w, y,
                     P, ch.d_t2[c], nt, ch.d_s2, 0, p->w_max, p->y_len,
-                    p->p_max, scale, beta0, n_rhs, grid, st);
+                    p->p_max, scale, beta0, n_rhs, grid, st, ch.x2[c]);
             if (precision == B2G_PREC_BF16X3)
                 return launch_seg_gemm_bf3<float, float, double>(
                     c, p->l32, (const float *)p->w, y, P, ch.d_t2[c], nt,
                     ch.d_s2, 0, p->w_max, p->y_len, p->p_max, scale, beta0,
-                    n_rhs, grid, st);
+                    n_rhs, grid, st, ch.x2[c]);
             return launch_seg_gemm<float, float, double, float, 2>(
                 c, p->l32, (const float *)p->w, y, P, ch.d_t2[c], nt,
                 ch.d_s2, 0, p->w_max, p->y_len, p->p_max, scale, beta0, n_rhs,
-                grid, st);
+                grid, st, ch.x2[c]);
         }));
         const int nr = (int)ch.red.size();
         if (nr > 0)

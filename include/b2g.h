@@ -140,6 +140,9 @@ int b2g_upload_plan(b2g_ctx ctx, b2g_buf lbuf, b2g_buf rbuf,
                     const b2g_block *yblocks, size_t n_yblocks, size_t x_len,
                     size_t y_len, unsigned flags, b2g_plan *out);
 int b2g_plan_destroy(b2g_plan plan);
+/* Re-derive the plan's reduced-precision shadows after the operator
+ * buffers' contents changed (same structure, new numbers).               */
+int b2g_plan_refresh(b2g_plan plan);
 /* Reference-order work of one H*psi, in multiply-adds
  * (sum over pairs of c_m*c_n*v_n + v_m*c_m*v_n, the reference's nflop,
  * src/core/batch_gemm.hpp:307).  FLOPs = 2 * this.                        */
@@ -180,6 +183,25 @@ int b2g_plan_stats(b2g_plan plan, uint64_t *n_tasks1, uint64_t *n_tasks2,
  * Replaces TensorFunctions::operator()(b, c, scale).                     */
 int b2g_hpsi(b2g_ctx ctx, b2g_plan plan, const double *x, double *y,
              int n_rhs, double scale, int accumulate, int precision);
+
+/* Partial expectations (the backward of E = sum_r <v_r| H |c_r>): the
+ * derivative of E with respect to every element of the left and right
+ * operator buffers, gl (lbuf length) and gr (rbuf length), FP64 device
+ * arrays, overwritten (accumulate = 0) or added to.  v: n_rhs stacked bra
+ * vectors (y_len each), c: n_rhs ket vectors (x_len each).  The
+ * derivative with respect to c is H^T v (b2g_hpsi of a symmetric H).
+ * Replaces the contraction behind
+ *   OperatorFunctions::tensor_left_partial_expectation / _right_
+ *                                  src/core/operator_functions.hpp:354-472
+ *   TensorFunctions::tensor_product_expectation
+ *                                  src/core/tensor_functions.hpp:1703-1878
+ * (per block instead of per operator: a caller sums the blocks of one
+ * operator, E_w = f_w <X_w, R_w>).  The gradient plan is built from the
+ * uploaded pair list on the first call and kept with the plan.          */
+int b2g_grad_ops(b2g_ctx ctx, b2g_plan plan, const double *v, const double *c,
+                 int n_rhs, double *gl, double *gr, int accumulate);
+/* algorithmic FLOPs of one right-hand side of b2g_grad_ops              */
+int b2g_grad_flops(b2g_plan plan, double *flops);
 
 /* Diagonal of H into d (device, y_len doubles, overwritten).            */
 int b2g_diag(b2g_ctx ctx, b2g_buf lbuf, b2g_buf rbuf,
@@ -275,6 +297,24 @@ int b2g_effham_hpsi(b2g_effham h, const double *x, double *y, int n_rhs,
                     double scale, int accumulate, int precision);
 /* Copy the diagonal (device, bra_len doubles).                            */
 int b2g_effham_diag(b2g_effham h, double *d);
+/* Replace the operator data (device FP64 arrays of the left / right
+ * buffer lengths; same structure): copies them in, refreshes the
+ * reduced-precision shadows and the diagonal.  For callers that own the
+ * numbers (e.g. an autograd graph) while the plan stays.                 */
+int b2g_effham_set_data(b2g_effham h, const double *left, const double *right);
+/* The effective Hamiltonian's own device operator buffers and lengths.   */
+int b2g_effham_data(b2g_effham h, double **left, size_t *left_len,
+                    double **right, size_t *right_len);
+/* Davidson driver of b2g_effham_eigs for one root: mode 1 (default)
+ * captures one iteration (H*psi of the newest vector, subspace matrix,
+ * Ritz pairs by device Jacobi, rotation, residual, Olsen, restart,
+ * orthogonalisation, append) into a hipGraph and replays it, the host
+ * reading a status header per iteration; mode 0 runs the host-driven loop
+ * (also used for k > 1 roots and while plan timing is enabled).           */
+int b2g_effham_set_davidson_mode(b2g_effham h, int mode);
+/* b2g_grad_ops on the effective Hamiltonian's plan.                      */
+int b2g_effham_grad_ops(b2g_effham h, const double *v, const double *c,
+                        int n_rhs, double *gl, double *gr, int accumulate);
 /* Davidson (IterativeMatrixFunctions::davidson,
  * src/core/iterative_matrix_functions.hpp:866-1173, Olsen preconditioner):
  * k roots from k initial device vectors `x` (overwritten with the Ritz

@@ -106,3 +106,20 @@ def test_davidson_bf16x3_with_refinement(heff):
     e, nd, nflop, td = h.eigs(x, conv_thrd=1e-10, precision="bf16x3",
                               refine=True)
     assert abs(e - float(pb.meta["dav_e"])) < 1e-8, (name, e)
+
+
+def test_davidson_graph_and_host_loop_agree(heff):
+    """The graph-captured device iteration (default for one root) and the
+    host-driven loop reach the reference energy; their energies agree."""
+    name, pb, h = heff
+    out = {}
+    for mode in (0, 1):
+        h.set_davidson_mode(mode)
+        x = dev(pb.golden["psi"])
+        e, nd, nflop, td = h.eigs(x, conv_thrd=1e-10)
+        out[mode] = (e, nd, x.cpu().numpy())
+        assert abs(e - float(pb.meta["dav_e"])) < 1e-8, (name, mode, e)
+    h.set_davidson_mode(1)
+    assert abs(out[0][0] - out[1][0]) < 1e-10, name
+    ov = abs(float(out[0][2] @ out[1][2]))
+    assert 1 - ov < 1e-8, (name, ov)

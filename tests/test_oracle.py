@@ -108,6 +108,38 @@ def test_planted_mistake_is_caught():
     assert rel(y, ref) > 1e-6
 
 
+@pytest.mark.parametrize("name", ["n2_su2", "hub8_sz"])
+def test_grad_ops_linearity_and_euler(name):
+    """E(L, R) = <v|H(L, R)|c> is linear in the left-operator data for fixed
+    right data and vice versa, so the oracle's partial expectations must give
+    E(L + dL, R) - E(L, R) = <g_L, dL> exactly (no finite-difference error),
+    the same for R, and Euler's identity E = <g_L, L> = <g_R, R>.  A sign,
+    index or transpose slip in either gradient fails these."""
+    import copy
+    pb = load(name)
+    rng = np.random.default_rng(17)
+    orc = O.Oracle(pb)
+    v = rng.standard_normal(orc.bra_len)
+    c = rng.standard_normal(orc.ket_len)
+    e0 = float(v @ orc.hpsi(c))
+    gl, gr = orc.grad_ops(v, c)
+    tol = 1e-10 * max(1.0, abs(e0))
+    assert abs(float(gl @ pb.lops.data) - e0) < tol
+    assert abs(float(gr @ pb.rops.data) - e0) < tol
+    for side in ("l", "r"):
+        pb2 = copy.copy(pb)
+        ops = copy.copy(pb.lops if side == "l" else pb.rops)
+        d = rng.standard_normal(len(ops.data))
+        ops.data = ops.data + d
+        if side == "l":
+            pb2.lops = ops
+        else:
+            pb2.rops = ops
+        e1 = float(v @ O.Oracle(pb2).hpsi(c))
+        g = gl if side == "l" else gr
+        assert abs((e1 - e0) - float(g @ d)) < 1e-10 * max(1.0, abs(e1)), side
+
+
 def test_reference_energies_known_answers():
     with open(fixture_path("energies").replace(".npz", ".json")) as f:
         en = json.load(f)
