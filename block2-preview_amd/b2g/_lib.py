@@ -30,6 +30,7 @@ class b2g_pair(C.Structure):
                 ("c_m", C.c_uint32), ("c_n", C.c_uint32),
                 ("v_m", C.c_uint32), ("v_n", C.c_uint32),
                 ("conj", C.c_uint32), ("kind", C.c_uint32),
+                ("c_ld", C.c_uint32), ("v_ld", C.c_uint32),
                 ("alpha", C.c_double)]
 
 
@@ -40,7 +41,8 @@ class b2g_block(C.Structure):
 class b2g_diag_term(C.Structure):
     _fields_ = [("a_off", C.c_uint64), ("b_off", C.c_uint64),
                 ("d_off", C.c_uint64), ("a_n", C.c_uint32),
-                ("b_n", C.c_uint32), ("alpha", C.c_double)]
+                ("b_n", C.c_uint32), ("r0", C.c_uint32), ("c0", C.c_uint32),
+                ("alpha", C.c_double)]
 
 
 class b2g_sparse_info(C.Structure):
@@ -51,13 +53,21 @@ class b2g_sparse_info(C.Structure):
 
 class b2g_operator(C.Structure):
     _fields_ = [("delta_quantum", C.c_uint64), ("factor", C.c_double),
-                ("info", C.c_int32), ("pad", C.c_int32),
+                ("info", C.c_int32), ("flags", C.c_int32),
                 ("data_off", C.c_uint64)]
 
 
 class b2g_term(C.Structure):
     _fields_ = [("a", C.c_int32), ("b", C.c_int32), ("conj", C.c_uint32),
-                ("pad", C.c_uint32), ("factor", C.c_double)]
+                ("kind", C.c_uint32), ("factor", C.c_double),
+                ("da", C.c_int32), ("db", C.c_int32), ("dconj", C.c_uint32),
+                ("pad", C.c_uint32)]
+
+
+class b2g_tp_conn(C.Structure):
+    _fields_ = [("n", C.c_int32 * 5), ("nc", C.c_int32), ("quanta", u64p),
+                ("idx", u32p), ("ia", u32p), ("ib", u32p), ("ic", u32p),
+                ("stride", u64p), ("factor", f64p)]
 
 
 class b2g_problem(C.Structure):
@@ -73,11 +83,25 @@ class b2g_problem(C.Structure):
                 ("right_ops", C.POINTER(b2g_operator)),
                 ("terms", C.POINTER(b2g_term)),
                 ("left_data", f64p), ("right_data", f64p),
-                ("left_len", C.c_size_t), ("right_len", C.c_size_t)]
+                ("left_len", C.c_size_t), ("right_len", C.c_size_t),
+                ("delayed_side", C.c_int32), ("n_dl_infos", C.c_int32),
+                ("n_dr_infos", C.c_int32), ("n_dl_ops", C.c_int32),
+                ("n_dr_ops", C.c_int32), ("pad_delayed", C.c_int32),
+                ("dl_infos", C.POINTER(b2g_sparse_info)),
+                ("dr_infos", C.POINTER(b2g_sparse_info)),
+                ("dl_ops", C.POINTER(b2g_operator)),
+                ("dr_ops", C.POINTER(b2g_operator)),
+                ("tp", C.POINTER(b2g_tp_conn))]
 
 
 # every symbol declared in include/b2g.h, with (restype, argtypes)
 _vp = C.c_void_p
+
+# matvec hook (b2g.h b2g_matvec_fn / b2g_kernel_fn)
+MATVEC_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                        C.c_double)
+KERNEL_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_double, MATVEC_FN,
+                        C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t)
 
 # kernel ids of one H*psi (b2g.h): phase-1 / phase-2 grouped GEMM per tile
 # configuration c = 3*i + j (TM = 32 << i, TN = 32 << j), split-K reduction
@@ -93,6 +117,7 @@ def kernel_name(k):
 
 _SIGS = {
     "b2g_last_error": (C.c_char_p, []),
+    "b2g_abi_sizes": (C.c_int, [C.POINTER(C.c_size_t), C.c_int]),
     "b2g_ctx_create": (C.c_int, [C.c_int, C.c_uint, C.POINTER(_vp)]),
     "b2g_ctx_destroy": (C.c_int, [_vp]),
     "b2g_ctx_synchronize": (C.c_int, [_vp]),
@@ -147,6 +172,7 @@ _SIGS = {
     "b2g_grad_flops": (C.c_int, [_vp, f64p]),
     "b2g_plan_refresh": (C.c_int, [_vp]),
     "b2g_effham_set_davidson_mode": (C.c_int, [_vp, C.c_int]),
+    "b2g_effham_set_kernel": (C.c_int, [_vp, _vp, _vp]),
     "b2g_effham_eigs": (C.c_int, [_vp, _vp, C.c_int, C.c_double, C.c_int,
                                   C.c_int, C.c_int, C.c_int, C.c_int, f64p,
                                   C.POINTER(C.c_int), u64p, f64p]),
@@ -188,3 +214,7 @@ def check(code):
 def ptr(a, ctype):
     """ctypes pointer to a contiguous numpy array."""
     return a.ctypes.data_as(C.POINTER(ctype))
+
+
+ABI_STRUCTS = ["b2g_pair", "b2g_block", "b2g_diag_term", "b2g_sparse_info",
+               "b2g_operator", "b2g_term", "b2g_tp_conn", "b2g_problem"]

@@ -44,8 +44,9 @@ def _measure(ctx, d, cfg, target_macs=5e10, reps=3):
     pairs = (L.b2g_pair * n)()
     yb = (L.b2g_block * n)()
     for i in range(n):
-        pairs[i] = L.b2g_pair((i % 4) * blk, ((i * 3) % 4) * blk, i * blk,
-                              i * blk, d, d, d, d, 0, 0, 1.0)
+        pairs[i] = L.b2g_pair(a_off=(i % 4) * blk, b_off=((i * 3) % 4) * blk,
+                              c_off=i * blk, v_off=i * blk, c_m=d, c_n=d,
+                              v_m=d, v_n=d, alpha=1.0)
         yb[i] = L.b2g_block(i * blk, d, d)
     old = os.environ.get("B2G_TILE")
     os.environ["B2G_TILE"] = "%dx%d" % (tm, tn)

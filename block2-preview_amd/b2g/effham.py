@@ -51,6 +51,18 @@ def _dptr(t):
     return C.c_void_p(t.data_ptr())
 
 
+class EffectiveKernel:
+    """Hook around every Davidson matvec, as block2's EffectiveKernel
+    (src/dmrg/effective_hamiltonian.hpp:83-93; Python trampoline
+    src/pybind/pybind_dmrg.hpp:1107-1160).  compute(beta, f, a, b, xs)
+    replaces b = beta * H a; f(a, b, scale) is the library's matvec on
+    float64 device tensors.  The default calls f(a, b, beta).  Install with
+    EffectiveHamiltonian.eff_kernel = kernel."""
+
+    def compute(self, beta, f, a, b, xs):
+        f(a, b, beta)
+
+
 class EffectiveHamiltonian:
     """Two-site effective Hamiltonian of one DMRG site on the GPU.
 
@@ -200,6 +212,48 @@ class EffectiveHamiltonian:
     def connections(self, which=0):
         """Host copy of the connection list (which 0: wfn, 1: diag)."""
         return connections(self.h, which)
+
+    @property
+    def eff_kernel(self):
+        return getattr(self, "_kernel", None)
+
+    @eff_kernel.setter
+    def eff_kernel(self, kernel):
+        """Install (or with None remove) an EffectiveKernel hook used by
+        eigs (the host-driven Davidson loop runs while one is set)."""
+        self._kernel = kernel
+        if kernel is None:
+            self._kernel_cb = None
+            L.check(self.lib.b2g_effham_set_kernel(self.h, None, None))
+            return
+        ctx, lib = self.ctx, self.lib
+
+        def trampoline(user, beta, f, f_data, a_ptr, b_ptr, n):
+            try:
+                dv = ctx.device
+                a = torch.empty(n, dtype=torch.float64, device=dv)
+                b = torch.zeros(n, dtype=torch.float64, device=dv)
+                L.check(lib.b2g_vec_copy(ctx.h, C.c_void_p(a_ptr), _dptr(a), n))
+                ctx.synchronize()
+
+                def matvec(x, y, scale):
+                    torch.cuda.current_stream(dv).synchronize()
+                    L.check(f(f_data, _dptr(x), _dptr(y), float(scale)))
+                    ctx.synchronize()
+
+                kernel.compute(beta, matvec, a, b, [])
+                torch.cuda.current_stream(dv).synchronize()
+                L.check(lib.b2g_vec_copy(ctx.h, _dptr(b), C.c_void_p(b_ptr), n))
+                ctx.synchronize()
+                return 0
+            except Exception:  # reported as B2G_ERR_STATE by the library
+                import traceback
+                traceback.print_exc()
+                return 1
+
+        self._kernel_cb = L.KERNEL_FN(trampoline)
+        L.check(self.lib.b2g_effham_set_kernel(
+            self.h, C.cast(self._kernel_cb, C.c_void_p), None))
 
     def set_davidson_mode(self, mode):
         """1: graph-captured device Davidson iteration (default, one root);

@@ -48,11 +48,14 @@ class InfoSet:
 
 
 class OpSet:
-    def __init__(self, dq, factor, info, off, data):
+    def __init__(self, dq, factor, info, off, data, flags=None):
         self.dq = np.ascontiguousarray(dq, dtype=np.uint64)
         self.factor = np.ascontiguousarray(factor, dtype=np.float64)
         self.info = np.ascontiguousarray(info, dtype=np.int32)
         self.off = np.ascontiguousarray(off, dtype=np.uint64)
+        # bit 0: structure only (a delayed operator never formed)
+        self.flags = (np.zeros(len(self.dq), np.int32) if flags is None else
+                      np.ascontiguousarray(flags, dtype=np.int32))
         # None: the operator data live only on the device (see to_c)
         self.data = (None if data is None else
                      np.ascontiguousarray(data, dtype=np.float64))
@@ -80,6 +83,18 @@ class Problem:
         self.term_factor = np.ascontiguousarray(term_factor, dtype=np.float64)
         self.meta = meta or {}
         self._keep = []
+        # delayed (three-tensor) strings: term_kind 1 uses sub-operators
+        # dlops[term_da], drops[term_db] of the delayed tensor on side
+        # delayed_side (1 left, 2 right; their data sit in that side's
+        # buffer) and the tensor-product lists tp[i] of that side's infos
+        nt = len(self.term_conj)
+        self.term_kind = np.zeros(nt, np.uint8)
+        self.term_da = np.full(nt, -1, np.int32)
+        self.term_db = np.full(nt, -1, np.int32)
+        self.term_dconj = np.zeros(nt, np.uint8)
+        self.delayed_side = 0
+        self.dlinfo = self.drinfo = self.dlops = self.drops = None
+        self.tp = []
 
     @property
     def ket_len(self):
@@ -107,10 +122,49 @@ class Problem:
                          z[p + "op_off"], z[p + "op_data"])
 
         sym = 1 if meta["sym"] == "su2" else 0
+        lops, rops = ops("l"), ops("r")
+        for o, p in ((lops, "l"), (rops, "r")):
+            if p + "op_flags" in z.files:
+                o.flags = np.ascontiguousarray(z[p + "op_flags"], np.int32)
+        side = int(meta.get("delayed_side", 0))
+        if side:
+            # sub-operator data appended to the delayed side's buffer
+            host = lops if side == 1 else rops
+            base = len(host.data)
+            dl, dr = ops("dl"), ops("dr")
+            dl.off = dl.off + np.uint64(base)
+            dr.off = dr.off + np.uint64(base + len(dl.data))
+            host.data = np.concatenate([host.data, dl.data, dr.data])
         pb = Problem(sym, int(meta["opdq"]), int(meta["hop_left_vacuum"]),
                      info("ket"), info("bra"), info("linfo"), info("rinfo"),
-                     ops("l"), ops("r"), z["term_conj"], z["term_a"],
+                     lops, rops, z["term_conj"], z["term_a"],
                      z["term_b"], z["term_factor"], meta)
+        if "term_kind" in z.files:
+            pb.term_kind = np.ascontiguousarray(z["term_kind"], np.uint8)
+            pb.term_da = np.ascontiguousarray(z["term_da"], np.int32)
+            pb.term_db = np.ascontiguousarray(z["term_db"], np.int32)
+            pb.term_dconj = np.ascontiguousarray(z["term_dconj"], np.uint8)
+        if side:
+            pb.delayed_side = side
+            pb.dlinfo, pb.drinfo = info("dlinfo"), info("drinfo")
+            pb.dlops, pb.drops = dl, dr
+            tn = z["tp_n"].reshape(-1, 5)
+            tnc = z["tp_nc"]
+            qo = co = 0
+            for i in range(len(tnc)):
+                nq = max(int(tn[i, 4]), 0)
+                nc = int(tnc[i])
+                pb.tp.append(dict(
+                    n=tn[i].astype(np.int32), nc=nc,
+                    quanta=np.ascontiguousarray(z["tp_quanta"][qo:qo + nq]),
+                    idx=np.ascontiguousarray(z["tp_idx"][qo:qo + nq]),
+                    ia=np.ascontiguousarray(z["tp_ia"][co:co + nc]),
+                    ib=np.ascontiguousarray(z["tp_ib"][co:co + nc]),
+                    ic=np.ascontiguousarray(z["tp_ic"][co:co + nc]),
+                    stride=np.ascontiguousarray(z["tp_stride"][co:co + nc]),
+                    factor=np.ascontiguousarray(z["tp_factor"][co:co + nc])))
+                qo += nq
+                co += nc
         pb.golden = {k: z[k] for k in z.files}
         return pb
 
@@ -136,18 +190,20 @@ class Problem:
             *[self._sparse_info(self.linfo, i) for i in range(len(self.linfo))])
         rin = (L.b2g_sparse_info * max(len(self.rinfo), 1))(
             *[self._sparse_info(self.rinfo, i) for i in range(len(self.rinfo))])
-        lo = (L.b2g_operator * max(len(self.lops), 1))(
-            *[L.b2g_operator(int(self.lops.dq[k]), float(self.lops.factor[k]),
-                             int(self.lops.info[k]), 0, int(self.lops.off[k]))
-              for k in range(len(self.lops))])
-        ro = (L.b2g_operator * max(len(self.rops), 1))(
-            *[L.b2g_operator(int(self.rops.dq[k]), float(self.rops.factor[k]),
-                             int(self.rops.info[k]), 0, int(self.rops.off[k]))
-              for k in range(len(self.rops))])
+        def op_arr(o):
+            return (L.b2g_operator * max(len(o), 1))(
+                *[L.b2g_operator(int(o.dq[k]), float(o.factor[k]),
+                                 int(o.info[k]), int(o.flags[k]),
+                                 int(o.off[k])) for k in range(len(o))])
+        lo, ro = op_arr(self.lops), op_arr(self.rops)
         nt = len(self.term_conj)
         tm = (L.b2g_term * max(nt, 1))(
-            *[L.b2g_term(int(self.term_a[t]), int(self.term_b[t]),
-                         int(self.term_conj[t]), 0, float(self.term_factor[t]))
+            *[L.b2g_term(a=int(self.term_a[t]), b=int(self.term_b[t]),
+                         conj=int(self.term_conj[t]),
+                         kind=int(self.term_kind[t]),
+                         factor=float(self.term_factor[t]),
+                         da=int(self.term_da[t]), db=int(self.term_db[t]),
+                         dconj=int(self.term_dconj[t]))
               for t in range(nt)])
         self._keep += [lin, rin, lo, ro, tm]
         p = L.b2g_problem()
@@ -175,5 +231,40 @@ class Problem:
             p.left_data = C.cast(C.c_void_p(lp), L.f64p)
             p.right_data = C.cast(C.c_void_p(rp), L.f64p)
             p.left_len, p.right_len = ll, rl
+        p.delayed_side = self.delayed_side
+        if self.delayed_side:
+            dli = (L.b2g_sparse_info * max(len(self.dlinfo), 1))(
+                *[self._sparse_info(self.dlinfo, i)
+                  for i in range(len(self.dlinfo))])
+            dri = (L.b2g_sparse_info * max(len(self.drinfo), 1))(
+                *[self._sparse_info(self.drinfo, i)
+                  for i in range(len(self.drinfo))])
+            dlo, dro = op_arr(self.dlops), op_arr(self.drops)
+            tps = (L.b2g_tp_conn * max(len(self.tp), 1))()
+            for i, t in enumerate(self.tp):
+                c = tps[i]
+                for k in range(5):
+                    c.n[k] = int(t["n"][k])
+                c.nc = int(t["nc"])
+                if c.nc > 0:
+                    arrs = [np.ascontiguousarray(t[k], dt) for k, dt in (
+                        ("quanta", np.uint64), ("idx", np.uint32),
+                        ("ia", np.uint32), ("ib", np.uint32),
+                        ("ic", np.uint32), ("stride", np.uint64),
+                        ("factor", np.float64))]
+                    self._keep += arrs
+                    c.quanta = L.ptr(arrs[0], C.c_uint64)
+                    c.idx = L.ptr(arrs[1], C.c_uint32)
+                    c.ia = L.ptr(arrs[2], C.c_uint32)
+                    c.ib = L.ptr(arrs[3], C.c_uint32)
+                    c.ic = L.ptr(arrs[4], C.c_uint32)
+                    c.stride = L.ptr(arrs[5], C.c_uint64)
+                    c.factor = L.ptr(arrs[6], C.c_double)
+            self._keep += [dli, dri, dlo, dro, tps]
+            p.n_dl_infos, p.n_dr_infos = len(self.dlinfo), len(self.drinfo)
+            p.n_dl_ops, p.n_dr_ops = len(self.dlops), len(self.drops)
+            p.dl_infos, p.dr_infos = dli, dri
+            p.dl_ops, p.dr_ops = dlo, dro
+            p.tp = tps
         self._keep.append(p)
         return p

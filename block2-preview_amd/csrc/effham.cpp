@@ -19,6 +19,7 @@
 #include <cstring>
 #include <memory>
 #include <numeric>
+#include <unordered_map>
 
 using namespace b2g;
 
@@ -35,6 +36,12 @@ struct b2g_effham_s {
     // Davidson driver: 1 graph-captured device iteration (one root),
     // 0 host-driven loop
     int dav_mode = 1;
+    // delayed strings present (their 1 x 1 sub-operator values are folded
+    // into the recorded factors, so the numbers cannot be replaced)
+    bool has_delayed = false;
+    // optional matvec hook (EffectiveKernel)
+    b2g_kernel_fn kernel = nullptr;
+    void *kernel_user = nullptr;
 };
 
 namespace b2g {
@@ -141,7 +148,8 @@ static int update_diag(b2g_effham h) {
     std::vector<b2g_diag_term> dt(h->drecs.size());
     for (size_t i = 0; i < dt.size(); i++) {
         const DiagRec &q = h->drecs[i];
-        dt[i] = b2g_diag_term{q.a_off, q.b_off, q.d_off, q.a_n, q.b_n, q.alpha};
+        dt[i] = b2g_diag_term{q.a_off, q.b_off, q.d_off, q.a_n,
+                              q.b_n,   q.r0,    q.c0,    q.alpha};
     }
     std::vector<b2g_block> db = blocks_of(h->prob.ket);
     return b2g_diag(h->ctx, h->lbuf, h->rbuf, dt.data(), dt.size(), db.data(),
@@ -198,7 +206,9 @@ extern "C" int b2g_effham_create(b2g_ctx ctx, const b2g_problem *pb,
             B2G_REQUIRE(infos[o.info].dq == o.delta_quantum, B2G_ERR_ARG,
                         std::string(what) +
                             " operator: delta quantum differs from its info");
-            B2G_REQUIRE(o.data_off + infos[o.info].total() <= len, B2G_ERR_ARG,
+            B2G_REQUIRE((o.flags & 1) ||
+                            o.data_off + infos[o.info].total() <= len,
+                        B2G_ERR_ARG,
                         std::string(what) + " operator: data out of range");
             dst.push_back(Operator{o.delta_quantum, o.factor, o.info,
                                    o.data_off});
@@ -210,12 +220,76 @@ extern "C" int b2g_effham_create(b2g_ctx ctx, const b2g_problem *pb,
         (r = take_ops(pb->right_ops, pb->n_right_ops, p.rinfos, pb->right_len,
                       p.rops, "right")) != B2G_OK)
         return r;
+    // delayed tensor: sub-operator sets and tensor-product connection lists
+    B2G_REQUIRE(pb->delayed_side >= 0 && pb->delayed_side <= 2, B2G_ERR_ARG,
+                "delayed_side must be 0, 1 or 2");
+    p.delayed_side = pb->delayed_side;
+    if (p.delayed_side != 0) {
+        B2G_REQUIRE(pb->n_dl_infos >= 0 && pb->n_dr_infos >= 0 &&
+                        pb->n_dl_ops >= 0 && pb->n_dr_ops >= 0 && pb->tp,
+                    B2G_ERR_ARG, "delayed tensor: bad counts or NULL tp");
+        for (int i = 0; i < pb->n_dl_infos; i++) {
+            if ((r = check_info(pb->dl_infos[i], "delayed left info")) != B2G_OK)
+                return r;
+            p.dlinfos.push_back(to_info(pb->dl_infos[i]));
+        }
+        for (int i = 0; i < pb->n_dr_infos; i++) {
+            if ((r = check_info(pb->dr_infos[i], "delayed right info")) !=
+                B2G_OK)
+                return r;
+            p.drinfos.push_back(to_info(pb->dr_infos[i]));
+        }
+        const size_t dlen =
+            p.delayed_side == 1 ? pb->left_len : pb->right_len;
+        if ((r = take_ops(pb->dl_ops, pb->n_dl_ops, p.dlinfos, dlen, p.dlops,
+                          "delayed left")) != B2G_OK ||
+            (r = take_ops(pb->dr_ops, pb->n_dr_ops, p.drinfos, dlen, p.drops,
+                          "delayed right")) != B2G_OK)
+            return r;
+        const std::vector<BlockInfo> &side =
+            p.delayed_side == 1 ? p.linfos : p.rinfos;
+        p.tp.resize(side.size());
+        for (size_t i = 0; i < side.size(); i++) {
+            const b2g_tp_conn &c = pb->tp[i];
+            Connections &d = p.tp[i];
+            if (c.nc <= 0)
+                continue;
+            B2G_REQUIRE(c.n[4] > 0 && c.quanta && c.idx && c.ia && c.ib &&
+                            c.ic && c.stride && c.factor,
+                        B2G_ERR_ARG, "tensor-product list: NULL arrays");
+            for (int k = 0; k < 5; k++)
+                d.n[k] = c.n[k];
+            d.quanta.assign(c.quanta, c.quanta + c.n[4]);
+            d.idx.assign(c.idx, c.idx + c.n[4]);
+            d.ia.assign(c.ia, c.ia + c.nc);
+            d.ib.assign(c.ib, c.ib + c.nc);
+            d.ic.assign(c.ic, c.ic + c.nc);
+            d.stride.assign(c.stride, c.stride + c.nc);
+            d.factor.assign(c.factor, c.factor + c.nc);
+            for (int k = 0; k < c.n[4]; k++)
+                B2G_REQUIRE(d.idx[k] <= (uint32_t)c.nc, B2G_ERR_ARG,
+                            "tensor-product list: class index out of range");
+            for (int k = 0; k < c.nc; k++)
+                B2G_REQUIRE(d.ic[k] < (uint32_t)side[i].n(), B2G_ERR_ARG,
+                            "tensor-product list: fused block out of range");
+        }
+    }
     for (int i = 0; i < pb->n_terms; i++) {
         const b2g_term &t = pb->terms[i];
         B2G_REQUIRE(t.a >= 0 && t.a < pb->n_left_ops && t.b >= 0 &&
-                        t.b < pb->n_right_ops && t.conj < 4,
+                        t.b < pb->n_right_ops && t.conj < 4 && t.kind < 2,
                     B2G_ERR_ARG, "term references a missing operator");
-        p.terms.push_back(Term{t.a, t.b, (uint8_t)t.conj, t.factor});
+        Term tm{t.a, t.b, (uint8_t)t.conj, t.factor};
+        if (t.kind == 1) {
+            B2G_REQUIRE(p.delayed_side != 0 && t.da >= 0 &&
+                            t.da < (int)p.dlops.size() && t.db >= 0 &&
+                            t.db < (int)p.drops.size() && t.dconj < 4,
+                        B2G_ERR_ARG,
+                        "delayed term references a missing sub-operator");
+            tm.three = true;
+            tm.da = t.da, tm.db = t.db, tm.dconj = (uint8_t)t.dconj;
+        }
+        p.terms.push_back(tm);
     }
     // connection lists (EffectiveHamiltonian ctor)
     SU2Coupling cg;
@@ -234,6 +308,42 @@ extern "C" int b2g_effham_create(b2g_ctx ctx, const b2g_problem *pb,
     B2G_REQUIRE(err.empty(), B2G_ERR_ARG, err);
     h->drecs = record_diag(p, h->dg, err);
     B2G_REQUIRE(err.empty(), B2G_ERR_ARG, err);
+    // fold the 1 x 1 delayed sub-operator values into the factors
+    std::vector<int64_t> soff;
+    for (auto &q : h->pairs)
+        if (q.scal_off >= 0)
+            soff.push_back(q.scal_off);
+    for (auto &q : h->drecs)
+        if (q.scal_off >= 0)
+            soff.push_back(q.scal_off);
+    if (!soff.empty()) {
+        const double *src =
+            p.delayed_side == 1 ? pb->left_data : pb->right_data;
+        std::sort(soff.begin(), soff.end());
+        soff.erase(std::unique(soff.begin(), soff.end()), soff.end());
+        std::unordered_map<int64_t, double> sval;
+        if (src != nullptr && (flags & B2G_EFFHAM_DEVICE_DATA)) {
+            B2G_REQUIRE(ctx, B2G_ERR_ARG, "device data needs a context");
+            std::vector<double> v(soff.size());
+            for (size_t k = 0; k < soff.size(); k++)
+                B2G_HIP(hipMemcpy(&v[k], src + soff[k], sizeof(double),
+                                  hipMemcpyDeviceToHost));
+            for (size_t k = 0; k < soff.size(); k++)
+                sval[soff[k]] = v[k];
+        } else if (src != nullptr) {
+            for (int64_t o : soff)
+                sval[o] = src[o];
+        }
+        h->has_delayed = true;
+        if (!sval.empty()) {
+            for (auto &q : h->pairs)
+                if (q.scal_off >= 0)
+                    q.alpha *= sval[q.scal_off];
+            for (auto &q : h->drecs)
+                if (q.scal_off >= 0)
+                    q.alpha *= sval[q.scal_off];
+        }
+    }
     h->macs = 0;
     for (auto &q : h->pairs)
         h->macs += pair_macs(q);
@@ -280,7 +390,8 @@ extern "C" int b2g_effham_create(b2g_ctx ctx, const b2g_problem *pb,
     for (size_t i = 0; i < dp.size(); i++) {
         const PairRec &q = h->pairs[i];
         dp[i] = b2g_pair{q.a_off, q.b_off, q.c_off, q.v_off, q.c_m, q.c_n,
-                         q.v_m, q.v_n, q.conj, 0, q.alpha};
+                         q.v_m, q.v_n, q.conj, q.kind, q.c_ld, q.v_ld,
+                         q.alpha};
     }
     std::vector<b2g_block> yb = blocks_of(p.bra);
     r = b2g_upload_plan(ctx, h->lbuf, h->rbuf, dp.data(), dp.size(), yb.data(),
@@ -358,7 +469,8 @@ extern "C" int b2g_effham_pairs(b2g_effham h, uint64_t *n_pairs,
         for (size_t i = 0; i < h->pairs.size(); i++) {
             const PairRec &q = h->pairs[i];
             pairs[i] = b2g_pair{q.a_off, q.b_off, q.c_off, q.v_off, q.c_m,
-                                q.c_n, q.v_m, q.v_n, q.conj, 0, q.alpha};
+                                q.c_n, q.v_m, q.v_n, q.conj, q.kind,
+                                q.c_ld, q.v_ld, q.alpha};
         }
     return B2G_OK;
 }
@@ -388,6 +500,9 @@ extern "C" int b2g_effham_set_data(b2g_effham h, const double *left,
                                    const double *right) {
     B2G_REQUIRE(h && left && right, B2G_ERR_ARG, "NULL arg");
     B2G_REQUIRE(h->plan, B2G_ERR_STATE, "effham was built host-only");
+    B2G_REQUIRE(!h->has_delayed, B2G_ERR_STATE,
+                "operator data of a problem with delayed strings are part "
+                "of its plan; create a new effective Hamiltonian");
     B2G_HIP(hipSetDevice(h->ctx->device));
     if (h->lbuf->n)
         B2G_HIP(hipMemcpyAsync(h->lbuf->ptr, left, h->lbuf->n * sizeof(double),
@@ -414,6 +529,14 @@ extern "C" int b2g_effham_data(b2g_effham h, double **left, size_t *left_len,
     return B2G_OK;
 }
 
+extern "C" int b2g_effham_set_kernel(b2g_effham h, b2g_kernel_fn kernel,
+                                     void *user) {
+    B2G_REQUIRE(h, B2G_ERR_ARG, "NULL effham");
+    h->kernel = kernel;
+    h->kernel_user = user;
+    return B2G_OK;
+}
+
 extern "C" int b2g_effham_set_davidson_mode(b2g_effham h, int mode) {
     B2G_REQUIRE(h && (mode == 0 || mode == 1), B2G_ERR_ARG, "bad mode");
     h->dav_mode = mode;
@@ -425,6 +548,9 @@ extern "C" int b2g_effham_grad_ops(b2g_effham h, const double *v,
                                    double *gr, int accumulate) {
     B2G_REQUIRE(h, B2G_ERR_ARG, "NULL effham");
     B2G_REQUIRE(h->plan, B2G_ERR_STATE, "effham was built host-only");
+    B2G_REQUIRE(!h->has_delayed, B2G_ERR_STATE,
+                "gradients of problems with delayed strings are not formed "
+                "(their 1 x 1 sub-operator values are plan constants)");
     return b2g_grad_ops(h->ctx, h->plan, v, c, n_rhs, gl, gr, accumulate);
 }
 
@@ -451,6 +577,29 @@ struct DavWork {
 static int dev_dot(b2g_ctx ctx, const double *x, const double *y, size_t n,
                    double &out) {
     return b2g_vec_dot(ctx, x, y, n, &out);
+}
+
+struct MatvecData {
+    b2g_effham h;
+    int precision;
+};
+
+static int default_matvec(void *d, const double *a, double *b, double scale) {
+    const MatvecData *m = (const MatvecData *)d;
+    return b2g_hpsi(m->h->ctx, m->h->plan, a, b, 1, scale, 0, m->precision);
+}
+
+// b = H a, through the user hook when one is set
+static int davidson_matvec(b2g_effham h, int precision, const double *a,
+                           double *b) {
+    if (h->kernel == nullptr)
+        return b2g_hpsi(h->ctx, h->plan, a, b, 1, 1.0, 0, precision);
+    B2G_HIP(hipStreamSynchronize(h->ctx->stream));
+    MatvecData md{h, precision};
+    const int r = h->kernel(h->kernel_user, 1.0, default_matvec, &md, a, b,
+                            h->prob.ket.total());
+    B2G_REQUIRE(r == 0, B2G_ERR_STATE, "the matvec hook returned an error");
+    return B2G_OK;
 }
 
 // One Davidson solve (IterativeMatrixFunctions::davidson,
@@ -512,7 +661,7 @@ static int davidson(b2g_effham h, double *x, int k, double conv_thrd,
     while (xiter < max_iter) {
         xiter++;
         for (int i = msig; i < m; i++, msig++) {
-            int r = b2g_hpsi(ctx, h->plan, bv(i), sv(i), 1, 1.0, 0, precision);
+            int r = davidson_matvec(h, precision, bv(i), sv(i));
             if (r)
                 return r;
             nmv++;
@@ -718,7 +867,8 @@ extern "C" int b2g_effham_eigs(b2g_effham h, double *x, int k,
     std::vector<double> ev;
     int nd = 0, nd2 = 0;
     uint64_t nmv = 0, nmv2 = 0;
-    const bool graph = k == 1 && h->dav_mode == 1 && !h->plan->timing;
+    const bool graph = k == 1 && h->dav_mode == 1 && !h->plan->timing &&
+                       h->kernel == nullptr;
     auto solve = [&](int prec, int &ndv, uint64_t &nm) {
         return graph ? davidson_graph(h, x, conv_thrd, max_iter, def_min,
                                       def_max, prec, ev, ndv, nm)

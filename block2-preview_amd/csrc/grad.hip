@@ -11,6 +11,8 @@
 //         = alpha W V^T          (A stored c_m x v_m, conj bit 0)
 //   dE/dB = alpha Z^T C          (B stored v_n x c_n)   Z = op(A)^T V
 //         = alpha C^T Z          (B stored c_n x v_n, conj bit 1)
+// C and V may be windows with row strides c_ld / v_ld (three-tensor
+// pairs); their factors are constants of the plan.
 // Stage 1 computes every distinct W and Z once into scratch; stage 2 sums,
 // per operator block, all its pairs' products along K (output-stationary,
 // no atomics, deterministic), in four launches split by transpose and by
@@ -125,7 +127,11 @@ static int grad_plan_build(b2g_plan p, GradPlan **out) {
     size_t i0 = 0;
     while (i0 < n) {
         GradChunk ch;
-        typedef std::tuple<uint64_t, uint64_t, uint32_t> Key;
+        // (operator block, vector window start, transpose bit, window rows,
+        // window columns, row stride)
+        typedef std::tuple<uint64_t, uint64_t, uint32_t, uint32_t, uint32_t,
+                           uint32_t>
+            Key;
         std::map<Key, uint64_t> wmap, zmap;
         std::map<size_t, std::pair<uint64_t, uint64_t>> pwz; // pair -> (W, Z)
         size_t i1 = i0;
@@ -133,19 +139,22 @@ static int grad_plan_build(b2g_plan p, GradPlan **out) {
             const b2g_pair &q = pairs[i1];
             if (q.c_m == 0 || q.c_n == 0 || q.v_m == 0 || q.v_n == 0)
                 continue;
-            const Key kw(q.b_off, q.c_off, q.conj & 2);
-            const Key kz(q.a_off, q.v_off, q.conj & 1);
+            const Key kw(q.b_off, q.c_off, q.conj & 2, q.c_m, q.c_n, q.c_ld);
+            const Key kz(q.a_off, q.v_off, q.conj & 1, q.v_m, q.v_n, q.v_ld);
             const uint64_t sz = (uint64_t)q.c_m * q.v_n;
             const uint64_t need = (wmap.count(kw) ? 0 : sz) +
                                   (zmap.count(kz) ? 0 : sz);
             if (ch.w_len + ch.z_len + need > cap && i1 > i0)
                 break;
+            // row strides of windowed C / V (three-tensor pairs)
+            const uint32_t cld = q.c_ld ? q.c_ld : q.c_n;
+            const uint32_t vld = q.v_ld ? q.v_ld : q.v_n;
             auto iw = wmap.find(kw);
             if (iw == wmap.end()) {
                 // W = C op(B)^T   (c_m x v_n), as phase 1 of H*psi
                 const bool tb = !(q.conj & 2);
                 const uint32_t si = (uint32_t)ch.w.s.size();
-                ch.w.s.push_back(make_seg(q.c_off, q.c_n, false, q.b_off,
+                ch.w.s.push_back(make_seg(q.c_off, cld, false, q.b_off,
                                           tb ? q.c_n : q.v_n, tb, q.c_n, 1.0));
                 add_tiles(ch.w.t, ch.w_len, q.c_m, q.v_n, q.c_n, si, si + 1, 0);
                 iw = wmap.emplace(kw, ch.w_len).first;
@@ -158,7 +167,7 @@ static int grad_plan_build(b2g_plan p, GradPlan **out) {
                 const bool ta = !(q.conj & 1);
                 const uint32_t si = (uint32_t)ch.z.s.size();
                 ch.z.s.push_back(make_seg(q.a_off, ta ? q.c_m : q.v_m, ta,
-                                          q.v_off, q.v_n, false, q.v_m, 1.0));
+                                          q.v_off, vld, false, q.v_m, 1.0));
                 add_tiles(ch.z.t, ch.z_len, q.c_m, q.v_n, q.v_m, si, si + 1, 0);
                 iz = zmap.emplace(kz, ch.z_len).first;
                 ch.z_len += sz;
@@ -177,29 +186,33 @@ static int grad_plan_build(b2g_plan p, GradPlan **out) {
         add_pass(ch.l0, a0, pairs,
                  [&](const b2g_pair &q, size_t i, uint32_t &r, uint32_t &c) {
                      r = q.v_m, c = q.c_m;
-                     return make_seg(q.v_off, q.v_n, false, pwz[i].first,
-                                     q.v_n, true, q.v_n, q.alpha);
+                     return make_seg(q.v_off, q.v_ld ? q.v_ld : q.v_n, false,
+                                     pwz[i].first, q.v_n, true, q.v_n,
+                                     q.alpha);
                  });
         // dA = alpha W V^T  (left W from scratch, right V^T)
         add_pass(ch.l1, a1, pairs,
                  [&](const b2g_pair &q, size_t i, uint32_t &r, uint32_t &c) {
                      r = q.c_m, c = q.v_m;
                      return make_seg(pwz[i].first, q.v_n, false, q.v_off,
-                                     q.v_n, true, q.v_n, q.alpha);
+                                     q.v_ld ? q.v_ld : q.v_n, true, q.v_n,
+                                     q.alpha);
                  });
         // dB = alpha Z^T C  (left Z^T from scratch, right C from c)
         add_pass(ch.r0, b0, pairs,
                  [&](const b2g_pair &q, size_t i, uint32_t &r, uint32_t &c) {
                      r = q.v_n, c = q.c_n;
                      return make_seg(pwz[i].second, q.v_n, true, q.c_off,
-                                     q.c_n, false, q.c_m, q.alpha);
+                                     q.c_ld ? q.c_ld : q.c_n, false, q.c_m,
+                                     q.alpha);
                  });
         // dB = alpha C^T Z  (left C^T from c, right Z from scratch)
         add_pass(ch.r1, b1, pairs,
                  [&](const b2g_pair &q, size_t i, uint32_t &r, uint32_t &c) {
                      r = q.c_n, c = q.v_n;
-                     return make_seg(q.c_off, q.c_n, true, pwz[i].second,
-                                     q.v_n, false, q.c_m, q.alpha);
+                     return make_seg(q.c_off, q.c_ld ? q.c_ld : q.c_n, true,
+                                     pwz[i].second, q.v_n, false, q.c_m,
+                                     q.alpha);
                  });
         for (Launch *L : {&ch.w, &ch.z, &ch.l0, &ch.l1, &ch.r0, &ch.r1})
             for (int c = 0; c < kTileConfigs; c++) {

@@ -356,8 +356,11 @@ __global__ __launch_bounds__(256) void diag_kernel(
             double s = 0.0;
             for (uint32_t t = blk.t_begin; t < blk.t_end; t++) {
                 const DiagTermDev tm = terms[t];
-                s += tm.alpha * L[tm.a_off + (uint64_t)i * (tm.a_n + 1)] *
-                     R[tm.b_off + (uint64_t)j * (tm.b_n + 1)];
+                // the term's a_n x b_n window starts at (r0, c0)
+                const uint32_t ii = i - tm.r0, jj = j - tm.c0;
+                if (ii < tm.a_n && jj < tm.b_n)
+                    s += tm.alpha * L[tm.a_off + (uint64_t)ii * (tm.a_n + 1)] *
+                         R[tm.b_off + (uint64_t)jj * (tm.b_n + 1)];
             }
             Dv[blk.d_off + e] = s;
         }

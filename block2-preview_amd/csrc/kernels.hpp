@@ -60,6 +60,7 @@ struct DiagBlock {
 struct DiagTermDev {
     uint64_t a_off, b_off;
     uint32_t a_n, b_n;
+    uint32_t r0, c0; // window origin inside the output block
     double alpha;
 };
 

@@ -256,6 +256,151 @@ static bool term_class(const Problem &p, const Term &t, const Connections &ci,
     return ci.find_class(t.conj, abdq, b, e);
 }
 
+// The delayed operator's tensor-product entries of a three-tensor string:
+// the class (dconj, dcdq.combine(op(dl).q, -op(dr).q)) of the delayed
+// operator's info (three_tensor_product_multiply, operator_functions.hpp:
+// 576-600).  Returns false with a message if absent.
+static bool delayed_class(const Problem &p, const Term &t,
+                          const Connections *&dinfo, int &b, int &e,
+                          double &dscale, std::string &err) {
+    const Symm s = p.symm;
+    const bool dleft = p.delayed_side == 1;
+    const Operator &dc = dleft ? p.lops[t.a] : p.rops[t.b];
+    const Operator &da = p.dlops[t.da], &db = p.drops[t.db];
+    if ((size_t)dc.info >= p.tp.size() || p.tp[dc.info].nc() == 0 ||
+        p.tp[dc.info].n[4] < 0) {
+        err = "delayed string without a tensor-product connection list";
+        return false;
+    }
+    dinfo = &p.tp[dc.info];
+    const uint64_t q = q_combine(s, dc.dq, (t.dconj & 1) ? q_neg(s, da.dq) : da.dq,
+                                 (t.dconj & 2) ? db.dq : q_neg(s, db.dq));
+    if (!dinfo->find_class(t.dconj, q, b, e)) {
+        err = "delayed string: class absent from the tensor-product list";
+        return false;
+    }
+    dscale = da.factor * db.factor;
+    return true;
+}
+
+// block (dims, element offset) of sub-operator o in its info
+struct SubBlock {
+    uint32_t m, n;
+    uint64_t off;
+};
+static SubBlock sub_block(const std::vector<BlockInfo> &infos,
+                          const Operator &o, uint32_t i) {
+    const BlockInfo &bi = infos[o.info];
+    return SubBlock{bi.nbra[i], bi.nket[i], o.data_off + bi.shift[i]};
+}
+
+// pairs of one three-tensor string (GMatrixFunctions::three_rotate,
+// src/core/matrix_functions.hpp:1046-1112, re-expressed as one rotate on
+// windows of C and V)
+static bool record_three(const Problem &p, const Term &t, size_t it,
+                         const Connections &wfn, int b, int e, double scale,
+                         std::vector<PairRec> &out, std::string &err) {
+    const Connections *dinfo = nullptr;
+    int db_ = 0, de_ = 0;
+    double dscale = 1.0;
+    if (!delayed_class(p, t, dinfo, db_, de_, dscale, err))
+        return false;
+    scale *= dscale;
+    if (std::abs(scale) < kTiny)
+        return true;
+    const bool dleft = p.delayed_side == 1;
+    const Operator &la = p.lops[t.a], &rb = p.rops[t.b];
+    const Operator &oda = p.dlops[t.da], &odb = p.drops[t.db];
+    const BlockInfo &ai = p.linfos[la.info], &bi = p.rinfos[rb.info];
+    auto fail = [&](const char *what, int il) {
+        std::ostringstream ss;
+        ss << "term " << it << " connection " << il << ": " << what;
+        err = ss.str();
+        return false;
+    };
+    for (int il = b; il < e; il++) {
+        const uint32_t ia = wfn.ia[il], ib = wfn.ib[il], ic = wfn.ic[il];
+        const uint64_t iv = wfn.stride[il];
+        const uint32_t idc = dleft ? ia : ib;
+        const uint32_t *lo = std::lower_bound(dinfo->ic.data() + db_,
+                                              dinfo->ic.data() + de_, idc);
+        for (int idl = (int)(lo - dinfo->ic.data()); idl < de_ &&
+                                                     dinfo->ic[idl] == idc;
+             idl++) {
+            const SubBlock da = sub_block(p.dlinfos, oda, dinfo->ia[idl]);
+            const SubBlock db = sub_block(p.drinfos, odb, dinfo->ib[idl]);
+            const uint64_t stride = dinfo->stride[idl];
+            const bool sa = da.m == 1 && da.n == 1, sb = db.m == 1 && db.n == 1;
+            if (!sa && !sb)
+                return fail("neither delayed sub-operator block is 1 x 1", il);
+            PairRec r;
+            r.kind = 1;
+            r.alpha = scale * wfn.factor[il] * dinfo->factor[idl];
+            r.scal_off = (int64_t)(sa ? da.off : db.off);
+            const uint32_t cm_full = p.ket.nbra[ic], cn_full = p.ket.nket[ic];
+            const uint32_t vm_full = p.bra.nbra[iv], vn_full = p.bra.nket[iv];
+            if (dleft) {
+                // c = (da x db)(.T) * C * ket(.T): rows of C and V
+                const bool cb = t.conj & 1;
+                const bool ca = ((t.dconj & 1) != 0) ^ cb;
+                const bool cbb = ((t.dconj & 2) != 0) ^ cb;
+                const uint32_t am = (ca ? da.m : da.n) * (cbb ? db.m : db.n);
+                const uint32_t cm = (ca ? da.n : da.m) * (cbb ? db.n : db.m);
+                const uint32_t bra_n = ai.nket[ia];
+                const uint64_t ast = cb ? stride / bra_n : stride % bra_n;
+                const uint64_t cst = cb ? stride % bra_n : stride / bra_n;
+                const SubBlock x = sa ? db : da;
+                const bool tx = sa ? cbb : ca;
+                if (ast + am > cm_full || cst + cm > vm_full)
+                    return fail("delayed window outside its block", il);
+                if ((tx ? x.m != am || x.n != cm : x.m != cm || x.n != am))
+                    return fail("delayed sub-operator shape does not chain", il);
+                r.a_off = x.off;
+                r.b_off = rb.data_off + bi.shift[ib];
+                r.c_off = p.ket.shift[ic] + ast * cn_full;
+                r.v_off = p.bra.shift[iv] + cst * vn_full;
+                r.c_m = am, r.c_n = cn_full, r.v_m = cm, r.v_n = vn_full;
+                r.conj = (tx ? 1u : 0u) | (t.conj & 2);
+                const uint32_t bm = bi.nbra[ib], bn = bi.nket[ib];
+                if (!((t.conj & 2) ? (bm == r.c_n && bn == r.v_n)
+                                   : (bm == r.v_n && bn == r.c_n)))
+                    return fail("block shapes do not chain", il);
+            } else {
+                // c = bra(.T) * C * (da x db)(.T): columns of C and V
+                const bool ck = !(t.conj & 2);
+                const bool ca = ((t.dconj & 1) != 0) ^ ck;
+                const bool cbb = ((t.dconj & 2) != 0) ^ ck;
+                const uint32_t kn = (ca ? da.m : da.n) * (cbb ? db.m : db.n);
+                const uint32_t km = (ca ? da.n : da.m) * (cbb ? db.n : db.m);
+                const uint32_t ket_n = bi.nket[ib];
+                const uint64_t ast = ck ? stride % ket_n : stride / ket_n;
+                const uint64_t cst = ck ? stride / ket_n : stride % ket_n;
+                const SubBlock y = sa ? db : da;
+                const bool ty = sa ? cbb : ca;
+                if (ast + km > cn_full || cst + kn > vn_full)
+                    return fail("delayed window outside its block", il);
+                // work = C_win (c_m x km) * op(y) (km x kn)
+                if ((ty ? y.m != kn || y.n != km : y.m != km || y.n != kn))
+                    return fail("delayed sub-operator shape does not chain", il);
+                r.a_off = la.data_off + ai.shift[ia];
+                r.b_off = y.off;
+                r.c_off = p.ket.shift[ic] + ast;
+                r.v_off = p.bra.shift[iv] + cst;
+                r.c_m = cm_full, r.c_n = km, r.v_m = vm_full, r.v_n = kn;
+                r.c_ld = cn_full, r.v_ld = vn_full;
+                // op(y) untransposed is the "C * B" form (conj bit 1)
+                r.conj = (t.conj & 1) | (ty ? 0u : 2u);
+                const uint32_t am = ai.nbra[ia], an = ai.nket[ia];
+                if (!((t.conj & 1) ? (am == r.c_m && an == r.v_m)
+                                   : (am == r.v_m && an == r.c_m)))
+                    return fail("block shapes do not chain", il);
+            }
+            out.push_back(r);
+        }
+    }
+    return true;
+}
+
 std::vector<PairRec> record_pairs(const Problem &p, const Connections &wfn,
                                   std::string &err) {
     std::vector<PairRec> out;
@@ -265,6 +410,18 @@ std::vector<PairRec> record_pairs(const Problem &p, const Connections &wfn,
         const double scale = t.factor * la.factor * rb.factor;
         if (std::abs(scale) < kTiny)
             continue;
+        if (t.three) {
+            int b = 0, e = 0;
+            if (!term_class(p, t, wfn, b, e)) {
+                std::ostringstream ss;
+                ss << "term " << it << " has no connection class";
+                err = ss.str();
+                return {};
+            }
+            if (!record_three(p, t, it, wfn, b, e, scale, out, err))
+                return {};
+            continue;
+        }
         int b = 0, e = 0;
         if (!term_class(p, t, wfn, b, e)) {
             std::ostringstream ss;
@@ -305,6 +462,75 @@ std::vector<PairRec> record_pairs(const Problem &p, const Connections &wfn,
     return out;
 }
 
+// diagonal of one three-tensor string (three_tensor_product_diagonal,
+// operator_functions.hpp:248-328; the diagonal sub-blocks only,
+// batch_gemm.hpp:516-565)
+static bool record_three_diag(const Problem &p, const Term &t, size_t it,
+                              const Connections &dg, int b, int e,
+                              double scale, std::vector<DiagRec> &out,
+                              std::string &err) {
+    const Connections *dinfo = nullptr;
+    int db_ = 0, de_ = 0;
+    double dscale = 1.0;
+    if (!delayed_class(p, t, dinfo, db_, de_, dscale, err))
+        return false;
+    scale *= dscale;
+    if (std::abs(scale) < kTiny)
+        return true;
+    const bool dleft = p.delayed_side == 1;
+    const Operator &la = p.lops[t.a], &rb = p.rops[t.b];
+    const Operator &oda = p.dlops[t.da], &odb = p.drops[t.db];
+    const BlockInfo &ai = p.linfos[la.info], &bi = p.rinfos[rb.info];
+    for (int il = b; il < e; il++) {
+        const uint32_t ia = dg.ia[il], ib = dg.ib[il], ic = dg.ic[il];
+        const uint32_t idc = dleft ? ia : ib;
+        const uint32_t dm = dleft ? ai.nbra[ia] : bi.nbra[ib];
+        const uint32_t *lo = std::lower_bound(dinfo->ic.data() + db_,
+                                              dinfo->ic.data() + de_, idc);
+        for (int idl = (int)(lo - dinfo->ic.data()); idl < de_ &&
+                                                     dinfo->ic[idl] == idc;
+             idl++) {
+            const uint64_t stride = dinfo->stride[idl];
+            const uint64_t dstrm = stride / dm, dstrn = stride % dm;
+            if (dstrm != dstrn)
+                continue; // off-diagonal sub-block
+            const SubBlock da = sub_block(p.dlinfos, oda, dinfo->ia[idl]);
+            const SubBlock db = sub_block(p.drinfos, odb, dinfo->ib[idl]);
+            const bool sa = da.m == 1 && da.n == 1, sb = db.m == 1 && db.n == 1;
+            const SubBlock x = sa ? db : da;
+            if ((!sa && !sb) || x.m != x.n) {
+                std::ostringstream ss;
+                ss << "term " << it << " diagonal connection " << il
+                   << ": delayed sub-operators neither 1 x 1 nor square";
+                err = ss.str();
+                return false;
+            }
+            DiagRec r;
+            r.d_off = p.ket.shift[ic];
+            r.alpha = scale * dg.factor[il] * dinfo->factor[idl];
+            r.scal_off = (int64_t)(sa ? da.off : db.off);
+            if (dleft) {
+                r.a_off = x.off, r.a_n = x.n;
+                r.b_off = rb.data_off + bi.shift[ib], r.b_n = bi.nbra[ib];
+                r.r0 = (uint32_t)dstrm, r.c0 = 0;
+            } else {
+                r.a_off = la.data_off + ai.shift[ia], r.a_n = ai.nbra[ia];
+                r.b_off = x.off, r.b_n = x.n;
+                r.r0 = 0, r.c0 = (uint32_t)dstrm;
+            }
+            if (r.r0 + r.a_n > p.ket.nbra[ic] || r.c0 + r.b_n > p.ket.nket[ic]) {
+                std::ostringstream ss;
+                ss << "term " << it << " diagonal connection " << il
+                   << ": delayed window outside its block";
+                err = ss.str();
+                return false;
+            }
+            out.push_back(r);
+        }
+    }
+    return true;
+}
+
 std::vector<DiagRec> record_diag(const Problem &p, const Connections &dg,
                                  std::string &err) {
     std::vector<DiagRec> out;
@@ -318,6 +544,11 @@ std::vector<DiagRec> record_diag(const Problem &p, const Connections &dg,
         // a string whose class has no diagonal connections adds nothing
         if (!term_class(p, t, dg, b, e))
             continue;
+        if (t.three) {
+            if (!record_three_diag(p, t, it, dg, b, e, scale, out, err))
+                return {};
+            continue;
+        }
         const BlockInfo &ai = p.linfos[la.info], &bi = p.rinfos[rb.info];
         for (int il = b; il < e; il++) {
             const uint32_t ia = dg.ia[il], ib = dg.ib[il], ic = dg.ic[il];

@@ -60,6 +60,11 @@ struct Term {
     int a, b;      // left / right operator index
     uint8_t conj;  // bit0: left transposed, bit1: right transposed
     double factor;
+    // delayed three-tensor string (OpSumProd): sub-operators dl[da], dr[db]
+    // of the delayed tensor with conj bits dconj
+    bool three = false;
+    int da = -1, db = -1;
+    uint8_t dconj = 0;
 };
 
 struct Problem {
@@ -69,20 +74,36 @@ struct Problem {
     std::vector<BlockInfo> linfos, rinfos; // sorted by dq
     std::vector<Operator> lops, rops;
     std::vector<Term> terms;
+    // delayed tensor (0 none, 1 left, 2 right): its sub-operator sets with
+    // their infos (data in the delayed side's buffer) and the tensor-product
+    // connection list of each info of the delayed side
+    int delayed_side = 0;
+    std::vector<BlockInfo> dlinfos, drinfos;
+    std::vector<Operator> dlops, drops;
+    std::vector<Connections> tp; // indexed like linfos / rinfos
 };
 
-// One recorded GEMM chain: V[v] += alpha * op(A) * C[c] * op(B)^T
+// One recorded GEMM chain: V[v] += alpha * op(A) * C[c] * op(B)^T.  For a
+// three-tensor chain C and V are windows with row strides c_ld / v_ld
+// (0: dense) and alpha still lacks the value of the 1 x 1 sub-operator
+// element at scal_off of the delayed side's buffer (-1: none).
 struct PairRec {
     uint64_t a_off, b_off, c_off, v_off;
     uint32_t c_m, c_n, v_m, v_n;
     uint32_t conj;
     double alpha;
+    uint32_t kind = 0;
+    uint32_t c_ld = 0, v_ld = 0;
+    int64_t scal_off = -1;
 };
 
+// D[d](r0 + i, c0 + j) += alpha * A(i,i) * B(j,j) (scal_off as above)
 struct DiagRec {
     uint64_t a_off, b_off, d_off;
     uint32_t a_n, b_n;
     double alpha;
+    uint32_t r0 = 0, c0 = 0;
+    int64_t scal_off = -1;
 };
 
 // Unique (conj, delta-quantum) sub-labels of H_eff's strings
@@ -107,11 +128,14 @@ void build_diag_connections(Connections &ci, Symm s, const SU2Coupling &cg,
 
 // Record the pair list of H*psi: for every string, every connection of its
 // class (OperatorFunctions::tensor_product_multiply,
-// src/core/operator_functions.hpp:474-542).
+// src/core/operator_functions.hpp:474-542); for a delayed string, every
+// matching tensor-product connection of the delayed operator's block as
+// well (three_tensor_product_multiply, :543-671).
 std::vector<PairRec> record_pairs(const Problem &p, const Connections &wfn,
                                   std::string &err);
 // Record the diagonal contributions
-// (OperatorFunctions::tensor_product_diagonal, operator_functions.hpp:212-247).
+// (OperatorFunctions::tensor_product_diagonal, operator_functions.hpp:212-247,
+// three_tensor_product_diagonal :248-328).
 std::vector<DiagRec> record_diag(const Problem &p, const Connections &dg,
                                  std::string &err);
 

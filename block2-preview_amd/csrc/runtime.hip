@@ -9,6 +9,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <limits>
 #include <map>
 #include <sstream>
 #include <tuple>
@@ -24,6 +25,17 @@ void set_error(const std::string &msg) { g_err = msg; }
 using namespace b2g;
 
 extern "C" const char *b2g_last_error(void) { return g_err.c_str(); }
+
+extern "C" int b2g_abi_sizes(size_t *out, int n) {
+    const size_t sz[] = {sizeof(b2g_pair),        sizeof(b2g_block),
+                         sizeof(b2g_diag_term),   sizeof(b2g_sparse_info),
+                         sizeof(b2g_operator),    sizeof(b2g_term),
+                         sizeof(b2g_tp_conn),     sizeof(b2g_problem)};
+    const int k = (int)(sizeof(sz) / sizeof(sz[0]));
+    for (int i = 0; i < n && i < k && out; i++)
+        out[i] = sz[i];
+    return k;
+}
 
 // ------------------------------------------------------------------ context
 extern "C" int b2g_ctx_create(int device, unsigned flags, b2g_ctx *out) {
@@ -342,7 +354,8 @@ static uint64_t ksplit() {
 // TM x TN slot of the partial buffer, and a reduction task sums the slots in
 // group order (deterministic) into the output.
 static void add_split_tiles(Chunk &ch, uint64_t d_off, uint32_t M, uint32_t N,
-                            uint32_t sb, uint32_t se, uint32_t beta) {
+                            uint32_t ldd, uint32_t sb, uint32_t se,
+                            uint32_t beta) {
     std::vector<std::pair<uint32_t, uint32_t>> groups;
     const uint64_t kmax = ksplit();
     uint64_t K = 0, acc = 0;
@@ -359,7 +372,7 @@ static void add_split_tiles(Chunk &ch, uint64_t d_off, uint32_t M, uint32_t N,
     const std::vector<Tile> tiles = tile_cover(M, N, K / groups.size());
     if (groups.size() == 1) {
         for (const Tile &t : tiles)
-            ch.t2[cfg_of(t)].push_back(make_task(d_off, N, M, N, t, sb, se,
+            ch.t2[cfg_of(t)].push_back(make_task(d_off, ldd, M, N, t, sb, se,
                                                  beta));
         return;
     }
@@ -378,7 +391,7 @@ static void add_split_tiles(Chunk &ch, uint64_t d_off, uint32_t M, uint32_t N,
         RedTask rt;
         rt.d_off = d_off;
         rt.p_off = p0;
-        rt.ldd = N;
+        rt.ldd = ldd;
         rt.M = M;
         rt.N = N;
         rt.row0 = t.row0;
@@ -490,126 +503,8 @@ uint64_t w_cap_elems() {
     return (uint64_t)1 << 30; // 1 Gi elements (8 GiB FP64) per right-hand side
 }
 
-static int compile_plan(b2g_plan p, const b2g_pair *pairs, size_t n_pairs,
-                        const b2g_block *yb, size_t n_yb) {
-    const size_t llen = p->lbuf->n, rlen = p->rbuf->n;
-    std::unordered_map<uint64_t, size_t> yidx;
-    for (size_t i = 0; i < n_yb; i++) {
-        B2G_REQUIRE(yb[i].off + (uint64_t)yb[i].m * yb[i].n <= p->y_len,
-                    B2G_ERR_ARG, "output block outside the output vector");
-        yidx[yb[i].off] = i;
-    }
-    p->macs = 0;
-    for (size_t i = 0; i < n_pairs; i++) {
-        const b2g_pair &q = pairs[i];
-        std::ostringstream ss;
-        ss << "pair " << i << ": ";
-        B2G_REQUIRE(q.kind == 0, B2G_ERR_ARG, ss.str() + "unsupported kind");
-        B2G_REQUIRE(q.a_off + (uint64_t)q.c_m * q.v_m <= llen, B2G_ERR_ARG,
-                    ss.str() + "A block outside the L buffer");
-        B2G_REQUIRE(q.b_off + (uint64_t)q.c_n * q.v_n <= rlen, B2G_ERR_ARG,
-                    ss.str() + "B block outside the R buffer");
-        B2G_REQUIRE(q.c_off + (uint64_t)q.c_m * q.c_n <= p->x_len, B2G_ERR_ARG,
-                    ss.str() + "C block outside the input vector");
-        auto it = yidx.find(q.v_off);
-        B2G_REQUIRE(it != yidx.end(), B2G_ERR_ARG,
-                    ss.str() + "V block is not an output block");
-        B2G_REQUIRE(yb[it->second].m == q.v_m && yb[it->second].n == q.v_n,
-                    B2G_ERR_ARG, ss.str() + "V block shape mismatch");
-        p->macs += (uint64_t)q.c_m * q.c_n * q.v_n +
-                   (uint64_t)q.v_m * q.c_m * q.v_n;
-    }
-    // split the pair list into chunks whose W scratch fits the cap
-    const uint64_t cap = w_cap_elems();
-    size_t i0 = 0;
-    bool first = true;
-    while (i0 < n_pairs || first) {
-        Chunk ch;
-        typedef std::tuple<uint64_t, uint64_t, uint32_t> WKey;
-        std::map<WKey, uint64_t> wmap; // -> W offset
-        std::vector<uint64_t> pw;      // W offset per pair of this chunk
-        size_t i1 = i0;
-        for (; i1 < n_pairs; i1++) {
-            const b2g_pair &q = pairs[i1];
-            if (q.c_m == 0 || q.c_n == 0 || q.v_m == 0 || q.v_n == 0) {
-                pw.push_back(0); // contributes nothing; never referenced
-                continue;
-            }
-            WKey key(q.b_off, q.c_off, q.conj & 2);
-            auto it = wmap.find(key);
-            if (it == wmap.end()) {
-                const uint64_t sz = (uint64_t)q.c_m * q.v_n;
-                if (ch.w_len + sz > cap && i1 > i0)
-                    break;
-                it = wmap.emplace(key, ch.w_len).first;
-                // phase 1: W = C * op(B)
-                Seg s;
-                s.a_off = q.c_off;
-                s.lda = q.c_n;
-                s.b_off = q.b_off;
-                const bool tb = !(q.conj & 2);
-                s.ldb = tb ? q.c_n : q.v_n;
-                s.K = q.c_n;
-                s.flags = tb ? 2u : 0u;
-                s.alpha = 1.0;
-                const uint32_t si = (uint32_t)ch.s1.size();
-                ch.s1.push_back(s);
-                add_tiles(ch.t1, ch.w_len, q.c_m, q.v_n, q.c_n, si, si + 1, 0);
-                ch.w_len += sz;
-            }
-            pw.push_back(it->second);
-        }
-        // phase 2: group this chunk's pairs by output block (record order)
-        std::map<uint64_t, std::vector<size_t>> byv;
-        for (size_t i = i0; i < i1; i++)
-            byv[pairs[i].v_off].push_back(i);
-        auto emit = [&](const b2g_block &blk, const std::vector<size_t> *lst,
-                        uint32_t beta) {
-            const uint32_t sb = (uint32_t)ch.s2.size();
-            if (lst != nullptr)
-                for (size_t i : *lst) {
-                    const b2g_pair &q = pairs[i];
-                    if (q.c_m == 0 || q.c_n == 0)
-                        continue; // K = 0: nothing to add
-                    Seg s;
-                    s.a_off = q.a_off;
-                    const bool ta = q.conj & 1;
-                    s.lda = ta ? q.v_m : q.c_m;
-                    s.b_off = pw[i - i0];
-                    s.ldb = q.v_n;
-                    s.K = q.c_m;
-                    s.flags = ta ? 1u : 0u;
-                    s.alpha = q.alpha;
-                    ch.s2.push_back(s);
-                }
-            add_split_tiles(ch, blk.off, blk.m, blk.n, sb,
-                            (uint32_t)ch.s2.size(), beta);
-        };
-        if (first) {
-            for (size_t b = 0; b < n_yb; b++) {
-                auto it = byv.find(yb[b].off);
-                emit(yb[b], it == byv.end() ? nullptr : &it->second, 0);
-            }
-        } else {
-            for (auto &kv : byv)
-                emit(yb[yidx[kv.first]], &kv.second, 1);
-        }
-        for (int c = 0; c < kTileConfigs; c++) {
-            ch.x1[c] = xcd_order(ch.t1[c], ch.s1);
-            ch.x2[c] = xcd_order(ch.t2[c], ch.s2);
-            ch.f1[c] = task_flops(ch.t1[c], ch.s1, c);
-            ch.f2[c] = task_flops(ch.t2[c], ch.s2, c);
-            p->kflops[B2G_K_PHASE1 + c] += ch.f1[c];
-            p->kflops[B2G_K_PHASE2 + c] += ch.f2[c];
-        }
-        p->w_max = std::max(p->w_max, ch.w_len);
-        p->p_max = std::max(p->p_max, ch.p_len);
-        p->chunks.push_back(std::move(ch));
-        first = false;
-        i0 = i1;
-        if (n_pairs == 0)
-            break;
-    }
+// upload all chunks' task / segment / reduction arrays in one allocation
+static int upload_chunks(b2g_plan p) {
     // upload all task/segment arrays in one allocation
     size_t bytes = 0;
     for (auto &ch : p->chunks) {
@@ -658,6 +553,173 @@ static int compile_plan(b2g_plan p, const b2g_pair *pairs, size_t n_pairs,
         ch.d_red = (RedTask *)(base + offs[i].red);
     }
     return B2G_OK;
+}
+
+static int compile_plan(b2g_plan p, const b2g_pair *pairs, size_t n_pairs,
+                        const b2g_block *yb, size_t n_yb) {
+    const size_t llen = p->lbuf->n, rlen = p->rbuf->n;
+    std::unordered_map<uint64_t, size_t> yidx;
+    std::vector<std::pair<uint64_t, size_t>> ysorted; // (offset, block)
+    for (size_t i = 0; i < n_yb; i++) {
+        B2G_REQUIRE(yb[i].off + (uint64_t)yb[i].m * yb[i].n <= p->y_len,
+                    B2G_ERR_ARG, "output block outside the output vector");
+        yidx[yb[i].off] = i;
+        ysorted.emplace_back(yb[i].off, i);
+    }
+    std::sort(ysorted.begin(), ysorted.end());
+    p->macs = 0;
+    std::vector<size_t> whole, windows; // pair indices by kind
+    for (size_t i = 0; i < n_pairs; i++) {
+        const b2g_pair &q = pairs[i];
+        std::ostringstream ss;
+        ss << "pair " << i << ": ";
+        B2G_REQUIRE(q.kind == 0 || q.kind == 1, B2G_ERR_ARG,
+                    ss.str() + "unsupported kind");
+        B2G_REQUIRE(q.a_off + (uint64_t)q.c_m * q.v_m <= llen, B2G_ERR_ARG,
+                    ss.str() + "A block outside the L buffer");
+        B2G_REQUIRE(q.b_off + (uint64_t)q.c_n * q.v_n <= rlen, B2G_ERR_ARG,
+                    ss.str() + "B block outside the R buffer");
+        const uint64_t cld = q.c_ld ? q.c_ld : q.c_n;
+        const uint64_t vld = q.v_ld ? q.v_ld : q.v_n;
+        B2G_REQUIRE(cld >= q.c_n && vld >= q.v_n, B2G_ERR_ARG,
+                    ss.str() + "row stride below the window width");
+        B2G_REQUIRE(q.c_m == 0 || q.c_n == 0 ||
+                        q.c_off + (q.c_m - 1) * cld + q.c_n <= p->x_len,
+                    B2G_ERR_ARG, ss.str() + "C block outside the input vector");
+        if (q.kind == 0) {
+            B2G_REQUIRE(q.c_ld == 0 && q.v_ld == 0, B2G_ERR_ARG,
+                        ss.str() + "whole-block pair with a row stride");
+            auto it = yidx.find(q.v_off);
+            B2G_REQUIRE(it != yidx.end(), B2G_ERR_ARG,
+                        ss.str() + "V block is not an output block");
+            B2G_REQUIRE(yb[it->second].m == q.v_m && yb[it->second].n == q.v_n,
+                        B2G_ERR_ARG, ss.str() + "V block shape mismatch");
+            whole.push_back(i);
+        } else {
+            // the window lies inside one output block, rows of that block
+            auto it = std::upper_bound(
+                ysorted.begin(), ysorted.end(),
+                std::make_pair(q.v_off, std::numeric_limits<size_t>::max()));
+            B2G_REQUIRE(it != ysorted.begin(), B2G_ERR_ARG,
+                        ss.str() + "V window before the first output block");
+            const b2g_block &bl = yb[(it - 1)->second];
+            const uint64_t rel = q.v_off - bl.off;
+            B2G_REQUIRE(bl.n > 0 && vld == bl.n && rel % bl.n + q.v_n <= bl.n &&
+                            rel / bl.n + q.v_m <= bl.m,
+                        B2G_ERR_ARG, ss.str() + "V window outside its block");
+            windows.push_back(i);
+        }
+        p->macs += (uint64_t)q.c_m * q.c_n * q.v_n +
+                   (uint64_t)q.v_m * q.c_m * q.v_n;
+    }
+    const uint64_t cap = w_cap_elems();
+    // Chunks of the pairs in `idx` whose W scratch fits the cap.  Whole-block
+    // pairs: the first chunk writes every output block (zero where no pair
+    // lands), later chunks add.  Windowed pairs: their chunks follow and add;
+    // tasks are grouped by identical windows (windows of one block are
+    // identical or disjoint), so no two tasks of a launch overlap.
+    auto build = [&](const std::vector<size_t> &idx, bool win) {
+        size_t j0 = 0;
+        bool first = !win;
+        while (j0 < idx.size() || first) {
+            Chunk ch;
+            typedef std::tuple<uint64_t, uint64_t, uint32_t, uint32_t, uint32_t,
+                               uint32_t>
+                WKey;
+            std::map<WKey, uint64_t> wmap;            // -> W offset
+            std::unordered_map<size_t, uint64_t> pw;  // pair -> W offset
+            size_t j1 = j0;
+            for (; j1 < idx.size(); j1++) {
+                const b2g_pair &q = pairs[idx[j1]];
+                if (q.c_m == 0 || q.c_n == 0 || q.v_m == 0 || q.v_n == 0)
+                    continue; // contributes nothing
+                WKey key(q.b_off, q.c_off, q.conj & 2, q.c_m, q.c_n, q.c_ld);
+                auto it = wmap.find(key);
+                if (it == wmap.end()) {
+                    const uint64_t sz = (uint64_t)q.c_m * q.v_n;
+                    if (ch.w_len + sz > cap && j1 > j0)
+                        break;
+                    it = wmap.emplace(key, ch.w_len).first;
+                    // phase 1: W = C * op(B)
+                    Seg s;
+                    s.a_off = q.c_off;
+                    s.lda = q.c_ld ? q.c_ld : q.c_n;
+                    s.b_off = q.b_off;
+                    const bool tb = !(q.conj & 2);
+                    s.ldb = tb ? q.c_n : q.v_n;
+                    s.K = q.c_n;
+                    s.flags = tb ? 2u : 0u;
+                    s.alpha = 1.0;
+                    const uint32_t si = (uint32_t)ch.s1.size();
+                    ch.s1.push_back(s);
+                    add_tiles(ch.t1, ch.w_len, q.c_m, q.v_n, q.c_n, si, si + 1,
+                              0);
+                    ch.w_len += sz;
+                }
+                pw[idx[j1]] = it->second;
+            }
+            // phase 2: group this chunk's pairs by output block or window
+            // (record order within a group)
+            typedef std::tuple<uint64_t, uint32_t, uint32_t, uint32_t> VKey;
+            std::map<VKey, std::vector<size_t>> byv;
+            for (size_t j = j0; j < j1; j++) {
+                const b2g_pair &q = pairs[idx[j]];
+                if (pw.count(idx[j]) == 0)
+                    continue;
+                byv[VKey(q.v_off, q.v_m, q.v_n, q.v_ld ? q.v_ld : q.v_n)]
+                    .push_back(idx[j]);
+            }
+            auto emit = [&](uint64_t off, uint32_t m, uint32_t n, uint32_t ldd,
+                            const std::vector<size_t> *lst, uint32_t beta) {
+                const uint32_t sb = (uint32_t)ch.s2.size();
+                if (lst != nullptr)
+                    for (size_t i : *lst) {
+                        const b2g_pair &q = pairs[i];
+                        Seg s;
+                        s.a_off = q.a_off;
+                        const bool ta = q.conj & 1;
+                        s.lda = ta ? q.v_m : q.c_m;
+                        s.b_off = pw[i];
+                        s.ldb = q.v_n;
+                        s.K = q.c_m;
+                        s.flags = ta ? 1u : 0u;
+                        s.alpha = q.alpha;
+                        ch.s2.push_back(s);
+                    }
+                add_split_tiles(ch, off, m, n, ldd, sb, (uint32_t)ch.s2.size(),
+                                beta);
+            };
+            if (first) {
+                for (size_t b = 0; b < n_yb; b++) {
+                    auto it = byv.find(VKey(yb[b].off, yb[b].m, yb[b].n,
+                                            yb[b].n));
+                    emit(yb[b].off, yb[b].m, yb[b].n, yb[b].n,
+                         it == byv.end() ? nullptr : &it->second, 0);
+                }
+            } else {
+                for (auto &kv : byv)
+                    emit(std::get<0>(kv.first), std::get<1>(kv.first),
+                         std::get<2>(kv.first), std::get<3>(kv.first),
+                         &kv.second, 1);
+            }
+            for (int c = 0; c < kTileConfigs; c++) {
+                ch.x1[c] = xcd_order(ch.t1[c], ch.s1);
+                ch.x2[c] = xcd_order(ch.t2[c], ch.s2);
+                ch.f1[c] = task_flops(ch.t1[c], ch.s1, c);
+                ch.f2[c] = task_flops(ch.t2[c], ch.s2, c);
+                p->kflops[B2G_K_PHASE1 + c] += ch.f1[c];
+                p->kflops[B2G_K_PHASE2 + c] += ch.f2[c];
+            }
+            p->w_max = std::max(p->w_max, ch.w_len);
+            p->p_max = std::max(p->p_max, ch.p_len);
+            p->chunks.push_back(std::move(ch));
+            first = false;
+            j0 = j1;
+        }
+    };
+    build(whole, false);
+    build(windows, true);
+    return upload_chunks(p);
 }
 
 } // namespace b2g
@@ -1000,8 +1062,9 @@ extern "C" int b2g_diag(b2g_ctx ctx, b2g_buf lbuf, b2g_buf rbuf,
         B2G_REQUIRE(it != didx.end(), B2G_ERR_ARG,
                     "diagonal term targets no output block");
         const b2g_block &blk = dblocks[it->second];
-        B2G_REQUIRE(blk.m == t.a_n && blk.n == t.b_n, B2G_ERR_ARG,
-                    "diagonal term shape mismatch");
+        B2G_REQUIRE((uint64_t)t.r0 + t.a_n <= blk.m &&
+                        (uint64_t)t.c0 + t.b_n <= blk.n,
+                    B2G_ERR_ARG, "diagonal term window outside its block");
         B2G_REQUIRE(t.a_off + (uint64_t)t.a_n * t.a_n <= lbuf->n &&
                         t.b_off + (uint64_t)t.b_n * t.b_n <= rbuf->n,
                     B2G_ERR_ARG, "diagonal term outside operator buffers");
@@ -1021,6 +1084,8 @@ extern "C" int b2g_diag(b2g_ctx ctx, b2g_buf lbuf, b2g_buf rbuf,
             t.b_off = terms[i].b_off;
             t.a_n = terms[i].a_n;
             t.b_n = terms[i].b_n;
+            t.r0 = terms[i].r0;
+            t.c0 = terms[i].c0;
             t.alpha = terms[i].alpha;
             dterms.push_back(t);
         }

@@ -67,6 +67,10 @@ typedef struct b2g_buf_s *b2g_buf;
 typedef struct b2g_plan_s *b2g_plan;
 typedef struct b2g_effham_s *b2g_effham;
 
+/* sizeof of the ABI structs, in this order: b2g_pair, b2g_block,
+ * b2g_diag_term, b2g_sparse_info, b2g_operator, b2g_term, b2g_tp_conn,
+ * b2g_problem.  Bindings check their layouts with it.  Returns the count. */
+int b2g_abi_sizes(size_t *out, int n);
 /* Last error message of the calling thread ("" if none). */
 const char *b2g_last_error(void);
 
@@ -108,7 +112,14 @@ typedef struct {
     uint64_t a_off, b_off, c_off, v_off;
     uint32_t c_m, c_n, v_m, v_n;
     uint32_t conj;
-    uint32_t kind; /* 0: two-tensor rotate */
+    /* 0: two-tensor rotate on whole blocks; 1: a three-tensor rotate
+     * (BatchGEMMSeq::three_rotate, src/core/batch_gemm.hpp:952-1022,
+     * GMatrixFunctions::three_rotate, matrix_functions.hpp:1046-1112) whose
+     * C and V are windows of their blocks: a range of rows (delayed left
+     * operator) or of columns (delayed right operator)                     */
+    uint32_t kind;
+    /* row strides of the C and V windows (0: dense, = c_n / v_n)          */
+    uint32_t c_ld, v_ld;
     double alpha;
 } b2g_pair;
 
@@ -119,14 +130,17 @@ typedef struct {
     uint32_t m, n;
 } b2g_block;
 
-/* Diagonal contribution  D[d](i,j) += alpha * A(i,i) * B(j,j)
+/* Diagonal contribution  D[d](r0 + i, c0 + j) += alpha * A(i,i) * B(j,j)
  * (AdvancedGEMM::tensor_product_diagonal, src/core/batch_gemm.hpp:507-513;
- * GMatrixFunctions::tensor_product_diagonal, matrix_functions.hpp:1204-1212).
+ * GMatrixFunctions::tensor_product_diagonal, matrix_functions.hpp:1204-1212;
+ * the three-tensor form, batch_gemm.hpp:516-565, adds a diagonal window).
  * A is a square block (a_n x a_n) at a_off in the L buffer, B square
- * (b_n x b_n) at b_off in the R buffer, D (a_n x b_n) at d_off.          */
+ * (b_n x b_n) at b_off in the R buffer; the a_n x b_n window at (r0, c0)
+ * lies inside the output block at d_off.                                  */
 typedef struct {
     uint64_t a_off, b_off, d_off;
     uint32_t a_n, b_n;
+    uint32_t r0, c0;
     double alpha;
 } b2g_diag_term;
 
@@ -235,23 +249,51 @@ typedef struct {
 
 /* One operator block set (an entry of lopt/ropt->ops): delta quantum,
  * scalar factor, the index of its info in left/right_op_infos and the
- * element offset of its data in the operator buffer.                      */
+ * element offset of its data in the operator buffer.  flags bit 0: the
+ * operator has structure only (a delayed operator of a DelayedOperatorTensor
+ * whose blocks are never formed; data_off is ignored).                    */
 typedef struct {
     uint64_t delta_quantum;
     double factor;
     int32_t info;
-    int32_t pad;
+    int32_t flags;
     uint64_t data_off;
 } b2g_operator;
 
-/* One OpProduct string of the H_eff OpSum (expr.hpp:503, 811):
- * H += factor * op(L[a]) (x) op(R[b]) with block2's conj bits.            */
+/* One string of the H_eff OpSum (expr.hpp:503, 619, 811).
+ * kind 0, OpProduct:  H += factor * op(L[a]) (x) op(R[b]) with block2's
+ *   conj bits.
+ * kind 1, OpSumProd (a delayed three-tensor string): the operator of the
+ *   delayed side (L[a] or R[b], structure only) is the tensor product of
+ *   sub-operators dl[da] and dr[db] of the delayed tensor, conjugated by the
+ *   bits of dconj (OpSumProd::conjs), and one of them is 1 x 1
+ *   (OperatorFunctions::three_tensor_product_multiply,
+ *   src/core/operator_functions.hpp:543-671).                            */
 typedef struct {
     int32_t a, b;
     uint32_t conj;
-    uint32_t pad;
+    uint32_t kind;
     double factor;
+    int32_t da, db;
+    uint32_t dconj;
+    uint32_t pad;
 } b2g_term;
+
+/* A tensor-product connection list (SparseMatrixInfo::ConnectionInfo built
+ * by initialize_tp, src/core/sparse_matrix.hpp:294-478) of a delayed
+ * operator's info: classes (dconj, combined quantum) in `quanta` with
+ * first entries `idx`, n[c] the first class of conj c, n[4] the class
+ * count; entries (ia: dl block, ib: dr block, ic: fused block, stride: the
+ * sub-block's row * fused-block columns + column, factor).                */
+typedef struct {
+    int32_t n[5];
+    int32_t nc;
+    const uint64_t *quanta;
+    const uint32_t *idx;
+    const uint32_t *ia, *ib, *ic;
+    const uint64_t *stride;
+    const double *factor;
+} b2g_tp_conn;
 
 typedef struct {
     int32_t symmetry;          /* B2G_SZ / B2G_SU2                         */
@@ -267,6 +309,19 @@ typedef struct {
     const double *left_data;   /* host FP64 operator blocks                */
     const double *right_data;
     size_t left_len, right_len;
+    /* Delayed (three-tensor) strings, DelayedOperatorTensor
+     * (src/core/operator_tensor.hpp:209-): delayed_side 0 (none), 1 (the
+     * left tensor is delayed) or 2 (the right one); the delayed tensor's
+     * sub-operator sets dl (its lopt) and dr (its ropt) with their own
+     * infos, whose data_off index the delayed side's data buffer; tp[i]
+     * the tensor-product connection list of the delayed side's info i
+     * (zeroed entries where none).                                        */
+    int32_t delayed_side;
+    int32_t n_dl_infos, n_dr_infos, n_dl_ops, n_dr_ops;
+    int32_t pad_delayed;
+    const b2g_sparse_info *dl_infos, *dr_infos;
+    const b2g_operator *dl_ops, *dr_ops;
+    const b2g_tp_conn *tp;
 } b2g_problem;
 
 /* Build connection lists, pair list, diagonal; upload.  flags as in
@@ -312,6 +367,21 @@ int b2g_effham_data(b2g_effham h, double **left, size_t *left_len,
  * reading a status header per iteration; mode 0 runs the host-driven loop
  * (also used for k > 1 roots and while plan timing is enabled).           */
 int b2g_effham_set_davidson_mode(b2g_effham h, int mode);
+/* User hook around every Davidson matvec (EffectiveKernel::compute,
+ * src/dmrg/effective_hamiltonian.hpp:83-93, used at :522-532): with a hook
+ * set, b2g_effham_eigs calls kernel(user, beta, f, f_data, a, b, n) where it
+ * would compute b = beta * H a, and the hook calls f(f_data, a, b, scale)
+ * (b = scale * H a on device vectors of length n) as it sees fit.  The
+ * context stream is idle when the hook is entered and must be idle again
+ * (its own work finished) when it returns; a non-zero return aborts the
+ * solve with B2G_ERR_STATE.  A hook runs the host-driven Davidson loop.
+ * kernel = NULL removes the hook.                                        */
+typedef int (*b2g_matvec_fn)(void *f_data, const double *a, double *b,
+                             double scale);
+typedef int (*b2g_kernel_fn)(void *user, double beta, b2g_matvec_fn f,
+                             void *f_data, const double *a, double *b,
+                             size_t n);
+int b2g_effham_set_kernel(b2g_effham h, b2g_kernel_fn kernel, void *user);
 /* b2g_grad_ops on the effective Hamiltonian's plan.                      */
 int b2g_effham_grad_ops(b2g_effham h, const double *v, const double *c,
                         int n_rhs, double *gl, double *gr, int accumulate);

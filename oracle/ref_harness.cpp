@@ -126,15 +126,36 @@ static void write_cinfo(const string &d, const string &p, const CI &ci) {
 template <typename S> struct OpSet {
     map<OpExpr<S> *, int> ids;
     vector<shared_ptr<SparseMatrix<S, double>>> mats;
+    // tf: when given, an operator of a delayed tensor that has an
+    // expression (DelayedOperatorTensor::exprs) is formed from it the way
+    // the reference's OpProduct dispatch forms it before each multiply
+    // (tensor_functions.hpp, Prod case: tensor_product of the delayed
+    // tensor's sub-operators)
     int get(const shared_ptr<OperatorTensor<S, double>> &opt,
-            const shared_ptr<OpExpr<S>> &x) {
+            const shared_ptr<OpExpr<S>> &x,
+            const shared_ptr<TensorFunctions<S, double>> &tf = nullptr) {
         auto it = ids.find(x.get());
         if (it != ids.end())
             return it->second;
         assert(opt->ops.count(x));
         int id = (int)mats.size();
         ids[x.get()] = id;
-        mats.push_back(opt->ops.at(x));
+        shared_ptr<SparseMatrix<S, double>> m = opt->ops.at(x);
+        if (m->data == nullptr && tf != nullptr &&
+            opt->get_type() == OperatorTensorTypes::Delayed) {
+            auto dopt = dynamic_pointer_cast<DelayedOperatorTensor<S, double>>(opt);
+            if (dopt->exprs.count(x)) {
+                auto mm = make_shared<SparseMatrix<S, double>>(
+                    make_shared<VectorAllocator<double>>());
+                mm->allocate(m->info);
+                const pair<size_t, double> zf = dopt->exprs.at(x);
+                tf->tensor_product(dopt->mat->data[zf.first] * zf.second,
+                                   dopt->lopt->ops, dopt->ropt->ops, mm);
+                mm->factor = m->factor;
+                m = mm;
+            }
+        }
+        mats.push_back(m);
         return id;
     }
     void write(const string &d, const string &p,
@@ -142,7 +163,7 @@ template <typename S> struct OpSet {
         const {
         vector<uint64_t> dq, off;
         vector<double> factor, data;
-        vector<int32_t> info_idx;
+        vector<int32_t> info_idx, flags;
         for (auto &m : mats) {
             dq.push_back(qdata(m->info->delta_quantum));
             factor.push_back(m->factor);
@@ -153,8 +174,34 @@ template <typename S> struct OpSet {
             assert(ii != -1 && infos[ii].second->n == m->info->n);
             info_idx.push_back(ii);
             off.push_back(data.size());
+            // a delayed operator has structure but no data
+            flags.push_back(m->data == nullptr ? 1 : 0);
+            if (m->data != nullptr)
+                data.insert(data.end(), m->data, m->data + m->total_memory);
+        }
+        write_npy(d + "/" + p + "op_dq.npy", dq);
+        write_npy(d + "/" + p + "op_factor.npy", factor);
+        write_npy(d + "/" + p + "op_info.npy", info_idx);
+        write_npy(d + "/" + p + "op_off.npy", off);
+        write_npy(d + "/" + p + "op_flags.npy", flags);
+        write_npy(d + "/" + p + "op_data.npy", data);
+    }
+    // sub-operators of a delayed tensor: each with its own info
+    void write_own(const string &d, const string &p) const {
+        InfoPack<S> ip;
+        vector<uint64_t> dq, off;
+        vector<double> factor, data;
+        vector<int32_t> info_idx;
+        for (auto &m : mats) {
+            dq.push_back(qdata(m->info->delta_quantum));
+            factor.push_back(m->factor);
+            info_idx.push_back((int32_t)info_idx.size());
+            ip.add(m->info);
+            off.push_back(data.size());
+            assert(m->data != nullptr);
             data.insert(data.end(), m->data, m->data + m->total_memory);
         }
+        ip.write(d, p + "info");
         write_npy(d + "/" + p + "op_dq.npy", dq);
         write_npy(d + "/" + p + "op_factor.npy", factor);
         write_npy(d + "/" + p + "op_info.npy", info_idx);
@@ -207,23 +254,111 @@ template <typename S> struct Dumper : CallbackKernel {
             strings.push_back(dynamic_pointer_cast<OpProduct<S, double>>(expr));
         else
             assert(false);
-        OpSet<S> lset, rset;
-        vector<uint8_t> tconj;
-        vector<int32_t> ta, tb;
+        OpSet<S> lset, rset, dlset, drset;
+        vector<uint8_t> tconj, tkind, tdconj;
+        vector<int32_t> ta, tb, tda, tdb;
         vector<double> tf;
+        const bool dleft =
+            op->lopt->get_type() == OperatorTensorTypes::Delayed;
+        const bool dright =
+            op->ropt->get_type() == OperatorTensorTypes::Delayed;
+        shared_ptr<DelayedOperatorTensor<S, double>> dopt =
+            dleft ? dynamic_pointer_cast<DelayedOperatorTensor<S, double>>(
+                        op->lopt)
+                  : dright
+                        ? dynamic_pointer_cast<
+                              DelayedOperatorTensor<S, double>>(op->ropt)
+                        : nullptr;
+        {
+            int nprod = 0, nsp = 0;
+            for (auto &s : strings)
+                (s->get_type() == OpTypes::SumProd ? nsp : nprod)++;
+            cerr << "site " << isite << ": " << nprod << " OpProduct, " << nsp
+                 << " OpSumProd strings; lopt "
+                 << (op->lopt->get_type() == OperatorTensorTypes::Delayed
+                         ? "delayed"
+                         : "normal")
+                 << ", ropt "
+                 << (op->ropt->get_type() == OperatorTensorTypes::Delayed
+                         ? "delayed"
+                         : "normal")
+                 << endl;
+        }
         for (auto &s : strings) {
-            assert(s->get_type() == OpTypes::Prod); // no delayed terms
             tconj.push_back(s->conj);
-            ta.push_back(lset.get(op->lopt, s->a));
-            tb.push_back(rset.get(op->ropt, s->b));
+            const bool prod = s->get_type() == OpTypes::Prod;
+            ta.push_back(lset.get(op->lopt, s->a, prod ? h->tf : nullptr));
+            tb.push_back(rset.get(op->ropt, s->b, prod ? h->tf : nullptr));
             tf.push_back(s->factor);
+            if (s->get_type() == OpTypes::SumProd) {
+                // delayed three-tensor string (tensor_functions.hpp SumProd
+                // dispatch): sub-operators of the delayed tensor
+                auto sp = dynamic_pointer_cast<OpSumProd<S, double>>(s);
+                assert(dopt != nullptr && sp->ops.size() == 2);
+                tkind.push_back(1);
+                tda.push_back(dlset.get(dopt->lopt, sp->ops[0]));
+                tdb.push_back(drset.get(dopt->ropt, sp->ops[1]));
+                tdconj.push_back((uint8_t)((int)sp->conjs[0] |
+                                           ((int)sp->conjs[1] << 1)));
+            } else {
+                assert(s->get_type() == OpTypes::Prod);
+                tkind.push_back(0);
+                tda.push_back(-1);
+                tdb.push_back(-1);
+                tdconj.push_back(0);
+            }
         }
         write_npy(d + "/term_conj.npy", tconj);
         write_npy(d + "/term_a.npy", ta);
         write_npy(d + "/term_b.npy", tb);
         write_npy(d + "/term_factor.npy", tf);
+        write_npy(d + "/term_kind.npy", tkind);
+        write_npy(d + "/term_da.npy", tda);
+        write_npy(d + "/term_db.npy", tdb);
+        write_npy(d + "/term_dconj.npy", tdconj);
         lset.write(d, "l", h->left_op_infos);
         rset.write(d, "r", h->right_op_infos);
+        if (dopt != nullptr) {
+            dlset.write_own(d, "dl");
+            drset.write_own(d, "dr");
+            // tensor-product connection lists of the delayed side's infos
+            const auto &side = dleft ? h->left_op_infos : h->right_op_infos;
+            vector<int32_t> tn, tnc;
+            vector<uint64_t> tq, tstride;
+            vector<uint32_t> tidx, tia, tib, tic;
+            vector<double> tfac;
+            for (auto &x : side) {
+                auto ci = x.second->cinfo;
+                int n5[5] = {-1, -1, -1, -1, -1}, nc = 0;
+                if (ci != nullptr) {
+                    for (int k = 0; k < 5; k++)
+                        n5[k] = ci->n[k];
+                    nc = (int)ci->nc;
+                    for (int k = 0; k < ci->n[4]; k++) {
+                        tq.push_back(qdata(ci->quanta[k]));
+                        tidx.push_back(ci->idx[k]);
+                    }
+                    for (int k = 0; k < nc; k++) {
+                        tia.push_back(ci->ia[k]);
+                        tib.push_back(ci->ib[k]);
+                        tic.push_back(ci->ic[k]);
+                        tstride.push_back(ci->stride[k]);
+                        tfac.push_back(ci->factor[k]);
+                    }
+                }
+                tn.insert(tn.end(), n5, n5 + 5);
+                tnc.push_back(nc);
+            }
+            write_npy(d + "/tp_n.npy", tn);
+            write_npy(d + "/tp_nc.npy", tnc);
+            write_npy(d + "/tp_quanta.npy", tq);
+            write_npy(d + "/tp_idx.npy", tidx);
+            write_npy(d + "/tp_ia.npy", tia);
+            write_npy(d + "/tp_ib.npy", tib);
+            write_npy(d + "/tp_ic.npy", tic);
+            write_npy(d + "/tp_stride.npy", tstride);
+            write_npy(d + "/tp_factor.npy", tfac);
+        }
         // reference ConnectionInfo (wfn) and recomputed diag ConnectionInfo
         assert(h->wfn_infos.size() >= 1 && h->wfn_infos[0] != nullptr);
         write_cinfo<S>(d, "cwfn", h->wfn_infos[0]);
@@ -304,6 +439,7 @@ template <typename S> struct Dumper : CallbackKernel {
         m << "dav_t_s " << get<3>(dres) << "\n";
         m << "const_e " << dmrg->me->mpo->const_e << "\n";
         m << "threads " << G.threads << "\n";
+        m << "delayed_side " << (dleft ? 1 : dright ? 2 : 0) << "\n";
     }
 };
 
@@ -425,6 +561,8 @@ int main(int argc, char **argv) {
             G.dump_conv = stod(v);
         else if (k == "--noise-type")
             G.noise_type = v;
+        else if (k == "--delayed")
+            G.delayed = stoi(v) != 0;
         else {
             cerr << "unknown option " << k << endl;
             return 2;

@@ -40,16 +40,25 @@ def write_hubbard(path, n, u, t):
 
 
 def run(name, fcidump, sym, bdims, noises, nsweeps, dump_sweep, dump_site,
-        work, dav_conv="1e-9"):
+        work, dav_conv="1e-9", delayed=False, names=None):
+    """One reference DMRG run; dumps site `dump_site` (an int, or a list of
+    sites written to the fixtures `names`).  delayed=True turns on the
+    reference's delayed contraction of the environments
+    (MovingEnvironment::delayed_contraction = normal_ops, the Python
+    driver's default), whose effective Hamiltonians hold three-tensor
+    strings."""
     out = os.path.join(work, name)
     if os.path.exists(out):
         shutil.rmtree(out)
+    sites = dump_site if isinstance(dump_site, list) else [dump_site]
+    names = names or [name]
     cmd = [B2REF, "--fcidump", fcidump, "--sym", sym, "--out", out,
            "--bdims", ",".join(map(str, bdims)),
            "--noises", ",".join(map(str, noises)),
            "--nsweeps", str(nsweeps), "--threads", "8",
-           "--dump-sweep", str(dump_sweep), "--dump-sites", str(dump_site),
-           "--dav-conv", dav_conv]
+           "--dump-sweep", str(dump_sweep),
+           "--dump-sites", ",".join(map(str, sites)),
+           "--dav-conv", dav_conv, "--delayed", "1" if delayed else "0"]
     print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True, stdout=subprocess.DEVNULL)
     energies = {}
@@ -58,7 +67,12 @@ def run(name, fcidump, sym, bdims, noises, nsweeps, dump_sweep, dump_site,
             p = line.split()
             if p[0] == "final":
                 energies["final"] = float(p[1])
-    site_dir = os.path.join(out, "site%d_sw%d" % (dump_site, dump_sweep))
+    for site, fname in zip(sites, names):
+        pack(os.path.join(out, "site%d_sw%d" % (site, dump_sweep)), fname)
+    return energies
+
+
+def pack(site_dir, name):
     if os.path.isdir(site_dir):
         arrs = {}
         for fn in sorted(os.listdir(site_dir)):
@@ -72,12 +86,37 @@ def run(name, fcidump, sym, bdims, noises, nsweeps, dump_sweep, dump_site,
         arrs["meta_json"] = np.frombuffer(json.dumps(meta).encode(), np.uint8)
         np.savez_compressed(os.path.join(HERE, name + ".npz"), **arrs)
         print("  wrote", name + ".npz", flush=True)
-    return energies
+
+
+def delayed_fixtures(work):
+    """Sites whose effective Hamiltonian has a delayed left (site 3) or
+    delayed right (site 4) operator tensor, N2/STO-3G SU(2) and Hubbard
+    L=8 SZ."""
+    data = os.path.join(REF, "data")
+    res = {}
+    res["n2_su2_delayed"] = run(
+        "n2_su2_delayed", os.path.join(data, "N2.STO3G.FCIDUMP"), "su2",
+        [100], [1e-6, 1e-7, 0], 8, 2, [3, 4], work, delayed=True,
+        names=["n2_su2_dleft", "n2_su2_dright"])
+    res["hub8_sz_delayed"] = run(
+        "hub8_sz_delayed", os.path.join(data, "HUBBARD-L8.FCIDUMP"), "sz",
+        [200], [1e-6, 0], 8, 1, [3, 4], work, delayed=True,
+        names=["hub8_sz_dleft", "hub8_sz_dright"])
+    return res
 
 
 def main():
     work = os.path.join(REPO, "_work", "fixtures")
     os.makedirs(work, exist_ok=True)
+    if "--delayed-only" in sys.argv:
+        res = delayed_fixtures(work)
+        with open(os.path.join(HERE, "energies.json")) as f:
+            old = json.load(f)
+        old.update(res)
+        with open(os.path.join(HERE, "energies.json"), "w") as f:
+            json.dump(old, f, indent=1, sort_keys=True)
+        print(json.dumps(res, indent=1))
+        return 0
     hub16 = os.path.join(work, "HUBBARD-L16-U4.FCIDUMP")
     write_hubbard(hub16, 16, 4.0, 1.0)
     data = os.path.join(REF, "data")
@@ -94,6 +133,7 @@ def main():
                                [1e-5, 1e-5, 1e-5, 1e-5, 0], 10, 4, 7, work,
                                dav_conv="1e-8")
     # energy-only runs (known answers)
+    res.update(delayed_fixtures(work))
     res["n2_su2_m500"] = run("n2_su2_m500",
                              os.path.join(data, "N2.STO3G.FCIDUMP"), "su2",
                              [500], [1e-6, 1e-7, 0], 10, -1, -1, work)
