@@ -15,14 +15,17 @@ from . import _lib as L
 
 
 class Context:
-    """A device context (one HIP stream) of libb2g."""
+    """A device context (one HIP stream) of libb2g.  Objects created on it
+    (effective Hamiltonians) are released before the context itself."""
 
     def __init__(self, device=0):
+        import weakref
         self.lib = L.lib()
         h = C.c_void_p()
         L.check(self.lib.b2g_ctx_create(int(device), 0, C.byref(h)))
         self.h = h
         self.device = torch.device("cuda", int(device))
+        self._children = weakref.WeakSet()
 
     def synchronize(self):
         L.check(self.lib.b2g_ctx_synchronize(self.h))
@@ -35,6 +38,8 @@ class Context:
 
     def close(self):
         if self.h:
+            for child in list(self._children):
+                child.deallocate()
             self.lib.b2g_ctx_destroy(self.h)
             self.h = None
 
@@ -89,6 +94,7 @@ class EffectiveHamiltonian:
         L.check(self.lib.b2g_effham_create(ctx.h, C.byref(cp), flags,
                                            C.byref(h)))
         self.h = h
+        ctx._children.add(self)
         kl, bl, npairs, macs = (C.c_uint64(), C.c_uint64(), C.c_uint64(),
                                 C.c_uint64())
         L.check(self.lib.b2g_effham_info(h, C.byref(kl), C.byref(bl),

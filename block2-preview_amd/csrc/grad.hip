@@ -285,13 +285,16 @@ static int grad_plan_build(b2g_plan p, GradPlan **out) {
     return B2G_OK;
 }
 
+// stage 1 (PH 3) overwrites its scratch; stage 2 (PH 4) adds to the
+// gradient buffers
 template <int PH>
 static hipError_t run_launch(const Launch &L, const double *A, const double *B,
                              double *D, int grid, hipStream_t st) {
+    const int beta0 = PH == 4 ? 1 : 0;
     for (int c = 0; c < kTileConfigs; c++) {
         const hipError_t e = launch_seg_gemm<double, double, double, double, PH>(
             c, A, B, D, D, L.d_t[c], (int)L.t[c].size(), L.d_s, 0, 0, 0, 0, 1.0,
-            1, 1, grid, st);
+            beta0, 1, grid, st);
         if (e != hipSuccess)
             return e;
     }
