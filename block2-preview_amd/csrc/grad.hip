@@ -112,7 +112,10 @@ static bool launch_in_bounds(const Launch &L, uint64_t a_len, uint64_t b_len,
                 const uint64_t bmax =
                     s.b_off + (tb ? (uint64_t)(t.N - 1) * s.ldb + (s.K - 1)
                                   : (uint64_t)(s.K - 1) * s.ldb + (t.N - 1));
-                if (amax >= a_len || bmax >= b_len)
+                // 32-bit byte offsets within a segment's blocks
+                const uint64_t lim = ((uint64_t)1 << 31) / 8 - 2;
+                if (amax >= a_len || bmax >= b_len || amax - s.a_off >= lim ||
+                    bmax - s.b_off >= lim)
                     return false;
             }
         }

@@ -3,6 +3,8 @@
 #include "kernels.hpp"
 #include "launch.hpp"
 
+#include <type_traits>
+
 namespace b2g {
 
 typedef double v4d __attribute__((ext_vector_type(4)));
@@ -32,18 +34,151 @@ template <> struct Mfma<float> {
     }
 };
 
+// Raw buffer resource over [p, p + 2^31 - 16) bytes: loads at byte offsets
+// at or past the record count return zero without touching memory (buffer
+// range checking), which masks out-of-block elements.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void *p) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0,
+                                             0x7FFFFFF0, 0x00020000);
+}
+constexpr uint32_t kMasked = 0x80000000u; // offset past the record count
+
+template <typename T>
+__device__ __forceinline__ T buf_load(__amdgpu_buffer_rsrc_t r, uint32_t off);
+template <>
+__device__ __forceinline__ double buf_load<double>(__amdgpu_buffer_rsrc_t r,
+                                                   uint32_t off) {
+    return __builtin_bit_cast(double,
+                              __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+}
+template <>
+__device__ __forceinline__ float buf_load<float>(__amdgpu_buffer_rsrc_t r,
+                                                 uint32_t off) {
+    return __builtin_bit_cast(float,
+                              __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+}
+
 // One workgroup = 4 waves in a 2x2 arrangement; the workgroup tile is
 // TM x TN, each wave owns (TM/2) x (TN/2) as (TM/32) x (TN/32) MFMA tiles.
 // K is staged through two LDS buffers in chunks of TK: while the MFMAs run on
 // one buffer, the next chunk's global loads are in flight in registers and
 // are then written to the other buffer, so one barrier per chunk suffices.
-// Loads are branch-free (out-of-range elements read the block's first
-// element; a bit mask zeroes them when the chunk is written to LDS, where
-// the per-segment alpha is applied too), so the loaded registers are not
-// consumed before the MFMAs of the current chunk have been issued.  Tasks are strided
-// over the grid (persistent launch); blockIdx.y selects the right-hand side.
+// Loads are buffer loads at 32-bit byte offsets from the segment's blocks;
+// elements outside the block (rows past M, columns past N, k past K) use an
+// offset past the buffer's record count and read as zero, so nothing is
+// masked when the chunk is written to LDS (where the per-segment alpha is
+// applied).  Element q of thread tid is e = tid + 256 q; since 256 is a
+// multiple of TK, TM and TN, one coordinate of e is the same for all q and
+// the other advances by a constant, so offsets advance by a uniform step.
+// Tasks are strided over the grid (persistent launch); blockIdx.y selects
+// the right-hand side.
 template <typename TA, typename TB, typename TC, int TM, int TN, int TK>
-struct TileLoader {
+struct TileLoaderB {
+    static constexpr int NA = TM * TK / 256, NB = TN * TK / 256;
+    static_assert(256 % TK == 0 && 256 % TM == 0 && 256 % TN == 0,
+                  "tile shape");
+    TA ra[NA];
+    TB rb[NB];
+    __device__ __forceinline__ void load(const TA *__restrict__ A,
+                                         const TB *__restrict__ B,
+                                         const Seg &sg, uint32_t k0,
+                                         const Task &task, int tid) {
+        const bool ta = sg.flags & 1, tb = (sg.flags >> 1) & 1;
+        const __amdgpu_buffer_rsrc_t rA = buf_rsrc(A + sg.a_off);
+        const __amdgpu_buffer_rsrc_t rB = buf_rsrc(B + sg.b_off);
+        if (!ta) { // A stored M x K: element (m, k) at m * lda + k
+            constexpr uint32_t DM = 256 / TK;
+            const uint32_t k = k0 + tid % TK;
+            uint32_t r = task.row0 + tid / TK;
+            uint32_t off = (r * sg.lda + k) * (uint32_t)sizeof(TA);
+            const uint32_t step = DM * sg.lda * (uint32_t)sizeof(TA);
+            const bool kok = k < sg.K;
+#pragma unroll
+            for (int q = 0; q < NA; q++) {
+                ra[q] = buf_load<TA>(rA, (kok && r < task.M) ? off : kMasked);
+                r += DM;
+                off += step;
+            }
+        } else { // A stored K x M: element (m, k) at k * lda + m
+            constexpr uint32_t DK = 256 / TM;
+            const uint32_t m = task.row0 + tid % TM;
+            uint32_t k = k0 + tid / TM;
+            uint32_t off = (k * sg.lda + m) * (uint32_t)sizeof(TA);
+            const uint32_t step = DK * sg.lda * (uint32_t)sizeof(TA);
+            const bool mok = m < task.M;
+#pragma unroll
+            for (int q = 0; q < NA; q++) {
+                ra[q] = buf_load<TA>(rA, (mok && k < sg.K) ? off : kMasked);
+                k += DK;
+                off += step;
+            }
+        }
+        if (!tb) { // B stored K x N: element (k, n) at k * ldb + n
+            constexpr uint32_t DK = 256 / TN;
+            const uint32_t n = task.col0 + tid % TN;
+            uint32_t k = k0 + tid / TN;
+            uint32_t off = (k * sg.ldb + n) * (uint32_t)sizeof(TB);
+            const uint32_t step = DK * sg.ldb * (uint32_t)sizeof(TB);
+            const bool nok = n < task.N;
+#pragma unroll
+            for (int q = 0; q < NB; q++) {
+                rb[q] = buf_load<TB>(rB, (nok && k < sg.K) ? off : kMasked);
+                k += DK;
+                off += step;
+            }
+        } else { // B stored N x K: element (k, n) at n * ldb + k
+            constexpr uint32_t DN = 256 / TK;
+            const uint32_t k = k0 + tid % TK;
+            uint32_t c = task.col0 + tid / TK;
+            uint32_t off = (c * sg.ldb + k) * (uint32_t)sizeof(TB);
+            const uint32_t step = DN * sg.ldb * (uint32_t)sizeof(TB);
+            const bool kok = k < sg.K;
+#pragma unroll
+            for (int q = 0; q < NB; q++) {
+                rb[q] = buf_load<TB>(rB, (kok && c < task.N) ? off : kMasked);
+                c += DN;
+                off += step;
+            }
+        }
+    }
+    // registers -> LDS buffer ([k][m] and [k][n] layouts), alpha applied;
+    // per layout, element q sits at a per-thread base plus a constant
+    __device__ __forceinline__ void store(TC *As, TC *Bs, const Seg &sg,
+                                          int tid) {
+        const bool ta = sg.flags & 1, tb = (sg.flags >> 1) & 1;
+        const TC alpha = (TC)sg.alpha;
+        if (!ta) { // m = tid / TK + q * 256 / TK, k = tid % TK
+            TC *p = As + (tid % TK) * (TM + 1) + tid / TK;
+#pragma unroll
+            for (int q = 0; q < NA; q++)
+                p[q * (256 / TK)] = (TC)ra[q] * alpha;
+        } else { // m = tid % TM, k = tid / TM + q * 256 / TM
+            TC *p = As + (tid / TM) * (TM + 1) + tid % TM;
+#pragma unroll
+            for (int q = 0; q < NA; q++)
+                p[q * (256 / TM) * (TM + 1)] = (TC)ra[q] * alpha;
+        }
+        if (!tb) { // n = tid % TN, k = tid / TN + q * 256 / TN
+            TC *p = Bs + (tid / TN) * (TN + 1) + tid % TN;
+#pragma unroll
+            for (int q = 0; q < NB; q++)
+                p[q * (256 / TN) * (TN + 1)] = (TC)rb[q];
+        } else { // n = tid / TK + q * 256 / TK, k = tid % TK
+            TC *p = Bs + (tid % TK) * (TN + 1) + tid / TK;
+#pragma unroll
+            for (int q = 0; q < NB; q++)
+                p[q * (256 / TK)] = (TC)rb[q];
+        }
+    }
+};
+
+// The earlier loader: 64-bit global addresses, out-of-block elements read
+// the block's first element and are zeroed through a bit mask when written
+// to LDS.  Measured faster than the buffer loader for 128-row A tiles,
+// where the buffer loader's offsets share registers with the previous
+// chunk's loads and wait on them; slower for the smaller tiles.
+template <typename TA, typename TB, typename TC, int TM, int TN, int TK>
+struct TileLoaderG {
     static constexpr int NA = TM * TK / 256, NB = TN * TK / 256;
     static_assert(NA + NB <= 32, "mask bits");
     TA ra[NA];
@@ -105,6 +240,11 @@ struct TileLoader {
         }
     }
 };
+
+template <typename TA, typename TB, typename TC, int TM, int TN, int TK>
+using TileLoader =
+    typename std::conditional<TM == 128, TileLoaderG<TA, TB, TC, TM, TN, TK>,
+                              TileLoaderB<TA, TB, TC, TM, TN, TK>>::type;
 
 template <typename TA, typename TB, typename TD, typename TC, int TM, int TN,
           int TK, int PHASE>

@@ -584,6 +584,13 @@ static int compile_plan(b2g_plan p, const b2g_pair *pairs, size_t n_pairs,
         const uint64_t vld = q.v_ld ? q.v_ld : q.v_n;
         B2G_REQUIRE(cld >= q.c_n && vld >= q.v_n, B2G_ERR_ARG,
                     ss.str() + "row stride below the window width");
+        // the kernels address each operand block with 32-bit byte offsets
+        const uint64_t lim = ((uint64_t)1 << 31) / 8 - 2;
+        B2G_REQUIRE((uint64_t)q.c_m * cld < lim && (uint64_t)q.v_m * vld < lim &&
+                        (uint64_t)q.c_m * q.v_m < lim &&
+                        (uint64_t)q.c_n * q.v_n < lim &&
+                        (uint64_t)q.c_m * q.v_n < lim,
+                    B2G_ERR_ARG, ss.str() + "block of 2^28 elements or more");
         B2G_REQUIRE(q.c_m == 0 || q.c_n == 0 ||
                         q.c_off + (q.c_m - 1) * cld + q.c_n <= p->x_len,
                     B2G_ERR_ARG, ss.str() + "C block outside the input vector");
