@@ -34,6 +34,21 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def profiled_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3
+    FETCH_SIZE + WRITE_SIZE summary (profiles/*.md, copy-calibrated), or
+    None when no profile of that kernel is committed."""
+    import glob
+    import re
+    for path in sorted(glob.glob(os.path.join(REPO, "profiles", "*.md"))):
+        with open(path) as f:
+            for line in f:
+                m = re.match(r"traffic_bytes_per_launch (\S+) (\S+)", line)
+                if m and m.group(1) == kernel:
+                    return float(m.group(2))
+    return None
+
+
 def cpu_baseline(pb, ldata, rdata, seconds, threads):
     """Oracle restatement timed on a bounded prefix of the recorded pairs:
     a 64-pair calibration run gives the cost per pair, then a prefix sized
@@ -284,7 +299,7 @@ def main():
                      "achieved": round(achieved, 4), "peak": FP64_MFMA_PEAK,
                      "unit": "TFLOP/s",
                      "frac": round(achieved / FP64_MFMA_PEAK, 4),
-                     "traffic": None},
+                     "traffic": profiled_traffic(dom)},
         "kernels_fp64": kernels,
         "autotune": {"block_dim": rep, "tile_rates_tflops":
                      [round(r, 2) for r in rates],
