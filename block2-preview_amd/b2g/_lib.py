@@ -105,12 +105,14 @@ KERNEL_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_double, MATVEC_FN,
 
 # kernel ids of one H*psi (b2g.h): phase-1 / phase-2 grouped GEMM per tile
 # configuration c = 3*i + j (TM = 32 << i, TN = 32 << j), split-K reduction
-K_PHASE1, K_PHASE2, K_SPLITK_REDUCE, NKERNELS = 0, 9, 18, 19
+K_PHASE1, K_PHASE2, K_SPLITK_REDUCE, K_PACK, NKERNELS = 0, 9, 18, 19, 20
 
 
 def kernel_name(k):
     if k == K_SPLITK_REDUCE:
         return "splitk_reduce"
+    if k == K_PACK:
+        return "pack_bf16"
     c = k % 9
     return "phase%d_%dx%d" % (1 if k < K_PHASE2 else 2, 32 << (c // 3),
                               32 << (c % 3))
@@ -171,6 +173,7 @@ _SIGS = {
                                C.c_int]),
     "b2g_grad_flops": (C.c_int, [_vp, f64p]),
     "b2g_plan_refresh": (C.c_int, [_vp]),
+    "b2g_plan_reserve": (C.c_int, [_vp, C.c_int, C.c_int]),
     "b2g_effham_set_davidson_mode": (C.c_int, [_vp, C.c_int]),
     "b2g_effham_set_kernel": (C.c_int, [_vp, _vp, _vp]),
     "b2g_effham_eigs": (C.c_int, [_vp, _vp, C.c_int, C.c_double, C.c_int,

@@ -1,18 +1,29 @@
-// Segmented grouped GEMM in split-BF16 three-term arithmetic (gfx950).
+// Segmented grouped GEMM in split-BF16 three-term arithmetic (gfx950) on
+// pre-split operands.
 //
-// Every operand value a (FP64 or FP32 in memory) is rounded to FP32 and split
-// into two BF16 numbers, hi = bf16(a) and lo = bf16(a - hi), so that
-// a = hi + lo to about 2^-17 relative.  A product is then
+// Every operand value a is split once into two BF16 numbers, hi = bf16(a)
+// and lo = bf16(a - hi), so that a = hi + lo to about 2^-17 relative.  A
+// product is then
 //     a * b ~= a_hi * b_hi + a_hi * b_lo + a_lo * b_hi
 // (the dropped a_lo * b_lo is ~2^-16 relative), each term one
 // v_mfma_f32_16x16x32_bf16 with FP32 accumulation.  Three BF16 MFMAs cost
 // 3 * 16 cycles per 16x16x32 block against 8 * 32 cycles for the same block
-// with FP32 16x16x4 MFMAs, so the split trades a ~1e-5 relative error for
-// 5x the matrix throughput; the Davidson that uses it refines in FP64.
+// with FP32 16x16x4 MFMAs; the Davidson that uses them refines in FP64.
+//
+// Packed planes.  A GEMM operand is held as a row-major R x ld matrix whose
+// rows run along the contraction index k (ld = K rounded up to 8, columns
+// [K, ld) zero), stored twice with identical layout: the hi plane and the lo
+// plane.  Eight consecutive k of a row are one aligned 16-byte vector, so the
+// tile loader issues 16-byte buffer loads and 16-byte LDS stores and does no
+// arithmetic.  The constant operators are packed once per plan (in the
+// orientation each pair reads them), the wavefunction once per H*psi, and
+// phase 1 writes its product W directly as packed W^T planes for phase 2.
 //
 // Task / segment semantics are those of seg_gemm_kernel (kernels.hip): a
-// task is one TM x TN output tile accumulating sum_s alpha_s op(A_s) op(B_s)
-// over its segments, the tile owned by exactly one task (no atomics).
+// task is one TM x TN output tile accumulating sum_s alpha_s A_s B_s^T over
+// its segments (A_s: M x K_s, B_s: N x K_s, both packed), the tile owned by
+// exactly one task (no atomics).  alpha_s is applied to each segment's FP32
+// partial sum when the segment ends.
 #include "kernels.hpp"
 #include "launch.hpp"
 
@@ -21,6 +32,7 @@ namespace b2g {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 // Lane l of v_mfma_f32_16x16x32_bf16 supplies A[l%16][8*(l/16) + i] and
 // B[8*(l/16) + i][l%16] (i < 8); accumulator element r of lane l is
@@ -29,142 +41,171 @@ __device__ __forceinline__ f32x4 mfma_bf16(bf16x8 a, bf16x8 b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
-__device__ __forceinline__ void split4(const float v[4], bf16x4 &hi,
-                                       bf16x4 &lo) {
+// raw buffer resource over one packed matrix: offsets are 32-bit byte
+// offsets below 2^31 (matrices hold fewer than 2^28 elements); an offset of
+// kPkMasked lies past the record count and reads as zero
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t pk_rsrc(const __bf16 *p) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<__bf16 *>(p),
+                                             (short)0, 0x7FFFFFF0, 0x00020000);
+}
+constexpr uint32_t kPkMasked = 0x80000000u;
+
+__device__ __forceinline__ u32x4 pk_load(__amdgpu_buffer_rsrc_t r,
+                                         uint32_t off) {
+    return __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0);
+}
+
+// ------------------------------------------------------------------ packing
+// One PackTile: rows [0, rows) x k groups [0, groups) of a packed matrix,
+// element (r, k) from src[r*src_ld + k] (trans: src[k*src_ld + r]) for
+// k < cols and zero for cols <= k < 8*groups.  256 threads: one 8-element
+// group each, consecutive lanes along the source's contiguous index.
+__global__ __launch_bounds__(256) void pack_bf16_kernel(
+    const double *__restrict__ src, __bf16 *__restrict__ hi,
+    __bf16 *__restrict__ lo, const PackTile *__restrict__ tiles, int ntiles,
+    uint64_t src_rs, uint64_t dst_rs) {
+    const int rhs = blockIdx.y;
+    src += rhs * src_rs;
+    hi += rhs * dst_rs;
+    lo += rhs * dst_rs;
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const PackTile pt = tiles[t];
+        uint32_t r, g;
+        if (pt.trans)
+            r = threadIdx.x & 31, g = threadIdx.x >> 5;
+        else
+            g = threadIdx.x & 7, r = threadIdx.x >> 3;
+        if (r >= pt.rows || g >= pt.groups)
+            continue;
+        bf16x8 h, l;
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-        const __bf16 h = (__bf16)v[i];
-        hi[i] = h;
-        lo[i] = (__bf16)(v[i] - (float)h);
+        for (int i = 0; i < 8; i++) {
+            const uint32_t k = 8 * g + i;
+            double v = 0.0;
+            if (k < pt.cols)
+                v = src[pt.src + (pt.trans ? (uint64_t)k * pt.src_ld + r
+                                           : (uint64_t)r * pt.src_ld + k)];
+            const __bf16 vh = (__bf16)(float)v;
+            h[i] = vh;
+            l[i] = (__bf16)(float)(v - (double)(float)vh);
+        }
+        const uint64_t o = pt.dst + (uint64_t)r * pt.dst_ld + 8 * g;
+        *(bf16x8 *)(hi + o) = h;
+        *(bf16x8 *)(lo + o) = l;
     }
 }
 
-// One workgroup = 4 waves (2x2); K staged through two LDS buffers in chunks
-// of 32 (one MFMA k-step).  LDS rows hold 32 k values of one tile row (A) or
-// one tile column (B) plus 8 of padding, so that a lane's 8 k values are one
-// 16-byte read and the 16 rows of a fragment fall on distinct banks.
-template <typename TA, typename TB, int TM, int TN> struct Bf3Tile {
-    static constexpr int TK = 32, KG = TK / 4, LDK = TK + 8;
-    static constexpr int GA = TM * KG / 256, GB = TN * KG / 256;
-    static_assert(GA >= 1 && GB >= 1 && 4 * (GA + GB) <= 32, "tile shape");
-    // per-buffer element counts (BF16): A hi, A lo, B hi, B lo
+hipError_t launch_pack_bf16(const double *src, __bf16 *hi, __bf16 *lo,
+                            const PackTile *tiles, int ntiles,
+                            uint64_t src_rs, uint64_t dst_rs, int n_rhs,
+                            int grid, hipStream_t st) {
+    if (ntiles <= 0)
+        return hipSuccess;
+    dim3 g((unsigned)std::min(ntiles, grid), (unsigned)n_rhs);
+    hipLaunchKernelGGL(pack_bf16_kernel, g, dim3(256), 0, st, src, hi, lo,
+                       tiles, ntiles, src_rs, dst_rs);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------- tile loader
+// One workgroup = 4 waves (2x2); K staged through two LDS buffers in steps
+// of 32 (one MFMA k-step).  An LDS row holds the 32 k values of one tile row
+// (A) or tile column (B) plus 8 of padding, so that a lane's 8 k values are
+// one 16-byte read and the 16 rows of a fragment fall on distinct banks.
+template <int TM, int TN> struct PkTile {
+    static constexpr int TK = 32, LDK = TK + 8;
+    static constexpr int VA = TM * TK / 8, VB = TN * TK / 8; // 16 B vectors
+    static constexpr int GA = (VA + 255) / 256, GB = (VB + 255) / 256;
     static constexpr int ASZ = TM * LDK, BSZ = TN * LDK;
-    static constexpr int BUF = 2 * ASZ + 2 * BSZ;
-    TA ra[GA][4];
-    TB rb[GB][4];
-    uint32_t okmask;
+    static constexpr int BUF = 2 * ASZ + 2 * BSZ; // A hi, A lo, B hi, B lo
+    u32x4 ah[GA], al[GA], bh[GB], bl[GB];
 
-    // (row, k-group) of load group g of thread tid: consecutive lanes run
-    // along the dimension that is contiguous in memory
-    static __device__ __forceinline__ void a_coord(bool ta, int e, int &m,
-                                                   int &kg) {
-        m = ta ? e % TM : e / KG;
-        kg = ta ? e / TM : e % KG;
+    // vector e of a tile: row and k group.  Eight consecutive lanes take one
+    // k group of eight consecutive rows, so that a wave's 16-byte LDS stores
+    // follow the bank pattern of the fragment reads (rows 80 bytes apart);
+    // a wave still covers 16 whole rows of 64 bytes per global load.
+    static __device__ __forceinline__ int vrow(int e) {
+        return (e & 7) + 8 * (e >> 5);
     }
-    static __device__ __forceinline__ void b_coord(bool tb, int e, int &n,
-                                                   int &kg) {
-        n = tb ? e / KG : e % TN;
-        kg = tb ? e % KG : e / TN;
-    }
+    static __device__ __forceinline__ int vgrp(int e) { return (e >> 3) & 3; }
 
-    __device__ __forceinline__ void load(const TA *__restrict__ A,
-                                         const TB *__restrict__ B,
-                                         const Seg &sg, uint32_t k0,
-                                         const Task &task, int tid) {
-        const bool ta = sg.flags & 1, tb = (sg.flags >> 1) & 1;
-        uint32_t mask = 0;
+    __device__ __forceinline__ void load(
+        __amdgpu_buffer_rsrc_t rAh, __amdgpu_buffer_rsrc_t rAl,
+        __amdgpu_buffer_rsrc_t rBh, __amdgpu_buffer_rsrc_t rBl, const Seg &sg,
+        uint32_t k0, const Task &task, int tid) {
 #pragma unroll
         for (int g = 0; g < GA; g++) {
-            int m, kg;
-            a_coord(ta, tid + g * 256, m, kg);
-            const uint32_t r = task.row0 + m;
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const uint32_t kk = k0 + 4 * kg + i;
-                const bool ok = r < task.M && kk < sg.K;
-                const uint64_t idx =
-                    ok ? sg.a_off + (ta ? (uint64_t)kk * sg.lda + r
-                                        : (uint64_t)r * sg.lda + kk)
-                       : sg.a_off;
-                ra[g][i] = A[idx];
-                mask |= (uint32_t)ok << (4 * g + i);
+            const int e = tid + 256 * g;
+            if (VA % 256 == 0 || e < VA) {
+                const uint32_t r = task.row0 + vrow(e);
+                const uint32_t k = k0 + 8 * vgrp(e);
+                const uint32_t off = (r < task.M && k < sg.K)
+                                         ? 2 * (r * sg.lda + k)
+                                         : kPkMasked;
+                ah[g] = pk_load(rAh, off);
+                al[g] = pk_load(rAl, off);
             }
         }
 #pragma unroll
         for (int g = 0; g < GB; g++) {
-            int n, kg;
-            b_coord(tb, tid + g * 256, n, kg);
-            const uint32_t c = task.col0 + n;
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const uint32_t kk = k0 + 4 * kg + i;
-                const bool ok = c < task.N && kk < sg.K;
-                const uint64_t idx =
-                    ok ? sg.b_off + (tb ? (uint64_t)c * sg.ldb + kk
-                                        : (uint64_t)kk * sg.ldb + c)
-                       : sg.b_off;
-                rb[g][i] = B[idx];
-                mask |= (uint32_t)ok << (4 * GA + 4 * g + i);
+            const int e = tid + 256 * g;
+            if (VB % 256 == 0 || e < VB) {
+                const uint32_t c = task.col0 + vrow(e);
+                const uint32_t k = k0 + 8 * vgrp(e);
+                const uint32_t off = (c < task.N && k < sg.K)
+                                         ? 2 * (c * sg.ldb + k)
+                                         : kPkMasked;
+                bh[g] = pk_load(rBh, off);
+                bl[g] = pk_load(rBl, off);
             }
         }
-        okmask = mask;
     }
 
-    // registers -> one LDS buffer: alpha applied to A, masked, split
-    __device__ __forceinline__ void store(__bf16 *buf, const Seg &sg,
-                                          int tid) const {
-        const bool ta = sg.flags & 1, tb = (sg.flags >> 1) & 1;
-        const float alpha = (float)sg.alpha;
-        __bf16 *Ah = buf, *Al = buf + ASZ, *Bh = buf + 2 * ASZ,
-               *Bl = buf + 2 * ASZ + BSZ;
+    __device__ __forceinline__ void store(__bf16 *buf, int tid) const {
 #pragma unroll
         for (int g = 0; g < GA; g++) {
-            int m, kg;
-            a_coord(ta, tid + g * 256, m, kg);
-            float v[4];
-#pragma unroll
-            for (int i = 0; i < 4; i++)
-                v[i] = ((okmask >> (4 * g + i)) & 1)
-                           ? (float)ra[g][i] * alpha
-                           : 0.0f;
-            bf16x4 hi, lo;
-            split4(v, hi, lo);
-            *(bf16x4 *)(Ah + m * LDK + 4 * kg) = hi;
-            *(bf16x4 *)(Al + m * LDK + 4 * kg) = lo;
+            const int e = tid + 256 * g;
+            if (VA % 256 == 0 || e < VA) {
+                const int o = vrow(e) * LDK + 8 * vgrp(e);
+                *(u32x4 *)(buf + o) = ah[g];
+                *(u32x4 *)(buf + ASZ + o) = al[g];
+            }
         }
 #pragma unroll
         for (int g = 0; g < GB; g++) {
-            int n, kg;
-            b_coord(tb, tid + g * 256, n, kg);
-            float v[4];
-#pragma unroll
-            for (int i = 0; i < 4; i++)
-                v[i] = ((okmask >> (4 * GA + 4 * g + i)) & 1)
-                           ? (float)rb[g][i]
-                           : 0.0f;
-            bf16x4 hi, lo;
-            split4(v, hi, lo);
-            *(bf16x4 *)(Bh + n * LDK + 4 * kg) = hi;
-            *(bf16x4 *)(Bl + n * LDK + 4 * kg) = lo;
+            const int e = tid + 256 * g;
+            if (VB % 256 == 0 || e < VB) {
+                const int o = vrow(e) * LDK + 8 * vgrp(e);
+                *(u32x4 *)(buf + 2 * ASZ + o) = bh[g];
+                *(u32x4 *)(buf + 2 * ASZ + BSZ + o) = bl[g];
+            }
         }
     }
 };
 
-template <typename TA, typename TB, typename TD, int TM, int TN>
-__global__ __launch_bounds__(256) void seg_gemm_bf3_kernel(
-    const TA *__restrict__ A, const TB *__restrict__ B, TD *__restrict__ D,
-    TD *__restrict__ P, const Task *__restrict__ tasks, int ntasks,
-    const Seg *__restrict__ segs, uint64_t a_rs, uint64_t b_rs, uint64_t d_rs,
-    uint64_t p_rs, double scale, int beta0, XcdSplit xs) {
-    typedef Bf3Tile<TA, TB, TM, TN> T;
+// OUT 0: D0/D1 are the hi/lo planes of the packed transpose of the output
+//        (element (m, n) of a task's block at d_off + n*ldd + m, ldd the
+//        packed row length; rows M <= m < ldd receive the zeros of the
+//        masked A rows, so the padding of W^T is rewritten by every launch);
+// OUT 1: D0 is the FP64 output (row-major, ldd), D1 the FP64 split-K
+//        partial buffer (task.beta bit 1), as in seg_gemm_kernel.
+template <int TM, int TN, int OUT>
+__global__ __launch_bounds__(256) void seg_gemm_pk_kernel(
+    const __bf16 *__restrict__ Ah, const __bf16 *__restrict__ Al,
+    const __bf16 *__restrict__ Bh, const __bf16 *__restrict__ Bl,
+    void *__restrict__ D0, void *__restrict__ D1,
+    const Task *__restrict__ tasks, int ntasks, const Seg *__restrict__ segs,
+    uint64_t a_rs, uint64_t b_rs, uint64_t d_rs, uint64_t p_rs, double scale,
+    int beta0, XcdSplit xs) {
+    typedef PkTile<TM, TN> T;
     constexpr int WM = TM / 2, WN = TN / 2;
     constexpr int FM = WM / 16, FN = WN / 16;
     __shared__ __attribute__((aligned(16))) __bf16 smem[2 * T::BUF];
     const int rhs = blockIdx.y;
-    A += rhs * a_rs;
-    B += rhs * b_rs;
-    D += rhs * d_rs;
-    P += rhs * p_rs;
+    Ah += rhs * a_rs;
+    Al += rhs * a_rs;
+    Bh += rhs * b_rs;
+    Bl += rhs * b_rs;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
     const int lr = lane & 15, lk = lane >> 4;
@@ -178,105 +219,166 @@ __global__ __launch_bounds__(256) void seg_gemm_bf3_kernel(
     }
     for (int t = t0; t < tend; t += tstep) {
         const Task task = tasks[t];
-        f32x4 acc[FM][FN];
+        // acc: the tile's sum; part: the running segment's unscaled sum
+        // (phase 1 tasks have one segment with alpha = 1: part is the sum)
+        f32x4 acc[FM][FN], part[FM][FN];
 #pragma unroll
         for (int i = 0; i < FM; i++)
 #pragma unroll
-            for (int j = 0; j < FN; j++)
+            for (int j = 0; j < FN; j++) {
                 acc[i][j] = f32x4{0, 0, 0, 0};
+                part[i][j] = f32x4{0, 0, 0, 0};
+            }
         uint32_t s = task.seg_begin, k0 = 0;
         if (s < task.seg_end) {
             Seg cur = segs[s];
             Seg nxt = cur;
             if (s + 1 < task.seg_end)
                 nxt = segs[s + 1];
+            auto rs = [&](const Seg &sg, __amdgpu_buffer_rsrc_t &a0,
+                          __amdgpu_buffer_rsrc_t &a1,
+                          __amdgpu_buffer_rsrc_t &b0,
+                          __amdgpu_buffer_rsrc_t &b1) {
+                a0 = pk_rsrc(Ah + sg.a_off);
+                a1 = pk_rsrc(Al + sg.a_off);
+                b0 = pk_rsrc(Bh + sg.b_off);
+                b1 = pk_rsrc(Bl + sg.b_off);
+            };
+            __amdgpu_buffer_rsrc_t rAh, rAl, rBh, rBl;
+            rs(cur, rAh, rAl, rBh, rBl);
             int buf = 0;
-            ld.load(A, B, cur, k0, task, tid);
-            ld.store(smem, cur, tid);
+            ld.load(rAh, rAl, rBh, rBl, cur, k0, task, tid);
+            ld.store(smem, tid);
             __syncthreads();
+            float alpha = (float)cur.alpha; // of the segment in LDS
             while (true) {
                 k0 += T::TK;
-                if (k0 >= cur.K) {
+                const bool seg_done = k0 >= cur.K;
+                const float a_done = alpha;
+                if (seg_done) {
                     k0 = 0;
                     s++;
                     cur = nxt;
                     if (s + 1 < task.seg_end)
                         nxt = segs[s + 1];
+                    rs(cur, rAh, rAl, rBh, rBl);
+                    alpha = (float)cur.alpha;
                 }
                 const bool more = s < task.seg_end;
                 if (more)
-                    ld.load(A, B, cur, k0, task, tid);
-                const __bf16 *Ah = smem + buf * T::BUF;
-                const __bf16 *Al = Ah + T::ASZ;
-                const __bf16 *Bh = Ah + 2 * T::ASZ;
-                const __bf16 *Bl = Bh + T::BSZ;
-                bf16x8 ah[FM], al[FM], bh[FN], bl[FN];
+                    ld.load(rAh, rAl, rBh, rBl, cur, k0, task, tid);
+                const __bf16 *sAh = smem + buf * T::BUF;
+                const __bf16 *sAl = sAh + T::ASZ;
+                const __bf16 *sBh = sAh + 2 * T::ASZ;
+                const __bf16 *sBl = sBh + T::BSZ;
+                bf16x8 fah[FM], fal[FM], fbh[FN], fbl[FN];
 #pragma unroll
                 for (int i = 0; i < FM; i++) {
                     const int o = (wm * WM + i * 16 + lr) * T::LDK + 8 * lk;
-                    ah[i] = *(const bf16x8 *)(Ah + o);
-                    al[i] = *(const bf16x8 *)(Al + o);
+                    fah[i] = *(const bf16x8 *)(sAh + o);
+                    fal[i] = *(const bf16x8 *)(sAl + o);
                 }
 #pragma unroll
                 for (int j = 0; j < FN; j++) {
                     const int o = (wn * WN + j * 16 + lr) * T::LDK + 8 * lk;
-                    bh[j] = *(const bf16x8 *)(Bh + o);
-                    bl[j] = *(const bf16x8 *)(Bl + o);
+                    fbh[j] = *(const bf16x8 *)(sBh + o);
+                    fbl[j] = *(const bf16x8 *)(sBl + o);
                 }
                 // small cross terms first, then the leading term
 #pragma unroll
                 for (int i = 0; i < FM; i++)
 #pragma unroll
                     for (int j = 0; j < FN; j++) {
-                        acc[i][j] = mfma_bf16(al[i], bh[j], acc[i][j]);
-                        acc[i][j] = mfma_bf16(ah[i], bl[j], acc[i][j]);
-                        acc[i][j] = mfma_bf16(ah[i], bh[j], acc[i][j]);
+                        part[i][j] = mfma_bf16(fal[i], fbh[j], part[i][j]);
+                        part[i][j] = mfma_bf16(fah[i], fbl[j], part[i][j]);
+                        part[i][j] = mfma_bf16(fah[i], fbh[j], part[i][j]);
                     }
+                if (OUT == 1 && seg_done) {
+                    mfma_settle<f32x4, FM * FN>(&part[0][0]);
+#pragma unroll
+                    for (int i = 0; i < FM; i++)
+#pragma unroll
+                        for (int j = 0; j < FN; j++) {
+                            acc[i][j] += a_done * part[i][j];
+                            part[i][j] = f32x4{0, 0, 0, 0};
+                        }
+                }
                 if (!more)
                     break;
                 buf ^= 1;
-                ld.store(smem + buf * T::BUF, cur, tid);
+                ld.store(smem + buf * T::BUF, tid);
                 __syncthreads();
             }
             __syncthreads();
         }
+        if constexpr (OUT == 0) {
+            mfma_settle<f32x4, FM * FN>(&part[0][0]);
+            __bf16 *Wh = (__bf16 *)D0 + rhs * d_rs;
+            __bf16 *Wl = (__bf16 *)D1 + rhs * d_rs;
 #pragma unroll
-        for (int i = 0; i < FM; i++)
+            for (int i = 0; i < FM; i++)
 #pragma unroll
-            for (int j = 0; j < FN; j++)
-#pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    const uint32_t row = task.row0 + wm * WM + i * 16 + 4 * lk + r;
+                for (int j = 0; j < FN; j++) {
+                    const uint32_t row = task.row0 + wm * WM + i * 16 + 4 * lk;
                     const uint32_t col = task.col0 + wn * WN + j * 16 + lr;
-                    if (row < task.M && col < task.N) {
-                        const uint64_t idx =
-                            task.d_off + (uint64_t)row * task.ldd + col;
-                        const double v = scale * (double)acc[i][j][r];
-                        if (task.beta & 2)
-                            P[idx] = (TD)v;
-                        else if ((task.beta | beta0) & 1)
-                            D[idx] = (TD)((double)D[idx] + v);
-                        else
-                            D[idx] = (TD)v;
+                    if (row < task.ldd && col < task.N) {
+                        bf16x4 h, l;
+#pragma unroll
+                        for (int r = 0; r < 4; r++) {
+                            const float v = part[i][j][r];
+                            const __bf16 vh = (__bf16)v;
+                            h[r] = vh;
+                            l[r] = (__bf16)(v - (float)vh);
+                        }
+                        const uint64_t o =
+                            task.d_off + (uint64_t)col * task.ldd + row;
+                        *(bf16x4 *)(Wh + o) = h;
+                        *(bf16x4 *)(Wl + o) = l;
                     }
                 }
+        } else {
+            double *D = (double *)D0 + rhs * d_rs;
+            double *P = (double *)D1 + rhs * p_rs;
+#pragma unroll
+            for (int i = 0; i < FM; i++)
+#pragma unroll
+                for (int j = 0; j < FN; j++)
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const uint32_t row =
+                            task.row0 + wm * WM + i * 16 + 4 * lk + r;
+                        const uint32_t col = task.col0 + wn * WN + j * 16 + lr;
+                        if (row < task.M && col < task.N) {
+                            const uint64_t idx =
+                                task.d_off + (uint64_t)row * task.ldd + col;
+                            const double v = scale * (double)acc[i][j][r];
+                            if (task.beta & 2)
+                                P[idx] = v;
+                            else if ((task.beta | beta0) & 1)
+                                D[idx] += v;
+                            else
+                                D[idx] = v;
+                        }
+                    }
+        }
     }
 }
 
-template <typename TA, typename TB, typename TD>
-hipError_t launch_seg_gemm_bf3(int cfg, const TA *A, const TB *B, TD *D,
-                               TD *P, const Task *tasks, int ntasks,
-                               const Seg *segs, uint64_t a_rs, uint64_t b_rs,
-                               uint64_t d_rs, uint64_t p_rs, double scale,
-                               int beta0, int n_rhs, int grid,
-                               hipStream_t st, XcdSplit xs) {
+template <int OUT>
+hipError_t launch_seg_gemm_pk(int cfg, const __bf16 *Ah, const __bf16 *Al,
+                              const __bf16 *Bh, const __bf16 *Bl, void *D0,
+                              void *D1, const Task *tasks, int ntasks,
+                              const Seg *segs, uint64_t a_rs, uint64_t b_rs,
+                              uint64_t d_rs, uint64_t p_rs, double scale,
+                              int beta0, int n_rhs, int grid, hipStream_t st,
+                              XcdSplit xs) {
     if (ntasks <= 0)
         return hipSuccess;
     dim3 g(task_grid(grid, ntasks, xs), (unsigned)n_rhs);
 #define B2G_LAUNCH(TM_, TN_)                                                   \
-    hipLaunchKernelGGL((seg_gemm_bf3_kernel<TA, TB, TD, TM_, TN_>), g,         \
-                       dim3(256), 0, st, A, B, D, P, tasks, ntasks, segs,      \
-                       a_rs, b_rs, d_rs, p_rs, scale, beta0, xs)
+    hipLaunchKernelGGL((seg_gemm_pk_kernel<TM_, TN_, OUT>), g, dim3(256), 0,   \
+                       st, Ah, Al, Bh, Bl, D0, D1, tasks, ntasks, segs, a_rs,  \
+                       b_rs, d_rs, p_rs, scale, beta0, xs)
     switch (cfg) {
     case 0: B2G_LAUNCH(32, 32); break;
     case 1: B2G_LAUNCH(32, 64); break;
@@ -292,13 +394,17 @@ hipError_t launch_seg_gemm_bf3(int cfg, const TA *A, const TB *B, TD *D,
     return hipGetLastError();
 }
 
-template hipError_t launch_seg_gemm_bf3<double, float, float>(
-    int, const double *, const float *, float *, float *, const Task *, int,
-    const Seg *, uint64_t, uint64_t, uint64_t, uint64_t, double, int, int,
-    int, hipStream_t, XcdSplit);
-template hipError_t launch_seg_gemm_bf3<float, float, double>(
-    int, const float *, const float *, double *, double *, const Task *, int,
-    const Seg *, uint64_t, uint64_t, uint64_t, uint64_t, double, int, int,
-    int, hipStream_t, XcdSplit);
+template hipError_t launch_seg_gemm_pk<0>(int, const __bf16 *, const __bf16 *,
+                                          const __bf16 *, const __bf16 *,
+                                          void *, void *, const Task *, int,
+                                          const Seg *, uint64_t, uint64_t,
+                                          uint64_t, uint64_t, double, int, int,
+                                          int, hipStream_t, XcdSplit);
+template hipError_t launch_seg_gemm_pk<1>(int, const __bf16 *, const __bf16 *,
+                                          const __bf16 *, const __bf16 *,
+                                          void *, void *, const Task *, int,
+                                          const Seg *, uint64_t, uint64_t,
+                                          uint64_t, uint64_t, double, int, int,
+                                          int, hipStream_t, XcdSplit);
 
 } // namespace b2g

@@ -347,6 +347,7 @@ __global__ __launch_bounds__(256) void seg_gemm_kernel(
             }
             __syncthreads(); // all waves done with LDS before the next task
         }
+        mfma_settle<acc_t, FM * FN>(&acc[0][0]);
 #pragma unroll
         for (int i = 0; i < FM; i++)
 #pragma unroll

@@ -8,6 +8,37 @@
 
 namespace b2g {
 
+// MFMA results land in their VGPRs several cycles after issue: VALU code that
+// reads or overwrites accumulators must trail the last MFMA by at least 19
+// wait states (16-pass FP64 MFMA; 8 for the BF16 16x16x32 one).  The
+// compiler pads straight-line code, but not every path into a block that
+// merges a k-loop's exits, so kernels call mfma_settle on their accumulators
+// before such code: 24 wait states in one asm statement that takes every
+// accumulator as an in/out operand, so that it follows all of their MFMAs
+// and all later uses of them follow it.
+#define B2G_SETTLE_NOPS "s_nop 7\n\ts_nop 7\n\ts_nop 7"
+template <typename V, int N> __device__ __forceinline__ void mfma_settle(V *x) {
+    if constexpr (N == 1) {
+        asm volatile(B2G_SETTLE_NOPS : "+v"(x[0]));
+    } else if constexpr (N == 2) {
+        asm volatile(B2G_SETTLE_NOPS : "+v"(x[0]), "+v"(x[1]));
+    } else if constexpr (N == 4) {
+        asm volatile(B2G_SETTLE_NOPS
+                     : "+v"(x[0]), "+v"(x[1]), "+v"(x[2]), "+v"(x[3]));
+    } else if constexpr (N == 8) {
+        asm volatile(B2G_SETTLE_NOPS
+                     : "+v"(x[0]), "+v"(x[1]), "+v"(x[2]), "+v"(x[3]),
+                       "+v"(x[4]), "+v"(x[5]), "+v"(x[6]), "+v"(x[7]));
+    } else {
+        static_assert(N == 16, "accumulator count");
+        asm volatile(B2G_SETTLE_NOPS
+                     : "+v"(x[0]), "+v"(x[1]), "+v"(x[2]), "+v"(x[3]),
+                       "+v"(x[4]), "+v"(x[5]), "+v"(x[6]), "+v"(x[7]),
+                       "+v"(x[8]), "+v"(x[9]), "+v"(x[10]), "+v"(x[11]),
+                       "+v"(x[12]), "+v"(x[13]), "+v"(x[14]), "+v"(x[15]));
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Segmented grouped GEMM.
 //
@@ -49,6 +80,17 @@ struct XcdSplit {
 struct RedTask {
     uint64_t d_off, p_off;
     uint32_t ldd, M, N, row0, col0, tm, tn, nparts, beta, pad;
+};
+
+// One tile of a packing job (split-BF16 operand planes, gemm_bf16x3.hip):
+// rows [0, rows) and 8-element k groups [0, groups) of a packed matrix at
+// element offset dst (row length dst_ld, a multiple of 8); element (r, k) is
+// src[src + r*src_ld + k] (trans: src[src + k*src_ld + r]) for k < cols and
+// zero for cols <= k < 8*groups.  rows <= 32, groups <= 8.
+struct PackTile {
+    uint64_t src, dst;
+    uint32_t src_ld, dst_ld;
+    uint32_t rows, cols, groups, trans;
 };
 
 // diagonal: output block with its contribution range

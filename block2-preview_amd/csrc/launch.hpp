@@ -31,14 +31,23 @@ hipError_t launch_splitk_reduce(TD *D, const TD *P, const RedTask *tasks,
                                 int beta0, int n_rhs, int grid,
                                 hipStream_t st);
 
-// split-BF16 three-term variant (gemm_bf16x3.hip), FP32 accumulation
-template <typename TA, typename TB, typename TD>
-hipError_t launch_seg_gemm_bf3(int cfg, const TA *A, const TB *B, TD *D,
-                               TD *P, const Task *tasks, int ntasks,
-                               const Seg *segs, uint64_t a_rs, uint64_t b_rs,
-                               uint64_t d_rs, uint64_t p_rs, double scale,
-                               int beta0, int n_rhs, int grid,
-                               hipStream_t st, XcdSplit xs = XcdSplit{});
+// split-BF16 three-term variant on packed operand planes (gemm_bf16x3.hip):
+// segments address packed A (M x lda) and packed B (N x ldb) planes, both
+// k-contiguous; OUT 0 writes the packed transpose of the product as hi / lo
+// planes (D0 / D1), OUT 1 writes FP64 D0 with split-K partials in D1
+template <int OUT>
+hipError_t launch_seg_gemm_pk(int cfg, const __bf16 *Ah, const __bf16 *Al,
+                              const __bf16 *Bh, const __bf16 *Bl, void *D0,
+                              void *D1, const Task *tasks, int ntasks,
+                              const Seg *segs, uint64_t a_rs, uint64_t b_rs,
+                              uint64_t d_rs, uint64_t p_rs, double scale,
+                              int beta0, int n_rhs, int grid, hipStream_t st,
+                              XcdSplit xs = XcdSplit{});
+// FP64 -> packed hi / lo BF16 planes, per right-hand side (src_rs, dst_rs)
+hipError_t launch_pack_bf16(const double *src, __bf16 *hi, __bf16 *lo,
+                            const PackTile *tiles, int ntiles,
+                            uint64_t src_rs, uint64_t dst_rs, int n_rhs,
+                            int grid, hipStream_t st);
 
 hipError_t launch_diag(const double *L, const double *R, double *Dv,
                        const DiagBlock *blocks, int nblocks,

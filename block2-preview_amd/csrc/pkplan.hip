@@ -1,0 +1,276 @@
+// Split-BF16 replay on packed operand planes: derivation from the replay
+// plan, device upload, scratch and operator re-packing.
+//
+// Every GEMM operand of the split-BF16 kernels is a packed matrix (rows run
+// along the contraction index, row length K rounded up to 8, hi and lo BF16
+// planes; gemm_bf16x3.hip).  From the plan's chunks:
+//   * phase 1 (W = C op(B)): A = the packed wavefunction window C (c_m x c_n,
+//     packed per H*psi), B = the packed right-operator block in the
+//     orientation op(B)^T (v_n x c_n); the product is written as packed W^T
+//     (v_n x round8(c_m));
+//   * phase 2 (V += alpha op(A) W): A = the packed left-operator block in the
+//     orientation op(A) (v_m x c_m), B = packed W^T; the FP64 output tasks,
+//     split-K reductions and partial buffer are the plan's own.
+// A block read in both orientations is packed twice; operator planes cost
+// 4 bytes per packed element (half of the FP64 buffers they come from).
+#include "common.hpp"
+#include "launch.hpp"
+#include "replay.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <tuple>
+#include <unordered_map>
+
+namespace b2g {
+
+namespace {
+
+uint64_t round8(uint64_t v) { return (v + 7) & ~(uint64_t)7; }
+
+// packed matrices of one source buffer, deduplicated by
+// (offset, row stride, orientation, rows, K)
+struct Packer {
+    typedef std::tuple<uint64_t, uint32_t, uint32_t, uint32_t, uint32_t> Key;
+    std::map<Key, uint64_t> where;
+    std::vector<PackTile> tiles;
+    uint64_t len = 0;
+    uint64_t src_len = 0;
+
+    // packed offset of rows x K matrix (r, k) = src[off + r*ld + k]
+    // (trans: src[off + k*ld + r]); -1 on an out-of-range source
+    int64_t add(uint64_t off, uint32_t ld, bool trans, uint32_t rows,
+                uint32_t K) {
+        const Key key(off, ld, trans ? 1u : 0u, rows, K);
+        auto it = where.find(key);
+        if (it != where.end())
+            return (int64_t)it->second;
+        if (rows == 0 || K == 0)
+            return -1;
+        const uint64_t last =
+            off + (trans ? (uint64_t)(K - 1) * ld + (rows - 1)
+                         : (uint64_t)(rows - 1) * ld + (K - 1));
+        if (last >= src_len)
+            return -1;
+        const uint64_t pld = round8(K);
+        if ((uint64_t)rows * pld >= ((uint64_t)1 << 28))
+            return -1;
+        const uint64_t base = len;
+        for (uint32_t r0 = 0; r0 < rows; r0 += 32)
+            for (uint64_t k0 = 0; k0 < pld; k0 += 64) {
+                PackTile t;
+                t.src = off + (trans ? k0 * ld + r0 : (uint64_t)r0 * ld + k0);
+                t.dst = base + (uint64_t)r0 * pld + k0;
+                t.src_ld = ld;
+                t.dst_ld = (uint32_t)pld;
+                t.rows = std::min<uint32_t>(32, rows - r0);
+                t.cols = K > k0 ? (uint32_t)std::min<uint64_t>(64, K - k0) : 0;
+                t.groups = (uint32_t)std::min<uint64_t>(8, (pld - k0) / 8);
+                t.trans = trans ? 1u : 0u;
+                tiles.push_back(t);
+            }
+        len += (uint64_t)rows * pld;
+        where.emplace(key, base);
+        return (int64_t)base;
+    }
+};
+
+struct HostChunk {
+    std::vector<Task> t1[kTileConfigs];
+    std::vector<Seg> s1, s2;
+};
+
+} // namespace
+
+void pk_destroy(PkPlan *k) {
+    if (k == nullptr)
+        return;
+    if (k->dev)
+        hipFree(k->dev);
+    if (k->op)
+        hipFree(k->op);
+    if (k->scratch)
+        hipFree(k->scratch);
+    delete k;
+}
+
+static int pk_pack_ops(b2g_plan p, PkPlan *k) {
+    const int grid = p->ctx->num_cu * 8;
+    hipStream_t st = p->ctx->stream;
+    __bf16 *op = (__bf16 *)k->op;
+    B2G_HIP(launch_pack_bf16((const double *)p->lbuf->ptr, op, op + k->l_len,
+                             k->d_ltiles, k->n_ltiles, 0, 0, 1, grid, st));
+    B2G_HIP(launch_pack_bf16((const double *)p->rbuf->ptr, op + 2 * k->l_len,
+                             op + 2 * k->l_len + k->r_len, k->d_rtiles,
+                             k->n_rtiles, 0, 0, 1, grid, st));
+    return B2G_OK;
+}
+
+static int pk_build_into(b2g_plan p, PkPlan *k) {
+    Packer L, R, X;
+    L.src_len = p->lbuf->n;
+    R.src_len = p->rbuf->n;
+    X.src_len = p->x_len;
+    std::vector<HostChunk> hc(p->chunks.size());
+    for (size_t ic = 0; ic < p->chunks.size(); ic++) {
+        const Chunk &ch = p->chunks[ic];
+        HostChunk &h = hc[ic];
+        // rows / columns of each segment's output block
+        std::vector<uint32_t> m1(ch.s1.size(), 0), n1(ch.s1.size(), 0);
+        std::vector<uint64_t> w1(ch.s1.size(), 0);
+        for (int c = 0; c < kTileConfigs; c++)
+            for (const Task &t : ch.t1[c]) {
+                B2G_REQUIRE(t.seg_end == t.seg_begin + 1 &&
+                                t.seg_begin < ch.s1.size(),
+                            B2G_ERR_STATE, "packed plan: phase-1 task shape");
+                m1[t.seg_begin] = t.M, n1[t.seg_begin] = t.N;
+                w1[t.seg_begin] = t.d_off;
+            }
+        std::vector<uint32_t> m2(ch.s2.size(), 0);
+        for (int c = 0; c < kTileConfigs; c++)
+            for (const Task &t : ch.t2[c]) {
+                B2G_REQUIRE(t.seg_begin <= t.seg_end &&
+                                t.seg_end <= ch.s2.size(),
+                            B2G_ERR_STATE, "packed plan: phase-2 task shape");
+                for (uint32_t s = t.seg_begin; s < t.seg_end; s++)
+                    m2[s] = t.M;
+            }
+        // packed W^T of each phase-1 segment, keyed by the plan's W offset
+        std::unordered_map<uint64_t, std::pair<uint64_t, uint32_t>> wmap;
+        std::vector<uint64_t> wpk(ch.s1.size(), 0);
+        uint64_t wlen = 0;
+        for (size_t si = 0; si < ch.s1.size(); si++) {
+            const Seg &s = ch.s1[si];
+            B2G_REQUIRE(m1[si] > 0 && n1[si] > 0, B2G_ERR_STATE,
+                        "packed plan: phase-1 segment without a task");
+            const uint32_t ldw = (uint32_t)round8(m1[si]);
+            B2G_REQUIRE((uint64_t)n1[si] * ldw < ((uint64_t)1 << 28),
+                        B2G_ERR_ARG, "packed plan: W block of 2^28 elements");
+            wpk[si] = wlen;
+            wmap[w1[si]] = std::make_pair(wlen, ldw);
+            wlen += (uint64_t)n1[si] * ldw;
+            const int64_t a = X.add(s.a_off, s.lda, false, m1[si], s.K);
+            const int64_t b = R.add(s.b_off, s.ldb, !(s.flags & 2), n1[si], s.K);
+            B2G_REQUIRE(a >= 0 && b >= 0, B2G_ERR_ARG,
+                        "packed plan: phase-1 operand outside its buffer");
+            Seg q = s;
+            q.a_off = (uint64_t)a, q.lda = (uint32_t)round8(s.K);
+            q.b_off = (uint64_t)b, q.ldb = (uint32_t)round8(s.K);
+            q.flags = 0, q.alpha = 1.0;
+            h.s1.push_back(q);
+        }
+        for (int c = 0; c < kTileConfigs; c++)
+            for (Task t : ch.t1[c]) {
+                t.d_off = wpk[t.seg_begin];
+                t.ldd = (uint32_t)round8(t.M);
+                h.t1[c].push_back(t);
+            }
+        for (size_t si = 0; si < ch.s2.size(); si++) {
+            const Seg &s = ch.s2[si];
+            auto it = wmap.find(s.b_off);
+            B2G_REQUIRE(it != wmap.end() && it->second.second == round8(s.K),
+                        B2G_ERR_STATE, "packed plan: phase-2 W not found");
+            B2G_REQUIRE(m2[si] > 0, B2G_ERR_STATE,
+                        "packed plan: phase-2 segment without a task");
+            const int64_t a = L.add(s.a_off, s.lda, s.flags & 1, m2[si], s.K);
+            B2G_REQUIRE(a >= 0, B2G_ERR_ARG,
+                        "packed plan: phase-2 operand outside its buffer");
+            Seg q = s;
+            q.a_off = (uint64_t)a, q.lda = (uint32_t)round8(s.K);
+            q.b_off = it->second.first, q.ldb = it->second.second;
+            q.flags = 0;
+            h.s2.push_back(q);
+        }
+        k->w_max = std::max(k->w_max, wlen);
+    }
+    k->l_len = L.len, k->r_len = R.len, k->x_len = X.len;
+    // one allocation for every task, segment and pack-tile array
+    size_t bytes = 0;
+    for (auto &h : hc) {
+        for (int c = 0; c < kTileConfigs; c++)
+            bytes += h.t1[c].size() * sizeof(Task);
+        bytes += (h.s1.size() + h.s2.size()) * sizeof(Seg);
+    }
+    bytes += (L.tiles.size() + R.tiles.size() + X.tiles.size()) *
+             sizeof(PackTile);
+    std::vector<char> host(bytes + 16);
+    B2G_HIP(hipMalloc(&k->dev, bytes + 16));
+    size_t off = 0;
+    auto put = [&](const void *src, size_t nb) {
+        const size_t o = off;
+        if (nb)
+            memcpy(host.data() + o, src, nb);
+        off += nb;
+        return (char *)k->dev + o;
+    };
+    for (auto &h : hc) {
+        PkChunk pc;
+        for (int c = 0; c < kTileConfigs; c++)
+            pc.d_t1[c] = (Task *)put(h.t1[c].data(),
+                                     h.t1[c].size() * sizeof(Task));
+        pc.d_s1 = (Seg *)put(h.s1.data(), h.s1.size() * sizeof(Seg));
+        pc.d_s2 = (Seg *)put(h.s2.data(), h.s2.size() * sizeof(Seg));
+        k->chunks.push_back(pc);
+    }
+    k->d_ltiles = (PackTile *)put(L.tiles.data(),
+                                  L.tiles.size() * sizeof(PackTile));
+    k->d_rtiles = (PackTile *)put(R.tiles.data(),
+                                  R.tiles.size() * sizeof(PackTile));
+    k->d_xtiles = (PackTile *)put(X.tiles.data(),
+                                  X.tiles.size() * sizeof(PackTile));
+    k->n_ltiles = (int)L.tiles.size();
+    k->n_rtiles = (int)R.tiles.size();
+    k->n_xtiles = (int)X.tiles.size();
+    B2G_REQUIRE(L.tiles.size() < (1u << 31) && R.tiles.size() < (1u << 31) &&
+                    X.tiles.size() < (1u << 31),
+                B2G_ERR_ARG, "packed plan: too many pack tiles");
+    B2G_HIP(hipMemcpyAsync(k->dev, host.data(), bytes, hipMemcpyHostToDevice,
+                           p->ctx->stream));
+    B2G_HIP(hipStreamSynchronize(p->ctx->stream));
+    B2G_HIP(hipMalloc(&k->op, std::max<uint64_t>(2 * (k->l_len + k->r_len), 8) *
+                                  sizeof(uint16_t)));
+    const int r = pk_pack_ops(p, k);
+    if (r != B2G_OK)
+        return r;
+    B2G_HIP(hipStreamSynchronize(p->ctx->stream));
+    return B2G_OK;
+}
+
+int pk_build(b2g_plan p) {
+    if (p->pk != nullptr)
+        return B2G_OK;
+    PkPlan *k = new PkPlan();
+    const int r = pk_build_into(p, k);
+    if (r != B2G_OK) {
+        hipStreamSynchronize(p->ctx->stream);
+        pk_destroy(k);
+        return r;
+    }
+    p->pk = k;
+    return B2G_OK;
+}
+
+int pk_reserve(b2g_plan p, int n_rhs) {
+    PkPlan *k = p->pk;
+    if (n_rhs <= k->scratch_rhs)
+        return B2G_OK;
+    if (k->scratch) {
+        B2G_HIP(hipStreamSynchronize(p->ctx->stream));
+        B2G_HIP(hipFree(k->scratch));
+        k->scratch = nullptr;
+        k->scratch_rhs = 0;
+    }
+    const size_t n = std::max<uint64_t>(2 * (k->x_len + k->w_max), 8) * n_rhs;
+    B2G_HIP(hipMalloc(&k->scratch, n * sizeof(uint16_t)));
+    k->scratch_rhs = n_rhs;
+    return B2G_OK;
+}
+
+int pk_refresh(b2g_plan p) {
+    if (p->pk == nullptr)
+        return B2G_OK;
+    return pk_pack_ops(p, p->pk);
+}
+
+} // namespace b2g

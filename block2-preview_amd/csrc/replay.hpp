@@ -11,6 +11,8 @@ namespace b2g {
 
 struct GradPlan; // grad.hip
 void grad_plan_destroy(GradPlan *g);
+struct PkPlan; // pkplan.hip
+void pk_destroy(PkPlan *k);
 
 struct Chunk {
     // host images; tasks are bucketed by tile configuration
@@ -60,6 +62,8 @@ struct b2g_plan_s {
     // use) and that plan
     std::vector<b2g_pair> pairs;
     b2g::GradPlan *grad = nullptr;
+    // the split-BF16 replay on packed operand planes (built on first use)
+    b2g::PkPlan *pk = nullptr;
 };
 
 
@@ -86,5 +90,45 @@ void sort_tasks(std::vector<Task> &tasks, const std::vector<Seg> &segs);
 uint64_t w_cap_elems();
 // allocate the H*psi scratch for n_rhs at precision ahead of a capture
 int plan_reserve(b2g_plan p, int n_rhs, int precision);
+
+// Split-BF16 replay on packed operand planes (pkplan.hip).  Derived from the
+// plan's chunks: phase 1 reads the packed wavefunction windows and packed
+// right-operator blocks and writes packed W^T; phase 2 reads packed
+// left-operator blocks and packed W^T and keeps the FP64 tasks, split-K
+// reductions and partial buffer of the plan.
+struct PkChunk {
+    Task *d_t1[kTileConfigs] = {}; // phase-1 tasks addressing packed W^T
+    Seg *d_s1 = nullptr, *d_s2 = nullptr;
+};
+struct PkPlan {
+    std::vector<PkChunk> chunks;
+    void *dev = nullptr;     // task, segment and pack-tile arrays
+    uint16_t *op = nullptr;  // packed operator planes: L hi, L lo, R hi, R lo
+    uint64_t l_len = 0, r_len = 0;
+    PackTile *d_ltiles = nullptr, *d_rtiles = nullptr, *d_xtiles = nullptr;
+    int n_ltiles = 0, n_rtiles = 0, n_xtiles = 0;
+    uint64_t x_len = 0; // packed wavefunction elements per right-hand side
+    uint64_t w_max = 0; // packed W^T elements per right-hand side
+    // packed wavefunction and W^T planes for scratch_rhs right-hand sides:
+    // x hi, x lo (x_len * rhs each), W hi, W lo (w_max * rhs each)
+    uint16_t *scratch = nullptr;
+    int scratch_rhs = 0;
+    const uint16_t *lh() const { return op; }
+    const uint16_t *ll() const { return op + l_len; }
+    const uint16_t *rh() const { return op + 2 * l_len; }
+    const uint16_t *rl() const { return op + 2 * l_len + r_len; }
+    uint16_t *xh() const { return scratch; }
+    uint16_t *xl() const { return scratch + x_len * scratch_rhs; }
+    uint16_t *wh() const { return scratch + 2 * x_len * scratch_rhs; }
+    uint16_t *wl() const {
+        return scratch + (2 * x_len + w_max) * scratch_rhs;
+    }
+};
+// build p->pk from the plan (packs the operator planes); no-op if built
+int pk_build(b2g_plan p);
+// packed scratch for n_rhs right-hand sides (grows only)
+int pk_reserve(b2g_plan p, int n_rhs);
+// re-pack the operator planes after the operator buffers changed
+int pk_refresh(b2g_plan p);
 
 } // namespace b2g

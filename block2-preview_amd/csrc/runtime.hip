@@ -787,7 +787,7 @@ extern "C" int b2g_plan_refresh(b2g_plan p) {
         B2G_HIP(launch_f64_to_f32((const double *)p->rbuf->ptr, p->r32,
                                   p->rbuf->n, p->ctx->stream));
     }
-    return B2G_OK;
+    return pk_refresh(p);
 }
 
 extern "C" int b2g_plan_destroy(b2g_plan p) {
@@ -810,6 +810,7 @@ extern "C" int b2g_plan_destroy(b2g_plan p) {
     if (p->pbuf)
         hipFree(p->pbuf);
     grad_plan_destroy(p->grad);
+    pk_destroy(p->pk);
     delete p;
     return B2G_OK;
 }
@@ -893,7 +894,14 @@ int plan_reserve(b2g_plan p, int n_rhs, int precision) {
     b2g_ctx ctx = p->ctx;
     const size_t es = precision == B2G_PREC_F64 ? 8 : 4;
     const size_t need = std::max<uint64_t>(p->w_max, 1) * n_rhs * es;
-    if (need > p->w_bytes) {
+    if (precision == B2G_PREC_BF16X3) {
+        // packed planes replace the W scratch
+        int r = pk_build(p);
+        if (r == B2G_OK)
+            r = pk_reserve(p, n_rhs);
+        if (r != B2G_OK)
+            return r;
+    } else if (need > p->w_bytes) {
         if (p->w) {
             B2G_HIP(hipStreamSynchronize(ctx->stream));
             B2G_HIP(hipFree(p->w));
@@ -919,6 +927,16 @@ int plan_reserve(b2g_plan p, int n_rhs, int precision) {
 
 } // namespace b2g
 
+extern "C" int b2g_plan_reserve(b2g_plan p, int n_rhs, int precision) {
+    B2G_REQUIRE(p, B2G_ERR_ARG, "NULL plan");
+    B2G_REQUIRE(n_rhs >= 1 && n_rhs <= 65535, B2G_ERR_ARG, "bad n_rhs");
+    B2G_REQUIRE(precision == B2G_PREC_F64 || precision == B2G_PREC_F32 ||
+                    precision == B2G_PREC_BF16X3,
+                B2G_ERR_ARG, "unsupported precision");
+    B2G_HIP(hipSetDevice(p->ctx->device));
+    return plan_reserve(p, n_rhs, precision);
+}
+
 extern "C" int b2g_hpsi(b2g_ctx ctx, b2g_plan p, const double *x, double *y,
                         int n_rhs, double scale, int accumulate,
                         int precision) {
@@ -927,9 +945,8 @@ extern "C" int b2g_hpsi(b2g_ctx ctx, b2g_plan p, const double *x, double *y,
     B2G_REQUIRE(precision == B2G_PREC_F64 || precision == B2G_PREC_F32 ||
                     precision == B2G_PREC_BF16X3,
                 B2G_ERR_ARG, "unsupported precision");
-    B2G_REQUIRE(precision == B2G_PREC_F64 || p->l32 != nullptr, B2G_ERR_STATE,
-                "plan was uploaded without reduced-precision shadows "
-                "(flags bit 0 or 1)");
+    B2G_REQUIRE(precision != B2G_PREC_F32 || p->l32 != nullptr, B2G_ERR_STATE,
+                "plan was uploaded without FP32 shadows (flags bit 0 or 1)");
     B2G_HIP(hipSetDevice(ctx->device));
     {
         const int r = plan_reserve(p, n_rhs, precision);
@@ -998,10 +1015,46 @@ extern "C" int b2g_hpsi(b2g_ctx ctx, b2g_plan p, const double *x, double *y,
         }
         return e;
     };
+    const PkPlan *pk = precision == B2G_PREC_BF16X3 ? p->pk : nullptr;
+    typedef const __bf16 *cbf;
+    if (pk != nullptr)
+        B2G_HIP(timed(B2G_K_PACK, ctx->stream, [&](hipStream_t st) {
+            return launch_pack_bf16(x, (__bf16 *)pk->xh(), (__bf16 *)pk->xl(),
+                                    pk->d_xtiles, pk->n_xtiles, p->x_len,
+                                    pk->x_len, n_rhs, grid, st);
+        }));
     for (size_t ic = 0; ic < p->chunks.size(); ic++) {
         const Chunk &ch = p->chunks[ic];
         // chunk 0 overwrites unless accumulating; later chunks' tasks add
         const int beta0 = ic == 0 ? (accumulate ? 1 : 0) : 1;
+        if (pk != nullptr) {
+            const PkChunk &kc = pk->chunks[ic];
+            B2G_HIP(phase(ch.t1, ch.f1, B2G_K_PHASE1,
+                          [&](int c, hipStream_t st) {
+                return launch_seg_gemm_pk<0>(
+                    c, (cbf)pk->xh(), (cbf)pk->xl(), (cbf)pk->rh(),
+                    (cbf)pk->rl(), pk->wh(), pk->wl(), kc.d_t1[c],
+                    (int)ch.t1[c].size(), kc.d_s1, pk->x_len, 0, pk->w_max,
+                    0, 1.0, 0, n_rhs, grid, st, ch.x1[c]);
+            }));
+            B2G_HIP(phase(ch.t2, ch.f2, B2G_K_PHASE2,
+                          [&](int c, hipStream_t st) {
+                return launch_seg_gemm_pk<1>(
+                    c, (cbf)pk->lh(), (cbf)pk->ll(), (cbf)pk->wh(),
+                    (cbf)pk->wl(), y, P, ch.d_t2[c], (int)ch.t2[c].size(),
+                    kc.d_s2, 0, pk->w_max, p->y_len, p->p_max, scale, beta0,
+                    n_rhs, grid, st, ch.x2[c]);
+            }));
+            const int nr = (int)ch.red.size();
+            if (nr > 0)
+                B2G_HIP(timed(B2G_K_SPLITK_REDUCE, ctx->stream,
+                              [&](hipStream_t st) {
+                    return launch_splitk_reduce<double>(
+                        y, P, ch.d_red, nr, p->y_len, p->p_max, beta0, n_rhs,
+                        grid, st);
+                }));
+            continue;
+        }
         // phase 1, phase 2, then the split-K reduction
         B2G_HIP(phase(ch.t1, ch.f1, B2G_K_PHASE1, [&](int c, hipStream_t st) {
             const int nt = (int)ch.t1[c].size();
@@ -1010,11 +1063,6 @@ extern "C" int b2g_hpsi(b2g_ctx ctx, b2g_plan p, const double *x, double *y,
                     c, x, (const double *)p->rbuf->ptr, (double *)p->w,
                     (double *)p->w, ch.d_t1[c], nt, ch.d_s1, p->x_len, 0,
                     p->w_max, 0, 1.0, 0, n_rhs, grid, st, ch.x1[c]);
-            if (precision == B2G_PREC_BF16X3)
-                return launch_seg_gemm_bf3<double, float, float>(
-                    c, x, p->r32, (float *)p->w, (float *)p->w, ch.d_t1[c], nt,
-                    ch.d_s1, p->x_len, 0, p->w_max, 0, 1.0, 0, n_rhs, grid,
-                    st, ch.x1[c]);
             return launch_seg_gemm<double, float, float, float, 1>(
                 c, x, p->r32, (float *)p->w, (float *)p->w, ch.d_t1[c], nt,
                 ch.d_s1, p->x_len, 0, p->w_max, 0, 1.0, 0, n_rhs, grid, st, ch.x1[c]);
@@ -1026,11 +1074,6 @@ extern "C" int b2g_hpsi(b2g_ctx ctx, b2g_plan p, const double *x, double *y,
                     c, (const double *)p->lbuf->ptr, (const double *)p->w, y,
                     P, ch.d_t2[c], nt, ch.d_s2, 0, p->w_max, p->y_len,
                     p->p_max, scale, beta0, n_rhs, grid, st, ch.x2[c]);
-            if (precision == B2G_PREC_BF16X3)
-                return launch_seg_gemm_bf3<float, float, double>(
-                    c, p->l32, (const float *)p->w, y, P, ch.d_t2[c], nt,
-                    ch.d_s2, 0, p->w_max, p->y_len, p->p_max, scale, beta0,
-                    n_rhs, grid, st, ch.x2[c]);
             return launch_seg_gemm<float, float, double, float, 2>(
                 c, p->l32, (const float *)p->w, y, P, ch.d_t2[c], nt,
                 ch.d_s2, 0, p->w_max, p->y_len, p->p_max, scale, beta0, n_rhs,

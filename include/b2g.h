@@ -147,16 +147,21 @@ typedef struct {
 /* Compile and upload a replay plan.  `lbuf`/`rbuf` hold the operator blocks
  * (FP64); x_len / y_len are the input/output vector lengths.  `flags` bit 0
  * or bit 1 also builds FP32 shadows of the operator buffers, which
- * B2G_PREC_F32 and B2G_PREC_BF16X3 read (the split-BF16 kernels split each
- * FP32 value into BF16 hi + lo when staging it into LDS).                 */
+ * B2G_PREC_F32 reads.  B2G_PREC_BF16X3 reads packed split-BF16 copies of the
+ * operator blocks (hi + lo planes, 4 bytes per element, in the orientation
+ * each pair uses), built on its first use or by b2g_plan_reserve.          */
 int b2g_upload_plan(b2g_ctx ctx, b2g_buf lbuf, b2g_buf rbuf,
                     const b2g_pair *pairs, size_t n_pairs,
                     const b2g_block *yblocks, size_t n_yblocks, size_t x_len,
                     size_t y_len, unsigned flags, b2g_plan *out);
 int b2g_plan_destroy(b2g_plan plan);
-/* Re-derive the plan's reduced-precision shadows after the operator
- * buffers' contents changed (same structure, new numbers).               */
+/* Re-derive the plan's reduced-precision shadows and packed planes after
+ * the operator buffers' contents changed (same structure, new numbers).   */
 int b2g_plan_refresh(b2g_plan plan);
+/* Allocate what b2g_hpsi needs for n_rhs right-hand sides at `precision`
+ * (scratch, and the packed operator planes of B2G_PREC_BF16X3), so that a
+ * following b2g_hpsi allocates nothing (e.g. under stream capture).        */
+int b2g_plan_reserve(b2g_plan plan, int n_rhs, int precision);
 /* Reference-order work of one H*psi, in multiply-adds
  * (sum over pairs of c_m*c_n*v_n + v_m*c_m*v_n, the reference's nflop,
  * src/core/batch_gemm.hpp:307).  FLOPs = 2 * this.                        */
@@ -164,11 +169,13 @@ int b2g_plan_macs(b2g_plan plan, uint64_t *out);
 /* Kernel ids of one H*psi for timing and accounting: the grouped GEMM of
  * phase 1 (work = C op(B)) in tile configuration c is B2G_K_PHASE1 + c,
  * phase 2 (V += alpha op(A) work) is B2G_K_PHASE2 + c (c = 3*i + j,
- * TM = 32 << i, TN = 32 << j), and the split-K reduction B2G_K_SPLITK_REDUCE. */
+ * TM = 32 << i, TN = 32 << j), the split-K reduction B2G_K_SPLITK_REDUCE,
+ * and the split-BF16 packing of the wavefunction B2G_K_PACK.              */
 #define B2G_K_PHASE1 0
 #define B2G_K_PHASE2 9
 #define B2G_K_SPLITK_REDUCE 18
-#define B2G_NKERNELS 19
+#define B2G_K_PACK 19
+#define B2G_NKERNELS 20
 /* Per-launch kernel timing with HIP events on the context stream: while
  * enabled, every b2g_hpsi brackets each kernel launch with two events;
  * b2g_plan_timing waits for them, writes per kernel id the summed duration

@@ -79,6 +79,18 @@ def test_hpsi_bf16x3(heff):
     assert err < 1e-4, name
 
 
+@pytest.mark.parametrize("precision", ["f64", "f32", "bf16x3"])
+def test_hpsi_repeat_bitwise(heff, precision):
+    # every output tile is owned by one task and reduced in a fixed order:
+    # repeated products are bitwise equal (a VALU read of an accumulator
+    # before its MFMA has landed shows up here as run-to-run noise)
+    name, pb, h = heff
+    x = dev(pb.golden["psi"])
+    ys = [h.multiply(x, precision=precision).cpu().numpy() for _ in range(4)]
+    for y in ys[1:]:
+        assert np.array_equal(y, ys[0]), (name, precision)
+
+
 def test_davidson_fp64_matches_reference(heff):
     name, pb, h = heff
     x = dev(pb.golden["psi"])
