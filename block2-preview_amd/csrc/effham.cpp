@@ -23,6 +23,43 @@
 
 using namespace b2g;
 
+namespace b2g {
+// Graph-captured Davidson: workspace and one instantiated iteration per
+// precision, kept across eigs calls.  A graph is re-captured when the plan's
+// scratch moved (scratch_gen) or the subspace size changed; the operator
+// data it reads is updated in place and stays valid.
+struct DavCache {
+    size_t n = 0, nv = 0;
+    double *B = nullptr, *S = nullptr, *q = nullptr, *t = nullptr;
+    double *qn = nullptr, *sn = nullptr, *partial = nullptr;
+    DavState *dst = nullptr, *hdr = nullptr; // hdr: pinned host
+    hipGraphExec_t exec[3] = {};
+    uint64_t gen[3] = {};
+    void drop_graphs() {
+        for (auto &e : exec)
+            if (e) {
+                hipGraphExecDestroy(e);
+                e = nullptr;
+            }
+    }
+    void release() {
+        drop_graphs();
+        for (double **p : {&B, &S, &q, &t, &qn, &sn, &partial})
+            if (*p) {
+                hipFree(*p);
+                *p = nullptr;
+            }
+        if (dst)
+            hipFree(dst);
+        if (hdr)
+            hipHostFree(hdr);
+        dst = hdr = nullptr;
+        n = nv = 0;
+    }
+    ~DavCache() { release(); }
+};
+} // namespace b2g
+
 struct b2g_effham_s {
     b2g_ctx ctx = nullptr;
     Problem prob;
@@ -42,6 +79,7 @@ struct b2g_effham_s {
     // optional matvec hook (EffectiveKernel)
     b2g_kernel_fn kernel = nullptr;
     void *kernel_user = nullptr;
+    b2g::DavCache dav;
 };
 
 namespace b2g {
@@ -416,6 +454,9 @@ extern "C" int b2g_effham_create(b2g_ctx ctx, const b2g_problem *pb,
 extern "C" int b2g_effham_destroy(b2g_effham h) {
     if (h == nullptr)
         return B2G_OK;
+    if (h->ctx)
+        hipStreamSynchronize(h->ctx->stream);
+    h->dav.release();
     b2g_plan_destroy(h->plan);
     b2g_buf_destroy(h->lbuf);
     b2g_buf_destroy(h->rbuf);
@@ -761,49 +802,37 @@ static int davidson_graph(b2g_effham h, double *x, double conv_thrd,
     // the subspace holds def_max vectors plus the one appended before the
     // restart check of the next iteration
     const size_t nv = (size_t)def_max + 1;
-    DavWork w;
-    double *qn = nullptr, *sn = nullptr, *partial = nullptr;
-    DavState *dst = nullptr;
-    DavState *hdr = nullptr;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    struct Cleanup {
-        double **a, **b, **c;
-        DavState **d, **e;
-        hipGraph_t *g;
-        hipGraphExec_t *x;
-        ~Cleanup() {
-            if (*x)
-                hipGraphExecDestroy(*x);
-            if (*g)
-                hipGraphDestroy(*g);
-            for (double **p : {a, b, c})
-                if (*p)
-                    hipFree(*p);
-            if (*d)
-                hipFree(*d);
-            if (*e)
-                hipHostFree(*e);
-        }
-    } cleanup{&qn, &sn, &partial, &dst, &hdr, &graph, &exec};
-    B2G_HIP(hipMalloc(&w.B, nv * n * sizeof(double)));
-    B2G_HIP(hipMalloc(&w.S, nv * n * sizeof(double)));
-    B2G_HIP(hipMalloc(&w.q, n * sizeof(double)));
-    B2G_HIP(hipMalloc(&w.t, n * sizeof(double)));
-    B2G_HIP(hipMalloc(&qn, n * sizeof(double)));
-    B2G_HIP(hipMalloc(&sn, n * sizeof(double)));
-    B2G_HIP(hipMalloc(&partial, davidson_partial_len() * sizeof(double)));
-    B2G_HIP(hipMalloc(&dst, sizeof(DavState)));
-    B2G_HIP(hipHostMalloc(&hdr, sizeof(DavState)));
+    DavCache &c = h->dav;
+    if (c.n != n || c.nv != nv) {
+        B2G_HIP(hipStreamSynchronize(st));
+        c.release();
+        auto fail = [&c](hipError_t e) {
+            if (e != hipSuccess)
+                c.release();
+            return e;
+        };
+        B2G_HIP(fail(hipMalloc(&c.B, nv * n * sizeof(double))));
+        B2G_HIP(fail(hipMalloc(&c.S, nv * n * sizeof(double))));
+        B2G_HIP(fail(hipMalloc(&c.q, n * sizeof(double))));
+        B2G_HIP(fail(hipMalloc(&c.t, n * sizeof(double))));
+        B2G_HIP(fail(hipMalloc(&c.qn, n * sizeof(double))));
+        B2G_HIP(fail(hipMalloc(&c.sn, n * sizeof(double))));
+        B2G_HIP(fail(hipMalloc(&c.partial,
+                               davidson_partial_len() * sizeof(double))));
+        B2G_HIP(fail(hipMalloc(&c.dst, sizeof(DavState))));
+        B2G_HIP(fail(hipHostMalloc(&c.hdr, sizeof(DavState))));
+        c.n = n;
+        c.nv = nv;
+    }
     // normalised start vector -> B[0] and qn
     double nrm2 = 0;
     int r = b2g_vec_dot(ctx, x, x, n, &nrm2);
     if (r)
         return r;
     B2G_REQUIRE(nrm2 > 1E-28, B2G_ERR_ARG, "Davidson initial guess has zero norm");
-    B2G_HIP(launch_copy(x, w.B, n, st));
-    B2G_HIP(launch_scale(1.0 / std::sqrt(nrm2), w.B, n, st));
-    B2G_HIP(launch_copy(w.B, qn, n, st));
+    B2G_HIP(launch_copy(x, c.B, n, st));
+    B2G_HIP(launch_scale(1.0 / std::sqrt(nrm2), c.B, n, st));
+    B2G_HIP(launch_copy(c.B, c.qn, n, st));
     DavState init;
     memset(&init, 0, sizeof init);
     init.m = 1;
@@ -811,25 +840,41 @@ static int davidson_graph(b2g_effham h, double *x, double conv_thrd,
     init.def_min = def_min;
     init.def_max = def_max;
     init.conv = conv_thrd;
-    B2G_HIP(hipMemcpyAsync(dst, &init, sizeof init, hipMemcpyHostToDevice, st));
+    B2G_HIP(hipMemcpyAsync(c.dst, &init, sizeof init, hipMemcpyHostToDevice,
+                           st));
     if ((r = plan_reserve(h->plan, 1, precision)) != B2G_OK)
         return r;
     B2G_HIP(hipStreamSynchronize(st));
-    // capture one iteration
-    B2G_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    int rc = b2g_hpsi(ctx, h->plan, qn, sn, 1, 1.0, 0, precision);
-    hipError_t e = rc == B2G_OK
-                       ? launch_davidson_step(dst, w.B, w.S, sn, w.q, w.t, qn,
-                                              h->diag, n, partial, st)
-                       : hipSuccess;
-    if (e == hipSuccess && rc == B2G_OK)
-        e = hipMemcpyAsync(hdr, dst, 64, hipMemcpyDeviceToHost, st);
-    const hipError_t e2 = hipStreamEndCapture(st, &graph);
-    if (rc != B2G_OK)
-        return rc;
-    B2G_HIP(e);
-    B2G_HIP(e2);
-    B2G_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    hipGraphExec_t &exec = c.exec[precision];
+    if (exec == nullptr || c.gen[precision] != h->plan->scratch_gen) {
+        if (exec) {
+            hipGraphExecDestroy(exec);
+            exec = nullptr;
+        }
+        // capture one iteration: H qn -> sn, the device step, and the copy
+        // of the state header to pinned host memory
+        hipGraph_t graph = nullptr;
+        B2G_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        int rc = b2g_hpsi(ctx, h->plan, c.qn, c.sn, 1, 1.0, 0, precision);
+        hipError_t e = rc == B2G_OK
+                           ? launch_davidson_step(c.dst, c.B, c.S, c.sn, c.q,
+                                                  c.t, c.qn, h->diag, n,
+                                                  c.partial, st)
+                           : hipSuccess;
+        if (e == hipSuccess && rc == B2G_OK)
+            e = hipMemcpyAsync(c.hdr, c.dst, 64, hipMemcpyDeviceToHost, st);
+        const hipError_t e2 = hipStreamEndCapture(st, &graph);
+        if (rc == B2G_OK && e == hipSuccess && e2 == hipSuccess)
+            e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (graph)
+            hipGraphDestroy(graph);
+        if (rc != B2G_OK)
+            return rc;
+        B2G_HIP(e2);
+        B2G_HIP(e);
+        c.gen[precision] = h->plan->scratch_gen;
+    }
+    DavState *hdr = c.hdr;
     nmv = 0;
     ndav = 0;
     int status = 0;
@@ -841,7 +886,7 @@ static int davidson_graph(b2g_effham h, double *x, double conv_thrd,
         ndav = hdr->iter;
     }
     eigvals.assign(1, hdr->energy);
-    B2G_HIP(launch_copy(w.B, x, n, st));
+    B2G_HIP(launch_copy(c.B, x, n, st));
     B2G_HIP(hipStreamSynchronize(st));
     if (status != 1) {
         set_error("Davidson (graph): not converged after " +

@@ -120,14 +120,15 @@ template <int TM, int TN> struct PkTile {
     static constexpr int BUF = 2 * ASZ + 2 * BSZ; // A hi, A lo, B hi, B lo
     u32x4 ah[GA], al[GA], bh[GB], bl[GB];
 
-    // vector e of a tile: row and k group.  Eight consecutive lanes take one
-    // k group of eight consecutive rows, so that a wave's 16-byte LDS stores
-    // follow the bank pattern of the fragment reads (rows 80 bytes apart);
-    // a wave still covers 16 whole rows of 64 bytes per global load.
+    // vector e of a tile: row and k group.  Sixteen consecutive lanes take
+    // one k group of sixteen consecutive rows, so that a wave's 16-byte LDS
+    // stores follow the bank pattern of the fragment reads (rows 80 bytes
+    // apart: any 16 of them cover distinct 16-byte bank groups of 256
+    // bytes); a wave still covers 16 whole rows of 64 bytes per global load.
     static __device__ __forceinline__ int vrow(int e) {
-        return (e & 7) + 8 * (e >> 5);
+        return (e & 15) + 16 * (e >> 6);
     }
-    static __device__ __forceinline__ int vgrp(int e) { return (e >> 3) & 3; }
+    static __device__ __forceinline__ int vgrp(int e) { return (e >> 4) & 3; }
 
     __device__ __forceinline__ void load(
         __amdgpu_buffer_rsrc_t rAh, __amdgpu_buffer_rsrc_t rAl,

@@ -372,6 +372,14 @@ __global__ __launch_bounds__(256) void seg_gemm_kernel(
     }
 }
 
+// One workgroup reduces a 1024-element slice of one tile (tiles hold at most
+// 128 x 128 = 16 slices): thread i takes elements 4i..4i+3 of the slice,
+// reading each part's four values as two 16-byte loads (partial slots are
+// 1024-element aligned and tile widths multiples of 32, so the four share a
+// row).  Parts are summed in increasing order: the result does not depend on
+// the launch shape.
+constexpr int kRedSlices = 16;
+
 template <typename TD>
 __global__ __launch_bounds__(256) void splitk_reduce_kernel(
     TD *__restrict__ D, const TD *__restrict__ P,
@@ -380,21 +388,34 @@ __global__ __launch_bounds__(256) void splitk_reduce_kernel(
     const int rhs = blockIdx.y;
     D += rhs * d_rs;
     P += rhs * p_rs;
-    for (int t = blockIdx.x; t < ntasks; t += gridDim.x) {
-        const RedTask rt = tasks[t];
+    const int64_t nwork = (int64_t)ntasks * kRedSlices;
+    for (int64_t w = blockIdx.x; w < nwork; w += gridDim.x) {
+        const RedTask rt = tasks[w / kRedSlices];
         const uint32_t tsz = rt.tm * rt.tn;
-        for (uint32_t e = threadIdx.x; e < tsz; e += 256) {
-            const uint32_t r = e / rt.tn, c = e % rt.tn;
-            const uint32_t row = rt.row0 + r, col = rt.col0 + c;
-            if (row >= rt.M || col >= rt.N)
-                continue;
-            double sum = 0.0;
-            for (uint32_t p = 0; p < rt.nparts; p++)
-                sum += (double)P[rt.p_off + (uint64_t)p * tsz + e];
-            const uint64_t idx = rt.d_off + (uint64_t)row * rt.ldd + col;
-            D[idx] = ((rt.beta | beta0) & 1) ? (TD)((double)D[idx] + sum)
-                                             : (TD)sum;
+        const uint32_t e = (uint32_t)(w % kRedSlices) * 1024 + 4 * threadIdx.x;
+        if (e >= tsz)
+            continue;
+        const uint32_t r = e / rt.tn, c = e % rt.tn;
+        const uint32_t row = rt.row0 + r, col = rt.col0 + c;
+        if (row >= rt.M || col >= rt.N)
+            continue;
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+        const TD *q = P + rt.p_off + e;
+#pragma unroll 4
+        for (uint32_t p = 0; p < rt.nparts; p++) {
+            const TD *qp = q + (uint64_t)p * tsz;
+            s0 += (double)qp[0];
+            s1 += (double)qp[1];
+            s2 += (double)qp[2];
+            s3 += (double)qp[3];
         }
+        const double sv[4] = {s0, s1, s2, s3};
+        const uint64_t idx = rt.d_off + (uint64_t)row * rt.ldd + col;
+        const bool acc = (rt.beta | beta0) & 1;
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+            if (col + i < rt.N)
+                D[idx + i] = acc ? (TD)((double)D[idx + i] + sv[i]) : (TD)sv[i];
     }
 }
 
@@ -405,7 +426,9 @@ hipError_t launch_splitk_reduce(TD *D, const TD *P, const RedTask *tasks,
                                 hipStream_t st) {
     if (ntasks <= 0)
         return hipSuccess;
-    dim3 g((unsigned)(grid < ntasks ? grid : ntasks), (unsigned)n_rhs);
+    const int64_t nwork = (int64_t)ntasks * kRedSlices;
+    const int64_t cap = (int64_t)grid * 4;
+    dim3 g((unsigned)(nwork < cap ? nwork : cap), (unsigned)n_rhs);
     hipLaunchKernelGGL(splitk_reduce_kernel<TD>, g, dim3(256), 0, st, D, P,
                        tasks, ntasks, d_rs, p_rs, beta0);
     return hipGetLastError();

@@ -77,8 +77,9 @@ struct Packer {
 };
 
 struct HostChunk {
-    std::vector<Task> t1[kTileConfigs];
+    std::vector<Task> t1[kTileConfigs], t2[kTileConfigs];
     std::vector<Seg> s1, s2;
+    XcdSplit x1[kTileConfigs] = {}, x2[kTileConfigs] = {};
 };
 
 } // namespace
@@ -182,6 +183,15 @@ static int pk_build_into(b2g_plan p, PkPlan *k) {
             q.flags = 0;
             h.s2.push_back(q);
         }
+        // tiles of one output block (one segment list) share their operand
+        // panels: keep them on one XCD, back to back (its L2 serves the
+        // re-reads); the split-K reductions address partial slots by task
+        // offsets, which reordering keeps
+        for (int c = 0; c < kTileConfigs; c++) {
+            h.t2[c] = ch.t2[c];
+            h.x1[c] = xcd_order(h.t1[c], h.s1, true);
+            h.x2[c] = xcd_order(h.t2[c], h.s2, true);
+        }
         k->w_max = std::max(k->w_max, wlen);
     }
     k->l_len = L.len, k->r_len = R.len, k->x_len = X.len;
@@ -189,7 +199,7 @@ static int pk_build_into(b2g_plan p, PkPlan *k) {
     size_t bytes = 0;
     for (auto &h : hc) {
         for (int c = 0; c < kTileConfigs; c++)
-            bytes += h.t1[c].size() * sizeof(Task);
+            bytes += (h.t1[c].size() + h.t2[c].size()) * sizeof(Task);
         bytes += (h.s1.size() + h.s2.size()) * sizeof(Seg);
     }
     bytes += (L.tiles.size() + R.tiles.size() + X.tiles.size()) *
@@ -206,9 +216,14 @@ static int pk_build_into(b2g_plan p, PkPlan *k) {
     };
     for (auto &h : hc) {
         PkChunk pc;
-        for (int c = 0; c < kTileConfigs; c++)
+        for (int c = 0; c < kTileConfigs; c++) {
             pc.d_t1[c] = (Task *)put(h.t1[c].data(),
                                      h.t1[c].size() * sizeof(Task));
+            pc.d_t2[c] = (Task *)put(h.t2[c].data(),
+                                     h.t2[c].size() * sizeof(Task));
+            pc.x1[c] = h.x1[c];
+            pc.x2[c] = h.x2[c];
+        }
         pc.d_s1 = (Seg *)put(h.s1.data(), h.s1.size() * sizeof(Seg));
         pc.d_s2 = (Seg *)put(h.s2.data(), h.s2.size() * sizeof(Seg));
         k->chunks.push_back(pc);
@@ -248,6 +263,7 @@ int pk_build(b2g_plan p) {
         return r;
     }
     p->pk = k;
+    p->scratch_gen++;
     return B2G_OK;
 }
 
@@ -264,6 +280,7 @@ int pk_reserve(b2g_plan p, int n_rhs) {
     const size_t n = std::max<uint64_t>(2 * (k->x_len + k->w_max), 8) * n_rhs;
     B2G_HIP(hipMalloc(&k->scratch, n * sizeof(uint16_t)));
     k->scratch_rhs = n_rhs;
+    p->scratch_gen++;
     return B2G_OK;
 }
 

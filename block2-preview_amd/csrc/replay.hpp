@@ -64,6 +64,9 @@ struct b2g_plan_s {
     b2g::GradPlan *grad = nullptr;
     // the split-BF16 replay on packed operand planes (built on first use)
     b2g::PkPlan *pk = nullptr;
+    // bumped whenever scratch that launches address (W, partials, packed
+    // planes) is reallocated: captured graphs of older generations are stale
+    uint64_t scratch_gen = 0;
 };
 
 
@@ -86,6 +89,10 @@ void add_tiles(std::vector<Task> *tasks, uint64_t d_off, uint32_t M,
 double task_flops(const std::vector<Task> &tasks, const std::vector<Seg> &segs,
                   int cfg);
 void sort_tasks(std::vector<Task> &tasks, const std::vector<Seg> &segs);
+// order one launch's tasks: grouped by segment list onto the 8 XCDs (the
+// returned per-XCD ranges), or sorted by cost (empty ranges)
+XcdSplit xcd_order(std::vector<Task> &tasks, const std::vector<Seg> &segs,
+                   bool grouped);
 // scratch elements per chunk and right-hand side (B2G_W_CAP)
 uint64_t w_cap_elems();
 // allocate the H*psi scratch for n_rhs at precision ahead of a capture
@@ -97,8 +104,10 @@ int plan_reserve(b2g_plan p, int n_rhs, int precision);
 // left-operator blocks and packed W^T and keeps the FP64 tasks, split-K
 // reductions and partial buffer of the plan.
 struct PkChunk {
-    Task *d_t1[kTileConfigs] = {}; // phase-1 tasks addressing packed W^T
+    // the plan's tasks in XCD-grouped order; phase-1 tasks address packed W^T
+    Task *d_t1[kTileConfigs] = {}, *d_t2[kTileConfigs] = {};
     Seg *d_s1 = nullptr, *d_s2 = nullptr;
+    XcdSplit x1[kTileConfigs] = {}, x2[kTileConfigs] = {};
 };
 struct PkPlan {
     std::vector<PkChunk> chunks;

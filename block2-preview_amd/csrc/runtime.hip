@@ -430,25 +430,28 @@ void sort_tasks(std::vector<Task> &tasks, const std::vector<Seg> &segs) {
                      });
 }
 
-// B2G_XCD=1 turns the per-XCD task ranges on (measured on the n=40 M=3000
-// middle-site workload: no gain over the cost-sorted list, so off)
+// B2G_XCD=1 turns the per-XCD task ranges on for the FP64 / FP32 kernels
+// (measured on the n=40 M=3000 middle-site workload: no gain over the
+// cost-sorted list, so off; the split-BF16 kernels, which need 5x the
+// operand bandwidth per flop, always use them)
 static bool xcd_enabled() {
     const char *e = getenv("B2G_XCD");
     return e != nullptr && atoi(e) != 0;
 }
 
-// Orders one launch's tasks for the 8 XCDs and returns their ranges.  Tasks
+// Orders one launch's tasks for the 8 XCDs and returns their ranges (not
+// grouped: sorted by cost, one range for all XCDs).  Tasks
 // with the same segment list (the tiles of one output block, or of one
 // split-K group) re-read the same operator and work panels, so they form a
 // group that stays on one XCD and runs back to back there (shared L2).
 // Groups are dealt longest first to the least-loaded XCD; each XCD runs its
 // groups in that order, tiles of a group in creation (row-major) order.
-static XcdSplit xcd_order(std::vector<Task> &tasks,
-                          const std::vector<Seg> &segs) {
+XcdSplit xcd_order(std::vector<Task> &tasks, const std::vector<Seg> &segs,
+                   bool grouped) {
     XcdSplit xs{};
     if (tasks.empty())
         return xs;
-    if (!xcd_enabled()) {
+    if (!grouped) {
         sort_tasks(tasks, segs);
         return xs;
     }
@@ -711,8 +714,8 @@ static int compile_plan(b2g_plan p, const b2g_pair *pairs, size_t n_pairs,
                          &kv.second, 1);
             }
             for (int c = 0; c < kTileConfigs; c++) {
-                ch.x1[c] = xcd_order(ch.t1[c], ch.s1);
-                ch.x2[c] = xcd_order(ch.t2[c], ch.s2);
+                ch.x1[c] = xcd_order(ch.t1[c], ch.s1, xcd_enabled());
+                ch.x2[c] = xcd_order(ch.t2[c], ch.s2, xcd_enabled());
                 ch.f1[c] = task_flops(ch.t1[c], ch.s1, c);
                 ch.f2[c] = task_flops(ch.t2[c], ch.s2, c);
                 p->kflops[B2G_K_PHASE1 + c] += ch.f1[c];
@@ -910,6 +913,7 @@ int plan_reserve(b2g_plan p, int n_rhs, int precision) {
         }
         B2G_HIP(hipMalloc(&p->w, need));
         p->w_bytes = need;
+        p->scratch_gen++;
     }
     const size_t pneed = std::max<uint64_t>(p->p_max, 1) * n_rhs * 8;
     if (pneed > p->p_bytes) {
@@ -921,6 +925,7 @@ int plan_reserve(b2g_plan p, int n_rhs, int precision) {
         }
         B2G_HIP(hipMalloc(&p->pbuf, pneed));
         p->p_bytes = pneed;
+        p->scratch_gen++;
     }
     return B2G_OK;
 }
@@ -1035,15 +1040,15 @@ extern "C" int b2g_hpsi(b2g_ctx ctx, b2g_plan p, const double *x, double *y,
                     c, (cbf)pk->xh(), (cbf)pk->xl(), (cbf)pk->rh(),
                     (cbf)pk->rl(), pk->wh(), pk->wl(), kc.d_t1[c],
                     (int)ch.t1[c].size(), kc.d_s1, pk->x_len, 0, pk->w_max,
-                    0, 1.0, 0, n_rhs, grid, st, ch.x1[c]);
+                    0, 1.0, 0, n_rhs, grid, st, kc.x1[c]);
             }));
             B2G_HIP(phase(ch.t2, ch.f2, B2G_K_PHASE2,
                           [&](int c, hipStream_t st) {
                 return launch_seg_gemm_pk<1>(
                     c, (cbf)pk->lh(), (cbf)pk->ll(), (cbf)pk->wh(),
-                    (cbf)pk->wl(), y, P, ch.d_t2[c], (int)ch.t2[c].size(),
+                    (cbf)pk->wl(), y, P, kc.d_t2[c], (int)ch.t2[c].size(),
                     kc.d_s2, 0, pk->w_max, p->y_len, p->p_max, scale, beta0,
-                    n_rhs, grid, st, ch.x2[c]);
+                    n_rhs, grid, st, kc.x2[c]);
             }));
             const int nr = (int)ch.red.size();
             if (nr > 0)
