@@ -46,69 +46,103 @@ __global__ void put_sigma_kernel(DavState *st, double *__restrict__ S,
         dst[i] = sn[i];
 }
 
-// partial Gram sums of <B_i|S_j>, i, j < m, over row tiles of 32
+// Partial Gram sums <B_i|S_j>, j <= i < m (the lower triangle, pair
+// p = i(i+1)/2 + j), per block over row chunks of rc rows.  A chunk's rows of
+// every B_i and S_j are staged in LDS (row stride rc + 1 doubles); the pairs
+// are spread over the 256 threads, tpp threads per pair each summing every
+// tpp-th row, or, past 256 pairs, one thread per pair slot (up to
+// kPairSlots slots).  Per block: partial[blockIdx * kPairStride + p].
+constexpr int kPairStride = 64 * 65 / 2;
+constexpr int kPairSlots = (kPairStride + 255) / 256;
+
+__device__ __forceinline__ void pair_ij(int p, int &i, int &j) {
+    i = (int)((sqrt(8.0 * p + 1.0) - 1.0) * 0.5);
+    while ((i + 1) * (i + 2) / 2 <= p)
+        i++;
+    while (i * (i + 1) / 2 > p)
+        i--;
+    j = p - i * (i + 1) / 2;
+}
+
 __global__ __launch_bounds__(256) void gram_kernel(
     const DavState *st, const double *__restrict__ X,
-    const double *__restrict__ Y, size_t n, double *__restrict__ partial) {
-    constexpr int RT = 32;
-    __shared__ double xs[64][RT + 1];
-    __shared__ double ys[64][RT + 1];
-    const int m = st->m, npair = m * m;
-    double acc[16];
+    const double *__restrict__ Y, size_t n, double *__restrict__ partial,
+    int rc) {
+    extern __shared__ double dyn[];
+    const int m = st->m, npair = m * (m + 1) / 2, ld = rc + 1;
+    double *xs = dyn, *ys = dyn + (size_t)m * ld;
+    const int tid = threadIdx.x;
+    const int tpp = npair <= 256 ? 256 / npair : 1;
+    const int phase = npair <= 256 ? tid % tpp : 0;
+    int pi[kPairSlots], pj[kPairSlots];
+    double acc[kPairSlots];
 #pragma unroll
-    for (int q = 0; q < 16; q++)
-        acc[q] = 0.0;
-    for (size_t r0 = (size_t)blockIdx.x * RT; r0 < n;
-         r0 += (size_t)gridDim.x * RT) {
-        for (int e = threadIdx.x; e < m * RT; e += 256) {
-            const int i = e / RT, r = e % RT;
-            const bool ok = r0 + r < n;
-            xs[i][r] = ok ? X[(size_t)i * n + r0 + r] : 0.0;
-            ys[i][r] = ok ? Y[(size_t)i * n + r0 + r] : 0.0;
+    for (int q = 0; q < kPairSlots; q++) {
+        const int p = npair <= 256 ? (q == 0 ? tid / tpp : npair)
+                                   : tid + 256 * q;
+        pi[q] = -1, pj[q] = 0, acc[q] = 0.0;
+        if (p < npair)
+            pair_ij(p, pi[q], pj[q]);
+    }
+    for (size_t r0 = (size_t)blockIdx.x * rc; r0 < n;
+         r0 += (size_t)gridDim.x * rc) {
+        const int rows = (int)(n - r0 < (size_t)rc ? n - r0 : (size_t)rc);
+        for (int e = tid; e < m * rc; e += 256) {
+            const int i = e / rc, r = e % rc;
+            const bool ok = r < rows;
+            xs[i * ld + r] = ok ? X[(size_t)i * n + r0 + r] : 0.0;
+            ys[i * ld + r] = ok ? Y[(size_t)i * n + r0 + r] : 0.0;
         }
         __syncthreads();
 #pragma unroll
-        for (int q = 0; q < 16; q++) {
-            const int p = threadIdx.x + q * 256;
-            if (p < npair) {
-                const int i = p / m, j = p % m;
-                double s = 0.0;
-                for (int r = 0; r < RT; r++)
-                    s += xs[i][r] * ys[j][r];
-                acc[q] += s;
+        for (int q = 0; q < kPairSlots; q++) {
+            if (pi[q] >= 0) {
+                const double *a = xs + pi[q] * ld, *b = ys + pj[q] * ld;
+                double sum = 0.0;
+                for (int r = phase; r < rows; r += tpp)
+                    sum += a[r] * b[r];
+                acc[q] += sum;
             }
         }
         __syncthreads();
     }
+    // combine the tpp phases of each pair (fixed order)
+    double *red = dyn;
+    if (npair <= 256) {
+        red[tid] = pi[0] >= 0 ? acc[0] : 0.0;
+        __syncthreads();
+        if (tid < npair) {
+            double sum = 0.0;
+            for (int k = 0; k < tpp; k++)
+                sum += red[tid * tpp + k];
+            partial[(size_t)blockIdx.x * kPairStride + tid] = sum;
+        }
+    } else {
 #pragma unroll
-    for (int q = 0; q < 16; q++) {
-        const int p = threadIdx.x + q * 256;
-        if (p < npair)
-            partial[(size_t)blockIdx.x * npair + p] = acc[q];
+        for (int q = 0; q < kPairSlots; q++)
+            if (pi[q] >= 0)
+                partial[(size_t)blockIdx.x * kPairStride + tid + 256 * q] =
+                    acc[q];
     }
 }
-
-// G = sum of the Gram partials, symmetrised from its lower triangle
+// one block per pair: G[i][j] = G[j][i] = sum over blocks (fixed order)
 __global__ __launch_bounds__(256) void gram_reduce_kernel(
     DavState *st, const double *__restrict__ partial, int nblk) {
-    const int m = st->m, npair = m * m;
-    for (int p = threadIdx.x; p < npair; p += 256) {
-        double s = 0.0;
-        for (int b = 0; b < nblk; b++)
-            s += partial[(size_t)b * npair + p];
-        st->G[p] = s;
-    }
-    __syncthreads();
-    for (int p = threadIdx.x; p < npair; p += 256) {
-        const int i = p / m, j = p % m;
-        if (j > i)
-            st->G[p] = st->G[j * m + i];
+    __shared__ double sh[256];
+    const int m = st->m, npair = m * (m + 1) / 2, p = blockIdx.x;
+    if (p >= npair)
+        return;
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += 256)
+        s += partial[(size_t)b * kPairStride + p];
+    s = bsum(s, sh);
+    if (threadIdx.x == 0) {
+        int i, j;
+        pair_ij(p, i, j);
+        st->G[i * m + j] = s;
+        st->G[j * m + i] = s;
     }
 }
-
-// Ritz pairs of the m x m subspace matrix: cyclic Jacobi, round-robin pair
-// order, rotation angle and stopping rule as the host solver; eigenvalues
-// ascending into ld, eigenvector j into V[j*m + i].
 __global__ __launch_bounds__(256) void ritz_kernel(DavState *st) {
     __shared__ double a[64][65];
     __shared__ double z[64][65];
@@ -216,31 +250,38 @@ __global__ __launch_bounds__(256) void ritz_kernel(DavState *st) {
 }
 
 // X_j <- sum_i V[j*m + i] X_i, j < m (in place, rows of n)
+// X_j <- sum_i V[j*m+i] X_i (j < m) for X = S and X = B, one row per thread;
+// the row's m values sit in registers (MAXM >= m, chosen from the subspace
+// capacity when the step is captured).
+template <int MAXM>
 __global__ __launch_bounds__(256) void rotate_dev_kernel(
-    const DavState *st, double *__restrict__ X, size_t n) {
-    __shared__ double vs[64 * 64];
+    const DavState *st, double *__restrict__ S, double *__restrict__ B,
+    size_t n) {
+    __shared__ double vs[MAXM * MAXM];
     const int m = st->m;
     for (int e = threadIdx.x; e < m * m; e += 256)
         vs[e] = st->V[e];
     __syncthreads();
     for (size_t r = blockIdx.x * (size_t)256 + threadIdx.x; r < n;
          r += (size_t)gridDim.x * 256) {
-        double x[64];
 #pragma unroll
-        for (int i = 0; i < 64; i++)
-            x[i] = i < m ? X[(size_t)i * n + r] : 0.0;
-        for (int j = 0; j < m; j++) {
-            double s = 0.0;
+        for (int pass = 0; pass < 2; pass++) {
+            double *X = pass == 0 ? S : B;
+            double x[MAXM];
 #pragma unroll
-            for (int i = 0; i < 64; i++)
-                if (i < m)
-                    s += vs[j * m + i] * x[i];
-            X[(size_t)j * n + r] = s;
+            for (int i = 0; i < MAXM; i++)
+                x[i] = i < m ? X[(size_t)i * n + r] : 0.0;
+            for (int j = 0; j < m; j++) {
+                double s = 0.0;
+#pragma unroll
+                for (int i = 0; i < MAXM; i++)
+                    if (i < m)
+                        s += vs[j * m + i] * x[i];
+                X[(size_t)j * n + r] = s;
+            }
         }
     }
 }
-
-// q = S_0 - ld_0 B_0 and partial sums of q.q
 __global__ __launch_bounds__(256) void residual_kernel(
     const DavState *st, const double *__restrict__ B,
     const double *__restrict__ S, double *__restrict__ q, size_t n,
@@ -413,9 +454,18 @@ __global__ void append_kernel(const DavState *st, const double *__restrict__ q,
     }
 }
 
-__global__ void bump_kernel(DavState *st) {
+// the last kernel of a step: grow the subspace, then publish the 64-byte
+// state header to the host (pinned, device-mapped memory)
+__global__ void bump_kernel(DavState *st, DavState *host) {
     if (st->status == 0)
         st->m++;
+    if (host != nullptr) {
+        const int *src = (const int *)st;
+        volatile int *dst = (volatile int *)host;
+        for (int k = 0; k < 16; k++)
+            dst[k] = src[k];
+        __threadfence_system();
+    }
 }
 
 dim3 vgrid(size_t n) {
@@ -433,16 +483,34 @@ dim3 vgrid(size_t n) {
 hipError_t launch_davidson_step(DavState *st, double *B, double *S,
                                 const double *sn, double *q, double *t,
                                 double *qn, const double *diag, size_t n,
-                                double *partial, hipStream_t s) {
+                                int nv, double *partial, DavState *host,
+                                hipStream_t s) {
+    if (nv < 2 || nv > 64)
+        return hipErrorInvalidValue;
     hipLaunchKernelGGL(put_sigma_kernel, vgrid(n), dim3(256), 0, s, st, S, sn,
                        n);
-    hipLaunchKernelGGL(gram_kernel, dim3(kGrid), dim3(256), 0, s, st, B, S, n,
-                       partial);
-    hipLaunchKernelGGL(gram_reduce_kernel, dim3(1), dim3(256), 0, s, st,
-                       partial, kGrid);
+    // Gram row chunk: both staging arrays within 64 KB of LDS
+    int rc = 256;
+    while (rc > 16 && 2 * nv * (rc + 1) * 8 > 65536)
+        rc /= 2;
+    hipLaunchKernelGGL(gram_kernel, dim3(kGrid), dim3(256),
+                       (size_t)2 * nv * (rc + 1) * sizeof(double), s, st, B, S,
+                       n, partial, rc);
+    hipLaunchKernelGGL(gram_reduce_kernel, dim3(kPairStride), dim3(256), 0, s,
+                       st, partial, kGrid);
     hipLaunchKernelGGL(ritz_kernel, dim3(1), dim3(256), 0, s, st);
-    hipLaunchKernelGGL(rotate_dev_kernel, vgrid(n), dim3(256), 0, s, st, S, n);
-    hipLaunchKernelGGL(rotate_dev_kernel, vgrid(n), dim3(256), 0, s, st, B, n);
+    if (nv <= 8)
+        hipLaunchKernelGGL(rotate_dev_kernel<8>, vgrid(n), dim3(256), 0, s, st,
+                           S, B, n);
+    else if (nv <= 16)
+        hipLaunchKernelGGL(rotate_dev_kernel<16>, vgrid(n), dim3(256), 0, s,
+                           st, S, B, n);
+    else if (nv <= 32)
+        hipLaunchKernelGGL(rotate_dev_kernel<32>, vgrid(n), dim3(256), 0, s,
+                           st, S, B, n);
+    else
+        hipLaunchKernelGGL(rotate_dev_kernel<64>, vgrid(n), dim3(256), 0, s,
+                           st, S, B, n);
     hipLaunchKernelGGL(residual_kernel, dim3(kGrid), dim3(256), 0, s, st, B,
                        S, q, n, partial);
     hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(256), 0, s, partial, kGrid,
@@ -467,10 +535,9 @@ hipError_t launch_davidson_step(DavState *st, double *B, double *S,
                        1, st->scal + 3);
     hipLaunchKernelGGL(append_kernel, vgrid(n), dim3(256), 0, s, st, q, qn, B,
                        n);
-    hipLaunchKernelGGL(bump_kernel, dim3(1), dim3(1), 0, s, st);
+    hipLaunchKernelGGL(bump_kernel, dim3(1), dim3(1), 0, s, st, host);
     return hipGetLastError();
 }
-
 int davidson_partial_len() { return kGrid * 64 * 64; }
 
 } // namespace b2g

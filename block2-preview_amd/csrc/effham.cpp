@@ -851,18 +851,17 @@ static int davidson_graph(b2g_effham h, double *x, double conv_thrd,
             hipGraphExecDestroy(exec);
             exec = nullptr;
         }
-        // capture one iteration: H qn -> sn, the device step, and the copy
-        // of the state header to pinned host memory
+        // capture one iteration: H qn -> sn and the device step, which ends
+        // by writing the state header to pinned host memory
         hipGraph_t graph = nullptr;
         B2G_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         int rc = b2g_hpsi(ctx, h->plan, c.qn, c.sn, 1, 1.0, 0, precision);
         hipError_t e = rc == B2G_OK
                            ? launch_davidson_step(c.dst, c.B, c.S, c.sn, c.q,
                                                   c.t, c.qn, h->diag, n,
-                                                  c.partial, st)
+                                                  (int)nv, c.partial, c.hdr,
+                                                  st)
                            : hipSuccess;
-        if (e == hipSuccess && rc == B2G_OK)
-            e = hipMemcpyAsync(c.hdr, c.dst, 64, hipMemcpyDeviceToHost, st);
         const hipError_t e2 = hipStreamEndCapture(st, &graph);
         if (rc == B2G_OK && e == hipSuccess && e2 == hipSuccess)
             e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);

@@ -73,12 +73,15 @@ hipError_t launch_olsen(double *q, const double *c, const double *aa,
                         double *sums, hipStream_t st);
 int red_grid();
 
-// One device Davidson step after sn = H qn (davidson_dev.hip); `partial`
-// holds davidson_partial_len() doubles.
+// One device Davidson step after sn = H qn (davidson_dev.hip): nv is the
+// subspace capacity (2..64 vectors of B and S), `partial` holds
+// davidson_partial_len() doubles, and the step ends by writing the state
+// header (64 bytes) to `host` (pinned memory; nullptr: not written).
 hipError_t launch_davidson_step(DavState *st, double *B, double *S,
                                 const double *sn, double *q, double *t,
                                 double *qn, const double *diag, size_t n,
-                                double *partial, hipStream_t s);
+                                int nv, double *partial, DavState *host,
+                                hipStream_t s);
 int davidson_partial_len();
 
 } // namespace b2g

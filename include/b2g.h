@@ -177,7 +177,9 @@ int b2g_plan_macs(b2g_plan plan, uint64_t *out);
 #define B2G_K_PACK 19
 #define B2G_NKERNELS 20
 /* Per-launch kernel timing with HIP events on the context stream: while
- * enabled, every b2g_hpsi brackets each kernel launch with two events;
+ * enabled, every b2g_hpsi runs its launches back to back on the context
+ * stream (no overlap on the auxiliary streams) and brackets each with two
+ * events;
  * b2g_plan_timing waits for them, writes per kernel id the summed duration
  * (ms) and the number of launches into ms[B2G_NKERNELS] and
  * n_launch[B2G_NKERNELS], then clears the record.                        */
