@@ -1,11 +1,12 @@
 """Tile autotuner for the grouped GEMM.
 
-Measures the FP64 throughput of the nine workgroup tile shapes (TM, TN in
-{32, 64, 128}) on uniform block problems whose block size is the
-multiply-add-weighted median output-block size of the problem to be solved
-(its sector-size histogram), and installs the rates with
-b2g_set_tile_rates, so that the tiling of every plan uploaded afterwards
-minimises its estimated time.  Measurement goes through the thin C ABI with
+Measures the throughput of the nine workgroup tile shapes (TM, TN in
+{32, 64, 128}) of the FP64 kernels, or of the split-BF16 kernels, on uniform
+block problems whose block size is the multiply-add-weighted median
+output-block size of the problem to be solved (its sector-size histogram),
+and installs the rates with b2g_set_tile_rates_for, so that the tiling of
+every plan uploaded afterwards (FP64 / FP32 kernels), or of every packed
+split-BF16 plan built afterwards, minimises its estimated time.  Measurement goes through the thin C ABI with
 HIP events on the library's stream.
 """
 import ctypes as C
@@ -33,7 +34,7 @@ def representative_dim(pairs):
     return int(d[o][np.searchsorted(cw, 0.5 * cw[-1])])
 
 
-def _measure(ctx, d, cfg, target_macs=5e10, reps=3):
+def _measure(ctx, d, cfg, prec=0, target_macs=5e10, reps=3):
     tm, tn = 32 << (cfg // 3), 32 << (cfg % 3)
     lib = L.lib()
     blk = d * d
@@ -53,7 +54,11 @@ def _measure(ctx, d, cfg, target_macs=5e10, reps=3):
     plan = C.c_void_p()
     try:
         L.check(lib.b2g_upload_plan(ctx, lb, rb, pairs, n, yb, n, n * blk,
-                                    n * blk, 0, C.byref(plan)))
+                                    n * blk, 1 if prec == 1 else 0,
+                                    C.byref(plan)))
+        # the packed split-BF16 plan is tiled when it is built: build it
+        # while the tile shape is forced
+        L.check(lib.b2g_plan_reserve(plan, 1, prec))
     finally:
         if old is None:
             os.environ.pop("B2G_TILE", None)
@@ -61,7 +66,8 @@ def _measure(ctx, d, cfg, target_macs=5e10, reps=3):
             os.environ["B2G_TILE"] = old
     x = torch.rand(n * blk, dtype=torch.float64, device="cuda")
     y = torch.empty_like(x)
-    L.check(lib.b2g_hpsi(ctx, plan, x.data_ptr(), y.data_ptr(), 1, 1.0, 0, 0))
+    L.check(lib.b2g_hpsi(ctx, plan, x.data_ptr(), y.data_ptr(), 1, 1.0, 0,
+                         prec))
     L.check(lib.b2g_ctx_synchronize(ctx))
     stream = torch.cuda.ExternalStream(lib.b2g_ctx_stream(ctx))
     e0 = torch.cuda.Event(enable_timing=True)
@@ -69,7 +75,7 @@ def _measure(ctx, d, cfg, target_macs=5e10, reps=3):
     e0.record(stream)
     for _ in range(reps):
         L.check(lib.b2g_hpsi(ctx, plan, x.data_ptr(), y.data_ptr(), 1, 1.0,
-                             0, 0))
+                             0, prec))
     e1.record(stream)
     e1.synchronize()
     ms = e0.elapsed_time(e1) / reps
@@ -79,12 +85,15 @@ def _measure(ctx, d, cfg, target_macs=5e10, reps=3):
     return 4.0 * n * d ** 3 / (ms * 1e-3) / 1e12
 
 
-def tune(ctx, dim=256, install=True):
-    """Measure the nine tile shapes at block size `dim`; returns TFLOP/s per
-    configuration and (optionally) installs them as the tiling rates."""
+def tune(ctx, dim=256, install=True, precision="f64"):
+    """Measure the nine tile shapes at block size `dim` with the kernels of
+    `precision` ("f64" or "bf16x3"); returns TFLOP/s (FP64-equivalent) per
+    configuration and (optionally) installs them as that precision's tiling
+    rates."""
     h = ctx.h if hasattr(ctx, "h") else ctx
-    rates = [_measure(h, int(dim), cfg) for cfg in range(9)]
+    prec = L.PREC[precision]
+    rates = [_measure(h, int(dim), cfg, prec) for cfg in range(9)]
     if install:
         arr = (C.c_double * 9)(*rates)
-        L.check(L.lib().b2g_set_tile_rates(arr, 9))
+        L.check(L.lib().b2g_set_tile_rates_for(prec, arr, 9))
     return rates

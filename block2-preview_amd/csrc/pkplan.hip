@@ -79,6 +79,8 @@ struct Packer {
 struct HostChunk {
     std::vector<Task> t1[kTileConfigs], t2[kTileConfigs];
     std::vector<Seg> s1, s2;
+    std::vector<RedTask> red;
+    uint64_t p_len = 0;
     XcdSplit x1[kTileConfigs] = {}, x2[kTileConfigs] = {};
 };
 
@@ -161,12 +163,13 @@ static int pk_build_into(b2g_plan p, PkPlan *k) {
             q.flags = 0, q.alpha = 1.0;
             h.s1.push_back(q);
         }
-        for (int c = 0; c < kTileConfigs; c++)
-            for (Task t : ch.t1[c]) {
-                t.d_off = wpk[t.seg_begin];
-                t.ldd = (uint32_t)round8(t.M);
-                h.t1[c].push_back(t);
-            }
+        // phase 1 re-tiled with the split-BF16 tile rates: one W^T block
+        // per segment
+        for (size_t si = 0; si < ch.s1.size(); si++)
+            for (const Tile &t : tile_cover(m1[si], n1[si], ch.s1[si].K, 1))
+                h.t1[cfg_of(t)].push_back(
+                    make_task(wpk[si], (uint32_t)round8(m1[si]), m1[si],
+                              n1[si], t, (uint32_t)si, (uint32_t)si + 1, 0));
         for (size_t si = 0; si < ch.s2.size(); si++) {
             const Seg &s = ch.s2[si];
             auto it = wmap.find(s.b_off);
@@ -183,16 +186,20 @@ static int pk_build_into(b2g_plan p, PkPlan *k) {
             q.flags = 0;
             h.s2.push_back(q);
         }
+        // phase 2 re-tiled likewise, block by block, with split-K partial
+        // slots and reductions of its own
+        for (const BlockJob &j : ch.jobs2)
+            add_split_tiles(h.s2, j, 1, h.t2, h.red, h.p_len);
         // tiles of one output block (one segment list) share their operand
         // panels: keep them on one XCD, back to back (its L2 serves the
-        // re-reads); the split-K reductions address partial slots by task
-        // offsets, which reordering keeps
+        // re-reads); the reductions address partial slots by task offsets,
+        // which reordering keeps
         for (int c = 0; c < kTileConfigs; c++) {
-            h.t2[c] = ch.t2[c];
             h.x1[c] = xcd_order(h.t1[c], h.s1, true);
             h.x2[c] = xcd_order(h.t2[c], h.s2, true);
         }
         k->w_max = std::max(k->w_max, wlen);
+        k->p_max = std::max(k->p_max, h.p_len);
     }
     k->l_len = L.len, k->r_len = R.len, k->x_len = X.len;
     // one allocation for every task, segment and pack-tile array
@@ -201,6 +208,7 @@ static int pk_build_into(b2g_plan p, PkPlan *k) {
         for (int c = 0; c < kTileConfigs; c++)
             bytes += (h.t1[c].size() + h.t2[c].size()) * sizeof(Task);
         bytes += (h.s1.size() + h.s2.size()) * sizeof(Seg);
+        bytes += h.red.size() * sizeof(RedTask);
     }
     bytes += (L.tiles.size() + R.tiles.size() + X.tiles.size()) *
              sizeof(PackTile);
@@ -223,7 +231,13 @@ static int pk_build_into(b2g_plan p, PkPlan *k) {
                                      h.t2[c].size() * sizeof(Task));
             pc.x1[c] = h.x1[c];
             pc.x2[c] = h.x2[c];
+            pc.n1[c] = h.t1[c].size();
+            pc.n2[c] = h.t2[c].size();
+            pc.f1[c] = task_flops(h.t1[c], h.s1, c);
+            pc.f2[c] = task_flops(h.t2[c], h.s2, c);
         }
+        pc.d_red = (RedTask *)put(h.red.data(), h.red.size() * sizeof(RedTask));
+        pc.n_red = (int)h.red.size();
         pc.d_s1 = (Seg *)put(h.s1.data(), h.s1.size() * sizeof(Seg));
         pc.d_s2 = (Seg *)put(h.s2.data(), h.s2.size() * sizeof(Seg));
         k->chunks.push_back(pc);

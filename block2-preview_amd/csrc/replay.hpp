@@ -14,11 +14,18 @@ void grad_plan_destroy(GradPlan *g);
 struct PkPlan; // pkplan.hip
 void pk_destroy(PkPlan *k);
 
+// one phase-2 output block (or window) and its segments [sb, se)
+struct BlockJob {
+    uint64_t d_off;
+    uint32_t ldd, M, N, sb, se, beta;
+};
+
 struct Chunk {
     // host images; tasks are bucketed by tile configuration
     std::vector<Task> t1[kTileConfigs], t2[kTileConfigs];
     std::vector<Seg> s1, s2;
     std::vector<RedTask> red; // split-K reductions of phase 2
+    std::vector<BlockJob> jobs2; // phase-2 blocks (re-tiled by the packed plan)
     uint64_t w_len = 0;       // W elements per right-hand side
     uint64_t p_len = 0;       // partial-tile elements per right-hand side
     // device copies (inside plan->dev)
@@ -78,7 +85,7 @@ struct Tile {
 };
 // tiling of an M x N output block for tasks of depth K (measured tile
 // rates, or B2G_TILE)
-std::vector<Tile> tile_cover(uint32_t M, uint32_t N, uint64_t K);
+std::vector<Tile> tile_cover(uint32_t M, uint32_t N, uint64_t K, int table);
 Task make_task(uint64_t d_off, uint32_t ldd, uint32_t M, uint32_t N,
                const Tile &t, uint32_t sb, uint32_t se, uint32_t beta);
 int cfg_of(const Tile &t);
@@ -86,6 +93,12 @@ int cfg_of(const Tile &t);
 void add_tiles(std::vector<Task> *tasks, uint64_t d_off, uint32_t M,
                uint32_t N, uint64_t K, uint32_t sb, uint32_t se,
                uint32_t beta);
+// phase-2 tiles of one block job into t2 (tile rates `table`: 0 FP64 /
+// FP32 kernels, 1 split-BF16 kernels); deep segment lists are cut into
+// split-K groups with partial slots appended at p_len and a reduction task
+void add_split_tiles(const std::vector<Seg> &s2, const BlockJob &j, int table,
+                     std::vector<Task> *t2, std::vector<RedTask> &red,
+                     uint64_t &p_len);
 double task_flops(const std::vector<Task> &tasks, const std::vector<Seg> &segs,
                   int cfg);
 void sort_tasks(std::vector<Task> &tasks, const std::vector<Seg> &segs);
@@ -104,9 +117,14 @@ int plan_reserve(b2g_plan p, int n_rhs, int precision);
 // left-operator blocks and packed W^T and keeps the FP64 tasks, split-K
 // reductions and partial buffer of the plan.
 struct PkChunk {
-    // the plan's tasks in XCD-grouped order; phase-1 tasks address packed W^T
+    // tasks tiled with the split-BF16 tile rates, XCD-grouped; phase-1 tasks
+    // address packed W^T; phase-2 split-K partials and reductions of its own
     Task *d_t1[kTileConfigs] = {}, *d_t2[kTileConfigs] = {};
     Seg *d_s1 = nullptr, *d_s2 = nullptr;
+    RedTask *d_red = nullptr;
+    int n_red = 0;
+    size_t n1[kTileConfigs] = {}, n2[kTileConfigs] = {};
+    double f1[kTileConfigs] = {}, f2[kTileConfigs] = {};
     XcdSplit x1[kTileConfigs] = {}, x2[kTileConfigs] = {};
 };
 struct PkPlan {
@@ -118,6 +136,7 @@ struct PkPlan {
     int n_ltiles = 0, n_rtiles = 0, n_xtiles = 0;
     uint64_t x_len = 0; // packed wavefunction elements per right-hand side
     uint64_t w_max = 0; // packed W^T elements per right-hand side
+    uint64_t p_max = 0; // split-K partial elements per right-hand side
     // packed wavefunction and W^T planes for scratch_rhs right-hand sides:
     // x hi, x lo (x_len * rhs each), W hi, W lo (w_max * rhs each)
     uint16_t *scratch = nullptr;

@@ -244,7 +244,12 @@ static uint64_t tile_cost(const Task &t, const std::vector<Seg> &segs) {
 // estimated time  sum(TM * TN * K / rate(TM, TN))  with measured per-config
 // rates (b2g_set_tile_rates, filled by the autotuner).  B2G_TILE=32|64|128
 // forces square tiles of one size.
-static double g_rates[kTileConfigs] = {30, 34, 36, 34, 42, 46, 36, 46, 50};
+// Two tables: [0] tiles of the FP64 / FP32 kernels, [1] tiles of the
+// split-BF16 kernels (packed plan), whose large tiles gain more over the
+// small ones.
+static double g_rates[2][kTileConfigs] = {
+    {30, 34, 36, 34, 42, 46, 36, 46, 50},
+    {8, 25, 30, 30, 55, 85, 40, 95, 165}};
 
 static int size_index(uint32_t t) { return t == 32 ? 0 : t == 64 ? 1 : 2; }
 
@@ -273,7 +278,8 @@ static std::vector<std::vector<uint32_t>> covers(uint32_t d) {
 }
 
 // tiles covering an M x N output block whose tasks have depth K
-std::vector<Tile> tile_cover(uint32_t M, uint32_t N, uint64_t K) {
+std::vector<Tile> tile_cover(uint32_t M, uint32_t N, uint64_t K, int table) {
+    const double *rates = g_rates[table == 1 ? 1 : 0];
     const char *e = getenv("B2G_TILE");
     std::vector<uint32_t> rows, cols;
     uint32_t fm = 0, fn = 0;
@@ -298,7 +304,7 @@ std::vector<Tile> tile_cover(uint32_t M, uint32_t N, uint64_t K) {
                 for (uint32_t a : rc)
                     for (uint32_t b : cc)
                         c += (double)a * b * (double)(K + 1) /
-                             g_rates[3 * size_index(a) + size_index(b)];
+                             rates[3 * size_index(a) + size_index(b)];
                 if (c < best)
                     best = c, rows = rc, cols = cc;
             }
@@ -339,7 +345,7 @@ int cfg_of(const Tile &t) {
 void add_tiles(std::vector<Task> *tasks, uint64_t d_off, uint32_t M,
                       uint32_t N, uint64_t K, uint32_t sb, uint32_t se,
                       uint32_t beta) {
-    for (const Tile &t : tile_cover(M, N, K))
+    for (const Tile &t : tile_cover(M, N, K, 0))
         tasks[cfg_of(t)].push_back(make_task(d_off, N, M, N, t, sb, se, beta));
 }
 
@@ -356,56 +362,62 @@ static uint64_t ksplit() {
 // ksplit(); each (tile, group) becomes a partial task writing a dense
 // TM x TN slot of the partial buffer, and a reduction task sums the slots in
 // group order (deterministic) into the output.
-static void add_split_tiles(Chunk &ch, uint64_t d_off, uint32_t M, uint32_t N,
-                            uint32_t ldd, uint32_t sb, uint32_t se,
-                            uint32_t beta) {
+void add_split_tiles(const std::vector<Seg> &s2, const BlockJob &j, int table,
+                     std::vector<Task> *t2, std::vector<RedTask> &red,
+                     uint64_t &p_len) {
     std::vector<std::pair<uint32_t, uint32_t>> groups;
     const uint64_t kmax = ksplit();
+    // depth as the kernels walk it: every segment takes whole k-steps
+    // (16 deep for the FP64 / FP32 kernels, 32 for the split-BF16 ones), so
+    // that a long list of shallow segments is split like a deep one
+    const uint64_t step = table == 1 ? 32 : 16;
     uint64_t K = 0, acc = 0;
-    uint32_t g0 = sb;
-    for (uint32_t k = sb; k < se; k++) {
-        K += ch.s2[k].K;
-        acc += ch.s2[k].K;
-        if (acc >= kmax && k + 1 < se) {
+    uint32_t g0 = j.sb;
+    for (uint32_t k = j.sb; k < j.se; k++) {
+        const uint64_t kk = (s2[k].K + step - 1) / step * step;
+        K += kk;
+        acc += kk;
+        if (acc >= kmax && k + 1 < j.se) {
             groups.emplace_back(g0, k + 1);
             g0 = k + 1, acc = 0;
         }
     }
-    groups.emplace_back(g0, se);
-    const std::vector<Tile> tiles = tile_cover(M, N, K / groups.size());
+    groups.emplace_back(g0, j.se);
+    const std::vector<Tile> tiles =
+        tile_cover(j.M, j.N, K / groups.size(), table);
     if (groups.size() == 1) {
         for (const Tile &t : tiles)
-            ch.t2[cfg_of(t)].push_back(make_task(d_off, ldd, M, N, t, sb, se,
-                                                 beta));
+            t2[cfg_of(t)].push_back(
+                make_task(j.d_off, j.ldd, j.M, j.N, t, j.sb, j.se, j.beta));
         return;
     }
     for (const Tile &t : tiles) {
         const uint64_t tsz = (uint64_t)t.tm * t.tn;
-        const uint64_t p0 = ch.p_len;
+        const uint64_t p0 = p_len;
         for (size_t g = 0; g < groups.size(); g++) {
             // slot of group g: element (row, col) of the block lands at
             // p0 + g*tsz + (row - row0)*tn + (col - col0)  (mod 2^64)
             const uint64_t doff = p0 + g * tsz - (uint64_t)t.row0 * t.tn -
                                   (uint64_t)t.col0;
-            ch.t2[cfg_of(t)].push_back(make_task(doff, t.tn, M, N, t,
-                                                 groups[g].first,
-                                                 groups[g].second, 2u));
+            t2[cfg_of(t)].push_back(make_task(doff, t.tn, j.M, j.N, t,
+                                              groups[g].first,
+                                              groups[g].second, 2u));
         }
         RedTask rt;
-        rt.d_off = d_off;
+        rt.d_off = j.d_off;
         rt.p_off = p0;
-        rt.ldd = ldd;
-        rt.M = M;
-        rt.N = N;
+        rt.ldd = j.ldd;
+        rt.M = j.M;
+        rt.N = j.N;
         rt.row0 = t.row0;
         rt.col0 = t.col0;
         rt.tm = t.tm;
         rt.tn = t.tn;
         rt.nparts = (uint32_t)groups.size();
-        rt.beta = beta;
+        rt.beta = j.beta;
         rt.pad = 0;
-        ch.red.push_back(rt);
-        ch.p_len += groups.size() * tsz;
+        red.push_back(rt);
+        p_len += groups.size() * tsz;
     }
 }
 
@@ -700,8 +712,10 @@ static int compile_plan(b2g_plan p, const b2g_pair *pairs, size_t n_pairs,
                         s.alpha = q.alpha;
                         ch.s2.push_back(s);
                     }
-                add_split_tiles(ch, off, m, n, ldd, sb, (uint32_t)ch.s2.size(),
-                                beta);
+                const BlockJob job{off, ldd, m, n, sb,
+                                   (uint32_t)ch.s2.size(), beta};
+                ch.jobs2.push_back(job);
+                add_split_tiles(ch.s2, job, 0, ch.t2, ch.red, ch.p_len);
             };
             if (first) {
                 for (size_t b = 0; b < n_yb; b++) {
@@ -821,14 +835,23 @@ extern "C" int b2g_plan_destroy(b2g_plan p) {
     return B2G_OK;
 }
 
-extern "C" int b2g_set_tile_rates(const double *rates, int n) {
+extern "C" int b2g_set_tile_rates_for(int precision, const double *rates,
+                                      int n) {
     B2G_REQUIRE(rates != nullptr && n == kTileConfigs, B2G_ERR_ARG,
                 "b2g_set_tile_rates: expects 9 rates");
+    B2G_REQUIRE(precision == B2G_PREC_F64 || precision == B2G_PREC_F32 ||
+                    precision == B2G_PREC_BF16X3,
+                B2G_ERR_ARG, "unsupported precision");
     for (int i = 0; i < n; i++)
         B2G_REQUIRE(rates[i] > 0, B2G_ERR_ARG, "rates must be positive");
+    const int t = precision == B2G_PREC_BF16X3 ? 1 : 0;
     for (int i = 0; i < n; i++)
-        g_rates[i] = rates[i];
+        g_rates[t][i] = rates[i];
     return B2G_OK;
+}
+
+extern "C" int b2g_set_tile_rates(const double *rates, int n) {
+    return b2g_set_tile_rates_for(B2G_PREC_F64, rates, n);
 }
 
 extern "C" int b2g_plan_set_timing(b2g_plan p, int enable) {
@@ -918,7 +941,9 @@ int plan_reserve(b2g_plan p, int n_rhs, int precision) {
         p->w_bytes = need;
         p->scratch_gen++;
     }
-    const size_t pneed = std::max<uint64_t>(p->p_max, 1) * n_rhs * 8;
+    const uint64_t pmax =
+        std::max<uint64_t>(p->p_max, p->pk != nullptr ? p->pk->p_max : 0);
+    const size_t pneed = std::max<uint64_t>(pmax, 1) * n_rhs * 8;
     if (pneed > p->p_bytes) {
         if (p->pbuf) {
             B2G_HIP(hipStreamSynchronize(ctx->stream));
@@ -993,11 +1018,11 @@ extern "C" int b2g_hpsi(b2g_ctx ctx, b2g_plan p, const double *x, double *y,
     // largest runs on the context stream, the others on the auxiliary
     // streams (forked from and joined back to the context stream), where
     // they fill the large launch's idle compute units.
-    auto phase = [&](const std::vector<Task> *tl, const double *fl, int kbase,
+    auto phase = [&](const size_t *cnt, const double *fl, int kbase,
                      auto &&launch) -> hipError_t {
         int order[kTileConfigs], n = 0;
         for (int c = 0; c < kTileConfigs; c++)
-            if (!tl[c].empty())
+            if (cnt[c] != 0)
                 order[n++] = c;
         std::sort(order, order + n,
                   [fl](int a, int b) { return fl[a] > fl[b]; });
@@ -1027,6 +1052,8 @@ extern "C" int b2g_hpsi(b2g_ctx ctx, b2g_plan p, const double *x, double *y,
         return e;
     };
     const PkPlan *pk = precision == B2G_PREC_BF16X3 ? p->pk : nullptr;
+    // split-K partials per right-hand side of the packed plan's tiling
+    const uint64_t pstride = pk != nullptr ? pk->p_max : 0;
     typedef const __bf16 *cbf;
     if (pk != nullptr)
         B2G_HIP(timed(B2G_K_PACK, ctx->stream, [&](hipStream_t st) {
@@ -1040,34 +1067,36 @@ extern "C" int b2g_hpsi(b2g_ctx ctx, b2g_plan p, const double *x, double *y,
         const int beta0 = ic == 0 ? (accumulate ? 1 : 0) : 1;
         if (pk != nullptr) {
             const PkChunk &kc = pk->chunks[ic];
-            B2G_HIP(phase(ch.t1, ch.f1, B2G_K_PHASE1,
+            B2G_HIP(phase(kc.n1, kc.f1, B2G_K_PHASE1,
                           [&](int c, hipStream_t st) {
                 return launch_seg_gemm_pk<0>(
                     c, (cbf)pk->xh(), (cbf)pk->xl(), (cbf)pk->rh(),
                     (cbf)pk->rl(), pk->wh(), pk->wl(), kc.d_t1[c],
-                    (int)ch.t1[c].size(), kc.d_s1, pk->x_len, 0, pk->w_max,
-                    0, 1.0, 0, n_rhs, grid, st, kc.x1[c]);
+                    (int)kc.n1[c], kc.d_s1, pk->x_len, 0, pk->w_max, 0, 1.0,
+                    0, n_rhs, grid, st, kc.x1[c]);
             }));
-            B2G_HIP(phase(ch.t2, ch.f2, B2G_K_PHASE2,
+            B2G_HIP(phase(kc.n2, kc.f2, B2G_K_PHASE2,
                           [&](int c, hipStream_t st) {
                 return launch_seg_gemm_pk<1>(
                     c, (cbf)pk->lh(), (cbf)pk->ll(), (cbf)pk->wh(),
-                    (cbf)pk->wl(), y, P, kc.d_t2[c], (int)ch.t2[c].size(),
-                    kc.d_s2, 0, pk->w_max, p->y_len, p->p_max, scale, beta0,
-                    n_rhs, grid, st, kc.x2[c]);
+                    (cbf)pk->wl(), y, P, kc.d_t2[c], (int)kc.n2[c], kc.d_s2,
+                    0, pk->w_max, p->y_len, pstride, scale, beta0, n_rhs,
+                    grid, st, kc.x2[c]);
             }));
-            const int nr = (int)ch.red.size();
-            if (nr > 0)
+            if (kc.n_red > 0)
                 B2G_HIP(timed(B2G_K_SPLITK_REDUCE, ctx->stream,
                               [&](hipStream_t st) {
                     return launch_splitk_reduce<double>(
-                        y, P, ch.d_red, nr, p->y_len, p->p_max, beta0, n_rhs,
-                        grid, st);
+                        y, P, kc.d_red, kc.n_red, p->y_len, pstride, beta0,
+                        n_rhs, grid, st);
                 }));
             continue;
         }
         // phase 1, phase 2, then the split-K reduction
-        B2G_HIP(phase(ch.t1, ch.f1, B2G_K_PHASE1, [&](int c, hipStream_t st) {
+        size_t n1[kTileConfigs], n2[kTileConfigs];
+        for (int c = 0; c < kTileConfigs; c++)
+            n1[c] = ch.t1[c].size(), n2[c] = ch.t2[c].size();
+        B2G_HIP(phase(n1, ch.f1, B2G_K_PHASE1, [&](int c, hipStream_t st) {
             const int nt = (int)ch.t1[c].size();
             if (precision == B2G_PREC_F64)
                 return launch_seg_gemm<double, double, double, double, 1>(
@@ -1078,7 +1107,7 @@ extern "C" int b2g_hpsi(b2g_ctx ctx, b2g_plan p, const double *x, double *y,
                 c, x, p->r32, (float *)p->w, (float *)p->w, ch.d_t1[c], nt,
                 ch.d_s1, p->x_len, 0, p->w_max, 0, 1.0, 0, n_rhs, grid, st, ch.x1[c]);
         }));
-        B2G_HIP(phase(ch.t2, ch.f2, B2G_K_PHASE2, [&](int c, hipStream_t st) {
+        B2G_HIP(phase(n2, ch.f2, B2G_K_PHASE2, [&](int c, hipStream_t st) {
             const int nt = (int)ch.t2[c].size();
             if (precision == B2G_PREC_F64)
                 return launch_seg_gemm<double, double, double, double, 2>(

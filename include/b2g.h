@@ -196,6 +196,10 @@ int b2g_plan_kernel_flops(b2g_plan plan, double *flops);
  * estimated time with these rates (the autotuner measures them).  Process
  * wide.                                                                   */
 int b2g_set_tile_rates(const double *rates, int n);
+/* The same for the kernels of one precision: B2G_PREC_F64 and B2G_PREC_F32
+ * share the table above; B2G_PREC_BF16X3 has its own, with which the packed
+ * split-BF16 plan is tiled when it is built.                              */
+int b2g_set_tile_rates_for(int precision, const double *rates, int n);
 /* Number of kernel tasks per phase and scratch size, for reporting.      */
 int b2g_plan_stats(b2g_plan plan, uint64_t *n_tasks1, uint64_t *n_tasks2,
                    uint64_t *scratch_elems);
