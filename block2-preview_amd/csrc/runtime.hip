@@ -245,11 +245,15 @@ static uint64_t tile_cost(const Task &t, const std::vector<Seg> &segs) {
 // rates (b2g_set_tile_rates, filled by the autotuner).  B2G_TILE=32|64|128
 // forces square tiles of one size.
 // Two tables: [0] tiles of the FP64 / FP32 kernels, [1] tiles of the
-// split-BF16 kernels (packed plan), whose large tiles gain more over the
-// small ones.
+// split-BF16 kernels (packed plan).  The split-BF16 default is the
+// per-shape throughput those kernels reach on a heavy-tailed sector
+// histogram (n=40, M=3000 SU(2) middle site, one right-hand side), where the
+// small tiles, fed by shallow segments, run far below their rate on uniform
+// blocks: 4.40 ms per product against 5.1 ms with rates measured on uniform
+// 436 x 436 blocks.
 static double g_rates[2][kTileConfigs] = {
     {30, 34, 36, 34, 42, 46, 36, 46, 50},
-    {8, 25, 30, 30, 55, 85, 40, 95, 165}};
+    {4, 15, 25, 20, 45, 80, 30, 90, 200}};
 
 static int size_index(uint32_t t) { return t == 32 ? 0 : t == 64 ? 1 : 2; }
 
