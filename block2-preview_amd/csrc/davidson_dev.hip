@@ -46,14 +46,9 @@ __global__ void put_sigma_kernel(DavState *st, double *__restrict__ S,
         dst[i] = sn[i];
 }
 
-// Partial Gram sums <B_i|S_j>, j <= i < m (the lower triangle, pair
-// p = i(i+1)/2 + j), per block over row chunks of rc rows.  A chunk's rows of
-// every B_i and S_j are staged in LDS (row stride rc + 1 doubles); the pairs
-// are spread over the 256 threads, tpp threads per pair each summing every
-// tpp-th row, or, past 256 pairs, one thread per pair slot (up to
-// kPairSlots slots).  Per block: partial[blockIdx * kPairStride + p].
+// Lower-triangle pairs j <= i < m of the Gram matrix: p = i(i+1)/2 + j;
+// per-block partial sums are kPairStride apart.
 constexpr int kPairStride = 64 * 65 / 2;
-constexpr int kPairSlots = (kPairStride + 255) / 256;
 
 __device__ __forceinline__ void pair_ij(int p, int &i, int &j) {
     i = (int)((sqrt(8.0 * p + 1.0) - 1.0) * 0.5);
@@ -64,67 +59,6 @@ __device__ __forceinline__ void pair_ij(int p, int &i, int &j) {
     j = p - i * (i + 1) / 2;
 }
 
-__global__ __launch_bounds__(256) void gram_kernel(
-    const DavState *st, const double *__restrict__ X,
-    const double *__restrict__ Y, size_t n, double *__restrict__ partial,
-    int rc) {
-    extern __shared__ double dyn[];
-    const int m = st->m, npair = m * (m + 1) / 2, ld = rc + 1;
-    double *xs = dyn, *ys = dyn + (size_t)m * ld;
-    const int tid = threadIdx.x;
-    const int tpp = npair <= 256 ? 256 / npair : 1;
-    const int phase = npair <= 256 ? tid % tpp : 0;
-    int pi[kPairSlots], pj[kPairSlots];
-    double acc[kPairSlots];
-#pragma unroll
-    for (int q = 0; q < kPairSlots; q++) {
-        const int p = npair <= 256 ? (q == 0 ? tid / tpp : npair)
-                                   : tid + 256 * q;
-        pi[q] = -1, pj[q] = 0, acc[q] = 0.0;
-        if (p < npair)
-            pair_ij(p, pi[q], pj[q]);
-    }
-    for (size_t r0 = (size_t)blockIdx.x * rc; r0 < n;
-         r0 += (size_t)gridDim.x * rc) {
-        const int rows = (int)(n - r0 < (size_t)rc ? n - r0 : (size_t)rc);
-        for (int e = tid; e < m * rc; e += 256) {
-            const int i = e / rc, r = e % rc;
-            const bool ok = r < rows;
-            xs[i * ld + r] = ok ? X[(size_t)i * n + r0 + r] : 0.0;
-            ys[i * ld + r] = ok ? Y[(size_t)i * n + r0 + r] : 0.0;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int q = 0; q < kPairSlots; q++) {
-            if (pi[q] >= 0) {
-                const double *a = xs + pi[q] * ld, *b = ys + pj[q] * ld;
-                double sum = 0.0;
-                for (int r = phase; r < rows; r += tpp)
-                    sum += a[r] * b[r];
-                acc[q] += sum;
-            }
-        }
-        __syncthreads();
-    }
-    // combine the tpp phases of each pair (fixed order)
-    double *red = dyn;
-    if (npair <= 256) {
-        red[tid] = pi[0] >= 0 ? acc[0] : 0.0;
-        __syncthreads();
-        if (tid < npair) {
-            double sum = 0.0;
-            for (int k = 0; k < tpp; k++)
-                sum += red[tid * tpp + k];
-            partial[(size_t)blockIdx.x * kPairStride + tid] = sum;
-        }
-    } else {
-#pragma unroll
-        for (int q = 0; q < kPairSlots; q++)
-            if (pi[q] >= 0)
-                partial[(size_t)blockIdx.x * kPairStride + tid + 256 * q] =
-                    acc[q];
-    }
-}
 // one block per pair: G[i][j] = G[j][i] = sum over blocks (fixed order)
 __global__ __launch_bounds__(256) void gram_reduce_kernel(
     DavState *st, const double *__restrict__ partial, int nblk) {
@@ -250,38 +184,132 @@ __global__ __launch_bounds__(256) void ritz_kernel(DavState *st) {
 }
 
 // X_j <- sum_i V[j*m + i] X_i, j < m (in place, rows of n)
-// X_j <- sum_i V[j*m+i] X_i (j < m) for X = S and X = B, one row per thread;
-// the row's m values sit in registers (MAXM >= m, chosen from the subspace
-// capacity when the step is captured).
-template <int MAXM>
-__global__ __launch_bounds__(256) void rotate_dev_kernel(
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+// FP64 MFMA 16x16x4: lane l supplies A[l%16][l/16] and B[l/16][l%16];
+// accumulator element r of lane l is D[4r + l/16][l%16].
+__device__ __forceinline__ v4d mfma64(double a, double b, v4d c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// Partial Gram sums <B_i|S_j>, j <= i < m, on FP64 MFMA: wave w of the grid
+// takes a contiguous slab of rows and accumulates the 16 x 16 tiles
+// (ti, tj), tj <= ti < MT, with K running along the rows (lane l supplies
+// B_{16ti + l%16} and S_{16tj + l%16} at row r + l/16).  The four waves of a
+// block are summed in LDS; per block: partial[blockIdx * kPairStride + p],
+// p = i(i+1)/2 + j.
+template <int MT>
+__global__ __launch_bounds__(256) void gram_mfma_kernel(
+    const DavState *st, const double *__restrict__ X,
+    const double *__restrict__ Y, size_t n, double *__restrict__ partial) {
+    constexpr int NP = 16 * MT * (16 * MT + 1) / 2;
+    __shared__ double red[4][NP];
+    const int m = st->m, npair = m * (m + 1) / 2;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    v4d acc[MT][MT];
+#pragma unroll
+    for (int a = 0; a < MT; a++)
+#pragma unroll
+        for (int b = 0; b < MT; b++)
+            acc[a][b] = v4d{0, 0, 0, 0};
+    const size_t nw = (size_t)gridDim.x * 4, gw = (size_t)blockIdx.x * 4 + wave;
+    const size_t slab = (((n + nw - 1) / nw) + 3) & ~(size_t)3;
+    const size_t rbeg = gw * slab;
+    const size_t rend = rbeg + slab < n ? rbeg + slab : n;
+#pragma unroll 4
+    for (size_t r = rbeg; r < rend; r += 4) {
+        const size_t rr = r + lk;
+        const bool ok = rr < rend;
+        double xa[MT], yb[MT];
+#pragma unroll
+        for (int t = 0; t < MT; t++) {
+            const int i = 16 * t + li;
+            xa[t] = (ok && i < m) ? X[(size_t)i * n + rr] : 0.0;
+            yb[t] = (ok && i < m) ? Y[(size_t)i * n + rr] : 0.0;
+        }
+#pragma unroll
+        for (int a = 0; a < MT; a++)
+#pragma unroll
+            for (int b = 0; b <= a; b++)
+                if (16 * a < m) // uniform: tiles past the subspace skipped
+                    acc[a][b] = mfma64(xa[a], yb[b], acc[a][b]);
+    }
+    mfma_settle<v4d, MT * MT>(&acc[0][0]);
+    for (int p = tid; p < 4 * NP; p += 256)
+        (&red[0][0])[p] = 0.0;
+    __syncthreads();
+#pragma unroll
+    for (int a = 0; a < MT; a++)
+#pragma unroll
+        for (int b = 0; b <= a; b++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int i = 16 * a + 4 * q + lk, j = 16 * b + li;
+                if (i < m && j <= i)
+                    red[wave][i * (i + 1) / 2 + j] = acc[a][b][q];
+            }
+    __syncthreads();
+    for (int p = tid; p < npair; p += 256)
+        partial[(size_t)blockIdx.x * kPairStride + p] =
+            ((red[0][p] + red[1][p]) + red[2][p]) + red[3][p];
+}
+
+// X <- V X (X_j = sum_i V[j*m+i] X_i, j < m) for X = S and X = B on FP64
+// MFMA: one wave per 16-row chunk, A = V (zero-padded to 16MT x 16MT, in
+// LDS), B = the chunk's m x 16 values (registers); every value of the chunk
+// is read before any is written, so the rotation is in place.
+template <int MT>
+__global__ __launch_bounds__(256) void rotate_mfma_kernel(
     const DavState *st, double *__restrict__ S, double *__restrict__ B,
     size_t n) {
-    __shared__ double vs[MAXM * MAXM];
+    constexpr int MAXM = 16 * MT, KT = 4 * MT;
+    __shared__ double vs[MAXM][MAXM + 1];
     const int m = st->m;
-    for (int e = threadIdx.x; e < m * m; e += 256)
-        vs[e] = st->V[e];
+    for (int e = threadIdx.x; e < MAXM * MAXM; e += 256) {
+        const int j = e / MAXM, i = e % MAXM;
+        vs[j][i] = (j < m && i < m) ? st->V[j * m + i] : 0.0;
+    }
     __syncthreads();
-    for (size_t r = blockIdx.x * (size_t)256 + threadIdx.x; r < n;
-         r += (size_t)gridDim.x * 256) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const size_t nchunk = (n + 15) / 16;
+    for (size_t c = (size_t)blockIdx.x * 4 + wave; c < nchunk;
+         c += (size_t)gridDim.x * 4) {
+        const size_t r = c * 16 + li;
+        const bool rok = r < n;
 #pragma unroll
         for (int pass = 0; pass < 2; pass++) {
             double *X = pass == 0 ? S : B;
-            double x[MAXM];
+            double b[KT];
 #pragma unroll
-            for (int i = 0; i < MAXM; i++)
-                x[i] = i < m ? X[(size_t)i * n + r] : 0.0;
-            for (int j = 0; j < m; j++) {
-                double s = 0.0;
-#pragma unroll
-                for (int i = 0; i < MAXM; i++)
-                    if (i < m)
-                        s += vs[j * m + i] * x[i];
-                X[(size_t)j * n + r] = s;
+            for (int kk = 0; kk < KT; kk++) {
+                const int k = 4 * kk + lk;
+                b[kk] = (k < m && rok) ? X[(size_t)k * n + r] : 0.0;
             }
+            v4d acc[MT];
+#pragma unroll
+            for (int t = 0; t < MT; t++) {
+                acc[t] = v4d{0, 0, 0, 0};
+#pragma unroll
+                for (int kk = 0; kk < KT; kk++)
+                    if (16 * t < m && 4 * kk < m) // uniform
+                        acc[t] = mfma64(vs[16 * t + li][4 * kk + lk], b[kk],
+                                        acc[t]);
+            }
+            mfma_settle<v4d, MT>(&acc[0]);
+#pragma unroll
+            for (int t = 0; t < MT; t++)
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const int j = 16 * t + 4 * q + lk;
+                    if (j < m && rok)
+                        X[(size_t)j * n + r] = acc[t][q];
+                }
         }
     }
 }
+
 __global__ __launch_bounds__(256) void residual_kernel(
     const DavState *st, const double *__restrict__ B,
     const double *__restrict__ S, double *__restrict__ q, size_t n,
@@ -489,28 +517,21 @@ hipError_t launch_davidson_step(DavState *st, double *B, double *S,
         return hipErrorInvalidValue;
     hipLaunchKernelGGL(put_sigma_kernel, vgrid(n), dim3(256), 0, s, st, S, sn,
                        n);
-    // Gram row chunk: both staging arrays within 64 KB of LDS
-    int rc = 256;
-    while (rc > 16 && 2 * nv * (rc + 1) * 8 > 65536)
-        rc /= 2;
-    hipLaunchKernelGGL(gram_kernel, dim3(kGrid), dim3(256),
-                       (size_t)2 * nv * (rc + 1) * sizeof(double), s, st, B, S,
-                       n, partial, rc);
+    const int mt = (nv + 15) / 16;
+#define B2G_MT(K, ...)                                                         \
+    switch (mt) {                                                              \
+    case 1: hipLaunchKernelGGL(K<1>, __VA_ARGS__); break;                      \
+    case 2: hipLaunchKernelGGL(K<2>, __VA_ARGS__); break;                      \
+    case 3: hipLaunchKernelGGL(K<3>, __VA_ARGS__); break;                      \
+    default: hipLaunchKernelGGL(K<4>, __VA_ARGS__); break;                     \
+    }
+    B2G_MT(gram_mfma_kernel, dim3(kGrid), dim3(256), 0, s, st, B, S, n,
+           partial);
     hipLaunchKernelGGL(gram_reduce_kernel, dim3(kPairStride), dim3(256), 0, s,
                        st, partial, kGrid);
     hipLaunchKernelGGL(ritz_kernel, dim3(1), dim3(256), 0, s, st);
-    if (nv <= 8)
-        hipLaunchKernelGGL(rotate_dev_kernel<8>, vgrid(n), dim3(256), 0, s, st,
-                           S, B, n);
-    else if (nv <= 16)
-        hipLaunchKernelGGL(rotate_dev_kernel<16>, vgrid(n), dim3(256), 0, s,
-                           st, S, B, n);
-    else if (nv <= 32)
-        hipLaunchKernelGGL(rotate_dev_kernel<32>, vgrid(n), dim3(256), 0, s,
-                           st, S, B, n);
-    else
-        hipLaunchKernelGGL(rotate_dev_kernel<64>, vgrid(n), dim3(256), 0, s,
-                           st, S, B, n);
+    B2G_MT(rotate_mfma_kernel, vgrid(n), dim3(256), 0, s, st, S, B, n);
+#undef B2G_MT
     hipLaunchKernelGGL(residual_kernel, dim3(kGrid), dim3(256), 0, s, st, B,
                        S, q, n, partial);
     hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(256), 0, s, partial, kGrid,

@@ -17,25 +17,30 @@ namespace b2g {
 // accumulator as an in/out operand, so that it follows all of their MFMAs
 // and all later uses of them follow it.
 #define B2G_SETTLE_NOPS "s_nop 7\n\ts_nop 7\n\ts_nop 7"
+// (counts that are not a power of two are settled in power-of-two groups,
+// each followed by its own wait states)
 template <typename V, int N> __device__ __forceinline__ void mfma_settle(V *x) {
-    if constexpr (N == 1) {
-        asm volatile(B2G_SETTLE_NOPS : "+v"(x[0]));
-    } else if constexpr (N == 2) {
-        asm volatile(B2G_SETTLE_NOPS : "+v"(x[0]), "+v"(x[1]));
-    } else if constexpr (N == 4) {
-        asm volatile(B2G_SETTLE_NOPS
-                     : "+v"(x[0]), "+v"(x[1]), "+v"(x[2]), "+v"(x[3]));
-    } else if constexpr (N == 8) {
-        asm volatile(B2G_SETTLE_NOPS
-                     : "+v"(x[0]), "+v"(x[1]), "+v"(x[2]), "+v"(x[3]),
-                       "+v"(x[4]), "+v"(x[5]), "+v"(x[6]), "+v"(x[7]));
-    } else {
-        static_assert(N == 16, "accumulator count");
+    if constexpr (N >= 16) {
         asm volatile(B2G_SETTLE_NOPS
                      : "+v"(x[0]), "+v"(x[1]), "+v"(x[2]), "+v"(x[3]),
                        "+v"(x[4]), "+v"(x[5]), "+v"(x[6]), "+v"(x[7]),
                        "+v"(x[8]), "+v"(x[9]), "+v"(x[10]), "+v"(x[11]),
                        "+v"(x[12]), "+v"(x[13]), "+v"(x[14]), "+v"(x[15]));
+        mfma_settle<V, N - 16>(x + 16);
+    } else if constexpr (N >= 8) {
+        asm volatile(B2G_SETTLE_NOPS
+                     : "+v"(x[0]), "+v"(x[1]), "+v"(x[2]), "+v"(x[3]),
+                       "+v"(x[4]), "+v"(x[5]), "+v"(x[6]), "+v"(x[7]));
+        mfma_settle<V, N - 8>(x + 8);
+    } else if constexpr (N >= 4) {
+        asm volatile(B2G_SETTLE_NOPS
+                     : "+v"(x[0]), "+v"(x[1]), "+v"(x[2]), "+v"(x[3]));
+        mfma_settle<V, N - 4>(x + 4);
+    } else if constexpr (N >= 2) {
+        asm volatile(B2G_SETTLE_NOPS : "+v"(x[0]), "+v"(x[1]));
+        mfma_settle<V, N - 2>(x + 2);
+    } else if constexpr (N == 1) {
+        asm volatile(B2G_SETTLE_NOPS : "+v"(x[0]));
     }
 }
 
