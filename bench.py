@@ -204,25 +204,27 @@ def main():
     # dominant kernel is the one with the largest summed time, and its
     # algorithmic FLOPs per launch are the plan's per-kernel count (tiles of
     # that launch, all right-hand sides) divided by its launches per step
-    kms, kn = ph
-    kfl = h.kernel_flops()
-    kernels = {}
-    for k in range(L.NKERNELS):
-        if kn[k] == 0:
-            continue
-        per_step = kn[k] // args.steps
-        avg = kms[k] / kn[k]
-        fl = kfl[k] * R / max(per_step, 1)
-        kernels[L.kernel_name(k)] = dict(
-            launches_per_step=per_step, avg_launch_ms=round(avg, 4),
-            tflops=round(fl / (avg * 1e-3) / 1e12, 3) if fl > 0 else None)
-    dom_k = max((k for k in range(L.NKERNELS) if kn[k] and kfl[k] > 0),
-                key=lambda k: kms[k])
-    dom = L.kernel_name(dom_k)
-    dur = kms[dom_k] / kn[dom_k]
-    per_step = kn[dom_k] // args.steps
-    achieved = kfl[dom_k] * R / max(per_step, 1) / (dur * 1e-3) / 1e12
-    t_gemm = sum(kms[k] for k in range(L.NKERNELS)) / args.steps
+    def kernel_table(ph, kfl):
+        kms, kn = ph
+        table = {}
+        for k in range(L.NKERNELS):
+            if kn[k] == 0:
+                continue
+            per_step = kn[k] // args.steps
+            avg = kms[k] / kn[k]
+            fl = kfl[k] * R / max(per_step, 1)
+            table[L.kernel_name(k)] = dict(
+                launches_per_step=per_step, avg_launch_ms=round(avg, 4),
+                tflops=round(fl / (avg * 1e-3) / 1e12, 3) if fl > 0 else None)
+        dk = max((k for k in range(L.NKERNELS) if kn[k] and kfl[k] > 0),
+                 key=lambda k: kms[k])
+        d_avg = kms[dk] / kn[dk]
+        d_ach = kfl[dk] * R / max(kn[dk] // args.steps, 1) / (d_avg * 1e-3) / 1e12
+        return (table, L.kernel_name(dk), d_avg, kn[dk] // args.steps, d_ach,
+                sum(kms[k] for k in range(L.NKERNELS)) / args.steps)
+
+    kernels, dom, dur, per_step, achieved, t_gemm = kernel_table(
+        ph, h.kernel_flops("f64"))
     log("FP64: %.3f ms/step, %.2f TFLOP/s; launches serialised for the "
         "per-kernel events: %.3f ms/step (%.3f ms in kernels)" % (
             ms64, tflops, ms64_serial, t_gemm))
@@ -235,8 +237,12 @@ def main():
     log("FP32: %.3f ms/step, %.2f TFLOP/s" % (ms32, tflops32))
     msb, _ = run("bf16x3", max(3, args.steps // 2), False)
     tflopsb = 2.0 * h.macs * R / (msb * 1e-3) / 1e12
-    log("split-BF16 x3: %.3f ms/step, %.2f TFLOP/s (FP64-equivalent work)" % (
-        msb, tflopsb))
+    _, phb = run("bf16x3", args.steps, True)
+    kernels_b, dom_b, dur_b, _, ach_b, _ = kernel_table(
+        phb, h.kernel_flops("bf16x3"))
+    log("split-BF16 x3: %.3f ms/step, %.2f TFLOP/s (FP64-equivalent work); "
+        "dominant %s %.3f ms/launch, %.1f TFLOP/s" % (
+            msb, tflopsb, dom_b, dur_b, ach_b))
 
     # fixed-tile baseline (64x64 everywhere) on the same problem
     os.environ["B2G_TILE"] = "64"
@@ -327,7 +333,13 @@ def main():
         "bf16x3": {"value": round(tflopsb, 4), "unit": "TFLOP/s",
                    "ms_per_step": round(msb, 4),
                    "note": "FP64-equivalent FLOPs; 3 BF16 MFMA products "
-                           "per multiply-add, FP32 accumulation"},
+                           "per multiply-add, FP32 accumulation",
+                   "dominant_kernel": {
+                       "kernel": dom_b, "avg_launch_ms": round(dur_b, 4),
+                       "achieved_fp64_equiv_tflops": round(ach_b, 3),
+                       "bf16_mfma_frac": round(3 * ach_b / 2500.0, 4),
+                       "traffic": profiled_traffic("bf16x3_" + dom_b)},
+                   "kernels": kernels_b},
         "cpu_baseline": cpu,
     }
     out.update(extra)

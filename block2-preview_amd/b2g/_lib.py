@@ -127,6 +127,7 @@ _SIGS = {
     "b2g_plan_set_timing": (C.c_int, [_vp, C.c_int]),
     "b2g_plan_timing": (C.c_int, [_vp, f64p, u64p]),
     "b2g_plan_kernel_flops": (C.c_int, [_vp, f64p]),
+    "b2g_plan_kernel_flops_for": (C.c_int, [_vp, C.c_int, f64p]),
     "b2g_effham_plan": (_vp, [_vp]),
     "b2g_set_tile_rates": (C.c_int, [f64p, C.c_int]),
     "b2g_set_tile_rates_for": (C.c_int, [C.c_int, f64p, C.c_int]),

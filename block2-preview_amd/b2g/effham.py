@@ -162,11 +162,13 @@ class EffectiveHamiltonian:
         L.check(self.lib.b2g_plan_timing(self.plan, ms, n))
         return list(ms), list(n)
 
-    def kernel_flops(self):
+    def kernel_flops(self, precision="f64"):
         """Algorithmic FLOPs of one H*psi per right-hand side, per kernel
-        id (list of L.NKERNELS)."""
+        id (list of L.NKERNELS), for the launches of `precision` (the
+        split-BF16 plan is tiled with its own tile rates)."""
         f = (C.c_double * L.NKERNELS)()
-        L.check(self.lib.b2g_plan_kernel_flops(self.plan, f))
+        L.check(self.lib.b2g_plan_kernel_flops_for(self.plan,
+                                                   L.PREC[precision], f))
         return list(f)
 
     def pairs(self):

@@ -63,10 +63,9 @@ extern "C" int b2g_ctx_create(int device, unsigned flags, b2g_ctx *out) {
         set_error("hipStreamCreate failed");
         return B2G_ERR_DEVICE;
     }
-    // B2G_STREAMS: streams per H*psi phase, default 3 (the launches of the
-    // smaller tile shapes fill the compute units that the largest launch
-    // leaves idle in its tail: -8% per product at one right-hand side on the
-    // n=40 M=3000 middle site)
+    // B2G_STREAMS: streams per H*psi phase, default 3 (with one right-hand
+    // side the launches of the smaller tile shapes fill the compute units
+    // that the largest launch leaves idle in its tail)
     int nstreams = 3;
     if (const char *e = getenv("B2G_STREAMS"))
         nstreams = std::max(1, std::min(4, atoi(e)));
@@ -897,6 +896,25 @@ extern "C" int b2g_plan_kernel_flops(b2g_plan p, double *flops) {
     return B2G_OK;
 }
 
+extern "C" int b2g_plan_kernel_flops_for(b2g_plan p, int precision,
+                                         double *flops) {
+    B2G_REQUIRE(p && flops, B2G_ERR_ARG, "NULL arg");
+    if (precision != B2G_PREC_BF16X3)
+        return b2g_plan_kernel_flops(p, flops);
+    B2G_HIP(hipSetDevice(p->ctx->device));
+    const int r = pk_build(p);
+    if (r != B2G_OK)
+        return r;
+    for (int k = 0; k < B2G_NKERNELS; k++)
+        flops[k] = 0;
+    for (const PkChunk &kc : p->pk->chunks)
+        for (int c = 0; c < kTileConfigs; c++) {
+            flops[B2G_K_PHASE1 + c] += kc.f1[c];
+            flops[B2G_K_PHASE2 + c] += kc.f2[c];
+        }
+    return B2G_OK;
+}
+
 extern "C" int b2g_plan_macs(b2g_plan p, uint64_t *out) {
     B2G_REQUIRE(p && out, B2G_ERR_ARG, "NULL arg");
     *out = p->macs;
@@ -1030,10 +1048,13 @@ extern "C" int b2g_hpsi(b2g_ctx ctx, b2g_plan p, const double *x, double *y,
                 order[n++] = c;
         std::sort(order, order + n,
                   [fl](int a, int b) { return fl[a] > fl[b]; });
-        // with per-kernel timing the launches run back to back, so that
-        // each event pair brackets one kernel alone
-        const int naux =
-            p->timing ? 0 : std::min<int>((int)ctx->aux.size(), n - 1);
+        // overlap only for one right-hand side (several fill the GPU by
+        // themselves: -2% with overlap at 4 RHS, +8% at 1 RHS on the n=40
+        // M=3000 middle site); with per-kernel timing the launches run back
+        // to back, so that each event pair brackets one kernel alone
+        const int naux = (p->timing || n_rhs > 1)
+                             ? 0
+                             : std::min<int>((int)ctx->aux.size(), n - 1);
         hipError_t e = hipSuccess;
         if (naux > 0) {
             e = hipEventRecord(ctx->fork, ctx->stream);

@@ -190,6 +190,10 @@ int b2g_plan_timing(b2g_plan plan, double *ms, uint64_t *n_launch);
  * reference's second-GEMM count; phase 1 computes each distinct work matrix
  * once).  flops[B2G_NKERNELS].                                            */
 int b2g_plan_kernel_flops(b2g_plan plan, double *flops);
+/* The same for the launches of one precision: B2G_PREC_BF16X3 counts the
+ * tiles of the packed split-BF16 plan (built here if it is not yet), whose
+ * tiling differs; the others are b2g_plan_kernel_flops.                   */
+int b2g_plan_kernel_flops_for(b2g_plan plan, int precision, double *flops);
 /* Relative throughput (any unit) of the 9 workgroup tile shapes
  * TM x TN, TM, TN in {32, 64, 128}, index 3*i + j with TM = 32 << i,
  * TN = 32 << j.  The tiling of plans uploaded afterwards minimises the
@@ -206,7 +210,9 @@ int b2g_plan_stats(b2g_plan plan, uint64_t *n_tasks1, uint64_t *n_tasks2,
 
 /* y[r] = (accumulate ? y[r] : 0) + scale * H * x[r]  for r < n_rhs.
  * x and y are device arrays of n_rhs stacked vectors (x_len, y_len each),
- * FP64.  precision: B2G_PREC_*.  Enqueued on the context stream.
+ * FP64.  precision: B2G_PREC_*.  Enqueued on the context stream (with one
+ * right-hand side the independent launches of a phase overlap on the
+ * context's auxiliary streams, forked from and joined back to it).
  * Replaces TensorFunctions::operator()(b, c, scale).                     */
 int b2g_hpsi(b2g_ctx ctx, b2g_plan plan, const double *x, double *y,
              int n_rhs, double scale, int accumulate, int precision);
