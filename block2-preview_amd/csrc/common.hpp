@@ -3,7 +3,10 @@
 
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <cstdint>
+#include <cstdio>
+#include <cstdlib>
 #include <string>
 #include <vector>
 
@@ -30,6 +33,29 @@ void set_error(const std::string &msg);
             return code;                                                       \
         }                                                                      \
     } while (0)
+
+// Wall time of the stages of a host-side setup (effective Hamiltonian,
+// plan, packed plan): with B2G_TIME_SETUP set, mark() prints the time since
+// the previous mark to stderr.
+class SetupTimer {
+  public:
+    explicit SetupTimer(const char *what)
+        : what_(what), on_(getenv("B2G_TIME_SETUP") != nullptr),
+          t_(std::chrono::steady_clock::now()) {}
+    void mark(const char *stage) {
+        if (!on_)
+            return;
+        const auto t = std::chrono::steady_clock::now();
+        fprintf(stderr, "b2g setup %s: %s %.2f ms\n", what_, stage,
+                std::chrono::duration<double, std::milli>(t - t_).count());
+        t_ = t;
+    }
+
+  private:
+    const char *what_;
+    bool on_;
+    std::chrono::steady_clock::time_point t_;
+};
 
 } // namespace b2g
 

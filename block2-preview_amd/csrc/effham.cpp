@@ -210,6 +210,7 @@ extern "C" int b2g_effham_create(b2g_ctx ctx, const b2g_problem *pb,
                     pb->n_left_ops >= 0 && pb->n_right_ops >= 0 &&
                     pb->n_terms >= 0,
                 B2G_ERR_ARG, "negative counts");
+    SetupTimer tm("effham");
     std::unique_ptr<b2g_effham_s> h(new b2g_effham_s());
     h->ctx = ctx;
     Problem &p = h->prob;
@@ -329,6 +330,7 @@ extern "C" int b2g_effham_create(b2g_ctx ctx, const b2g_problem *pb,
         }
         p.terms.push_back(tm);
     }
+    tm.mark("problem");
     // connection lists (EffectiveHamiltonian ctor)
     SU2Coupling cg;
     std::vector<SubLabel> subl = uniq_sub_labels(p);
@@ -341,6 +343,7 @@ extern "C" int b2g_effham_create(b2g_ctx ctx, const b2g_problem *pb,
                            p.ket);
     build_wfn_connections(h->wfn, p.symm, cg, p.ket.dq, p.bra.dq, p.opdq, subl,
                           ai, bi, p.ket, p.bra);
+    tm.mark("connection lists");
     std::string err;
     h->pairs = record_pairs(p, h->wfn, err);
     B2G_REQUIRE(err.empty(), B2G_ERR_ARG, err);
@@ -385,6 +388,7 @@ extern "C" int b2g_effham_create(b2g_ctx ctx, const b2g_problem *pb,
     h->macs = 0;
     for (auto &q : h->pairs)
         h->macs += pair_macs(q);
+    tm.mark("pair and diagonal records");
     if (flags & B2G_EFFHAM_HOST_ONLY) {
         // plan only: connection lists and pairs, no device state
         *out = h.release();
@@ -424,6 +428,7 @@ extern "C" int b2g_effham_create(b2g_ctx ctx, const b2g_problem *pb,
             return r;
         }
     }
+    tm.mark("operator upload");
     std::vector<b2g_pair> dp(h->pairs.size());
     for (size_t i = 0; i < dp.size(); i++) {
         const PairRec &q = h->pairs[i];
@@ -435,6 +440,7 @@ extern "C" int b2g_effham_create(b2g_ctx ctx, const b2g_problem *pb,
     r = b2g_upload_plan(ctx, h->lbuf, h->rbuf, dp.data(), dp.size(), yb.data(),
                         yb.size(), p.ket.total(), p.bra.total(), flags,
                         &h->plan);
+    tm.mark("plan upload");
     if (r == B2G_OK) {
         const size_t dl = std::max<uint64_t>(p.ket.total(), 1);
         if (hipMalloc(&h->diag, dl * sizeof(double)) != hipSuccess) {
@@ -443,6 +449,7 @@ extern "C" int b2g_effham_create(b2g_ctx ctx, const b2g_problem *pb,
         } else
             r = update_diag(h.get());
     }
+    tm.mark("diagonal");
     if (r != B2G_OK) {
         b2g_effham_destroy(h.release());
         return r;

@@ -56,54 +56,68 @@ __device__ __forceinline__ u32x4 pk_load(__amdgpu_buffer_rsrc_t r,
 }
 
 // ------------------------------------------------------------------ packing
-// One PackTile: rows [0, rows) x k groups [0, groups) of a packed matrix,
-// element (r, k) from src[r*src_ld + k] (trans: src[k*src_ld + r]) for
-// k < cols and zero for cols <= k < 8*groups.  256 threads: one 8-element
-// group each, consecutive lanes along the source's contiguous index.
+// Tile t of a job: its matrix by binary search on tile0, then rows
+// [32 tr, 32 tr + 32) x k groups [8 tk, 8 tk + 8) of it.  256 threads: one
+// 8-element group each, consecutive lanes along the source's contiguous
+// index.
 __global__ __launch_bounds__(256) void pack_bf16_kernel(
     const double *__restrict__ src, __bf16 *__restrict__ hi,
-    __bf16 *__restrict__ lo, const PackTile *__restrict__ tiles, int ntiles,
-    uint64_t src_rs, uint64_t dst_rs) {
+    __bf16 *__restrict__ lo, const PackMat *__restrict__ mats, int nmats,
+    uint32_t ntiles, uint64_t src_rs, uint64_t dst_rs) {
     const int rhs = blockIdx.y;
     src += rhs * src_rs;
     hi += rhs * dst_rs;
     lo += rhs * dst_rs;
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        const PackTile pt = tiles[t];
+    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        int lo_m = 0, hi_m = nmats - 1; // last matrix with tile0 <= t
+        while (lo_m < hi_m) {
+            const int mid = (lo_m + hi_m + 1) >> 1;
+            if (mats[mid].tile0 <= t)
+                lo_m = mid;
+            else
+                hi_m = mid - 1;
+        }
+        const PackMat pm = mats[lo_m];
+        const uint32_t ktiles = (pm.dst_ld + 63) / 64;
+        const uint32_t local = t - pm.tile0;
+        const uint32_t r0 = (local / ktiles) * 32, k0 = (local % ktiles) * 64;
         uint32_t r, g;
-        if (pt.trans)
+        if (pm.trans)
             r = threadIdx.x & 31, g = threadIdx.x >> 5;
         else
             g = threadIdx.x & 7, r = threadIdx.x >> 3;
-        if (r >= pt.rows || g >= pt.groups)
+        r += r0;
+        const uint32_t k = k0 + 8 * g;
+        if (r >= pm.rows || k >= pm.dst_ld)
             continue;
         bf16x8 h, l;
 #pragma unroll
         for (int i = 0; i < 8; i++) {
-            const uint32_t k = 8 * g + i;
+            const uint32_t kk = k + i;
             double v = 0.0;
-            if (k < pt.cols)
-                v = src[pt.src + (pt.trans ? (uint64_t)k * pt.src_ld + r
-                                           : (uint64_t)r * pt.src_ld + k)];
+            if (kk < pm.cols)
+                v = src[pm.src + (pm.trans ? (uint64_t)kk * pm.src_ld + r
+                                           : (uint64_t)r * pm.src_ld + kk)];
             const __bf16 vh = (__bf16)(float)v;
             h[i] = vh;
             l[i] = (__bf16)(float)(v - (double)(float)vh);
         }
-        const uint64_t o = pt.dst + (uint64_t)r * pt.dst_ld + 8 * g;
+        const uint64_t o = pm.dst + (uint64_t)r * pm.dst_ld + k;
         *(bf16x8 *)(hi + o) = h;
         *(bf16x8 *)(lo + o) = l;
     }
 }
 
 hipError_t launch_pack_bf16(const double *src, __bf16 *hi, __bf16 *lo,
-                            const PackTile *tiles, int ntiles,
+                            const PackMat *mats, int nmats, uint32_t ntiles,
                             uint64_t src_rs, uint64_t dst_rs, int n_rhs,
                             int grid, hipStream_t st) {
-    if (ntiles <= 0)
+    if (nmats <= 0 || ntiles == 0)
         return hipSuccess;
-    dim3 g((unsigned)std::min(ntiles, grid), (unsigned)n_rhs);
-    hipLaunchKernelGGL(pack_bf16_kernel, g, dim3(256), 0, st, src, hi, lo,
-                       tiles, ntiles, src_rs, dst_rs);
+    const uint32_t g = ntiles < (uint32_t)grid * 4 ? ntiles : (uint32_t)grid * 4;
+    hipLaunchKernelGGL(pack_bf16_kernel, dim3(g, (unsigned)n_rhs), dim3(256),
+                       0, st, src, hi, lo, mats, nmats, ntiles, src_rs,
+                       dst_rs);
     return hipGetLastError();
 }
 

@@ -87,15 +87,16 @@ struct RedTask {
     uint32_t ldd, M, N, row0, col0, tm, tn, nparts, beta, pad;
 };
 
-// One tile of a packing job (split-BF16 operand planes, gemm_bf16x3.hip):
-// rows [0, rows) and 8-element k groups [0, groups) of a packed matrix at
-// element offset dst (row length dst_ld, a multiple of 8); element (r, k) is
-// src[src + r*src_ld + k] (trans: src[src + k*src_ld + r]) for k < cols and
-// zero for cols <= k < 8*groups.  rows <= 32, groups <= 8.
-struct PackTile {
+// One matrix of a packing job (split-BF16 operand planes, gemm_bf16x3.hip):
+// a rows x dst_ld packed matrix at element offset dst (dst_ld = cols rounded
+// up to 8), element (r, k) = src[src + r*src_ld + k] (trans:
+// src[src + k*src_ld + r]) for k < cols and zero for cols <= k < dst_ld.  It
+// is packed in tiles of 32 rows x 64 k, numbered from tile0 on in row-major
+// tile order; the matrices of a job are sorted by tile0.
+struct PackMat {
     uint64_t src, dst;
     uint32_t src_ld, dst_ld;
-    uint32_t rows, cols, groups, trans;
+    uint32_t rows, cols, trans, tile0;
 };
 
 // diagonal: output block with its contribution range

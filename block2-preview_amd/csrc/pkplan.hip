@@ -33,9 +33,19 @@ uint64_t round8(uint64_t v) { return (v + 7) & ~(uint64_t)7; }
 // (offset, row stride, orientation, rows, K)
 struct Packer {
     typedef std::tuple<uint64_t, uint32_t, uint32_t, uint32_t, uint32_t> Key;
-    std::map<Key, uint64_t> where;
-    std::vector<PackTile> tiles;
-    uint64_t len = 0;
+    struct KeyHash {
+        size_t operator()(const Key &k) const {
+            uint64_t h = std::get<0>(k) * 0x9E3779B97F4A7C15ull;
+            h ^= ((uint64_t)std::get<1>(k) << 32 | std::get<2>(k)) +
+                 0x632BE59BD9B4E019ull + (h << 6) + (h >> 2);
+            h ^= ((uint64_t)std::get<3>(k) << 32 | std::get<4>(k)) + (h << 6) +
+                 (h >> 2);
+            return (size_t)h;
+        }
+    };
+    std::unordered_map<Key, uint64_t, KeyHash> where;
+    std::vector<PackMat> mats;
+    uint64_t len = 0, ntiles = 0;
     uint64_t src_len = 0;
 
     // packed offset of rows x K matrix (r, k) = src[off + r*ld + k]
@@ -57,19 +67,17 @@ struct Packer {
         if ((uint64_t)rows * pld >= ((uint64_t)1 << 28))
             return -1;
         const uint64_t base = len;
-        for (uint32_t r0 = 0; r0 < rows; r0 += 32)
-            for (uint64_t k0 = 0; k0 < pld; k0 += 64) {
-                PackTile t;
-                t.src = off + (trans ? k0 * ld + r0 : (uint64_t)r0 * ld + k0);
-                t.dst = base + (uint64_t)r0 * pld + k0;
-                t.src_ld = ld;
-                t.dst_ld = (uint32_t)pld;
-                t.rows = std::min<uint32_t>(32, rows - r0);
-                t.cols = K > k0 ? (uint32_t)std::min<uint64_t>(64, K - k0) : 0;
-                t.groups = (uint32_t)std::min<uint64_t>(8, (pld - k0) / 8);
-                t.trans = trans ? 1u : 0u;
-                tiles.push_back(t);
-            }
+        PackMat m;
+        m.src = off;
+        m.dst = base;
+        m.src_ld = ld;
+        m.dst_ld = (uint32_t)pld;
+        m.rows = rows;
+        m.cols = K;
+        m.trans = trans ? 1u : 0u;
+        m.tile0 = (uint32_t)ntiles;
+        mats.push_back(m);
+        ntiles += (uint64_t)((rows + 31) / 32) * ((pld + 63) / 64);
         len += (uint64_t)rows * pld;
         where.emplace(key, base);
         return (int64_t)base;
@@ -103,14 +111,16 @@ static int pk_pack_ops(b2g_plan p, PkPlan *k) {
     hipStream_t st = p->ctx->stream;
     __bf16 *op = (__bf16 *)k->op;
     B2G_HIP(launch_pack_bf16((const double *)p->lbuf->ptr, op, op + k->l_len,
-                             k->d_ltiles, k->n_ltiles, 0, 0, 1, grid, st));
+                             k->d_lmats, k->n_lmats, k->n_ltiles, 0, 0, 1,
+                             grid, st));
     B2G_HIP(launch_pack_bf16((const double *)p->rbuf->ptr, op + 2 * k->l_len,
-                             op + 2 * k->l_len + k->r_len, k->d_rtiles,
-                             k->n_rtiles, 0, 0, 1, grid, st));
+                             op + 2 * k->l_len + k->r_len, k->d_rmats,
+                             k->n_rmats, k->n_rtiles, 0, 0, 1, grid, st));
     return B2G_OK;
 }
 
 static int pk_build_into(b2g_plan p, PkPlan *k) {
+    SetupTimer tm("packed plan");
     Packer L, R, X;
     L.src_len = p->lbuf->n;
     R.src_len = p->rbuf->n;
@@ -131,14 +141,12 @@ static int pk_build_into(b2g_plan p, PkPlan *k) {
                 w1[t.seg_begin] = t.d_off;
             }
         std::vector<uint32_t> m2(ch.s2.size(), 0);
-        for (int c = 0; c < kTileConfigs; c++)
-            for (const Task &t : ch.t2[c]) {
-                B2G_REQUIRE(t.seg_begin <= t.seg_end &&
-                                t.seg_end <= ch.s2.size(),
-                            B2G_ERR_STATE, "packed plan: phase-2 task shape");
-                for (uint32_t s = t.seg_begin; s < t.seg_end; s++)
-                    m2[s] = t.M;
-            }
+        for (const BlockJob &j : ch.jobs2) {
+            B2G_REQUIRE(j.sb <= j.se && j.se <= ch.s2.size(), B2G_ERR_STATE,
+                        "packed plan: phase-2 block job shape");
+            for (uint32_t s = j.sb; s < j.se; s++)
+                m2[s] = j.M;
+        }
         // packed W^T of each phase-1 segment, keyed by the plan's W offset
         std::unordered_map<uint64_t, std::pair<uint64_t, uint32_t>> wmap;
         std::vector<uint64_t> wpk(ch.s1.size(), 0);
@@ -186,10 +194,12 @@ static int pk_build_into(b2g_plan p, PkPlan *k) {
             q.flags = 0;
             h.s2.push_back(q);
         }
+        tm.mark("segments and pack lists");
         // phase 2 re-tiled likewise, block by block, with split-K partial
         // slots and reductions of its own
         for (const BlockJob &j : ch.jobs2)
             add_split_tiles(h.s2, j, 1, h.t2, h.red, h.p_len);
+        tm.mark("phase-2 tiles");
         // tiles of one output block (one segment list) share their operand
         // panels: keep them on one XCD, back to back (its L2 serves the
         // re-reads); the reductions address partial slots by task offsets,
@@ -201,6 +211,7 @@ static int pk_build_into(b2g_plan p, PkPlan *k) {
         k->w_max = std::max(k->w_max, wlen);
         k->p_max = std::max(k->p_max, h.p_len);
     }
+    tm.mark("task order");
     k->l_len = L.len, k->r_len = R.len, k->x_len = X.len;
     // one allocation for every task, segment and pack-tile array
     size_t bytes = 0;
@@ -210,8 +221,11 @@ static int pk_build_into(b2g_plan p, PkPlan *k) {
         bytes += (h.s1.size() + h.s2.size()) * sizeof(Seg);
         bytes += h.red.size() * sizeof(RedTask);
     }
-    bytes += (L.tiles.size() + R.tiles.size() + X.tiles.size()) *
-             sizeof(PackTile);
+    bytes += (L.mats.size() + R.mats.size() + X.mats.size()) * sizeof(PackMat);
+    B2G_REQUIRE(L.ntiles < (1ull << 32) && R.ntiles < (1ull << 32) &&
+                    X.ntiles < (1ull << 32) && L.mats.size() < (1u << 31) &&
+                    R.mats.size() < (1u << 31) && X.mats.size() < (1u << 31),
+                B2G_ERR_ARG, "packed plan: too many pack tiles");
     std::vector<char> host(bytes + 16);
     B2G_HIP(hipMalloc(&k->dev, bytes + 16));
     size_t off = 0;
@@ -242,27 +256,27 @@ static int pk_build_into(b2g_plan p, PkPlan *k) {
         pc.d_s2 = (Seg *)put(h.s2.data(), h.s2.size() * sizeof(Seg));
         k->chunks.push_back(pc);
     }
-    k->d_ltiles = (PackTile *)put(L.tiles.data(),
-                                  L.tiles.size() * sizeof(PackTile));
-    k->d_rtiles = (PackTile *)put(R.tiles.data(),
-                                  R.tiles.size() * sizeof(PackTile));
-    k->d_xtiles = (PackTile *)put(X.tiles.data(),
-                                  X.tiles.size() * sizeof(PackTile));
-    k->n_ltiles = (int)L.tiles.size();
-    k->n_rtiles = (int)R.tiles.size();
-    k->n_xtiles = (int)X.tiles.size();
-    B2G_REQUIRE(L.tiles.size() < (1u << 31) && R.tiles.size() < (1u << 31) &&
-                    X.tiles.size() < (1u << 31),
-                B2G_ERR_ARG, "packed plan: too many pack tiles");
+    k->d_lmats = (PackMat *)put(L.mats.data(), L.mats.size() * sizeof(PackMat));
+    k->d_rmats = (PackMat *)put(R.mats.data(), R.mats.size() * sizeof(PackMat));
+    k->d_xmats = (PackMat *)put(X.mats.data(), X.mats.size() * sizeof(PackMat));
+    k->n_lmats = (int)L.mats.size();
+    k->n_rmats = (int)R.mats.size();
+    k->n_xmats = (int)X.mats.size();
+    k->n_ltiles = (uint32_t)L.ntiles;
+    k->n_rtiles = (uint32_t)R.ntiles;
+    k->n_xtiles = (uint32_t)X.ntiles;
     B2G_HIP(hipMemcpyAsync(k->dev, host.data(), bytes, hipMemcpyHostToDevice,
                            p->ctx->stream));
     B2G_HIP(hipStreamSynchronize(p->ctx->stream));
+    tm.mark("upload");
     B2G_HIP(hipMalloc(&k->op, std::max<uint64_t>(2 * (k->l_len + k->r_len), 8) *
                                   sizeof(uint16_t)));
+    tm.mark("planes allocated");
     const int r = pk_pack_ops(p, k);
     if (r != B2G_OK)
         return r;
     B2G_HIP(hipStreamSynchronize(p->ctx->stream));
+    tm.mark("operators packed");
     return B2G_OK;
 }
 

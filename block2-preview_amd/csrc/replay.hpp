@@ -132,8 +132,10 @@ struct PkPlan {
     void *dev = nullptr;     // task, segment and pack-tile arrays
     uint16_t *op = nullptr;  // packed operator planes: L hi, L lo, R hi, R lo
     uint64_t l_len = 0, r_len = 0;
-    PackTile *d_ltiles = nullptr, *d_rtiles = nullptr, *d_xtiles = nullptr;
-    int n_ltiles = 0, n_rtiles = 0, n_xtiles = 0;
+    // pack jobs: operator blocks (L, R) and wavefunction windows (X)
+    PackMat *d_lmats = nullptr, *d_rmats = nullptr, *d_xmats = nullptr;
+    int n_lmats = 0, n_rmats = 0, n_xmats = 0;
+    uint32_t n_ltiles = 0, n_rtiles = 0, n_xtiles = 0;
     uint64_t x_len = 0; // packed wavefunction elements per right-hand side
     uint64_t w_max = 0; // packed W^T elements per right-hand side
     uint64_t p_max = 0; // split-K partial elements per right-hand side

@@ -5,6 +5,7 @@
 #include "replay.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -230,11 +231,17 @@ extern "C" size_t b2g_buf_size(b2g_buf buf) { return buf ? buf->n : 0; }
 // -------------------------------------------------------------------- plans
 namespace b2g {
 
-static uint64_t tile_cost(const Task &t, const std::vector<Seg> &segs) {
-    uint64_t k = 0;
-    for (uint32_t s = t.seg_begin; s < t.seg_end; s++)
-        k += segs[s].K;
-    return k + 1;
+// depth of each task (sum of its segments' K, + 1), from a prefix sum over
+// the segment list
+static std::vector<uint64_t> task_depths(const std::vector<Task> &tasks,
+                                         const std::vector<Seg> &segs) {
+    std::vector<uint64_t> pre(segs.size() + 1, 0);
+    for (size_t s = 0; s < segs.size(); s++)
+        pre[s + 1] = pre[s] + segs[s].K;
+    std::vector<uint64_t> d(tasks.size());
+    for (size_t i = 0; i < tasks.size(); i++)
+        d[i] = pre[tasks[i].seg_end] - pre[tasks[i].seg_begin] + 1;
+    return d;
 }
 
 // Tiling policy.  Every output block is covered by workgroup tiles whose
@@ -281,7 +288,18 @@ static std::vector<std::vector<uint32_t>> covers(uint32_t d) {
 }
 
 // tiles covering an M x N output block whose tasks have depth K
+// Tilings are memoised per (M, N, table, forced shape): the estimated time
+// of every cover scales with K + 1 alike, so the choice does not depend on
+// K.  The memo is per host thread and cleared when rates change.
+static std::atomic<uint64_t> g_rates_gen{1};
+struct TileMemo {
+    uint64_t gen = 0;
+    std::unordered_map<uint64_t, std::vector<Tile>> map;
+};
+static thread_local TileMemo t_memo;
+
 std::vector<Tile> tile_cover(uint32_t M, uint32_t N, uint64_t K, int table) {
+    (void)K;
     const double *rates = g_rates[table == 1 ? 1 : 0];
     const char *e = getenv("B2G_TILE");
     std::vector<uint32_t> rows, cols;
@@ -296,6 +314,19 @@ std::vector<Tile> tile_cover(uint32_t M, uint32_t N, uint64_t K, int table) {
         if (nr >= 1 && okv(a) && okv(b))
             fm = a, fn = b;
     }
+    if (t_memo.gen != g_rates_gen) {
+        t_memo.map.clear();
+        t_memo.gen = g_rates_gen;
+    }
+    const bool memo = M < (1u << 24) && N < (1u << 24);
+    const uint64_t key = ((uint64_t)M << 36) | ((uint64_t)N << 12) |
+                         ((uint64_t)(table == 1) << 11) |
+                         ((uint64_t)(fm >> 5) << 8) | ((uint64_t)(fn >> 5) << 4);
+    if (memo) {
+        auto it = t_memo.map.find(key);
+        if (it != t_memo.map.end())
+            return it->second;
+    }
     if (fm != 0) {
         rows.assign((M + fm - 1) / fm, fm);
         cols.assign((N + fn - 1) / fn, fn);
@@ -306,7 +337,7 @@ std::vector<Tile> tile_cover(uint32_t M, uint32_t N, uint64_t K, int table) {
                 double c = 0;
                 for (uint32_t a : rc)
                     for (uint32_t b : cc)
-                        c += (double)a * b * (double)(K + 1) /
+                        c += (double)a * b /
                              rates[3 * size_index(a) + size_index(b)];
                 if (c < best)
                     best = c, rows = rc, cols = cc;
@@ -322,6 +353,8 @@ std::vector<Tile> tile_cover(uint32_t M, uint32_t N, uint64_t K, int table) {
         }
         r0 += a;
     }
+    if (memo)
+        t_memo.map.emplace(key, out);
     return out;
 }
 
@@ -430,22 +463,28 @@ double task_flops(const std::vector<Task> &tasks,
                          const std::vector<Seg> &segs, int cfg) {
     double f = 0;
     const uint32_t tm = 32u << (cfg / 3), tn = 32u << (cfg % 3);
-    for (const Task &t : tasks) {
-        uint64_t k = 0;
-        for (uint32_t s = t.seg_begin; s < t.seg_end; s++)
-            k += segs[s].K;
+    const std::vector<uint64_t> d = task_depths(tasks, segs);
+    for (size_t i = 0; i < tasks.size(); i++) {
+        const Task &t = tasks[i];
         const uint32_t m = std::min(tm, t.M - t.row0);
         const uint32_t n = std::min(tn, t.N - t.col0);
-        f += 2.0 * m * n * (double)k;
+        f += 2.0 * m * n * (double)(d[i] - 1);
     }
     return f;
 }
 
 void sort_tasks(std::vector<Task> &tasks, const std::vector<Seg> &segs) {
-    std::stable_sort(tasks.begin(), tasks.end(),
-                     [&segs](const Task &a, const Task &b) {
-                         return tile_cost(a, segs) > tile_cost(b, segs);
-                     });
+    const std::vector<uint64_t> d = task_depths(tasks, segs);
+    std::vector<size_t> order(tasks.size());
+    for (size_t i = 0; i < order.size(); i++)
+        order[i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&d](size_t a, size_t b) { return d[a] > d[b]; });
+    std::vector<Task> out;
+    out.reserve(tasks.size());
+    for (size_t i : order)
+        out.push_back(tasks[i]);
+    tasks.swap(out);
 }
 
 // B2G_XCD=1 turns the per-XCD task ranges on for the FP64 / FP32 kernels
@@ -473,44 +512,57 @@ XcdSplit xcd_order(std::vector<Task> &tasks, const std::vector<Seg> &segs,
         sort_tasks(tasks, segs);
         return xs;
     }
-    std::vector<std::pair<uint32_t, std::vector<size_t>>> groups;
-    std::unordered_map<uint32_t, size_t> gid;
-    for (size_t i = 0; i < tasks.size(); i++) {
-        auto it = gid.find(tasks[i].seg_begin);
-        if (it == gid.end()) {
-            it = gid.emplace(tasks[i].seg_begin, groups.size()).first;
-            groups.push_back({tasks[i].seg_begin, {}});
+    // groups: runs of equal seg_begin after a stable sort of the task
+    // indices (creation order kept inside a group)
+    std::vector<size_t> idx(tasks.size());
+    for (size_t i = 0; i < idx.size(); i++)
+        idx[i] = i;
+    std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) {
+        return tasks[a].seg_begin < tasks[b].seg_begin;
+    });
+    const std::vector<uint64_t> depth = task_depths(tasks, segs);
+    std::vector<size_t> gstart;
+    std::vector<double> gcost;
+    for (size_t k = 0; k < idx.size(); k++) {
+        if (k == 0 ||
+            tasks[idx[k]].seg_begin != tasks[idx[k - 1]].seg_begin) {
+            gstart.push_back(k);
+            gcost.push_back(0.0);
         }
-        groups[it->second].second.push_back(i);
+        gcost.back() += (double)depth[idx[k]];
     }
-    std::vector<double> gcost(groups.size(), 0.0);
-    for (size_t g = 0; g < groups.size(); g++)
-        for (size_t i : groups[g].second)
-            gcost[g] += (double)tile_cost(tasks[i], segs);
-    std::vector<size_t> order(groups.size());
-    for (size_t g = 0; g < order.size(); g++)
+    gstart.push_back(idx.size());
+    const size_t ng = gcost.size();
+    std::vector<size_t> order(ng);
+    for (size_t g = 0; g < ng; g++)
         order[g] = g;
     std::stable_sort(order.begin(), order.end(),
                      [&](size_t a, size_t b) { return gcost[a] > gcost[b]; });
+    // longest group first to the least-loaded XCD
     double load[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    std::vector<size_t> per[8];
+    std::vector<int> gx(ng);
+    size_t cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (size_t g : order) {
         int x = 0;
         for (int y = 1; y < 8; y++)
             if (load[y] < load[x])
                 x = y;
         load[x] += gcost[g];
-        for (size_t i : groups[g].second)
-            per[x].push_back(i);
+        gx[g] = x;
+        cnt[x] += gstart[g + 1] - gstart[g];
     }
-    std::vector<Task> out;
-    out.reserve(tasks.size());
+    size_t pos[8];
+    size_t acc = 0;
     for (int x = 0; x < 8; x++) {
-        xs.off[x] = (uint32_t)out.size();
-        for (size_t i : per[x])
-            out.push_back(tasks[i]);
+        xs.off[x] = (uint32_t)acc;
+        pos[x] = acc;
+        acc += cnt[x];
     }
-    xs.off[8] = (uint32_t)out.size();
+    std::vector<Task> out(tasks.size());
+    for (size_t g : order)
+        for (size_t k = gstart[g]; k < gstart[g + 1]; k++)
+            out[pos[gx[g]]++] = tasks[idx[k]];
+    xs.off[8] = (uint32_t)acc;
     tasks.swap(out);
     return xs;
 }
@@ -579,6 +631,7 @@ static int upload_chunks(b2g_plan p) {
 
 static int compile_plan(b2g_plan p, const b2g_pair *pairs, size_t n_pairs,
                         const b2g_block *yb, size_t n_yb) {
+    SetupTimer tm("plan");
     const size_t llen = p->lbuf->n, rlen = p->rbuf->n;
     std::unordered_map<uint64_t, size_t> yidx;
     std::vector<std::pair<uint64_t, size_t>> ysorted; // (offset, block)
@@ -655,7 +708,19 @@ static int compile_plan(b2g_plan p, const b2g_pair *pairs, size_t n_pairs,
             typedef std::tuple<uint64_t, uint64_t, uint32_t, uint32_t, uint32_t,
                                uint32_t>
                 WKey;
-            std::map<WKey, uint64_t> wmap;            // -> W offset
+            struct WHash {
+                size_t operator()(const WKey &k) const {
+                    uint64_t h = std::get<0>(k) * 0x9E3779B97F4A7C15ull;
+                    h ^= std::get<1>(k) + 0x632BE59BD9B4E019ull + (h << 6) +
+                         (h >> 2);
+                    h ^= ((uint64_t)std::get<2>(k) << 48 |
+                          (uint64_t)std::get<3>(k) << 24 | std::get<4>(k)) +
+                         (h << 6) + (h >> 2);
+                    h ^= std::get<5>(k) + (h << 6) + (h >> 2);
+                    return (size_t)h;
+                }
+            };
+            std::unordered_map<WKey, uint64_t, WHash> wmap; // -> W offset
             std::unordered_map<size_t, uint64_t> pw;  // pair -> W offset
             size_t j1 = j0;
             for (; j1 < idx.size(); j1++) {
@@ -733,6 +798,7 @@ static int compile_plan(b2g_plan p, const b2g_pair *pairs, size_t n_pairs,
                          std::get<2>(kv.first), std::get<3>(kv.first),
                          &kv.second, 1);
             }
+            tm.mark("chunk tasks");
             for (int c = 0; c < kTileConfigs; c++) {
                 ch.x1[c] = xcd_order(ch.t1[c], ch.s1, xcd_enabled());
                 ch.x2[c] = xcd_order(ch.t2[c], ch.s2, xcd_enabled());
@@ -741,6 +807,7 @@ static int compile_plan(b2g_plan p, const b2g_pair *pairs, size_t n_pairs,
                 p->kflops[B2G_K_PHASE1 + c] += ch.f1[c];
                 p->kflops[B2G_K_PHASE2 + c] += ch.f2[c];
             }
+            tm.mark("chunk order and flops");
             p->w_max = std::max(p->w_max, ch.w_len);
             p->p_max = std::max(p->p_max, ch.p_len);
             p->chunks.push_back(std::move(ch));
@@ -750,7 +817,9 @@ static int compile_plan(b2g_plan p, const b2g_pair *pairs, size_t n_pairs,
     };
     build(whole, false);
     build(windows, true);
-    return upload_chunks(p);
+    const int r = upload_chunks(p);
+    tm.mark("upload");
+    return r;
 }
 
 } // namespace b2g
@@ -850,6 +919,7 @@ extern "C" int b2g_set_tile_rates_for(int precision, const double *rates,
     const int t = precision == B2G_PREC_BF16X3 ? 1 : 0;
     for (int i = 0; i < n; i++)
         g_rates[t][i] = rates[i];
+    g_rates_gen++;
     return B2G_OK;
 }
 
@@ -1083,7 +1153,8 @@ extern "C" int b2g_hpsi(b2g_ctx ctx, b2g_plan p, const double *x, double *y,
     if (pk != nullptr)
         B2G_HIP(timed(B2G_K_PACK, ctx->stream, [&](hipStream_t st) {
             return launch_pack_bf16(x, (__bf16 *)pk->xh(), (__bf16 *)pk->xl(),
-                                    pk->d_xtiles, pk->n_xtiles, p->x_len,
+                                    pk->d_xmats, pk->n_xmats, pk->n_xtiles,
+                                    p->x_len,
                                     pk->x_len, n_rhs, grid, st);
         }));
     for (size_t ic = 0; ic < p->chunks.size(); ic++) {
