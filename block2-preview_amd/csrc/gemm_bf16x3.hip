@@ -201,7 +201,8 @@ template <int TM, int TN> struct PkTile {
 // OUT 0: D0/D1 are the hi/lo planes of the packed transpose of the output
 //        (element (m, n) of a task's block at d_off + n*ldd + m, ldd the
 //        packed row length; rows M <= m < ldd receive the zeros of the
-//        masked A rows, so the padding of W^T is rewritten by every launch);
+//        masked A rows, so the padding of W^T is rewritten by every launch),
+//        written through LDS as 16-byte vectors along m;
 // OUT 1: D0 is the FP64 output (row-major, ldd), D1 the FP64 split-K
 //        partial buffer (task.beta bit 1), as in seg_gemm_kernel.
 template <int TM, int TN, int OUT>
@@ -327,30 +328,46 @@ __global__ __launch_bounds__(256) void seg_gemm_pk_kernel(
             __syncthreads();
         }
         if constexpr (OUT == 0) {
+            // W^T through LDS: the tile (row m, column n) lands in a float
+            // [TM][TN + 4] array in the (free) staging buffers, then every
+            // thread writes 8 consecutive m of one W^T row as a 16-byte
+            // vector per plane (whole rows of 256 bytes per 16 lanes)
             mfma_settle<f32x4, FM * FN>(&part[0][0]);
-            __bf16 *Wh = (__bf16 *)D0 + rhs * d_rs;
-            __bf16 *Wl = (__bf16 *)D1 + rhs * d_rs;
+            constexpr int LDT = TN + 4;
+            static_assert(TM * LDT * 4 <= 2 * T::BUF * 2, "W^T staging");
+            float *tile = (float *)smem;
+            __syncthreads();
 #pragma unroll
             for (int i = 0; i < FM; i++)
 #pragma unroll
-                for (int j = 0; j < FN; j++) {
-                    const uint32_t row = task.row0 + wm * WM + i * 16 + 4 * lk;
-                    const uint32_t col = task.col0 + wn * WN + j * 16 + lr;
-                    if (row < task.ldd && col < task.N) {
-                        bf16x4 h, l;
+                for (int j = 0; j < FN; j++)
 #pragma unroll
-                        for (int r = 0; r < 4; r++) {
-                            const float v = part[i][j][r];
-                            const __bf16 vh = (__bf16)v;
-                            h[r] = vh;
-                            l[r] = (__bf16)(v - (float)vh);
-                        }
-                        const uint64_t o =
-                            task.d_off + (uint64_t)col * task.ldd + row;
-                        *(bf16x4 *)(Wh + o) = h;
-                        *(bf16x4 *)(Wl + o) = l;
+                    for (int r = 0; r < 4; r++)
+                        tile[(wm * WM + i * 16 + 4 * lk + r) * LDT + wn * WN +
+                             j * 16 + lr] = part[i][j][r];
+            __syncthreads();
+            __bf16 *Wh = (__bf16 *)D0 + rhs * d_rs;
+            __bf16 *Wl = (__bf16 *)D1 + rhs * d_rs;
+            constexpr int MG = TM / 8;
+            for (int v = tid; v < TN * MG; v += 256) {
+                const int n = v / MG, m0 = 8 * (v % MG);
+                const uint32_t row = task.row0 + m0, col = task.col0 + n;
+                if (row < task.ldd && col < task.N) {
+                    bf16x8 h, l;
+#pragma unroll
+                    for (int q = 0; q < 8; q++) {
+                        const float x = tile[(m0 + q) * LDT + n];
+                        const __bf16 xh = (__bf16)x;
+                        h[q] = xh;
+                        l[q] = (__bf16)(x - (float)xh);
                     }
+                    const uint64_t o =
+                        task.d_off + (uint64_t)col * task.ldd + row;
+                    *(bf16x8 *)(Wh + o) = h;
+                    *(bf16x8 *)(Wl + o) = l;
                 }
+            }
+            __syncthreads(); // the staging buffers are the next task's
         } else {
             double *D = (double *)D0 + rhs * d_rs;
             double *P = (double *)D1 + rhs * p_rs;

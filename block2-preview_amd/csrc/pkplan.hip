@@ -84,6 +84,13 @@ struct Packer {
     }
 };
 
+// phase-1 tasks grouped by their right-operator block (B2G_PK_GROUP_B=0:
+// by W block)
+static bool pk_group_b() {
+    const char *e = getenv("B2G_PK_GROUP_B");
+    return e == nullptr || atoi(e) != 0;
+}
+
 struct HostChunk {
     std::vector<Task> t1[kTileConfigs], t2[kTileConfigs];
     std::vector<Seg> s1, s2;
@@ -205,7 +212,7 @@ static int pk_build_into(b2g_plan p, PkPlan *k) {
         // re-reads); the reductions address partial slots by task offsets,
         // which reordering keeps
         for (int c = 0; c < kTileConfigs; c++) {
-            h.x1[c] = xcd_order(h.t1[c], h.s1, true);
+            h.x1[c] = xcd_order(h.t1[c], h.s1, true, pk_group_b());
             h.x2[c] = xcd_order(h.t2[c], h.s2, true);
         }
         k->w_max = std::max(k->w_max, wlen);

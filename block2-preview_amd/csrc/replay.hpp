@@ -102,10 +102,11 @@ void add_split_tiles(const std::vector<Seg> &s2, const BlockJob &j, int table,
 double task_flops(const std::vector<Task> &tasks, const std::vector<Seg> &segs,
                   int cfg);
 void sort_tasks(std::vector<Task> &tasks, const std::vector<Seg> &segs);
-// order one launch's tasks: grouped by segment list onto the 8 XCDs (the
-// returned per-XCD ranges), or sorted by cost (empty ranges)
+// order one launch's tasks: grouped onto the 8 XCDs (the returned per-XCD
+// ranges) by segment list, or (by_b) by the B operand of their first
+// segment; or sorted by cost (empty ranges)
 XcdSplit xcd_order(std::vector<Task> &tasks, const std::vector<Seg> &segs,
-                   bool grouped);
+                   bool grouped, bool by_b = false);
 // scratch elements per chunk and right-hand side (B2G_W_CAP)
 uint64_t w_cap_elems();
 // allocate the H*psi scratch for n_rhs at precision ahead of a capture

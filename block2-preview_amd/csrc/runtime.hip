@@ -504,7 +504,7 @@ static bool xcd_enabled() {
 // Groups are dealt longest first to the least-loaded XCD; each XCD runs its
 // groups in that order, tiles of a group in creation (row-major) order.
 XcdSplit xcd_order(std::vector<Task> &tasks, const std::vector<Seg> &segs,
-                   bool grouped) {
+                   bool grouped, bool by_b) {
     XcdSplit xs{};
     if (tasks.empty())
         return xs;
@@ -517,15 +517,18 @@ XcdSplit xcd_order(std::vector<Task> &tasks, const std::vector<Seg> &segs,
     std::vector<size_t> idx(tasks.size());
     for (size_t i = 0; i < idx.size(); i++)
         idx[i] = i;
-    std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) {
-        return tasks[a].seg_begin < tasks[b].seg_begin;
-    });
+    // group key: the segment list, or (by_b) the B operand of the first
+    // segment, so that every task reading one B block runs on one XCD
+    std::vector<uint64_t> key(tasks.size());
+    for (size_t i = 0; i < tasks.size(); i++)
+        key[i] = by_b ? segs[tasks[i].seg_begin].b_off : tasks[i].seg_begin;
+    std::stable_sort(idx.begin(), idx.end(),
+                     [&](size_t a, size_t b) { return key[a] < key[b]; });
     const std::vector<uint64_t> depth = task_depths(tasks, segs);
     std::vector<size_t> gstart;
     std::vector<double> gcost;
     for (size_t k = 0; k < idx.size(); k++) {
-        if (k == 0 ||
-            tasks[idx[k]].seg_begin != tasks[idx[k - 1]].seg_begin) {
+        if (k == 0 || key[idx[k]] != key[idx[k - 1]]) {
             gstart.push_back(k);
             gcost.push_back(0.0);
         }
