@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/b2g.h"
@@ -33,6 +34,77 @@ void set_error(const std::string &msg);
             return code;                                                       \
         }                                                                      \
     } while (0)
+
+// Open-addressing hash map of the plan builders: linear probing in a
+// power-of-two table kept at most half full, no erase.  One plan performs
+// 10^4-10^5 lookups; a flat table avoids the per-node allocation and pointer
+// chasing of std::unordered_map.  Pointers returned by find() / emplace()
+// are invalidated by the next emplace().
+template <typename K, typename V, typename H> class FlatMap {
+  public:
+    explicit FlatMap(size_t expect = 0) { grow(expect); }
+    size_t size() const { return n_; }
+    void reserve(size_t expect) { grow(expect); }
+    V *find(const K &k) {
+        for (size_t i = slot(k);; i = (i + 1) & mask_) {
+            if (!used_[i])
+                return nullptr;
+            if (keys_[i] == k)
+                return &vals_[i];
+        }
+    }
+    // the value of k, inserted as v when absent; second: inserted
+    std::pair<V *, bool> emplace(const K &k, const V &v) {
+        if (2 * (n_ + 1) > used_.size())
+            grow(n_ + 1);
+        size_t i = slot(k);
+        for (; used_[i]; i = (i + 1) & mask_)
+            if (keys_[i] == k)
+                return std::make_pair(&vals_[i], false);
+        used_[i] = 1;
+        keys_[i] = k;
+        vals_[i] = v;
+        n_++;
+        return std::make_pair(&vals_[i], true);
+    }
+
+  private:
+    size_t slot(const K &k) const {
+        uint64_t h = (uint64_t)H()(k); // murmur3 finaliser over any hash
+        h ^= h >> 33;
+        h *= 0xFF51AFD7ED558CCDull;
+        h ^= h >> 33;
+        h *= 0xC4CEB9FE1A85EC53ull;
+        h ^= h >> 33;
+        return (size_t)h & mask_;
+    }
+    void grow(size_t expect) {
+        size_t cap = 16;
+        while (cap < 2 * expect)
+            cap <<= 1;
+        if (cap <= used_.size())
+            return;
+        std::vector<K> ok(cap);
+        std::vector<V> ov(cap);
+        std::vector<uint8_t> ou(cap, 0);
+        ok.swap(keys_);
+        ov.swap(vals_);
+        ou.swap(used_);
+        mask_ = cap - 1;
+        n_ = 0;
+        for (size_t i = 0; i < ou.size(); i++)
+            if (ou[i])
+                emplace(ok[i], ov[i]);
+    }
+    std::vector<K> keys_;
+    std::vector<V> vals_;
+    std::vector<uint8_t> used_;
+    size_t mask_ = 0, n_ = 0;
+};
+
+struct U64Hash {
+    size_t operator()(uint64_t v) const { return (size_t)v; }
+};
 
 // Wall time of the stages of a host-side setup (effective Hamiltonian,
 // plan, packed plan): with B2G_TIME_SETUP set, mark() prints the time since
@@ -72,6 +144,10 @@ struct b2g_ctx_s {
     double *red = nullptr;
     size_t red_len = 0;
     double *red_host = nullptr; // pinned
+    // pinned staging buffer of the plan uploads (grown on demand; every
+    // upload waits for its copy before returning)
+    char *stage = nullptr;
+    size_t stage_len = 0;
 };
 
 struct b2g_buf_s {

@@ -809,6 +809,7 @@ static int davidson_graph(b2g_effham h, double *x, double conv_thrd,
     // the subspace holds def_max vectors plus the one appended before the
     // restart check of the next iteration
     const size_t nv = (size_t)def_max + 1;
+    SetupTimer tm("davidson");
     DavCache &c = h->dav;
     if (c.n != n || c.nv != nv) {
         B2G_HIP(hipStreamSynchronize(st));
@@ -830,6 +831,7 @@ static int davidson_graph(b2g_effham h, double *x, double conv_thrd,
         B2G_HIP(fail(hipHostMalloc(&c.hdr, sizeof(DavState))));
         c.n = n;
         c.nv = nv;
+        tm.mark("workspace");
     }
     // normalised start vector -> B[0] and qn
     double nrm2 = 0;
@@ -852,6 +854,7 @@ static int davidson_graph(b2g_effham h, double *x, double conv_thrd,
     if ((r = plan_reserve(h->plan, 1, precision)) != B2G_OK)
         return r;
     B2G_HIP(hipStreamSynchronize(st));
+    tm.mark("plan reserve");
     hipGraphExec_t &exec = c.exec[precision];
     if (exec == nullptr || c.gen[precision] != h->plan->scratch_gen) {
         if (exec) {
@@ -879,6 +882,7 @@ static int davidson_graph(b2g_effham h, double *x, double conv_thrd,
         B2G_HIP(e2);
         B2G_HIP(e);
         c.gen[precision] = h->plan->scratch_gen;
+        tm.mark("graph capture");
     }
     DavState *hdr = c.hdr;
     nmv = 0;
@@ -891,6 +895,7 @@ static int davidson_graph(b2g_effham h, double *x, double conv_thrd,
         status = hdr->status;
         ndav = hdr->iter;
     }
+    tm.mark("iterations");
     eigvals.assign(1, hdr->energy);
     B2G_HIP(launch_copy(c.B, x, n, st));
     B2G_HIP(hipStreamSynchronize(st));

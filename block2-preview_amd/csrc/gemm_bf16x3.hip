@@ -56,7 +56,7 @@ __device__ __forceinline__ u32x4 pk_load(__amdgpu_buffer_rsrc_t r,
 }
 
 // ------------------------------------------------------------------ packing
-// Tile t of a job: its matrix by binary search on tile0, then rows
+// Tile t of a job: its matrix (the last one with tile0 <= t), then rows
 // [32 tr, 32 tr + 32) x k groups [8 tk, 8 tk + 8) of it.  256 threads: one
 // 8-element group each, consecutive lanes along the source's contiguous
 // index.
@@ -68,16 +68,26 @@ __global__ __launch_bounds__(256) void pack_bf16_kernel(
     src += rhs * src_rs;
     hi += rhs * dst_rs;
     lo += rhs * dst_rs;
-    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        int lo_m = 0, hi_m = nmats - 1; // last matrix with tile0 <= t
-        while (lo_m < hi_m) {
-            const int mid = (lo_m + hi_m + 1) >> 1;
-            if (mats[mid].tile0 <= t)
-                lo_m = mid;
-            else
-                hi_m = mid - 1;
-        }
-        const PackMat pm = mats[lo_m];
+    // a contiguous range of tiles per workgroup: one binary search for the
+    // first tile's matrix, then a forward walk over the tile0 list
+    const uint64_t per = (ntiles + (uint64_t)gridDim.x - 1) / gridDim.x;
+    const uint64_t b0 = blockIdx.x * per;
+    if (b0 >= ntiles)
+        return;
+    const uint32_t t_begin = (uint32_t)b0;
+    const uint32_t t_end = (uint32_t)(b0 + per < ntiles ? b0 + per : ntiles);
+    int m = 0, hi_m = nmats - 1; // last matrix with tile0 <= t_begin
+    while (m < hi_m) {
+        const int mid = (m + hi_m + 1) >> 1;
+        if (mats[mid].tile0 <= t_begin)
+            m = mid;
+        else
+            hi_m = mid - 1;
+    }
+    for (uint32_t t = t_begin; t < t_end; t++) {
+        while (m + 1 < nmats && mats[m + 1].tile0 <= t)
+            m++;
+        const PackMat pm = mats[m];
         const uint32_t ktiles = (pm.dst_ld + 63) / 64;
         const uint32_t local = t - pm.tile0;
         const uint32_t r0 = (local / ktiles) * 32, k0 = (local % ktiles) * 64;

@@ -123,6 +123,7 @@ static bool launch_in_bounds(const Launch &L, uint64_t a_len, uint64_t b_len,
 }
 
 static int grad_plan_build(b2g_plan p, GradPlan **out) {
+    tile_env_refresh();
     GradPlan *g = new GradPlan();
     const b2g_pair *pairs = p->pairs.data();
     const size_t n = p->pairs.size();

@@ -516,7 +516,7 @@ __global__ __launch_bounds__(256) void diag_kernel(
         const DiagBlock blk = blocks[bi];
         const uint32_t mn = blk.m * blk.n;
         for (uint32_t e = threadIdx.x; e < mn; e += blockDim.x) {
-            const uint32_t i = e / blk.n, j = e % blk.n;
+            const uint32_t i = blk.row0 + e / blk.n, j = e % blk.n;
             double s = 0.0;
             for (uint32_t t = blk.t_begin; t < blk.t_end; t++) {
                 const DiagTermDev tm = terms[t];

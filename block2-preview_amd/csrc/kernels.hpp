@@ -100,10 +100,13 @@ struct PackMat {
 };
 
 // diagonal: output block with its contribution range
+// rows [row0, row0 + m) of an output block of n columns, stored from d_off;
+// terms [t_begin, t_end) are the block's terms whose rows meet the piece
 struct DiagBlock {
     uint64_t d_off;
     uint32_t m, n;
     uint32_t t_begin, t_end;
+    uint32_t row0, pad;
 };
 struct DiagTermDev {
     uint64_t a_off, b_off;

@@ -20,6 +20,7 @@
 #include <algorithm>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <tuple>
 #include <unordered_map>
 
@@ -43,7 +44,7 @@ struct Packer {
             return (size_t)h;
         }
     };
-    std::unordered_map<Key, uint64_t, KeyHash> where;
+    FlatMap<Key, uint64_t, KeyHash> where;
     std::vector<PackMat> mats;
     uint64_t len = 0, ntiles = 0;
     uint64_t src_len = 0;
@@ -53,9 +54,8 @@ struct Packer {
     int64_t add(uint64_t off, uint32_t ld, bool trans, uint32_t rows,
                 uint32_t K) {
         const Key key(off, ld, trans ? 1u : 0u, rows, K);
-        auto it = where.find(key);
-        if (it != where.end())
-            return (int64_t)it->second;
+        if (const uint64_t *w = where.find(key))
+            return (int64_t)*w;
         if (rows == 0 || K == 0)
             return -1;
         const uint64_t last =
@@ -128,14 +128,29 @@ static int pk_pack_ops(b2g_plan p, PkPlan *k) {
 
 static int pk_build_into(b2g_plan p, PkPlan *k) {
     SetupTimer tm("packed plan");
+    tile_env_refresh();
+    const bool group_b = pk_group_b();
     Packer L, R, X;
     L.src_len = p->lbuf->n;
     R.src_len = p->rbuf->n;
     X.src_len = p->x_len;
+    {
+        size_t n1 = 0, n2 = 0;
+        for (const Chunk &ch : p->chunks)
+            n1 += ch.s1.size(), n2 += ch.s2.size();
+        X.where.reserve(n1);
+        R.where.reserve(n1);
+        L.where.reserve(n2);
+        X.mats.reserve(n1);
+        R.mats.reserve(n1);
+        L.mats.reserve(n2);
+    }
     std::vector<HostChunk> hc(p->chunks.size());
     for (size_t ic = 0; ic < p->chunks.size(); ic++) {
         const Chunk &ch = p->chunks[ic];
         HostChunk &h = hc[ic];
+        h.s1.reserve(ch.s1.size());
+        h.s2.reserve(ch.s2.size());
         // rows / columns of each segment's output block
         std::vector<uint32_t> m1(ch.s1.size(), 0), n1(ch.s1.size(), 0);
         std::vector<uint64_t> w1(ch.s1.size(), 0);
@@ -154,8 +169,10 @@ static int pk_build_into(b2g_plan p, PkPlan *k) {
             for (uint32_t s = j.sb; s < j.se; s++)
                 m2[s] = j.M;
         }
+        tm.mark("segment shapes");
         // packed W^T of each phase-1 segment, keyed by the plan's W offset
-        std::unordered_map<uint64_t, std::pair<uint64_t, uint32_t>> wmap;
+        FlatMap<uint64_t, std::pair<uint64_t, uint32_t>, U64Hash> wmap(
+            ch.s1.size());
         std::vector<uint64_t> wpk(ch.s1.size(), 0);
         uint64_t wlen = 0;
         for (size_t si = 0; si < ch.s1.size(); si++) {
@@ -166,7 +183,8 @@ static int pk_build_into(b2g_plan p, PkPlan *k) {
             B2G_REQUIRE((uint64_t)n1[si] * ldw < ((uint64_t)1 << 28),
                         B2G_ERR_ARG, "packed plan: W block of 2^28 elements");
             wpk[si] = wlen;
-            wmap[w1[si]] = std::make_pair(wlen, ldw);
+            *wmap.emplace(w1[si], std::make_pair(wlen, ldw)).first =
+                std::make_pair(wlen, ldw);
             wlen += (uint64_t)n1[si] * ldw;
             const int64_t a = X.add(s.a_off, s.lda, false, m1[si], s.K);
             const int64_t b = R.add(s.b_off, s.ldb, !(s.flags & 2), n1[si], s.K);
@@ -178,6 +196,7 @@ static int pk_build_into(b2g_plan p, PkPlan *k) {
             q.flags = 0, q.alpha = 1.0;
             h.s1.push_back(q);
         }
+        tm.mark("phase-1 pack lists");
         // phase 1 re-tiled with the split-BF16 tile rates: one W^T block
         // per segment
         for (size_t si = 0; si < ch.s1.size(); si++)
@@ -185,10 +204,11 @@ static int pk_build_into(b2g_plan p, PkPlan *k) {
                 h.t1[cfg_of(t)].push_back(
                     make_task(wpk[si], (uint32_t)round8(m1[si]), m1[si],
                               n1[si], t, (uint32_t)si, (uint32_t)si + 1, 0));
+        tm.mark("phase-1 tiles");
         for (size_t si = 0; si < ch.s2.size(); si++) {
             const Seg &s = ch.s2[si];
-            auto it = wmap.find(s.b_off);
-            B2G_REQUIRE(it != wmap.end() && it->second.second == round8(s.K),
+            const std::pair<uint64_t, uint32_t> *wp = wmap.find(s.b_off);
+            B2G_REQUIRE(wp != nullptr && wp->second == round8(s.K),
                         B2G_ERR_STATE, "packed plan: phase-2 W not found");
             B2G_REQUIRE(m2[si] > 0, B2G_ERR_STATE,
                         "packed plan: phase-2 segment without a task");
@@ -197,11 +217,11 @@ static int pk_build_into(b2g_plan p, PkPlan *k) {
                         "packed plan: phase-2 operand outside its buffer");
             Seg q = s;
             q.a_off = (uint64_t)a, q.lda = (uint32_t)round8(s.K);
-            q.b_off = it->second.first, q.ldb = it->second.second;
+            q.b_off = wp->first, q.ldb = wp->second;
             q.flags = 0;
             h.s2.push_back(q);
         }
-        tm.mark("segments and pack lists");
+        tm.mark("phase-2 pack lists");
         // phase 2 re-tiled likewise, block by block, with split-K partial
         // slots and reductions of its own
         for (const BlockJob &j : ch.jobs2)
@@ -212,7 +232,7 @@ static int pk_build_into(b2g_plan p, PkPlan *k) {
         // re-reads); the reductions address partial slots by task offsets,
         // which reordering keeps
         for (int c = 0; c < kTileConfigs; c++) {
-            h.x1[c] = xcd_order(h.t1[c], h.s1, true, pk_group_b());
+            h.x1[c] = xcd_order(h.t1[c], h.s1, true, group_b);
             h.x2[c] = xcd_order(h.t2[c], h.s2, true);
         }
         k->w_max = std::max(k->w_max, wlen);
@@ -220,6 +240,14 @@ static int pk_build_into(b2g_plan p, PkPlan *k) {
     }
     tm.mark("task order");
     k->l_len = L.len, k->r_len = R.len, k->x_len = X.len;
+    if (getenv("B2G_TIME_SETUP") != nullptr)
+        fprintf(stderr,
+                "b2g setup packed plan: %zu/%zu/%zu matrices, %llu/%llu/%llu "
+                "packed elements (L/R/psi), sources %zu/%zu\n",
+                L.mats.size(), R.mats.size(), X.mats.size(),
+                (unsigned long long)L.len, (unsigned long long)R.len,
+                (unsigned long long)X.len, (size_t)L.src_len,
+                (size_t)R.src_len);
     // one allocation for every task, segment and pack-tile array
     size_t bytes = 0;
     for (auto &h : hc) {
@@ -233,13 +261,16 @@ static int pk_build_into(b2g_plan p, PkPlan *k) {
                     X.ntiles < (1ull << 32) && L.mats.size() < (1u << 31) &&
                     R.mats.size() < (1u << 31) && X.mats.size() < (1u << 31),
                 B2G_ERR_ARG, "packed plan: too many pack tiles");
-    std::vector<char> host(bytes + 16);
+    char *host = nullptr;
+    const int rs = ctx_stage(p->ctx, bytes + 16, &host);
+    if (rs != B2G_OK)
+        return rs;
     B2G_HIP(hipMalloc(&k->dev, bytes + 16));
     size_t off = 0;
     auto put = [&](const void *src, size_t nb) {
         const size_t o = off;
         if (nb)
-            memcpy(host.data() + o, src, nb);
+            memcpy(host + o, src, nb);
         off += nb;
         return (char *)k->dev + o;
     };
@@ -272,7 +303,7 @@ static int pk_build_into(b2g_plan p, PkPlan *k) {
     k->n_ltiles = (uint32_t)L.ntiles;
     k->n_rtiles = (uint32_t)R.ntiles;
     k->n_xtiles = (uint32_t)X.ntiles;
-    B2G_HIP(hipMemcpyAsync(k->dev, host.data(), bytes, hipMemcpyHostToDevice,
+    B2G_HIP(hipMemcpyAsync(k->dev, host, bytes, hipMemcpyHostToDevice,
                            p->ctx->stream));
     B2G_HIP(hipStreamSynchronize(p->ctx->stream));
     tm.mark("upload");

@@ -83,9 +83,13 @@ namespace b2g {
 struct Tile {
     uint32_t row0, col0, tm, tn;
 };
+// reads the tiling settings of the environment (B2G_TILE, B2G_KSPLIT) for
+// the plans this host thread builds next; every plan builder calls it first
+void tile_env_refresh();
 // tiling of an M x N output block for tasks of depth K (measured tile
-// rates, or B2G_TILE)
-std::vector<Tile> tile_cover(uint32_t M, uint32_t N, uint64_t K, int table);
+// rates, or B2G_TILE); the reference stays valid until the next call
+const std::vector<Tile> &tile_cover(uint32_t M, uint32_t N, uint64_t K,
+                                    int table);
 Task make_task(uint64_t d_off, uint32_t ldd, uint32_t M, uint32_t N,
                const Tile &t, uint32_t sb, uint32_t se, uint32_t beta);
 int cfg_of(const Tile &t);
@@ -111,6 +115,9 @@ XcdSplit xcd_order(std::vector<Task> &tasks, const std::vector<Seg> &segs,
 uint64_t w_cap_elems();
 // allocate the H*psi scratch for n_rhs at precision ahead of a capture
 int plan_reserve(b2g_plan p, int n_rhs, int precision);
+// the context's pinned staging buffer, at least `bytes` long (host uploads
+// are written there, copied on the context stream and waited for)
+int ctx_stage(b2g_ctx ctx, size_t bytes, char **out);
 
 // Split-BF16 replay on packed operand planes (pkplan.hip).  Derived from the
 // plan's chunks: phase 1 reads the packed wavefunction windows and packed

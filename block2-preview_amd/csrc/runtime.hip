@@ -12,6 +12,7 @@
 #include <string>
 #include <limits>
 #include <map>
+#include <memory>
 #include <sstream>
 #include <tuple>
 #include <unordered_map>
@@ -129,6 +130,8 @@ extern "C" int b2g_ctx_destroy(b2g_ctx ctx) {
     hipEventDestroy(ctx->fork);
     hipFree(ctx->red);
     hipHostFree(ctx->red_host);
+    if (ctx->stage)
+        hipHostFree(ctx->stage);
     hipStreamDestroy(ctx->stream);
     delete ctx;
     return B2G_OK;
@@ -295,16 +298,20 @@ static std::atomic<uint64_t> g_rates_gen{1};
 struct TileMemo {
     uint64_t gen = 0;
     std::unordered_map<uint64_t, std::vector<Tile>> map;
+    std::vector<Tile> big; // blocks too large for the memo key
 };
 static thread_local TileMemo t_memo;
 
-std::vector<Tile> tile_cover(uint32_t M, uint32_t N, uint64_t K, int table) {
-    (void)K;
-    const double *rates = g_rates[table == 1 ? 1 : 0];
-    const char *e = getenv("B2G_TILE");
-    std::vector<uint32_t> rows, cols;
-    uint32_t fm = 0, fn = 0;
-    if (e != nullptr) {
+// tiling settings of the environment, read once per plan build
+struct TileEnv {
+    uint32_t fm = 0, fn = 0; // forced tile shape (0: measured rates)
+    uint64_t ksplit = 2048;  // split-K depth threshold of phase 2
+};
+static thread_local TileEnv t_env;
+
+void tile_env_refresh() {
+    TileEnv v;
+    if (const char *e = getenv("B2G_TILE")) {
         // "T" (square) or "TMxTN", sizes from {32, 64, 128}
         unsigned a = 0, b = 0;
         const int nr = sscanf(e, "%ux%u", &a, &b);
@@ -312,8 +319,20 @@ std::vector<Tile> tile_cover(uint32_t M, uint32_t N, uint64_t K, int table) {
             b = a;
         auto okv = [](unsigned t) { return t == 32 || t == 64 || t == 128; };
         if (nr >= 1 && okv(a) && okv(b))
-            fm = a, fn = b;
+            v.fm = a, v.fn = b;
     }
+    if (const char *e = getenv("B2G_KSPLIT"))
+        if (atoll(e) > 0)
+            v.ksplit = (uint64_t)atoll(e);
+    t_env = v;
+}
+
+const std::vector<Tile> &tile_cover(uint32_t M, uint32_t N, uint64_t K,
+                                    int table) {
+    (void)K;
+    const double *rates = g_rates[table == 1 ? 1 : 0];
+    std::vector<uint32_t> rows, cols;
+    const uint32_t fm = t_env.fm, fn = t_env.fn;
     if (t_memo.gen != g_rates_gen) {
         t_memo.map.clear();
         t_memo.gen = g_rates_gen;
@@ -354,8 +373,9 @@ std::vector<Tile> tile_cover(uint32_t M, uint32_t N, uint64_t K, int table) {
         r0 += a;
     }
     if (memo)
-        t_memo.map.emplace(key, out);
-    return out;
+        return t_memo.map.emplace(key, std::move(out)).first->second;
+    t_memo.big.swap(out);
+    return t_memo.big;
 }
 
 Task make_task(uint64_t d_off, uint32_t ldd, uint32_t M, uint32_t N,
@@ -385,14 +405,6 @@ void add_tiles(std::vector<Task> *tasks, uint64_t d_off, uint32_t M,
         tasks[cfg_of(t)].push_back(make_task(d_off, N, M, N, t, sb, se, beta));
 }
 
-// maximum depth of one phase-2 task before its segment list is split
-static uint64_t ksplit() {
-    const char *e = getenv("B2G_KSPLIT");
-    if (e != nullptr && atoll(e) > 0)
-        return (uint64_t)atoll(e);
-    return 2048;
-}
-
 // Phase-2 tiles of an output block whose segments [sb, se) are deep: the
 // segment list is cut at segment boundaries into groups of depth about
 // ksplit(); each (tile, group) becomes a partial task writing a dense
@@ -402,7 +414,8 @@ void add_split_tiles(const std::vector<Seg> &s2, const BlockJob &j, int table,
                      std::vector<Task> *t2, std::vector<RedTask> &red,
                      uint64_t &p_len) {
     std::vector<std::pair<uint32_t, uint32_t>> groups;
-    const uint64_t kmax = ksplit();
+    // maximum depth of one phase-2 task before its segment list is split
+    const uint64_t kmax = t_env.ksplit;
     // depth as the kernels walk it: every segment takes whole k-steps
     // (16 deep for the FP64 / FP32 kernels, 32 for the split-BF16 ones), so
     // that a long list of shallow segments is split like a deep one
@@ -419,7 +432,7 @@ void add_split_tiles(const std::vector<Seg> &s2, const BlockJob &j, int table,
         }
     }
     groups.emplace_back(g0, j.se);
-    const std::vector<Tile> tiles =
+    const std::vector<Tile> &tiles =
         tile_cover(j.M, j.N, K / groups.size(), table);
     if (groups.size() == 1) {
         for (const Tile &t : tiles)
@@ -473,17 +486,43 @@ double task_flops(const std::vector<Task> &tasks,
     return f;
 }
 
+// Sorts (key, index) pairs given in ascending index order by key, equal
+// keys in index order: LSD radix sort over 16-bit digits for keys below
+// 2^32 (two linear passes), std::sort otherwise.
+void sort_keyed(std::vector<std::pair<uint64_t, size_t>> &v) {
+    uint64_t mx = 0;
+    for (const auto &e : v)
+        mx = std::max(mx, e.first);
+    if (v.size() < 4096 || mx >= ((uint64_t)1 << 32)) {
+        std::sort(v.begin(), v.end());
+        return;
+    }
+    std::vector<std::pair<uint64_t, size_t>> tmp(v.size());
+    std::vector<size_t> cnt(65537);
+    for (int sh = 0; sh < 32 && (mx >> sh) != 0; sh += 16) {
+        std::fill(cnt.begin(), cnt.end(), 0);
+        for (const auto &e : v)
+            cnt[((e.first >> sh) & 0xFFFF) + 1]++;
+        for (size_t d = 0; d < 65536; d++)
+            cnt[d + 1] += cnt[d];
+        for (const auto &e : v)
+            tmp[cnt[(e.first >> sh) & 0xFFFF]++] = e;
+        v.swap(tmp);
+    }
+}
+
+// deepest first, creation order among equal depths
 void sort_tasks(std::vector<Task> &tasks, const std::vector<Seg> &segs) {
     const std::vector<uint64_t> d = task_depths(tasks, segs);
-    std::vector<size_t> order(tasks.size());
+    const uint64_t dmax =
+        d.empty() ? 0 : *std::max_element(d.begin(), d.end());
+    std::vector<std::pair<uint64_t, size_t>> order(tasks.size());
     for (size_t i = 0; i < order.size(); i++)
-        order[i] = i;
-    std::stable_sort(order.begin(), order.end(),
-                     [&d](size_t a, size_t b) { return d[a] > d[b]; });
-    std::vector<Task> out;
-    out.reserve(tasks.size());
-    for (size_t i : order)
-        out.push_back(tasks[i]);
+        order[i] = std::make_pair(dmax - d[i], i);
+    sort_keyed(order);
+    std::vector<Task> out(tasks.size());
+    for (size_t i = 0; i < order.size(); i++)
+        out[i] = tasks[order[i].second];
     tasks.swap(out);
 }
 
@@ -512,35 +551,40 @@ XcdSplit xcd_order(std::vector<Task> &tasks, const std::vector<Seg> &segs,
         sort_tasks(tasks, segs);
         return xs;
     }
-    // groups: runs of equal seg_begin after a stable sort of the task
-    // indices (creation order kept inside a group)
-    std::vector<size_t> idx(tasks.size());
-    for (size_t i = 0; i < idx.size(); i++)
-        idx[i] = i;
-    // group key: the segment list, or (by_b) the B operand of the first
-    // segment, so that every task reading one B block runs on one XCD
-    std::vector<uint64_t> key(tasks.size());
-    for (size_t i = 0; i < tasks.size(); i++)
-        key[i] = by_b ? segs[tasks[i].seg_begin].b_off : tasks[i].seg_begin;
-    std::stable_sort(idx.begin(), idx.end(),
-                     [&](size_t a, size_t b) { return key[a] < key[b]; });
+    // groups: runs of equal key after sorting (key, index) pairs, so that
+    // creation order is kept inside a group.  Group key: the segment list,
+    // or (by_b) the B operand of the first segment, so that every task
+    // reading one B block runs on one XCD
+    std::vector<std::pair<uint64_t, size_t>> ki(tasks.size());
+    bool sorted = true;
+    for (size_t i = 0; i < tasks.size(); i++) {
+        ki[i] = std::make_pair(
+            by_b ? segs[tasks[i].seg_begin].b_off : tasks[i].seg_begin, i);
+        sorted = sorted && (i == 0 || ki[i - 1].first <= ki[i].first);
+    }
+    if (!sorted)
+        sort_keyed(ki);
     const std::vector<uint64_t> depth = task_depths(tasks, segs);
     std::vector<size_t> gstart;
-    std::vector<double> gcost;
-    for (size_t k = 0; k < idx.size(); k++) {
-        if (k == 0 || key[idx[k]] != key[idx[k - 1]]) {
+    std::vector<uint64_t> gcost;
+    for (size_t k = 0; k < ki.size(); k++) {
+        if (k == 0 || ki[k].first != ki[k - 1].first) {
             gstart.push_back(k);
-            gcost.push_back(0.0);
+            gcost.push_back(0);
         }
-        gcost.back() += (double)depth[idx[k]];
+        gcost.back() += depth[ki[k].second];
     }
-    gstart.push_back(idx.size());
+    gstart.push_back(ki.size());
     const size_t ng = gcost.size();
+    // costliest group first, lower group index among equal costs
+    const uint64_t cmax = *std::max_element(gcost.begin(), gcost.end());
+    std::vector<std::pair<uint64_t, size_t>> gord(ng);
+    for (size_t g = 0; g < ng; g++)
+        gord[g] = std::make_pair(cmax - gcost[g], g);
+    sort_keyed(gord);
     std::vector<size_t> order(ng);
     for (size_t g = 0; g < ng; g++)
-        order[g] = g;
-    std::stable_sort(order.begin(), order.end(),
-                     [&](size_t a, size_t b) { return gcost[a] > gcost[b]; });
+        order[g] = gord[g].second;
     // longest group first to the least-loaded XCD
     double load[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     std::vector<int> gx(ng);
@@ -550,7 +594,7 @@ XcdSplit xcd_order(std::vector<Task> &tasks, const std::vector<Seg> &segs,
         for (int y = 1; y < 8; y++)
             if (load[y] < load[x])
                 x = y;
-        load[x] += gcost[g];
+        load[x] += (double)gcost[g];
         gx[g] = x;
         cnt[x] += gstart[g + 1] - gstart[g];
     }
@@ -564,7 +608,7 @@ XcdSplit xcd_order(std::vector<Task> &tasks, const std::vector<Seg> &segs,
     std::vector<Task> out(tasks.size());
     for (size_t g : order)
         for (size_t k = gstart[g]; k < gstart[g + 1]; k++)
-            out[pos[gx[g]]++] = tasks[idx[k]];
+            out[pos[gx[g]]++] = tasks[ki[k].second];
     xs.off[8] = (uint32_t)acc;
     tasks.swap(out);
     return xs;
@@ -580,9 +624,26 @@ uint64_t w_cap_elems() {
     return (uint64_t)1 << 30; // 1 Gi elements (8 GiB FP64) per right-hand side
 }
 
+int ctx_stage(b2g_ctx ctx, size_t bytes, char **out) {
+    if (bytes > ctx->stage_len) {
+        B2G_HIP(hipStreamSynchronize(ctx->stream));
+        if (ctx->stage) {
+            B2G_HIP(hipHostFree(ctx->stage));
+            ctx->stage = nullptr;
+            ctx->stage_len = 0;
+        }
+        const size_t mib = (size_t)1 << 20;
+        const size_t len =
+            (std::max(bytes, 2 * ctx->stage_len) + mib - 1) / mib * mib;
+        B2G_HIP(hipHostMalloc((void **)&ctx->stage, len));
+        ctx->stage_len = len;
+    }
+    *out = ctx->stage;
+    return B2G_OK;
+}
+
 // upload all chunks' task / segment / reduction arrays in one allocation
 static int upload_chunks(b2g_plan p) {
-    // upload all task/segment arrays in one allocation
     size_t bytes = 0;
     for (auto &ch : p->chunks) {
         for (int c = 0; c < kTileConfigs; c++)
@@ -590,13 +651,16 @@ static int upload_chunks(b2g_plan p) {
         bytes += (ch.s1.size() + ch.s2.size()) * sizeof(Seg);
         bytes += ch.red.size() * sizeof(RedTask);
     }
-    std::vector<char> host(bytes + 16);
+    char *host = nullptr;
+    const int rs = ctx_stage(p->ctx, bytes + 16, &host);
+    if (rs != B2G_OK)
+        return rs;
     B2G_HIP(hipMalloc(&p->dev, bytes + 16));
     size_t off = 0;
     auto put = [&](const void *src, size_t nb) -> size_t {
         size_t o = off;
         if (nb)
-            memcpy(host.data() + o, src, nb);
+            memcpy(host + o, src, nb);
         off += nb;
         return o;
     };
@@ -615,7 +679,7 @@ static int upload_chunks(b2g_plan p) {
         o.red = put(ch.red.data(), ch.red.size() * sizeof(RedTask));
         offs.push_back(o);
     }
-    B2G_HIP(hipMemcpyAsync(p->dev, host.data(), bytes, hipMemcpyHostToDevice,
+    B2G_HIP(hipMemcpyAsync(p->dev, host, bytes, hipMemcpyHostToDevice,
                            p->ctx->stream));
     B2G_HIP(hipStreamSynchronize(p->ctx->stream));
     char *base = (char *)p->dev;
@@ -635,50 +699,56 @@ static int upload_chunks(b2g_plan p) {
 static int compile_plan(b2g_plan p, const b2g_pair *pairs, size_t n_pairs,
                         const b2g_block *yb, size_t n_yb) {
     SetupTimer tm("plan");
+    tile_env_refresh();
     const size_t llen = p->lbuf->n, rlen = p->rbuf->n;
-    std::unordered_map<uint64_t, size_t> yidx;
+    FlatMap<uint64_t, size_t, U64Hash> yidx(n_yb);
     std::vector<std::pair<uint64_t, size_t>> ysorted; // (offset, block)
     for (size_t i = 0; i < n_yb; i++) {
         B2G_REQUIRE(yb[i].off + (uint64_t)yb[i].m * yb[i].n <= p->y_len,
                     B2G_ERR_ARG, "output block outside the output vector");
-        yidx[yb[i].off] = i;
+        *yidx.emplace(yb[i].off, i).first = i;
         ysorted.emplace_back(yb[i].off, i);
     }
     std::sort(ysorted.begin(), ysorted.end());
     p->macs = 0;
     std::vector<size_t> whole, windows; // pair indices by kind
+    // output block of each whole-block pair (by pair index)
+    std::vector<uint32_t> yof(n_pairs, 0);
     for (size_t i = 0; i < n_pairs; i++) {
         const b2g_pair &q = pairs[i];
-        std::ostringstream ss;
-        ss << "pair " << i << ": ";
+        // the message is only built for a failing check
+        auto msg = [i](const char *what) {
+            return "pair " + std::to_string(i) + ": " + what;
+        };
         B2G_REQUIRE(q.kind == 0 || q.kind == 1, B2G_ERR_ARG,
-                    ss.str() + "unsupported kind");
+                    msg("unsupported kind"));
         B2G_REQUIRE(q.a_off + (uint64_t)q.c_m * q.v_m <= llen, B2G_ERR_ARG,
-                    ss.str() + "A block outside the L buffer");
+                    msg("A block outside the L buffer"));
         B2G_REQUIRE(q.b_off + (uint64_t)q.c_n * q.v_n <= rlen, B2G_ERR_ARG,
-                    ss.str() + "B block outside the R buffer");
+                    msg("B block outside the R buffer"));
         const uint64_t cld = q.c_ld ? q.c_ld : q.c_n;
         const uint64_t vld = q.v_ld ? q.v_ld : q.v_n;
         B2G_REQUIRE(cld >= q.c_n && vld >= q.v_n, B2G_ERR_ARG,
-                    ss.str() + "row stride below the window width");
+                    msg("row stride below the window width"));
         // the kernels address each operand block with 32-bit byte offsets
         const uint64_t lim = ((uint64_t)1 << 31) / 8 - 2;
         B2G_REQUIRE((uint64_t)q.c_m * cld < lim && (uint64_t)q.v_m * vld < lim &&
                         (uint64_t)q.c_m * q.v_m < lim &&
                         (uint64_t)q.c_n * q.v_n < lim &&
                         (uint64_t)q.c_m * q.v_n < lim,
-                    B2G_ERR_ARG, ss.str() + "block of 2^28 elements or more");
+                    B2G_ERR_ARG, msg("block of 2^28 elements or more"));
         B2G_REQUIRE(q.c_m == 0 || q.c_n == 0 ||
                         q.c_off + (q.c_m - 1) * cld + q.c_n <= p->x_len,
-                    B2G_ERR_ARG, ss.str() + "C block outside the input vector");
+                    B2G_ERR_ARG, msg("C block outside the input vector"));
         if (q.kind == 0) {
             B2G_REQUIRE(q.c_ld == 0 && q.v_ld == 0, B2G_ERR_ARG,
-                        ss.str() + "whole-block pair with a row stride");
-            auto it = yidx.find(q.v_off);
-            B2G_REQUIRE(it != yidx.end(), B2G_ERR_ARG,
-                        ss.str() + "V block is not an output block");
-            B2G_REQUIRE(yb[it->second].m == q.v_m && yb[it->second].n == q.v_n,
-                        B2G_ERR_ARG, ss.str() + "V block shape mismatch");
+                        msg("whole-block pair with a row stride"));
+            const size_t *yi = yidx.find(q.v_off);
+            B2G_REQUIRE(yi != nullptr, B2G_ERR_ARG,
+                        msg("V block is not an output block"));
+            B2G_REQUIRE(yb[*yi].m == q.v_m && yb[*yi].n == q.v_n, B2G_ERR_ARG,
+                        msg("V block shape mismatch"));
+            yof[i] = (uint32_t)*yi;
             whole.push_back(i);
         } else {
             // the window lies inside one output block, rows of that block
@@ -686,12 +756,12 @@ static int compile_plan(b2g_plan p, const b2g_pair *pairs, size_t n_pairs,
                 ysorted.begin(), ysorted.end(),
                 std::make_pair(q.v_off, std::numeric_limits<size_t>::max()));
             B2G_REQUIRE(it != ysorted.begin(), B2G_ERR_ARG,
-                        ss.str() + "V window before the first output block");
+                        msg("V window before the first output block"));
             const b2g_block &bl = yb[(it - 1)->second];
             const uint64_t rel = q.v_off - bl.off;
             B2G_REQUIRE(bl.n > 0 && vld == bl.n && rel % bl.n + q.v_n <= bl.n &&
                             rel / bl.n + q.v_m <= bl.m,
-                        B2G_ERR_ARG, ss.str() + "V window outside its block");
+                        B2G_ERR_ARG, msg("V window outside its block"));
             windows.push_back(i);
         }
         p->macs += (uint64_t)q.c_m * q.c_n * q.v_n +
@@ -723,20 +793,28 @@ static int compile_plan(b2g_plan p, const b2g_pair *pairs, size_t n_pairs,
                     return (size_t)h;
                 }
             };
-            std::unordered_map<WKey, uint64_t, WHash> wmap; // -> W offset
-            std::unordered_map<size_t, uint64_t> pw;  // pair -> W offset
+            FlatMap<WKey, uint64_t, WHash> wmap(idx.size() - j0); // -> W offset
+            // W offset of each pair of the chunk (by position in idx; none
+            // for a pair that contributes nothing)
+            const uint64_t kNone = ~(uint64_t)0;
+            std::vector<uint64_t> pw;
+            pw.reserve(idx.size() - j0);
             size_t j1 = j0;
             for (; j1 < idx.size(); j1++) {
                 const b2g_pair &q = pairs[idx[j1]];
-                if (q.c_m == 0 || q.c_n == 0 || q.v_m == 0 || q.v_n == 0)
-                    continue; // contributes nothing
+                if (q.c_m == 0 || q.c_n == 0 || q.v_m == 0 || q.v_n == 0) {
+                    pw.push_back(kNone); // contributes nothing
+                    continue;
+                }
                 WKey key(q.b_off, q.c_off, q.conj & 2, q.c_m, q.c_n, q.c_ld);
-                auto it = wmap.find(key);
-                if (it == wmap.end()) {
+                const uint64_t *wo = wmap.find(key);
+                uint64_t w_off;
+                if (wo == nullptr) {
                     const uint64_t sz = (uint64_t)q.c_m * q.v_n;
                     if (ch.w_len + sz > cap && j1 > j0)
                         break;
-                    it = wmap.emplace(key, ch.w_len).first;
+                    w_off = ch.w_len;
+                    wmap.emplace(key, w_off);
                     // phase 1: W = C * op(B)
                     Seg s;
                     s.a_off = q.c_off;
@@ -752,54 +830,77 @@ static int compile_plan(b2g_plan p, const b2g_pair *pairs, size_t n_pairs,
                     add_tiles(ch.t1, ch.w_len, q.c_m, q.v_n, q.c_n, si, si + 1,
                               0);
                     ch.w_len += sz;
-                }
-                pw[idx[j1]] = it->second;
-            }
-            // phase 2: group this chunk's pairs by output block or window
-            // (record order within a group)
-            typedef std::tuple<uint64_t, uint32_t, uint32_t, uint32_t> VKey;
-            std::map<VKey, std::vector<size_t>> byv;
-            for (size_t j = j0; j < j1; j++) {
-                const b2g_pair &q = pairs[idx[j]];
-                if (pw.count(idx[j]) == 0)
-                    continue;
-                byv[VKey(q.v_off, q.v_m, q.v_n, q.v_ld ? q.v_ld : q.v_n)]
-                    .push_back(idx[j]);
+                } else
+                    w_off = *wo;
+                pw.push_back(w_off);
             }
             auto emit = [&](uint64_t off, uint32_t m, uint32_t n, uint32_t ldd,
-                            const std::vector<size_t> *lst, uint32_t beta) {
+                            const size_t *lst, size_t nl, uint32_t beta) {
                 const uint32_t sb = (uint32_t)ch.s2.size();
-                if (lst != nullptr)
-                    for (size_t i : *lst) {
-                        const b2g_pair &q = pairs[i];
-                        Seg s;
-                        s.a_off = q.a_off;
-                        const bool ta = q.conj & 1;
-                        s.lda = ta ? q.v_m : q.c_m;
-                        s.b_off = pw[i];
-                        s.ldb = q.v_n;
-                        s.K = q.c_m;
-                        s.flags = ta ? 1u : 0u;
-                        s.alpha = q.alpha;
-                        ch.s2.push_back(s);
-                    }
+                for (size_t t = 0; t < nl; t++) {
+                    const size_t j = lst[t];
+                    const b2g_pair &q = pairs[idx[j]];
+                    Seg s;
+                    s.a_off = q.a_off;
+                    const bool ta = q.conj & 1;
+                    s.lda = ta ? q.v_m : q.c_m;
+                    s.b_off = pw[j - j0];
+                    s.ldb = q.v_n;
+                    s.K = q.c_m;
+                    s.flags = ta ? 1u : 0u;
+                    s.alpha = q.alpha;
+                    ch.s2.push_back(s);
+                }
                 const BlockJob job{off, ldd, m, n, sb,
                                    (uint32_t)ch.s2.size(), beta};
                 ch.jobs2.push_back(job);
                 add_split_tiles(ch.s2, job, 0, ch.t2, ch.red, ch.p_len);
             };
+            // phase 2: this chunk's pairs grouped by output block or window,
+            // record order within a group
+            std::vector<size_t> js; // positions in idx, grouped
+            js.reserve(j1 - j0);
             if (first) {
-                for (size_t b = 0; b < n_yb; b++) {
-                    auto it = byv.find(VKey(yb[b].off, yb[b].m, yb[b].n,
-                                            yb[b].n));
+                // whole blocks: counting sort by output block, then one job
+                // per output block (zero where no pair lands)
+                std::vector<size_t> start(n_yb + 1, 0);
+                for (size_t j = j0; j < j1; j++)
+                    if (pw[j - j0] != kNone)
+                        start[yof[idx[j]] + 1]++;
+                for (size_t b = 0; b < n_yb; b++)
+                    start[b + 1] += start[b];
+                js.resize(start[n_yb]);
+                std::vector<size_t> fill(start.begin(), start.end() - 1);
+                for (size_t j = j0; j < j1; j++)
+                    if (pw[j - j0] != kNone)
+                        js[fill[yof[idx[j]]]++] = j;
+                for (size_t b = 0; b < n_yb; b++)
                     emit(yb[b].off, yb[b].m, yb[b].n, yb[b].n,
-                         it == byv.end() ? nullptr : &it->second, 0);
-                }
+                         js.data() + start[b], start[b + 1] - start[b], 0);
             } else {
-                for (auto &kv : byv)
-                    emit(std::get<0>(kv.first), std::get<1>(kv.first),
-                         std::get<2>(kv.first), std::get<3>(kv.first),
-                         &kv.second, 1);
+                // windows: ordered by (offset, rows, columns, stride)
+                typedef std::tuple<uint64_t, uint32_t, uint32_t, uint32_t> VKey;
+                auto vkey = [&](size_t j) {
+                    const b2g_pair &q = pairs[idx[j]];
+                    return VKey(q.v_off, q.v_m, q.v_n, q.v_ld ? q.v_ld : q.v_n);
+                };
+                std::vector<std::pair<VKey, size_t>> kv;
+                kv.reserve(j1 - j0);
+                for (size_t j = j0; j < j1; j++)
+                    if (pw[j - j0] != kNone)
+                        kv.emplace_back(vkey(j), j);
+                std::sort(kv.begin(), kv.end());
+                for (auto &e : kv)
+                    js.push_back(e.second);
+                for (size_t g0 = 0; g0 < kv.size();) {
+                    size_t g1 = g0 + 1;
+                    while (g1 < kv.size() && kv[g1].first == kv[g0].first)
+                        g1++;
+                    const VKey &k = kv[g0].first;
+                    emit(std::get<0>(k), std::get<1>(k), std::get<2>(k),
+                         std::get<3>(k), js.data() + g0, g1 - g0, 1);
+                    g0 = g1;
+                }
             }
             tm.mark("chunk tasks");
             for (int c = 0; c < kTileConfigs; c++) {
@@ -1238,48 +1339,93 @@ extern "C" int b2g_diag(b2g_ctx ctx, b2g_buf lbuf, b2g_buf rbuf,
     B2G_REQUIRE(terms != nullptr || n_terms == 0, B2G_ERR_ARG, "NULL terms");
     B2G_REQUIRE(lbuf->dtype == B2G_F64 && rbuf->dtype == B2G_F64, B2G_ERR_ARG,
                 "operator buffers must be FP64");
-    std::unordered_map<uint64_t, size_t> didx;
+    FlatMap<uint64_t, size_t, U64Hash> didx(n_dblocks);
     for (size_t i = 0; i < n_dblocks; i++) {
         B2G_REQUIRE(dblocks[i].off + (uint64_t)dblocks[i].m * dblocks[i].n <=
                         d_len,
                     B2G_ERR_ARG, "diagonal block outside the output");
-        didx[dblocks[i].off] = i;
+        *didx.emplace(dblocks[i].off, i).first = i;
     }
-    std::vector<std::vector<size_t>> by(n_dblocks);
+    // Work pieces: each block is cut into runs of `rows` rows (about 2048
+    // elements), so that large blocks spread over many workgroups; a piece
+    // lists the block's terms whose row window meets it, in term order (the
+    // sum of every element runs over the same terms in the same order as
+    // for the whole block).
+    std::vector<uint32_t> rows(n_dblocks), pfirst(n_dblocks + 1, 0);
+    for (size_t b = 0; b < n_dblocks; b++) {
+        const uint32_t n = std::max<uint32_t>(dblocks[b].n, 1);
+        rows[b] = std::max<uint32_t>(1, 2048 / n);
+        pfirst[b + 1] = pfirst[b] + (dblocks[b].m + rows[b] - 1) / rows[b];
+    }
+    const size_t npieces = pfirst[n_dblocks];
+    std::vector<uint32_t> tblock(n_terms);
+    std::vector<size_t> cnt(npieces + 1, 0);
     for (size_t i = 0; i < n_terms; i++) {
         const b2g_diag_term &t = terms[i];
-        auto it = didx.find(t.d_off);
-        B2G_REQUIRE(it != didx.end(), B2G_ERR_ARG,
+        const size_t *bi = didx.find(t.d_off);
+        B2G_REQUIRE(bi != nullptr, B2G_ERR_ARG,
                     "diagonal term targets no output block");
-        const b2g_block &blk = dblocks[it->second];
+        const b2g_block &blk = dblocks[*bi];
         B2G_REQUIRE((uint64_t)t.r0 + t.a_n <= blk.m &&
                         (uint64_t)t.c0 + t.b_n <= blk.n,
                     B2G_ERR_ARG, "diagonal term window outside its block");
         B2G_REQUIRE(t.a_off + (uint64_t)t.a_n * t.a_n <= lbuf->n &&
                         t.b_off + (uint64_t)t.b_n * t.b_n <= rbuf->n,
                     B2G_ERR_ARG, "diagonal term outside operator buffers");
-        by[it->second].push_back(i);
+        tblock[i] = (uint32_t)*bi;
+        if (t.a_n == 0 || t.b_n == 0)
+            continue;
+        const uint32_t rb = rows[*bi];
+        for (uint32_t q = t.r0 / rb; q <= (t.r0 + t.a_n - 1) / rb; q++)
+            cnt[pfirst[*bi] + q + 1]++;
     }
-    std::vector<DiagBlock> blocks(n_dblocks);
-    std::vector<DiagTermDev> dterms;
-    dterms.reserve(n_terms);
-    for (size_t b = 0; b < n_dblocks; b++) {
-        blocks[b].d_off = dblocks[b].off;
-        blocks[b].m = dblocks[b].m;
-        blocks[b].n = dblocks[b].n;
-        blocks[b].t_begin = (uint32_t)dterms.size();
-        for (size_t i : by[b]) {
-            DiagTermDev t;
-            t.a_off = terms[i].a_off;
-            t.b_off = terms[i].b_off;
-            t.a_n = terms[i].a_n;
-            t.b_n = terms[i].b_n;
-            t.r0 = terms[i].r0;
-            t.c0 = terms[i].c0;
-            t.alpha = terms[i].alpha;
-            dterms.push_back(t);
+    for (size_t q = 0; q < npieces; q++)
+        cnt[q + 1] += cnt[q];
+    std::vector<DiagBlock> blocks(npieces);
+    for (size_t b = 0; b < n_dblocks; b++)
+        for (uint32_t q = 0; q < pfirst[b + 1] - pfirst[b]; q++) {
+            DiagBlock &pc = blocks[pfirst[b] + q];
+            pc.row0 = q * rows[b];
+            pc.m = std::min<uint32_t>(rows[b], dblocks[b].m - pc.row0);
+            pc.n = dblocks[b].n;
+            pc.d_off = dblocks[b].off + (uint64_t)pc.row0 * dblocks[b].n;
+            pc.t_begin = (uint32_t)cnt[pfirst[b] + q];
+            pc.t_end = (uint32_t)cnt[pfirst[b] + q + 1];
+            pc.pad = 0;
         }
-        blocks[b].t_end = (uint32_t)dterms.size();
+    B2G_REQUIRE(cnt[npieces] < ((uint64_t)1 << 32), B2G_ERR_ARG,
+                "diagonal: too many term pieces");
+    std::vector<DiagTermDev> dterms(cnt[npieces]);
+    std::vector<size_t> fill(cnt.begin(), cnt.end() - 1);
+    for (size_t i = 0; i < n_terms; i++) {
+        const b2g_diag_term &t = terms[i];
+        if (t.a_n == 0 || t.b_n == 0)
+            continue;
+        DiagTermDev d;
+        d.a_off = t.a_off;
+        d.b_off = t.b_off;
+        d.a_n = t.a_n;
+        d.b_n = t.b_n;
+        d.r0 = t.r0;
+        d.c0 = t.c0;
+        d.alpha = t.alpha;
+        const uint32_t b = tblock[i], rb = rows[b];
+        for (uint32_t q = t.r0 / rb; q <= (t.r0 + t.a_n - 1) / rb; q++)
+            dterms[fill[pfirst[b] + q]++] = d;
+    }
+    // costliest pieces first for the workgroups' grid-stride loop
+    std::vector<std::pair<uint64_t, size_t>> cost(npieces);
+    for (size_t q = 0; q < npieces; q++)
+        cost[q] = std::make_pair(
+            ~((uint64_t)blocks[q].m * blocks[q].n *
+              (blocks[q].t_end - blocks[q].t_begin + 1)),
+            q);
+    std::sort(cost.begin(), cost.end());
+    {
+        std::vector<DiagBlock> sorted(npieces);
+        for (size_t q = 0; q < npieces; q++)
+            sorted[q] = blocks[cost[q].second];
+        blocks.swap(sorted);
     }
     B2G_HIP(hipSetDevice(ctx->device));
     const size_t nb1 = blocks.size() * sizeof(DiagBlock),
