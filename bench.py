@@ -303,6 +303,44 @@ def main():
                     tag, emx, ndmx, tmx, t64 / tmx, tmc, t64c / tmc,
                     abs(emx - e64), max(0.0, 1.0 - ov)))
         extra["davidson"] = dav
+        # FP64 truncation of the converged FP64 wavefunction on the device
+        # (density matrix, block eigensolver, selection, split), forward
+        # direction, k = M states, as a sweep does after the solve
+        from b2g import truncate as TR
+        o, n = int(pb.ket.off[0]), int(pb.ket.n[0])
+        kinfo = TR.BlockInfo(pb.ket.quanta[o:o + n], pb.ket.nbra[o:o + n],
+                             pb.ket.nket[o:o + n], pb.ket.shift[o:o + n],
+                             int(pb.ket.dq[0]))
+
+        def trunc():
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            sp = TR.split_density_matrix(ctx, 1, kinfo, x64, args.m, 1e-14,
+                                         True, True)
+            torch.cuda.synchronize()
+            return sp, time.perf_counter() - t
+        trunc()
+        sp, t_split = trunc()
+        dmi, dmd = sp.dm
+        mats = [dmd[int(dmi.shift[i]):int(dmi.shift[i]) +
+                    int(dmi.nbra[i]) ** 2].view(int(dmi.nbra[i]), -1)
+                for i in range(len(dmi))]
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        _, _, sweeps = TR.eigh_batched(ctx, mats)
+        torch.cuda.synchronize()
+        t_eig = time.perf_counter() - t
+        extra["truncation"] = dict(
+            k=args.m, cutoff=1e-14, s=round(t_split, 4),
+            s_eigensolver=round(t_eig, 4), jacobi_sweeps=sweeps,
+            error=sp.error, dm_blocks=len(dmi),
+            dm_states=int(dmi.nbra.sum()), max_dm_block=int(dmi.nbra.max()),
+            kept=int(sp.left[0].nket.sum()))
+        log("truncation (k=%d): %.4fs (eigensolver %.4fs, %d sweeps), "
+            "%d density-matrix blocks, %d states, largest %d, error %.3e" % (
+                args.m, t_split, t_eig, sweeps, len(dmi),
+                int(dmi.nbra.sum()), int(dmi.nbra.max()), sp.error))
+        trunc_host = (kinfo, x64.cpu().numpy())
 
     cpu = None
     if not args.no_cpu:
@@ -310,6 +348,21 @@ def main():
         cpu = cpu_baseline(pb, ldev.cpu().numpy(), rdev.cpu().numpy(),
                            args.cpu_seconds, threads)
         log("CPU baseline:", cpu)
+        if "truncation" in extra:
+            # the same truncation by the oracle on the host (numpy / LAPACK)
+            import truncate_oracle
+            ki, xh = trunc_host
+            oi = dict(quanta=ki.quanta, nbra=ki.nbra, nket=ki.nket,
+                      shift=ki.shift, dq=ki.dq)
+            t = time.perf_counter()
+            r = truncate_oracle.split_density_matrix(1, oi, xh, args.m, 1e-14,
+                                                     True, True, 0)
+            t_cpu = time.perf_counter() - t
+            extra["truncation"]["cpu_oracle_s"] = round(t_cpu, 4)
+            extra["truncation"]["abs_error_vs_cpu"] = abs(r["error"] -
+                                                          extra["truncation"]["error"])
+            log("truncation on the host (numpy/LAPACK, %d threads): %.4fs" % (
+                threads, t_cpu))
 
     out = {
         "metric": METRIC,

@@ -181,6 +181,19 @@ _SIGS = {
     "b2g_effham_eigs": (C.c_int, [_vp, _vp, C.c_int, C.c_double, C.c_int,
                                   C.c_int, C.c_int, C.c_int, C.c_int, f64p,
                                   C.POINTER(C.c_int), u64p, f64p]),
+    "b2g_syev_batched": (C.c_int, [_vp, C.c_int, u32p, u64p, _vp, _vp, _vp,
+                                   C.POINTER(C.c_int)]),
+    "b2g_split_density_matrix": (C.c_int, [_vp, C.c_int,
+                                           C.POINTER(b2g_sparse_info), _vp,
+                                           C.c_int, C.c_double, C.c_int,
+                                           C.c_int, C.c_uint, C.c_double,
+                                           C.POINTER(_vp)]),
+    "b2g_split_result": (C.c_int, [_vp, C.c_int, C.POINTER(b2g_sparse_info),
+                                   C.POINTER(_vp), C.POINTER(C.c_size_t)]),
+    "b2g_split_error": (C.c_int, [_vp, f64p]),
+    "b2g_split_values": (C.c_int, [_vp, C.c_int, f64p,
+                                   C.POINTER(C.c_size_t)]),
+    "b2g_split_destroy": (C.c_int, [_vp]),
 }
 
 _lib = None

@@ -15,7 +15,8 @@ OBJ = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 SOURCES = ["kernels.hip", "gemm_bf16x3.hip", "pkplan.hip", "davidson_dev.hip",
-           "runtime.hip", "grad.hip", "effham.cpp", "plan.cpp"]
+           "runtime.hip", "grad.hip", "truncate.hip", "effham.cpp",
+           "plan.cpp"]
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     "-Wno-unused-result", "-Wno-unused-value",

@@ -416,6 +416,61 @@ int b2g_effham_eigs(b2g_effham h, double *x, int k, double conv_thrd,
                     int refine, double *energies, int *ndav,
                     uint64_t *nflop, double *tdav);
 
+/* ------------------------------------------------------------ truncation */
+/* Batched symmetric eigensolver (GMatrixFunctions::eigs,
+ * src/core/matrix_functions.hpp:1672-1688: xsyev on each block).  Matrix i
+ * of order n[i] (at most 65536) is read row-major from a + off[i] (device,
+ * not modified).  Its eigenvalues go to w (device, compact: matrix i from
+ * sum_{j<i} n[j]) in ascending order, its eigenvectors to v + off[i] as
+ * rows (row j belongs to eigenvalue j), each signed so that its
+ * largest-magnitude component is positive.  FP64 block Jacobi on the
+ * device; sweeps (may be NULL) receives the number of Jacobi sweeps.     */
+int b2g_syev_batched(b2g_ctx ctx, int n_mats, const uint32_t *n,
+                     const uint64_t *off, const double *a, double *w,
+                     double *v, int *sweeps);
+
+/* Truncation types (TruncationTypes, src/dmrg/moving_environment.hpp:51-58):
+ * selection by eigenvalue (PHYSICAL), eigenvalue / multiplicity (REDUCED)
+ * or eigenvalue * multiplicity (REDUCED_INVERSED); SPECTRA_WITH_MULTIPLICITY
+ * repeats each spectrum value by its multiplicity; KEEP_ONE * m keeps up to
+ * m states of every sector beyond the bond dimension.                      */
+#define B2G_TRUNC_PHYSICAL 0u
+#define B2G_TRUNC_REDUCED 1u
+#define B2G_TRUNC_REDUCED_INVERSED 2u
+#define B2G_TRUNC_REAL_DENSITY_MATRIX 4u
+#define B2G_TRUNC_SPECTRA_WITH_MULTIPLICITY 8u
+#define B2G_TRUNC_KEEP_ONE 16u
+
+typedef struct b2g_split_s *b2g_split;
+/* Density-matrix truncation of a two-site wavefunction on the device
+ * (MovingEnvironment::density_matrix + split_density_matrix,
+ * src/dmrg/moving_environment.hpp:3554-3577, 4250-4345; noise 0, one
+ * wavefunction; trans_product scale = `scale`).  psi: device FP64 data of
+ * the block-sparse wavefunction `wfn`.  trace_right != 0 (a forward sweep):
+ * the result is the left rotation matrix (kept eigenvectors as columns)
+ * and the right wavefunction L^T psi; trace_right == 0: the left
+ * wavefunction psi R^T and the right rotation matrix (eigenvectors as
+ * rows).  k: states to keep (-1: all), cutoff on the (reduced) eigenvalue,
+ * normalize: 2-norm scaling of the wavefunction factor.  Eigenvectors are
+ * signed as in b2g_syev_batched.                                           */
+int b2g_split_density_matrix(b2g_ctx ctx, int symmetry,
+                             const b2g_sparse_info *wfn, const double *psi,
+                             int k, double cutoff, int trace_right,
+                             int normalize, unsigned trunc_type, double scale,
+                             b2g_split *out);
+/* Result parts: which 0 = left factor, 1 = right factor, 2 = the density
+ * matrix (before diagonalisation).  The info arrays are owned by the
+ * split; data is device memory owned by the split.                        */
+int b2g_split_result(b2g_split s, int which, b2g_sparse_info *info,
+                     double **data, size_t *len);
+/* Truncation error (sum of the discarded positive eigenvalues).          */
+int b2g_split_error(b2g_split s, double *error);
+/* which 0: all eigenvalues, per density-matrix block in ascending order;
+ * which 1: the wavefunction spectra sqrt(max(0, w)) (store_wfn_spectra).
+ * *n: capacity in, count out (out = NULL queries the count).             */
+int b2g_split_values(b2g_split s, int which, double *out, size_t *n);
+int b2g_split_destroy(b2g_split s);
+
 #ifdef __cplusplus
 }
 #endif

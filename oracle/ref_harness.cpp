@@ -229,6 +229,72 @@ template <typename S> struct Dumper : CallbackKernel {
         dump(h_eff, d, isite);
         ndumped++;
     }
+    // Density-matrix truncation of the Davidson vector by the reference
+    // (MovingEnvironment::density_matrix with noise 0 and
+    // split_density_matrix with normalize, as the sweep calls them,
+    // src/dmrg/sweep_algorithm.hpp:430-433), in three cases:
+    //   sR: trace_right (forward), Physical, k = 2/3 of all states;
+    //   sL: trace_left (backward), Physical, k = 1/2;
+    //   sK: trace_left, Reduced | KeepOne * 1, k = 1/3.
+    // Written per case: the density matrix (info + data, before
+    // diagonalisation), the two factors (info + data), the error, the
+    // wavefunction spectra and the settings.
+    void dump_split(const shared_ptr<SparseMatrix<S, double>> &wfn,
+                    const string &d) const {
+        struct Case {
+            const char *tag;
+            bool trace_right;
+            int num, den;
+            TruncationTypes tt;
+        };
+        const Case cases[3] = {
+            {"sR", true, 2, 3, TruncationTypes::Physical},
+            {"sL", false, 1, 2, TruncationTypes::Physical},
+            {"sK", false, 1, 3,
+             TruncationTypes::Reduced | TruncationTypes::KeepOne}};
+        const double cutoff = 1E-14;
+        for (const Case &c : cases) {
+            const string t = string(c.tag) + "_";
+            auto dm = MovingEnvironment<S, double, double>::density_matrix(
+                S(0), wfn, c.trace_right, 0.0, NoiseTypes::None);
+            InfoPack<S> dp;
+            dp.add(dm->info);
+            dp.write(d, t + "dm");
+            write_npy(d + "/" + t + "dm_data.npy", dm->data, dm->total_memory);
+            int k_total = 0;
+            for (int i = 0; i < dm->info->n; i++)
+                k_total += dm->info->n_states_bra[i];
+            const int k = max(1, k_total * c.num / c.den);
+            shared_ptr<SparseMatrix<S, double>> left, right;
+            vector<double> spectra;
+            const double err =
+                MovingEnvironment<S, double, double>::split_density_matrix(
+                    dm, wfn, k, c.trace_right, true, left, right, cutoff, true,
+                    spectra, c.tt);
+            InfoPack<S> lp, rp;
+            lp.add(left->info);
+            rp.add(right->info);
+            lp.write(d, t + "left");
+            rp.write(d, t + "right");
+            write_npy(d + "/" + t + "left_data.npy", left->data,
+                      left->total_memory);
+            write_npy(d + "/" + t + "right_data.npy", right->data,
+                      right->total_memory);
+            write_npy(d + "/" + t + "spectra.npy", spectra);
+            vector<double> scal = {err, (double)k, cutoff,
+                                   (double)(ubond_t)c.tt,
+                                   c.trace_right ? 1.0 : 0.0};
+            write_npy(d + "/" + t + "settings.npy", scal);
+            // the stacks are LIFO: release as the sweep does
+            // (src/dmrg/sweep_algorithm.hpp:644-657)
+            right->info->deallocate();
+            right->deallocate();
+            left->info->deallocate();
+            left->deallocate();
+            dm->info->deallocate();
+            dm->deallocate();
+        }
+    }
     void dump(const shared_ptr<EffectiveHamiltonian<S, double>> &h, string d,
               int isite) const {
         auto op = h->op;
@@ -417,6 +483,7 @@ template <typename S> struct Dumper : CallbackKernel {
         auto dres = h->eigs(nullptr, false, G.dump_conv, 0.0, 5000, -1, 2, 50,
                             DavidsonTypes::Normal, 0.0);
         vector<double> psi_dav(h->ket->data, h->ket->data + nw);
+        dump_split(h->ket, d);
         memcpy(h->ket->data, psi.data(), nw * sizeof(double));
         write_npy(d + "/psi_dav.npy", psi_dav);
         ofstream m(d + "/meta.txt");

@@ -1,0 +1,211 @@
+"""ORACLE -- TEST INFRASTRUCTURE ONLY.
+
+numpy restatement of block2's density-matrix truncation of a two-site
+wavefunction, the checker for libb2g's b2g_split_density_matrix and
+b2g_syev_batched.  Imported only by tests/ (and bench.py's CPU leg).
+
+Follows:
+  SZLongLong / SU2LongLong get_bra, get_ket, operator-, multiplicity
+      src/core/symmetry.hpp:656-732, 1184-1290
+  SparseMatrixInfo::initialize_dm           src/core/sparse_matrix.hpp:677-724
+  OperatorFunctions::trans_product (noise 0) src/core/operator_functions.hpp:776-835
+  MovingEnvironment::truncate_density_matrix src/dmrg/moving_environment.hpp:3716-3853
+  MovingEnvironment::split_density_matrix    src/dmrg/moving_environment.hpp:4250-4345
+  rotation_matrix_info_from_density_matrix / wavefunction_info_from_density_matrix
+                                             src/dmrg/moving_environment.hpp:4013-4083
+  GMatrixFunctions::eigs (xsyev)             src/core/matrix_functions.hpp:1672-1688
+The eigenproblems go to numpy.linalg.eigh (LAPACK syevd); eigenvalues come
+back ascending like xsyev's.  Eigenvector signs are fixed here by making the
+largest-magnitude component of every vector positive (the first one on
+ties), the convention libb2g documents; the reference leaves them to LAPACK,
+so comparisons with its dumps use sign-free quantities.
+"""
+import numpy as np
+
+SZ, SU2 = 0, 1
+M16 = 0xFFFF
+
+
+def _n(q):
+    return ((q >> 48) & M16) - (1 << 16 if (q >> 63) & 1 else 0)
+
+
+def q_neg(sym, q):
+    """-q: N (and 2Sz for SZ) negated, 2S fields kept."""
+    n = (-_n(q)) & M16
+    if sym == SZ:
+        tz = (q >> 16) & M16
+        tz = (-(tz - (1 << 16) if tz & 0x8000 else tz)) & M16
+        return (n << 48) | (tz << 16) | (q & M16)
+    return (n << 48) | (q & 0xFFFFFFFFFFFF)
+
+
+def q_get_ket(sym, q):
+    """ket part: SU2 copies 2S into the 2S_low field; SZ: q itself."""
+    if sym == SZ:
+        return q
+    return (q & 0xFFFF0000FFFFFFFF) | ((q & 0xFFFF0000) << 16)
+
+
+def q_get_bra(sym, q, dq):
+    """bra part of a composite label for delta quantum dq."""
+    if sym == SZ:
+        n = (_n(q) + _n(dq)) & M16
+        tq, td = (q >> 16) & M16, (dq >> 16) & M16
+        tz = (tq + td) & M16
+        return (n << 48) | (tz << 16) | ((q ^ dq) & M16)
+    n = (_n(q) + _n(dq)) & M16
+    lo = (q >> 32) & M16
+    return (n << 48) | (lo << 32) | (lo << 16) | ((q ^ dq) & M16)
+
+
+def multiplicity(sym, q):
+    return 1 if sym == SZ else ((q >> 16) & M16) + 1
+
+
+def signed_eigh(a):
+    """eigenvalues ascending, eigenvectors as rows, each signed so that its
+    largest-magnitude component (first on ties) is positive"""
+    w, v = np.linalg.eigh(a)
+    v = v.T.copy()
+    for j in range(v.shape[0]):
+        i = int(np.argmax(np.abs(v[j])))
+        if v[j, i] < 0:
+            v[j] = -v[j]
+    return w, v
+
+
+def density_matrix(sym, info, psi, trace_right, scale=1.0):
+    """info: dict(quanta, nbra, nket, shift, dq).  Returns (quanta, dims,
+    blocks) of the density matrix, blocks dense (dim x dim)."""
+    qs = info["quanta"]
+    dq = int(info["dq"])
+    lab = [q_get_bra(sym, int(q), dq) if trace_right
+           else q_neg(sym, q_get_ket(sym, int(q))) for q in qs]
+    uq = sorted(set(lab))
+    dims = [0] * len(uq)
+    for i, l in enumerate(lab):
+        dims[uq.index(l)] = int(info["nbra"][i] if trace_right
+                                else info["nket"][i])
+    blocks = [np.zeros((d, d)) for d in dims]
+    for i, l in enumerate(lab):
+        m, n, o = int(info["nbra"][i]), int(info["nket"][i]), int(info["shift"][i])
+        x = psi[o:o + m * n].reshape(m, n)
+        b = uq.index(l)
+        blocks[b] += scale * (x @ x.T if trace_right else x.T @ x)
+    return uq, dims, blocks
+
+
+def truncate(sym, quanta, w, k, cutoff, trunc_type):
+    """The reference's selection: w = list of ascending eigenvalue arrays per
+    density-matrix block.  Returns (ss, error) with ss the kept (block,
+    index) pairs in (block, index) order."""
+    wr = []
+    for b, wb in enumerate(w):
+        mult = float(multiplicity(sym, quanta[b]))
+        if trunc_type & 1:
+            wr.append(wb * (1.0 / mult))
+        elif trunc_type & 2:
+            wr.append(wb * mult)
+        else:
+            wr.append(wb)
+    ss = [(b, j) for b in range(len(w)) for j in range(len(w[b]))]
+    k_total = len(ss)
+    error = 0.0
+    if k != -1:
+        ss.sort(key=lambda x: -wr[x[0]][x[1]])
+        keep = trunc_type // 16
+        if keep == 0:
+            for i in range(k, k_total):
+                x = w[ss[i][0]][ss[i][1]]
+                if x > 0:
+                    error += x
+            k = min(k, k_total)
+            while k > 1 and wr[ss[k - 1][0]][ss[k - 1][1]] < cutoff:
+                x = w[ss[k - 1][0]][ss[k - 1][1]]
+                if x > 0:
+                    error += x
+                k -= 1
+            ss = ss[:k]
+        else:
+            mask = [0] * len(w)
+            smask = [0] * k_total
+            for i in range(k_total):
+                mask[ss[i][0]] += 1
+                smask[i] = mask[ss[i][0]] > keep
+            for i in range(k, k_total):
+                x = w[ss[i][0]][ss[i][1]]
+                if x > 0 and smask[i]:
+                    error += x
+            k = min(k, k_total)
+            while k > 1 and wr[ss[k - 1][0]][ss[k - 1][1]] < cutoff:
+                x = w[ss[k - 1][0]][ss[k - 1][1]]
+                if x > 0 and smask[k - 1]:
+                    error += x
+                k -= 1
+            ss = ss[:k] + [ss[i] for i in range(k, k_total) if not smask[i]]
+        ss.sort()
+    return ss, error
+
+
+def split_density_matrix(sym, info, psi, k, cutoff, trace_right,
+                         normalize=True, trunc_type=0, scale=1.0):
+    """Returns dict(error, eigvals (list per block), dm=(quanta, dims,
+    blocks), left=(quanta, nbra, nket, data), right=(...), spectra)."""
+    uq, dims, blocks = density_matrix(sym, info, psi, trace_right, scale)
+    eig = [signed_eigh(b) if b.size else (np.zeros(0), np.zeros((0, 0)))
+           for b in blocks]
+    w = [e[0] for e in eig]
+    ss, error = truncate(sym, uq, w, k, cutoff, trunc_type)
+    with_multi = bool(trunc_type & 8)
+    spectra = []
+    for b, wb in enumerate(w):
+        reps = multiplicity(sym, uq[b]) if with_multi else 1
+        for _ in range(reps):
+            spectra += list(np.sqrt(np.maximum(0.0, wb)))
+    ilr, im = [], []
+    for b, _ in ss:
+        if not ilr or ilr[-1] != b:
+            ilr.append(b)
+            im.append(1)
+        else:
+            im[-1] += 1
+    # rotation matrix
+    rq, rnb, rnk, rdata = [], [], [], []
+    iss = 0
+    for i, b in enumerate(ilr):
+        vecs = np.stack([eig[b][1][ss[iss + j][1]] for j in range(im[i])])
+        iss += im[i]
+        rq.append(uq[b])
+        if trace_right:
+            rnb.append(dims[b]), rnk.append(im[i]), rdata.append(vecs.T)
+        else:
+            rnb.append(im[i]), rnk.append(dims[b]), rdata.append(vecs)
+    # wavefunction factor
+    qs = info["quanta"]
+    dq = int(info["dq"])
+    lab = [q_get_bra(sym, int(q), dq) if trace_right
+           else q_neg(sym, q_get_ket(sym, int(q))) for q in qs]
+    wq, wblk = [], []
+    for i, b in enumerate(ilr):
+        for iw in range(len(qs)):
+            if uq.index(lab[iw]) != b:
+                continue
+            m, n, o = (int(info["nbra"][iw]), int(info["nket"][iw]),
+                       int(info["shift"][iw]))
+            x = psi[o:o + m * n].reshape(m, n)
+            wq.append(int(qs[iw]))
+            wblk.append(rdata[i].T @ x if trace_right else x @ rdata[i].T)
+    order = sorted(range(len(wq)), key=lambda j: wq[j])
+    wq = [wq[j] for j in order]
+    wblk = [wblk[j] for j in order]
+    if normalize:
+        nrm = np.sqrt(sum(float((b * b).sum()) for b in wblk))
+        if nrm > 0:
+            wblk = [b / nrm for b in wblk]
+    rot = (rq, rnb, rnk, rdata)
+    wav = (wq, [b.shape[0] for b in wblk], [b.shape[1] for b in wblk], wblk)
+    return dict(error=error, eigvals=w, dm=(uq, dims, blocks),
+                left=rot if trace_right else wav,
+                right=wav if trace_right else rot, spectra=spectra,
+                ss=ss)

@@ -1,0 +1,184 @@
+"""The truncation oracle (oracle/truncate_oracle.py) against the reference.
+
+The golden fixtures hold, for the Davidson vector of each dumped site, the
+reference's own MovingEnvironment::density_matrix and split_density_matrix
+results (oracle/ref_harness.cpp dump_split) in three cases: trace_right
+Physical (sR), trace_left Physical (sL), trace_left Reduced | KeepOne (sK).
+Checked here, on the CPU:
+  * density-matrix structure exact and data to 1e-13 (absolute; the
+    density matrix of a normalised vector has trace 1);
+  * truncation error to 1e-12 and the squared spectra to 1e-13 (absolute;
+    sqrt of eigenvalues near zero is not compared directly);
+  * the factors' block structure exactly, and sign-free quantities of the
+    factors: the truncated wavefunction left * right, block by block, to
+    1e-9 (the rotation basis inside a degenerate kept space is open; the
+    kept space itself is fixed only up to O(eps ||D|| / gap) by the
+    eigenvalue gap at the truncation boundary, weighted by the square root
+    of the boundary eigenvalues: 1e-10-level differences between two
+    correct eigensolvers on the L=16 Hubbard site);
+  * independent of the reference: error + kept weight = trace of the
+    density matrix, orthonormal rotation matrices, eigen residuals of the
+    oracle's eigenpairs, and a planted mistake (a sign error in the
+    selection) that the reference comparison catches.
+"""
+import json
+
+import numpy as np
+import pytest
+
+import truncate_oracle as T
+from conftest import FIXTURES, DELAYED_FIXTURES, fixture_path
+
+CASES = ["sR", "sL", "sK"]
+ALL = FIXTURES + DELAYED_FIXTURES
+
+
+def load(name):
+    z = np.load(fixture_path(name))
+    meta = json.loads(bytes(z["meta_json"]).decode())
+    sym = T.SU2 if meta["sym"] == "su2" else T.SZ
+    info = dict(quanta=z["ket_quanta"], nbra=z["ket_nbra"],
+                nket=z["ket_nket"], shift=z["ket_shift"],
+                dq=int(z["ket_dq"][0]))
+    return z, sym, info
+
+
+def ref_part(z, tag, part):
+    p = tag + "_" + part
+    return (z[p + "_quanta"].astype(np.uint64), z[p + "_nbra"],
+            z[p + "_nket"], z[p + "_shift"], z[p + "_data"])
+
+
+def blocks_of(quanta, nbra, nket, shift, data):
+    return {int(q): data[int(o):int(o) + int(m) * int(n)].reshape(int(m), int(n))
+            for q, m, n, o in zip(quanta, nbra, nket, shift)}
+
+
+def recon(sym, info, res_left, res_right, trace_right):
+    """dense reconstruction of the truncated two-site wavefunction:
+    dict wavefunction quantum -> (n_bra x n_ket) block"""
+    qs = [int(q) for q in info["quanta"]]
+    dq = info["dq"]
+    lab = {q: (T.q_get_bra(sym, q, dq) if trace_right
+               else T.q_neg(sym, T.q_get_ket(sym, q))) for q in qs}
+    rot, wav = (res_left, res_right) if trace_right else (res_right, res_left)
+    out = {}
+    for q, w in wav.items():
+        r = rot[lab[q]]
+        out[q] = r @ w if trace_right else w @ r
+    return out
+
+
+def oracle_parts(res):
+    def as_dict(p):
+        q, nb, nk, data = p
+        return {int(a): np.asarray(d).reshape(m, n)
+                for a, m, n, d in zip(q, nb, nk, data)}
+    return as_dict(res["left"]), as_dict(res["right"])
+
+
+@pytest.mark.parametrize("name", ALL)
+@pytest.mark.parametrize("tag", CASES)
+def test_oracle_split_matches_reference(name, tag):
+    z, sym, info = load(name)
+    if tag + "_settings" not in z.files:
+        pytest.skip("fixture without truncation dump")
+    err_ref, k, cutoff, tt, tr = z[tag + "_settings"]
+    psi = z["psi_dav"]
+    res = T.split_density_matrix(sym, info, psi, int(k), cutoff, bool(tr),
+                                 True, int(tt))
+    # density matrix
+    dq, dn, dnk, dsh, ddata = ref_part(z, tag, "dm")
+    uq, dims, blocks = res["dm"]
+    assert [int(x) for x in dq] == uq and list(dn) == dims
+    for i, b in enumerate(blocks):
+        o = int(dsh[i])
+        ref = ddata[o:o + dims[i] ** 2].reshape(dims[i], dims[i])
+        assert np.abs(b - ref).max(initial=0) < 1e-13, (name, tag, i)
+    # error and spectra
+    assert abs(res["error"] - err_ref) < 1e-12, (name, tag, res["error"], err_ref)
+    sp = np.array(res["spectra"])
+    assert sp.shape == z[tag + "_spectra"].shape
+    assert np.abs(sp ** 2 - z[tag + "_spectra"] ** 2).max(initial=0) < 1e-13
+    # factor structure
+    for part, mine in (("left", res["left"]), ("right", res["right"])):
+        q, nb, nk, _, _ = ref_part(z, tag, part)
+        assert [int(x) for x in q] == list(mine[0]), (name, tag, part)
+        assert list(nb) == list(mine[1]) and list(nk) == list(mine[2])
+    # truncated wavefunction (sign-free)
+    lref = blocks_of(*ref_part(z, tag, "left"))
+    rref = blocks_of(*ref_part(z, tag, "right"))
+    lo, ro = oracle_parts(res)
+    a = recon(sym, info, lref, rref, bool(tr))
+    b = recon(sym, info, lo, ro, bool(tr))
+    assert a.keys() == b.keys()
+    for q in a:
+        assert np.abs(a[q] - b[q]).max(initial=0) < 1e-9, (name, tag, q)
+
+
+@pytest.mark.parametrize("name", ["hub8_su2", "n2_sz", "hub16_sz_m200"])
+def test_oracle_split_invariants(name):
+    z, sym, info = load(name)
+    psi = z["psi_dav"]
+    for tr in (True, False):
+        k_all = sum(int(m if tr else n) for m, n in zip(info["nbra"], info["nket"]))
+        res = T.split_density_matrix(sym, info, psi, max(1, k_all // 3),
+                                     1e-14, tr, False, 0)
+        trace = sum(float(np.trace(b)) for b in res["dm"][2])
+        kept = sum(float(res["eigvals"][b][j]) for b, j in res["ss"])
+        assert abs(res["error"] + kept - trace) < 1e-12
+        # eigenpairs of every block
+        for blk in res["dm"][2]:
+            w, v = T.signed_eigh(blk)
+            assert np.abs(blk @ v.T - v.T * w).max(initial=0) < 1e-12
+            assert np.abs(v @ v.T - np.eye(len(w))).max(initial=0) < 1e-12
+        # rotation matrices have orthonormal kept vectors
+        rot = res["left"] if tr else res["right"]
+        for d in rot[3]:
+            g = d.T @ d if tr else d @ d.T
+            assert np.abs(g - np.eye(g.shape[0])).max(initial=0) < 1e-12
+        # without normalisation, the kept weight is the truncated norm
+        lo, ro = oracle_parts(res)
+        rec = recon(sym, info, lo, ro, tr)
+        assert abs(sum(float((x * x).sum()) for x in rec.values()) - kept) < 1e-12
+
+
+def test_planted_selection_mistake_is_caught():
+    """A selection that keeps the smallest instead of the largest states
+    (a dropped minus sign in the sort) must fail the reference check."""
+    z, sym, info = load("hub8_su2")
+    err_ref, k, cutoff, tt, tr = z["sR_settings"]
+    orig = T.truncate
+
+    def wrong(sym_, quanta, w, k_, cutoff_, tt_):
+        neg = [-x for x in w]
+        ss, _ = orig(sym_, quanta, neg, k_, -1e300, tt_)
+        err = sum(float(w[b][j]) for b in range(len(w))
+                  for j in range(len(w[b])) if (b, j) not in set(ss))
+        return ss, err
+    T.truncate = wrong
+    try:
+        res = T.split_density_matrix(sym, info, z["psi_dav"], int(k), cutoff,
+                                     bool(tr), True, int(tt))
+    finally:
+        T.truncate = orig
+    assert abs(res["error"] - err_ref) > 1e-6
+
+
+def test_label_algebra():
+    """get_bra / get_ket / negation against values the reference's label
+    layout fixes (SU2LongLong: N bits 48-63, 2S_low 32-47, 2S 16-31, pg
+    0-15)."""
+    def su2(n, lo, tw, pg=0):
+        return ((n & 0xFFFF) << 48) | (lo << 32) | (tw << 16) | pg
+    q = su2(4, 1, 3, 2)
+    assert T.q_get_ket(T.SU2, q) == su2(4, 3, 3, 2)
+    assert T.q_get_bra(T.SU2, q, su2(1, 1, 1, 3)) == su2(5, 1, 1, 1)
+    assert T.q_neg(T.SU2, q) == su2(-4, 1, 3, 2)
+    assert T.multiplicity(T.SU2, q) == 4
+
+    def sz(n, tz, pg=0):
+        return ((n & 0xFFFF) << 48) | ((tz & 0xFFFF) << 16) | pg
+    assert T.q_neg(T.SZ, sz(3, -1, 1)) == sz(-3, 1, 1)
+    assert T.q_get_bra(T.SZ, sz(3, -1), sz(1, 1)) == sz(4, 0)
+    assert T.q_get_ket(T.SZ, sz(3, 1)) == sz(3, 1)
