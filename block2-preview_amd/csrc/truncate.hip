@@ -134,23 +134,35 @@ __global__ __launch_bounds__(256) void jacobi_norm_kernel(
         bl.floor_abs = 64.0 * 2.220446049250313e-16 * sqrt(sh[0]);
 }
 
-// One 64 x 64 pair subproblem per workgroup: cyclic Jacobi in LDS until a
-// sweep applies no rotation (at most max_sweeps); writes the accumulated
-// rotation Q and whether it differs from I, and counts rotating pairs.
+// One 64 x 64 pair subproblem per workgroup: cyclic Jacobi in LDS, sweeps
+// until one applies no rotation (at most max_sweeps); writes the
+// accumulated rotation Q and whether it differs from I, and counts rotating
+// pairs.  Thread t owns column (row) t % 64 of pairs t / 64 + 4 i in the
+// row (column) phase; the pairs of a round touch disjoint rows and columns,
+// so each thread loads all its operands before storing.  The 2 x 2 block of
+// each pair, touched only by its own rotation, is written with its exact
+// rotated values in the column phase: three barriers per round.
 __global__ __launch_bounds__(256) void jacobi_pair_kernel(
     const JacBlock *__restrict__ blocks, const JacTask *__restrict__ tasks,
     const double *__restrict__ A, double *__restrict__ Q,
     int *__restrict__ qflag, int round, double tol, int max_sweeps,
     unsigned *__restrict__ nrot) {
+    constexpr int PT = kJB / 4; // pairs per thread and phase
     __shared__ double S[kJS * kJLD];
     __shared__ double U[kJS * kJLD];
     __shared__ double rc[kJB], rs[kJB], rpp[kJB], rqq[kJB];
-    __shared__ int rot[kJB];
-    __shared__ int any_rot;
+    __shared__ int rp[kJB], rq[kJB], rot[kJB];
+    __shared__ int any_rot, need;
+    __shared__ unsigned rmask[2]; // rotating pairs of a round, by parity
     const JacTask tk = tasks[blockIdx.x];
     const JacBlock bl = blocks[tk.blk];
     const uint32_t N = bl.nb * kJB;
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, li = tid & (kJS - 1), k0 = tid >> 6;
+    // rotation test of entry (p, q)
+    auto wants = [&](double apq, double app, double aqq) {
+        return fabs(apq) > bl.floor_abs &&
+               fabs(apq) > tol * sqrt(fabs(app * aqq));
+    };
     int bi, bj;
     rr_pair((int)bl.nb, round % (int)(bl.nb - 1), (int)tk.x, bi, bj);
     for (int e = tid; e < kJS * kJS; e += 256) {
@@ -162,18 +174,27 @@ __global__ __launch_bounds__(256) void jacobi_pair_kernel(
     int rotated = 0;
     for (int sw = 0; sw < max_sweeps; sw++) {
         if (tid == 0)
-            any_rot = 0;
+            any_rot = 0, need = 0;
         __syncthreads();
+        // skip the sweep when no off-diagonal entry passes the test
+        for (int e = tid; e < kJS * kJS; e += 256) {
+            const int i = e / kJS, j = e % kJS;
+            if (i < j && wants(S[i * kJLD + j], S[i * kJLD + i],
+                               S[j * kJLD + j]))
+                need = 1;
+        }
+        __syncthreads();
+        if (!need)
+            break;
         for (int r = 0; r < kJS - 1; r++) {
+            int f = 0;
             if (tid < kJB) {
                 int p, q;
                 rr_pair(kJS, r, tid, p, q);
                 const double apq = S[p * kJLD + q], app = S[p * kJLD + p],
                              aqq = S[q * kJLD + q];
                 double c = 1.0, s = 0.0;
-                int f = 0;
-                if (fabs(apq) > bl.floor_abs &&
-                    fabs(apq) > tol * sqrt(fabs(app * aqq))) {
+                if (wants(apq, app, aqq)) {
                     // symmetric Schur rotation zeroing a_pq
                     const double th = (aqq - app) / (2.0 * apq);
                     double t;
@@ -191,47 +212,73 @@ __global__ __launch_bounds__(256) void jacobi_pair_kernel(
                 }
                 rc[tid] = c;
                 rs[tid] = s;
+                rp[tid] = p;
+                rq[tid] = q;
                 rot[tid] = f;
             }
+            // the angle threads are lanes 0..31 of wave 0: its ballot is the
+            // round's rotation mask.  Slot r & 1 is rewritten two rounds
+            // later, after a barrier every wave must reach once it has read
+            // the slot.
+            const unsigned long long bm = __ballot(f);
+            if (tid == 0)
+                rmask[r & 1] = (unsigned)bm;
             __syncthreads();
-            // rows p, q of every rotating pair (J^T A)
-            for (int e = tid; e < kJB * kJS; e += 256) {
-                const int k = e / kJS, col = e % kJS;
-                if (rot[k]) {
-                    int p, q;
-                    rr_pair(kJS, r, k, p, q);
-                    const double c = rc[k], s = rs[k];
-                    const double a = S[p * kJLD + col], b = S[q * kJLD + col];
-                    S[p * kJLD + col] = c * a - s * b;
-                    S[q * kJLD + col] = s * a + c * b;
+            // a round without rotations leaves S and U as they are
+            if (rmask[r & 1] == 0u)
+                continue;
+            // rows p, q of the rotating pairs, column li
+            {
+                double a[PT], b[PT];
+#pragma unroll
+                for (int i = 0; i < PT; i++) {
+                    const int k = k0 + 4 * i;
+                    if (rot[k]) {
+                        a[i] = S[rp[k] * kJLD + li];
+                        b[i] = S[rq[k] * kJLD + li];
+                    }
+                }
+#pragma unroll
+                for (int i = 0; i < PT; i++) {
+                    const int k = k0 + 4 * i;
+                    if (rot[k]) {
+                        const double c = rc[k], s = rs[k];
+                        S[rp[k] * kJLD + li] = c * a[i] - s * b[i];
+                        S[rq[k] * kJLD + li] = s * a[i] + c * b[i];
+                    }
                 }
             }
             __syncthreads();
-            // columns p, q (A J) and the accumulated rotation (U J)
-            for (int e = tid; e < kJB * kJS; e += 256) {
-                const int k = e / kJS, row = e % kJS;
-                if (rot[k]) {
-                    int p, q;
-                    rr_pair(kJS, r, k, p, q);
-                    const double c = rc[k], s = rs[k];
-                    double a = S[row * kJLD + p], b = S[row * kJLD + q];
-                    S[row * kJLD + p] = c * a - s * b;
-                    S[row * kJLD + q] = s * a + c * b;
-                    a = U[row * kJLD + p], b = U[row * kJLD + q];
-                    U[row * kJLD + p] = c * a - s * b;
-                    U[row * kJLD + q] = s * a + c * b;
+            // columns p, q of the rotating pairs, row li, and U
+            {
+                double a[PT], b[PT], u[PT], v[PT];
+#pragma unroll
+                for (int i = 0; i < PT; i++) {
+                    const int k = k0 + 4 * i;
+                    if (rot[k]) {
+                        a[i] = S[li * kJLD + rp[k]];
+                        b[i] = S[li * kJLD + rq[k]];
+                        u[i] = U[li * kJLD + rp[k]];
+                        v[i] = U[li * kJLD + rq[k]];
+                    }
                 }
-            }
-            __syncthreads();
-            // the 2 x 2 block of each pair is only touched by its own
-            // rotation: set it to the exact result
-            if (tid < kJB && rot[tid]) {
-                int p, q;
-                rr_pair(kJS, r, tid, p, q);
-                S[p * kJLD + q] = 0.0;
-                S[q * kJLD + p] = 0.0;
-                S[p * kJLD + p] = rpp[tid];
-                S[q * kJLD + q] = rqq[tid];
+#pragma unroll
+                for (int i = 0; i < PT; i++) {
+                    const int k = k0 + 4 * i;
+                    if (rot[k]) {
+                        const int p = rp[k], q = rq[k];
+                        const double c = rc[k], s = rs[k];
+                        double np = c * a[i] - s * b[i], nq = s * a[i] + c * b[i];
+                        if (li == p)
+                            np = rpp[k], nq = 0.0;
+                        else if (li == q)
+                            np = 0.0, nq = rqq[k];
+                        S[li * kJLD + p] = np;
+                        S[li * kJLD + q] = nq;
+                        U[li * kJLD + p] = c * u[i] - s * v[i];
+                        U[li * kJLD + q] = s * u[i] + c * v[i];
+                    }
+                }
             }
             __syncthreads();
         }
@@ -251,14 +298,14 @@ __global__ __launch_bounds__(256) void jacobi_pair_kernel(
     }
 }
 
-// C (rows x 64, LDS) = X (rows x 64) * op(Y) (64 x 64), op = I or ^T:
-// thread t owns rows (t / 16) + 16 i, columns 4 (t % 16) .. + 3
-template <int ROWS, bool TY>
+// acc = op(Y) X or X Y for one thread's outputs of a (ROWS x 64) result
+// held in LDS with row stride kJLD: thread t owns rows (t / 16) + 16 i,
+// columns 4 (t % 16) .. + 3.  TL: acc = Y^T X (Y 64 x ROWS), else X Y.
+template <int ROWS, bool TL>
 __device__ __forceinline__ void lds_mm64(const double *X, const double *Y,
-                                         double *C) {
+                                         double (&acc)[ROWS / 16][4]) {
     const int tid = threadIdx.x, tr = tid >> 4, tc = (tid & 15) * 4;
     constexpr int RI = ROWS / 16;
-    double acc[RI][4];
 #pragma unroll
     for (int i = 0; i < RI; i++)
 #pragma unroll
@@ -268,35 +315,33 @@ __device__ __forceinline__ void lds_mm64(const double *X, const double *Y,
         double y[4];
 #pragma unroll
         for (int j = 0; j < 4; j++)
-            y[j] = TY ? Y[(tc + j) * kJLD + k] : Y[k * kJLD + tc + j];
+            y[j] = TL ? X[k * kJLD + tc + j] : Y[k * kJLD + tc + j];
 #pragma unroll
         for (int i = 0; i < RI; i++) {
-            const double x = X[(tr + 16 * i) * kJLD + k];
+            const double x =
+                TL ? Y[k * kJLD + tr + 16 * i] : X[(tr + 16 * i) * kJLD + k];
 #pragma unroll
             for (int j = 0; j < 4; j++)
                 acc[i][j] = fma(x, y[j], acc[i][j]);
         }
     }
-#pragma unroll
-    for (int i = 0; i < RI; i++)
-#pragma unroll
-        for (int j = 0; j < 4; j++)
-            C[(tr + 16 * i) * kJLD + tc + j] = acc[i][j];
 }
 
 // Region update A_PR <- Q_P^T A_PR Q_R (kind 1) and V slice update
-// V[rows, P] <- V[rows, P] Q_P (kind 2) of one round.
+// V[rows, P] <- V[rows, P] Q_P (kind 2) of one round.  Two LDS buffers:
+// the operand and one Q; the product A_PR Q_R goes back into the operand
+// buffer and the final result from registers to memory.
 __global__ __launch_bounds__(256) void jacobi_update_kernel(
     const JacBlock *__restrict__ blocks, const JacTask *__restrict__ tasks,
     double *__restrict__ A, double *__restrict__ V,
     const double *__restrict__ Q, const int *__restrict__ qflag, int round) {
-    __shared__ double lds[3 * kJS * kJLD];
-    double *X = lds, *Y = lds + kJS * kJLD, *Z = lds + 2 * kJS * kJLD;
+    __shared__ double X[kJS * kJLD];
+    __shared__ double Y[kJS * kJLD];
     const JacTask tk = tasks[blockIdx.x];
     const JacBlock bl = blocks[tk.blk];
     const uint32_t N = bl.nb * kJB;
     const int r = round % (int)(bl.nb - 1);
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, tr = tid >> 4, tc = (tid & 15) * 4;
     if (tk.kind == 1) {
         const int fp = qflag[bl.f_off + tk.x], fr = qflag[bl.f_off + tk.y];
         if (!fp && !fr)
@@ -309,60 +354,40 @@ __global__ __launch_bounds__(256) void jacobi_update_kernel(
             X[l * kJLD + c] = A[bl.a_off + (uint64_t)pair_index(pi, pj, l) * N +
                                 pair_index(ri, rj, c)];
         }
-        double *cur = X, *spare = Z;
+        double acc[4][4];
         if (fr) {
             const uint64_t qo = bl.q_off + (uint64_t)tk.y * kJS * kJS;
             for (int e = tid; e < kJS * kJS; e += 256)
                 Y[(e / kJS) * kJLD + (e % kJS)] = Q[qo + e];
             __syncthreads();
-            lds_mm64<kJS, false>(cur, Y, spare); // X Q_R
+            lds_mm64<kJS, false>(X, Y, acc); // X Q_R
             __syncthreads();
-            double *t = cur;
-            cur = spare;
-            spare = t;
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    X[(tr + 16 * i) * kJLD + tc + j] = acc[i][j];
         }
         if (fp) {
             const uint64_t qo = bl.q_off + (uint64_t)tk.x * kJS * kJS;
             for (int e = tid; e < kJS * kJS; e += 256)
                 Y[(e / kJS) * kJLD + (e % kJS)] = Q[qo + e];
             __syncthreads();
-            // T = Q_P^T cur: T[row][col] = sum_k Q_P[k][row] cur[k][col]
-            double *T = spare;
-            {
-                const int tr = tid >> 4, tc = (tid & 15) * 4;
-                double acc[4][4];
-#pragma unroll
-                for (int i = 0; i < 4; i++)
-#pragma unroll
-                    for (int j = 0; j < 4; j++)
-                        acc[i][j] = 0.0;
-                for (int k = 0; k < kJS; k++) {
-                    double y[4];
-#pragma unroll
-                    for (int j = 0; j < 4; j++)
-                        y[j] = cur[k * kJLD + tc + j];
-#pragma unroll
-                    for (int i = 0; i < 4; i++) {
-                        const double q = Y[k * kJLD + tr + 16 * i];
-#pragma unroll
-                        for (int j = 0; j < 4; j++)
-                            acc[i][j] = fma(q, y[j], acc[i][j]);
-                    }
-                }
-#pragma unroll
-                for (int i = 0; i < 4; i++)
-#pragma unroll
-                    for (int j = 0; j < 4; j++)
-                        T[(tr + 16 * i) * kJLD + tc + j] = acc[i][j];
-            }
+            lds_mm64<kJS, true>(X, Y, acc); // Q_P^T X
+        } else {
             __syncthreads();
-            cur = T;
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    acc[i][j] = X[(tr + 16 * i) * kJLD + tc + j];
         }
-        for (int e = tid; e < kJS * kJS; e += 256) {
-            const int l = e / kJS, c = e % kJS;
-            A[bl.a_off + (uint64_t)pair_index(pi, pj, l) * N +
-              pair_index(ri, rj, c)] = cur[l * kJLD + c];
-        }
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                A[bl.a_off + (uint64_t)pair_index(pi, pj, tr + 16 * i) * N +
+                  pair_index(ri, rj, tc + j)] = acc[i][j];
     } else {
         if (!qflag[bl.f_off + tk.x])
             return;
@@ -378,13 +403,14 @@ __global__ __launch_bounds__(256) void jacobi_update_kernel(
         for (int e = tid; e < kJS * kJS; e += 256)
             Y[(e / kJS) * kJLD + (e % kJS)] = Q[qo + e];
         __syncthreads();
-        lds_mm64<kJB, false>(X, Y, Z);
-        __syncthreads();
-        for (int e = tid; e < kJB * kJS; e += 256) {
-            const int l = e / kJS, c = e % kJS;
-            V[bl.a_off + (uint64_t)(row0 + l) * N + pair_index(pi, pj, c)] =
-                Z[l * kJLD + c];
-        }
+        double acc[kJB / 16][4];
+        lds_mm64<kJB, false>(X, Y, acc);
+#pragma unroll
+        for (int i = 0; i < kJB / 16; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                V[bl.a_off + (uint64_t)(row0 + tr + 16 * i) * N +
+                  pair_index(pi, pj, tc + j)] = acc[i][j];
     }
 }
 
@@ -639,15 +665,22 @@ static int syev_run(b2g_ctx ctx, const std::vector<uint32_t> &n,
                            0, st, s.d_blk, s.A);
         B2G_HIP(hipGetLastError());
     }
-    const int max_sweeps = 40;
+    const int max_sweeps = 60;
     const double tol = 2.220446049250313e-16;
+    // cyclic sweeps per pair subproblem and round (B2G_JACOBI_INNER): the
+    // outer iteration converges with partially diagonalised subproblems,
+    // at a fraction of the LDS work per round
+    int inner = 1;
+    if (const char *e = getenv("B2G_JACOBI_INNER"))
+        if (atoi(e) > 0)
+            inner = atoi(e);
     int sweep = 0;
     for (; sweep < max_sweeps; sweep++) {
         B2G_HIP(hipMemsetAsync(s.nrot, 0, sizeof(unsigned), st));
         for (uint32_t r = 0; r + 1 < s.max_nb; r++) {
             hipLaunchKernelGGL(jacobi_pair_kernel, dim3((unsigned)s.n_pair),
                                dim3(256), 0, st, s.d_blk, s.d_pair, s.A, s.Q,
-                               s.qflag, (int)r, tol, 16, s.nrot);
+                               s.qflag, (int)r, tol, inner, s.nrot);
             B2G_HIP(hipGetLastError());
             hipLaunchKernelGGL(jacobi_update_kernel, dim3((unsigned)s.n_upd),
                                dim3(256), 0, st, s.d_blk, s.d_upd, s.A,
@@ -657,6 +690,9 @@ static int syev_run(b2g_ctx ctx, const std::vector<uint32_t> &n,
         B2G_HIP(hipMemcpyAsync(s.h_nrot, s.nrot, sizeof(unsigned),
                                hipMemcpyDeviceToHost, st));
         B2G_HIP(hipStreamSynchronize(st));
+        if (getenv("B2G_JACOBI_TRACE") != nullptr)
+            fprintf(stderr, "b2g jacobi sweep %d: %u rotating pair subproblems\n",
+                    sweep, *s.h_nrot);
         if (*s.h_nrot == 0)
             break;
     }
