@@ -52,9 +52,12 @@ namespace b2g {
 
 namespace {
 
-constexpr int kJB = 32;      // column block of the block Jacobi
+constexpr int kJB = 16;      // column block of the block Jacobi
 constexpr int kJS = 2 * kJB; // order of a pair subproblem
 constexpr int kJLD = kJS + 1;
+constexpr int kJG = 256 / kJS; // thread groups of a pair workgroup
+constexpr int kTC = kJS / 4;   // threads across the columns of a result
+constexpr int kRP = 256 / kTC; // result rows per pass
 
 // one matrix of a batch: A and V are N x N (N = 32 nb) from a_off; the
 // round's Qs are nb / 2 matrices of 64 x 64 from q_off, their flags from
@@ -65,7 +68,7 @@ struct JacBlock {
     double floor_abs; // off-diagonal entries below this are not rotated
 };
 // pair task (kind 0): x = pair index; region task (kind 1): x = row pair,
-// y = column pair; V task (kind 2): x = pair, y = 32-row block
+// y = column pair; V task (kind 2): x = pair, y = slice of kJS rows
 struct JacTask {
     uint32_t blk, kind, x, y;
 };
@@ -134,10 +137,10 @@ __global__ __launch_bounds__(256) void jacobi_norm_kernel(
         bl.floor_abs = 64.0 * 2.220446049250313e-16 * sqrt(sh[0]);
 }
 
-// One 64 x 64 pair subproblem per workgroup: cyclic Jacobi in LDS, sweeps
-// until one applies no rotation (at most max_sweeps); writes the
+// One kJS x kJS pair subproblem per workgroup: cyclic Jacobi in LDS,
+// sweeps until one applies no rotation (at most max_sweeps); writes the
 // accumulated rotation Q and whether it differs from I, and counts rotating
-// pairs.  Thread t owns column (row) t % 64 of pairs t / 64 + 4 i in the
+// pairs.  Thread t owns column (row) t % kJS of pairs t / kJS + kJG i in the
 // row (column) phase; the pairs of a round touch disjoint rows and columns,
 // so each thread loads all its operands before storing.  The 2 x 2 block of
 // each pair, touched only by its own rotation, is written with its exact
@@ -147,7 +150,7 @@ __global__ __launch_bounds__(256) void jacobi_pair_kernel(
     const double *__restrict__ A, double *__restrict__ Q,
     int *__restrict__ qflag, int round, double tol, int max_sweeps,
     unsigned *__restrict__ nrot) {
-    constexpr int PT = kJB / 4; // pairs per thread and phase
+    constexpr int PT = kJB / kJG; // pairs per thread and phase
     __shared__ double S[kJS * kJLD];
     __shared__ double U[kJS * kJLD];
     __shared__ double rc[kJB], rs[kJB], rpp[kJB], rqq[kJB];
@@ -157,7 +160,7 @@ __global__ __launch_bounds__(256) void jacobi_pair_kernel(
     const JacTask tk = tasks[blockIdx.x];
     const JacBlock bl = blocks[tk.blk];
     const uint32_t N = bl.nb * kJB;
-    const int tid = threadIdx.x, li = tid & (kJS - 1), k0 = tid >> 6;
+    const int tid = threadIdx.x, li = tid % kJS, k0 = tid / kJS;
     // rotation test of entry (p, q)
     auto wants = [&](double apq, double app, double aqq) {
         return fabs(apq) > bl.floor_abs &&
@@ -216,7 +219,7 @@ __global__ __launch_bounds__(256) void jacobi_pair_kernel(
                 rq[tid] = q;
                 rot[tid] = f;
             }
-            // the angle threads are lanes 0..31 of wave 0: its ballot is the
+            // the angle threads are lanes 0..kJB-1 of wave 0: its ballot is the
             // round's rotation mask.  Slot r & 1 is rewritten two rounds
             // later, after a barrier every wave must reach once it has read
             // the slot.
@@ -232,7 +235,7 @@ __global__ __launch_bounds__(256) void jacobi_pair_kernel(
                 double a[PT], b[PT];
 #pragma unroll
                 for (int i = 0; i < PT; i++) {
-                    const int k = k0 + 4 * i;
+                    const int k = k0 + kJG * i;
                     if (rot[k]) {
                         a[i] = S[rp[k] * kJLD + li];
                         b[i] = S[rq[k] * kJLD + li];
@@ -240,7 +243,7 @@ __global__ __launch_bounds__(256) void jacobi_pair_kernel(
                 }
 #pragma unroll
                 for (int i = 0; i < PT; i++) {
-                    const int k = k0 + 4 * i;
+                    const int k = k0 + kJG * i;
                     if (rot[k]) {
                         const double c = rc[k], s = rs[k];
                         S[rp[k] * kJLD + li] = c * a[i] - s * b[i];
@@ -254,7 +257,7 @@ __global__ __launch_bounds__(256) void jacobi_pair_kernel(
                 double a[PT], b[PT], u[PT], v[PT];
 #pragma unroll
                 for (int i = 0; i < PT; i++) {
-                    const int k = k0 + 4 * i;
+                    const int k = k0 + kJG * i;
                     if (rot[k]) {
                         a[i] = S[li * kJLD + rp[k]];
                         b[i] = S[li * kJLD + rq[k]];
@@ -264,7 +267,7 @@ __global__ __launch_bounds__(256) void jacobi_pair_kernel(
                 }
 #pragma unroll
                 for (int i = 0; i < PT; i++) {
-                    const int k = k0 + 4 * i;
+                    const int k = k0 + kJG * i;
                     if (rot[k]) {
                         const int p = rp[k], q = rq[k];
                         const double c = rc[k], s = rs[k];
@@ -298,14 +301,14 @@ __global__ __launch_bounds__(256) void jacobi_pair_kernel(
     }
 }
 
-// acc = op(Y) X or X Y for one thread's outputs of a (ROWS x 64) result
-// held in LDS with row stride kJLD: thread t owns rows (t / 16) + 16 i,
-// columns 4 (t % 16) .. + 3.  TL: acc = Y^T X (Y 64 x ROWS), else X Y.
+// acc = op(Y) X or X Y for one thread's outputs of a (ROWS x kJS) result
+// held in LDS with row stride kJLD: thread t owns rows t / kTC + kRP i,
+// columns 4 (t % kTC) .. + 3.  TL: acc = Y^T X (Y kJS x ROWS), else X Y.
 template <int ROWS, bool TL>
-__device__ __forceinline__ void lds_mm64(const double *X, const double *Y,
-                                         double (&acc)[ROWS / 16][4]) {
-    const int tid = threadIdx.x, tr = tid >> 4, tc = (tid & 15) * 4;
-    constexpr int RI = ROWS / 16;
+__device__ __forceinline__ void lds_mm(const double *X, const double *Y,
+                                       double (&acc)[ROWS / kRP][4]) {
+    const int tid = threadIdx.x, tr = tid / kTC, tc = (tid % kTC) * 4;
+    constexpr int RI = ROWS / kRP;
 #pragma unroll
     for (int i = 0; i < RI; i++)
 #pragma unroll
@@ -318,8 +321,8 @@ __device__ __forceinline__ void lds_mm64(const double *X, const double *Y,
             y[j] = TL ? X[k * kJLD + tc + j] : Y[k * kJLD + tc + j];
 #pragma unroll
         for (int i = 0; i < RI; i++) {
-            const double x =
-                TL ? Y[k * kJLD + tr + 16 * i] : X[(tr + 16 * i) * kJLD + k];
+            const double x = TL ? Y[k * kJLD + tr + kRP * i]
+                                : X[(tr + kRP * i) * kJLD + k];
 #pragma unroll
             for (int j = 0; j < 4; j++)
                 acc[i][j] = fma(x, y[j], acc[i][j]);
@@ -341,7 +344,8 @@ __global__ __launch_bounds__(256) void jacobi_update_kernel(
     const JacBlock bl = blocks[tk.blk];
     const uint32_t N = bl.nb * kJB;
     const int r = round % (int)(bl.nb - 1);
-    const int tid = threadIdx.x, tr = tid >> 4, tc = (tid & 15) * 4;
+    const int tid = threadIdx.x, tr = tid / kTC, tc = (tid % kTC) * 4;
+    constexpr int RI = kJS / kRP;
     if (tk.kind == 1) {
         const int fp = qflag[bl.f_off + tk.x], fr = qflag[bl.f_off + tk.y];
         if (!fp && !fr)
@@ -354,47 +358,47 @@ __global__ __launch_bounds__(256) void jacobi_update_kernel(
             X[l * kJLD + c] = A[bl.a_off + (uint64_t)pair_index(pi, pj, l) * N +
                                 pair_index(ri, rj, c)];
         }
-        double acc[4][4];
+        double acc[RI][4];
         if (fr) {
             const uint64_t qo = bl.q_off + (uint64_t)tk.y * kJS * kJS;
             for (int e = tid; e < kJS * kJS; e += 256)
                 Y[(e / kJS) * kJLD + (e % kJS)] = Q[qo + e];
             __syncthreads();
-            lds_mm64<kJS, false>(X, Y, acc); // X Q_R
+            lds_mm<kJS, false>(X, Y, acc); // X Q_R
             __syncthreads();
 #pragma unroll
-            for (int i = 0; i < 4; i++)
+            for (int i = 0; i < RI; i++)
 #pragma unroll
                 for (int j = 0; j < 4; j++)
-                    X[(tr + 16 * i) * kJLD + tc + j] = acc[i][j];
+                    X[(tr + kRP * i) * kJLD + tc + j] = acc[i][j];
         }
         if (fp) {
             const uint64_t qo = bl.q_off + (uint64_t)tk.x * kJS * kJS;
             for (int e = tid; e < kJS * kJS; e += 256)
                 Y[(e / kJS) * kJLD + (e % kJS)] = Q[qo + e];
             __syncthreads();
-            lds_mm64<kJS, true>(X, Y, acc); // Q_P^T X
+            lds_mm<kJS, true>(X, Y, acc); // Q_P^T X
         } else {
             __syncthreads();
 #pragma unroll
-            for (int i = 0; i < 4; i++)
+            for (int i = 0; i < RI; i++)
 #pragma unroll
                 for (int j = 0; j < 4; j++)
-                    acc[i][j] = X[(tr + 16 * i) * kJLD + tc + j];
+                    acc[i][j] = X[(tr + kRP * i) * kJLD + tc + j];
         }
 #pragma unroll
-        for (int i = 0; i < 4; i++)
+        for (int i = 0; i < RI; i++)
 #pragma unroll
             for (int j = 0; j < 4; j++)
-                A[bl.a_off + (uint64_t)pair_index(pi, pj, tr + 16 * i) * N +
+                A[bl.a_off + (uint64_t)pair_index(pi, pj, tr + kRP * i) * N +
                   pair_index(ri, rj, tc + j)] = acc[i][j];
     } else {
         if (!qflag[bl.f_off + tk.x])
             return;
         int pi, pj;
         rr_pair((int)bl.nb, r, (int)tk.x, pi, pj);
-        const uint32_t row0 = tk.y * kJB;
-        for (int e = tid; e < kJB * kJS; e += 256) {
+        const uint32_t row0 = tk.y * kJS;
+        for (int e = tid; e < kJS * kJS; e += 256) {
             const int l = e / kJS, c = e % kJS;
             X[l * kJLD + c] =
                 V[bl.a_off + (uint64_t)(row0 + l) * N + pair_index(pi, pj, c)];
@@ -403,13 +407,13 @@ __global__ __launch_bounds__(256) void jacobi_update_kernel(
         for (int e = tid; e < kJS * kJS; e += 256)
             Y[(e / kJS) * kJLD + (e % kJS)] = Q[qo + e];
         __syncthreads();
-        double acc[kJB / 16][4];
-        lds_mm64<kJB, false>(X, Y, acc);
+        double acc[RI][4];
+        lds_mm<kJS, false>(X, Y, acc);
 #pragma unroll
-        for (int i = 0; i < kJB / 16; i++)
+        for (int i = 0; i < RI; i++)
 #pragma unroll
             for (int j = 0; j < 4; j++)
-                V[bl.a_off + (uint64_t)(row0 + tr + 16 * i) * N +
+                V[bl.a_off + (uint64_t)(row0 + tr + kRP * i) * N +
                   pair_index(pi, pj, tc + j)] = acc[i][j];
     }
 }
@@ -625,7 +629,7 @@ static int syev_run(b2g_ctx ctx, const std::vector<uint32_t> &n,
             for (uint32_t b = 0; b < h; b++)
                 upd.push_back(JacTask{(uint32_t)i, 1, a, b});
         for (uint32_t a = 0; a < h; a++)
-            for (uint32_t rb = 0; rb < s.blk[i].nb; rb++)
+            for (uint32_t rb = 0; rb < h; rb++) // slices of kJS rows
                 upd.push_back(JacTask{(uint32_t)i, 2, a, rb});
     }
     s.n_pair = pairs.size();
