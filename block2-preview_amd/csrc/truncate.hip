@@ -114,18 +114,22 @@ __global__ __launch_bounds__(256) void jacobi_init_kernel(
     }
 }
 
-// floor_abs of each matrix: 64 eps ||A_b||_F (one workgroup per matrix)
+// floor_abs of each matrix: 64 eps ||A_b||_F.  Stage 1: workgroup
+// (chunk, b) sums the squares of rows chunk, chunk + gridDim.x, ... of
+// matrix b into part[b * gridDim.x + chunk]; stage 2 (jacobi_floor_kernel)
+// adds the parts of each matrix in order (deterministic).
 __global__ __launch_bounds__(256) void jacobi_norm_kernel(
-    JacBlock *__restrict__ blocks, const double *__restrict__ A) {
+    const JacBlock *__restrict__ blocks, const double *__restrict__ A,
+    double *__restrict__ part) {
     __shared__ double sh[256];
-    JacBlock &bl = blocks[blockIdx.x];
+    const JacBlock bl = blocks[blockIdx.y];
     const uint32_t N = bl.nb * kJB;
-    const uint64_t NN = (uint64_t)N * N;
     double s = 0.0;
-    for (uint64_t e = threadIdx.x; e < NN; e += 256) {
-        const double v = A[bl.a_off + e];
-        s += v * v;
-    }
+    for (uint32_t i = blockIdx.x; i < N; i += gridDim.x)
+        for (uint32_t j = threadIdx.x; j < N; j += 256) {
+            const double v = A[bl.a_off + (uint64_t)i * N + j];
+            s = fma(v, v, s);
+        }
     sh[threadIdx.x] = s;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
@@ -134,7 +138,19 @@ __global__ __launch_bounds__(256) void jacobi_norm_kernel(
         __syncthreads();
     }
     if (threadIdx.x == 0)
-        bl.floor_abs = 64.0 * 2.220446049250313e-16 * sqrt(sh[0]);
+        part[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = sh[0];
+}
+
+__global__ void jacobi_floor_kernel(JacBlock *__restrict__ blocks, int nblocks,
+                                    const double *__restrict__ part,
+                                    int nparts) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nblocks)
+        return;
+    double s = 0.0;
+    for (int c = 0; c < nparts; c++)
+        s += part[(size_t)b * nparts + c];
+    blocks[b].floor_abs = 64.0 * 2.220446049250313e-16 * sqrt(s);
 }
 
 // One kJS x kJS pair subproblem per workgroup: cyclic Jacobi in LDS,
@@ -419,28 +435,36 @@ __global__ __launch_bounds__(256) void jacobi_update_kernel(
 }
 
 // Eigenvalues as Rayleigh quotients against the original matrix:
-// w[p] = sum_i V[i][p] Y[i][p] / sum_i V[i][p]^2, Y = A_orig V (compact).  The diagonal
-// left by the block updates carries an absolute error of a few eps ||A||
-// per update; the quotient of the converged vector is accurate to
-// O(eps ||A||) with the vector's own error entering quadratically.
+// w[p] = sum_i V[i][p] Y[i][p] / sum_i V[i][p]^2, Y = A_orig V (compact).
+// The diagonal left by the block updates carries an absolute error of a
+// few eps ||A|| per update; the quotient of the converged vector is
+// accurate to O(eps ||A||) with the vector's own error entering
+// quadratically.  Workgroup (g, b): columns 64 g .. 64 g + 63 of matrix b,
+// four row slices per column summed in LDS in a fixed order.
 __global__ __launch_bounds__(256) void rayleigh_kernel(
-    const JacBlock *__restrict__ blocks, int nblocks,
-    const uint64_t *__restrict__ w_off, const uint64_t *__restrict__ y_off,
-    const double *__restrict__ V, const double *__restrict__ Y,
-    double *__restrict__ w) {
-    for (int b = blockIdx.y; b < nblocks; b += gridDim.y) {
-        const JacBlock bl = blocks[b];
-        const uint32_t N = bl.nb * kJB;
-        for (uint32_t p = blockIdx.x * 256 + threadIdx.x; p < bl.n;
-             p += gridDim.x * 256) {
-            double s = 0.0, nn = 0.0;
-            for (uint32_t i = 0; i < bl.n; i++) {
-                const double v = V[bl.a_off + (uint64_t)i * N + p];
-                s = fma(v, Y[y_off[b] + (uint64_t)i * bl.n + p], s);
-                nn = fma(v, v, nn);
-            }
-            w[w_off[b] + p] = s / nn;
+    const JacBlock *__restrict__ blocks, const uint64_t *__restrict__ w_off,
+    const uint64_t *__restrict__ y_off, const double *__restrict__ V,
+    const double *__restrict__ Y, double *__restrict__ w) {
+    __shared__ double sn[4][64], sd[4][64];
+    const int b = blockIdx.y;
+    const JacBlock bl = blocks[b];
+    const uint32_t N = bl.nb * kJB;
+    const uint32_t p = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int sl = threadIdx.x >> 6;
+    double s = 0.0, nn = 0.0;
+    if (p < bl.n)
+        for (uint32_t i = sl; i < bl.n; i += 4) {
+            const double v = V[bl.a_off + (uint64_t)i * N + p];
+            s = fma(v, Y[y_off[b] + (uint64_t)i * bl.n + p], s);
+            nn = fma(v, v, nn);
         }
+    sn[sl][threadIdx.x & 63] = s;
+    sd[sl][threadIdx.x & 63] = nn;
+    __syncthreads();
+    if (sl == 0 && p < bl.n) {
+        const int c = threadIdx.x;
+        w[w_off[b] + p] = ((sn[0][c] + sn[1][c]) + (sn[2][c] + sn[3][c])) /
+                          ((sd[0][c] + sd[1][c]) + (sd[2][c] + sd[3][c]));
     }
 }
 
@@ -619,6 +643,8 @@ static int syev_run(b2g_ctx ctx, const std::vector<uint32_t> &n,
     w_sorted.assign(s.w_len, 0.0);
     if (nm == 0)
         return B2G_OK;
+    B2G_REQUIRE(nm <= 65535, B2G_ERR_ARG,
+                "eigensolver: more than 65535 matrices");
     // task lists: pairs, then regions and V slices
     std::vector<JacTask> pairs, upd;
     for (size_t i = 0; i < nm; i++) {
@@ -665,9 +691,19 @@ static int syev_run(b2g_ctx ctx, const std::vector<uint32_t> &n,
         hipLaunchKernelGGL(jacobi_init_kernel, dim3(gx, gy), dim3(256), 0, st,
                            s.d_blk, (int)nm, s.d_src_off, src, s.A, s.V);
         B2G_HIP(hipGetLastError());
-        hipLaunchKernelGGL(jacobi_norm_kernel, dim3((unsigned)nm), dim3(256),
-                           0, st, s.d_blk, s.A);
+        const unsigned nparts = 32;
+        double *part = nullptr;
+        B2G_HIP(hipMalloc(&part, nm * nparts * sizeof(double)));
+        std::unique_ptr<double, int (*)(void *)> part_guard(
+            part, [](void *p) { return (int)hipFree(p); });
+        hipLaunchKernelGGL(jacobi_norm_kernel, dim3(nparts, (unsigned)nm),
+                           dim3(256), 0, st, s.d_blk, s.A, part);
         B2G_HIP(hipGetLastError());
+        hipLaunchKernelGGL(jacobi_floor_kernel,
+                           dim3((unsigned)((nm + 255) / 256)), dim3(256), 0,
+                           st, s.d_blk, (int)nm, part, (int)nparts);
+        B2G_HIP(hipGetLastError());
+        B2G_HIP(hipStreamSynchronize(st));
     }
     const int max_sweeps = 60;
     const double tol = 2.220446049250313e-16;
@@ -733,11 +769,10 @@ static int syev_run(b2g_ctx ctx, const std::vector<uint32_t> &n,
             return r;
         unsigned gx = 1;
         for (size_t i = 0; i < nm; i++)
-            gx = std::max<unsigned>(gx, (n[i] + 255) / 256);
-        hipLaunchKernelGGL(rayleigh_kernel,
-                           dim3(gx, (unsigned)std::min<size_t>(nm, 65535)),
-                           dim3(256), 0, st, s.d_blk, (int)nm, s.d_w_off,
-                           s.d_src_off, s.V, Y, s.w);
+            gx = std::max<unsigned>(gx, (n[i] + 63) / 64);
+        hipLaunchKernelGGL(rayleigh_kernel, dim3(gx, (unsigned)nm), dim3(256),
+                           0, st, s.d_blk, s.d_w_off, s.d_src_off, s.V, Y,
+                           s.w);
         B2G_HIP(hipGetLastError());
         B2G_HIP(hipStreamSynchronize(st));
     }

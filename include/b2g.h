@@ -418,8 +418,9 @@ int b2g_effham_eigs(b2g_effham h, double *x, int k, double conv_thrd,
 
 /* ------------------------------------------------------------ truncation */
 /* Batched symmetric eigensolver (GMatrixFunctions::eigs,
- * src/core/matrix_functions.hpp:1672-1688: xsyev on each block).  Matrix i
- * of order n[i] (at most 65536) is read row-major from a + off[i] (device,
+ * src/core/matrix_functions.hpp:1672-1688: xsyev on each block).  At most
+ * 65535 matrices; matrix i of order n[i] (at most 65536) is read row-major
+ * from a + off[i] (device,
  * not modified).  Its eigenvalues go to w (device, compact: matrix i from
  * sum_{j<i} n[j]) in ascending order, its eigenvectors to v + off[i] as
  * rows (row j belongs to eigenvalue j), each signed so that its
