@@ -267,10 +267,17 @@ def main():
         # cold: a fresh effective Hamiltonian per solve, as at every site of
         # a sweep (graph capture, workspace and, for split-BF16, the packed
         # operator planes included); warm: a second solve on the same one
+        t_create = {}
+
         def solve(prec, fresh):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
             hh = (EffectiveHamiltonian(pb, ctx, fp32=True,
                                        device_data=(ldev, rdev))
                   if fresh else h)
+            torch.cuda.synchronize()
+            if fresh:
+                t_create[prec] = time.perf_counter() - t0
             x = guess.clone()
             e, nd, _, t = hh.eigs(x, conv_thrd=1e-9, precision=prec,
                                   refine=prec != "f64")
@@ -341,6 +348,32 @@ def main():
                 args.m, t_split, t_eig, sweeps, len(dmi),
                 int(dmi.nbra.sum()), int(dmi.nbra.max()), sp.error))
         trunc_host = (kinfo, x64.cpu().numpy())
+        # one two-site step of a sweep at this site: a fresh effective
+        # Hamiltonian (setup), the cold solve, the truncation; the middle
+        # site is the largest of the chain, so 2 (n - 1) steps bound a
+        # sweep from above
+        steps = 2 * (args.n_orb - 1)
+        site = {}
+        for tag, tc, key in (("fp64", t64c, "f64"),
+                             ("bf16x3", dav["mixed_bf16x3"]["s_cold"],
+                              "bf16x3"),
+                             ("f32", dav["mixed_f32"]["s_cold"], "f32")):
+            st = t_create[key] + tc + t_split
+            site[tag] = dict(setup_s=round(t_create[key], 4),
+                             solve_s=round(tc, 4),
+                             truncation_s=round(t_split, 4),
+                             step_s=round(st, 4),
+                             sweep_upper_bound_s=round(steps * st, 3))
+        site["speedup_bf16x3_vs_fp64"] = round(
+            site["fp64"]["step_s"] / site["bf16x3"]["step_s"], 3)
+        extra["site_step"] = site
+        log("site step (setup + cold solve + truncation): FP64 %.4fs, "
+            "split-BF16+refine %.4fs (%.2fx); sweep bound (%d steps): "
+            "%.2fs / %.2fs" % (site["fp64"]["step_s"],
+                               site["bf16x3"]["step_s"],
+                               site["speedup_bf16x3_vs_fp64"], steps,
+                               site["fp64"]["sweep_upper_bound_s"],
+                               site["bf16x3"]["sweep_upper_bound_s"]))
 
     cpu = None
     if not args.no_cpu:

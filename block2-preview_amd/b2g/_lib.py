@@ -194,6 +194,15 @@ _SIGS = {
     "b2g_split_values": (C.c_int, [_vp, C.c_int, f64p,
                                    C.POINTER(C.c_size_t)]),
     "b2g_split_destroy": (C.c_int, [_vp]),
+    "b2g_smat_load": (C.c_int, [C.c_char_p, C.c_int, C.c_int,
+                                C.POINTER(_vp)]),
+    "b2g_smat_get": (C.c_int, [_vp, C.POINTER(b2g_sparse_info),
+                               C.POINTER(C.c_int), f64p, C.POINTER(f64p),
+                               C.POINTER(C.c_size_t)]),
+    "b2g_smat_save": (C.c_int, [C.c_char_p, C.POINTER(b2g_sparse_info),
+                                C.c_int, C.c_int, C.c_double, f64p,
+                                C.c_size_t]),
+    "b2g_smat_destroy": (C.c_int, [_vp]),
 }
 
 _lib = None

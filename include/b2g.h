@@ -472,6 +472,27 @@ int b2g_split_error(b2g_split s, double *error);
 int b2g_split_values(b2g_split s, int which, double *out, size_t *n);
 int b2g_split_destroy(b2g_split s);
 
+/* ------------------------------------------------------------ file formats */
+/* block2's binary sparse-matrix files (SparseMatrix::save_data /
+ * load_data, src/core/sparse_matrix.hpp:908-989, with the info first when
+ * saved with save_info: SparseMatrixInfo::save_data / load_data, :504-575),
+ * read into and written from host memory.  ubond_size: bytes of block2's
+ * ubond_t (4 in its large-bond build, else 2 or 1).  flags: the info's
+ * last byte, block_shifts_fl_size << 2 | is_wavefunction (block2 writes
+ * 8 << 2 | 1 for an FP64 wavefunction).  Compressed storage is refused.   */
+typedef struct b2g_smat_s *b2g_smat;
+int b2g_smat_load(const char *path, int with_info, int ubond_size,
+                  b2g_smat *out);
+/* info (requires with_info at load), flags, factor, data, len: any may be
+ * NULL; arrays stay owned by the handle.                                  */
+int b2g_smat_get(b2g_smat m, b2g_sparse_info *info, int *flags,
+                 double *factor, const double **data, size_t *len);
+/* info NULL: the matrix part only (save_data without save_info).          */
+int b2g_smat_save(const char *path, const b2g_sparse_info *info, int flags,
+                  int ubond_size, double factor, const double *data,
+                  size_t len);
+int b2g_smat_destroy(b2g_smat m);
+
 #ifdef __cplusplus
 }
 #endif

@@ -14,7 +14,9 @@
 //   * psi, H*psi computed by the reference in SeqTypes::None mode
 //     (operator_functions.hpp:474-542 -> matrix_functions.hpp:998-1009),
 //   * the diagonal (effective_hamiltonian.hpp:190-202),
-//   * a Davidson solve on that site (effective_hamiltonian.hpp:480-568).
+//   * a Davidson solve on that site (effective_hamiltonian.hpp:480-568),
+//     its vector also in block2's sparse-matrix file format, and the
+//     reference's density-matrix truncation of it (dump_split).
 // Output goes to a directory of .npy files plus meta.txt; tests/golden/
 // make_fixtures.py packs them into the committed .npz fixtures.
 // Nothing in the product links or loads this program.
@@ -483,6 +485,9 @@ template <typename S> struct Dumper : CallbackKernel {
         auto dres = h->eigs(nullptr, false, G.dump_conv, 0.0, 5000, -1, 2, 50,
                             DavidsonTypes::Normal, 0.0);
         vector<double> psi_dav(h->ket->data, h->ket->data + nw);
+        // the Davidson vector in block2's own file format (info + data,
+        // SparseMatrix::save_data with save_info)
+        h->ket->save_data(d + "/ket_dav.smat", true);
         dump_split(h->ket, d);
         memcpy(h->ket->data, psi.data(), nw * sizeof(double));
         write_npy(d + "/psi_dav.npy", psi_dav);

@@ -78,6 +78,11 @@ def pack(site_dir, name):
         for fn in sorted(os.listdir(site_dir)):
             if fn.endswith(".npy"):
                 arrs[fn[:-4]] = np.load(os.path.join(site_dir, fn))
+            elif fn.endswith(".smat"):
+                # block2 sparse-matrix files, kept byte for byte
+                with open(os.path.join(site_dir, fn), "rb") as f:
+                    arrs[fn[:-5] + "_smat"] = np.frombuffer(f.read(),
+                                                            np.uint8)
         meta = {}
         with open(os.path.join(site_dir, "meta.txt")) as f:
             for line in f:
