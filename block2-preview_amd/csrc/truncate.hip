@@ -177,10 +177,11 @@ __global__ __launch_bounds__(256) void jacobi_pair_kernel(
     const JacBlock bl = blocks[tk.blk];
     const uint32_t N = bl.nb * kJB;
     const int tid = threadIdx.x, li = tid % kJS, k0 = tid / kJS;
-    // rotation test of entry (p, q)
+    // rotation test of entry (p, q): |a_pq| above the floor and above
+    // tol sqrt(|a_pp a_qq|) (compared squared)
+    const double tol2 = tol * tol;
     auto wants = [&](double apq, double app, double aqq) {
-        return fabs(apq) > bl.floor_abs &&
-               fabs(apq) > tol * sqrt(fabs(app * aqq));
+        return fabs(apq) > bl.floor_abs && apq * apq > tol2 * fabs(app * aqq);
     };
     int bi, bj;
     rr_pair((int)bl.nb, round % (int)(bl.nb - 1), (int)tk.x, bi, bj);
@@ -222,7 +223,7 @@ __global__ __launch_bounds__(256) void jacobi_pair_kernel(
                     else
                         t = (th >= 0.0 ? 1.0 : -1.0) /
                             (fabs(th) + sqrt(th * th + 1.0));
-                    c = 1.0 / sqrt(t * t + 1.0);
+                    c = rsqrt(t * t + 1.0);
                     s = t * c;
                     rpp[tid] = app - t * apq;
                     rqq[tid] = aqq + t * apq;
