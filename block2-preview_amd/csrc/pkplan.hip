@@ -21,6 +21,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <thread>
 #include <tuple>
 #include <unordered_map>
 
@@ -170,13 +171,13 @@ static int pk_build_into(b2g_plan p, PkPlan *k) {
                 m2[s] = j.M;
         }
         tm.mark("segment shapes");
-        // packed W^T of each phase-1 segment, keyed by the plan's W offset
+        // packed W^T layout of the phase-1 segments, keyed by the plan's W
+        // offset (read by the phase-2 pass)
         FlatMap<uint64_t, std::pair<uint64_t, uint32_t>, U64Hash> wmap(
             ch.s1.size());
         std::vector<uint64_t> wpk(ch.s1.size(), 0);
         uint64_t wlen = 0;
         for (size_t si = 0; si < ch.s1.size(); si++) {
-            const Seg &s = ch.s1[si];
             B2G_REQUIRE(m1[si] > 0 && n1[si] > 0, B2G_ERR_STATE,
                         "packed plan: phase-1 segment without a task");
             const uint32_t ldw = (uint32_t)round8(m1[si]);
@@ -186,59 +187,83 @@ static int pk_build_into(b2g_plan p, PkPlan *k) {
             *wmap.emplace(w1[si], std::make_pair(wlen, ldw)).first =
                 std::make_pair(wlen, ldw);
             wlen += (uint64_t)n1[si] * ldw;
-            const int64_t a = X.add(s.a_off, s.lda, false, m1[si], s.K);
-            const int64_t b = R.add(s.b_off, s.ldb, !(s.flags & 2), n1[si], s.K);
-            B2G_REQUIRE(a >= 0 && b >= 0, B2G_ERR_ARG,
-                        "packed plan: phase-1 operand outside its buffer");
-            Seg q = s;
-            q.a_off = (uint64_t)a, q.lda = (uint32_t)round8(s.K);
-            q.b_off = (uint64_t)b, q.ldb = (uint32_t)round8(s.K);
-            q.flags = 0, q.alpha = 1.0;
-            h.s1.push_back(q);
         }
-        tm.mark("phase-1 pack lists");
-        // phase 1 re-tiled with the split-BF16 tile rates: one W^T block
-        // per segment
-        for (size_t si = 0; si < ch.s1.size(); si++)
-            for (const Tile &t : tile_cover(m1[si], n1[si], ch.s1[si].K, 1))
-                h.t1[cfg_of(t)].push_back(
-                    make_task(wpk[si], (uint32_t)round8(m1[si]), m1[si],
-                              n1[si], t, (uint32_t)si, (uint32_t)si + 1, 0));
-        tm.mark("phase-1 tiles");
-        for (size_t si = 0; si < ch.s2.size(); si++) {
-            const Seg &s = ch.s2[si];
-            const std::pair<uint64_t, uint32_t> *wp = wmap.find(s.b_off);
-            B2G_REQUIRE(wp != nullptr && wp->second == round8(s.K),
-                        B2G_ERR_STATE, "packed plan: phase-2 W not found");
-            B2G_REQUIRE(m2[si] > 0, B2G_ERR_STATE,
-                        "packed plan: phase-2 segment without a task");
-            const int64_t a = L.add(s.a_off, s.lda, s.flags & 1, m2[si], s.K);
-            B2G_REQUIRE(a >= 0, B2G_ERR_ARG,
-                        "packed plan: phase-2 operand outside its buffer");
-            Seg q = s;
-            q.a_off = (uint64_t)a, q.lda = (uint32_t)round8(s.K);
-            q.b_off = wp->first, q.ldb = wp->second;
-            q.flags = 0;
-            h.s2.push_back(q);
-        }
-        tm.mark("phase-2 pack lists");
-        // phase 2 re-tiled likewise, block by block, with split-K partial
-        // slots and reductions of its own
-        for (const BlockJob &j : ch.jobs2)
-            add_split_tiles(h.s2, j, 1, h.t2, h.red, h.p_len);
-        tm.mark("phase-2 tiles");
-        // tiles of one output block (one segment list) share their operand
-        // panels: keep them on one XCD, back to back (its L2 serves the
-        // re-reads); the reductions address partial slots by task offsets,
-        // which reordering keeps
-        for (int c = 0; c < kTileConfigs; c++) {
-            h.x1[c] = xcd_order(h.t1[c], h.s1, true, group_b);
+        // The phase-1 operand lists (psi windows and right operators
+        // packed) and the phase-2 ones (left operators) share only the W
+        // layout above, read-only: phase 1's run on a second host thread
+        // while this one builds phase 2's and tiles both phases (the tiling
+        // memo is per host thread and stays warm on this one).
+        const char *err1 = nullptr;
+        std::thread th1([&]() {
+            for (size_t si = 0; si < ch.s1.size(); si++) {
+                const Seg &s = ch.s1[si];
+                const int64_t a = X.add(s.a_off, s.lda, false, m1[si], s.K);
+                const int64_t b =
+                    R.add(s.b_off, s.ldb, !(s.flags & 2), n1[si], s.K);
+                if (a < 0 || b < 0) {
+                    err1 = "packed plan: phase-1 operand outside its buffer";
+                    return;
+                }
+                Seg q = s;
+                q.a_off = (uint64_t)a, q.lda = (uint32_t)round8(s.K);
+                q.b_off = (uint64_t)b, q.ldb = (uint32_t)round8(s.K);
+                q.flags = 0, q.alpha = 1.0;
+                h.s1.push_back(q);
+            }
+        });
+        auto phase2 = [&]() -> const char * {
+            for (size_t si = 0; si < ch.s2.size(); si++) {
+                const Seg &s = ch.s2[si];
+                const std::pair<uint64_t, uint32_t> *wp = wmap.find(s.b_off);
+                if (wp == nullptr || wp->second != round8(s.K))
+                    return "packed plan: phase-2 W not found";
+                if (m2[si] == 0)
+                    return "packed plan: phase-2 segment without a task";
+                const int64_t a =
+                    L.add(s.a_off, s.lda, s.flags & 1, m2[si], s.K);
+                if (a < 0)
+                    return "packed plan: phase-2 operand outside its buffer";
+                Seg q = s;
+                q.a_off = (uint64_t)a, q.lda = (uint32_t)round8(s.K);
+                q.b_off = wp->first, q.ldb = wp->second;
+                q.flags = 0;
+                h.s2.push_back(q);
+            }
+            // one W^T block per phase-1 segment, tiled with the split-BF16
+            // rates (segment K from the plan's own segments)
+            for (size_t si = 0; si < ch.s1.size(); si++)
+                for (const Tile &t :
+                     tile_cover(m1[si], n1[si], ch.s1[si].K, 1))
+                    h.t1[cfg_of(t)].push_back(make_task(
+                        wpk[si], (uint32_t)round8(m1[si]), m1[si], n1[si], t,
+                        (uint32_t)si, (uint32_t)si + 1, 0));
+            // output blocks re-tiled likewise, with split-K partial slots
+            // and reductions of their own
+            for (const BlockJob &j : ch.jobs2)
+                add_split_tiles(h.s2, j, 1, h.t2, h.red, h.p_len);
+            return nullptr;
+        };
+        const char *err2 = phase2();
+        th1.join();
+        B2G_REQUIRE(err1 == nullptr, B2G_ERR_ARG, err1 ? err1 : "");
+        B2G_REQUIRE(err2 == nullptr, B2G_ERR_STATE, err2 ? err2 : "");
+        tm.mark("pack lists and tiles (two threads)");
+        // task order, the two phases on two threads
+        std::thread th2([&]() {
+            for (int c = 0; c < kTileConfigs; c++)
+                h.x1[c] = xcd_order(h.t1[c], h.s1, true, group_b);
+        });
+        for (int c = 0; c < kTileConfigs; c++)
             h.x2[c] = xcd_order(h.t2[c], h.s2, true);
-        }
+        th2.join();
+        tm.mark("task order (two threads)");
+        // (task order: tiles of one output block (one segment list) share
+        // their operand panels and stay on one XCD, back to back, where its
+        // L2 serves the re-reads; the reductions address partial slots by
+        // task offsets, which reordering keeps)
         k->w_max = std::max(k->w_max, wlen);
         k->p_max = std::max(k->p_max, h.p_len);
     }
-    tm.mark("task order");
     k->l_len = L.len, k->r_len = R.len, k->x_len = X.len;
     if (getenv("B2G_TIME_SETUP") != nullptr)
         fprintf(stderr,
