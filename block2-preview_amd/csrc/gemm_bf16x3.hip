@@ -56,51 +56,69 @@ __device__ __forceinline__ u32x4 pk_load(__amdgpu_buffer_rsrc_t r,
 }
 
 // ------------------------------------------------------------------ packing
-// Tile t of a job: its matrix (the last one with tile0 <= t), then rows
-// [32 tr, 32 tr + 32) x k groups [8 tk, 8 tk + 8) of it.  256 threads: one
-// 8-element group each, consecutive lanes along the source's contiguous
-// index.
+// One thread per 8-element group (8 consecutive k of one packed row).  The
+// groups of a job are numbered matrix after matrix from each matrix's g0:
+// row-major (row, k group) when the source runs along k, column-major
+// (k group, row) for a transposed source, so that consecutive lanes read
+// consecutive source addresses.  A workgroup takes 256 * kPackGPT
+// consecutive groups; the group offsets of the matrices they span (at most
+// one matrix per group) are staged in LDS for each thread's binary search.
+constexpr int kPackGPT = 4;
+
 __global__ __launch_bounds__(256) void pack_bf16_kernel(
     const double *__restrict__ src, __bf16 *__restrict__ hi,
     __bf16 *__restrict__ lo, const PackMat *__restrict__ mats, int nmats,
-    uint32_t ntiles, uint64_t src_rs, uint64_t dst_rs) {
+    uint64_t ngroups, uint64_t src_rs, uint64_t dst_rs) {
+    constexpr int G = 256 * kPackGPT;
+    __shared__ uint64_t sg0[G + 1];
+    __shared__ int sm[2];
     const int rhs = blockIdx.y;
     src += rhs * src_rs;
     hi += rhs * dst_rs;
     lo += rhs * dst_rs;
-    // a contiguous range of tiles per workgroup: one binary search for the
-    // first tile's matrix, then a forward walk over the tile0 list
-    const uint64_t per = (ntiles + (uint64_t)gridDim.x - 1) / gridDim.x;
-    const uint64_t b0 = blockIdx.x * per;
-    if (b0 >= ntiles)
+    const uint64_t base = (uint64_t)blockIdx.x * G;
+    if (base >= ngroups)
         return;
-    const uint32_t t_begin = (uint32_t)b0;
-    const uint32_t t_end = (uint32_t)(b0 + per < ntiles ? b0 + per : ntiles);
-    int m = 0, hi_m = nmats - 1; // last matrix with tile0 <= t_begin
-    while (m < hi_m) {
-        const int mid = (m + hi_m + 1) >> 1;
-        if (mats[mid].tile0 <= t_begin)
-            m = mid;
-        else
-            hi_m = mid - 1;
+    const uint64_t last = (base + G < ngroups ? base + G : ngroups) - 1;
+    if (threadIdx.x < 2) {
+        // last matrix with g0 <= base (thread 0) / <= last (thread 1)
+        const uint64_t key = threadIdx.x == 0 ? base : last;
+        int a = 0, b = nmats - 1;
+        while (a < b) {
+            const int mid = (a + b + 1) >> 1;
+            if (mats[mid].g0 <= key)
+                a = mid;
+            else
+                b = mid - 1;
+        }
+        sm[threadIdx.x] = a;
     }
-    for (uint32_t t = t_begin; t < t_end; t++) {
-        while (m + 1 < nmats && mats[m + 1].tile0 <= t)
-            m++;
-        const PackMat pm = mats[m];
-        const uint32_t ktiles = (pm.dst_ld + 63) / 64;
-        const uint32_t local = t - pm.tile0;
-        const uint32_t r0 = (local / ktiles) * 32, k0 = (local % ktiles) * 64;
-        uint32_t r, g;
+    __syncthreads();
+    const int m0 = sm[0], nm = sm[1] - sm[0] + 1; // nm <= G
+    for (int i = threadIdx.x; i < nm; i += 256)
+        sg0[i] = mats[m0 + i].g0;
+    __syncthreads();
+    for (int it = 0; it < kPackGPT; it++) {
+        const uint64_t g = base + (uint64_t)it * 256 + threadIdx.x;
+        if (g > last)
+            break;
+        int a = 0, b = nm - 1;
+        while (a < b) {
+            const int mid = (a + b + 1) >> 1;
+            if (sg0[mid] <= g)
+                a = mid;
+            else
+                b = mid - 1;
+        }
+        const PackMat pm = mats[m0 + a];
+        const uint32_t l = (uint32_t)(g - pm.g0);
+        const uint32_t kg = pm.dst_ld >> 3;
+        uint32_t r, k;
         if (pm.trans)
-            r = threadIdx.x & 31, g = threadIdx.x >> 5;
+            k = (l / pm.rows) * 8, r = l % pm.rows;
         else
-            g = threadIdx.x & 7, r = threadIdx.x >> 3;
-        r += r0;
-        const uint32_t k = k0 + 8 * g;
-        if (r >= pm.rows || k >= pm.dst_ld)
-            continue;
-        bf16x8 h, l;
+            r = l / kg, k = (l % kg) * 8;
+        bf16x8 h, lw;
 #pragma unroll
         for (int i = 0; i < 8; i++) {
             const uint32_t kk = k + i;
@@ -110,24 +128,27 @@ __global__ __launch_bounds__(256) void pack_bf16_kernel(
                                            : (uint64_t)r * pm.src_ld + kk)];
             const __bf16 vh = (__bf16)(float)v;
             h[i] = vh;
-            l[i] = (__bf16)(float)(v - (double)(float)vh);
+            lw[i] = (__bf16)(float)(v - (double)(float)vh);
         }
         const uint64_t o = pm.dst + (uint64_t)r * pm.dst_ld + k;
         *(bf16x8 *)(hi + o) = h;
-        *(bf16x8 *)(lo + o) = l;
+        *(bf16x8 *)(lo + o) = lw;
     }
 }
 
 hipError_t launch_pack_bf16(const double *src, __bf16 *hi, __bf16 *lo,
-                            const PackMat *mats, int nmats, uint32_t ntiles,
+                            const PackMat *mats, int nmats, uint64_t ngroups,
                             uint64_t src_rs, uint64_t dst_rs, int n_rhs,
                             int grid, hipStream_t st) {
-    if (nmats <= 0 || ntiles == 0)
+    (void)grid;
+    if (nmats <= 0 || ngroups == 0)
         return hipSuccess;
-    const uint32_t g = ntiles < (uint32_t)grid * 4 ? ntiles : (uint32_t)grid * 4;
-    hipLaunchKernelGGL(pack_bf16_kernel, dim3(g, (unsigned)n_rhs), dim3(256),
-                       0, st, src, hi, lo, mats, nmats, ntiles, src_rs,
-                       dst_rs);
+    const uint64_t g = (ngroups + 256 * kPackGPT - 1) / (256 * kPackGPT);
+    if (g >= (1ull << 31))
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL(pack_bf16_kernel, dim3((unsigned)g, (unsigned)n_rhs),
+                       dim3(256), 0, st, src, hi, lo, mats, nmats, ngroups,
+                       src_rs, dst_rs);
     return hipGetLastError();
 }
 

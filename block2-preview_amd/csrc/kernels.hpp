@@ -91,12 +91,13 @@ struct RedTask {
 // a rows x dst_ld packed matrix at element offset dst (dst_ld = cols rounded
 // up to 8), element (r, k) = src[src + r*src_ld + k] (trans:
 // src[src + k*src_ld + r]) for k < cols and zero for cols <= k < dst_ld.  It
-// is packed in tiles of 32 rows x 64 k, numbered from tile0 on in row-major
-// tile order; the matrices of a job are sorted by tile0.
+// is packed in groups of 8 consecutive k of one row, rows * dst_ld / 8 of
+// them numbered from g0 on; the matrices of a job are sorted by g0.
 struct PackMat {
     uint64_t src, dst;
     uint32_t src_ld, dst_ld;
-    uint32_t rows, cols, trans, tile0;
+    uint32_t rows, cols, trans, pad;
+    uint64_t g0; // first 8-element group of the matrix in its pack job
 };
 
 // diagonal: output block with its contribution range

@@ -44,9 +44,9 @@ hipError_t launch_seg_gemm_pk(int cfg, const __bf16 *Ah, const __bf16 *Al,
                               int beta0, int n_rhs, int grid, hipStream_t st,
                               XcdSplit xs = XcdSplit{});
 // FP64 -> packed hi / lo BF16 planes, per right-hand side (src_rs, dst_rs):
-// the nmats matrices of `mats` (ntiles tiles in all)
+// the nmats matrices of `mats` (ngroups 8-element groups in all)
 hipError_t launch_pack_bf16(const double *src, __bf16 *hi, __bf16 *lo,
-                            const PackMat *mats, int nmats, uint32_t ntiles,
+                            const PackMat *mats, int nmats, uint64_t ngroups,
                             uint64_t src_rs, uint64_t dst_rs, int n_rhs,
                             int grid, hipStream_t st);
 

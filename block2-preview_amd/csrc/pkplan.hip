@@ -47,7 +47,7 @@ struct Packer {
     };
     FlatMap<Key, uint64_t, KeyHash> where;
     std::vector<PackMat> mats;
-    uint64_t len = 0, ntiles = 0;
+    uint64_t len = 0, ngroups = 0; // packed elements, 8-element groups
     uint64_t src_len = 0;
 
     // packed offset of rows x K matrix (r, k) = src[off + r*ld + k]
@@ -76,9 +76,10 @@ struct Packer {
         m.rows = rows;
         m.cols = K;
         m.trans = trans ? 1u : 0u;
-        m.tile0 = (uint32_t)ntiles;
+        m.pad = 0;
+        m.g0 = ngroups;
         mats.push_back(m);
-        ntiles += (uint64_t)((rows + 31) / 32) * ((pld + 63) / 64);
+        ngroups += (uint64_t)rows * (pld / 8);
         len += (uint64_t)rows * pld;
         where.emplace(key, base);
         return (int64_t)base;
@@ -119,11 +120,11 @@ static int pk_pack_ops(b2g_plan p, PkPlan *k) {
     hipStream_t st = p->ctx->stream;
     __bf16 *op = (__bf16 *)k->op;
     B2G_HIP(launch_pack_bf16((const double *)p->lbuf->ptr, op, op + k->l_len,
-                             k->d_lmats, k->n_lmats, k->n_ltiles, 0, 0, 1,
+                             k->d_lmats, k->n_lmats, k->n_lgroups, 0, 0, 1,
                              grid, st));
     B2G_HIP(launch_pack_bf16((const double *)p->rbuf->ptr, op + 2 * k->l_len,
                              op + 2 * k->l_len + k->r_len, k->d_rmats,
-                             k->n_rmats, k->n_rtiles, 0, 0, 1, grid, st));
+                             k->n_rmats, k->n_rgroups, 0, 0, 1, grid, st));
     return B2G_OK;
 }
 
@@ -282,10 +283,10 @@ static int pk_build_into(b2g_plan p, PkPlan *k) {
         bytes += h.red.size() * sizeof(RedTask);
     }
     bytes += (L.mats.size() + R.mats.size() + X.mats.size()) * sizeof(PackMat);
-    B2G_REQUIRE(L.ntiles < (1ull << 32) && R.ntiles < (1ull << 32) &&
-                    X.ntiles < (1ull << 32) && L.mats.size() < (1u << 31) &&
+    B2G_REQUIRE(L.ngroups < (1ull << 40) && R.ngroups < (1ull << 40) &&
+                    X.ngroups < (1ull << 40) && L.mats.size() < (1u << 31) &&
                     R.mats.size() < (1u << 31) && X.mats.size() < (1u << 31),
-                B2G_ERR_ARG, "packed plan: too many pack tiles");
+                B2G_ERR_ARG, "packed plan: too many packed matrices");
     char *host = nullptr;
     const int rs = ctx_stage(p->ctx, bytes + 16, &host);
     if (rs != B2G_OK)
@@ -325,9 +326,9 @@ static int pk_build_into(b2g_plan p, PkPlan *k) {
     k->n_lmats = (int)L.mats.size();
     k->n_rmats = (int)R.mats.size();
     k->n_xmats = (int)X.mats.size();
-    k->n_ltiles = (uint32_t)L.ntiles;
-    k->n_rtiles = (uint32_t)R.ntiles;
-    k->n_xtiles = (uint32_t)X.ntiles;
+    k->n_lgroups = L.ngroups;
+    k->n_rgroups = R.ngroups;
+    k->n_xgroups = X.ngroups;
     B2G_HIP(hipMemcpyAsync(k->dev, host, bytes, hipMemcpyHostToDevice,
                            p->ctx->stream));
     B2G_HIP(hipStreamSynchronize(p->ctx->stream));

@@ -143,7 +143,7 @@ struct PkPlan {
     // pack jobs: operator blocks (L, R) and wavefunction windows (X)
     PackMat *d_lmats = nullptr, *d_rmats = nullptr, *d_xmats = nullptr;
     int n_lmats = 0, n_rmats = 0, n_xmats = 0;
-    uint32_t n_ltiles = 0, n_rtiles = 0, n_xtiles = 0;
+    uint64_t n_lgroups = 0, n_rgroups = 0, n_xgroups = 0; // 8-element groups
     uint64_t x_len = 0; // packed wavefunction elements per right-hand side
     uint64_t w_max = 0; // packed W^T elements per right-hand side
     uint64_t p_max = 0; // split-K partial elements per right-hand side

@@ -1257,7 +1257,7 @@ extern "C" int b2g_hpsi(b2g_ctx ctx, b2g_plan p, const double *x, double *y,
     if (pk != nullptr)
         B2G_HIP(timed(B2G_K_PACK, ctx->stream, [&](hipStream_t st) {
             return launch_pack_bf16(x, (__bf16 *)pk->xh(), (__bf16 *)pk->xl(),
-                                    pk->d_xmats, pk->n_xmats, pk->n_xtiles,
+                                    pk->d_xmats, pk->n_xmats, pk->n_xgroups,
                                     p->x_len,
                                     pk->x_len, n_rhs, grid, st);
         }));
