@@ -129,19 +129,37 @@ class Split:
 
 def split_density_matrix(ctx, symmetry, info, psi, k, cutoff=0.0,
                          trace_right=True, normalize=True,
-                         trunc_type=TRUNC_PHYSICAL, scale=1.0):
+                         trunc_type=TRUNC_PHYSICAL, scale=1.0, weights=None):
     """Truncate the wavefunction psi (device FP64 tensor with the layout of
-    `info`) to k states (-1: all) through its density matrix."""
+    `info`) to k states (-1: all) through its density matrix.  A 2-d psi
+    holds state-averaged roots (one per row, density matrix weighted by
+    `weights`, multi_split_density_matrix); the wavefunction factor then
+    comes back with one row per root."""
     lib = ctx.lib
     ci = info.to_c()
     psi = psi.contiguous()
     torch.cuda.synchronize(ctx.device)
     h = C.c_void_p()
-    L.check(lib.b2g_split_density_matrix(
-        ctx.h, int(symmetry), C.byref(ci), C.c_void_p(psi.data_ptr()),
-        int(k), float(cutoff), int(bool(trace_right)), int(bool(normalize)),
-        int(trunc_type), float(scale), C.byref(h)))
+    if psi.dim() == 1:
+        L.check(lib.b2g_split_density_matrix(
+            ctx.h, int(symmetry), C.byref(ci), C.c_void_p(psi.data_ptr()),
+            int(k), float(cutoff), int(bool(trace_right)),
+            int(bool(normalize)), int(trunc_type), float(scale), C.byref(h)))
+    else:
+        nr = psi.shape[0]
+        w = np.ascontiguousarray(
+            np.full(nr, 1.0 / nr) if weights is None else weights,
+            dtype=np.float64)
+        assert w.size == nr
+        L.check(lib.b2g_split_density_matrix_multi(
+            ctx.h, int(symmetry), C.byref(ci), C.c_void_p(psi.data_ptr()),
+            nr, L.ptr(w, C.c_double), int(k), float(cutoff),
+            int(bool(trace_right)), int(bool(normalize)), int(trunc_type),
+            float(scale), C.byref(h)))
     try:
+        nroots = C.c_int(1)
+        L.check(lib.b2g_split_n_roots(h, C.byref(nroots)))
+        wav_part = 1 if trace_right else 0
         parts = []
         for which in range(3):
             ri = L.b2g_sparse_info()
@@ -149,8 +167,12 @@ def split_density_matrix(ctx, symmetry, info, psi, k, cutoff=0.0,
             n = C.c_size_t()
             L.check(lib.b2g_split_result(h, which, C.byref(ri), C.byref(dp),
                                          C.byref(n)))
-            parts.append((BlockInfo.from_c(ri),
-                          _dev_copy(ctx, dp.value, n.value)))
+            if which == wav_part and psi.dim() == 2:
+                data = _dev_copy(ctx, dp.value,
+                                 n.value * nroots.value).view(nroots.value, -1)
+            else:
+                data = _dev_copy(ctx, dp.value, n.value)
+            parts.append((BlockInfo.from_c(ri), data))
         err = C.c_double()
         L.check(lib.b2g_split_error(h, C.byref(err)))
         vals = []

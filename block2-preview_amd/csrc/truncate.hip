@@ -843,6 +843,7 @@ extern "C" int b2g_syev_batched(b2g_ctx ctx, int n_mats, const uint32_t *n,
 // ------------------------------------------------------------------ split
 struct b2g_split_s {
     b2g_ctx ctx = nullptr;
+    int n_roots = 1; // the wavefunction part holds n_roots consecutive roots
     double error = 0.0;
     std::vector<double> eigvals; // per density-matrix block, ascending
     std::vector<double> spectra; // sqrt(max(0, w)) (store_wfn_spectra)
@@ -904,14 +905,16 @@ void set_shifts(b2g_split_s::Part &p) {
 
 } // namespace b2g
 
-extern "C" int b2g_split_density_matrix(b2g_ctx ctx, int symmetry,
-                                        const b2g_sparse_info *wfn,
-                                        const double *psi, int k,
-                                        double cutoff, int trace_right,
-                                        int normalize, unsigned trunc_type,
-                                        double scale, b2g_split *out) {
+namespace b2g {
+
+static int split_impl(b2g_ctx ctx, int symmetry, const b2g_sparse_info *wfn,
+                      const double *psi, int n_roots, const double *weights,
+                      int k, double cutoff, int trace_right, int normalize,
+                      unsigned trunc_type, double scale, b2g_split *out) {
     B2G_REQUIRE(ctx && wfn && out && (psi || wfn->n == 0), B2G_ERR_ARG,
                 "NULL arg");
+    B2G_REQUIRE(n_roots >= 1 && weights != nullptr, B2G_ERR_ARG,
+                "need at least one root and its weight");
     B2G_REQUIRE(symmetry == B2G_SZ || symmetry == B2G_SU2, B2G_ERR_ARG,
                 "unknown symmetry");
     B2G_REQUIRE(wfn->n >= 0 && (wfn->n == 0 || (wfn->quanta &&
@@ -929,6 +932,7 @@ extern "C" int b2g_split_density_matrix(b2g_ctx ctx, int symmetry,
     const uint64_t wdq = wfn->delta_quantum;
     std::unique_ptr<b2g_split_s> h(new b2g_split_s());
     h->ctx = ctx;
+    h->n_roots = n_roots;
     // ---- density-matrix info (SparseMatrixInfo::initialize_dm)
     std::vector<uint64_t> dq_of(nw);
     for (int i = 0; i < nw; i++)
@@ -953,6 +957,12 @@ extern "C" int b2g_split_density_matrix(b2g_ctx ctx, int symmetry,
         B2G_REQUIRE(sz < ((uint64_t)1 << 28), B2G_ERR_ARG,
                     "wavefunction block of 2^28 elements or more");
     }
+    // elements of one root (roots are stored one after the other)
+    uint64_t psi_len = 0;
+    for (int i = 0; i < nw; i++)
+        psi_len = std::max<uint64_t>(
+            psi_len, wfn->block_shifts[i] + (uint64_t)wfn->n_states_bra[i] *
+                                                wfn->n_states_ket[i]);
     for (int i = 0; i < nw; i++) {
         const uint32_t d = trace_right ? wfn->n_states_bra[i]
                                        : wfn->n_states_ket[i];
@@ -973,13 +983,15 @@ extern "C" int b2g_split_density_matrix(b2g_ctx ctx, int symmetry,
         for (int b = 0; b < nd; b++) {
             const uint32_t sb = (uint32_t)ts.s.size();
             uint64_t K = 0;
+            for (int rt = 0; rt < n_roots; rt++)
             for (int i : by[b]) {
                 const uint32_t nbr = wfn->n_states_bra[i],
                                nkt = wfn->n_states_ket[i];
                 Seg s;
-                s.a_off = wfn->block_shifts[i];
-                s.b_off = wfn->block_shifts[i];
-                s.alpha = scale;
+                s.a_off = (uint64_t)rt * psi_len + wfn->block_shifts[i];
+                s.b_off = s.a_off;
+                // DensityMatrix of root rt with factor sqrt(w scale)
+                s.alpha = weights[rt] * scale;
                 if (trace_right) { // psi psi^T: K = ket dimension
                     s.lda = nkt, s.ldb = nkt, s.K = nkt, s.flags = 2;
                 } else { // psi^T psi: K = bra dimension
@@ -1124,7 +1136,8 @@ extern "C" int b2g_split_density_matrix(b2g_ctx ctx, int symmetry,
     sort_part(wav);
     set_shifts(wav);
     B2G_HIP(hipMalloc(&rot.data, std::max<uint64_t>(rot.len, 1) * sizeof(double)));
-    B2G_HIP(hipMalloc(&wav.data, std::max<uint64_t>(wav.len, 1) * sizeof(double)));
+    B2G_HIP(hipMalloc(&wav.data, std::max<uint64_t>(wav.len * n_roots, 1) *
+                                     sizeof(double)));
     // ---- rotation matrix: kept eigenvectors (rows of E) as columns
     // (trace_right) or rows
     {
@@ -1162,9 +1175,10 @@ extern "C" int b2g_split_density_matrix(b2g_ctx ctx, int symmetry,
             B2G_HIP(e2);
         }
     }
-    // ---- the other factor: L^T psi (trace_right) or psi R^T
+    // ---- the other factor: L^T psi (trace_right) or psi R^T, per root
     {
         TaskSet ts;
+        for (int rt = 0; rt < n_roots; rt++)
         for (int i = 0; i < kk; i++)
             for (int iw : dm_to_wfn[ilr[i]]) {
                 const int o = find_q(wav.quanta, wfn->quanta[iw]);
@@ -1172,13 +1186,16 @@ extern "C" int b2g_split_density_matrix(b2g_ctx ctx, int symmetry,
                 Seg s;
                 s.alpha = 1.0;
                 uint32_t M, N;
+                const uint64_t po = (uint64_t)rt * psi_len;
                 if (trace_right) { // (im x n_bra)(n_bra x n_ket)
                     s.a_off = rot.shift[i], s.lda = im[i];
-                    s.b_off = wfn->block_shifts[iw], s.ldb = wfn->n_states_ket[iw];
+                    s.b_off = po + wfn->block_shifts[iw];
+                    s.ldb = wfn->n_states_ket[iw];
                     s.K = dm.nb[ilr[i]], s.flags = 1;
                     M = im[i], N = wfn->n_states_ket[iw];
                 } else { // (n_bra x n_ket)(n_ket x im)
-                    s.a_off = wfn->block_shifts[iw], s.lda = wfn->n_states_ket[iw];
+                    s.a_off = po + wfn->block_shifts[iw];
+                    s.lda = wfn->n_states_ket[iw];
                     s.b_off = rot.shift[i], s.ldb = dm.nb[ilr[i]];
                     s.K = dm.nb[ilr[i]], s.flags = 2;
                     M = wfn->n_states_bra[iw], N = im[i];
@@ -1187,8 +1204,8 @@ extern "C" int b2g_split_density_matrix(b2g_ctx ctx, int symmetry,
                 if (s.K > 0)
                     ts.s.push_back(s);
                 if (M > 0 && N > 0)
-                    add_tiles(ts.t, wav.shift[o], M, N, s.K, sb,
-                              (uint32_t)ts.s.size(), 0);
+                    add_tiles(ts.t, (uint64_t)rt * wav.len + wav.shift[o], M,
+                              N, s.K, sb, (uint32_t)ts.s.size(), 0);
             }
         int r = ts.upload(ctx);
         if (r == B2G_OK)
@@ -1197,18 +1214,47 @@ extern "C" int b2g_split_density_matrix(b2g_ctx ctx, int symmetry,
         if (r != B2G_OK)
             return r;
     }
-    if (normalize && wav.len > 0) {
+    for (int rt = 0; normalize && wav.len > 0 && rt < n_roots; rt++) {
+        double *wr = wav.data + (uint64_t)rt * wav.len;
         double nrm2 = 0.0;
-        int r = b2g_vec_dot(ctx, wav.data, wav.data, wav.len, &nrm2);
+        int r = b2g_vec_dot(ctx, wr, wr, wav.len, &nrm2);
         if (r != B2G_OK)
             return r;
-        if (nrm2 > 0.0 && (r = b2g_vec_scale(ctx, 1.0 / std::sqrt(nrm2),
-                                             wav.data, wav.len)) != B2G_OK)
+        if (nrm2 > 0.0 &&
+            (r = b2g_vec_scale(ctx, 1.0 / std::sqrt(nrm2), wr, wav.len)) !=
+                B2G_OK)
             return r;
     }
     B2G_HIP(hipStreamSynchronize(ctx->stream));
     tm.mark("split");
     *out = h.release();
+    return B2G_OK;
+}
+
+} // namespace b2g
+
+extern "C" int b2g_split_density_matrix(b2g_ctx ctx, int symmetry,
+                                        const b2g_sparse_info *wfn,
+                                        const double *psi, int k,
+                                        double cutoff, int trace_right,
+                                        int normalize, unsigned trunc_type,
+                                        double scale, b2g_split *out) {
+    const double one = 1.0;
+    return split_impl(ctx, symmetry, wfn, psi, 1, &one, k, cutoff,
+                      trace_right, normalize, trunc_type, scale, out);
+}
+
+extern "C" int b2g_split_density_matrix_multi(
+    b2g_ctx ctx, int symmetry, const b2g_sparse_info *wfn, const double *psi,
+    int n_roots, const double *weights, int k, double cutoff, int trace_right,
+    int normalize, unsigned trunc_type, double scale, b2g_split *out) {
+    return split_impl(ctx, symmetry, wfn, psi, n_roots, weights, k, cutoff,
+                      trace_right, normalize, trunc_type, scale, out);
+}
+
+extern "C" int b2g_split_n_roots(b2g_split s, int *n) {
+    B2G_REQUIRE(s && n, B2G_ERR_ARG, "NULL arg");
+    *n = s->n_roots;
     return B2G_OK;
 }
 

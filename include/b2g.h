@@ -459,9 +459,25 @@ int b2g_split_density_matrix(b2g_ctx ctx, int symmetry,
                              int k, double cutoff, int trace_right,
                              int normalize, unsigned trunc_type, double scale,
                              b2g_split *out);
+/* State-averaged form (MovingEnvironment::density_matrix_with_multi_target
+ * + multi_split_density_matrix, src/dmrg/moving_environment.hpp:3582-3608,
+ * 4347-4462): n_roots wavefunctions of the one structure `wfn`, stored one
+ * after the other in psi (each of the structure's length), weights[r]
+ * (host) scaling root r's density matrix (times `scale`).  One rotation
+ * matrix; the wavefunction factor holds the n_roots new roots one after the
+ * other (b2g_split_result's len is one root's), each normalised alone.    */
+int b2g_split_density_matrix_multi(b2g_ctx ctx, int symmetry,
+                                   const b2g_sparse_info *wfn,
+                                   const double *psi, int n_roots,
+                                   const double *weights, int k,
+                                   double cutoff, int trace_right,
+                                   int normalize, unsigned trunc_type,
+                                   double scale, b2g_split *out);
+int b2g_split_n_roots(b2g_split s, int *n);
 /* Result parts: which 0 = left factor, 1 = right factor, 2 = the density
  * matrix (before diagonalisation).  The info arrays are owned by the
- * split; data is device memory owned by the split.                        */
+ * split; data is device memory owned by the split (the wavefunction
+ * factor: n_roots consecutive roots of len elements each).                */
 int b2g_split_result(b2g_split s, int which, b2g_sparse_info *info,
                      double **data, size_t *len);
 /* Truncation error (sum of the discarded positive eigenvalues).          */

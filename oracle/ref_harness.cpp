@@ -297,6 +297,72 @@ template <typename S> struct Dumper : CallbackKernel {
             dm->deallocate();
         }
     }
+    // State-averaged truncation by the reference (density_matrix_with_
+    // multi_target + multi_split_density_matrix,
+    // src/dmrg/moving_environment.hpp:3582-3608, 4347-4462): two roots of
+    // the same symmetry, `a` and `b`
+    // (each a one-component SparseMatrixGroup), weights 0.7 / 0.3, forward,
+    // Physical, k = 2/3 of all states.  Written with tag sM: density
+    // matrix, rotation matrix, each root's new wavefunction, error,
+    // spectra, settings.
+    void dump_multi_split(const shared_ptr<SparseMatrix<S, double>> &a,
+                          const vector<double> &bdata, const string &d) const {
+        const string t = "sM_";
+        vector<shared_ptr<SparseMatrixGroup<S, double>>> wfns;
+        for (int r = 0; r < 2; r++) {
+            auto g = make_shared<SparseMatrixGroup<S, double>>();
+            g->allocate(vector<shared_ptr<SparseMatrixInfo<S>>>{a->info});
+            const double *src = r == 0 ? a->data : bdata.data();
+            memcpy(g->data, src, sizeof(double) * a->total_memory);
+            wfns.push_back(g);
+        }
+        const vector<double> weights = {0.7, 0.3};
+        auto dm = MovingEnvironment<S, double, double>::
+            density_matrix_with_multi_target(S(0), wfns, weights, true, 0.0,
+                                             NoiseTypes::None);
+        InfoPack<S> dp;
+        dp.add(dm->info);
+        dp.write(d, t + "dm");
+        write_npy(d + "/" + t + "dm_data.npy", dm->data, dm->total_memory);
+        int k_total = 0;
+        for (int i = 0; i < dm->info->n; i++)
+            k_total += dm->info->n_states_bra[i];
+        const int k = max(1, k_total * 2 / 3);
+        const double cutoff = 1E-14;
+        vector<shared_ptr<SparseMatrixGroup<S, double>>> new_wfns;
+        shared_ptr<SparseMatrix<S, double>> rot;
+        vector<double> spectra;
+        const double err =
+            MovingEnvironment<S, double, double>::multi_split_density_matrix(
+                dm, wfns, k, true, true, new_wfns, rot, cutoff, true, spectra,
+                TruncationTypes::Physical);
+        InfoPack<S> rp, wp;
+        rp.add(rot->info);
+        rp.write(d, t + "left");
+        write_npy(d + "/" + t + "left_data.npy", rot->data, rot->total_memory);
+        wp.add(new_wfns[0]->infos[0]);
+        wp.write(d, t + "right");
+        vector<double> wd;
+        for (auto &g : new_wfns)
+            wd.insert(wd.end(), g->data, g->data + g->total_memory);
+        write_npy(d + "/" + t + "right_data.npy", wd);
+        write_npy(d + "/" + t + "spectra.npy", spectra);
+        write_npy(d + "/" + t + "roots.npy", bdata);
+        vector<double> scal = {err, (double)k, cutoff, 0.0, 1.0};
+        write_npy(d + "/" + t + "settings.npy", scal);
+        // release in reverse order of allocation on each LIFO stack:
+        // data (roots, density matrix, rotation, new roots) and infos
+        // (density matrix, rotation, new roots' shared info)
+        for (int r = (int)new_wfns.size() - 1; r >= 0; r--)
+            new_wfns[r]->deallocate();
+        rot->deallocate();
+        dm->deallocate();
+        for (int r = (int)wfns.size() - 1; r >= 0; r--)
+            wfns[r]->deallocate();
+        new_wfns[0]->infos[0]->deallocate();
+        rot->info->deallocate();
+        dm->info->deallocate();
+    }
     void dump(const shared_ptr<EffectiveHamiltonian<S, double>> &h, string d,
               int isite) const {
         auto op = h->op;
@@ -489,6 +555,16 @@ template <typename S> struct Dumper : CallbackKernel {
         // SparseMatrix::save_data with save_info)
         h->ket->save_data(d + "/ket_dav.smat", true);
         dump_split(h->ket, d);
+        {
+            // second root: the seeded vector x2, normalised
+            vector<double> xb(x2);
+            double nrm = 0;
+            for (double v : xb)
+                nrm += v * v;
+            for (double &v : xb)
+                v /= sqrt(nrm);
+            dump_multi_split(h->ket, xb, d);
+        }
         memcpy(h->ket->data, psi.data(), nw * sizeof(double));
         write_npy(d + "/psi_dav.npy", psi_dav);
         ofstream m(d + "/meta.txt");

@@ -209,3 +209,77 @@ def split_density_matrix(sym, info, psi, k, cutoff, trace_right,
                 left=rot if trace_right else wav,
                 right=wav if trace_right else rot, spectra=spectra,
                 ss=ss)
+
+
+def multi_split_density_matrix(sym, info, psis, weights, k, cutoff,
+                               trace_right, normalize=True, trunc_type=0,
+                               scale=1.0):
+    """State-averaged roots of one structure (density_matrix_with_multi_
+    target + multi_split_density_matrix, src/dmrg/moving_environment.hpp:
+    3582-3608, 4347-4462): density matrix sum_r w_r scale psi_r psi_r^T,
+    one rotation matrix, each root's factor normalised alone.  Returns
+    dict(error, eigvals, dm, rot=(quanta, nbra, nket, data), wav=[per root
+    (quanta, nbra, nket, data)], spectra, ss)."""
+    base = None
+    for r, (p, w) in enumerate(zip(psis, weights)):
+        uq, dims, blocks = density_matrix(sym, info, p, trace_right,
+                                          w * scale)
+        if base is None:
+            base = (uq, dims, [b.copy() for b in blocks])
+        else:
+            for a, b in zip(base[2], blocks):
+                a += b
+    uq, dims, blocks = base
+    eig = [signed_eigh(b) if b.size else (np.zeros(0), np.zeros((0, 0)))
+           for b in blocks]
+    w = [e[0] for e in eig]
+    ss, error = truncate(sym, uq, w, k, cutoff, trunc_type)
+    spectra = []
+    for b, wb in enumerate(w):
+        reps = multiplicity(sym, uq[b]) if trunc_type & 8 else 1
+        for _ in range(reps):
+            spectra += list(np.sqrt(np.maximum(0.0, wb)))
+    ilr, im = [], []
+    for b, _ in ss:
+        if not ilr or ilr[-1] != b:
+            ilr.append(b)
+            im.append(1)
+        else:
+            im[-1] += 1
+    rq, rnb, rnk, rdata = [], [], [], []
+    iss = 0
+    for i, b in enumerate(ilr):
+        vecs = np.stack([eig[b][1][ss[iss + j][1]] for j in range(im[i])])
+        iss += im[i]
+        rq.append(uq[b])
+        if trace_right:
+            rnb.append(dims[b]), rnk.append(im[i]), rdata.append(vecs.T)
+        else:
+            rnb.append(im[i]), rnk.append(dims[b]), rdata.append(vecs)
+    qs = info["quanta"]
+    dq = int(info["dq"])
+    lab = [q_get_bra(sym, int(q), dq) if trace_right
+           else q_neg(sym, q_get_ket(sym, int(q))) for q in qs]
+    wavs = []
+    for p in psis:
+        wq, wblk = [], []
+        for i, b in enumerate(ilr):
+            for iw in range(len(qs)):
+                if uq.index(lab[iw]) != b:
+                    continue
+                m, n, o = (int(info["nbra"][iw]), int(info["nket"][iw]),
+                           int(info["shift"][iw]))
+                x = p[o:o + m * n].reshape(m, n)
+                wq.append(int(qs[iw]))
+                wblk.append(rdata[i].T @ x if trace_right else x @ rdata[i].T)
+        order = sorted(range(len(wq)), key=lambda j: wq[j])
+        wq = [wq[j] for j in order]
+        wblk = [wblk[j] for j in order]
+        if normalize:
+            nrm = np.sqrt(sum(float((b * b).sum()) for b in wblk))
+            if nrm > 0:
+                wblk = [b / nrm for b in wblk]
+        wavs.append((wq, [b.shape[0] for b in wblk],
+                     [b.shape[1] for b in wblk], wblk))
+    return dict(error=error, eigvals=w, dm=(uq, dims, blocks),
+                rot=(rq, rnb, rnk, rdata), wav=wavs, spectra=spectra, ss=ss)

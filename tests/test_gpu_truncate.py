@@ -144,6 +144,36 @@ def check_factors(sym, info, psi, mine, dref, tr, name):
         assert np.abs(raw[q] / nrm - wav[q]).max(initial=0) < 1e-12, (name, q)
 
 
+def kept_matches(z, tag, s, sym, tt, tr, cutoff, name):
+    """Kept states per sector equal to the reference's, except for states
+    whose (reduced) eigenvalue lies within 1e-13 of the cutoff, where FP64
+    eigensolvers (LAPACK's absolute error is n eps ||D|| ~ 1e-14 here) may
+    fall on either side.  Returns whether they are equal everywhere."""
+    sp = z[tag + "_spectra"]
+    dmi = s.dm[0]
+    rot_part = "left" if tr else "right"
+    rp = tag + "_" + rot_part
+    kept_ref = {int(q): int(m) for q, m in zip(
+        z[rp + "_quanta"], z[rp + ("_nket" if tr else "_nbra")])}
+    rbi = s.left[0] if tr else s.right[0]
+    kept = {int(q): int(m) for q, m in zip(
+        rbi.quanta, rbi.nket if tr else rbi.nbra)}
+    amb, o = {}, 0
+    for i, q in enumerate(dmi.quanta):
+        n = int(dmi.nbra[i])
+        lam = sp[o:o + n] ** 2
+        if tt & 1:
+            lam = lam / T.multiplicity(sym, int(q))
+        amb[int(q)] = int((np.abs(lam - cutoff) < 1e-13).sum())
+        o += n
+    strict = True
+    for q in set(kept) | set(kept_ref):
+        d = abs(kept.get(q, 0) - kept_ref.get(q, 0))
+        assert d <= amb.get(q, 0), (name, tag, q, kept.get(q), kept_ref.get(q))
+        strict = strict and d == 0
+    return strict
+
+
 @pytest.mark.parametrize("name", FIXTURES + DELAYED_FIXTURES)
 @pytest.mark.parametrize("tag", CASES)
 def test_split_matches_reference(gpu_ctx, name, tag):
@@ -165,30 +195,7 @@ def test_split_matches_reference(gpu_ctx, name, tag):
     sp = z[tag + "_spectra"]
     assert s.spectra.shape == sp.shape
     assert np.abs(s.spectra ** 2 - sp ** 2).max(initial=0) < 1e-13
-    # kept states per sector: equal to the reference's, except for states
-    # whose (reduced) eigenvalue lies within 1e-13 of the cutoff, where FP64
-    # eigensolvers (LAPACK's absolute error is n eps ||D|| ~ 1e-14 here)
-    # may fall on either side
-    rot_part = "left" if tr else "right"
-    rp = tag + "_" + rot_part
-    kept_ref = {int(q): int(m) for q, m in zip(
-        z[rp + "_quanta"], z[rp + ("_nket" if tr else "_nbra")])}
-    rbi = s.left[0] if tr else s.right[0]
-    kept = {int(q): int(m) for q, m in zip(
-        rbi.quanta, rbi.nket if tr else rbi.nbra)}
-    amb, o = {}, 0
-    for i, q in enumerate(dmi.quanta):
-        n = int(dmi.nbra[i])
-        lam = sp[o:o + n] ** 2
-        if int(tt) & 1:
-            lam = lam / T.multiplicity(sym, int(q))
-        amb[int(q)] = int((np.abs(lam - cutoff) < 1e-13).sum())
-        o += n
-    strict = True
-    for q in set(kept) | set(kept_ref):
-        d = abs(kept.get(q, 0) - kept_ref.get(q, 0))
-        assert d <= amb.get(q, 0), (name, tag, q, kept.get(q), kept_ref.get(q))
-        strict = strict and d == 0
+    strict = kept_matches(z, tag, s, sym, int(tt), bool(tr), cutoff, name)
     mine = {"left": part_blocks(*s.left), "right": part_blocks(*s.right)}
     dref = {int(q): dmi.block(z[tag + "_dm_data"], i)
             for i, q in enumerate(dmi.quanta)}
@@ -254,3 +261,114 @@ def test_split_synthetic_matches_oracle(gpu_ctx, trace_right):
     mine = {"left": part_blocks(*s.left), "right": part_blocks(*s.right)}
     check_factors(sym, info, psi, mine, dict(zip(uq, blocks)), trace_right,
                   "synthetic")
+
+
+def root_parts(s, r, tr):
+    """(rotation blocks, root r's wavefunction-factor blocks) of a
+    state-averaged split, as the left/right dict check_factors takes"""
+    wav = s.right if tr else s.left
+    rot = s.left if tr else s.right
+    w = {"wav": part_blocks(wav[0], wav[1][r]), "rot": part_blocks(*rot)}
+    return ({"left": w["rot"], "right": w["wav"]} if tr
+            else {"left": w["wav"], "right": w["rot"]})
+
+
+@pytest.mark.parametrize("name", FIXTURES + DELAYED_FIXTURES)
+def test_multi_split_matches_reference(gpu_ctx, name):
+    """State-averaged split of two roots (the Davidson vector and a
+    normalised random vector, weights 0.7 / 0.3) against the reference's
+    density_matrix_with_multi_target + multi_split_density_matrix: the
+    same checks as the single-root split, with the factor checks for each
+    root."""
+    z, sym, info = load(name)
+    if "sM_settings" not in z.files:
+        pytest.skip("fixture without multi-root dump")
+    err_ref, k, cutoff, tt, tr = z["sM_settings"]
+    print("case", name, "sM", flush=True)
+    roots = [z["psi_dav"], z["sM_roots"]]
+    s = TR.split_density_matrix(gpu_ctx, sym, info, dev(np.stack(roots)),
+                                int(k), cutoff, bool(tr), True, int(tt),
+                                weights=[0.7, 0.3])
+    dmi, dmd = s.dm
+    assert np.array_equal(dmi.quanta, z["sM_dm_quanta"].astype(np.uint64))
+    assert np.array_equal(dmi.nbra, z["sM_dm_nbra"])
+    assert np.abs(dmd.cpu().numpy() - z["sM_dm_data"]).max(initial=0) < 1e-13
+    assert abs(s.error - err_ref) < 1e-12, (name, s.error, err_ref)
+    sp = z["sM_spectra"]
+    assert s.spectra.shape == sp.shape
+    assert np.abs(s.spectra ** 2 - sp ** 2).max(initial=0) < 1e-13
+    strict = kept_matches(z, "sM", s, sym, int(tt), bool(tr), cutoff, name)
+    wav = s.right if tr else s.left
+    assert wav[1].shape == (2, wav[0].total)
+    dref = {int(q): dmi.block(z["sM_dm_data"], i)
+            for i, q in enumerate(dmi.quanta)}
+    for r in range(2):
+        check_factors(sym, info, roots[r], root_parts(s, r, bool(tr)), dref,
+                      bool(tr), (name, r))
+    if not strict:
+        return
+    for part, (bi, _) in (("left", s.left), ("right", s.right)):
+        p = "sM_" + part
+        assert np.array_equal(bi.quanta, z[p + "_quanta"].astype(np.uint64))
+        assert np.array_equal(bi.nbra, z[p + "_nbra"])
+        assert np.array_equal(bi.nket, z[p + "_nket"])
+    lb = TR.BlockInfo(z["sM_left_quanta"], z["sM_left_nbra"],
+                      z["sM_left_nket"], z["sM_left_shift"])
+    rb = TR.BlockInfo(z["sM_right_quanta"], z["sM_right_nbra"],
+                      z["sM_right_nket"], z["sM_right_shift"])
+    lref = {int(q): lb.block(z["sM_left_data"], i)
+            for i, q in enumerate(lb.quanta)}
+    rdata = z["sM_right_data"].reshape(2, -1)
+    for r in range(2):
+        rref = {int(q): rb.block(rdata[r], i) for i, q in enumerate(rb.quanta)}
+        mine = root_parts(s, r, True)
+        a = recon(sym, info, lref, rref, True)
+        b = recon(sym, info, mine["left"], mine["right"], True)
+        assert a.keys() == b.keys()
+        for q in a:
+            assert np.abs(a[q] - b[q]).max(initial=0) < 1e-7, (name, r, q)
+
+
+@pytest.mark.parametrize("trace_right", [True, False])
+def test_multi_split_synthetic_matches_oracle(gpu_ctx, trace_right):
+    """Three random roots of the synthetic n=26 middle-site structure,
+    weights 0.5 / 0.3 / 0.2, both directions, against the oracle."""
+    from b2g import synth
+    pb = synth.middle_site(n_orb=26, m=1000, max_dim=300, seed=11,
+                           data="device")
+    o, n = int(pb.ket.off[0]), int(pb.ket.n[0])
+    info = TR.BlockInfo(pb.ket.quanta[o:o + n], pb.ket.nbra[o:o + n],
+                        pb.ket.nket[o:o + n], pb.ket.shift[o:o + n],
+                        int(pb.ket.dq[0]))
+    rng = np.random.default_rng(5)
+    psis = rng.standard_normal((3, info.total))
+    psis /= np.linalg.norm(psis, axis=1)[:, None]
+    wts = [0.5, 0.3, 0.2]
+    sym = T.SU2
+    k = 800
+    s = TR.split_density_matrix(gpu_ctx, sym, info, dev(psis), k, 1e-14,
+                                trace_right, True, 0, weights=wts)
+    oi = dict(quanta=info.quanta, nbra=info.nbra, nket=info.nket,
+              shift=info.shift, dq=info.dq)
+    res = T.multi_split_density_matrix(sym, oi, list(psis), wts, k, 1e-14,
+                                       trace_right, True, 0)
+    assert abs(s.error - res["error"]) < 1e-12
+    wo = np.concatenate([np.asarray(x) for x in res["eigvals"]])
+    assert np.abs(s.eigvals - wo).max() < 1e-13
+    uq, dims, blocks = res["dm"]
+    dmi, dmd = s.dm
+    assert [int(x) for x in dmi.quanta] == uq
+    dd = dmd.cpu().numpy()
+    for i, blk in enumerate(blocks):
+        assert np.abs(dmi.block(dd, i) - blk).max(initial=0) < 1e-13
+    rot = s.left if trace_right else s.right
+    assert [int(x) for x in rot[0].quanta] == list(res["rot"][0])
+    assert list(rot[0].nbra) == list(res["rot"][1])
+    assert list(rot[0].nket) == list(res["rot"][2])
+    wav = s.right if trace_right else s.left
+    for r in range(3):
+        wq, wnb, wnk, _ = res["wav"][r]
+        assert [int(x) for x in wav[0].quanta] == wq
+        assert list(wav[0].nbra) == wnb and list(wav[0].nket) == wnk
+        check_factors(sym, info, psis[r], root_parts(s, r, trace_right),
+                      dict(zip(uq, blocks)), trace_right, ("synthetic", r))

@@ -182,3 +182,76 @@ def test_label_algebra():
     assert T.q_neg(T.SZ, sz(3, -1, 1)) == sz(-3, 1, 1)
     assert T.q_get_bra(T.SZ, sz(3, -1), sz(1, 1)) == sz(4, 0)
     assert T.q_get_ket(T.SZ, sz(3, 1)) == sz(3, 1)
+
+
+def wav_blocks(info_part, data):
+    q, nb, nk, sh = info_part
+    return {int(a): data[int(o):int(o) + int(m) * int(n)].reshape(int(m), int(n))
+            for a, m, n, o in zip(q, nb, nk, sh)}
+
+
+@pytest.mark.parametrize("name", ALL)
+def test_oracle_multi_split_matches_reference(name):
+    """State-averaged split of two roots (the Davidson vector and a
+    normalised random vector of the same structure, weights 0.7 / 0.3):
+    MovingEnvironment::density_matrix_with_multi_target +
+    multi_split_density_matrix (moving_environment.hpp:3582-3608,
+    4347-4462), trace_right, Physical, k = 2/3 of the states."""
+    z, sym, info = load(name)
+    if "sM_settings" not in z.files:
+        pytest.skip("fixture without multi-root dump")
+    err_ref, k, cutoff, tt, tr = z["sM_settings"]
+    psis = [z["psi_dav"], z["sM_roots"]]
+    res = T.multi_split_density_matrix(sym, info, psis, [0.7, 0.3], int(k),
+                                       cutoff, bool(tr), True, int(tt))
+    dq, dn, dnk, dsh, ddata = ref_part(z, "sM", "dm")
+    uq, dims, blocks = res["dm"]
+    assert [int(x) for x in dq] == uq and list(dn) == dims
+    for i, b in enumerate(blocks):
+        o = int(dsh[i])
+        ref = ddata[o:o + dims[i] ** 2].reshape(dims[i], dims[i])
+        assert np.abs(b - ref).max(initial=0) < 1e-13, (name, i)
+    assert abs(res["error"] - err_ref) < 1e-12, (name, res["error"], err_ref)
+    sp = np.array(res["spectra"])
+    assert sp.shape == z["sM_spectra"].shape
+    assert np.abs(sp ** 2 - z["sM_spectra"] ** 2).max(initial=0) < 1e-13
+    # rotation structure; every root's factor has the structure of root 0
+    rq, rnb, rnk, _, rdata = ref_part(z, "sM", "left")
+    assert [int(x) for x in rq] == list(res["rot"][0])
+    assert list(rnb) == list(res["rot"][1]) and list(rnk) == list(res["rot"][2])
+    wq, wnb, wnk, wsh, wdata = ref_part(z, "sM", "right")
+    per_root = int(wsh[-1]) + int(wnb[-1]) * int(wnk[-1]) if len(wsh) else 0
+    assert wdata.size == 2 * per_root
+    lref = blocks_of(rq, rnb, rnk, _, rdata)
+    lo = {int(a): d for a, d in zip(res["rot"][0], res["rot"][3])}
+    for r in range(2):
+        mq, mnb, mnk, mdata = res["wav"][r]
+        assert [int(x) for x in wq] == mq, (name, r)
+        assert list(wnb) == mnb and list(wnk) == mnk
+        rref = wav_blocks((wq, wnb, wnk, wsh),
+                          wdata[r * per_root:(r + 1) * per_root])
+        ro = {int(a): d for a, d in zip(mq, mdata)}
+        a = recon(sym, info, lref, rref, True)
+        b = recon(sym, info, lo, ro, True)
+        assert a.keys() == b.keys()
+        for q in a:
+            assert np.abs(a[q] - b[q]).max(initial=0) < 1e-9, (name, r, q)
+        # each root's factor is normalised on its own
+        assert abs(sum(float((x * x).sum()) for x in ro.values()) - 1) < 1e-12
+
+
+def test_oracle_multi_split_reduces_to_single_root():
+    """weights (1, 0) give the single-root split of the first root"""
+    z, sym, info = load("hub8_sz")
+    psi = z["psi_dav"]
+    k = 40
+    one = T.split_density_matrix(sym, info, psi, k, 1e-14, True, True, 0)
+    two = T.multi_split_density_matrix(sym, info, [psi, z["sM_roots"]],
+                                       [1.0, 0.0], k, 1e-14, True, True, 0)
+    assert abs(one["error"] - two["error"]) < 1e-15
+    for a, b in zip(one["dm"][2], two["dm"][2]):
+        assert np.abs(a - b).max(initial=0) < 1e-15
+    lo, ro = oracle_parts(one)
+    wq, _, _, wd = two["wav"][0]
+    for q, d in zip(wq, wd):
+        assert np.abs(ro[q] - d).max() < 1e-12
