@@ -554,6 +554,12 @@ template <typename S> struct Dumper : CallbackKernel {
         // the Davidson vector in block2's own file format (info + data,
         // SparseMatrix::save_data with save_info)
         h->ket->save_data(d + "/ket_dav.smat", true);
+        // the whole effective problem in block2's EFFPROB1 file (what
+        // EffectiveHamiltonian::save_eff_problem writes; operator data
+        // other than delayed expressions are not part of it)
+        EffectiveProblem<S, double>::save_problem_file(
+            d + "/eff_problem.smat", h->op, h->cmat, h->vmat, h->opdq,
+            h->operator_quanta, h->wfn_infos, isite, dmrg->forward);
         dump_split(h->ket, d);
         {
             // second root: the seeded vector x2, normalised

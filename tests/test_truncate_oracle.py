@@ -11,11 +11,12 @@ Checked here, on the CPU:
     sqrt of eigenvalues near zero is not compared directly);
   * the factors' block structure exactly, and sign-free quantities of the
     factors: the truncated wavefunction left * right, block by block, to
-    1e-9 (the rotation basis inside a degenerate kept space is open; the
-    kept space itself is fixed only up to O(eps ||D|| / gap) by the
-    eigenvalue gap at the truncation boundary, weighted by the square root
-    of the boundary eigenvalues: 1e-10-level differences between two
-    correct eigensolvers on the L=16 Hubbard site);
+    1e-8 (the rotation basis inside a degenerate kept space is open; kept
+    states whose eigenvalues lie at the rounding level of ||D||, eps ||D||
+    ~ 1e-16 .. 1e-14, are fixed only up to rotations among themselves by
+    any FP64 eigensolver and carry weight sqrt(lambda) ~ 1e-8 in the
+    wavefunction: 1e-9-level differences between numpy's LAPACK and the
+    reference's MKL on the L=16 Hubbard site);
   * independent of the reference: error + kept weight = trace of the
     density matrix, orthonormal rotation matrices, eigen residuals of the
     oracle's eigenpairs, and a planted mistake (a sign error in the
@@ -113,7 +114,7 @@ def test_oracle_split_matches_reference(name, tag):
     b = recon(sym, info, lo, ro, bool(tr))
     assert a.keys() == b.keys()
     for q in a:
-        assert np.abs(a[q] - b[q]).max(initial=0) < 1e-9, (name, tag, q)
+        assert np.abs(a[q] - b[q]).max(initial=0) < 1e-8, (name, tag, q)
 
 
 @pytest.mark.parametrize("name", ["hub8_su2", "n2_sz", "hub16_sz_m200"])
@@ -235,7 +236,7 @@ def test_oracle_multi_split_matches_reference(name):
         b = recon(sym, info, lo, ro, True)
         assert a.keys() == b.keys()
         for q in a:
-            assert np.abs(a[q] - b[q]).max(initial=0) < 1e-9, (name, r, q)
+            assert np.abs(a[q] - b[q]).max(initial=0) < 1e-8, (name, r, q)
         # each root's factor is normalised on its own
         assert abs(sum(float((x * x).sum()) for x in ro.values()) - 1) < 1e-12
 
