@@ -9,7 +9,9 @@ SparseMatrixInfo and ConnectionInfo, its element count), the ket and bra
 infos, the operator quantum and the wavefunction ConnectionInfos.  This
 module parses such a file and builds the Problem the device engine takes,
 structure only: the operator data (which the file does not hold) are
-attached afterwards, in the order Problem.lops / rops lists the operators.
+attached afterwards, in the order Problem.lops / rops lists the operators,
+by the caller or from block2's OperatorTensor files of the two
+environments (load_operator_tensor, attach_operator_data).
 
 Serialisations followed:
   EffectiveProblem header / tensors / metadata  effective_problem.hpp:72-243
@@ -19,6 +21,7 @@ Serialisations followed:
                                                   869-885, 931-950
   save_symbolic                                 src/core/symbolic.hpp:164-215
   SparseMatrixInfo::save_data                   src/core/sparse_matrix.hpp:555-575
+  SparseMatrix::save_data (operator numbers)    src/core/sparse_matrix.hpp:954-973
   DelayedSparseMatrix::save_data                src/core/delayed_sparse_matrix.hpp:94-103
 
 Operators are numbered the way the effective Hamiltonian meets them: by
@@ -212,6 +215,51 @@ class _Reader:
                           for _ in range(self.i32())]
         return t
 
+    def data_tensor(self):
+        """OperatorTensor::save_data(ofs) (operator_tensor.hpp:122-148):
+        markers, symbolics, then per operator its key, matrix type,
+        SparseMatrixInfo and SparseMatrix::save_data payload (factor,
+        element count, elements)."""
+        lr = self.u8()
+        t = dict(lmat=None, rmat=None)
+        if lr in (1, 2):
+            t["lmat"] = self.symbolic()
+            if lr == 1:
+                t["rmat"] = t["lmat"]
+        elif lr == 3:
+            t["rmat"] = self.symbolic()
+        elif lr == 0:
+            t["lmat"] = self.symbolic()
+            t["rmat"] = self.symbolic()
+        elif lr != 4:
+            raise FormatError("bad operator-tensor marker %d" % lr)
+        ops = []
+        for _ in range(self.i32()):
+            key = self.expr()
+            tp = self.u8()
+            info = self.info()
+            m = dict(type=tp, info=info, factor=1.0, data=None)
+            if tp == NORMAL:
+                m["factor"] = self.f64()
+                total = self.u64()
+                if total == (1 << 64) - 1:
+                    raise FormatError("compressed operator storage is not "
+                                      "supported")
+                if total and total != _total(info):
+                    raise FormatError("operator element count %d differs "
+                                      "from its info (%d)"
+                                      % (total, _total(info)))
+                m["data"] = self.arr(np.float64, total)
+            elif tp == DELAYED:
+                m["m"] = self.u16()
+                m["expr"] = self.expr()
+            else:
+                raise FormatError("operator matrix type %d (CSR / archived) "
+                                  "is not supported" % tp)
+            ops.append((key, m))
+        t["ops"] = ops
+        return t
+
     def metadata(self):
         if not self.u8():
             return None
@@ -268,6 +316,66 @@ def load_eff_problem(path):
     """Parse an EFFPROB1 file (block2's EFF-PROB.FORW.i / EFF-PROB.BACKW.i)."""
     with open(path, "rb") as f:
         return EffProblemFile(f.read())
+
+
+def load_operator_tensor(path):
+    """Parse a file written by block2's OperatorTensor::save_data(ofs)
+    (operator_tensor.hpp:122-148, plain storage, not pointer_only): a dict
+    with the symbolic markers lmat / rmat and ops, a list of (key
+    expression, dict(type, info, factor, data)).  block2 writes operators
+    that were never allocated (those of a delayed tensor) with no
+    elements."""
+    with open(path, "rb") as f:
+        return tensor_from_bytes(f.read())
+
+
+def tensor_from_bytes(buf):
+    r = _Reader(buf)
+    t = r.data_tensor()
+    if r.p != len(r.b):
+        raise FormatError("%d bytes after the operator tensor"
+                          % (len(r.b) - r.p))
+    return t
+
+
+def attach_operator_data(pb, left, right, dleft=None, dright=None):
+    """Fill the operator buffers (and factors) of a Problem read by
+    to_problem from parsed operator tensors (load_operator_tensor): the
+    environment tensors `left` / `right` and, for a delayed side, the
+    tensors of its sub-operators `dleft` / `dright`.  Operators are found
+    by key; each must carry exactly the elements its info describes."""
+    keys = pb.op_keys
+    bufs = [np.zeros(pb.data_len[0]), np.zeros(pb.data_len[1])]
+    lists = [(pb.lops, pb.linfo, False, keys[0], left, 0),
+             (pb.rops, pb.rinfo, False, keys[1], right, 1)]
+    if pb.delayed_side:
+        if dleft is None or dright is None:
+            raise FormatError("a delayed problem needs the tensors of the "
+                              "delayed side's sub-operators")
+        side = pb.delayed_side - 1
+        lists += [(pb.dlops, pb.dlinfo, True, keys[2], dleft, side),
+                  (pb.drops, pb.drinfo, True, keys[3], dright, side)]
+    for ops, infos, own, kl, tensor, side in lists:
+        if tensor is None:
+            raise FormatError("operator tensor missing")
+        by = {}
+        for k, m in tensor["ops"]:
+            by.setdefault(_key(k), m)
+        for j, key in enumerate(kl):
+            if ops.flags[j] & 1:
+                continue
+            m = by.get(key)
+            n = infos.total(j if own else int(ops.info[j]))
+            if m is None or m["data"] is None or len(m["data"]) != n:
+                raise FormatError(
+                    "operator %r has no data of %d elements in its tensor "
+                    "(an operator formed from a delayed expression must be "
+                    "formed by the caller)" % (key, n))
+            o = int(ops.off[j])
+            bufs[side][o:o + n] = m["data"]
+            ops.factor[j] = m["factor"]
+    pb.lops.data, pb.rops.data = bufs
+    return pb
 
 
 def _key(e):
@@ -431,4 +539,6 @@ def to_problem(f):
             else:
                 pb.tp.append(dict(c, nc=int(c["nc"])))
     pb.data_len = {0: ln, 1: rn}
+    order = lambda sd: sorted(sd.ids, key=sd.ids.get) if sd else []
+    pb.op_keys = [order(ls), order(rs), order(dls), order(drs)]
     return pb

@@ -560,6 +560,22 @@ template <typename S> struct Dumper : CallbackKernel {
         EffectiveProblem<S, double>::save_problem_file(
             d + "/eff_problem.smat", h->op, h->cmat, h->vmat, h->opdq,
             h->operator_quanta, h->wfn_infos, isite, dmrg->forward);
+        // the environment operator tensors with their numbers
+        // (OperatorTensor::save_data), and for a delayed tensor the tensors
+        // of its sub-operators
+        auto save_opt = [&d](const shared_ptr<OperatorTensor<S, double>> &t,
+                             const string &fn) {
+            ofstream ofs(d + "/" + fn, ios::binary);
+            t->save_data(ofs);
+        };
+        save_opt(h->op->lopt, "lopt.smat");
+        save_opt(h->op->ropt, "ropt.smat");
+        for (auto &t : {h->op->lopt, h->op->ropt})
+            if (t->get_type() == OperatorTensorTypes::Delayed) {
+                auto dt = dynamic_pointer_cast<DelayedOperatorTensor<S, double>>(t);
+                save_opt(dt->lopt, "dlopt.smat");
+                save_opt(dt->ropt, "dropt.smat");
+            }
         dump_split(h->ket, d);
         {
             // second root: the seeded vector x2, normalised

@@ -19,7 +19,12 @@ Checked on the CPU:
     the host plan of the file's
     problem replayed in numpy reproduces the reference's H*psi of both
     dumped vectors (1e-12 relative) with the reference's MAC count;
-  * damaged files (bad magic, truncated, trailing bytes) are refused.
+  * the environments' OperatorTensor files (OperatorTensor::save_data)
+    supply exactly the dumped operator numbers and factors, and the
+    problem assembled from the three files reproduces the reference's
+    H*psi;
+  * damaged files (bad magic, truncated, trailing bytes) and missing
+    operator data are refused.
 """
 import numpy as np
 import pytest
@@ -151,6 +156,50 @@ def test_effprob_problem_reproduces_reference_hpsi(name):
         assert effham.host_plan_info(h)["macs"] == int(pb.meta["nflop_mac"])
     finally:
         effham.destroy(h)
+
+
+def tensors(pb):
+    g = pb.golden
+    return [EP.tensor_from_bytes(bytes(g[k])) if k in g else None
+            for k in ("lopt_smat", "ropt_smat", "dlopt_smat", "dropt_smat")]
+
+
+@pytest.mark.parametrize("name", ALL)
+def test_operator_tensor_files_complete_the_problem(name):
+    """The environments' OperatorTensor files (OperatorTensor::save_data,
+    written by the harness next to the EFFPROB1 file) fill the file
+    problem's operator buffers with exactly the dumped numbers and factors,
+    and the assembled problem reproduces the reference's H*psi."""
+    pb, buf = load(name)
+    if "lopt_smat" not in pb.golden:
+        pytest.skip("fixture without operator-tensor files")
+    fp = EP.to_problem(EP.EffProblemFile(buf))
+    EP.attach_operator_data(fp, *tensors(pb))
+    ref = EP.to_problem(EP.EffProblemFile(buf))
+    attach(ref, pb)
+    assert np.array_equal(fp.lops.data, ref.lops.data)
+    assert np.array_equal(fp.rops.data, ref.rops.data)
+    for (fo, _, _, ft), (po, _, _, pt) in zip(op_sides(fp), op_sides(pb)):
+        for t in range(len(ft)):
+            if ft[t] >= 0:
+                assert fo.factor[ft[t]] == po.factor[pt[t]]
+    h = effham.host_plan(fp)
+    try:
+        y = replay(fp, _pairs(h), pb.golden["psi"])
+        ref_y = pb.golden["hpsi"]
+        assert np.linalg.norm(y - ref_y) <= 1e-12 * np.linalg.norm(ref_y)
+    finally:
+        effham.destroy(h)
+
+
+def test_operator_tensor_attach_refuses_missing_data():
+    pb, buf = load("hub8_sz")
+    fp = EP.to_problem(EP.EffProblemFile(buf))
+    left, right = tensors(pb)[:2]
+    with pytest.raises(EP.FormatError, match="has no data"):
+        EP.attach_operator_data(fp, left, dict(right, ops=right["ops"][1:]))
+    with pytest.raises(EP.FormatError, match="after the operator tensor"):
+        EP.tensor_from_bytes(bytes(pb.golden["lopt_smat"]) + b"\0")
 
 
 def test_effprob_refuses_damaged_files():

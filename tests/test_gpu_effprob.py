@@ -47,3 +47,25 @@ def test_effprob_hpsi_matches_reference(gpu_ctx, name):
         assert rel(h.diag.cpu().numpy(), g["diag"]) < 1e-12
     finally:
         h.deallocate()
+
+
+@pytest.mark.parametrize("name", FIXTURES + DELAYED_FIXTURES)
+def test_saved_files_hpsi_matches_reference(gpu_ctx, name):
+    """The problem assembled from block2's files alone: EFFPROB1 structure
+    plus the environments' OperatorTensor files (and the delayed side's
+    sub-operator tensors)."""
+    pb = Problem.from_npz(fixture_path(name))
+    g = pb.golden
+    if "eff_problem_smat" not in g or "lopt_smat" not in g:
+        pytest.skip("fixture without the saved files")
+    print("case", name, flush=True)
+    fp = EP.to_problem(EP.EffProblemFile(bytes(g["eff_problem_smat"])))
+    ts = [EP.tensor_from_bytes(bytes(g[k])) if k in g else None
+          for k in ("lopt_smat", "ropt_smat", "dlopt_smat", "dropt_smat")]
+    EP.attach_operator_data(fp, *ts)
+    h = EffectiveHamiltonian(fp, gpu_ctx)
+    try:
+        assert rel(h.multiply(dev(g["psi"])).cpu().numpy(), g["hpsi"]) < 1e-12
+        assert rel(h.multiply(dev(g["x2"])).cpu().numpy(), g["hx2"]) < 1e-12
+    finally:
+        h.deallocate()
