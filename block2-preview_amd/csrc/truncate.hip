@@ -56,8 +56,6 @@ constexpr int kJB = 16;      // column block of the block Jacobi
 constexpr int kJS = 2 * kJB; // order of a pair subproblem
 constexpr int kJLD = kJS + 1;
 constexpr int kJG = 256 / kJS; // thread groups of a pair workgroup
-constexpr int kTC = kJS / 4;   // threads across the columns of a result
-constexpr int kRP = 256 / kTC; // result rows per pass
 
 // one matrix of a batch: A and V are N x N (N = 32 nb) from a_off; the
 // round's Qs are nb / 2 matrices of 64 x 64 from q_off, their flags from
@@ -318,33 +316,37 @@ __global__ __launch_bounds__(256) void jacobi_pair_kernel(
     }
 }
 
-// acc = op(Y) X or X Y for one thread's outputs of a (ROWS x kJS) result
-// held in LDS with row stride kJLD: thread t owns rows t / kTC + kRP i,
-// columns 4 (t % kTC) .. + 3.  TL: acc = Y^T X (Y kJS x ROWS), else X Y.
-template <int ROWS, bool TL>
-__device__ __forceinline__ void lds_mm(const double *X, const double *Y,
-                                       double (&acc)[ROWS / kRP][4]) {
-    const int tid = threadIdx.x, tr = tid / kTC, tc = (tid % kTC) * 4;
-    constexpr int RI = ROWS / kRP;
+typedef double jv4d __attribute__((ext_vector_type(4)));
+
+// C = X Y (TL: C = Y^T X) for kJS x kJS (32 x 32) operands held in LDS with
+// row stride kJLD, on FP64 MFMA (16x16x4): wave w of the 256-thread
+// workgroup computes the 16 x 16 quadrant of rows 16 (w >> 1) .., columns
+// 16 (w & 1) ..; lane l feeds A[l % 16][l / 16] and B[l / 16][l % 16] per
+// k-step, and accumulator element r of lane l is C[row][col] with
+// (row, col) = mm_pos(r).
+static_assert(kJS == 32, "the MFMA products assume 32 x 32 subproblems");
+template <bool TL>
+__device__ __forceinline__ jv4d lds_mm(const double *X, const double *Y) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int lr = lane & 15, lk = lane >> 4;
+    const int m0 = 16 * (w >> 1), n0 = 16 * (w & 1);
+    jv4d acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int i = 0; i < RI; i++)
-#pragma unroll
-        for (int j = 0; j < 4; j++)
-            acc[i][j] = 0.0;
-    for (int k = 0; k < kJS; k++) {
-        double y[4];
-#pragma unroll
-        for (int j = 0; j < 4; j++)
-            y[j] = TL ? X[k * kJLD + tc + j] : Y[k * kJLD + tc + j];
-#pragma unroll
-        for (int i = 0; i < RI; i++) {
-            const double x = TL ? Y[k * kJLD + tr + kRP * i]
-                                : X[(tr + kRP * i) * kJLD + k];
-#pragma unroll
-            for (int j = 0; j < 4; j++)
-                acc[i][j] = fma(x, y[j], acc[i][j]);
-        }
+    for (int ks = 0; ks < kJS / 4; ks++) {
+        const int k = 4 * ks + lk;
+        const double av = TL ? Y[k * kJLD + m0 + lr] : X[(m0 + lr) * kJLD + k];
+        const double bv = TL ? X[k * kJLD + n0 + lr] : Y[k * kJLD + n0 + lr];
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
     }
+    mfma_settle<jv4d, 1>(&acc);
+    return acc;
+}
+
+// position of accumulator element r of this thread in the 32 x 32 result
+__device__ __forceinline__ void mm_pos(int r, int &row, int &col) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    row = 16 * (w >> 1) + 4 * r + (lane >> 4);
+    col = 16 * (w & 1) + (lane & 15);
 }
 
 // Region update A_PR <- Q_P^T A_PR Q_R (kind 1) and V slice update
@@ -361,8 +363,7 @@ __global__ __launch_bounds__(256) void jacobi_update_kernel(
     const JacBlock bl = blocks[tk.blk];
     const uint32_t N = bl.nb * kJB;
     const int r = round % (int)(bl.nb - 1);
-    const int tid = threadIdx.x, tr = tid / kTC, tc = (tid % kTC) * 4;
-    constexpr int RI = kJS / kRP;
+    const int tid = threadIdx.x;
     if (tk.kind == 1) {
         const int fp = qflag[bl.f_off + tk.x], fr = qflag[bl.f_off + tk.y];
         if (!fp && !fr)
@@ -375,40 +376,43 @@ __global__ __launch_bounds__(256) void jacobi_update_kernel(
             X[l * kJLD + c] = A[bl.a_off + (uint64_t)pair_index(pi, pj, l) * N +
                                 pair_index(ri, rj, c)];
         }
-        double acc[RI][4];
+        jv4d acc;
         if (fr) {
             const uint64_t qo = bl.q_off + (uint64_t)tk.y * kJS * kJS;
             for (int e = tid; e < kJS * kJS; e += 256)
                 Y[(e / kJS) * kJLD + (e % kJS)] = Q[qo + e];
             __syncthreads();
-            lds_mm<kJS, false>(X, Y, acc); // X Q_R
+            acc = lds_mm<false>(X, Y); // X Q_R
             __syncthreads();
 #pragma unroll
-            for (int i = 0; i < RI; i++)
-#pragma unroll
-                for (int j = 0; j < 4; j++)
-                    X[(tr + kRP * i) * kJLD + tc + j] = acc[i][j];
+            for (int r4 = 0; r4 < 4; r4++) {
+                int row, col;
+                mm_pos(r4, row, col);
+                X[row * kJLD + col] = acc[r4];
+            }
         }
         if (fp) {
             const uint64_t qo = bl.q_off + (uint64_t)tk.x * kJS * kJS;
             for (int e = tid; e < kJS * kJS; e += 256)
                 Y[(e / kJS) * kJLD + (e % kJS)] = Q[qo + e];
             __syncthreads();
-            lds_mm<kJS, true>(X, Y, acc); // Q_P^T X
+            acc = lds_mm<true>(X, Y); // Q_P^T X
         } else {
             __syncthreads();
 #pragma unroll
-            for (int i = 0; i < RI; i++)
-#pragma unroll
-                for (int j = 0; j < 4; j++)
-                    acc[i][j] = X[(tr + kRP * i) * kJLD + tc + j];
+            for (int r4 = 0; r4 < 4; r4++) {
+                int row, col;
+                mm_pos(r4, row, col);
+                acc[r4] = X[row * kJLD + col];
+            }
         }
 #pragma unroll
-        for (int i = 0; i < RI; i++)
-#pragma unroll
-            for (int j = 0; j < 4; j++)
-                A[bl.a_off + (uint64_t)pair_index(pi, pj, tr + kRP * i) * N +
-                  pair_index(ri, rj, tc + j)] = acc[i][j];
+        for (int r4 = 0; r4 < 4; r4++) {
+            int row, col;
+            mm_pos(r4, row, col);
+            A[bl.a_off + (uint64_t)pair_index(pi, pj, row) * N +
+              pair_index(ri, rj, col)] = acc[r4];
+        }
     } else {
         if (!qflag[bl.f_off + tk.x])
             return;
@@ -424,14 +428,14 @@ __global__ __launch_bounds__(256) void jacobi_update_kernel(
         for (int e = tid; e < kJS * kJS; e += 256)
             Y[(e / kJS) * kJLD + (e % kJS)] = Q[qo + e];
         __syncthreads();
-        double acc[RI][4];
-        lds_mm<kJS, false>(X, Y, acc);
+        const jv4d acc = lds_mm<false>(X, Y);
 #pragma unroll
-        for (int i = 0; i < RI; i++)
-#pragma unroll
-            for (int j = 0; j < 4; j++)
-                V[bl.a_off + (uint64_t)(row0 + tr + kRP * i) * N +
-                  pair_index(pi, pj, tc + j)] = acc[i][j];
+        for (int r4 = 0; r4 < 4; r4++) {
+            int row, col;
+            mm_pos(r4, row, col);
+            V[bl.a_off + (uint64_t)(row0 + row) * N + pair_index(pi, pj, col)] =
+                acc[r4];
+        }
     }
 }
 
