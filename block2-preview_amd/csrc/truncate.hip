@@ -55,7 +55,6 @@ namespace {
 constexpr int kJB = 16;      // column block of the block Jacobi
 constexpr int kJS = 2 * kJB; // order of a pair subproblem
 constexpr int kJLD = kJS + 1;
-constexpr int kJG = 256 / kJS; // thread groups of a pair workgroup
 
 // one matrix of a batch: A and V are N x N (N = 32 nb) from a_off; the
 // round's Qs are nb / 2 matrices of 64 x 64 from q_off, their flags from
@@ -154,17 +153,18 @@ __global__ void jacobi_floor_kernel(JacBlock *__restrict__ blocks, int nblocks,
 // One kJS x kJS pair subproblem per workgroup: cyclic Jacobi in LDS,
 // sweeps until one applies no rotation (at most max_sweeps); writes the
 // accumulated rotation Q and whether it differs from I, and counts rotating
-// pairs.  Thread t owns column (row) t % kJS of pairs t / kJS + kJG i in the
-// row (column) phase; the pairs of a round touch disjoint rows and columns,
-// so each thread loads all its operands before storing.  The 2 x 2 block of
-// each pair, touched only by its own rotation, is written with its exact
-// rotated values in the column phase: three barriers per round.
+// pairs.  The kJB rotations of a round act on disjoint row / column pairs,
+// so thread (k, l) applies rotation k to the rows and rotation l to the
+// columns of the 2 x 2 block where they meet, reading its four entries
+// before writing them; the diagonal block of each rotating pair gets its
+// exact rotated values.  Two barriers per round: after the angles and
+// after the block updates.
 __global__ __launch_bounds__(256) void jacobi_pair_kernel(
     const JacBlock *__restrict__ blocks, const JacTask *__restrict__ tasks,
     const double *__restrict__ A, double *__restrict__ Q,
     int *__restrict__ qflag, int round, double tol, int max_sweeps,
     unsigned *__restrict__ nrot) {
-    constexpr int PT = kJB / kJG; // pairs per thread and phase
+    static_assert(kJB * kJB == 256, "one thread per 2 x 2 block of a round");
     __shared__ double S[kJS * kJLD];
     __shared__ double U[kJS * kJLD];
     __shared__ double rc[kJB], rs[kJB], rpp[kJB], rqq[kJB];
@@ -174,7 +174,7 @@ __global__ __launch_bounds__(256) void jacobi_pair_kernel(
     const JacTask tk = tasks[blockIdx.x];
     const JacBlock bl = blocks[tk.blk];
     const uint32_t N = bl.nb * kJB;
-    const int tid = threadIdx.x, li = tid % kJS, k0 = tid / kJS;
+    const int tid = threadIdx.x;
     // rotation test of entry (p, q): |a_pq| above the floor and above
     // tol sqrt(|a_pp a_qq|) (compared squared)
     const double tol2 = tol * tol;
@@ -245,56 +245,36 @@ __global__ __launch_bounds__(256) void jacobi_pair_kernel(
             // a round without rotations leaves S and U as they are
             if (rmask[r & 1] == 0u)
                 continue;
-            // rows p, q of the rotating pairs, column li
+            // the 2 x 2 block of pair k's rows and pair l's columns, for
+            // thread (k, l): rows rotated by pair k, then columns by pair l
+            // (a pair that does not rotate has c = 1, s = 0), and the
+            // columns of U by pair l.  The blocks partition S and U.
             {
-                double a[PT], b[PT];
-#pragma unroll
-                for (int i = 0; i < PT; i++) {
-                    const int k = k0 + kJG * i;
-                    if (rot[k]) {
-                        a[i] = S[rp[k] * kJLD + li];
-                        b[i] = S[rq[k] * kJLD + li];
-                    }
-                }
-#pragma unroll
-                for (int i = 0; i < PT; i++) {
-                    const int k = k0 + kJG * i;
-                    if (rot[k]) {
-                        const double c = rc[k], s = rs[k];
-                        S[rp[k] * kJLD + li] = c * a[i] - s * b[i];
-                        S[rq[k] * kJLD + li] = s * a[i] + c * b[i];
-                    }
-                }
-            }
-            __syncthreads();
-            // columns p, q of the rotating pairs, row li, and U
-            {
-                double a[PT], b[PT], u[PT], v[PT];
-#pragma unroll
-                for (int i = 0; i < PT; i++) {
-                    const int k = k0 + kJG * i;
-                    if (rot[k]) {
-                        a[i] = S[li * kJLD + rp[k]];
-                        b[i] = S[li * kJLD + rq[k]];
-                        u[i] = U[li * kJLD + rp[k]];
-                        v[i] = U[li * kJLD + rq[k]];
-                    }
-                }
-#pragma unroll
-                for (int i = 0; i < PT; i++) {
-                    const int k = k0 + kJG * i;
-                    if (rot[k]) {
-                        const int p = rp[k], q = rq[k];
-                        const double c = rc[k], s = rs[k];
-                        double np = c * a[i] - s * b[i], nq = s * a[i] + c * b[i];
-                        if (li == p)
-                            np = rpp[k], nq = 0.0;
-                        else if (li == q)
-                            np = 0.0, nq = rqq[k];
-                        S[li * kJLD + p] = np;
-                        S[li * kJLD + q] = nq;
-                        U[li * kJLD + p] = c * u[i] - s * v[i];
-                        U[li * kJLD + q] = s * u[i] + c * v[i];
+                const int k = tid >> 4, l = tid & 15;
+                const unsigned m = rmask[r & 1];
+                const bool rk = (m >> k) & 1u, rl = (m >> l) & 1u;
+                if (rk || rl) {
+                    const int pk = rp[k], qk = rq[k], pl = rp[l], ql = rq[l];
+                    const double ck = rc[k], sk = rs[k], cl = rc[l], sl = rs[l];
+                    const double a = S[pk * kJLD + pl], b = S[pk * kJLD + ql],
+                                 c = S[qk * kJLD + pl], d = S[qk * kJLD + ql];
+                    const double xa = ck * a - sk * c, xb = ck * b - sk * d;
+                    const double xc = sk * a + ck * c, xd = sk * b + ck * d;
+                    double na = cl * xa - sl * xb, nb = sl * xa + cl * xb;
+                    double nc = cl * xc - sl * xd, nd = sl * xc + cl * xd;
+                    if (k == l) // exact rotated 2 x 2 block of the pair
+                        na = rpp[k], nb = 0.0, nc = 0.0, nd = rqq[k];
+                    S[pk * kJLD + pl] = na;
+                    S[pk * kJLD + ql] = nb;
+                    S[qk * kJLD + pl] = nc;
+                    S[qk * kJLD + ql] = nd;
+                    if (rl) {
+                        const double u0 = U[pk * kJLD + pl], u1 = U[pk * kJLD + ql],
+                                     u2 = U[qk * kJLD + pl], u3 = U[qk * kJLD + ql];
+                        U[pk * kJLD + pl] = cl * u0 - sl * u1;
+                        U[pk * kJLD + ql] = sl * u0 + cl * u1;
+                        U[qk * kJLD + pl] = cl * u2 - sl * u3;
+                        U[qk * kJLD + ql] = sl * u2 + cl * u3;
                     }
                 }
             }
