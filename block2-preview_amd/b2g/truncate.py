@@ -24,6 +24,8 @@ TRUNC_REDUCED_INVERSED = 2
 TRUNC_REAL_DENSITY_MATRIX = 4
 TRUNC_SPECTRA_WITH_MULTIPLICITY = 8
 TRUNC_KEEP_ONE = 16
+NOISE_WAVEFUNCTION = 1
+NOISE_DENSITY_MATRIX = 2
 
 
 class BlockInfo:
@@ -129,18 +131,28 @@ class Split:
 
 def split_density_matrix(ctx, symmetry, info, psi, k, cutoff=0.0,
                          trace_right=True, normalize=True,
-                         trunc_type=TRUNC_PHYSICAL, scale=1.0, weights=None):
+                         trunc_type=TRUNC_PHYSICAL, scale=1.0, weights=None,
+                         noise=0.0, noise_type=NOISE_WAVEFUNCTION, seed=0):
     """Truncate the wavefunction psi (device FP64 tensor with the layout of
     `info`) to k states (-1: all) through its density matrix.  A 2-d psi
     holds state-averaged roots (one per row, density matrix weighted by
     `weights`, multi_split_density_matrix); the wavefunction factor then
-    comes back with one row per root."""
+    comes back with one row per root.  noise > 0 (one wavefunction) adds
+    block2's wavefunction or density-matrix noise, drawn from std::mt19937
+    seeded with `seed` as block2's Random after Random::rand_seed(seed)."""
     lib = ctx.lib
     ci = info.to_c()
     psi = psi.contiguous()
     torch.cuda.synchronize(ctx.device)
     h = C.c_void_p()
-    if psi.dim() == 1:
+    if noise:
+        assert psi.dim() == 1, "noise is added to one wavefunction"
+        L.check(lib.b2g_split_density_matrix_noise(
+            ctx.h, int(symmetry), C.byref(ci), C.c_void_p(psi.data_ptr()),
+            float(noise), int(noise_type), int(seed), int(k), float(cutoff),
+            int(bool(trace_right)), int(bool(normalize)), int(trunc_type),
+            float(scale), C.byref(h)))
+    elif psi.dim() == 1:
         L.check(lib.b2g_split_density_matrix(
             ctx.h, int(symmetry), C.byref(ci), C.c_void_p(psi.data_ptr()),
             int(k), float(cutoff), int(bool(trace_right)),

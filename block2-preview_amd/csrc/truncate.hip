@@ -45,6 +45,7 @@
 #include <cstring>
 #include <memory>
 #include <numeric>
+#include <random>
 #include <string>
 #include <utility>
 
@@ -521,6 +522,30 @@ __global__ __launch_bounds__(256) void row_copy_kernel(
 namespace {
 
 // grouped-GEMM tasks of one truncation product, uploaded in one allocation
+// Density-matrix noise: D_b[i][j] += coef[b] T_b[i][j] on the lower
+// triangle (i >= j), mirrored to the upper one, so that the matrix handed
+// to the eigensolver is the symmetric matrix of its lower triangle.  Upper
+// entries are only written, never read: no thread depends on another.
+__global__ __launch_bounds__(256) void dm_noise_kernel(
+    const uint64_t *__restrict__ off, const uint32_t *__restrict__ dim,
+    const double *__restrict__ coef, int nb, const double *__restrict__ T,
+    double *__restrict__ D) {
+    for (int b = blockIdx.y; b < nb; b += gridDim.y) {
+        const uint32_t n = dim[b];
+        const uint64_t o = off[b], nn = (uint64_t)n * n;
+        const double c = coef[b];
+        for (uint64_t e = (uint64_t)blockIdx.x * 256 + threadIdx.x; e < nn;
+             e += (uint64_t)gridDim.x * 256) {
+            const uint32_t i = (uint32_t)(e / n), j = (uint32_t)(e % n);
+            if (j > i)
+                continue;
+            const double v = D[o + e] + c * T[o + e];
+            D[o + e] = v;
+            D[o + (uint64_t)j * n + i] = v;
+        }
+    }
+}
+
 struct TaskSet {
     std::vector<Task> t[kTileConfigs];
     std::vector<Seg> s;
@@ -891,10 +916,98 @@ void set_shifts(b2g_split_s::Part &p) {
 
 namespace b2g {
 
+// OperatorFunctions::trans_product's noise (src/core/operator_functions.hpp:
+// 776-835, called by MovingEnvironment::density_matrix with sqrt(noise),
+// moving_environment.hpp:3554-3577), with block2's generator
+// (Random::fill, src/core/utils.hpp:226-252): std::mt19937 seeded with
+// `seed`, one uniform_real_distribution<double> draw per element in storage
+// order.
+//   wavefunction:   T over the wavefunction's layout, uniform in
+//                   [-0.5, 0.5); D += (noise / |T|^2) T T^T per wavefunction
+//                   block (the density-matrix products of T)
+//   density matrix: T over the density matrix's layout, uniform in [0, 1);
+//                   D_b += (noise / |T|) T_b once per wavefunction block of
+//                   sector b; the eigensolver then sees the lower triangle
+//                   (block2's xsyev "U" on row-major storage)
+static int add_noise(b2g_ctx ctx, const b2g_sparse_info *wfn, uint64_t psi_len,
+                     const b2g_split_s::Part &dm, const std::vector<int> &wfn_dm,
+                     const TaskSet &ts, int trace_right, double noise,
+                     int noise_type, unsigned seed) {
+    (void)trace_right;
+    const bool wav = noise_type == B2G_NOISE_WAVEFUNCTION;
+    const uint64_t len = wav ? psi_len : dm.len;
+    if (len == 0)
+        return B2G_OK;
+    std::vector<double> T(len);
+    {
+        std::mt19937 gen(seed);
+        std::uniform_real_distribution<double> distr(wav ? -0.5 : 0.0,
+                                                     wav ? 0.5 : 1.0);
+        for (uint64_t i = 0; i < len; i++)
+            T[i] = distr(gen);
+    }
+    double ss = 0.0;
+    for (uint64_t i = 0; i < len; i++)
+        ss += T[i] * T[i];
+    const double nrm = std::sqrt(ss);
+    B2G_REQUIRE(nrm > 0.0, B2G_ERR_ARG, "noise: zero random tensor");
+    double *dT = nullptr;
+    B2G_HIP(hipMalloc(&dT, len * sizeof(double)));
+    std::unique_ptr<double, int (*)(void *)> t_guard(
+        dT, [](void *p) { return (int)hipFree(p); });
+    B2G_HIP(hipMemcpy(dT, T.data(), len * sizeof(double),
+                      hipMemcpyHostToDevice));
+    if (wav) {
+        // the same products as the density matrix, on T, accumulated
+        TaskSet tn;
+        tn.s = ts.s;
+        for (Seg &sg : tn.s)
+            sg.alpha = noise / ss;
+        for (int c = 0; c < kTileConfigs; c++) {
+            tn.t[c] = ts.t[c];
+            for (Task &t : tn.t[c])
+                t.beta |= 1u;
+        }
+        int r = tn.upload(ctx);
+        if (r == B2G_OK)
+            r = tn.run(ctx, dT, dT, dm.data);
+        if (r != B2G_OK)
+            return r;
+        B2G_HIP(hipStreamSynchronize(ctx->stream));
+        return B2G_OK;
+    }
+    const int nd = (int)dm.quanta.size();
+    std::vector<double> coef(nd, 0.0);
+    for (size_t i = 0; i < wfn_dm.size(); i++)
+        coef[wfn_dm[i]] += noise / nrm;
+    std::vector<uint32_t> dim(dm.nb.begin(), dm.nb.end());
+    const size_t bytes = nd * (sizeof(uint64_t) + sizeof(uint32_t) + sizeof(double));
+    char *dbuf = nullptr;
+    B2G_HIP(hipMalloc(&dbuf, bytes));
+    std::unique_ptr<char, int (*)(void *)> b_guard(
+        dbuf, [](void *p) { return (int)hipFree(p); });
+    uint64_t *d_off = (uint64_t *)dbuf;
+    double *d_coef = (double *)(dbuf + nd * sizeof(uint64_t));
+    uint32_t *d_dim = (uint32_t *)(dbuf + nd * (sizeof(uint64_t) + sizeof(double)));
+    B2G_HIP(hipMemcpy(d_off, dm.shift.data(), nd * sizeof(uint64_t),
+                      hipMemcpyHostToDevice));
+    B2G_HIP(hipMemcpy(d_coef, coef.data(), nd * sizeof(double),
+                      hipMemcpyHostToDevice));
+    B2G_HIP(hipMemcpy(d_dim, dim.data(), nd * sizeof(uint32_t),
+                      hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(dm_noise_kernel, dim3(16, (unsigned)std::min(nd, 4096)),
+                       dim3(256), 0, ctx->stream, d_off, d_dim, d_coef, nd, dT,
+                       dm.data);
+    B2G_HIP(hipGetLastError());
+    B2G_HIP(hipStreamSynchronize(ctx->stream));
+    return B2G_OK;
+}
+
 static int split_impl(b2g_ctx ctx, int symmetry, const b2g_sparse_info *wfn,
                       const double *psi, int n_roots, const double *weights,
                       int k, double cutoff, int trace_right, int normalize,
-                      unsigned trunc_type, double scale, b2g_split *out) {
+                      unsigned trunc_type, double scale, double noise,
+                      int noise_type, unsigned seed, b2g_split *out) {
     B2G_REQUIRE(ctx && wfn && out && (psi || wfn->n == 0), B2G_ERR_ARG,
                 "NULL arg");
     B2G_REQUIRE(n_roots >= 1 && weights != nullptr, B2G_ERR_ARG,
@@ -907,6 +1020,12 @@ static int split_impl(b2g_ctx ctx, int symmetry, const b2g_sparse_info *wfn,
                                                 wfn->block_shifts)),
                 B2G_ERR_ARG, "bad wavefunction info");
     B2G_REQUIRE(k >= -1, B2G_ERR_ARG, "k must be -1 (keep all) or >= 0");
+    B2G_REQUIRE(noise == 0.0 || ((noise_type == B2G_NOISE_WAVEFUNCTION ||
+                                  noise_type == B2G_NOISE_DENSITY_MATRIX) &&
+                                 noise > 0.0 && n_roots == 1),
+                B2G_ERR_ARG,
+                "noise: positive, of type wavefunction or density matrix, "
+                "one root");
     *out = nullptr;
     B2G_HIP(hipSetDevice(ctx->device));
     SetupTimer tm("split");
@@ -998,6 +1117,12 @@ static int split_impl(b2g_ctx ctx, int symmetry, const b2g_sparse_info *wfn,
         if (r != B2G_OK)
             return r;
         B2G_HIP(hipStreamSynchronize(ctx->stream));
+        if (noise != 0.0) {
+            r = add_noise(ctx, wfn, psi_len, dm, wfn_dm, ts, trace_right, noise,
+                          noise_type, seed);
+            if (r != B2G_OK)
+                return r;
+        }
     }
     tm.mark("density matrix");
     // ---- eigen-decomposition of every block
@@ -1225,7 +1350,19 @@ extern "C" int b2g_split_density_matrix(b2g_ctx ctx, int symmetry,
                                         double scale, b2g_split *out) {
     const double one = 1.0;
     return split_impl(ctx, symmetry, wfn, psi, 1, &one, k, cutoff,
-                      trace_right, normalize, trunc_type, scale, out);
+                      trace_right, normalize, trunc_type, scale, 0.0, 0, 0u,
+                      out);
+}
+
+extern "C" int b2g_split_density_matrix_noise(
+    b2g_ctx ctx, int symmetry, const b2g_sparse_info *wfn, const double *psi,
+    double noise, int noise_type, unsigned seed, int k, double cutoff,
+    int trace_right, int normalize, unsigned trunc_type, double scale,
+    b2g_split *out) {
+    const double one = 1.0;
+    return split_impl(ctx, symmetry, wfn, psi, 1, &one, k, cutoff,
+                      trace_right, normalize, trunc_type, scale, noise,
+                      noise_type, seed, out);
 }
 
 extern "C" int b2g_split_density_matrix_multi(
@@ -1233,7 +1370,8 @@ extern "C" int b2g_split_density_matrix_multi(
     int n_roots, const double *weights, int k, double cutoff, int trace_right,
     int normalize, unsigned trunc_type, double scale, b2g_split *out) {
     return split_impl(ctx, symmetry, wfn, psi, n_roots, weights, k, cutoff,
-                      trace_right, normalize, trunc_type, scale, out);
+                      trace_right, normalize, trunc_type, scale, 0.0, 0, 0u,
+                      out);
 }
 
 extern "C" int b2g_split_n_roots(b2g_split s, int *n) {

@@ -441,6 +441,9 @@ int b2g_syev_batched(b2g_ctx ctx, int n_mats, const uint32_t *n,
 #define B2G_TRUNC_REAL_DENSITY_MATRIX 4u
 #define B2G_TRUNC_SPECTRA_WITH_MULTIPLICITY 8u
 #define B2G_TRUNC_KEEP_ONE 16u
+/* block2's NoiseTypes values taken by b2g_split_density_matrix_noise */
+#define B2G_NOISE_WAVEFUNCTION 1
+#define B2G_NOISE_DENSITY_MATRIX 2
 
 typedef struct b2g_split_s *b2g_split;
 /* Density-matrix truncation of a two-site wavefunction on the device
@@ -474,6 +477,23 @@ int b2g_split_density_matrix_multi(b2g_ctx ctx, int symmetry,
                                    int normalize, unsigned trunc_type,
                                    double scale, b2g_split *out);
 int b2g_split_n_roots(b2g_split s, int *n);
+/* With noise (MovingEnvironment::density_matrix with noise > 0 and
+ * NoiseTypes Wavefunction or DensityMatrix, src/dmrg/moving_environment.hpp:
+ * 3554-3577, through OperatorFunctions::trans_product,
+ * src/core/operator_functions.hpp:776-835): the random tensor is drawn on
+ * the host from std::mt19937 seeded with `seed`, in block2's order
+ * (Random::fill, src/core/utils.hpp:226-252), so a reference run whose
+ * generator was seeded with the same value (Random::rand_seed(seed))
+ * adds the same noise.  One wavefunction.  Density-matrix noise leaves the
+ * density matrix symmetric from its lower triangle, the part block2's
+ * xsyev reads.                                                           */
+int b2g_split_density_matrix_noise(b2g_ctx ctx, int symmetry,
+                                   const b2g_sparse_info *wfn,
+                                   const double *psi, double noise,
+                                   int noise_type, unsigned seed, int k,
+                                   double cutoff, int trace_right,
+                                   int normalize, unsigned trunc_type,
+                                   double scale, b2g_split *out);
 /* Result parts: which 0 = left factor, 1 = right factor, 2 = the density
  * matrix (before diagonalisation).  The info arrays are owned by the
  * split; data is device memory owned by the split (the wavefunction

@@ -248,17 +248,27 @@ template <typename S> struct Dumper : CallbackKernel {
             bool trace_right;
             int num, den;
             TruncationTypes tt;
+            double noise;
+            NoiseTypes nt;
         };
-        const Case cases[3] = {
-            {"sR", true, 2, 3, TruncationTypes::Physical},
-            {"sL", false, 1, 2, TruncationTypes::Physical},
+        // noisy cases draw from block2's generator freshly seeded with 4321
+        const Case cases[5] = {
+            {"sR", true, 2, 3, TruncationTypes::Physical, 0.0, NoiseTypes::None},
+            {"sL", false, 1, 2, TruncationTypes::Physical, 0.0, NoiseTypes::None},
             {"sK", false, 1, 3,
-             TruncationTypes::Reduced | TruncationTypes::KeepOne}};
+             TruncationTypes::Reduced | TruncationTypes::KeepOne, 0.0,
+             NoiseTypes::None},
+            {"sW", true, 2, 3, TruncationTypes::Physical, 1E-6,
+             NoiseTypes::Wavefunction},
+            {"sD", false, 2, 3, TruncationTypes::Physical, 1E-6,
+             NoiseTypes::DensityMatrix}};
         const double cutoff = 1E-14;
         for (const Case &c : cases) {
             const string t = string(c.tag) + "_";
+            if (c.noise != 0.0)
+                Random::rand_seed(4321);
             auto dm = MovingEnvironment<S, double, double>::density_matrix(
-                S(0), wfn, c.trace_right, 0.0, NoiseTypes::None);
+                S(0), wfn, c.trace_right, c.noise, c.nt);
             InfoPack<S> dp;
             dp.add(dm->info);
             dp.write(d, t + "dm");
@@ -286,6 +296,8 @@ template <typename S> struct Dumper : CallbackKernel {
             vector<double> scal = {err, (double)k, cutoff,
                                    (double)(ubond_t)c.tt,
                                    c.trace_right ? 1.0 : 0.0};
+            if (c.noise != 0.0) // noise, its type (block2's NoiseTypes), seed
+                scal.insert(scal.end(), {c.noise, (double)(int)c.nt, 4321.0});
             write_npy(d + "/" + t + "settings.npy", scal);
             // the stacks are LIFO: release as the sweep does
             // (src/dmrg/sweep_algorithm.hpp:644-657)

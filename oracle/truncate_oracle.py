@@ -11,6 +11,8 @@ Follows:
   OperatorFunctions::trans_product (noise 0) src/core/operator_functions.hpp:776-835
   MovingEnvironment::truncate_density_matrix src/dmrg/moving_environment.hpp:3716-3853
   MovingEnvironment::split_density_matrix    src/dmrg/moving_environment.hpp:4250-4345
+  trans_product noise, Random::fill          src/core/operator_functions.hpp:776-835,
+                                             src/core/utils.hpp:226-252
   rotation_matrix_info_from_density_matrix / wavefunction_info_from_density_matrix
                                              src/dmrg/moving_environment.hpp:4013-4083
   GMatrixFunctions::eigs (xsyev)             src/core/matrix_functions.hpp:1672-1688
@@ -96,6 +98,56 @@ def density_matrix(sym, info, psi, trace_right, scale=1.0):
     return uq, dims, blocks
 
 
+def block2_uniform(seed, n, a, b):
+    """n draws of block2's Random::fill (src/core/utils.hpp:226-252) after
+    Random::rand_seed(seed): std::mt19937(seed) through libstdc++'s
+    uniform_real_distribution<double>(a, b), i.e. generate_canonical with
+    two 32-bit outputs x0, x1: u = (x0 + x1 2^32) / 2^64 (below 1), then
+    u (b - a) + a.  numpy's RandomState(seed) runs the same MT19937 stream
+    (init_genrand seeding); randint over [0, 2^32) returns its raw 32-bit
+    outputs."""
+    rs = np.random.RandomState(seed)
+    x = rs.randint(0, 2 ** 32, size=2 * n, dtype=np.uint64).astype(np.float64)
+    u = (x[0::2] + x[1::2] * 2.0 ** 32) / 2.0 ** 64
+    u = np.where(u >= 1.0, np.nextafter(1.0, 0.0), u)
+    return u * (b - a) + a
+
+
+def add_noise(sym, info, blocks, uq, trace_right, noise, noise_type, seed):
+    """OperatorFunctions::trans_product's noise (src/core/operator_functions.
+    hpp:776-835) as MovingEnvironment::density_matrix calls it with
+    sqrt(noise) (src/dmrg/moving_environment.hpp:3554-3577).  Returns the
+    density-matrix blocks as block2 stores them (density-matrix noise makes
+    them asymmetric)."""
+    qs = info["quanta"]
+    dq = int(info["dq"])
+    lab = [q_get_bra(sym, int(q), dq) if trace_right
+           else q_neg(sym, q_get_ket(sym, int(q))) for q in qs]
+    if noise_type == 1:  # Wavefunction
+        n = sum(int(m) * int(k) for m, k in zip(info["nbra"], info["nket"]))
+        t = block2_uniform(seed, n, -0.5, 0.5)
+        ns = noise / float(t @ t)
+        _, _, tb = density_matrix(sym, info, t, trace_right, ns)
+        return [b + x for b, x in zip(blocks, tb)]
+    # DensityMatrix: T over the density matrix's layout, once per
+    # wavefunction block of each sector
+    n = sum(b.size for b in blocks)
+    t = block2_uniform(seed, n, 0.0, 1.0)
+    ns = noise / float(np.sqrt(t @ t))
+    out, o = [], 0
+    for bi, b in enumerate(blocks):
+        cnt = sum(1 for l in lab if uq.index(l) == bi)
+        out.append(b + cnt * ns * t[o:o + b.size].reshape(b.shape))
+        o += b.size
+    return out
+
+
+def lower_symmetric(b):
+    """the symmetric matrix of b's lower triangle (block2's xsyev "U" on
+    row-major storage reads the row-major lower triangle)"""
+    return np.tril(b) + np.tril(b, -1).T
+
+
 def truncate(sym, quanta, w, k, cutoff, trunc_type):
     """The reference's selection: w = list of ascending eigenvalue arrays per
     density-matrix block.  Returns (ss, error) with ss the kept (block,
@@ -149,12 +201,20 @@ def truncate(sym, quanta, w, k, cutoff, trunc_type):
 
 
 def split_density_matrix(sym, info, psi, k, cutoff, trace_right,
-                         normalize=True, trunc_type=0, scale=1.0):
+                         normalize=True, trunc_type=0, scale=1.0, noise=0.0,
+                         noise_type=0, seed=0):
     """Returns dict(error, eigvals (list per block), dm=(quanta, dims,
-    blocks), left=(quanta, nbra, nket, data), right=(...), spectra)."""
+    blocks), left=(quanta, nbra, nket, data), right=(...), spectra).
+    noise > 0 adds block2's wavefunction (noise_type 1) or density-matrix
+    (2) noise drawn as after Random::rand_seed(seed); dm then holds the
+    blocks as block2 stores them, and the eigenproblems take their lower
+    triangles."""
     uq, dims, blocks = density_matrix(sym, info, psi, trace_right, scale)
-    eig = [signed_eigh(b) if b.size else (np.zeros(0), np.zeros((0, 0)))
-           for b in blocks]
+    if noise:
+        blocks = add_noise(sym, info, blocks, uq, trace_right, noise,
+                           noise_type, seed)
+    eig = [signed_eigh(lower_symmetric(b)) if b.size
+           else (np.zeros(0), np.zeros((0, 0))) for b in blocks]
     w = [e[0] for e in eig]
     ss, error = truncate(sym, uq, w, k, cutoff, trunc_type)
     with_multi = bool(trunc_type & 8)

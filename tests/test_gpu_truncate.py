@@ -8,8 +8,9 @@ numpy.linalg.eigh to 1e-12 * ||A||_2, residual ||A v - w v|| and
 orthonormality to 1e-11 * ||A||_2; eigenvectors of simple eigenvalues equal
 to the oracle's signed ones to 1e-9.
 
-Split: on every golden fixture and truncation case, against the
-reference's own dumps (the oracle is pinned to the same dumps on the CPU):
+Split: on every golden fixture and truncation case (three without noise,
+two with block2's wavefunction / density-matrix noise drawn from its
+generator seeded with 4321), against the reference's own dumps (the oracle is pinned to the same dumps on the CPU):
 density matrix to 1e-13, kept states per sector exactly (up to states
 whose eigenvalue lies within 1e-13 of the cutoff, where FP64 eigensolvers
 may fall on either side; then the structure and reconstruction
@@ -41,7 +42,13 @@ from b2g import truncate as TR
 
 pytestmark = pytest.mark.gpu
 
-CASES = ["sR", "sL", "sK"]
+CASES = ["sR", "sL", "sK", "sW", "sD"]
+
+
+def settings(z, tag):
+    """err, k, cutoff, trunc_type, trace_right, noise, noise_type, seed"""
+    st = [float(x) for x in z[tag + "_settings"]]
+    return st + [0.0, 0.0, 0.0][:8 - len(st)]
 
 
 def dev(x):
@@ -180,16 +187,24 @@ def test_split_matches_reference(gpu_ctx, name, tag):
     z, sym, info = load(name)
     if tag + "_settings" not in z.files:
         pytest.skip("fixture without truncation dump")
-    err_ref, k, cutoff, tt, tr = z[tag + "_settings"]
+    err_ref, k, cutoff, tt, tr, noise, nt, seed = settings(z, tag)
     print("case", name, tag, flush=True)
     s = TR.split_density_matrix(gpu_ctx, sym, info, dev(z["psi_dav"]),
-                                int(k), cutoff, bool(tr), True, int(tt))
-    # density matrix
+                                int(k), cutoff, bool(tr), True, int(tt),
+                                noise=noise, noise_type=int(nt),
+                                seed=int(seed))
+    # density matrix (with density-matrix noise block2's is asymmetric and
+    # its eigensolver reads the lower triangle; the device keeps that
+    # triangle, mirrored)
     dmi, dmd = s.dm
     p = tag + "_dm"
     assert np.array_equal(dmi.quanta, z[p + "_quanta"].astype(np.uint64))
     assert np.array_equal(dmi.nbra, z[p + "_nbra"])
-    assert np.abs(dmd.cpu().numpy() - z[p + "_data"]).max(initial=0) < 1e-13
+    dref = {int(q): T.lower_symmetric(dmi.block(z[p + "_data"], i))
+            for i, q in enumerate(dmi.quanta)}
+    dd = dmd.cpu().numpy()
+    for i, q in enumerate(dmi.quanta):
+        assert np.abs(dmi.block(dd, i) - dref[int(q)]).max(initial=0) < 1e-13
     # error, spectra
     assert abs(s.error - err_ref) < 1e-12, (name, tag, s.error, err_ref)
     sp = z[tag + "_spectra"]
@@ -197,8 +212,6 @@ def test_split_matches_reference(gpu_ctx, name, tag):
     assert np.abs(s.spectra ** 2 - sp ** 2).max(initial=0) < 1e-13
     strict = kept_matches(z, tag, s, sym, int(tt), bool(tr), cutoff, name)
     mine = {"left": part_blocks(*s.left), "right": part_blocks(*s.right)}
-    dref = {int(q): dmi.block(z[tag + "_dm_data"], i)
-            for i, q in enumerate(dmi.quanta)}
     check_factors(sym, info, z["psi_dav"], mine, dref, bool(tr), name)
     if not strict:
         return

@@ -2,8 +2,13 @@
 
 The golden fixtures hold, for the Davidson vector of each dumped site, the
 reference's own MovingEnvironment::density_matrix and split_density_matrix
-results (oracle/ref_harness.cpp dump_split) in three cases: trace_right
-Physical (sR), trace_left Physical (sL), trace_left Reduced | KeepOne (sK).
+results (oracle/ref_harness.cpp dump_split) in five cases: trace_right
+Physical (sR), trace_left Physical (sL), trace_left Reduced | KeepOne (sK),
+and with noise 1e-6 drawn from block2's generator seeded with 4321:
+wavefunction noise, trace_right (sW), density-matrix noise, trace_left
+(sD).  The noisy density matrices pin the oracle's restatement of block2's
+random stream (std::mt19937 through uniform_real_distribution) and of where
+the noise enters.
 Checked here, on the CPU:
   * density-matrix structure exact and data to 1e-13 (absolute; the
     density matrix of a normalised vector has trace 1);
@@ -30,7 +35,13 @@ import pytest
 import truncate_oracle as T
 from conftest import FIXTURES, DELAYED_FIXTURES, fixture_path
 
-CASES = ["sR", "sL", "sK"]
+CASES = ["sR", "sL", "sK", "sW", "sD"]
+
+
+def settings(z, tag):
+    """err, k, cutoff, trunc_type, trace_right, noise, noise_type, seed"""
+    st = [float(x) for x in z[tag + "_settings"]]
+    return st + [0.0, 0.0, 0.0][:8 - len(st)]
 ALL = FIXTURES + DELAYED_FIXTURES
 
 
@@ -84,10 +95,11 @@ def test_oracle_split_matches_reference(name, tag):
     z, sym, info = load(name)
     if tag + "_settings" not in z.files:
         pytest.skip("fixture without truncation dump")
-    err_ref, k, cutoff, tt, tr = z[tag + "_settings"]
+    err_ref, k, cutoff, tt, tr, noise, nt, seed = settings(z, tag)
     psi = z["psi_dav"]
     res = T.split_density_matrix(sym, info, psi, int(k), cutoff, bool(tr),
-                                 True, int(tt))
+                                 True, int(tt), 1.0, noise, int(nt),
+                                 int(seed))
     # density matrix
     dq, dn, dnk, dsh, ddata = ref_part(z, tag, "dm")
     uq, dims, blocks = res["dm"]
@@ -164,6 +176,21 @@ def test_planted_selection_mistake_is_caught():
     finally:
         T.truncate = orig
     assert abs(res["error"] - err_ref) > 1e-6
+
+
+def test_noise_depends_on_the_seed():
+    """a different seed gives a different noisy density matrix: the
+    reference comparison of the noisy cases checks the random stream"""
+    z, sym, info = load("hub8_sz")
+    err_ref, k, cutoff, tt, tr, noise, nt, seed = settings(z, "sD")
+    a = T.split_density_matrix(sym, info, z["psi_dav"], int(k), cutoff,
+                               bool(tr), True, int(tt), 1.0, noise, int(nt),
+                               int(seed) + 1)
+    sh, dd = z["sD_dm_shift"], z["sD_dm_data"]
+    uq, dims, blocks = a["dm"]
+    diff = max(np.abs(b - dd[int(sh[i]):int(sh[i]) + dims[i] ** 2].reshape(
+        dims[i], dims[i])).max(initial=0) for i, b in enumerate(blocks))
+    assert diff > 1e-10
 
 
 def test_label_algebra():
