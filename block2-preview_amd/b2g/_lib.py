@@ -200,9 +200,9 @@ _SIGS = {
         C.POINTER(_vp)]),
     "b2g_split_n_roots": (C.c_int, [_vp, C.POINTER(C.c_int)]),
     "b2g_split_density_matrix_noise": (C.c_int, [
-        _vp, C.c_int, C.POINTER(b2g_sparse_info), _vp, C.c_double, C.c_int,
-        C.c_uint, C.c_int, C.c_double, C.c_int, C.c_int, C.c_uint,
-        C.c_double, C.POINTER(_vp)]),
+        _vp, C.c_int, C.POINTER(b2g_sparse_info), _vp, C.c_int, f64p,
+        C.c_double, C.c_int, C.c_uint, C.c_int, C.c_double, C.c_int,
+        C.c_int, C.c_uint, C.c_double, C.POINTER(_vp)]),
     "b2g_smat_load": (C.c_int, [C.c_char_p, C.c_int, C.c_int,
                                 C.POINTER(_vp)]),
     "b2g_smat_get": (C.c_int, [_vp, C.POINTER(b2g_sparse_info),

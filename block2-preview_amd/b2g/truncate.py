@@ -137,21 +137,26 @@ def split_density_matrix(ctx, symmetry, info, psi, k, cutoff=0.0,
     `info`) to k states (-1: all) through its density matrix.  A 2-d psi
     holds state-averaged roots (one per row, density matrix weighted by
     `weights`, multi_split_density_matrix); the wavefunction factor then
-    comes back with one row per root.  noise > 0 (one wavefunction) adds
-    block2's wavefunction or density-matrix noise, drawn from std::mt19937
-    seeded with `seed` as block2's Random after Random::rand_seed(seed)."""
+    comes back with one row per root.  noise > 0 adds block2's
+    wavefunction or density-matrix noise (for each root), drawn from
+    std::mt19937 seeded with `seed` as block2's Random after
+    Random::rand_seed(seed)."""
     lib = ctx.lib
     ci = info.to_c()
     psi = psi.contiguous()
     torch.cuda.synchronize(ctx.device)
     h = C.c_void_p()
     if noise:
-        assert psi.dim() == 1, "noise is added to one wavefunction"
+        nr = 1 if psi.dim() == 1 else psi.shape[0]
+        w = np.ascontiguousarray(
+            np.full(nr, 1.0 / nr) if weights is None else weights,
+            dtype=np.float64)
+        assert w.size == nr
         L.check(lib.b2g_split_density_matrix_noise(
             ctx.h, int(symmetry), C.byref(ci), C.c_void_p(psi.data_ptr()),
-            float(noise), int(noise_type), int(seed), int(k), float(cutoff),
-            int(bool(trace_right)), int(bool(normalize)), int(trunc_type),
-            float(scale), C.byref(h)))
+            nr, L.ptr(w, C.c_double), float(noise), int(noise_type),
+            int(seed), int(k), float(cutoff), int(bool(trace_right)),
+            int(bool(normalize)), int(trunc_type), float(scale), C.byref(h)))
     elif psi.dim() == 1:
         L.check(lib.b2g_split_density_matrix(
             ctx.h, int(symmetry), C.byref(ci), C.c_void_p(psi.data_ptr()),

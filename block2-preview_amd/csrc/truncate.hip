@@ -929,40 +929,58 @@ namespace b2g {
 //                   D_b += (noise / |T|) T_b once per wavefunction block of
 //                   sector b; the eigensolver then sees the lower triangle
 //                   (block2's xsyev "U" on row-major storage)
+// State-averaged roots (density_matrix_with_multi_target,
+// moving_environment.hpp:3582-3608) each draw their own T from the
+// continuing stream, in root order.
 static int add_noise(b2g_ctx ctx, const b2g_sparse_info *wfn, uint64_t psi_len,
-                     const b2g_split_s::Part &dm, const std::vector<int> &wfn_dm,
-                     const TaskSet &ts, int trace_right, double noise,
-                     int noise_type, unsigned seed) {
-    (void)trace_right;
+                     int n_roots, const b2g_split_s::Part &dm,
+                     const std::vector<int> &wfn_dm, const TaskSet &ts,
+                     double noise, int noise_type, unsigned seed) {
+    (void)wfn;
     const bool wav = noise_type == B2G_NOISE_WAVEFUNCTION;
+    // one draw per root from the continuing stream (every root's
+    // trans_product draws its own tensor)
     const uint64_t len = wav ? psi_len : dm.len;
     if (len == 0)
         return B2G_OK;
-    std::vector<double> T(len);
+    std::vector<double> T(len * n_roots), scal(n_roots);
     {
         std::mt19937 gen(seed);
         std::uniform_real_distribution<double> distr(wav ? -0.5 : 0.0,
                                                      wav ? 0.5 : 1.0);
-        for (uint64_t i = 0; i < len; i++)
+        for (uint64_t i = 0; i < len * n_roots; i++)
             T[i] = distr(gen);
     }
-    double ss = 0.0;
-    for (uint64_t i = 0; i < len; i++)
-        ss += T[i] * T[i];
-    const double nrm = std::sqrt(ss);
-    B2G_REQUIRE(nrm > 0.0, B2G_ERR_ARG, "noise: zero random tensor");
+    for (int r = 0; r < n_roots; r++) {
+        double ss = 0.0;
+        for (uint64_t i = 0; i < len; i++)
+            ss += T[r * len + i] * T[r * len + i];
+        B2G_REQUIRE(ss > 0.0, B2G_ERR_ARG, "noise: zero random tensor");
+        scal[r] = wav ? noise / ss : noise / std::sqrt(ss);
+    }
+    if (!wav) {
+        // the roots' noise summed on the host: sum_r scal[r] T_r
+        for (uint64_t i = 0; i < len; i++) {
+            double v = scal[0] * T[i];
+            for (int r = 1; r < n_roots; r++)
+                v += scal[r] * T[r * len + i];
+            T[i] = v;
+        }
+        T.resize(len);
+    }
     double *dT = nullptr;
-    B2G_HIP(hipMalloc(&dT, len * sizeof(double)));
+    B2G_HIP(hipMalloc(&dT, T.size() * sizeof(double)));
     std::unique_ptr<double, int (*)(void *)> t_guard(
         dT, [](void *p) { return (int)hipFree(p); });
-    B2G_HIP(hipMemcpy(dT, T.data(), len * sizeof(double),
+    B2G_HIP(hipMemcpy(dT, T.data(), T.size() * sizeof(double),
                       hipMemcpyHostToDevice));
     if (wav) {
-        // the same products as the density matrix, on T, accumulated
+        // the same products as the density matrix, on each root's T
+        // (segments of root r start at r * psi_len), accumulated
         TaskSet tn;
         tn.s = ts.s;
         for (Seg &sg : tn.s)
-            sg.alpha = noise / ss;
+            sg.alpha = scal[(int)(sg.a_off / psi_len)];
         for (int c = 0; c < kTileConfigs; c++) {
             tn.t[c] = ts.t[c];
             for (Task &t : tn.t[c])
@@ -979,7 +997,7 @@ static int add_noise(b2g_ctx ctx, const b2g_sparse_info *wfn, uint64_t psi_len,
     const int nd = (int)dm.quanta.size();
     std::vector<double> coef(nd, 0.0);
     for (size_t i = 0; i < wfn_dm.size(); i++)
-        coef[wfn_dm[i]] += noise / nrm;
+        coef[wfn_dm[i]] += 1.0;
     std::vector<uint32_t> dim(dm.nb.begin(), dm.nb.end());
     const size_t bytes = nd * (sizeof(uint64_t) + sizeof(uint32_t) + sizeof(double));
     char *dbuf = nullptr;
@@ -1022,10 +1040,9 @@ static int split_impl(b2g_ctx ctx, int symmetry, const b2g_sparse_info *wfn,
     B2G_REQUIRE(k >= -1, B2G_ERR_ARG, "k must be -1 (keep all) or >= 0");
     B2G_REQUIRE(noise == 0.0 || ((noise_type == B2G_NOISE_WAVEFUNCTION ||
                                   noise_type == B2G_NOISE_DENSITY_MATRIX) &&
-                                 noise > 0.0 && n_roots == 1),
+                                 noise > 0.0),
                 B2G_ERR_ARG,
-                "noise: positive, of type wavefunction or density matrix, "
-                "one root");
+                "noise: positive, of type wavefunction or density matrix");
     *out = nullptr;
     B2G_HIP(hipSetDevice(ctx->device));
     SetupTimer tm("split");
@@ -1118,7 +1135,7 @@ static int split_impl(b2g_ctx ctx, int symmetry, const b2g_sparse_info *wfn,
             return r;
         B2G_HIP(hipStreamSynchronize(ctx->stream));
         if (noise != 0.0) {
-            r = add_noise(ctx, wfn, psi_len, dm, wfn_dm, ts, trace_right, noise,
+            r = add_noise(ctx, wfn, psi_len, n_roots, dm, wfn_dm, ts, noise,
                           noise_type, seed);
             if (r != B2G_OK)
                 return r;
@@ -1356,11 +1373,10 @@ extern "C" int b2g_split_density_matrix(b2g_ctx ctx, int symmetry,
 
 extern "C" int b2g_split_density_matrix_noise(
     b2g_ctx ctx, int symmetry, const b2g_sparse_info *wfn, const double *psi,
-    double noise, int noise_type, unsigned seed, int k, double cutoff,
-    int trace_right, int normalize, unsigned trunc_type, double scale,
-    b2g_split *out) {
-    const double one = 1.0;
-    return split_impl(ctx, symmetry, wfn, psi, 1, &one, k, cutoff,
+    int n_roots, const double *weights, double noise, int noise_type,
+    unsigned seed, int k, double cutoff, int trace_right, int normalize,
+    unsigned trunc_type, double scale, b2g_split *out) {
+    return split_impl(ctx, symmetry, wfn, psi, n_roots, weights, k, cutoff,
                       trace_right, normalize, trunc_type, scale, noise,
                       noise_type, seed, out);
 }

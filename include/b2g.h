@@ -484,12 +484,16 @@ int b2g_split_n_roots(b2g_split s, int *n);
  * the host from std::mt19937 seeded with `seed`, in block2's order
  * (Random::fill, src/core/utils.hpp:226-252), so a reference run whose
  * generator was seeded with the same value (Random::rand_seed(seed))
- * adds the same noise.  One wavefunction.  Density-matrix noise leaves the
+ * adds the same noise.  Density-matrix noise leaves the
  * density matrix symmetric from its lower triangle, the part block2's
- * xsyev reads.                                                           */
+ * xsyev reads.  n_roots / weights as in b2g_split_density_matrix_multi
+ * (density_matrix_with_multi_target, moving_environment.hpp:3582-3608:
+ * each root draws its own tensor from the continuing stream); one root
+ * with weight 1 is density_matrix's case.                                */
 int b2g_split_density_matrix_noise(b2g_ctx ctx, int symmetry,
                                    const b2g_sparse_info *wfn,
-                                   const double *psi, double noise,
+                                   const double *psi, int n_roots,
+                                   const double *weights, double noise,
                                    int noise_type, unsigned seed, int k,
                                    double cutoff, int trace_right,
                                    int normalize, unsigned trunc_type,

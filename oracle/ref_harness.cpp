@@ -318,8 +318,10 @@ template <typename S> struct Dumper : CallbackKernel {
     // matrix, rotation matrix, each root's new wavefunction, error,
     // spectra, settings.
     void dump_multi_split(const shared_ptr<SparseMatrix<S, double>> &a,
-                          const vector<double> &bdata, const string &d) const {
-        const string t = "sM_";
+                          const vector<double> &bdata, const string &d,
+                          const string &tag = "sM", double noise = 0.0,
+                          NoiseTypes nt = NoiseTypes::None) const {
+        const string t = tag + "_";
         vector<shared_ptr<SparseMatrixGroup<S, double>>> wfns;
         for (int r = 0; r < 2; r++) {
             auto g = make_shared<SparseMatrixGroup<S, double>>();
@@ -329,9 +331,11 @@ template <typename S> struct Dumper : CallbackKernel {
             wfns.push_back(g);
         }
         const vector<double> weights = {0.7, 0.3};
+        if (noise != 0.0)
+            Random::rand_seed(4321);
         auto dm = MovingEnvironment<S, double, double>::
-            density_matrix_with_multi_target(S(0), wfns, weights, true, 0.0,
-                                             NoiseTypes::None);
+            density_matrix_with_multi_target(S(0), wfns, weights, true, noise,
+                                             nt);
         InfoPack<S> dp;
         dp.add(dm->info);
         dp.write(d, t + "dm");
@@ -361,6 +365,8 @@ template <typename S> struct Dumper : CallbackKernel {
         write_npy(d + "/" + t + "spectra.npy", spectra);
         write_npy(d + "/" + t + "roots.npy", bdata);
         vector<double> scal = {err, (double)k, cutoff, 0.0, 1.0};
+        if (noise != 0.0) // noise, its type (block2's NoiseTypes), seed
+            scal.insert(scal.end(), {noise, (double)(int)nt, 4321.0});
         write_npy(d + "/" + t + "settings.npy", scal);
         // release in reverse order of allocation on each LIFO stack:
         // data (roots, density matrix, rotation, new roots) and infos
@@ -598,6 +604,10 @@ template <typename S> struct Dumper : CallbackKernel {
             for (double &v : xb)
                 v /= sqrt(nrm);
             dump_multi_split(h->ket, xb, d);
+            dump_multi_split(h->ket, xb, d, "sN", 1E-6,
+                             NoiseTypes::Wavefunction);
+            dump_multi_split(h->ket, xb, d, "sQ", 1E-6,
+                             NoiseTypes::DensityMatrix);
         }
         memcpy(h->ket->data, psi.data(), nw * sizeof(double));
         write_npy(d + "/psi_dav.npy", psi_dav);

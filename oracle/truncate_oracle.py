@@ -113,32 +113,41 @@ def block2_uniform(seed, n, a, b):
     return u * (b - a) + a
 
 
-def add_noise(sym, info, blocks, uq, trace_right, noise, noise_type, seed):
+def add_noise(sym, info, blocks, uq, trace_right, noise, noise_type, seed,
+              n_roots=1):
     """OperatorFunctions::trans_product's noise (src/core/operator_functions.
     hpp:776-835) as MovingEnvironment::density_matrix calls it with
-    sqrt(noise) (src/dmrg/moving_environment.hpp:3554-3577).  Returns the
-    density-matrix blocks as block2 stores them (density-matrix noise makes
-    them asymmetric)."""
+    sqrt(noise) (src/dmrg/moving_environment.hpp:3554-3577), once per root
+    for density_matrix_with_multi_target (:3582-3608: each root draws its
+    own tensor from the continuing stream).  Returns the density-matrix
+    blocks as block2 stores them (density-matrix noise makes them
+    asymmetric)."""
     qs = info["quanta"]
     dq = int(info["dq"])
     lab = [q_get_bra(sym, int(q), dq) if trace_right
            else q_neg(sym, q_get_ket(sym, int(q))) for q in qs]
+    out = [b.copy() for b in blocks]
     if noise_type == 1:  # Wavefunction
         n = sum(int(m) * int(k) for m, k in zip(info["nbra"], info["nket"]))
-        t = block2_uniform(seed, n, -0.5, 0.5)
-        ns = noise / float(t @ t)
-        _, _, tb = density_matrix(sym, info, t, trace_right, ns)
-        return [b + x for b, x in zip(blocks, tb)]
+        ts = block2_uniform(seed, n * n_roots, -0.5, 0.5)
+        for r in range(n_roots):
+            t = ts[r * n:(r + 1) * n]
+            _, _, tb = density_matrix(sym, info, t, trace_right,
+                                      noise / float(t @ t))
+            out = [b + x for b, x in zip(out, tb)]
+        return out
     # DensityMatrix: T over the density matrix's layout, once per
     # wavefunction block of each sector
     n = sum(b.size for b in blocks)
-    t = block2_uniform(seed, n, 0.0, 1.0)
-    ns = noise / float(np.sqrt(t @ t))
-    out, o = [], 0
-    for bi, b in enumerate(blocks):
-        cnt = sum(1 for l in lab if uq.index(l) == bi)
-        out.append(b + cnt * ns * t[o:o + b.size].reshape(b.shape))
-        o += b.size
+    ts = block2_uniform(seed, n * n_roots, 0.0, 1.0)
+    for r in range(n_roots):
+        t = ts[r * n:(r + 1) * n]
+        ns = noise / float(np.sqrt(t @ t))
+        o = 0
+        for bi, b in enumerate(blocks):
+            cnt = sum(1 for l in lab if uq.index(l) == bi)
+            out[bi] = out[bi] + cnt * ns * t[o:o + b.size].reshape(b.shape)
+            o += b.size
     return out
 
 
@@ -273,7 +282,7 @@ def split_density_matrix(sym, info, psi, k, cutoff, trace_right,
 
 def multi_split_density_matrix(sym, info, psis, weights, k, cutoff,
                                trace_right, normalize=True, trunc_type=0,
-                               scale=1.0):
+                               scale=1.0, noise=0.0, noise_type=0, seed=0):
     """State-averaged roots of one structure (density_matrix_with_multi_
     target + multi_split_density_matrix, src/dmrg/moving_environment.hpp:
     3582-3608, 4347-4462): density matrix sum_r w_r scale psi_r psi_r^T,
@@ -290,8 +299,11 @@ def multi_split_density_matrix(sym, info, psis, weights, k, cutoff,
             for a, b in zip(base[2], blocks):
                 a += b
     uq, dims, blocks = base
-    eig = [signed_eigh(b) if b.size else (np.zeros(0), np.zeros((0, 0)))
-           for b in blocks]
+    if noise:
+        blocks = add_noise(sym, info, blocks, uq, trace_right, noise,
+                           noise_type, seed, len(psis))
+    eig = [signed_eigh(lower_symmetric(b)) if b.size
+           else (np.zeros(0), np.zeros((0, 0))) for b in blocks]
     w = [e[0] for e in eig]
     ss, error = truncate(sym, uq, w, k, cutoff, trunc_type)
     spectra = []

@@ -287,51 +287,58 @@ def root_parts(s, r, tr):
 
 
 @pytest.mark.parametrize("name", FIXTURES + DELAYED_FIXTURES)
-def test_multi_split_matches_reference(gpu_ctx, name):
+@pytest.mark.parametrize("tag", ["sM", "sN", "sQ"])
+def test_multi_split_matches_reference(gpu_ctx, name, tag):
     """State-averaged split of two roots (the Davidson vector and a
     normalised random vector, weights 0.7 / 0.3) against the reference's
     density_matrix_with_multi_target + multi_split_density_matrix: the
     same checks as the single-root split, with the factor checks for each
-    root."""
+    root; without noise (sM) and with block2's wavefunction (sN) and
+    density-matrix (sQ) noise 1e-6, one draw per root from its generator
+    seeded with 4321."""
     z, sym, info = load(name)
-    if "sM_settings" not in z.files:
+    if tag + "_settings" not in z.files:
         pytest.skip("fixture without multi-root dump")
-    err_ref, k, cutoff, tt, tr = z["sM_settings"]
-    print("case", name, "sM", flush=True)
-    roots = [z["psi_dav"], z["sM_roots"]]
+    err_ref, k, cutoff, tt, tr, noise, nt, seed = settings(z, tag)
+    print("case", name, tag, flush=True)
+    roots = [z["psi_dav"], z[tag + "_roots"]]
     s = TR.split_density_matrix(gpu_ctx, sym, info, dev(np.stack(roots)),
                                 int(k), cutoff, bool(tr), True, int(tt),
-                                weights=[0.7, 0.3])
+                                weights=[0.7, 0.3], noise=noise,
+                                noise_type=int(nt), seed=int(seed))
     dmi, dmd = s.dm
-    assert np.array_equal(dmi.quanta, z["sM_dm_quanta"].astype(np.uint64))
-    assert np.array_equal(dmi.nbra, z["sM_dm_nbra"])
-    assert np.abs(dmd.cpu().numpy() - z["sM_dm_data"]).max(initial=0) < 1e-13
+    p = tag + "_dm"
+    assert np.array_equal(dmi.quanta, z[p + "_quanta"].astype(np.uint64))
+    assert np.array_equal(dmi.nbra, z[p + "_nbra"])
+    dref = {int(q): T.lower_symmetric(dmi.block(z[p + "_data"], i))
+            for i, q in enumerate(dmi.quanta)}
+    dd = dmd.cpu().numpy()
+    for i, q in enumerate(dmi.quanta):
+        assert np.abs(dmi.block(dd, i) - dref[int(q)]).max(initial=0) < 1e-13
     assert abs(s.error - err_ref) < 1e-12, (name, s.error, err_ref)
-    sp = z["sM_spectra"]
+    sp = z[tag + "_spectra"]
     assert s.spectra.shape == sp.shape
     assert np.abs(s.spectra ** 2 - sp ** 2).max(initial=0) < 1e-13
-    strict = kept_matches(z, "sM", s, sym, int(tt), bool(tr), cutoff, name)
+    strict = kept_matches(z, tag, s, sym, int(tt), bool(tr), cutoff, name)
     wav = s.right if tr else s.left
     assert wav[1].shape == (2, wav[0].total)
-    dref = {int(q): dmi.block(z["sM_dm_data"], i)
-            for i, q in enumerate(dmi.quanta)}
     for r in range(2):
         check_factors(sym, info, roots[r], root_parts(s, r, bool(tr)), dref,
                       bool(tr), (name, r))
     if not strict:
         return
     for part, (bi, _) in (("left", s.left), ("right", s.right)):
-        p = "sM_" + part
+        p = tag + "_" + part
         assert np.array_equal(bi.quanta, z[p + "_quanta"].astype(np.uint64))
         assert np.array_equal(bi.nbra, z[p + "_nbra"])
         assert np.array_equal(bi.nket, z[p + "_nket"])
-    lb = TR.BlockInfo(z["sM_left_quanta"], z["sM_left_nbra"],
-                      z["sM_left_nket"], z["sM_left_shift"])
-    rb = TR.BlockInfo(z["sM_right_quanta"], z["sM_right_nbra"],
-                      z["sM_right_nket"], z["sM_right_shift"])
-    lref = {int(q): lb.block(z["sM_left_data"], i)
+    lb = TR.BlockInfo(z[tag + "_left_quanta"], z[tag + "_left_nbra"],
+                      z[tag + "_left_nket"], z[tag + "_left_shift"])
+    rb = TR.BlockInfo(z[tag + "_right_quanta"], z[tag + "_right_nbra"],
+                      z[tag + "_right_nket"], z[tag + "_right_shift"])
+    lref = {int(q): lb.block(z[tag + "_left_data"], i)
             for i, q in enumerate(lb.quanta)}
-    rdata = z["sM_right_data"].reshape(2, -1)
+    rdata = z[tag + "_right_data"].reshape(2, -1)
     for r in range(2):
         rref = {int(q): rb.block(rdata[r], i) for i, q in enumerate(rb.quanta)}
         mine = root_parts(s, r, True)

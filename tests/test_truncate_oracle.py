@@ -219,20 +219,25 @@ def wav_blocks(info_part, data):
 
 
 @pytest.mark.parametrize("name", ALL)
-def test_oracle_multi_split_matches_reference(name):
+@pytest.mark.parametrize("tag", ["sM", "sN", "sQ"])
+def test_oracle_multi_split_matches_reference(name, tag):
     """State-averaged split of two roots (the Davidson vector and a
     normalised random vector of the same structure, weights 0.7 / 0.3):
     MovingEnvironment::density_matrix_with_multi_target +
     multi_split_density_matrix (moving_environment.hpp:3582-3608,
-    4347-4462), trace_right, Physical, k = 2/3 of the states."""
+    4347-4462), trace_right, Physical, k = 2/3 of the states; without
+    noise (sM), with wavefunction (sN) and density-matrix (sQ) noise 1e-6
+    from block2's generator seeded with 4321 (one tensor per root from the
+    continuing stream)."""
     z, sym, info = load(name)
-    if "sM_settings" not in z.files:
+    if tag + "_settings" not in z.files:
         pytest.skip("fixture without multi-root dump")
-    err_ref, k, cutoff, tt, tr = z["sM_settings"]
-    psis = [z["psi_dav"], z["sM_roots"]]
+    err_ref, k, cutoff, tt, tr, noise, nt, seed = settings(z, tag)
+    psis = [z["psi_dav"], z[tag + "_roots"]]
     res = T.multi_split_density_matrix(sym, info, psis, [0.7, 0.3], int(k),
-                                       cutoff, bool(tr), True, int(tt))
-    dq, dn, dnk, dsh, ddata = ref_part(z, "sM", "dm")
+                                       cutoff, bool(tr), True, int(tt), 1.0,
+                                       noise, int(nt), int(seed))
+    dq, dn, dnk, dsh, ddata = ref_part(z, tag, "dm")
     uq, dims, blocks = res["dm"]
     assert [int(x) for x in dq] == uq and list(dn) == dims
     for i, b in enumerate(blocks):
@@ -241,13 +246,13 @@ def test_oracle_multi_split_matches_reference(name):
         assert np.abs(b - ref).max(initial=0) < 1e-13, (name, i)
     assert abs(res["error"] - err_ref) < 1e-12, (name, res["error"], err_ref)
     sp = np.array(res["spectra"])
-    assert sp.shape == z["sM_spectra"].shape
-    assert np.abs(sp ** 2 - z["sM_spectra"] ** 2).max(initial=0) < 1e-13
+    assert sp.shape == z[tag + "_spectra"].shape
+    assert np.abs(sp ** 2 - z[tag + "_spectra"] ** 2).max(initial=0) < 1e-13
     # rotation structure; every root's factor has the structure of root 0
-    rq, rnb, rnk, _, rdata = ref_part(z, "sM", "left")
+    rq, rnb, rnk, _, rdata = ref_part(z, tag, "left")
     assert [int(x) for x in rq] == list(res["rot"][0])
     assert list(rnb) == list(res["rot"][1]) and list(rnk) == list(res["rot"][2])
-    wq, wnb, wnk, wsh, wdata = ref_part(z, "sM", "right")
+    wq, wnb, wnk, wsh, wdata = ref_part(z, tag, "right")
     per_root = int(wsh[-1]) + int(wnb[-1]) * int(wnk[-1]) if len(wsh) else 0
     assert wdata.size == 2 * per_root
     lref = blocks_of(rq, rnb, rnk, _, rdata)
