@@ -352,16 +352,32 @@ __global__ __launch_bounds__(256) void jacobi_update_kernel(
         int pi, pj, ri, rj;
         rr_pair((int)bl.nb, r, (int)tk.x, pi, pj);
         rr_pair((int)bl.nb, r, (int)tk.y, ri, rj);
-        for (int e = tid; e < kJS * kJS; e += 256) {
-            const int l = e / kJS, c = e % kJS;
-            X[l * kJLD + c] = A[bl.a_off + (uint64_t)pair_index(pi, pj, l) * N +
-                                pair_index(ri, rj, c)];
+        // every global operand into registers first (one memory latency):
+        // element e = tid + 256 i of the region, of Q_R and of Q_P
+        constexpr int EPT = kJS * kJS / 256;
+        const uint64_t qro = bl.q_off + (uint64_t)tk.y * kJS * kJS;
+        const uint64_t qpo = bl.q_off + (uint64_t)tk.x * kJS * kJS;
+        double xa[EPT], qr[EPT], qp[EPT];
+#pragma unroll
+        for (int i = 0; i < EPT; i++) {
+            const int e = tid + 256 * i, l = e / kJS, c = e % kJS;
+            xa[i] = A[bl.a_off + (uint64_t)pair_index(pi, pj, l) * N +
+                      pair_index(ri, rj, c)];
+            qr[i] = fr ? Q[qro + e] : 0.0;
+            qp[i] = fp ? Q[qpo + e] : 0.0;
+        }
+#pragma unroll
+        for (int i = 0; i < EPT; i++) {
+            const int e = tid + 256 * i;
+            X[(e / kJS) * kJLD + (e % kJS)] = xa[i];
         }
         jv4d acc;
         if (fr) {
-            const uint64_t qo = bl.q_off + (uint64_t)tk.y * kJS * kJS;
-            for (int e = tid; e < kJS * kJS; e += 256)
-                Y[(e / kJS) * kJLD + (e % kJS)] = Q[qo + e];
+#pragma unroll
+            for (int i = 0; i < EPT; i++) {
+                const int e = tid + 256 * i;
+                Y[(e / kJS) * kJLD + (e % kJS)] = qr[i];
+            }
             __syncthreads();
             acc = lds_mm<false>(X, Y); // X Q_R
             __syncthreads();
@@ -373,9 +389,11 @@ __global__ __launch_bounds__(256) void jacobi_update_kernel(
             }
         }
         if (fp) {
-            const uint64_t qo = bl.q_off + (uint64_t)tk.x * kJS * kJS;
-            for (int e = tid; e < kJS * kJS; e += 256)
-                Y[(e / kJS) * kJLD + (e % kJS)] = Q[qo + e];
+#pragma unroll
+            for (int i = 0; i < EPT; i++) {
+                const int e = tid + 256 * i;
+                Y[(e / kJS) * kJLD + (e % kJS)] = qp[i];
+            }
             __syncthreads();
             acc = lds_mm<true>(X, Y); // Q_P^T X
         } else {
@@ -400,14 +418,21 @@ __global__ __launch_bounds__(256) void jacobi_update_kernel(
         int pi, pj;
         rr_pair((int)bl.nb, r, (int)tk.x, pi, pj);
         const uint32_t row0 = tk.y * kJS;
-        for (int e = tid; e < kJS * kJS; e += 256) {
-            const int l = e / kJS, c = e % kJS;
-            X[l * kJLD + c] =
-                V[bl.a_off + (uint64_t)(row0 + l) * N + pair_index(pi, pj, c)];
-        }
+        constexpr int EPT = kJS * kJS / 256;
         const uint64_t qo = bl.q_off + (uint64_t)tk.x * kJS * kJS;
-        for (int e = tid; e < kJS * kJS; e += 256)
-            Y[(e / kJS) * kJLD + (e % kJS)] = Q[qo + e];
+        double va[EPT], qp[EPT];
+#pragma unroll
+        for (int i = 0; i < EPT; i++) {
+            const int e = tid + 256 * i, l = e / kJS, c = e % kJS;
+            va[i] = V[bl.a_off + (uint64_t)(row0 + l) * N + pair_index(pi, pj, c)];
+            qp[i] = Q[qo + e];
+        }
+#pragma unroll
+        for (int i = 0; i < EPT; i++) {
+            const int e = tid + 256 * i;
+            X[(e / kJS) * kJLD + (e % kJS)] = va[i];
+            Y[(e / kJS) * kJLD + (e % kJS)] = qp[i];
+        }
         __syncthreads();
         const jv4d acc = lds_mm<false>(X, Y);
 #pragma unroll
