@@ -19,6 +19,7 @@ import re
 import shutil
 import subprocess
 import sys
+import tempfile
 
 import numpy as np
 
@@ -111,8 +112,13 @@ def delayed_fixtures(work):
 
 
 def main():
-    work = os.path.join(REPO, "_work", "fixtures")
-    os.makedirs(work, exist_ok=True)
+    # raw harness dumps go to a scratch directory removed afterwards; only
+    # the packed fixtures are kept
+    with tempfile.TemporaryDirectory(prefix="b2g_fixtures_") as work:
+        return build_all(work)
+
+
+def build_all(work):
     if "--delayed-only" in sys.argv:
         res = delayed_fixtures(work)
         with open(os.path.join(HERE, "energies.json")) as f:
