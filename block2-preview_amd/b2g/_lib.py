@@ -94,6 +94,22 @@ class b2g_problem(C.Structure):
                 ("tp", C.POINTER(b2g_tp_conn))]
 
 
+class b2g_davidson_opts(C.Structure):
+    _fields_ = [("conv_thrd", C.c_double), ("rel_conv_thrd", C.c_double),
+                ("shift", C.c_double), ("max_iter", C.c_int),
+                ("soft_max_iter", C.c_int), ("def_min", C.c_int),
+                ("def_max", C.c_int), ("davidson_type", C.c_int),
+                ("precision", C.c_int), ("refine", C.c_int),
+                ("n_ortho", C.c_int), ("ortho", C.POINTER(C.c_void_p)),
+                ("proj_weights", f64p)]
+
+
+# DavidsonTypes bits taken by b2g_effham_eigs_ex (b2g.h B2G_DAV_*)
+DAV = {"Normal": 0, "GreaterThan": 1, "LessThan": 2, "CloseTo": 4,
+       "DavidsonPrecond": 32, "NoPrecond": 64, "Exact": 256,
+       "ElementProj": 1024}
+
+
 # every symbol declared in include/b2g.h, with (restype, argtypes)
 _vp = C.c_void_p
 
@@ -181,6 +197,10 @@ _SIGS = {
     "b2g_effham_eigs": (C.c_int, [_vp, _vp, C.c_int, C.c_double, C.c_int,
                                   C.c_int, C.c_int, C.c_int, C.c_int, f64p,
                                   C.POINTER(C.c_int), u64p, f64p]),
+    "b2g_davidson_opts_default": (None, [C.POINTER(b2g_davidson_opts)]),
+    "b2g_effham_eigs_ex": (C.c_int, [_vp, _vp, C.c_int,
+                                     C.POINTER(b2g_davidson_opts), f64p,
+                                     C.POINTER(C.c_int), u64p, f64p]),
     "b2g_syev_batched": (C.c_int, [_vp, C.c_int, u32p, u64p, _vp, _vp, _vp,
                                    C.POINTER(C.c_int)]),
     "b2g_split_density_matrix": (C.c_int, [_vp, C.c_int,
@@ -253,4 +273,5 @@ def ptr(a, ctype):
 
 
 ABI_STRUCTS = ["b2g_pair", "b2g_block", "b2g_diag_term", "b2g_sparse_info",
-               "b2g_operator", "b2g_term", "b2g_tp_conn", "b2g_problem"]
+               "b2g_operator", "b2g_term", "b2g_tp_conn", "b2g_problem",
+               "b2g_davidson_opts"]

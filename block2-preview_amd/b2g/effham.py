@@ -9,6 +9,8 @@ the device-memory plumbing: every operation runs in libb2g.so).
 """
 import ctypes as C
 
+import numpy as np
+
 import torch
 
 from . import _lib as L
@@ -193,24 +195,58 @@ class EffectiveHamiltonian:
 
     def eigs(self, ket, iprint=False, conv_thrd=5E-6, rel_conv_thrd=0.0,
              max_iter=5000, soft_max_iter=-1, deflation_min_size=2,
-             deflation_max_size=50, precision="f64", refine=True):
-        """Davidson for the lowest roots (eigs,
-        effective_hamiltonian.hpp:480-568).  ket: (n,) or (k, n) CUDA
-        tensor, overwritten with the Ritz vector(s).  Returns
-        (energy or energies, ndav, nflop, tdav) with nflop in the reference's
-        multiply-add convention."""
-        if rel_conv_thrd != 0.0 or soft_max_iter != -1:
-            raise ValueError("rel_conv_thrd / soft_max_iter are not supported")
+             deflation_max_size=50, davidson_type="Normal", shift=0.0,
+             ortho_bra=None, projection_weights=None, precision="f64",
+             refine=True):
+        """Davidson for the targeted roots (eigs,
+        effective_hamiltonian.hpp:480-568; k roots as the MultiMPS eigs,
+        :1708-1800).  ket: (n,) or (k, n) CUDA tensor, overwritten with the
+        Ritz vector(s).  davidson_type: a DavidsonTypes name or bit mask
+        ("Normal", "GreaterThan", "LessThan", "CloseTo", "DavidsonPrecond",
+        "NoPrecond", "Exact", "ElementProj", or e.g. "CloseTo|NoPrecond").
+        ortho_bra: CUDA tensors of ket length (orthogonal states), with
+        projection_weights optional.  Returns (energy or energies, ndav,
+        nflop, tdav) with nflop in the reference's multiply-add
+        convention."""
         k = 1 if ket.dim() == 1 else ket.shape[0]
         if ket.numel() != k * self.ket_len:
             raise ValueError("ket must hold k x %d values" % self.ket_len)
+        if isinstance(davidson_type, str):
+            t = 0
+            for part in davidson_type.split("|"):
+                if part.strip() not in L.DAV:
+                    raise ValueError("unsupported Davidson type %r" % part)
+                t |= L.DAV[part.strip()]
+            davidson_type = t
+        o = L.b2g_davidson_opts()
+        self.lib.b2g_davidson_opts_default(C.byref(o))
+        o.conv_thrd, o.rel_conv_thrd = float(conv_thrd), float(rel_conv_thrd)
+        o.shift = float(shift)
+        o.max_iter, o.soft_max_iter = int(max_iter), int(soft_max_iter)
+        o.def_min, o.def_max = int(deflation_min_size), int(deflation_max_size)
+        o.davidson_type = int(davidson_type)
+        o.precision, o.refine = L.PREC[precision], 1 if refine else 0
+        keep = []
+        if ortho_bra:
+            ors = [v.contiguous() for v in ortho_bra]
+            for v in ors:
+                if v.numel() != self.ket_len or v.dtype != torch.float64:
+                    raise ValueError("ortho_bra vectors: %d FP64 values" %
+                                     self.ket_len)
+            arr = (C.c_void_p * len(ors))(*[v.data_ptr() for v in ors])
+            keep += [ors, arr]
+            o.n_ortho, o.ortho = len(ors), C.cast(arr, C.POINTER(C.c_void_p))
+            if projection_weights is not None:
+                w = np.ascontiguousarray(projection_weights, np.float64)
+                if w.size != len(ors):
+                    raise ValueError("one projection weight per ortho_bra")
+                keep.append(w)
+                o.proj_weights = L.ptr(w, C.c_double)
         en = (C.c_double * k)()
         nd, nfl, td = C.c_int(), C.c_uint64(), C.c_double()
         torch.cuda.current_stream(self.ctx.device).synchronize()
-        code = self.lib.b2g_effham_eigs(
-            self.h, _dptr(ket), k, float(conv_thrd), int(max_iter),
-            int(deflation_min_size), int(deflation_max_size),
-            L.PREC[precision], 1 if refine else 0, en, C.byref(nd),
+        code = self.lib.b2g_effham_eigs_ex(
+            self.h, _dptr(ket), k, C.byref(o), en, C.byref(nd),
             C.byref(nfl), C.byref(td))
         self.ctx.synchronize()
         L.check(code)

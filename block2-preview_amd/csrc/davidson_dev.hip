@@ -1,4 +1,4 @@
-// Device-resident Davidson iteration (one root), built to be captured once
+// Device-resident Davidson iteration (k roots), built to be captured once
 // into a hipGraph and replayed: every step reads the subspace size and the
 // Ritz data from a DavState in device memory, so one graph serves all
 // iterations and the host only reads a small status header per iteration.
@@ -6,10 +6,14 @@
 // Steps follow IterativeMatrixFunctions::davidson
 // (src/core/iterative_matrix_functions.hpp:866-1173, DavidsonTypes::Normal):
 // sigma of the newest vector; subspace matrix alpha = <b_i|sigma_j>;
-// Ritz pairs; rotation of b and sigma; residual q = sigma_0 - lambda b_0 and
-// the squared-norm test; Olsen preconditioner (:93-108); restart to
-// def_min vectors at def_max; orthogonalisation of q against the subspace
-// and append.  The Ritz problem is solved on the device by cyclic Jacobi in
+// Ritz pairs; rotation of b and sigma; re-check of the roots converged so
+// far (the first that fails becomes the current root ck again); residual
+// q = sigma_ck - lambda_ck b_ck and the squared-norm test against
+// conv + lambda^2 rel^2 (a root converges only once the subspace holds k
+// vectors); Olsen (:93-108), Davidson (:66-72) or no preconditioner;
+// restart to def_min vectors at def_max; orthogonalisation of q against
+// the subspace and append.  A step after which a root converged appends
+// nothing: the host then replays the step alone, without a product.  The Ritz problem is solved on the device by cyclic Jacobi in
 // round-robin order (m/2 disjoint rotations at a time); the orthogonalisation
 // is classical Gram-Schmidt applied twice (as orthogonal as one modified
 // pass, with fixed launches).
@@ -36,9 +40,11 @@ __device__ __forceinline__ double bsum(double v, double *sh) {
     return r;
 }
 
-// S[m-1] = sn
+// S[m-1] = sn, when the previous step appended vector m-1
 __global__ void put_sigma_kernel(DavState *st, double *__restrict__ S,
                                  const double *__restrict__ sn, size_t n) {
+    if (!st->appended)
+        return;
     const int m = st->m;
     double *dst = S + (size_t)(m - 1) * n;
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
@@ -310,16 +316,63 @@ __global__ __launch_bounds__(256) void rotate_mfma_kernel(
     }
 }
 
+// squared residual norms |S_i - ld_i B_i|^2 of the roots i < ck (the ones
+// converged so far), per block
+__global__ __launch_bounds__(256) void recheck_kernel(
+    const DavState *st, const double *__restrict__ B,
+    const double *__restrict__ S, size_t n, double *__restrict__ partial) {
+    __shared__ double sh[256];
+    const int ck = st->ck;
+    for (int i = 0; i < ck; i++) {
+        const double l = st->ld[i];
+        const double *b = B + (size_t)i * n, *sg = S + (size_t)i * n;
+        double s = 0.0;
+        for (size_t r = blockIdx.x * (size_t)256 + threadIdx.x; r < n;
+             r += (size_t)gridDim.x * 256) {
+            const double v = sg[r] - l * b[r];
+            s += v * v;
+        }
+        s = bsum(s, sh);
+        if (threadIdx.x == 0)
+            partial[(size_t)blockIdx.x * 64 + i] = s;
+    }
+}
+
+// the first root i < ck whose residual fails the test becomes ck again
+// (one block; sums in fixed block order)
+__global__ __launch_bounds__(256) void recheck_select_kernel(
+    DavState *st, const double *__restrict__ partial, int nblk) {
+    __shared__ double sh[256];
+    const int ck = st->ck;
+    int nck = ck;
+    for (int i = 0; i < ck; i++) {
+        double s = 0.0;
+        for (int b = threadIdx.x; b < nblk; b += 256)
+            s += partial[(size_t)b * 64 + i];
+        s = bsum(s, sh);
+        const double l = st->ld[i];
+        if (fabs(s) >= st->conv + l * l * st->rel2) {
+            nck = i;
+            break;
+        }
+    }
+    if (threadIdx.x == 0)
+        st->ck = nck;
+}
+
+// q = S_ck - ld_ck B_ck and partial sums of |q|^2
 __global__ __launch_bounds__(256) void residual_kernel(
     const DavState *st, const double *__restrict__ B,
     const double *__restrict__ S, double *__restrict__ q, size_t n,
     double *__restrict__ partial) {
     __shared__ double sh[256];
-    const double l = st->ld[0];
+    const int ck = st->ck;
+    const double l = st->ld[ck];
+    const double *b = B + (size_t)ck * n, *sg = S + (size_t)ck * n;
     double s = 0.0;
     for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < n;
          i += (size_t)gridDim.x * 256) {
-        const double v = S[i] - l * B[i];
+        const double v = sg[i] - l * b[i];
         q[i] = v;
         s += v * v;
     }
@@ -343,13 +396,19 @@ __global__ __launch_bounds__(256) void reduce_kernel(
     }
 }
 
-// Olsen, first pass (ld from the state)
+// Preconditioner, first pass (root ck): Olsen's t = c / (ld - aa),
+// q = q / (ld - aa) where |ld - aa| > 1e-12, with partial sums of c.q and
+// c.t; Davidson's (type bit 32) only divides q; NoPrecond (64) leaves q
 __global__ __launch_bounds__(256) void olsen1_kernel(
-    const DavState *st, double *__restrict__ q, const double *__restrict__ c,
+    const DavState *st, double *__restrict__ q, const double *__restrict__ B,
     const double *__restrict__ aa, double *__restrict__ t, size_t n,
     double *__restrict__ partial) {
     __shared__ double sh[256];
-    const double ld = st->ld[0];
+    const int ck = st->ck, type = st->type;
+    if (type & 64)
+        return;
+    const double ld = st->ld[ck];
+    const double *c = B + (size_t)ck * n;
     double s_cq = 0.0, s_ct = 0.0;
     for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < n;
          i += (size_t)gridDim.x * 256) {
@@ -365,6 +424,8 @@ __global__ __launch_bounds__(256) void olsen1_kernel(
         s_cq += ci * qi;
         s_ct += ci * ti;
     }
+    if (type & 32)
+        return;
     s_cq = bsum(s_cq, sh);
     s_ct = bsum(s_ct, sh);
     if (threadIdx.x == 0) {
@@ -375,24 +436,34 @@ __global__ __launch_bounds__(256) void olsen1_kernel(
 
 __global__ void olsen2_kernel(const DavState *st, double *__restrict__ q,
                               const double *__restrict__ t, size_t n) {
+    if (st->type & (32 | 64))
+        return;
     const double a = -st->scal[1] / st->scal[2];
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
          i += (size_t)gridDim.x * blockDim.x)
         q[i] += a * t[i];
 }
 
-// convergence test and restart (one thread)
+// convergence test of root ck and restart (one thread)
 __global__ void control_kernel(DavState *st) {
     st->iter++;
+    const int ck = st->ck;
     st->qq = st->scal[0];
     st->energy = st->ld[0];
-    if (fabs(st->qq) < st->conv) {
-        st->status = 1;
-        return;
+    st->appended = 0;
+    const double l = st->ld[ck];
+    if (fabs(st->qq) < st->conv + l * l * st->rel2 && st->m >= st->k) {
+        st->ck = ck + 1;
+        if (ck + 1 == st->k)
+            st->status = 1;
+    } else {
+        if (st->m >= st->def_max)
+            st->m = st->def_min;
+        st->appended = 1;
     }
-    if (st->m >= st->def_max)
-        st->m = st->def_min;
-    if (st->iter >= st->max_iter)
+    if (st->status == 0 && st->iter == st->soft_max_iter)
+        st->status = 3;
+    else if (st->status == 0 && st->iter >= st->max_iter)
         st->status = 2;
 }
 
@@ -401,7 +472,7 @@ __global__ __launch_bounds__(256) void gs_dots_kernel(
     const DavState *st, const double *__restrict__ B,
     const double *__restrict__ q, size_t n, double *__restrict__ partial) {
     __shared__ double sh[256];
-    if (st->status != 0)
+    if (!st->appended)
         return;
     const int m = st->m;
     for (int j = 0; j < m; j++) {
@@ -418,7 +489,7 @@ __global__ __launch_bounds__(256) void gs_dots_kernel(
 __global__ __launch_bounds__(256) void gs_reduce_kernel(
     DavState *st, const double *__restrict__ partial, int nblk) {
     __shared__ double sh[256];
-    if (st->status != 0)
+    if (!st->appended)
         return;
     const int m = st->m;
     for (int j = 0; j < m; j++) {
@@ -436,7 +507,7 @@ __global__ __launch_bounds__(256) void gs_update_kernel(
     const DavState *st, const double *__restrict__ B, double *__restrict__ q,
     size_t n) {
     __shared__ double d[64];
-    if (st->status != 0)
+    if (!st->appended)
         return;
     const int m = st->m;
     for (int j = threadIdx.x; j < m; j += 256)
@@ -455,7 +526,7 @@ __global__ __launch_bounds__(256) void norm_kernel(
     const DavState *st, const double *__restrict__ q, size_t n,
     double *__restrict__ partial) {
     __shared__ double sh[256];
-    if (st->status != 0)
+    if (!st->appended)
         return;
     double s = 0.0;
     for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < n;
@@ -470,7 +541,7 @@ __global__ __launch_bounds__(256) void norm_kernel(
 __global__ void append_kernel(const DavState *st, const double *__restrict__ q,
                               double *__restrict__ qn, double *__restrict__ B,
                               size_t n) {
-    if (st->status != 0)
+    if (!st->appended)
         return;
     const double f = 1.0 / sqrt(st->scal[3]);
     double *dst = B + (size_t)st->m * n;
@@ -482,15 +553,15 @@ __global__ void append_kernel(const DavState *st, const double *__restrict__ q,
     }
 }
 
-// the last kernel of a step: grow the subspace, then publish the 64-byte
+// the last kernel of a step: grow the subspace, then publish the 128-byte
 // state header to the host (pinned, device-mapped memory)
 __global__ void bump_kernel(DavState *st, DavState *host) {
-    if (st->status == 0)
+    if (st->appended)
         st->m++;
     if (host != nullptr) {
         const int *src = (const int *)st;
         volatile int *dst = (volatile int *)host;
-        for (int k = 0; k < 16; k++)
+        for (int k = 0; k < 32; k++)
             dst[k] = src[k];
         __threadfence_system();
     }
@@ -532,6 +603,10 @@ hipError_t launch_davidson_step(DavState *st, double *B, double *S,
     hipLaunchKernelGGL(ritz_kernel, dim3(1), dim3(256), 0, s, st);
     B2G_MT(rotate_mfma_kernel, vgrid(n), dim3(256), 0, s, st, S, B, n);
 #undef B2G_MT
+    hipLaunchKernelGGL(recheck_kernel, dim3(kGrid), dim3(256), 0, s, st, B, S,
+                       n, partial);
+    hipLaunchKernelGGL(recheck_select_kernel, dim3(1), dim3(256), 0, s, st,
+                       partial, kGrid);
     hipLaunchKernelGGL(residual_kernel, dim3(kGrid), dim3(256), 0, s, st, B,
                        S, q, n, partial);
     hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(256), 0, s, partial, kGrid,

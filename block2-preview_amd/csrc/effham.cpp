@@ -33,14 +33,16 @@ struct DavCache {
     double *B = nullptr, *S = nullptr, *q = nullptr, *t = nullptr;
     double *qn = nullptr, *sn = nullptr, *partial = nullptr;
     DavState *dst = nullptr, *hdr = nullptr; // hdr: pinned host
-    hipGraphExec_t exec[3] = {};
+    // per precision: [1] product + step, [0] step alone
+    hipGraphExec_t exec[3][2] = {};
     uint64_t gen[3] = {};
     void drop_graphs() {
-        for (auto &e : exec)
-            if (e) {
-                hipGraphExecDestroy(e);
-                e = nullptr;
-            }
+        for (auto &ep : exec)
+            for (auto &e : ep)
+                if (e) {
+                    hipGraphExecDestroy(e);
+                    e = nullptr;
+                }
     }
     void release() {
         drop_graphs();
@@ -607,8 +609,11 @@ namespace b2g {
 
 struct DavWork {
     double *B = nullptr, *S = nullptr, *q = nullptr, *t = nullptr;
+    double *O = nullptr;    // orthogonal states
     double *scal = nullptr; // device scalars (>= 64*64 + 16)
     ~DavWork() {
+        if (O)
+            hipFree(O);
         if (B)
             hipFree(B);
         if (S)
@@ -650,47 +655,133 @@ static int davidson_matvec(b2g_effham h, int precision, const double *a,
     return B2G_OK;
 }
 
-// One Davidson solve (IterativeMatrixFunctions::davidson,
-// src/core/iterative_matrix_functions.hpp:866-1173; DavidsonTypes::Normal,
-// Olsen preconditioner, no orthogonal-state projection).
-static int davidson(b2g_effham h, double *x, int k, double conv_thrd,
-                    int max_iter, int def_min, int def_max, int precision,
-                    std::vector<double> &eigvals, int &ndav, uint64_t &nmv) {
+// Davidson options shared by the host loop and the graph: the arguments of
+// EffectiveHamiltonian::eigs (src/dmrg/effective_hamiltonian.hpp:480-490).
+struct DavOpts {
+    double conv = 5E-6, rel = 0.0, shift = 0.0;
+    int max_iter = 5000, soft_max_iter = -1, def_min = 2, def_max = 50;
+    int type = B2G_DAV_NORMAL;
+    int n_ortho = 0;
+    const double *const *ortho = nullptr; // device vectors
+    const double *proj_weights = nullptr; // host, n_ortho values or nullptr
+};
+
+// Order of the Ritz values a Davidson type targets
+// (iterative_matrix_functions.hpp:1031-1061): ascending for Normal, else
+// closest to / at or below / above the shift first.  Stable.
+static void ritz_order(int type, double shift, const std::vector<double> &ld,
+                       int m, std::vector<int> &idx) {
+    idx.resize(m);
+    std::iota(idx.begin(), idx.end(), 0);
+    auto before = [type, shift, &ld](int i, int j) {
+        const double a = ld[i], b = ld[j];
+        if (type & B2G_DAV_CLOSE_TO)
+            return std::abs(a - shift) < std::abs(b - shift);
+        if (type & B2G_DAV_LESS_THAN) {
+            if ((shift >= a) != (shift >= b))
+                return shift >= a;
+            if (shift >= a)
+                return shift - a < shift - b;
+            return a - shift > b - shift;
+        }
+        if (type & B2G_DAV_GREATER_THAN) {
+            if ((shift > a) != (shift > b))
+                return shift > b;
+            if (shift > a)
+                return shift - a > shift - b;
+            return a - shift < b - shift;
+        }
+        return false;
+    };
+    if (type & (B2G_DAV_CLOSE_TO | B2G_DAV_LESS_THAN | B2G_DAV_GREATER_THAN))
+        std::stable_sort(idx.begin(), idx.end(), before);
+}
+
+// One Davidson solve with the vectors on the device and the m x m Ritz
+// problem on the host (IterativeMatrixFunctions::davidson,
+// src/core/iterative_matrix_functions.hpp:866-1173): the k roots' products
+// of one iteration run as one multi-RHS launch; DavidsonTypes Normal,
+// GreaterThan, LessThan and CloseTo with the Olsen (:93-108), Davidson
+// (:66-72) or no preconditioner; orthogonal states projected out or, with
+// projection weights, added to H as w|v><v|; rel_conv_thrd and
+// soft_max_iter.
+static int davidson(b2g_effham h, double *x, int k, const DavOpts &o,
+                    int precision, std::vector<double> &eigvals, int &ndav,
+                    uint64_t &nmv, bool &soft_stop) {
     b2g_ctx ctx = h->ctx;
     hipStream_t st = ctx->stream;
     const size_t n = h->prob.ket.total();
     B2G_REQUIRE(h->prob.bra.total() == n, B2G_ERR_ARG,
                 "Davidson needs bra and ket of equal size");
     B2G_REQUIRE(k >= 1 && k <= 32, B2G_ERR_ARG, "bad number of roots");
+    int def_min = o.def_min, def_max = o.def_max;
     if (def_min < k)
         def_min = k;
     if (def_max < k + k / 2)
         def_max = k + k / 2;
     B2G_REQUIRE(def_max <= 64 && def_min < def_max, B2G_ERR_ARG,
                 "deflation sizes must satisfy k <= def_min < def_max <= 64");
+    int nor = o.n_ortho, nwg = 0;
+    B2G_REQUIRE(nor >= 0 && nor <= 64 && (nor == 0 || o.ortho), B2G_ERR_ARG,
+                "bad orthogonal states");
+    for (int j = 0; j < nor; j++)
+        B2G_REQUIRE(o.ortho[j], B2G_ERR_ARG, "NULL orthogonal state");
+    if (nor > 0 && o.proj_weights && !(o.type & B2G_DAV_ELEMENT_PROJ))
+        nwg = nor, nor = 0;
+    const int nv = std::max(nor, nwg);
+    const double rel2 = o.rel * o.rel;
     DavWork w;
     B2G_HIP(hipMalloc(&w.B, (size_t)def_max * n * sizeof(double)));
     B2G_HIP(hipMalloc(&w.S, (size_t)def_max * n * sizeof(double)));
     B2G_HIP(hipMalloc(&w.q, n * sizeof(double)));
     B2G_HIP(hipMalloc(&w.t, n * sizeof(double)));
     B2G_HIP(hipMalloc(&w.scal, (64 * 64 + 64) * sizeof(double)));
+    if (nv)
+        B2G_HIP(hipMalloc(&w.O, (size_t)nv * n * sizeof(double)));
     double *Bv = w.B, *Sv = w.S;
     auto bv = [&](int i) { return Bv + (size_t)i * n; };
     auto sv = [&](int i) { return Sv + (size_t)i * n; };
+    auto ov = [&](int i) { return w.O + (size_t)i * n; };
+    int r;
+    // the orthogonal states, made orthogonal among themselves (:912-919)
+    std::vector<double> orn(nv, 0.0);
+    for (int i = 0; i < nv; i++)
+        B2G_HIP(launch_copy(o.ortho[i], ov(i), n, st));
+    for (int i = 0; i < nor; i++) {
+        for (int j = 0; j < i; j++)
+            if (std::abs(orn[j]) > 1E-14) {
+                double d;
+                if ((r = dev_dot(ctx, ov(j), ov(i), n, d)))
+                    return r;
+                B2G_HIP(launch_axpy(-d / orn[j], ov(j), ov(i), n, st));
+            }
+        if ((r = dev_dot(ctx, ov(i), ov(i), n, orn[i])))
+            return r;
+    }
+    // q -= sum_j <o_j|q> / |o_j|^2 o_j
+    auto project = [&](double *v) -> int {
+        for (int j = 0; j < nor; j++)
+            if (std::abs(orn[j]) > 1E-14) {
+                double d;
+                int rr = dev_dot(ctx, ov(j), v, n, d);
+                if (rr)
+                    return rr;
+                B2G_HIP(launch_axpy(-d / orn[j], ov(j), v, n, st));
+            }
+        return B2G_OK;
+    };
     for (int i = 0; i < k; i++)
         B2G_HIP(launch_copy(x + (size_t)i * n, bv(i), n, st));
     int m = k;
     for (int i = 0; i < k; i++) {
         for (int j = 0; j < i; j++) {
             double d;
-            int r = dev_dot(ctx, bv(j), bv(i), n, d);
-            if (r)
+            if ((r = dev_dot(ctx, bv(j), bv(i), n, d)))
                 return r;
             B2G_HIP(launch_axpy(-d, bv(j), bv(i), n, st));
         }
         double nrm2;
-        int r = dev_dot(ctx, bv(i), bv(i), n, nrm2);
-        if (r)
+        if ((r = dev_dot(ctx, bv(i), bv(i), n, nrm2)))
             return r;
         if (std::abs(nrm2) < 1E-14 && i > 0) {
             m = i;
@@ -700,19 +791,53 @@ static int davidson(b2g_effham h, double *x, int k, double conv_thrd,
                     "Davidson initial guess has zero norm");
         B2G_HIP(launch_scale(1.0 / std::sqrt(nrm2), bv(i), n, st));
     }
+    for (int i = 0; i < m; i++) {
+        if ((r = project(bv(i))))
+            return r;
+        double nrm2;
+        if ((r = dev_dot(ctx, bv(i), bv(i), n, nrm2)))
+            return r;
+        B2G_REQUIRE(std::abs(nrm2) >= 1E-14, B2G_ERR_ARG,
+                    "cannot generate Davidson initial guess " +
+                        std::to_string(i) +
+                        " orthogonal to the given states");
+        B2G_HIP(launch_scale(1.0 / std::sqrt(nrm2), bv(i), n, st));
+    }
     eigvals.assign(k, 0.0);
     std::vector<double> ld, V, G(64 * 64), A;
+    std::vector<int> idx(def_max);
+    std::iota(idx.begin(), idx.end(), 0);
     int ck = 0, msig = 0, xiter = 0;
     double qq = 0.0;
     nmv = 0;
+    soft_stop = false;
     bool converged = false;
-    while (xiter < max_iter) {
+    const bool ordered = (o.type & (B2G_DAV_CLOSE_TO | B2G_DAV_LESS_THAN |
+                                    B2G_DAV_GREATER_THAN)) != 0;
+    while (xiter < o.max_iter &&
+           (o.soft_max_iter == -1 || xiter < o.soft_max_iter)) {
         xiter++;
-        for (int i = msig; i < m; i++, msig++) {
-            int r = davidson_matvec(h, precision, bv(i), sv(i));
-            if (r)
-                return r;
-            nmv++;
+        if (msig < m) {
+            // the products of all new vectors: one multi-RHS launch (one
+            // call per vector through a user hook)
+            if (h->kernel == nullptr) {
+                if ((r = b2g_hpsi(ctx, h->plan, bv(msig), sv(msig), m - msig,
+                                  1.0, 0, precision)))
+                    return r;
+            } else
+                for (int i = msig; i < m; i++)
+                    if ((r = davidson_matvec(h, precision, bv(i), sv(i))))
+                        return r;
+            for (int i = msig; i < m; i++)
+                for (int j = 0; j < nwg; j++) {
+                    double d;
+                    if ((r = dev_dot(ctx, ov(j), bv(i), n, d)))
+                        return r;
+                    B2G_HIP(launch_axpy(d * o.proj_weights[j], ov(j), sv(i),
+                                        n, st));
+                }
+            nmv += m - msig;
+            msig = m;
         }
         // subspace matrix alpha(i, j) = <b_i | sigma_j>, lower triangle
         B2G_HIP(launch_gram(Bv, m, Sv, m, n, n, ctx->red, w.scal, st));
@@ -729,81 +854,115 @@ static int davidson(b2g_effham h, double *x, int k, double conv_thrd,
                                hipMemcpyHostToDevice, st));
         B2G_HIP(launch_rotate(Sv, n, n, m, w.scal, st));
         B2G_HIP(launch_rotate(Bv, n, n, m, w.scal, st));
+        ritz_order(o.type, o.shift, ld, m, idx);
         // re-check roots already converged
         for (int i = 0; i < ck; i++) {
-            B2G_HIP(launch_copy(sv(i), w.q, n, st));
-            B2G_HIP(launch_axpy(-ld[i], bv(i), w.q, n, st));
+            const int ii = idx[i];
+            B2G_HIP(launch_copy(sv(ii), w.q, n, st));
+            B2G_HIP(launch_axpy(-ld[ii], bv(ii), w.q, n, st));
             double d;
-            int r = dev_dot(ctx, w.q, w.q, n, d);
-            if (r)
+            if ((r = dev_dot(ctx, w.q, w.q, n, d)))
                 return r;
-            if (std::abs(d) >= conv_thrd) {
+            if (std::abs(d) >= o.conv + ld[ii] * ld[ii] * rel2) {
                 ck = i;
                 break;
             }
         }
-        B2G_HIP(launch_copy(sv(ck), w.q, n, st));
-        B2G_HIP(launch_axpy(-ld[ck], bv(ck), w.q, n, st));
-        int r = dev_dot(ctx, w.q, w.q, n, qq);
-        if (r)
+        const int ick = idx[ck];
+        B2G_HIP(launch_copy(sv(ick), w.q, n, st));
+        B2G_HIP(launch_axpy(-ld[ick], bv(ick), w.q, n, st));
+        if ((r = project(w.q)))
             return r;
-        B2G_HIP(launch_olsen(w.q, bv(ck), h->diag, ld[ck], w.t, n, ctx->red,
-                             w.scal + 64 * 64, st));
+        if ((r = dev_dot(ctx, w.q, w.q, n, qq)))
+            return r;
+        if (o.type & B2G_DAV_DAVIDSON_PRECOND)
+            B2G_HIP(launch_dav_precond(w.q, h->diag, ld[ick], n, st));
+        else if (!(o.type & B2G_DAV_NO_PRECOND))
+            B2G_HIP(launch_olsen(w.q, bv(ick), h->diag, ld[ick], w.t, n,
+                                 ctx->red, w.scal + 64 * 64, st));
         for (int i = 0; i <= ck; i++)
-            eigvals[i] = ld[i];
-        if (std::abs(qq) < conv_thrd) {
+            eigvals[i] = ld[idx[i]];
+        for (int i = ck + 1; i < k; i++)
+            eigvals[i] = 0.0;
+        if (std::abs(qq) < o.conv + eigvals[ck] * eigvals[ck] * rel2 &&
+            m >= k) {
             ck++;
             if (ck == k) {
                 converged = true;
                 break;
             }
         } else {
-            if (m >= def_max)
+            bool deflate = false;
+            if (m >= def_max) {
                 m = msig = def_min;
+                deflate = ordered;
+            }
+            if (deflate) {
+                // keep the targeted Ritz pairs first (:1105-1133): vector j
+                // of B and S becomes the one of idx[j], through a buffer
+                double *T = nullptr;
+                B2G_HIP(hipMalloc(&T, (size_t)m * n * sizeof(double)));
+                std::unique_ptr<double, int (*)(void *)> tg(
+                    T, [](void *p) { return (int)hipFree(p); });
+                for (double *X : {Bv, Sv}) {
+                    for (int j = 0; j < m; j++)
+                        B2G_HIP(launch_copy(X + (size_t)idx[j] * n,
+                                            T + (size_t)j * n, n, st));
+                    B2G_HIP(launch_copy(T, X, (size_t)m * n, st));
+                }
+            }
             // one Gram-Schmidt pass of q against the subspace
             for (int j = 0; j < m; j++) {
                 B2G_HIP(launch_dot(bv(j), w.q, n, ctx->red, w.scal, st));
                 B2G_HIP(launch_axpy_dev(w.scal, -1.0, bv(j), w.q, n, st));
             }
+            if ((r = project(w.q)))
+                return r;
             double nq;
-            r = dev_dot(ctx, w.q, w.q, n, nq);
-            if (r)
+            if ((r = dev_dot(ctx, w.q, w.q, n, nq)))
                 return r;
             B2G_HIP(launch_scale(1.0 / std::sqrt(nq), w.q, n, st));
             B2G_HIP(launch_copy(w.q, bv(m), n, st));
             m++;
         }
+        if (xiter == o.soft_max_iter) {
+            soft_stop = true;
+            break;
+        }
     }
     for (int i = 0; i < k; i++)
-        B2G_HIP(launch_copy(bv(i), x + (size_t)i * n, n, st));
+        B2G_HIP(launch_copy(bv(idx[i]), x + (size_t)i * n, n, st));
     B2G_HIP(hipStreamSynchronize(st));
     ndav = xiter;
-    if (!converged) {
+    if (!converged && !soft_stop) {
         set_error("Davidson: only " + std::to_string(ck) + " converged");
         return B2G_ERR_NOCONV;
     }
     return B2G_OK;
 }
 
-// Graph-captured Davidson for one root: the first iteration runs eagerly
-// into the capture, every later iteration replays the same hipGraph (the
-// H*psi of the newest vector plus launch_davidson_step), and the host reads
-// a 64-byte status header from pinned memory after each replay.
-static int davidson_graph(b2g_effham h, double *x, double conv_thrd,
-                          int max_iter, int def_min, int def_max,
+// Graph-captured Davidson for k roots of a Normal-type solve: the first
+// m - 1 products run eagerly as one multi-RHS launch, then every iteration
+// replays one of two hipGraphs, the product of the newest vector plus
+// launch_davidson_step, or the step alone after a step that appended no
+// vector (a root converged), and the host reads a 128-byte status header
+// from pinned memory after each replay.
+static int davidson_graph(b2g_effham h, double *x, int k, const DavOpts &o,
                           int precision, std::vector<double> &eigvals,
-                          int &ndav, uint64_t &nmv) {
+                          int &ndav, uint64_t &nmv, bool &soft_stop) {
     b2g_ctx ctx = h->ctx;
     hipStream_t st = ctx->stream;
     const size_t n = h->prob.ket.total();
     B2G_REQUIRE(h->prob.bra.total() == n, B2G_ERR_ARG,
                 "Davidson needs bra and ket of equal size");
-    if (def_min < 1)
-        def_min = 1;
+    B2G_REQUIRE(k >= 1 && k <= 32, B2G_ERR_ARG, "bad number of roots");
+    int def_min = std::max(o.def_min, k), def_max = o.def_max;
+    if (def_max < k + k / 2)
+        def_max = k + k / 2;
     if (def_max < 2)
         def_max = 2;
     B2G_REQUIRE(def_max <= 63 && def_min < def_max, B2G_ERR_ARG,
-                "graph Davidson: need 1 <= def_min < def_max <= 63");
+                "graph Davidson: need k <= def_min < def_max <= 63");
     B2G_REQUIRE(!h->plan->timing, B2G_ERR_STATE,
                 "graph Davidson: disable plan timing first");
     // the subspace holds def_max vectors plus the one appended before the
@@ -833,39 +992,73 @@ static int davidson_graph(b2g_effham h, double *x, double conv_thrd,
         c.nv = nv;
         tm.mark("workspace");
     }
-    // normalised start vector -> B[0] and qn
-    double nrm2 = 0;
-    int r = b2g_vec_dot(ctx, x, x, n, &nrm2);
-    if (r)
+    // the initial vectors, Gram-Schmidt normalised (:927-955); dependent
+    // ones cut the subspace to the independent leading ones
+    auto bv = [&](int i) { return c.B + (size_t)i * n; };
+    int r;
+    for (int i = 0; i < k; i++)
+        B2G_HIP(launch_copy(x + (size_t)i * n, bv(i), n, st));
+    int m = k;
+    for (int i = 0; i < k; i++) {
+        for (int j = 0; j < i; j++) {
+            double d;
+            if ((r = dev_dot(ctx, bv(j), bv(i), n, d)))
+                return r;
+            B2G_HIP(launch_axpy(-d, bv(j), bv(i), n, st));
+        }
+        double nrm2;
+        if ((r = dev_dot(ctx, bv(i), bv(i), n, nrm2)))
+            return r;
+        if (std::abs(nrm2) < 1E-14 && i > 0) {
+            m = i;
+            break;
+        }
+        B2G_REQUIRE(std::abs(nrm2) >= 1E-14, B2G_ERR_ARG,
+                    "Davidson initial guess has zero norm");
+        B2G_HIP(launch_scale(1.0 / std::sqrt(nrm2), bv(i), n, st));
+    }
+    if ((r = plan_reserve(h->plan, std::max(1, m - 1), precision)) != B2G_OK)
         return r;
-    B2G_REQUIRE(nrm2 > 1E-28, B2G_ERR_ARG, "Davidson initial guess has zero norm");
-    B2G_HIP(launch_copy(x, c.B, n, st));
-    B2G_HIP(launch_scale(1.0 / std::sqrt(nrm2), c.B, n, st));
-    B2G_HIP(launch_copy(c.B, c.qn, n, st));
+    nmv = 0;
+    if (m > 1) {
+        if ((r = b2g_hpsi(ctx, h->plan, c.B, c.S, m - 1, 1.0, 0, precision)))
+            return r;
+        nmv += m - 1;
+    }
+    B2G_HIP(launch_copy(bv(m - 1), c.qn, n, st));
     DavState init;
     memset(&init, 0, sizeof init);
-    init.m = 1;
-    init.max_iter = max_iter;
+    init.m = m;
+    init.max_iter = o.max_iter;
+    init.soft_max_iter = o.soft_max_iter;
     init.def_min = def_min;
     init.def_max = def_max;
-    init.conv = conv_thrd;
+    init.k = k;
+    init.appended = 1;
+    init.type = o.type & (B2G_DAV_DAVIDSON_PRECOND | B2G_DAV_NO_PRECOND);
+    init.conv = o.conv;
+    init.rel2 = o.rel * o.rel;
     B2G_HIP(hipMemcpyAsync(c.dst, &init, sizeof init, hipMemcpyHostToDevice,
                            st));
     if ((r = plan_reserve(h->plan, 1, precision)) != B2G_OK)
         return r;
     B2G_HIP(hipStreamSynchronize(st));
     tm.mark("plan reserve");
-    hipGraphExec_t &exec = c.exec[precision];
-    if (exec == nullptr || c.gen[precision] != h->plan->scratch_gen) {
+    for (int with_product = 1; with_product >= 0; with_product--) {
+        hipGraphExec_t &exec = c.exec[precision][with_product];
+        if (exec != nullptr && c.gen[precision] == h->plan->scratch_gen)
+            continue;
         if (exec) {
             hipGraphExecDestroy(exec);
             exec = nullptr;
         }
-        // capture one iteration: H qn -> sn and the device step, which ends
-        // by writing the state header to pinned host memory
+        // capture one iteration: (H qn -> sn and) the device step, which
+        // ends by writing the state header to pinned host memory
         hipGraph_t graph = nullptr;
         B2G_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        int rc = b2g_hpsi(ctx, h->plan, c.qn, c.sn, 1, 1.0, 0, precision);
+        int rc = with_product
+                     ? b2g_hpsi(ctx, h->plan, c.qn, c.sn, 1, 1.0, 0, precision)
+                     : B2G_OK;
         hipError_t e = rc == B2G_OK
                            ? launch_davidson_step(c.dst, c.B, c.S, c.sn, c.q,
                                                   c.t, c.qn, h->diag, n,
@@ -881,59 +1074,181 @@ static int davidson_graph(b2g_effham h, double *x, double conv_thrd,
             return rc;
         B2G_HIP(e2);
         B2G_HIP(e);
-        c.gen[precision] = h->plan->scratch_gen;
-        tm.mark("graph capture");
+        tm.mark(with_product ? "graph capture" : "graph capture (step)");
     }
+    c.gen[precision] = h->plan->scratch_gen;
     DavState *hdr = c.hdr;
-    nmv = 0;
     ndav = 0;
     int status = 0;
+    bool product = true;
     while (status == 0) {
-        B2G_HIP(hipGraphLaunch(exec, st));
+        B2G_HIP(hipGraphLaunch(c.exec[precision][product ? 1 : 0], st));
         B2G_HIP(hipStreamSynchronize(st));
-        nmv++;
+        if (product)
+            nmv++;
         status = hdr->status;
         ndav = hdr->iter;
+        product = hdr->appended != 0;
     }
     tm.mark("iterations");
-    eigvals.assign(1, hdr->energy);
-    B2G_HIP(launch_copy(c.B, x, n, st));
+    // energies and vectors of the k lowest Ritz pairs (roots past the
+    // converged ones report 0 after a soft stop, as the reference)
+    std::vector<double> ld(k);
+    B2G_HIP(hipMemcpyAsync(ld.data(), c.dst->ld, k * sizeof(double),
+                           hipMemcpyDeviceToHost, st));
+    B2G_HIP(launch_copy(c.B, x, (size_t)k * n, st));
     B2G_HIP(hipStreamSynchronize(st));
-    if (status != 1) {
-        set_error("Davidson (graph): not converged after " +
-                  std::to_string(ndav) + " iterations");
-        return B2G_ERR_NOCONV;
+    // roots with a value: up to ck, or below ck when the last step
+    // converged a root (it appended no vector)
+    const int nval = hdr->appended ? hdr->ck + 1 : hdr->ck;
+    eigvals.assign(k, 0.0);
+    for (int i = 0; i < k && i < nval; i++)
+        eigvals[i] = ld[i];
+    soft_stop = status == 3;
+    if (status == 1 || status == 3)
+        return B2G_OK;
+    set_error("Davidson (graph): not converged after " +
+              std::to_string(ndav) + " iterations");
+    return B2G_ERR_NOCONV;
+}
+
+} // namespace b2g
+
+namespace b2g {
+
+// EffectiveHamiltonian::eigs with DavidsonTypes::Exact
+// (IterativeMatrixFunctions::exact_diagonalization,
+// src/core/iterative_matrix_functions.hpp:110-209): the dense matrix from
+// products of unit vectors (multi-RHS launches), the device eigensolver,
+// then the k targeted eigenpairs.  ndav = n, as the reference.
+static int exact_diag(b2g_effham h, double *x, int k, const DavOpts &o,
+                      int precision, std::vector<double> &eigvals, int &ndav,
+                      uint64_t &nmv) {
+    b2g_ctx ctx = h->ctx;
+    hipStream_t st = ctx->stream;
+    const size_t n = h->prob.ket.total();
+    B2G_REQUIRE(h->prob.bra.total() == n, B2G_ERR_ARG,
+                "exact diagonalisation needs bra and ket of equal size");
+    B2G_REQUIRE(n >= 1 && n <= 16352 && (size_t)k <= n, B2G_ERR_ARG,
+                "exact diagonalisation: order above 16352 or below k");
+    B2G_REQUIRE(h->kernel == nullptr, B2G_ERR_STATE,
+                "exact diagonalisation with a matvec hook is not supported");
+    double *E = nullptr, *Hm = nullptr, *w = nullptr;
+    B2G_HIP(hipMalloc(&Hm, n * n * sizeof(double)));
+    std::unique_ptr<double, int (*)(void *)> g1(
+        Hm, [](void *p) { return (int)hipFree(p); });
+    B2G_HIP(hipMalloc(&E, n * n * sizeof(double)));
+    std::unique_ptr<double, int (*)(void *)> g2(
+        E, [](void *p) { return (int)hipFree(p); });
+    B2G_HIP(hipMalloc(&w, n * sizeof(double)));
+    std::unique_ptr<double, int (*)(void *)> g3(
+        w, [](void *p) { return (int)hipFree(p); });
+    // rows of the identity, 64 at a time (staged in E), their products as
+    // rows of Hm (H is symmetric: row i of Hm = (H e_i)^T = column i)
+    const size_t batch = 64;
+    std::vector<double> eye(batch * n);
+    for (size_t i = 0; i < n; i += batch) {
+        const size_t nb = std::min(batch, n - i);
+        std::fill(eye.begin(), eye.end(), 0.0);
+        for (size_t j = 0; j < nb; j++)
+            eye[j * n + i + j] = 1.0;
+        B2G_HIP(hipMemcpyAsync(E, eye.data(), nb * n * sizeof(double),
+                               hipMemcpyHostToDevice, st));
+        int r = b2g_hpsi(ctx, h->plan, E, Hm + i * n, (int)nb, 1.0, 0,
+                         precision);
+        if (r)
+            return r;
+        B2G_HIP(hipStreamSynchronize(st));
     }
+    nmv = n;
+    B2G_HIP(hipStreamSynchronize(st));
+    const uint32_t nn = (uint32_t)n;
+    const uint64_t off = 0;
+    int r = b2g_syev_batched(ctx, 1, &nn, &off, Hm, w, E, nullptr);
+    if (r)
+        return r;
+    std::vector<double> ld(n);
+    B2G_HIP(hipMemcpy(ld.data(), w, n * sizeof(double),
+                      hipMemcpyDeviceToHost));
+    std::vector<int> idx;
+    ritz_order(o.type, o.shift, ld, (int)n, idx);
+    eigvals.assign(k, 0.0);
+    for (int i = 0; i < k; i++) {
+        eigvals[i] = ld[idx[i]];
+        B2G_HIP(launch_copy(E + (size_t)idx[i] * n, x + (size_t)i * n, n,
+                            st));
+    }
+    B2G_HIP(hipStreamSynchronize(st));
+    ndav = (int)n;
     return B2G_OK;
 }
 
 } // namespace b2g
 
-extern "C" int b2g_effham_eigs(b2g_effham h, double *x, int k,
-                               double conv_thrd, int max_iter, int def_min,
-                               int def_max, int precision, int refine,
-                               double *energies, int *ndav, uint64_t *nflop,
-                               double *tdav) {
-    B2G_REQUIRE(h && x && energies, B2G_ERR_ARG, "NULL arg");
+extern "C" void b2g_davidson_opts_default(b2g_davidson_opts *o) {
+    if (!o)
+        return;
+    memset(o, 0, sizeof *o);
+    o->conv_thrd = 5E-6;
+    o->max_iter = 5000;
+    o->soft_max_iter = -1;
+    o->def_min = 2;
+    o->def_max = 50;
+    o->davidson_type = B2G_DAV_NORMAL;
+    o->precision = B2G_PREC_F64;
+    o->refine = 1;
+}
+
+extern "C" int b2g_effham_eigs_ex(b2g_effham h, double *x, int k,
+                                  const b2g_davidson_opts *opts,
+                                  double *energies, int *ndav,
+                                  uint64_t *nflop, double *tdav) {
+    B2G_REQUIRE(h && x && energies && opts, B2G_ERR_ARG, "NULL arg");
     B2G_REQUIRE(h->plan, B2G_ERR_STATE, "effham was built host-only");
+    const int precision = opts->precision;
     B2G_REQUIRE(precision == B2G_PREC_F64 || precision == B2G_PREC_F32 ||
                     precision == B2G_PREC_BF16X3,
                 B2G_ERR_ARG, "unsupported precision");
+    const int known = B2G_DAV_GREATER_THAN | B2G_DAV_LESS_THAN |
+                      B2G_DAV_CLOSE_TO | B2G_DAV_DAVIDSON_PRECOND |
+                      B2G_DAV_NO_PRECOND | B2G_DAV_EXACT |
+                      B2G_DAV_ELEMENT_PROJ;
+    B2G_REQUIRE((opts->davidson_type & ~known) == 0, B2G_ERR_ARG,
+                "unsupported Davidson type (Harmonic, NonHermitian and "
+                "LeftEigen are not built)");
+    B2G_REQUIRE(opts->max_iter >= 1 && opts->soft_max_iter >= -1,
+                B2G_ERR_ARG, "bad iteration limits");
+    DavOpts o;
+    o.conv = opts->conv_thrd;
+    o.rel = opts->rel_conv_thrd;
+    o.shift = opts->shift;
+    o.max_iter = opts->max_iter;
+    o.soft_max_iter = opts->soft_max_iter;
+    o.def_min = opts->def_min;
+    o.def_max = opts->def_max;
+    o.type = opts->davidson_type;
+    o.n_ortho = opts->n_ortho;
+    o.ortho = opts->ortho;
+    o.proj_weights = opts->proj_weights;
     auto t0 = std::chrono::steady_clock::now();
     std::vector<double> ev;
     int nd = 0, nd2 = 0;
     uint64_t nmv = 0, nmv2 = 0;
-    const bool graph = k == 1 && h->dav_mode == 1 && !h->plan->timing &&
-                       h->kernel == nullptr;
-    auto solve = [&](int prec, int &ndv, uint64_t &nm) {
-        return graph ? davidson_graph(h, x, conv_thrd, max_iter, def_min,
-                                      def_max, prec, ev, ndv, nm)
-                     : davidson(h, x, k, conv_thrd, max_iter, def_min,
-                                def_max, prec, ev, ndv, nm);
+    bool soft = false, soft2 = false;
+    const bool exact = (o.type & B2G_DAV_EXACT) != 0;
+    const bool graph = !exact && h->dav_mode == 1 && !h->plan->timing &&
+                       h->kernel == nullptr && o.n_ortho == 0 &&
+                       !(o.type & (B2G_DAV_GREATER_THAN | B2G_DAV_LESS_THAN |
+                                   B2G_DAV_CLOSE_TO));
+    auto solve = [&](int prec, int &ndv, uint64_t &nm, bool &sf) {
+        if (exact)
+            return exact_diag(h, x, k, o, prec, ev, ndv, nm);
+        return graph ? davidson_graph(h, x, k, o, prec, ev, ndv, nm, sf)
+                     : davidson(h, x, k, o, prec, ev, ndv, nm, sf);
     };
-    int r = solve(precision, nd, nmv);
-    if (r == B2G_OK && precision != B2G_PREC_F64 && refine)
-        r = solve(B2G_PREC_F64, nd2, nmv2);
+    int r = solve(precision, nd, nmv, soft);
+    if (r == B2G_OK && precision != B2G_PREC_F64 && opts->refine && !soft)
+        r = solve(B2G_PREC_F64, nd2, nmv2, soft2);
     auto t1 = std::chrono::steady_clock::now();
     for (int i = 0; i < k && i < (int)ev.size(); i++)
         energies[i] = ev[i];
@@ -944,4 +1259,20 @@ extern "C" int b2g_effham_eigs(b2g_effham h, double *x, int k,
     if (tdav)
         *tdav = std::chrono::duration<double>(t1 - t0).count();
     return r;
+}
+
+extern "C" int b2g_effham_eigs(b2g_effham h, double *x, int k,
+                               double conv_thrd, int max_iter, int def_min,
+                               int def_max, int precision, int refine,
+                               double *energies, int *ndav, uint64_t *nflop,
+                               double *tdav) {
+    b2g_davidson_opts o;
+    b2g_davidson_opts_default(&o);
+    o.conv_thrd = conv_thrd;
+    o.max_iter = max_iter;
+    o.def_min = def_min;
+    o.def_max = def_max;
+    o.precision = precision;
+    o.refine = refine;
+    return b2g_effham_eigs_ex(h, x, k, &o, energies, ndav, nflop, tdav);
 }

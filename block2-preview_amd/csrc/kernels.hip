@@ -846,6 +846,26 @@ hipError_t launch_olsen(double *q, const double *c, const double *aa,
     return hipGetLastError();
 }
 
+// Davidson's diagonal preconditioner (IterativeMatrixFunctions::
+// davidson_precondition, src/core/iterative_matrix_functions.hpp:66-72):
+// q = q / (ld - aa) where |ld - aa| > 1e-12
+__global__ __launch_bounds__(256) void dav_precond_kernel(
+    double *__restrict__ q, const double *__restrict__ aa, double ld,
+    size_t n) {
+    for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * 256) {
+        const double d = ld - aa[i];
+        if (fabs(d) > 1E-12)
+            q[i] /= d;
+    }
+}
+hipError_t launch_dav_precond(double *q, const double *aa, double ld,
+                              size_t n, hipStream_t st) {
+    hipLaunchKernelGGL(dav_precond_kernel, grid_for(n), dim3(256), 0, st, q,
+                       aa, ld, n);
+    return hipGetLastError();
+}
+
 int red_grid() { return kRedGrid; }
 
 } // namespace b2g

@@ -2,6 +2,8 @@
 // algebra.  gfx950 (CDNA4) only: 64-lane wavefronts, FP64/FP32 MFMA.
 #pragma once
 
+#include <cstddef>
+
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -119,19 +121,29 @@ struct DiagTermDev {
 // Device-resident state of the graph-captured Davidson (davidson_dev.hip).
 // The first 64 bytes are the header the host reads after each iteration.
 struct DavState {
+    // the first 128 bytes are the header the last kernel of a step copies
+    // to pinned host memory
     int m;        // vectors in the subspace
-    int status;   // 0 running, 1 converged, 2 iteration limit
+    int status;   // 0 running, 1 converged, 2 iteration limit, 3 soft limit
     int iter;     // iterations done
     int max_iter;
     int def_min, def_max;
-    int pad0, pad1;
+    int k;        // roots
+    int ck;       // roots converged so far
+    int appended; // the step appended a vector (the next product is needed)
+    int soft_max_iter; // -1: none
+    int type;     // DavidsonTypes preconditioner bits (32, 64)
+    int pad0[5];
     double conv;   // threshold on the squared residual norm
     double qq;     // squared residual norm of the last iteration
     double energy; // lowest Ritz value of the last iteration
-    double pad2;
+    double rel2;   // rel_conv_thrd^2
+    double pad1[4];
     double scal[8]; // reduction results: |q|^2, Olsen c.q and c.t, |q|^2
-    double ld[64], dots[64];
+    double ld[64], dots[64], rn[64];
     double G[64 * 64], V[64 * 64];
 };
+static_assert(offsetof(DavState, scal) == 128, "DavState header");
+
 
 } // namespace b2g

@@ -72,6 +72,8 @@ hipError_t launch_rotate(double *X, size_t ld, size_t n, int m,
 hipError_t launch_olsen(double *q, const double *c, const double *aa,
                         double ld, double *t, size_t n, double *partial,
                         double *sums, hipStream_t st);
+hipError_t launch_dav_precond(double *q, const double *aa, double ld,
+                              size_t n, hipStream_t st);
 int red_grid();
 
 // One device Davidson step after sn = H qn (davidson_dev.hip): nv is the

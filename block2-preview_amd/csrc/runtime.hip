@@ -32,7 +32,8 @@ extern "C" int b2g_abi_sizes(size_t *out, int n) {
     const size_t sz[] = {sizeof(b2g_pair),        sizeof(b2g_block),
                          sizeof(b2g_diag_term),   sizeof(b2g_sparse_info),
                          sizeof(b2g_operator),    sizeof(b2g_term),
-                         sizeof(b2g_tp_conn),     sizeof(b2g_problem)};
+                         sizeof(b2g_tp_conn),     sizeof(b2g_problem),
+                         sizeof(b2g_davidson_opts)};
     const int k = (int)(sizeof(sz) / sizeof(sz[0]));
     for (int i = 0; i < n && i < k && out; i++)
         out[i] = sz[i];

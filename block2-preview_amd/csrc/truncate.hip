@@ -15,12 +15,12 @@
 //   * density matrix blocks D[b] = sum over wavefunction blocks a of
 //     psi_a psi_a^T (trace_right) or psi_a^T psi_a, as output-stationary
 //     grouped-GEMM tasks (the H*psi engine, FP64 MFMA);
-//   * a batched symmetric eigensolver: block Jacobi over 32-column blocks.
+//   * a batched symmetric eigensolver: block Jacobi over 16-column blocks.
 //     Each round pairs the column blocks of every matrix (round-robin); the
-//     64 x 64 subproblem of a pair is diagonalised by cyclic Jacobi in LDS,
+//     32 x 32 subproblem of a pair is diagonalised by cyclic Jacobi in LDS,
 //     giving an orthogonal Q, and the whole matrix is updated as
 //     A <- J^T A J, V <- V J with J the block-diagonal of the round's Qs:
-//     every 64 x 64 region (pair P rows, pair R columns) becomes
+//     every 32 x 32 region (pair P rows, pair R columns) becomes
 //     Q_P^T A_PR Q_R.  Matrices are padded to a multiple of 32 with zero rows
 //     and columns, which no rotation touches (a zero off-diagonal entry is
 //     never rotated), so positions >= n stay out of the spectrum.  A global
@@ -55,6 +55,12 @@ namespace {
 
 constexpr int kJB = 16;      // column block of the block Jacobi
 constexpr int kJS = 2 * kJB; // order of a pair subproblem
+// largest matrix order of the eigensolver: the padded order (a multiple of
+// 2 kJB) squared stays below 2^28 elements
+constexpr int kMaxSyevOrder = 16352;
+static_assert((uint64_t)kMaxSyevOrder * kMaxSyevOrder < ((uint64_t)1 << 28) &&
+                  kMaxSyevOrder % (2 * kJB) == 0,
+              "kMaxSyevOrder");
 constexpr int kJLD = kJS + 1;
 
 // one matrix of a batch: A and V are N x N (N = 32 nb) from a_off; the
@@ -661,6 +667,13 @@ static int syev_run(b2g_ctx ctx, const std::vector<uint32_t> &n,
         b.nb = std::max<uint32_t>(2, (n[i] + kJB - 1) / kJB);
         b.nb += b.nb & 1;
         const uint64_t N = (uint64_t)b.nb * kJB;
+        // the Rayleigh-quotient product addresses the original block and
+        // the padded eigenvector block with 32-bit byte offsets from each
+        // block's start (2^31 bytes): both must hold fewer than 2^28
+        // elements
+        B2G_REQUIRE(N * N < ((uint64_t)1 << 28), B2G_ERR_ARG,
+                    "eigensolver: matrix order above " +
+                        std::to_string(kMaxSyevOrder));
         b.n = n[i];
         b.a_off = s.a_len;
         b.q_off = s.q_len;
@@ -862,7 +875,8 @@ extern "C" int b2g_syev_batched(b2g_ctx ctx, int n_mats, const uint32_t *n,
     std::vector<uint32_t> nv(n, n + n_mats);
     std::vector<uint64_t> ov(off, off + n_mats);
     for (int i = 0; i < n_mats; i++)
-        B2G_REQUIRE(nv[i] <= 65536, B2G_ERR_ARG, "matrix order above 65536");
+        B2G_REQUIRE(nv[i] <= (uint32_t)kMaxSyevOrder, B2G_ERR_ARG,
+                    "matrix order above " + std::to_string(kMaxSyevOrder));
     std::vector<double> ws;
     const int r = syev_run(ctx, nv, ov, a, v, ws, sweeps);
     if (r != B2G_OK)

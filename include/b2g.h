@@ -69,7 +69,7 @@ typedef struct b2g_effham_s *b2g_effham;
 
 /* sizeof of the ABI structs, in this order: b2g_pair, b2g_block,
  * b2g_diag_term, b2g_sparse_info, b2g_operator, b2g_term, b2g_tp_conn,
- * b2g_problem.  Bindings check their layouts with it.  Returns the count. */
+ * b2g_problem, b2g_davidson_opts.  Bindings check their layouts with it.  Returns the count. */
 int b2g_abi_sizes(size_t *out, int n);
 /* Last error message of the calling thread ("" if none). */
 const char *b2g_last_error(void);
@@ -415,11 +415,60 @@ int b2g_effham_eigs(b2g_effham h, double *x, int k, double conv_thrd,
                     int max_iter, int def_min, int def_max, int precision,
                     int refine, double *energies, int *ndav,
                     uint64_t *nflop, double *tdav);
+/* DavidsonTypes (src/core/matrix_functions.hpp:281-301) that
+ * b2g_effham_eigs_ex takes: the target order (Normal: lowest; GreaterThan,
+ * LessThan, CloseTo a shift), the preconditioner (Olsen by default,
+ * DavidsonPrecond, NoPrecond), Exact (dense diagonalisation) and
+ * ElementProj.  Harmonic, NonHermitian and LeftEigen are refused.         */
+#define B2G_DAV_NORMAL 0
+#define B2G_DAV_GREATER_THAN 1
+#define B2G_DAV_LESS_THAN 2
+#define B2G_DAV_CLOSE_TO 4
+#define B2G_DAV_DAVIDSON_PRECOND 32
+#define B2G_DAV_NO_PRECOND 64
+#define B2G_DAV_EXACT 256
+#define B2G_DAV_ELEMENT_PROJ 1024
+/* The arguments of EffectiveHamiltonian::eigs
+ * (src/dmrg/effective_hamiltonian.hpp:480-490) beyond the vectors:
+ * ortho holds n_ortho device vectors of ket length (ortho_bra: projected out
+ * of the subspace and the residual, or, with proj_weights (host, n_ortho
+ * values) and without B2G_DAV_ELEMENT_PROJ, added to H as w|v><v|;
+ * iterative_matrix_functions.hpp:891-954, 974-976, 1071-1078).
+ * Convergence: |r|^2 < conv_thrd + |lambda|^2 rel_conv_thrd^2.
+ * soft_max_iter >= 0 stops after that many iterations without an error
+ * (roots not converged report energy 0, as the reference).               */
+typedef struct b2g_davidson_opts {
+    double conv_thrd, rel_conv_thrd, shift;
+    int max_iter, soft_max_iter;
+    int def_min, def_max;
+    int davidson_type;
+    int precision, refine;
+    int n_ortho;
+    const double *const *ortho;
+    const double *proj_weights;
+} b2g_davidson_opts;
+/* the reference's defaults: conv 5e-6, rel 0, max_iter 5000, no soft limit,
+ * deflation 2 / 50, Normal, FP64                                           */
+void b2g_davidson_opts_default(b2g_davidson_opts *o);
+/* Davidson with the full option set (EffectiveHamiltonian::eigs /
+ * EffectiveHamiltonian<S, FL, MultiMPS>::eigs, effective_hamiltonian.hpp:
+ * 480-568, 1708-1800; IterativeMatrixFunctions::davidson,
+ * iterative_matrix_functions.hpp:866-1173, exact_diagonalization :110-209):
+ * k roots from the k device vectors `x`, overwritten with the targeted
+ * Ritz vectors; energies (host, k).  Normal-type solves without orthogonal
+ * states and hook run graph-captured on the device (b2g_effham_set_
+ * davidson_mode 1, the default), everything else on the host-driven
+ * loop.  ndav counts iterations, nflop the reference's multiply-adds of
+ * all products.                                                            */
+int b2g_effham_eigs_ex(b2g_effham h, double *x, int k,
+                       const b2g_davidson_opts *opts, double *energies,
+                       int *ndav, uint64_t *nflop, double *tdav);
 
 /* ------------------------------------------------------------ truncation */
 /* Batched symmetric eigensolver (GMatrixFunctions::eigs,
  * src/core/matrix_functions.hpp:1672-1688: xsyev on each block).  At most
- * 65535 matrices; matrix i of order n[i] (at most 65536) is read row-major
+ * 65535 matrices; matrix i of order n[i] (at most 16352, so that padded
+ * blocks stay below 2^28 elements) is read row-major
  * from a + off[i] (device,
  * not modified).  Its eigenvalues go to w (device, compact: matrix i from
  * sum_{j<i} n[j]) in ascending order, its eigenvectors to v + off[i] as

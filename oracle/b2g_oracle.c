@@ -789,12 +789,74 @@ static void jacobi(int m, double *a, double *w, double *v) {
     free(idx), free(z);
 }
 
+/* Order of the Ritz values the solver targets (iterative_matrix_functions.hpp
+ * :1031-1061): ascending for Normal, else closest to / below / above shift. */
+static int ord_before(int type, double shift, double a, double b) {
+    if (type & 4) /* CloseTo */
+        return fabs(a - shift) < fabs(b - shift);
+    if (type & 2) { /* LessThan */
+        if ((shift >= a) != (shift >= b))
+            return shift >= a;
+        if (shift >= a)
+            return shift - a < shift - b;
+        return a - shift > b - shift;
+    }
+    if (type & 1) { /* GreaterThan */
+        if ((shift > a) != (shift > b))
+            return shift > b;
+        if (shift > a)
+            return shift - a > shift - b;
+        return a - shift < b - shift;
+    }
+    return 0;
+}
+static void ord_sort(int type, double shift, const double *ld, int *idx,
+                     int m) {
+    for (int i = 0; i < m; i++)
+        idx[i] = i;
+    if (!(type & 7))
+        return;
+    for (int i = 1; i < m; i++) /* stable insertion sort */
+        for (int j = i; j > 0 && ord_before(type, shift, ld[idx[j]],
+                                            ld[idx[j - 1]]); j--) {
+            int t = idx[j];
+            idx[j] = idx[j - 1], idx[j - 1] = t;
+        }
+}
+
+/* IterativeMatrixFunctions::davidson (iterative_matrix_functions.hpp
+ * :866-1173): real symmetric, DavidsonTypes Normal / GreaterThan / LessThan /
+ * CloseTo with DavidsonPrecond (:66-72) or NoPrecond or the default Olsen
+ * preconditioner (:93-108); orthogonal states ors projected out
+ * (:912-919, 939-954, 1071-1078, 1134-1146), or with proj_weights added to H
+ * as w|v><v| (:891-896, 974-976); convergence on
+ * |q|^2 < conv + |lambda|^2 rel^2 once m >= k (:1062-1070, 1098-1103);
+ * soft_max_iter (:966, 1150-1154).  Returns 0 on convergence, 1 at max_iter,
+ * 2 when soft_max_iter stopped it.  nthreads: threads of the products (their
+ * results do not depend on it). */
 int orc_davidson(const orc_problem *p, double *x, int k, double conv,
-                 int max_iter, int def_min, int def_max, double *energies,
-                 int *ndav) {
+                 double rel_conv, int max_iter, int soft_max_iter,
+                 int def_min, int def_max, int type, double shift, int nor,
+                 const double *ors_in, const double *proj_weights,
+                 double *energies, int *ndav, int nthreads) {
     uint64_t n = orc_ket_len(p);
+    int nwg = 0;
+    if (nor > 0 && proj_weights != NULL && !(type & 1024))
+        nwg = nor, nor = 0;
     if (def_min < k) def_min = k;
     if (def_max < k + k / 2) def_max = k + k / 2;
+    const int nv = nwg > nor ? nwg : nor;
+    double *ors = malloc(sizeof(double) * n * (nv ? nv : 1));
+    if (nv)
+        memcpy(ors, ors_in, sizeof(double) * n * nv);
+    double *orn = malloc(sizeof(double) * (nv ? nv : 1));
+    for (int i = 0; i < nor; i++) {
+        for (int j = 0; j < i; j++)
+            if (fabs(orn[j]) > 1E-14)
+                vaxpy(-vdot(ors + j * n, ors + i * n, n) / orn[j], ors + j * n,
+                      ors + i * n, n);
+        orn[i] = vdot(ors + i * n, ors + i * n, n);
+    }
     double *bs = malloc(sizeof(double) * n * def_max);
     double *ss = malloc(sizeof(double) * n * def_max);
     double *q = malloc(sizeof(double) * n), *t = malloc(sizeof(double) * n);
@@ -814,15 +876,31 @@ int orc_davidson(const orc_problem *p, double *x, int k, double conv,
         for (uint64_t r = 0; r < n; r++)
             bs[i * n + r] /= nx;
     }
+    for (int i = 0; i < m; i++) {
+        for (int j = 0; j < nor; j++)
+            if (fabs(orn[j]) > 1E-14)
+                vaxpy(-vdot(ors + j * n, bs + i * n, n) / orn[j], ors + j * n,
+                      bs + i * n, n);
+        double nx = sqrt(vdot(bs + i * n, bs + i * n, n));
+        for (uint64_t r = 0; r < n; r++)
+            bs[i * n + r] /= nx;
+    }
     double *ld = malloc(sizeof(double) * def_max), *al = malloc(sizeof(double) * def_max * def_max);
     double *ev = malloc(sizeof(double) * def_max * def_max);
+    int *idx = malloc(sizeof(int) * def_max);
     int ck = 0, msig = 0, xiter = 0, rc = 1;
     double qq = 0;
     for (int i = 0; i < k; i++) energies[i] = 0;
-    while (xiter < max_iter) {
+    for (int i = 0; i < def_max; i++)
+        idx[i] = i;
+    while (xiter < max_iter && (soft_max_iter == -1 || xiter < soft_max_iter)) {
         xiter++;
-        for (int i = msig; i < m; i++, msig++)
-            orc_hpsi(p, bs + i * n, ss + i * n, 1);
+        for (int i = msig; i < m; i++, msig++) {
+            orc_hpsi(p, bs + i * n, ss + i * n, nthreads);
+            for (int j = 0; j < nwg; j++)
+                vaxpy(vdot(ors + j * n, bs + i * n, n) * proj_weights[j],
+                      ors + j * n, ss + i * n, n);
+        }
         for (int i = 0; i < m; i++)
             for (int j = 0; j <= i; j++)
                 al[i * m + j] = al[j * m + i] = vdot(bs + i * n, ss + j * n, n);
@@ -839,49 +917,84 @@ int orc_davidson(const orc_problem *p, double *x, int k, double conv,
             for (int i = 0; i < m; i++)
                 vaxpy(ev[j * m + i], tmp + i * n, bs + j * n, n);
         }
+        ord_sort(type, shift, ld, idx, m);
         for (int i = 0; i < ck; i++) {
-            memcpy(q, ss + i * n, sizeof(double) * n);
-            vaxpy(-ld[i], bs + i * n, q, n);
-            if (fabs(vdot(q, q, n)) >= conv) {
+            int ii = idx[i];
+            memcpy(q, ss + ii * n, sizeof(double) * n);
+            vaxpy(-ld[ii], bs + ii * n, q, n);
+            if (fabs(vdot(q, q, n)) >=
+                conv + ld[ii] * ld[ii] * rel_conv * rel_conv) {
                 ck = i;
                 break;
             }
         }
-        memcpy(q, ss + ck * n, sizeof(double) * n);
-        vaxpy(-ld[ck], bs + ck * n, q, n);
+        int ick = idx[ck];
+        memcpy(q, ss + ick * n, sizeof(double) * n);
+        vaxpy(-ld[ick], bs + ick * n, q, n);
+        for (int j = 0; j < nor; j++)
+            if (fabs(orn[j]) > 1E-14)
+                vaxpy(-vdot(ors + j * n, q, n) / orn[j], ors + j * n, q, n);
         qq = vdot(q, q, n);
-        /* Olsen (iterative_matrix_functions.hpp:93-108) */
-        const double *c = bs + ck * n;
-        for (uint64_t r = 0; r < n; r++) {
-            t[r] = c[r];
-            double dd = ld[ck] - aa[r];
-            if (fabs(dd) > 1E-12)
-                t[r] /= dd, q[r] /= dd;
+        const double lk = ld[ick];
+        if (type & 32) { /* DavidsonPrecond (:66-72) */
+            for (uint64_t r = 0; r < n; r++)
+                if (fabs(lk - aa[r]) > 1E-12)
+                    q[r] /= lk - aa[r];
+        } else if (!(type & 64)) { /* Olsen (:93-108) */
+            const double *c = bs + ick * n;
+            for (uint64_t r = 0; r < n; r++) {
+                t[r] = c[r];
+                double dd = lk - aa[r];
+                if (fabs(dd) > 1E-12)
+                    t[r] /= dd, q[r] /= dd;
+            }
+            vaxpy(-vdot(c, q, n) / vdot(c, t, n), t, q, n);
         }
-        vaxpy(-vdot(c, q, n) / vdot(c, t, n), t, q, n);
         for (int i = 0; i <= ck; i++)
-            energies[i] = ld[i];
-        if (fabs(qq) < conv) {
+            energies[i] = ld[idx[i]];
+        for (int i = ck + 1; i < k; i++)
+            energies[i] = 0;
+        if (fabs(qq) < conv + energies[ck] * energies[ck] * rel_conv * rel_conv &&
+            m >= k) {
             ck++;
             if (ck == k) {
                 rc = 0;
                 break;
             }
         } else {
-            if (m >= def_max)
+            int defl = 0;
+            if (m >= def_max) {
                 m = msig = def_min;
+                defl = (type & 7) != 0;
+            }
+            if (defl) { /* keep the targeted Ritz pairs (:1105-1133) */
+                for (int j = 0; j < m; j++)
+                    memcpy(tmp + j * n, bs + idx[j] * n, sizeof(double) * n);
+                memcpy(bs, tmp, sizeof(double) * n * m);
+                for (int j = 0; j < m; j++)
+                    memcpy(tmp + j * n, ss + idx[j] * n, sizeof(double) * n);
+                memcpy(ss, tmp, sizeof(double) * n * m);
+            }
             for (int j = 0; j < m; j++)
                 vaxpy(-vdot(bs + j * n, q, n), bs + j * n, q, n);
+            for (int j = 0; j < nor; j++)
+                if (fabs(orn[j]) > 1E-14)
+                    vaxpy(-vdot(ors + j * n, q, n) / orn[j], ors + j * n, q, n);
             double nq = sqrt(vdot(q, q, n));
             for (uint64_t r = 0; r < n; r++)
                 bs[m * n + r] = q[r] / nq;
             m++;
         }
+        if (xiter == soft_max_iter) {
+            rc = 2;
+            break;
+        }
     }
-    memcpy(x, bs, sizeof(double) * n * k);
+    for (int i = 0; i < k; i++)
+        memcpy(x + i * n, bs + idx[i] * n, sizeof(double) * n);
     *ndav = xiter;
     free(bs), free(ss), free(q), free(t), free(tmp), free(aa), free(ld);
-    free(al), free(ev);
+    free(al), free(ev), free(idx), free(ors), free(orn);
     return rc;
 }
 

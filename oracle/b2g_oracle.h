@@ -19,7 +19,9 @@
  *   diagonal      src/core/operator_functions.hpp:212-247 +
  *                 matrix_functions.hpp:1204-1212
  *   Davidson      src/core/iterative_matrix_functions.hpp:93-108 (Olsen),
- *                 866-1173 (davidson, DavidsonTypes::Normal)
+ *                 66-72 (Davidson), 866-1173 (davidson: Normal,
+ *                 GreaterThan, LessThan, CloseTo, preconditioners, ors,
+ *                 projection weights, rel_conv_thrd, soft_max_iter)
  *   partial expectations (gradients)
  *                 src/core/operator_functions.hpp:354-472
  *
@@ -81,11 +83,17 @@ void orc_hpsi(const orc_problem *p, const double *x, double *y,
 uint64_t orc_hpsi_sample(const orc_problem *p, const double *x, double *y,
                          uint64_t max_pairs, int nthreads);
 void orc_diag(const orc_problem *p, double *d);
-/* k roots; x holds k vectors (overwritten with Ritz vectors).  Returns 0 on
- * convergence, 1 if max_iter was hit. */
+/* k roots; x holds k vectors (overwritten with the targeted Ritz vectors).
+ * type: DavidsonTypes bits (Normal 0, GreaterThan 1, LessThan 2, CloseTo 4,
+ * DavidsonPrecond 32, NoPrecond 64, ElementProj 1024); ors: nor vectors of
+ * ket length (projected out, or added to H with proj_weights when that is not
+ * NULL).  soft_max_iter -1: none.  nthreads: threads of the products.  Returns 0 on convergence, 1 if max_iter
+ * was hit, 2 if soft_max_iter stopped it. */
 int orc_davidson(const orc_problem *p, double *x, int k, double conv_thrd,
-                 int max_iter, int def_min, int def_max, double *energies,
-                 int *ndav);
+                 double rel_conv_thrd, int max_iter, int soft_max_iter,
+                 int def_min, int def_max, int type, double shift, int nor,
+                 const double *ors, const double *proj_weights,
+                 double *energies, int *ndav, int nthreads);
 /* Gradients of E = <v| H |c>:  gl / gr receive dE/d(left / right operator
  * data), laid out like the operator buffers (overwritten). */
 void orc_grad_ops(const orc_problem *p, const double *v, const double *c,

@@ -51,8 +51,10 @@ def lib():
         L.orc_diag.argtypes = [C.c_void_p, _f64]
         L.orc_davidson.restype = C.c_int
         L.orc_davidson.argtypes = [C.c_void_p, _f64, C.c_int, C.c_double,
-                                   C.c_int, C.c_int, C.c_int, _f64,
-                                   C.POINTER(C.c_int)]
+                                   C.c_double, C.c_int, C.c_int, C.c_int,
+                                   C.c_int, C.c_int, C.c_double, C.c_int,
+                                   C.c_void_p, C.c_void_p, _f64,
+                                   C.POINTER(C.c_int), C.c_int]
         L.orc_grad_ops.argtypes = [C.c_void_p, _f64, _f64, _f64, _f64]
         L.orc_sixj.restype = C.c_double
         L.orc_sixj.argtypes = [C.c_int] * 6
@@ -139,13 +141,27 @@ class Oracle:
         lib().orc_diag(self.h, _p(d, _f64))
         return d
 
-    def davidson(self, x, conv=1e-10, max_iter=500, def_min=2, def_max=50):
+    def davidson(self, x, conv=1e-10, max_iter=500, def_min=2, def_max=50,
+                 rel_conv=0.0, soft_max_iter=-1, davidson_type=0, shift=0.0,
+                 ors=None, proj_weights=None, nthreads=1):
+        """Returns (energies, Ritz vectors, ndav, rc); rc 0 converged, 1
+        max_iter, 2 soft_max_iter."""
         x = np.array(x, np.float64, copy=True, order="C")
         k = 1 if x.ndim == 1 else x.shape[0]
         e = np.zeros(k)
         nd = C.c_int()
-        rc = lib().orc_davidson(self.h, _p(x, _f64), k, conv, max_iter,
-                                def_min, def_max, _p(e, _f64), C.byref(nd))
+        nor, op, wp = 0, None, None
+        if ors is not None and len(ors):
+            ors = np.ascontiguousarray(np.atleast_2d(ors), np.float64)
+            nor, op = ors.shape[0], ors.ctypes.data
+            if proj_weights is not None:
+                proj_weights = np.ascontiguousarray(proj_weights, np.float64)
+                assert len(proj_weights) == nor
+                wp = proj_weights.ctypes.data
+        rc = lib().orc_davidson(self.h, _p(x, _f64), k, conv, rel_conv,
+                                max_iter, soft_max_iter, def_min, def_max,
+                                davidson_type, shift, nor, op, wp,
+                                _p(e, _f64), C.byref(nd), int(nthreads))
         return e, x, nd.value, rc
 
     def grad_ops(self, v, c):

@@ -96,7 +96,9 @@ def test_davidson_fp64_matches_reference(heff):
     x = dev(pb.golden["psi"])
     e, nd, nflop, td = h.eigs(x, conv_thrd=1e-10)
     assert abs(e - float(pb.meta["dav_e"])) < 1e-8, (name, e)
-    assert nflop == nd * 0 + nflop and nflop > 0
+    # same start vector and steps as the reference's Davidson
+    assert nd == int(pb.meta["dav_ndav"]), (name, nd)
+    assert nflop == int(pb.meta["dav_nflop_mac"]), (name, nflop)
     # Ritz vector is normalised and an eigenvector
     xv = x.cpu().numpy()
     assert abs(np.linalg.norm(xv) - 1) < 1e-10

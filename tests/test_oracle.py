@@ -74,6 +74,8 @@ def test_davidson_matches_reference(case):
     e, x, nd, rc = orc.davidson(pb.golden["psi"], conv=1e-10)
     assert rc == 0
     assert abs(e[0] - float(pb.meta["dav_e"])) < 1e-8
+    # same start vector, same steps: the reference's iteration count
+    assert nd == int(pb.meta["dav_ndav"]), (name, nd)
     # the Ritz vector is an eigenvector
     hx = orc.hpsi(x)
     assert np.linalg.norm(hx - e[0] * x) < 1e-4
@@ -96,6 +98,110 @@ def test_bruteforce_dense_hamiltonian():
     assert abs(w[0] - float(pb.meta["dav_e"])) < 1e-9
     # H*psi of the fixture vector through the dense matrix
     assert rel(H @ pb.golden["psi"], pb.golden["hpsi"]) < 1e-12
+
+
+@pytest.fixture(scope="module")
+def dense_hub8():
+    """Dense H_eff of the Hubbard L=8 SU(2) site (unit vectors through the
+    oracle) and its full spectrum (numpy eigh)."""
+    pb = load("hub8_su2")
+    orc = O.Oracle(pb)
+    n = orc.ket_len
+    H = np.empty((n, n))
+    e = np.zeros(n)
+    for i in range(n):
+        e[i] = 1.0
+        H[:, i] = orc.hpsi(e)
+        e[i] = 0.0
+    w, v = np.linalg.eigh(H)
+    return pb, orc, H, w, v
+
+
+def _guess(pb, k, seed=3):
+    rng = np.random.default_rng(seed)
+    x = np.array([pb.golden["psi"]] + [rng.standard_normal(len(
+        pb.golden["psi"])) for _ in range(k - 1)])
+    return x
+
+
+def test_davidson_k_roots_against_eigh(dense_hub8):
+    """k = 4 roots: the lowest four eigenvalues of the dense matrix, Ritz
+    vectors orthonormal eigenvectors."""
+    pb, orc, H, w, v = dense_hub8
+    e, x, nd, rc = orc.davidson(_guess(pb, 4), conv=1e-11)
+    assert rc == 0
+    assert np.abs(e - w[:4]).max() < 1e-9, (e, w[:4])
+    assert np.abs(x @ x.T - np.eye(4)).max() < 1e-8
+    for i in range(4):
+        assert np.linalg.norm(H @ x[i] - e[i] * x[i]) < 1e-4
+
+
+def test_davidson_dependent_guesses(dense_hub8):
+    """Two identical guesses: the subspace starts from one vector (m < k)
+    and roots may only converge once m >= k; both roots still come out."""
+    pb, orc, H, w, v = dense_hub8
+    x0 = np.array([pb.golden["psi"], pb.golden["psi"]])
+    e, x, nd, rc = orc.davidson(x0, conv=1e-11)
+    assert rc == 0
+    assert np.abs(e - w[:2]).max() < 1e-9, (e, w[:2])
+
+
+def test_davidson_orthogonal_states(dense_hub8):
+    """ors = the ground state: projected out, the solver finds the first
+    excited state; with projection weight 10 (H + 10|v0><v0|) likewise."""
+    pb, orc, H, w, v = dense_hub8
+    g = v[:, 0]
+    e, x, nd, rc = orc.davidson(pb.golden["psi"], conv=1e-11, ors=[g])
+    assert rc == 0 and abs(e[0] - w[1]) < 1e-9, (e, w[:3])
+    assert abs(float(x @ g)) < 1e-8
+    e, x, nd, rc = orc.davidson(pb.golden["psi"], conv=1e-11, ors=[g],
+                                proj_weights=[10.0])
+    assert rc == 0 and abs(e[0] - w[1]) < 1e-9, (e, w[:3])
+
+
+@pytest.mark.parametrize("dtype,pick", [
+    (4, lambda w, s: w[np.argmin(np.abs(w - s))]),        # CloseTo
+    (2, lambda w, s: w[w <= s].max()),                     # LessThan
+    (1, lambda w, s: w[w > s].min()),                      # GreaterThan
+])
+def test_davidson_shift_types(dense_hub8, dtype, pick):
+    """CloseTo / LessThan / GreaterThan a shift between the third and the
+    fourth eigenvalue converge to the eigenvalue the type selects, from a
+    start vector that mixes the third and fourth eigenvectors with noise.
+    (LessThan ranks Ritz values at or below the shift first, closest first,
+    then those above it, farthest first: from a start with no component
+    below the shift it climbs to the top of the spectrum, as the reference
+    does.)"""
+    pb, orc, H, w, v = dense_hub8
+    shift = 0.3 * w[2] + 0.7 * w[3]
+    x0 = (v[:, 2] + v[:, 3]) / np.sqrt(2) + 0.05 * np.random.default_rng(
+        11).standard_normal(len(w)) / np.sqrt(len(w))
+    e, x, nd, rc = orc.davidson(x0, conv=1e-11,
+                                davidson_type=dtype, shift=shift,
+                                max_iter=2000)
+    assert rc == 0
+    assert abs(e[0] - pick(w, shift)) < 1e-8, (dtype, e, w[:6])
+
+
+@pytest.mark.parametrize("dtype", [32, 64])
+def test_davidson_preconditioners(dense_hub8, dtype):
+    """Davidson's diagonal preconditioner and no preconditioner reach the
+    same ground state as Olsen's."""
+    pb, orc, H, w, v = dense_hub8
+    e, x, nd, rc = orc.davidson(pb.golden["psi"], conv=1e-11,
+                                davidson_type=dtype, max_iter=2000)
+    assert rc == 0 and abs(e[0] - w[0]) < 1e-9
+
+
+def test_davidson_soft_max_iter_and_rel_conv(dense_hub8):
+    pb, orc, H, w, v = dense_hub8
+    e, x, nd, rc = orc.davidson(pb.golden["psi"], conv=1e-14,
+                                soft_max_iter=3)
+    assert rc == 2 and nd == 3
+    # a relative threshold of 1e-2 * |E| on the residual norm stops earlier
+    e0, _, nd0, _ = orc.davidson(pb.golden["psi"], conv=1e-12)
+    e1, _, nd1, rc1 = orc.davidson(pb.golden["psi"], conv=0.0, rel_conv=1e-2)
+    assert rc1 == 0 and nd1 < nd0 and abs(e1[0] - w[0]) < 1e-3
 
 
 def test_planted_mistake_is_caught():
