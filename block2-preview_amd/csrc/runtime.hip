@@ -1167,23 +1167,14 @@ extern "C" int b2g_plan_reserve(b2g_plan p, int n_rhs, int precision) {
     return plan_reserve(p, n_rhs, precision);
 }
 
-extern "C" int b2g_hpsi(b2g_ctx ctx, b2g_plan p, const double *x, double *y,
-                        int n_rhs, double scale, int accumulate,
-                        int precision) {
-    B2G_REQUIRE(ctx && p && x && y, B2G_ERR_ARG, "NULL arg");
-    B2G_REQUIRE(n_rhs >= 1 && n_rhs <= 65535, B2G_ERR_ARG, "bad n_rhs");
-    B2G_REQUIRE(precision == B2G_PREC_F64 || precision == B2G_PREC_F32 ||
-                    precision == B2G_PREC_BF16X3,
-                B2G_ERR_ARG, "unsupported precision");
-    B2G_REQUIRE(precision != B2G_PREC_F32 || p->l32 != nullptr, B2G_ERR_STATE,
-                "plan was uploaded without FP32 shadows (flags bit 0 or 1)");
-    B2G_HIP(hipSetDevice(ctx->device));
-    {
-        const int r = plan_reserve(p, n_rhs, precision);
-        if (r != B2G_OK)
-            return r;
-    }
-    double *P = (double *)p->pbuf;
+namespace b2g {
+
+// One H*psi with the FP64 / FP32 W scratch at `W` and the split-K partials
+// at `P` (for split-BF16 the packed planes of the plan); allocates nothing
+// itself.
+static int hpsi_run(b2g_ctx ctx, b2g_plan p, const double *x, double *y,
+                    int n_rhs, double scale, int accumulate, int precision,
+                    void *W, double *P) {
     const int grid = ctx->num_cu * 8;
     // brackets one launch on stream `st` with timing events when enabled
     auto timed = [&](int kernel, hipStream_t st,
@@ -1301,22 +1292,22 @@ extern "C" int b2g_hpsi(b2g_ctx ctx, b2g_plan p, const double *x, double *y,
             const int nt = (int)ch.t1[c].size();
             if (precision == B2G_PREC_F64)
                 return launch_seg_gemm<double, double, double, double, 1>(
-                    c, x, (const double *)p->rbuf->ptr, (double *)p->w,
-                    (double *)p->w, ch.d_t1[c], nt, ch.d_s1, p->x_len, 0,
+                    c, x, (const double *)p->rbuf->ptr, (double *)W,
+                    (double *)W, ch.d_t1[c], nt, ch.d_s1, p->x_len, 0,
                     p->w_max, 0, 1.0, 0, n_rhs, grid, st, ch.x1[c]);
             return launch_seg_gemm<double, float, float, float, 1>(
-                c, x, p->r32, (float *)p->w, (float *)p->w, ch.d_t1[c], nt,
+                c, x, p->r32, (float *)W, (float *)W, ch.d_t1[c], nt,
                 ch.d_s1, p->x_len, 0, p->w_max, 0, 1.0, 0, n_rhs, grid, st, ch.x1[c]);
         }));
         B2G_HIP(phase(n2, ch.f2, B2G_K_PHASE2, [&](int c, hipStream_t st) {
             const int nt = (int)ch.t2[c].size();
             if (precision == B2G_PREC_F64)
                 return launch_seg_gemm<double, double, double, double, 2>(
-                    c, (const double *)p->lbuf->ptr, (const double *)p->w, y,
+                    c, (const double *)p->lbuf->ptr, (const double *)W, y,
                     P, ch.d_t2[c], nt, ch.d_s2, 0, p->w_max, p->y_len,
                     p->p_max, scale, beta0, n_rhs, grid, st, ch.x2[c]);
             return launch_seg_gemm<float, float, double, float, 2>(
-                c, p->l32, (const float *)p->w, y, P, ch.d_t2[c], nt,
+                c, p->l32, (const float *)W, y, P, ch.d_t2[c], nt,
                 ch.d_s2, 0, p->w_max, p->y_len, p->p_max, scale, beta0, n_rhs,
                 grid, st, ch.x2[c]);
         }));
@@ -1329,6 +1320,28 @@ extern "C" int b2g_hpsi(b2g_ctx ctx, b2g_plan p, const double *x, double *y,
             }));
     }
     return B2G_OK;
+}
+
+} // namespace b2g
+
+extern "C" int b2g_hpsi(b2g_ctx ctx, b2g_plan p, const double *x, double *y,
+                        int n_rhs, double scale, int accumulate,
+                        int precision) {
+    B2G_REQUIRE(ctx && p && x && y, B2G_ERR_ARG, "NULL arg");
+    B2G_REQUIRE(n_rhs >= 1 && n_rhs <= 65535, B2G_ERR_ARG, "bad n_rhs");
+    B2G_REQUIRE(precision == B2G_PREC_F64 || precision == B2G_PREC_F32 ||
+                    precision == B2G_PREC_BF16X3,
+                B2G_ERR_ARG, "unsupported precision");
+    B2G_REQUIRE(precision != B2G_PREC_F32 || p->l32 != nullptr, B2G_ERR_STATE,
+                "plan was uploaded without FP32 shadows (flags bit 0 or 1)");
+    B2G_HIP(hipSetDevice(ctx->device));
+    {
+        const int r = plan_reserve(p, n_rhs, precision);
+        if (r != B2G_OK)
+            return r;
+    }
+    return hpsi_run(ctx, p, x, y, n_rhs, scale, accumulate, precision, p->w,
+                    (double *)p->pbuf);
 }
 
 // ----------------------------------------------------------------- diagonal

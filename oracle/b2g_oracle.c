@@ -710,6 +710,87 @@ uint64_t orc_hpsi_sample(const orc_problem *p, const double *x, double *y,
     return hpsi_n(p, x, y, n, nthreads);
 }
 
+/* rotate_pair with each GEMM's rows spread over threads (every output
+ * element still sums in the same order) */
+static void rotate_pair_rows(const orc_problem *p, const pair_t *r,
+                             const double *x, double *y, double *work,
+                             int nthreads) {
+    const double *A = p->ldata + r->a_off, *B = p->rdata + r->b_off;
+    const double *C = x + r->c_off;
+    double *V = y + r->v_off;
+    const int64_t cm = r->cm, cn = r->cn, vm = r->vm, vn = r->vn;
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+    for (int64_t i = 0; i < cm; i++)
+        for (int64_t j = 0; j < vn; j++) {
+            double s = 0;
+            if (r->conj & 2)
+                for (int64_t k = 0; k < cn; k++)
+                    s += C[i * cn + k] * B[k * vn + j];
+            else
+                for (int64_t k = 0; k < cn; k++)
+                    s += C[i * cn + k] * B[j * cn + k];
+            work[i * vn + j] = s;
+        }
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+    for (int64_t i = 0; i < vm; i++)
+        for (int64_t j = 0; j < vn; j++) {
+            double s = 0;
+            if (r->conj & 1)
+                for (int64_t k = 0; k < cm; k++)
+                    s += A[k * vm + i] * work[k * vn + j];
+            else
+                for (int64_t k = 0; k < cm; k++)
+                    s += A[i * cm + k] * work[k * vn + j];
+            V[i * vn + j] += r->alpha * s;
+        }
+}
+
+uint64_t orc_hpsi_blocks(const orc_problem *p, const double *x, double *y,
+                         int nsel, const int32_t *blocks, int nthreads) {
+    memset(y, 0, orc_bra_len(p) * sizeof(double));
+    if (nthreads <= 0)
+        nthreads = 1;
+    uint64_t macs = 0, maxw = 1;
+    for (int64_t i = 0; i < p->np; i++) {
+        const pair_t *r = &p->pairs[i];
+        if ((uint64_t)r->cm * r->vn > maxw)
+            maxw = (uint64_t)r->cm * r->vn;
+    }
+    double *work = malloc(maxw * sizeof(double));
+    for (int b = 0; b < nsel; b++) {
+        if (blocks[b] < 0 || blocks[b] >= p->bra.n)
+            continue;
+        const uint64_t v_off = p->bra.sh[blocks[b]];
+        for (int64_t i = 0; i < p->np; i++) {
+            const pair_t *r = &p->pairs[i];
+            if (r->v_off != v_off)
+                continue;
+            rotate_pair_rows(p, r, x, y, work, nthreads);
+            macs += (uint64_t)r->cm * r->cn * r->vn +
+                    (uint64_t)r->vm * r->cm * r->vn;
+        }
+    }
+    free(work);
+    return macs;
+}
+
+void orc_block_macs(const orc_problem *p, uint64_t *macs) {
+    memset(macs, 0, sizeof(uint64_t) * p->bra.n);
+    for (int64_t i = 0; i < p->np; i++) {
+        const pair_t *r = &p->pairs[i];
+        int lo = 0, hi = p->bra.n - 1;
+        while (lo < hi) { /* block_shifts ascend */
+            int mid = (lo + hi + 1) / 2;
+            if (p->bra.sh[mid] <= r->v_off)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        macs[lo] += (uint64_t)r->cm * r->cn * r->vn +
+                    (uint64_t)r->vm * r->cm * r->vn;
+    }
+}
+
 /* D[ic](i,j) += alpha A(i,i) B(j,j) (matrix_functions.hpp:1204-1212) */
 void orc_diag(const orc_problem *p, double *d) {
     memset(d, 0, orc_ket_len(p) * sizeof(double));

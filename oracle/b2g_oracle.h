@@ -82,6 +82,14 @@ void orc_hpsi(const orc_problem *p, const double *x, double *y,
  * sample); returns the multiply-adds performed. */
 uint64_t orc_hpsi_sample(const orc_problem *p, const double *x, double *y,
                          uint64_t max_pairs, int nthreads);
+/* y = H x restricted to the output (bra) blocks listed in `blocks` (the
+ * other blocks of y are zero); each pair's GEMM rows are spread over
+ * nthreads, every element summed in record order.  Returns the
+ * multiply-adds performed. */
+uint64_t orc_hpsi_blocks(const orc_problem *p, const double *x, double *y,
+                         int nsel, const int32_t *blocks, int nthreads);
+/* multiply-adds of one H*psi per output (bra) block */
+void orc_block_macs(const orc_problem *p, uint64_t *macs);
 void orc_diag(const orc_problem *p, double *d);
 /* k roots; x holds k vectors (overwritten with the targeted Ritz vectors).
  * type: DavidsonTypes bits (Normal 0, GreaterThan 1, LessThan 2, CloseTo 4,

@@ -49,6 +49,10 @@ def lib():
         L.orc_hpsi_sample.argtypes = [C.c_void_p, _f64, _f64, C.c_uint64,
                                       C.c_int]
         L.orc_diag.argtypes = [C.c_void_p, _f64]
+        L.orc_hpsi_blocks.restype = C.c_uint64
+        L.orc_hpsi_blocks.argtypes = [C.c_void_p, _f64, _f64, C.c_int,
+                                      _i32, C.c_int]
+        L.orc_block_macs.argtypes = [C.c_void_p, _u64]
         L.orc_davidson.restype = C.c_int
         L.orc_davidson.argtypes = [C.c_void_p, _f64, C.c_int, C.c_double,
                                    C.c_double, C.c_int, C.c_int, C.c_int,
@@ -135,6 +139,21 @@ class Oracle:
         macs = lib().orc_hpsi_sample(self.h, _p(x, _f64), _p(y, _f64),
                                      int(max_pairs), nthreads)
         return y, macs
+
+    def hpsi_blocks(self, x, blocks, nthreads=1):
+        """H x on the listed output blocks only (others zero); returns
+        (y, multiply-adds)."""
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.zeros(self.bra_len)
+        b = np.ascontiguousarray(blocks, np.int32)
+        macs = lib().orc_hpsi_blocks(self.h, _p(x, _f64), _p(y, _f64),
+                                     len(b), _p(b, _i32), int(nthreads))
+        return y, macs
+
+    def block_macs(self):
+        m = np.zeros(int(self.pb.bra.n[0]), np.uint64)
+        lib().orc_block_macs(self.h, _p(m, _u64))
+        return m
 
     def diag(self):
         d = np.zeros(self.ket_len)
