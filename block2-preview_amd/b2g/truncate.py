@@ -140,7 +140,9 @@ def split_density_matrix(ctx, symmetry, info, psi, k, cutoff=0.0,
     comes back with one row per root.  noise > 0 adds block2's
     wavefunction or density-matrix noise (for each root), drawn from
     std::mt19937 seeded with `seed` as block2's Random after
-    Random::rand_seed(seed)."""
+    Random::rand_seed(seed).  Each call starts a new generator: with noise
+    the seed must be nonzero (block2 takes 0 as a clock seed) and a sweep
+    passes a new one at each site."""
     lib = ctx.lib
     ci = info.to_c()
     psi = psi.contiguous()

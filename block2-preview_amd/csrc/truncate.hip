@@ -21,7 +21,7 @@
 //     giving an orthogonal Q, and the whole matrix is updated as
 //     A <- J^T A J, V <- V J with J the block-diagonal of the round's Qs:
 //     every 32 x 32 region (pair P rows, pair R columns) becomes
-//     Q_P^T A_PR Q_R.  Matrices are padded to a multiple of 32 with zero rows
+//     Q_P^T A_PR Q_R.  Matrices are padded to a multiple of 16 with zero rows
 //     and columns, which no rotation touches (a zero off-diagonal entry is
 //     never rotated), so positions >= n stay out of the spectrum.  A global
 //     sweep is max(nb) - 1 rounds (smaller matrices cycle through their own
@@ -1415,6 +1415,12 @@ extern "C" int b2g_split_density_matrix_noise(
     int n_roots, const double *weights, double noise, int noise_type,
     unsigned seed, int k, double cutoff, int trace_right, int normalize,
     unsigned trunc_type, double scale, b2g_split *out) {
+    // block2's Random::rand_seed(0) seeds from the clock, so no fixed stream
+    // corresponds to seed 0; a caller that kept one default seed would also
+    // add the same tensor at every site
+    B2G_REQUIRE(noise == 0.0 || seed != 0u, B2G_ERR_ARG,
+                "b2g_split_density_matrix_noise: seed 0 (block2 takes 0 as a "
+                "clock seed); pass a nonzero seed, a new one per site");
     return split_impl(ctx, symmetry, wfn, psi, n_roots, weights, k, cutoff,
                       trace_right, normalize, trunc_type, scale, noise,
                       noise_type, seed, out);

@@ -533,7 +533,10 @@ int b2g_split_n_roots(b2g_split s, int *n);
  * the host from std::mt19937 seeded with `seed`, in block2's order
  * (Random::fill, src/core/utils.hpp:226-252), so a reference run whose
  * generator was seeded with the same value (Random::rand_seed(seed))
- * adds the same noise.  Density-matrix noise leaves the
+ * adds the same noise.  Every call starts a new generator, while block2
+ * draws from one stream that continues across sites: seed 0 (a clock seed
+ * in block2) is refused with B2G_ERR_ARG when noise != 0, and a sweep
+ * passes a new seed at each site.  Density-matrix noise leaves the
  * density matrix symmetric from its lower triangle, the part block2's
  * xsyev reads.  n_roots / weights as in b2g_split_density_matrix_multi
  * (density_matrix_with_multi_target, moving_environment.hpp:3582-3608:

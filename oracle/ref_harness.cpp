@@ -19,10 +19,16 @@
 //     reference's density-matrix truncation of it (dump_split).
 // Output goes to a directory of .npy files plus meta.txt; tests/golden/
 // make_fixtures.py packs them into the committed .npz fixtures.
-// Nothing in the product links or loads this program.
+// Nothing in the product links or loads this program.  Built with
+// -DB2G_SHIM_CHECK (oracle/_ref/b2shim) it also runs the block2 binding of
+// libb2g.so on each dumped site (shim_check).
 
 #include "block2_core.hpp"
 #include "block2_dmrg.hpp"
+#ifdef B2G_SHIM_CHECK
+// the block2 binding of libb2g.so (built as oracle/_ref/b2shim)
+#include "b2g_block2.hpp"
+#endif
 
 #include <cstdio>
 #include <fstream>
@@ -211,6 +217,50 @@ template <typename S> struct OpSet {
         write_npy(d + "/" + p + "op_data.npy", data);
     }
 };
+
+#ifdef B2G_SHIM_CHECK
+// The block2 binding (block2-preview_amd/integration/b2g_block2.hpp) run on
+// the live effective Hamiltonian: its b2g_problem, built into a host-only
+// b2g effective Hamiltonian, must give the reference's own ConnectionInfo
+// (wfn) entry by entry and the reference's multiply-add count.  Writes
+// shim.txt: strings, OpSumProd strings, connections, entries equal, factor
+// max relative difference, b2g MACs, reference MACs.
+template <typename S>
+static void shim_check(const shared_ptr<EffectiveHamiltonian<S, double>> &h,
+                       const string &d, size_t ref_nflop) {
+    b2g_block2::ProblemPack<S> pk(*h);
+    b2g_effham eh = nullptr;
+    b2g_block2::check(
+        b2g_effham_create(nullptr, &pk.pb, B2G_EFFHAM_HOST_ONLY, &eh));
+    uint64_t nc = 0;
+    b2g_block2::check(b2g_effham_connections(eh, 0, &nc, nullptr, nullptr,
+                                             nullptr, nullptr, nullptr));
+    vector<uint32_t> ia(nc), ib(nc), ic(nc);
+    vector<uint64_t> st(nc);
+    vector<double> fa(nc);
+    b2g_block2::check(b2g_effham_connections(eh, 0, &nc, ia.data(), ib.data(),
+                                             ic.data(), st.data(), fa.data()));
+    uint64_t kl = 0, bl = 0, np = 0, macs = 0;
+    b2g_block2::check(b2g_effham_info(eh, &kl, &bl, &np, &macs));
+    b2g_effham_destroy(eh);
+    auto ci = h->wfn_infos[0];
+    size_t equal = 0;
+    double fdiff = 0.0;
+    if ((size_t)ci->nc == nc)
+        for (size_t k = 0; k < nc; k++) {
+            equal += ia[k] == ci->ia[k] && ib[k] == ci->ib[k] &&
+                     ic[k] == ci->ic[k] && st[k] == ci->stride[k];
+            fdiff = max(fdiff, abs(fa[k] - ci->factor[k]) /
+                                   max(abs(ci->factor[k]), 1e-300));
+        }
+    int nsp = 0;
+    for (int t = 0; t < pk.pb.n_terms; t++)
+        nsp += pk.pb.terms[t].kind == 1;
+    ofstream f(d + "/shim.txt");
+    f << setprecision(17) << pk.pb.n_terms << " " << nsp << " " << nc << " "
+      << equal << " " << fdiff << " " << macs << " " << ref_nflop << "\n";
+}
+#endif
 
 template <typename S> struct Dumper : CallbackKernel {
     DMRG<S, double, double> *dmrg = nullptr;
@@ -551,6 +601,9 @@ template <typename S> struct Dumper : CallbackKernel {
         h->tf->opf->seq->cumulative_nflop = 0;
         h->tf->opf->seq->mode = mode;
         write_npy(d + "/hpsi.npy", hpsi);
+#ifdef B2G_SHIM_CHECK
+        shim_check<S>(h, d, nflop);
+#endif
         // a second, random input vector (seeded) and its H*x
         vector<double> x2(nw), hx2(h->bra->total_memory, 0.0);
         {
