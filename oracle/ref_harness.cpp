@@ -70,6 +70,35 @@ static void write_npy(const string &path, const vector<T> &v) {
     write_npy(path, v.data(), v.size());
 }
 
+template <typename T> static vector<T> read_npy(const string &path) {
+    ifstream f(path, ios::binary);
+    if (!f.good())
+        throw runtime_error("cannot open " + path);
+    char magic[6];
+    uint8_t ver[2];
+    f.read(magic, 6);
+    f.read((char *)ver, 2);
+    uint32_t hl = 0;
+    if (ver[0] == 1) {
+        uint16_t h = 0;
+        f.read((char *)&h, 2);
+        hl = h;
+    } else
+        f.read((char *)&hl, 4);
+    string hdr(hl, ' ');
+    f.read(&hdr[0], hl);
+    if (hdr.find(npy_descr<T>()) == string::npos ||
+        hdr.find("'fortran_order': False") == string::npos)
+        throw runtime_error("unexpected dtype or order in " + path);
+    const size_t start = (size_t)f.tellg();
+    f.seekg(0, ios::end);
+    const size_t n = ((size_t)f.tellg() - start) / sizeof(T);
+    vector<T> v(n);
+    f.seekg(start);
+    f.read((char *)v.data(), n * sizeof(T));
+    return v;
+}
+
 // --------------------------------------------------------------- settings
 struct Settings {
     string fcidump, sym = "su2", outdir = "ref_out";
@@ -80,6 +109,8 @@ struct Settings {
     double dav_conv = 1E-8, dump_conv = 1E-10;
     bool delayed = false;
     string noise_type = "dm";
+    string load_dir;   // --load-problem: block2's H*psi on a given problem
+    int reps = 3;
 };
 static Settings G;
 
@@ -767,6 +798,181 @@ template <typename S> static int run() {
     return 0;
 }
 
+// --load-problem DIR: an effective Hamiltonian handed over as arrays (the
+// layout of the dumps above: ket_/bra_/linfo_/rinfo_ infos, lop_/rop_
+// operators, term_ strings, scalars.npy = symmetry, H's delta quantum,
+// hop_left_vacuum; x.npy = n_rhs stacked input vectors) is built into
+// block2's own objects (SparseMatrixInfo, SparseMatrix, OpElement keys,
+// OpSum of OpProduct, DelayedOperatorTensor) and block2's
+// EffectiveHamiltonian (src/dmrg/effective_hamiltonian.hpp:126-224) is
+// constructed on it.  Written back: diag.npy (its diagonal), hx.npy (H x for
+// each input by block2's Tasked replay, the path eigs takes,
+// effective_hamiltonian.hpp:514-521 -> batch_gemm.hpp:1570-1691), and
+// timing.txt (threads, block2's multiply-add count of one product, the
+// duration of each of `reps` timed products of the first vector, the
+// constructor's duration).
+template <typename S>
+static shared_ptr<SparseMatrixInfo<S>>
+load_info(const string &d, const string &p, int i, bool wfn) {
+    static map<string, vector<uint64_t>> c64;
+    static map<string, vector<uint32_t>> c32;
+    auto g64 = [&](const string &k) -> const vector<uint64_t> & {
+        string f = d + "/" + p + "_" + k + ".npy";
+        if (!c64.count(f))
+            c64[f] = read_npy<uint64_t>(f);
+        return c64[f];
+    };
+    auto g32 = [&](const string &k) -> const vector<uint32_t> & {
+        string f = d + "/" + p + "_" + k + ".npy";
+        if (!c32.count(f))
+            c32[f] = read_npy<uint32_t>(f);
+        return c32[f];
+    };
+    const uint32_t n = g32("n")[i], off = g32("off")[i];
+    auto info = make_shared<SparseMatrixInfo<S>>();
+    info->allocate((int)n);
+    info->delta_quantum.data = g64("dq")[i];
+    info->is_fermion = g32("fermion")[i] != 0;
+    info->is_wavefunction = wfn;
+    for (uint32_t j = 0; j < n; j++) {
+        info->quanta[j].data = g64("quanta")[off + j];
+        info->n_states_bra[j] = (ubond_t)g32("nbra")[off + j];
+        info->n_states_ket[j] = (ubond_t)g32("nket")[off + j];
+    }
+    // block2's own offsets must be the ones the arrays were laid out with
+    const uint64_t total = info->template get_total_memory<double>();
+    for (uint32_t j = 0; j < n; j++)
+        if (info->block_shifts[j] != g32("shift")[off + j])
+            throw runtime_error("block shifts differ from block2's in " + p);
+    (void)total;
+    return info;
+}
+
+template <typename S> static int run_load() {
+    const string d = G.load_dir;
+    vector<uint64_t> sc = read_npy<uint64_t>(d + "/scalars.npy");
+    S opdq, hlv;
+    opdq.data = sc[1];
+    hlv.data = sc[2];
+    auto cg = make_shared<CG<S>>();
+    auto opf = make_shared<OperatorFunctions<S, double>>(cg);
+    auto tf = make_shared<TensorFunctions<S, double>>(opf);
+    auto ket_info = load_info<S>(d, "ket", 0, true);
+    auto bra_info = load_info<S>(d, "bra", 0, true);
+    auto side = [&](const string &ip, const string &op, int half,
+                    vector<pair<S, shared_ptr<SparseMatrixInfo<S>>>> &infos,
+                    shared_ptr<OperatorTensor<S, double>> &opt,
+                    vector<shared_ptr<OpExpr<S>>> &keys,
+                    shared_ptr<vector<double>> &data) {
+        const vector<uint64_t> idq = read_npy<uint64_t>(d + "/" + ip + "_dq.npy");
+        for (size_t i = 0; i < idq.size(); i++) {
+            auto info = load_info<S>(d, ip, (int)i, false);
+            infos.push_back(make_pair(info->delta_quantum, info));
+        }
+        const vector<uint64_t> odq = read_npy<uint64_t>(d + "/" + op + "op_dq.npy");
+        const vector<double> ofac = read_npy<double>(d + "/" + op + "op_factor.npy");
+        const vector<int32_t> oinfo = read_npy<int32_t>(d + "/" + op + "op_info.npy");
+        const vector<uint64_t> ooff = read_npy<uint64_t>(d + "/" + op + "op_off.npy");
+        data = make_shared<vector<double>>(read_npy<double>(d + "/" + op + "op_data.npy"));
+        opt = make_shared<OperatorTensor<S, double>>();
+        for (size_t k = 0; k < odq.size(); k++) {
+            S q;
+            q.data = odq[k];
+            // a distinct name per operator: X with a two-index site label
+            auto key = make_shared<OpElement<S, double>>(
+                OpNames::X,
+                SiteIndex((uint16_t)((k & 0xFFF) | (half << 11)),
+                          (uint16_t)(k >> 11)),
+                q, 1.0);
+            auto m = make_shared<SparseMatrix<S, double>>();
+            m->info = infos[oinfo[k]].second;
+            m->total_memory = m->info->template get_total_memory<double>();
+            if (ooff[k] + m->total_memory > data->size())
+                throw runtime_error("operator data outside the buffer");
+            m->data = data->data() + ooff[k];
+            m->factor = ofac[k];
+            shared_ptr<OpExpr<S>> ekey = key;
+            opt->ops[ekey] = m;
+            keys.push_back(ekey);
+        }
+    };
+    vector<pair<S, shared_ptr<SparseMatrixInfo<S>>>> linfos, rinfos;
+    shared_ptr<OperatorTensor<S, double>> lopt, ropt;
+    vector<shared_ptr<OpExpr<S>>> lkeys, rkeys;
+    shared_ptr<vector<double>> ldata, rdata;
+    side("linfo", "l", 0, linfos, lopt, lkeys, ldata);
+    side("rinfo", "r", 1, rinfos, ropt, rkeys, rdata);
+    const vector<uint8_t> tconj = read_npy<uint8_t>(d + "/term_conj.npy");
+    const vector<int32_t> ta = read_npy<int32_t>(d + "/term_a.npy");
+    const vector<int32_t> tb = read_npy<int32_t>(d + "/term_b.npy");
+    const vector<double> tfac = read_npy<double>(d + "/term_factor.npy");
+    vector<shared_ptr<OpProduct<S, double>>> strings;
+    for (size_t t = 0; t < ta.size(); t++)
+        strings.push_back(make_shared<OpProduct<S, double>>(
+            dynamic_pointer_cast<OpElement<S, double>>(lkeys[ta[t]]),
+            dynamic_pointer_cast<OpElement<S, double>>(rkeys[tb[t]]),
+            tfac[t], tconj[t]));
+    auto hop = make_shared<OpElement<S, double>>(OpNames::H, SiteIndex(),
+                                                 opdq, 1.0);
+    auto hop_mat = make_shared<SymbolicColumnVector<S>>(1);
+    (*hop_mat)[0] = hop;
+    auto op = make_shared<DelayedOperatorTensor<S, double>>();
+    op->dops.push_back(hop);
+    op->mat = make_shared<SymbolicColumnVector<S>>(1);
+    op->mat->data[0] = make_shared<OpSum<S, double>>(strings);
+    op->lopt = lopt;
+    op->ropt = ropt;
+    const vector<double> xs = read_npy<double>(d + "/x.npy");
+    auto ket = make_shared<SparseMatrix<S, double>>();
+    ket->allocate(ket_info);
+    auto bra = make_shared<SparseMatrix<S, double>>();
+    bra->allocate(bra_info);
+    const size_t nk = ket->total_memory, nb = bra->total_memory;
+    if (xs.size() % nk != 0)
+        throw runtime_error("x.npy is not a stack of ket vectors");
+    const size_t nrhs = xs.size() / nk;
+    Timer t;
+    t.get_time();
+    auto h = make_shared<EffectiveHamiltonian<S, double>>(
+        linfos, rinfos, op, bra, ket, hop, hop_mat, hlv, tf, true);
+    const double t_ctor = t.get_time();
+    write_npy(d + "/diag.npy", h->diag->data, h->diag->total_memory);
+    h->tf->opf->seq->cumulative_nflop = 0;
+    h->precompute();
+    vector<double> hx(nrhs * nb, 0.0), tt;
+    size_t nflop = 0;
+    for (size_t r = 0; r < nrhs; r++) {
+        h->tf->opf->seq->cumulative_nflop = 0;
+        h->tf->operator()(
+            GMatrix<double>((double *)xs.data() + r * nk, (MKL_INT)nk, 1),
+            GMatrix<double>(hx.data() + r * nb, (MKL_INT)nb, 1), 1.0);
+        nflop = h->tf->opf->seq->cumulative_nflop;
+    }
+    vector<double> y(nb);
+    for (int rep = 0; rep < G.reps; rep++) {
+        memset(y.data(), 0, nb * sizeof(double));
+        t.get_time();
+        h->tf->operator()(
+            GMatrix<double>((double *)xs.data(), (MKL_INT)nk, 1),
+            GMatrix<double>(y.data(), (MKL_INT)nb, 1), 1.0);
+        tt.push_back(t.get_time());
+    }
+    h->post_precompute();
+    write_npy(d + "/hx.npy", hx);
+    ofstream m(d + "/timing.txt");
+    m << setprecision(17) << "threads " << G.threads << "\n"
+      << "nflop_mac " << nflop << "\n"
+      << "t_ctor_s " << t_ctor << "\n"
+      << "t_hpsi_s";
+    for (double x : tt)
+        m << " " << x;
+    m << "\n";
+    h->deallocate();
+    bra->deallocate();
+    ket->deallocate();
+    return 0;
+}
+
 static vector<double> parse_dlist(const string &s) {
     vector<double> r;
     stringstream ss(s);
@@ -808,11 +1014,17 @@ int main(int argc, char **argv) {
             G.noise_type = v;
         else if (k == "--delayed")
             G.delayed = stoi(v) != 0;
+        else if (k == "--load-problem")
+            G.load_dir = v;
+        else if (k == "--reps")
+            G.reps = stoi(v);
         else {
             cerr << "unknown option " << k << endl;
             return 2;
         }
     }
+    if (G.load_dir != "")
+        G.outdir = G.load_dir;
     mkdir(G.outdir.c_str(), 0755);
     frame_<double>() = make_shared<DataFrame<double>>(
         (size_t)1 << 26, (size_t)1 << 33, G.outdir + "/scratch");
@@ -822,7 +1034,9 @@ int main(int argc, char **argv) {
         ThreadingTypes::OperatorBatchedGEMM | ThreadingTypes::Global,
         G.threads, G.threads, 1);
     threading_()->seq_type = SeqTypes::Tasked;
-    int r = G.sym == "sz" ? run<SZ>() : run<SU2>();
+    int r = G.load_dir != ""
+                ? (G.sym == "sz" ? run_load<SZ>() : run_load<SU2>())
+                : (G.sym == "sz" ? run<SZ>() : run<SU2>());
     frame_<double>()->activate(0);
     frame_<double>() = nullptr;
     return r;
