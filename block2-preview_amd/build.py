@@ -16,7 +16,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 SOURCES = ["kernels.hip", "gemm_bf16x3.hip", "pkplan.hip", "davidson_dev.hip",
            "runtime.hip", "grad.hip", "truncate.hip", "effham.cpp",
-           "plan.cpp", "formats.cpp"]
+           "plan.cpp", "formats.cpp", "env.hip", "envplan.cpp"]
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     "-Wno-unused-result", "-Wno-unused-value",

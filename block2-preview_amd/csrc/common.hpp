@@ -155,4 +155,5 @@ struct b2g_buf_s {
     void *ptr = nullptr;
     size_t n = 0;
     int dtype = B2G_F64;
+    bool owned = true; // false: a view of memory it must not free
 };

@@ -502,6 +502,11 @@ template hipError_t launch_seg_gemm<double, double, double, double, 5>(
     int, const double *, const double *, double *, double *, const Task *, int, const Seg *,
     uint64_t, uint64_t, uint64_t, uint64_t, double, int, int, int,
     hipStream_t, XcdSplit);
+// 6: block-product lists of environment moves (two-site wavefunctions)
+template hipError_t launch_seg_gemm<double, double, double, double, 6>(
+    int, const double *, const double *, double *, double *, const Task *, int, const Seg *,
+    uint64_t, uint64_t, uint64_t, uint64_t, double, int, int, int,
+    hipStream_t, XcdSplit);
 template hipError_t launch_seg_gemm<double, float, float, float, 1>(
     int, const double *, const float *, float *, float *, const Task *, int, const Seg *,
     uint64_t, uint64_t, uint64_t, uint64_t, double, int, int, int,

@@ -181,9 +181,24 @@ extern "C" int b2g_buf_destroy(b2g_buf buf) {
         return B2G_OK;
     hipSetDevice(buf->ctx->device);
     hipStreamSynchronize(buf->ctx->stream);
-    if (buf->ptr)
+    if (buf->ptr && buf->owned)
         hipFree(buf->ptr);
     delete buf;
+    return B2G_OK;
+}
+
+extern "C" int b2g_buf_wrap(b2g_ctx ctx, void *ptr, size_t n, int dtype,
+                            b2g_buf *out) {
+    B2G_REQUIRE(ctx != nullptr && out != nullptr && (ptr || n == 0),
+                B2G_ERR_ARG, "NULL arg");
+    B2G_REQUIRE(dtype_size(dtype) != 0, B2G_ERR_ARG, "bad dtype");
+    b2g_buf b = new b2g_buf_s();
+    b->ctx = ctx;
+    b->ptr = ptr;
+    b->n = n;
+    b->dtype = dtype;
+    b->owned = false;
+    *out = b;
     return B2G_OK;
 }
 

@@ -585,6 +585,167 @@ int b2g_smat_save(const char *path, const b2g_sparse_info *info, int flags,
                   size_t len);
 int b2g_smat_destroy(b2g_smat m);
 
+/* ------------------------------- environment blocking and renormalisation */
+/* The steps either side of H*psi in a sweep, on the device: forming the
+ * operators of an enlarged block (left block (x) site, site (x) right
+ * block), renormalising them with the MPS tensor, the normal-to-
+ * complementary transform at the MPO's switch site, and the two-site
+ * wavefunction moves between sites.
+ *
+ * A StateInfo (src/core/state_info.hpp:59-): n quanta in ascending order
+ * with their state counts.                                                */
+typedef struct {
+    int32_t n;
+    int32_t pad;
+    const uint64_t *quanta;
+    const uint32_t *n_states;
+} b2g_state_info;
+
+/* A window-sum plan: records  C_window += scale * op(A) (x) op(B)  grouped
+ * by destination window, each window summed in record order by one
+ * workgroup (no atomics).  It replays GMatrixFunctions::tensor_product
+ * (src/core/matrix_functions.hpp:1269-) for block formation, and the block
+ * copies of OperatorFunctions::iadd (src/core/operator_functions.hpp:
+ * 134-174) and SparseMatrix::swap_to_fused_left / right
+ * (src/core/sparse_matrix.hpp:1823-1968).                                 */
+typedef struct b2g_winplan_s *b2g_winplan;
+
+/* One string of a contraction expression: operator c of the result +=
+ * factor * op(A[a]) (x) op(B[b]), conj bits as block2's (the OpProduct
+ * strings of MPO::left_operator_exprs / right_operator_exprs entries,
+ * divided by the result name's factor as TensorFunctions::left_contract
+ * does).                                                                  */
+typedef struct {
+    int32_t c, a, b;
+    uint32_t conj;
+    double factor;
+} b2g_contract_term;
+
+/* TensorFunctions::left_contract / right_contract
+ * (src/core/tensor_functions.hpp:2842-2986) through tensor_product
+ * (:2184-2288) and OperatorFunctions::tensor_product
+ * (src/core/operator_functions.hpp:672-711): the operators C of the
+ * enlarged block on the fused bases bra = bra_a (x) bra_b, ket = ket_a (x)
+ * ket_b (StateInfo::tensor_product, src/core/state_info.hpp:231-254; for a
+ * left block A is the left environment and B the site, for a right block A
+ * is the site and B the right environment).  Operator sets are given as
+ * for an effective Hamiltonian: infos sorted by delta quantum, data
+ * offsets into the A, B and C buffers.  The tensor-product connection lists of every C info
+ * (ConnectionInfo::initialize_tp, src/core/sparse_matrix.hpp:294-422, for
+ * the sub-labels of Partition::get_uniq_sub_labels,
+ * src/dmrg/partition.hpp:368-480) are built here.                        */
+typedef struct {
+    int32_t symmetry;
+    int32_t n_terms;
+    b2g_state_info bra_a, ket_a, bra_b, ket_b, bra, ket;
+    int32_t n_a_infos, n_b_infos, n_c_infos;
+    int32_t n_a_ops, n_b_ops, n_c_ops;
+    const uint64_t *a_info_dq, *b_info_dq, *c_info_dq;
+    const b2g_sparse_info *a_infos, *b_infos, *c_infos;
+    const b2g_operator *a_ops, *b_ops, *c_ops;
+    const b2g_contract_term *terms;
+} b2g_contract;
+
+/* flags: B2G_EFFHAM_HOST_ONLY builds the host records only (queries work,
+ * b2g_winplan_run refuses).                                               */
+int b2g_contract_plan(b2g_ctx ctx, const b2g_contract *c, unsigned flags,
+                      b2g_winplan *out);
+/* The tensor-product connection list of C info `info` of a contraction
+ * plan, in block2's ConnectionInfo layout (n[5]; classes: quanta, idx;
+ * entries: ia, ib, ic, stride, factor).  Counts in *n_classes / *n_conn;
+ * any array may be NULL.                                                   */
+int b2g_winplan_connections(b2g_winplan w, int info, int32_t *n,
+                            uint64_t *n_classes, uint64_t *quanta,
+                            uint32_t *idx, uint64_t *n_conn, uint32_t *ia,
+                            uint32_t *ib, uint32_t *ic, uint64_t *stride,
+                            double *factor);
+
+/* One string of a numerical transform: operator c += factor * op(a) (conj:
+ * transposed, with block2's transpose factor), both in one operator set.  */
+typedef struct {
+    int32_t c, a;
+    uint32_t conj;
+    uint32_t pad;
+    double factor;
+} b2g_transform_term;
+
+/* TensorFunctions::numerical_transform (src/core/tensor_functions.hpp:
+ * 2462-2516) through OperatorFunctions::iadd: new operators of an
+ * environment from the existing ones (the MPO's switch from normal to
+ * complementary operators).  Run with a == c (one buffer): no term may read
+ * an operator some term writes.                                           */
+int b2g_transform_plan(b2g_ctx ctx, int symmetry, int n_infos,
+                       const uint64_t *info_dq,
+                       const b2g_sparse_info *infos, int n_ops,
+                       const b2g_operator *ops, int n_terms,
+                       const b2g_transform_term *terms, unsigned flags,
+                       b2g_winplan *out);
+
+/* SparseMatrix::swap_to_fused_left (to_left != 0: [l x (m r)] ->
+ * [(l m) x r]) / swap_to_fused_right ([(l m) x r] -> [l x (m r)])
+ * (src/core/sparse_matrix.hpp:1823-1968): old_fused / new_fused are the
+ * fused bases (m (x) r and l (x) m, or the reverse) as MPSInfo::
+ * swap_wfn_to_fused_left / right form them (src/dmrg/mps.hpp:716-790).   */
+int b2g_swap_plan(b2g_ctx ctx, int symmetry, int to_left,
+                  const b2g_state_info *l, const b2g_state_info *m,
+                  const b2g_state_info *r, const b2g_state_info *old_fused,
+                  const b2g_state_info *new_fused,
+                  const b2g_sparse_info *old_wfn,
+                  const b2g_sparse_info *new_wfn, unsigned flags,
+                  b2g_winplan *out);
+
+/* C = (accumulate ? C : 0) + the plan's sums.  a, b, c: device FP64.      */
+int b2g_winplan_run(b2g_winplan w, const double *a, const double *b,
+                    double *c, int accumulate);
+/* windows, records, elements read (A and B) and written, and the length
+ * of C the plan covers                                                    */
+int b2g_winplan_stats(b2g_winplan w, uint64_t *n_groups, uint64_t *n_recs,
+                      uint64_t *src_elems, uint64_t *dst_elems,
+                      uint64_t *c_len);
+int b2g_winplan_destroy(b2g_winplan w);
+
+/* TensorFunctions::left_rotate / right_rotate (src/core/tensor_functions.hpp:
+ * 2365-2403) through OperatorFunctions::tensor_rotate
+ * (src/core/operator_functions.hpp:175-211): for every operator,
+ * C[op] = rot^T A[op] rot (trans == 0, left) or rot A[op] rot^T (trans !=
+ * 0, right), as a replay plan of GEMM-chain pairs (b2g_hpsi with x = the A
+ * operator data, y = the C operator data, n_rhs 1).  rot: the MPS tensor
+ * (delta quantum vacuum) whose data is rot_buf.  A and C operator sets as
+ * in b2g_contract; a C operator is rotated from the A operator of the same
+ * index.                                                                  */
+int b2g_rotate_plan(b2g_ctx ctx, int symmetry, int trans,
+                    const b2g_sparse_info *rot, b2g_buf rot_buf,
+                    int n_a_infos, const uint64_t *a_info_dq,
+                    const b2g_sparse_info *a_infos, int n_c_infos,
+                    const uint64_t *c_info_dq,
+                    const b2g_sparse_info *c_infos, int n_ops,
+                    const b2g_operator *a_ops, const b2g_operator *c_ops,
+                    size_t a_len, size_t c_len, unsigned flags,
+                    b2g_plan *out);
+
+/* A list of block products D[i] = scale * L[il] R[ir] on the device
+ * (grouped-GEMM kernels): SparseMatrix::contract
+ * (src/core/sparse_matrix.hpp:1777-1801), the two-site wavefunction of
+ * MovingEnvironment::contract_two_dot (src/dmrg/moving_environment.hpp:
+ * 3361-3404).  l_is_wfn != 0: wfn = wfn x rot (lmat a wavefunction, rmat
+ * a right-canonical tensor); 0: wfn = rot x wfn.                          */
+typedef struct b2g_gemmlist_s *b2g_gemmlist;
+int b2g_contract_wfn_plan(b2g_ctx ctx, int symmetry,
+                          const b2g_sparse_info *lmat, int l_is_wfn,
+                          const b2g_sparse_info *rmat,
+                          const b2g_sparse_info *out, double scale,
+                          b2g_gemmlist *plan);
+/* D = the products (every D block written; blocks without a product 0)  */
+int b2g_gemmlist_run(b2g_gemmlist g, const double *l, const double *r,
+                     double *d);
+int b2g_gemmlist_stats(b2g_gemmlist g, uint64_t *n_products,
+                       uint64_t *macs);
+int b2g_gemmlist_destroy(b2g_gemmlist g);
+
+/* A non-owning buffer over existing device memory (n elements of dtype):
+ * destroying it leaves the memory alone.                                  */
+int b2g_buf_wrap(b2g_ctx ctx, void *ptr, size_t n, int dtype, b2g_buf *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -111,6 +111,13 @@ struct Settings {
     string noise_type = "dm";
     string load_dir;   // --load-problem: block2's H*psi on a given problem
     int reps = 3;
+    // --save-sweep 1: block2's MPO and initial MPS written in its own file
+    // formats (MPO::save_data, MPSInfo::save_data, SparseMatrix::save_data)
+    // under OUT/sweep, with a record of every sweep step; --seed fixes the
+    // generator for the random initial MPS (0: block2's clock seed) and
+    // --noise-seed re-seeds it right before the sweeps (0: no re-seed)
+    bool save_sweep = false;
+    unsigned seed = 0, noise_seed = 0;
 };
 static Settings G;
 
@@ -296,7 +303,18 @@ static void shim_check(const shared_ptr<EffectiveHamiltonian<S, double>> &h,
 template <typename S> struct Dumper : CallbackKernel {
     DMRG<S, double, double> *dmrg = nullptr;
     mutable int ndumped = 0;
+    // one record per sweep step (DMRG::sweep::iter.end): sweep, step,
+    // the step's Davidson energy and discarded weight
+    mutable vector<double> steps;
     void compute(const string &name, int iprint) const override {
+        if (name == "DMRG::sweep::iter.end" && dmrg != nullptr &&
+            dmrg->sweep_energies.size() != 0) {
+            steps.push_back((double)dmrg->isweep);
+            steps.push_back((double)(dmrg->sweep_energies.size() - 1));
+            steps.push_back((double)dmrg->sweep_energies.back().back());
+            steps.push_back((double)dmrg->sweep_discarded_weights.back());
+            return;
+        }
         if (name != "DMRG::sweep::iter.eff_ham" || dmrg == nullptr)
             return;
         auto h_eff = dmrg->current_eff_ham;
@@ -727,6 +745,28 @@ template <> SU2 make_target<SU2>(int n, int twos, int pg) {
     return SU2(n, twos, pg);
 }
 
+// --save-sweep: the inputs of the sweeps in block2's own file formats
+// (MPO::save_data, src/dmrg/mpo.hpp:795-886; MPSInfo::save_data,
+// src/dmrg/mps.hpp:169-191; SparseMatrix::save_data with its info,
+// src/core/sparse_matrix.hpp:954-989), and the MPS layout (mps.txt: sites,
+// center, dot, canonical form).
+template <typename S>
+static void save_sweep_inputs(const shared_ptr<MPO<S, double>> &mpo,
+                              const shared_ptr<MPS<S, double>> &mps,
+                              const shared_ptr<MPSInfo<S>> &info) {
+    string d = G.outdir + "/sweep";
+    mkdir(d.c_str(), 0755);
+    mpo->save_data(d + "/mpo.bin");
+    info->save_data(d + "/mps_info.bin");
+    for (int i = 0; i < mps->n_sites; i++)
+        if (mps->tensors[i] != nullptr)
+            mps->tensors[i]->save_data(d + "/mps_" + to_string(i) + ".bin",
+                                       true);
+    ofstream m(d + "/mps.txt");
+    m << mps->n_sites << " " << mps->center << " " << mps->dot << " "
+      << mps->canonical_form << "\n";
+}
+
 template <typename S> static int run() {
     shared_ptr<FCIDUMP<double>> fcidump = make_shared<FCIDUMP<double>>();
     fcidump->read(G.fcidump);
@@ -749,13 +789,15 @@ template <typename S> static int run() {
         mpo, make_shared<RuleQC<S, double>>(), true, true,
         OpNamesSet({OpNames::R, OpNames::RD}));
     cerr << "MPO done T = " << t.get_time() << endl;
-    Random::rand_seed(0);
+    Random::rand_seed(G.seed);
     auto mps_info =
         make_shared<MPSInfo<S>>(mpo->n_sites, vacuum, target, mpo->basis);
     mps_info->set_bond_dimension(G.bdims[0]);
     auto mps = make_shared<MPS<S, double>>(mpo->n_sites, 0, 2);
     mps->initialize(mps_info);
     mps->random_canonicalize();
+    if (G.save_sweep)
+        save_sweep_inputs<S>(mpo, mps, mps_info);
     mps->save_mutable();
     mps->deallocate();
     mps_info->save_mutable();
@@ -778,6 +820,8 @@ template <typename S> static int run() {
     auto dumper = make_shared<Dumper<S>>();
     dumper->dmrg = dmrg.get();
     callback_() = dumper;
+    if (G.noise_seed != 0)
+        Random::rand_seed(G.noise_seed);
     double energy = dmrg->solve(G.nsweeps, mps->center == 0, G.dav_conv * 0.1);
     callback_() = make_shared<CallbackKernel>();
     ofstream ofs(G.outdir + "/energies.txt");
@@ -789,6 +833,25 @@ template <typename S> static int run() {
                 << (i < dmrg->sweep_time.size() ? dmrg->sweep_time[i] : 0.0)
                 << "\n";
     ofs << "dumped " << dumper->ndumped << "\n";
+    if (G.save_sweep) {
+        write_npy(G.outdir + "/sweep/steps.npy", dumper->steps);
+        ofstream sf(G.outdir + "/sweep/settings.txt");
+        sf << setprecision(17) << "tol " << G.dav_conv * 0.1 << "\n"
+           << "nsweeps " << G.nsweeps << "\n"
+           << "noise_type " << G.noise_type << "\n"
+           << "noise_seed " << G.noise_seed << "\n"
+           << "final " << energy << "\n";
+        sf << "bdims";
+        for (int b : G.bdims)
+            sf << " " << b;
+        sf << "\nnoises";
+        for (double x : G.noises)
+            sf << " " << x;
+        sf << "\ndav_conv";
+        for (double x : dmrg->davidson_conv_thrds)
+            sf << " " << x;
+        sf << "\n";
+    }
     cerr << "E = " << setprecision(15) << energy << endl;
     mps_info->deallocate();
     me->remove_partition_files();
@@ -1018,6 +1081,12 @@ int main(int argc, char **argv) {
             G.load_dir = v;
         else if (k == "--reps")
             G.reps = stoi(v);
+        else if (k == "--save-sweep")
+            G.save_sweep = stoi(v) != 0;
+        else if (k == "--seed")
+            G.seed = (unsigned)stoul(v);
+        else if (k == "--noise-seed")
+            G.noise_seed = (unsigned)stoul(v);
         else {
             cerr << "unknown option " << k << endl;
             return 2;
