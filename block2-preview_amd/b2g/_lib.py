@@ -104,6 +104,39 @@ class b2g_davidson_opts(C.Structure):
                 ("proj_weights", f64p)]
 
 
+class b2g_state_info(C.Structure):
+    _fields_ = [("n", C.c_int32), ("pad", C.c_int32), ("quanta", u64p),
+                ("n_states", u32p)]
+
+
+class b2g_contract_term(C.Structure):
+    _fields_ = [("c", C.c_int32), ("a", C.c_int32), ("b", C.c_int32),
+                ("conj", C.c_uint32), ("factor", C.c_double)]
+
+
+class b2g_transform_term(C.Structure):
+    _fields_ = [("c", C.c_int32), ("a", C.c_int32), ("conj", C.c_uint32),
+                ("pad", C.c_uint32), ("factor", C.c_double)]
+
+
+class b2g_contract(C.Structure):
+    _fields_ = [("symmetry", C.c_int32), ("n_terms", C.c_int32),
+                ("bra_a", b2g_state_info), ("ket_a", b2g_state_info),
+                ("bra_b", b2g_state_info), ("ket_b", b2g_state_info),
+                ("bra", b2g_state_info), ("ket", b2g_state_info),
+                ("n_a_infos", C.c_int32), ("n_b_infos", C.c_int32),
+                ("n_c_infos", C.c_int32), ("n_a_ops", C.c_int32),
+                ("n_b_ops", C.c_int32), ("n_c_ops", C.c_int32),
+                ("a_info_dq", u64p), ("b_info_dq", u64p), ("c_info_dq", u64p),
+                ("a_infos", C.POINTER(b2g_sparse_info)),
+                ("b_infos", C.POINTER(b2g_sparse_info)),
+                ("c_infos", C.POINTER(b2g_sparse_info)),
+                ("a_ops", C.POINTER(b2g_operator)),
+                ("b_ops", C.POINTER(b2g_operator)),
+                ("c_ops", C.POINTER(b2g_operator)),
+                ("terms", C.POINTER(b2g_contract_term))]
+
+
 # DavidsonTypes bits taken by b2g_effham_eigs_ex (b2g.h B2G_DAV_*)
 DAV = {"Normal": 0, "GreaterThan": 1, "LessThan": 2, "CloseTo": 4,
        "DavidsonPrecond": 32, "NoPrecond": 64, "Exact": 256,
@@ -232,6 +265,45 @@ _SIGS = {
                                 C.c_int, C.c_int, C.c_double, f64p,
                                 C.c_size_t]),
     "b2g_smat_destroy": (C.c_int, [_vp]),
+    "b2g_contract_plan": (C.c_int, [_vp, C.POINTER(b2g_contract), C.c_uint,
+                                    C.POINTER(_vp)]),
+    "b2g_winplan_connections": (C.c_int, [
+        _vp, C.c_int, C.POINTER(C.c_int32), u64p, u64p, u32p, u64p, u32p,
+        u32p, u32p, u64p, f64p]),
+    "b2g_transform_plan": (C.c_int, [
+        _vp, C.c_int, C.c_int, u64p, C.POINTER(b2g_sparse_info), C.c_int,
+        C.POINTER(b2g_operator), C.c_int, C.POINTER(b2g_transform_term),
+        C.c_uint, C.POINTER(_vp)]),
+    "b2g_swap_plan": (C.c_int, [
+        _vp, C.c_int, C.c_int, C.POINTER(b2g_state_info),
+        C.POINTER(b2g_state_info), C.POINTER(b2g_state_info),
+        C.POINTER(b2g_state_info), C.POINTER(b2g_state_info),
+        C.POINTER(b2g_sparse_info), C.POINTER(b2g_sparse_info), C.c_uint,
+        C.POINTER(_vp)]),
+    "b2g_winplan_run": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int]),
+    "b2g_winplan_stats": (C.c_int, [_vp, u64p, u64p, u64p, u64p, u64p]),
+    "b2g_winplan_destroy": (C.c_int, [_vp]),
+    "b2g_rotate_plan": (C.c_int, [
+        _vp, C.c_int, C.c_int, C.POINTER(b2g_sparse_info), _vp, C.c_int,
+        u64p, C.POINTER(b2g_sparse_info), C.c_int, u64p,
+        C.POINTER(b2g_sparse_info), C.c_int, C.POINTER(b2g_operator),
+        C.POINTER(b2g_operator), C.c_size_t, C.c_size_t, C.c_uint,
+        C.POINTER(_vp)]),
+    "b2g_contract_wfn_plan": (C.c_int, [
+        _vp, C.c_int, C.POINTER(b2g_sparse_info), C.c_int,
+        C.POINTER(b2g_sparse_info), C.POINTER(b2g_sparse_info), C.c_double,
+        C.POINTER(_vp)]),
+    "b2g_gemmlist_run": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "b2g_gemmlist_stats": (C.c_int, [_vp, u64p, u64p]),
+    "b2g_gemmlist_destroy": (C.c_int, [_vp]),
+    "b2g_rng_create": (C.c_int, [C.c_uint, C.POINTER(_vp)]),
+    "b2g_rng_destroy": (C.c_int, [_vp]),
+    "b2g_split_density_matrix_noise_rng": (C.c_int, [
+        _vp, C.c_int, C.POINTER(b2g_sparse_info), _vp, C.c_int, f64p,
+        C.c_double, C.c_int, _vp, C.c_int, C.c_double, C.c_int,
+        C.c_int, C.c_uint, C.c_double, C.POINTER(_vp)]),
+    "b2g_buf_wrap": (C.c_int, [_vp, _vp, C.c_size_t, C.c_int,
+                               C.POINTER(_vp)]),
 }
 
 _lib = None

@@ -1,0 +1,719 @@
+"""Two-site DMRG sweeps with every numerical step on the GPU.
+
+The sweep follows block2's DMRG::solve / sweep / update_two_dot
+(src/dmrg/sweep_algorithm.hpp:3037-3130, 2556-2705, 814-1182) over
+MovingEnvironment (src/dmrg/moving_environment.hpp):
+
+  environment blocking   left_contract / right_contract (:1745-2068)
+                         -> b2g_contract_plan (window sums on the device)
+  renormalisation        left/right_contract_rotate (:226-460, 460-640)
+                         -> b2g_rotate_plan (grouped-GEMM replay)
+  normal/complementary   numerical_transform at the MPO's schemer sites
+                         -> b2g_transform_plan
+  effective Hamiltonian  eff_ham (:2071-2209) -> EffectiveHamiltonian
+  Davidson               two_dot_eigs_and_perturb (:1183-1266) -> eigs
+  truncation             density_matrix + split_density_matrix
+                         -> truncate.split_density_matrix (device FP64)
+  wavefunction moves     contract_two_dot (:3361-3404), propagate_wfn
+                         (:4490-4516) -> b2g_contract_wfn_plan,
+                         b2g_swap_plan
+
+Inputs are block2's own files: the MPO (MPO::save_data) and the starting
+MPS (MPSInfo::save_data and each tensor's SparseMatrix::save_data), read
+by b2g.mpo.  The operator expressions, site operators and bases come from
+the MPO; the library builds every connection list and block layout.
+
+Supported: real FP64, SZ and SU(2), MPOs whose contraction and H_eff
+expressions are sums of two-operator products (MPOQC conventional, with
+its schemer), density-matrix decomposition with no noise, wavefunction or
+density-matrix noise.  Perturbative noise and delayed contraction are not
+built here.
+"""
+import ctypes as C
+import time
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from . import states as Q
+from .effham import EffectiveHamiltonian
+from .mpo import load_mpo, load_mps
+from .problem import InfoSet, OpSet, Problem
+from . import truncate as T
+
+ZERO_T = ("zero",)
+
+
+def _key(e):
+    """Identity of an operator element: name, site indices, quantum (its
+    scalar factor belongs to the expression)."""
+    if e[0] != "elem":
+        raise ValueError("expected an operator element, got %s" % e[0])
+    return (e[1], e[2], e[4])
+
+
+def _unref(e):
+    return e[2] if e[0] == "exprref" else e
+
+
+def _products(e):
+    """(factor, conj, a, b) of every string of an expression that is a sum
+    of two-operator products."""
+    e = _unref(e)
+    if e[0] == "zero":
+        return []
+    if e[0] == "prod":
+        return [(e[1], e[2], e[3], e[4])]
+    if e[0] == "sum":
+        out = []
+        for x in e[1]:
+            if x[0] != "prod":
+                raise NotImplementedError(
+                    "expression string of type %s (intermediate or delayed "
+                    "operators) is not supported" % x[0])
+            out.append((x[1], x[2], x[3], x[4]))
+        return out
+    raise NotImplementedError("expression of type %s" % e[0])
+
+
+def _sinfo(si, keep):
+    q = np.ascontiguousarray(si.q, np.uint64)
+    n = np.ascontiguousarray(si.n, np.uint32)
+    keep += [q, n]
+    return L.b2g_state_info(len(q), 0, L.ptr(q, C.c_uint64),
+                            L.ptr(n, C.c_uint32))
+
+
+def _cinfo(lay, keep):
+    arrs = [np.ascontiguousarray(lay.quanta, np.uint64),
+            np.ascontiguousarray(lay.nbra, np.uint32),
+            np.ascontiguousarray(lay.nket, np.uint32),
+            np.ascontiguousarray(lay.shift, np.uint64)]
+    keep += arrs
+    return L.b2g_sparse_info(int(lay.dq), int(lay.fermion), len(lay),
+                             L.ptr(arrs[0], C.c_uint64),
+                             L.ptr(arrs[1], C.c_uint32),
+                             L.ptr(arrs[2], C.c_uint32),
+                             L.ptr(arrs[3], C.c_uint64))
+
+
+def _dev(t):
+    return C.c_void_p(t.data_ptr())
+
+
+class OpTensor:
+    """The operators of one block (an OperatorTensor's ops): keys in order,
+    one block layout per delta quantum (sorted), device FP64 data."""
+
+    def __init__(self, sym, bra, ket, keys, dqs, device, factors=None):
+        self.sym, self.bra, self.ket = sym, bra, ket
+        self.keys = list(keys)
+        self.index = {k: i for i, k in enumerate(self.keys)}
+        self.dq = [int(d) for d in dqs]
+        self.info_dq = sorted(set(self.dq))
+        self.infos = [Q.initialize(sym, bra, ket, d, Q.q_is_fermion(d))
+                      for d in self.info_dq]
+        self.factor = (np.ones(len(keys)) if factors is None else
+                       np.asarray(factors, np.float64))
+        self.info_of = [self.info_dq.index(d) for d in self.dq]
+        self.off = []
+        t = 0
+        for i in self.info_of:
+            self.off.append(t)
+            t += self.infos[i].total
+        self.total = t
+        self.data = torch.zeros(max(t, 1), dtype=torch.float64, device=device)
+
+    def layout(self, k):
+        return self.infos[self.info_of[k]]
+
+    def c_side(self, keep):
+        dq = np.ascontiguousarray(self.info_dq, np.uint64)
+        keep.append(dq)
+        inf = (L.b2g_sparse_info * max(len(self.infos), 1))(
+            *[_cinfo(x, keep) for x in self.infos])
+        ops = (L.b2g_operator * max(len(self.keys), 1))(
+            *[L.b2g_operator(self.dq[k], float(self.factor[k]),
+                             self.info_of[k], 0, self.off[k])
+              for k in range(len(self.keys))])
+        keep += [inf, ops]
+        return len(self.infos), dq, inf, len(self.keys), ops
+
+    def subset(self, keys):
+        """The operators `keys` of this tensor (same data, new index)."""
+        r = object.__new__(OpTensor)
+        r.sym, r.bra, r.ket = self.sym, self.bra, self.ket
+        r.keys = list(keys)
+        r.index = {k: i for i, k in enumerate(r.keys)}
+        pos = [self.index[k] for k in r.keys]
+        r.dq = [self.dq[p] for p in pos]
+        r.info_dq, r.infos = self.info_dq, self.infos
+        r.info_of = [self.info_of[p] for p in pos]
+        r.off = [self.off[p] for p in pos]
+        r.factor = np.array([self.factor[p] for p in pos])
+        r.total, r.data = self.total, self.data
+        return r
+
+
+class SiteOps:
+    """The site operators of one MPO tensor on the device."""
+
+    def __init__(self, sym, tensor, site_op_infos, device):
+        self.sym = sym
+        self.keys, self.dq, self.factor, self.off = [], [], [], []
+        lay = {}
+        for dq, d in site_op_infos:
+            lay[int(dq)] = Q.Layout.from_dict(d, wfn=False)
+        self.info_dq = sorted(lay)
+        self.infos = [lay[d] for d in self.info_dq]
+        chunks, t = [], 0
+        for k, m in tensor["ops"]:
+            key = _key(k)
+            d = int(m["info"]["dq"])
+            if not Q.Layout.from_dict(m["info"], wfn=False).same(lay[d]):
+                raise ValueError("site operator layout differs from its "
+                                 "site_op_infos entry")
+            self.keys.append(key)
+            self.dq.append(d)
+            self.factor.append(float(m["factor"]))
+            self.off.append(t)
+            data = np.asarray(m["data"], np.float64)
+            if data.size != lay[d].total:
+                data = np.zeros(lay[d].total)
+            chunks.append(data)
+            t += data.size
+        self.index = {k: i for i, k in enumerate(self.keys)}
+        self.info_of = [self.info_dq.index(d) for d in self.dq]
+        self.host = np.concatenate(chunks) if chunks else np.zeros(1)
+        self.data = torch.tensor(self.host if self.host.size else [0.0],
+                                 dtype=torch.float64, device=device)
+        self.lmat, self.rmat = tensor["lmat"], tensor["rmat"]
+
+    def c_side(self, keep):
+        dq = np.ascontiguousarray(self.info_dq, np.uint64)
+        keep.append(dq)
+        inf = (L.b2g_sparse_info * max(len(self.infos), 1))(
+            *[_cinfo(x, keep) for x in self.infos])
+        ops = (L.b2g_operator * max(len(self.keys), 1))(
+            *[L.b2g_operator(self.dq[k], self.factor[k], self.info_of[k], 0,
+                             self.off[k]) for k in range(len(self.keys))])
+        keep += [inf, ops]
+        return len(self.infos), dq, inf, len(self.keys), ops
+
+    def block_of(self, k, q):
+        """host copy of site operator k's block with composite quanta q"""
+        lay = self.infos[self.info_of[k]]
+        i = lay.find(q)
+        if i < 0:
+            return None
+        o = self.off[k] + int(lay.shift[i])
+        return self.host[o:o + int(lay.nbra[i]) * int(lay.nket[i])]
+
+
+class MPSTensor:
+    """An MPS tensor on the device: block layout and data."""
+
+    def __init__(self, lay, data):
+        self.lay, self.data = lay, data
+
+
+def _layout_of_split(bi, wfn):
+    return Q.Layout(bi.quanta, bi.nbra, bi.nket, bi.dq, bi.fermion, wfn)
+
+
+class DMRG:
+    """block2's DMRG (sweep_algorithm.hpp) on the GPU for one MPS.
+
+    DMRG(ctx, symmetry ("sz" / "su2"), mpo_file, mps_dir, bond_dims,
+    noises) reads block2's files;
+    solve(n_sweeps, forward, tol) runs the sweeps and returns the final
+    energy; sweep_energies / energies / discarded_weights / sweep_time hold
+    the per-sweep results and `steps` one record per sweep step (sweep,
+    step, energy, discarded weight, Davidson iterations, seconds)."""
+
+    def __init__(self, ctx, symmetry, mpo_file, mps_dir, bond_dims, noises,
+                 ubond=4, noise_type="dm", noise_seed=0):
+        self.ctx = ctx
+        self.dev = ctx.device
+        self.lib = L.lib()
+        self.mpo = load_mpo(mpo_file, ubond)
+        self.mps = load_mps(mps_dir, ubond)
+        m, ps = self.mpo, self.mps
+        self.n = m["n_sites"]
+        if ps["dot"] != 2:
+            raise NotImplementedError("two-site MPS only")
+        info = ps["info"]
+        self.sym = {"sz": Q.SZ, "su2": Q.SU2}[symmetry]
+        s = self.sym
+        self.target = int(info["target"])
+        self.vacuum = int(info["vacuum"])
+        self.basis = [Q.StateInfo.from_dict(b) for b in info["basis"]]
+        self.ldf = [Q.StateInfo.from_dict(b) for b in info["left_dims_fci"]]
+        self.rdf = [Q.StateInfo.from_dict(b) for b in info["right_dims_fci"]]
+        self.hq = int(m["op"][4])
+        self.const_e = float(m["const_e"])
+        self.site = [SiteOps(s, m["tensors"][i], m["site_op_infos"][i],
+                             self.dev) for i in range(self.n)]
+        self.schemer = m["schemer"]
+        self.bond_dims = list(bond_dims)
+        self.noises = list(noises)
+        self.noise_type = {"dm": T.NOISE_DENSITY_MATRIX,
+                           "wfn": T.NOISE_WAVEFUNCTION}[noise_type]
+        self.rng = T.Rng(noise_seed) if noise_seed else None
+        self.davidson_max_iter = 5000
+        self.davidson_soft_max_iter = 4000
+        self.davidson_def_min, self.davidson_def_max = 2, 50
+        self.cutoff = 1e-14
+        self.center = ps["center"]
+        self._load_tensors()
+
+    # ------------------------------------------------------------ MPS
+    def _load_tensors(self):
+        s, n = self.sym, self.n
+        vac = Q.StateInfo([self.vacuum], [1])
+        self.left_dims = [None] * (n + 1)
+        self.right_dims = [None] * (n + 1)
+        self.left_dims[0] = vac
+        self.right_dims[n] = vac
+        self.tensors = [None] * n
+        ts = self.mps["tensors"]
+        c = self.center
+        for i in range(n):
+            if ts[i] is None:
+                continue
+            lay = Q.Layout.from_dict(ts[i]["info"])
+            data = torch.tensor(ts[i]["data"] * ts[i]["factor"],
+                                dtype=torch.float64, device=self.dev)
+            self.tensors[i] = MPSTensor(lay, data)
+        for i in range(c + 2, n):
+            self.right_dims[i] = Q.extract_state_info(
+                s, self.tensors[i].lay, False)
+        for i in range(0, c):
+            self.left_dims[i + 1] = Q.extract_state_info(
+                s, self.tensors[i].lay, True)
+        # the two-site tensor at the center
+        wl = self._wfn_layout(c)
+        if not wl.same(self.tensors[c].lay):
+            raise ValueError("center tensor layout differs from the fused "
+                             "bases of the MPS")
+
+    def _fused_left(self, i):
+        return Q.tensor_product(self.sym, self.left_dims[i], self.basis[i],
+                                self.ldf[i + 1])
+
+    def _fused_right(self, i):
+        return Q.tensor_product(self.sym, self.basis[i], self.right_dims[i + 1],
+                                self.rdf[i])
+
+    def _wfn_layout(self, i):
+        return Q.initialize(self.sym, self._fused_left(i),
+                            self._fused_right(i + 1), self.target, False,
+                            True)
+
+    # ------------------------------------------------------ contractions
+    def _run_contract(self, A, B, Cs, bra_a, ket_a, bra_b, ket_b, terms):
+        keep = []
+        d = L.b2g_contract()
+        d.symmetry = self.sym
+        d.bra_a, d.ket_a = _sinfo(bra_a, keep), _sinfo(ket_a, keep)
+        d.bra_b, d.ket_b = _sinfo(bra_b, keep), _sinfo(ket_b, keep)
+        d.bra, d.ket = _sinfo(Cs.bra, keep), _sinfo(Cs.ket, keep)
+        d.n_a_infos, d.a_info_dq, d.a_infos, d.n_a_ops, d.a_ops = \
+            A.c_side(keep)
+        d.n_b_infos, d.b_info_dq, d.b_infos, d.n_b_ops, d.b_ops = \
+            B.c_side(keep)
+        d.n_c_infos, d.c_info_dq, d.c_infos, d.n_c_ops, d.c_ops = \
+            Cs.c_side(keep)
+        tm = (L.b2g_contract_term * max(len(terms), 1))(
+            *[L.b2g_contract_term(c, a, b, cj, f)
+              for (c, a, b, cj, f) in terms])
+        keep.append(tm)
+        d.n_terms, d.terms = len(terms), tm
+        h = C.c_void_p()
+        L.check(self.lib.b2g_contract_plan(self.ctx.h, C.byref(d), 0,
+                                           C.byref(h)))
+        try:
+            torch.cuda.current_stream(self.dev).synchronize()
+            L.check(self.lib.b2g_winplan_run(h, _dev(A.data), _dev(B.data),
+                                             _dev(Cs.data), 0))
+            self.ctx.synchronize()
+        finally:
+            self.lib.b2g_winplan_destroy(h)
+
+    def _vacuum_env(self):
+        vac = Q.StateInfo([self.vacuum], [1])
+        e = OpTensor(self.sym, vac, vac, [("I",)], [self.vacuum], self.dev)
+        e.data.fill_(1.0)
+        return e
+
+    def _names(self, sym_entry):
+        return [x for x in sym_entry["data"]]
+
+    def left_block(self, i, env):
+        """Operators of (left environment of site i) (x) site i on the fused
+        basis (MovingEnvironment::left_contract; site 0: left_assign)."""
+        s, m = self.sym, self.mpo
+        fused = self._fused_left(i)
+        names = [x for x in m["left_operator_names"][i]["data"]]
+        keys = [_key(x) for x in names]
+        Cs = OpTensor(s, fused, fused, keys, [x[4] for x in names], self.dev)
+        site = self.site[i]
+        vac = Q.StateInfo([self.vacuum], [1])
+        if i == 0:
+            # left_assign: the site tensor's row vector (tensor_functions.hpp
+            # :185-221) as products with the vacuum identity
+            A = self._vacuum_env()
+            lm = site.lmat["data"]
+            terms = []
+            for j, x in enumerate(lm):
+                if x[0] == "zero":
+                    continue
+                terms.append((j, 0, site.index[_key(x)], 0, 1.0))
+            self._run_contract(A, site, Cs, vac, vac, self.basis[i],
+                               self.basis[i], terms)
+            return Cs
+        exprs = m["left_operator_exprs"][i]["data"]
+        terms = []
+        for j, e in enumerate(exprs):
+            fn = names[j][3]
+            for f, cj, a, b in _products(e):
+                terms.append((j, env.index[_key(a)], site.index[_key(b)], cj,
+                              f / fn))
+        self._run_contract(env, site, Cs, env.bra, env.ket, self.basis[i],
+                           self.basis[i], terms)
+        return Cs
+
+    def right_block(self, i, env):
+        """Operators of site i (x) (right environment of site i) on the
+        fused basis (right_contract; last site: right_assign)."""
+        s, m = self.sym, self.mpo
+        fused = self._fused_right(i)
+        names = [x for x in m["right_operator_names"][i]["data"]]
+        keys = [_key(x) for x in names]
+        Cs = OpTensor(s, fused, fused, keys, [x[4] for x in names], self.dev)
+        site = self.site[i]
+        vac = Q.StateInfo([self.vacuum], [1])
+        if i == self.n - 1:
+            B = self._vacuum_env()
+            rm = site.rmat["data"]
+            terms = []
+            for j, x in enumerate(rm):
+                if x[0] == "zero":
+                    continue
+                terms.append((j, site.index[_key(x)], 0, 0, 1.0))
+            self._run_contract(site, B, Cs, self.basis[i], self.basis[i], vac,
+                               vac, terms)
+            return Cs
+        exprs = m["right_operator_exprs"][i]["data"]
+        terms = []
+        for j, e in enumerate(exprs):
+            fn = names[j][3]
+            for f, cj, a, b in _products(e):
+                terms.append((j, site.index[_key(a)], env.index[_key(b)], cj,
+                              f / fn))
+        self._run_contract(site, env, Cs, self.basis[i], self.basis[i],
+                           env.bra, env.ket, terms)
+        return Cs
+
+    def rotate(self, blk, rot, bond, trans):
+        """Renormalised operators on `bond` (left_rotate: trans False,
+        rot^T A rot; right_rotate: trans True, rot A rot^T)."""
+        s = self.sym
+        out = OpTensor(s, bond, bond, blk.keys, blk.dq, self.dev,
+                       blk.factor)
+        out.factor = np.ones(len(blk.keys))
+        keep = []
+        na, adq, ainf, nop, aops = blk.c_side(keep)
+        nc, cdq, cinf, _, cops = out.c_side(keep)
+        rbuf = C.c_void_p()
+        L.check(self.lib.b2g_buf_wrap(self.ctx.h, _dev(rot.data),
+                                      rot.data.numel(), L.B2G_F64,
+                                      C.byref(rbuf)))
+        plan = C.c_void_p()
+        try:
+            ri = _cinfo(rot.lay, keep)
+            L.check(self.lib.b2g_rotate_plan(
+                self.ctx.h, s, 1 if trans else 0, C.byref(ri), rbuf, na, L.ptr(adq, C.c_uint64),
+                ainf, nc, L.ptr(cdq, C.c_uint64), cinf, nop, aops, cops,
+                blk.total, out.total, 0, C.byref(plan)))
+            torch.cuda.current_stream(self.dev).synchronize()
+            L.check(self.lib.b2g_hpsi(self.ctx.h, plan, _dev(blk.data),
+                                      _dev(out.data), 1, 1.0, 0,
+                                      L.PREC["f64"]))
+            self.ctx.synchronize()
+        finally:
+            if plan:
+                self.lib.b2g_plan_destroy(plan)
+            self.lib.b2g_buf_destroy(rbuf)
+        return out
+
+    def transform(self, env, names, exprs):
+        """TensorFunctions::numerical_transform + post_numerical_transform:
+        the operators `names` of the environment, new ones formed from
+        existing ones by `exprs`."""
+        s = self.sym
+        keys = [_key(x) for x in names["data"]]
+        new = [k for k in keys if k not in env.index]
+        dqs = {k: x[4] for k, x in zip(keys, names["data"])}
+        if new:
+            # an environment holding both sets (the new operators zero)
+            allk = env.keys + new
+            out = OpTensor(s, env.bra, env.ket, allk,
+                           env.dq + [dqs[k] for k in new], self.dev)
+            out.factor[:len(env.keys)] = env.factor
+            for k in env.keys:
+                a, b = env.index[k], out.index[k]
+                n = env.layout(a).total
+                out.data[out.off[b]:out.off[b] + n] = \
+                    env.data[env.off[a]:env.off[a] + n]
+            env = out
+        terms = []
+        for j, e in enumerate(exprs["data"]):
+            e = _unref(e)
+            if e[0] == "zero":
+                continue
+            fn = names["data"][j][3]
+            if e[0] != "sum":
+                raise NotImplementedError("transform expression %s" % e[0])
+            for x in e[1]:
+                if x[0] != "prod" or x[4] is not None:
+                    raise NotImplementedError("transform string")
+                terms.append((env.index[keys[j]], env.index[_key(x[3])],
+                              x[2] & 1, x[1] / fn))
+        if terms:
+            keep = []
+            ni, dq, inf, nop, ops = env.c_side(keep)
+            tm = (L.b2g_transform_term * len(terms))(
+                *[L.b2g_transform_term(c, a, cj, 0, f)
+                  for (c, a, cj, f) in terms])
+            h = C.c_void_p()
+            L.check(self.lib.b2g_transform_plan(
+                self.ctx.h, s, ni, L.ptr(dq, C.c_uint64), inf, nop, ops,
+                len(terms), tm, 0, C.byref(h)))
+            try:
+                torch.cuda.current_stream(self.dev).synchronize()
+                L.check(self.lib.b2g_winplan_run(h, _dev(env.data), None,
+                                                 _dev(env.data), 1))
+                self.ctx.synchronize()
+            finally:
+                self.lib.b2g_winplan_destroy(h)
+        return env.subset(keys)
+
+    # ------------------------------------------------- wavefunction moves
+    def _contract_wfn(self, lmat, l_is_wfn, rmat, out_lay):
+        keep = []
+        li, ri, oi = _cinfo(lmat.lay, keep), _cinfo(rmat.lay, keep), \
+            _cinfo(out_lay, keep)
+        h = C.c_void_p()
+        L.check(self.lib.b2g_contract_wfn_plan(
+            self.ctx.h, self.sym, C.byref(li), 1 if l_is_wfn else 0,
+            C.byref(ri), C.byref(oi), 1.0, C.byref(h)))
+        out = torch.zeros(max(out_lay.total, 1), dtype=torch.float64,
+                          device=self.dev)
+        try:
+            torch.cuda.current_stream(self.dev).synchronize()
+            L.check(self.lib.b2g_gemmlist_run(h, _dev(lmat.data),
+                                              _dev(rmat.data), _dev(out)))
+            self.ctx.synchronize()
+        finally:
+            self.lib.b2g_gemmlist_destroy(h)
+        return MPSTensor(out_lay, out)
+
+    def _swap(self, wfn, i, to_left):
+        """MPSInfo::swap_wfn_to_fused_left / right at site i."""
+        s = self.sym
+        l, mm, r = self.left_dims[i], self.basis[i], self.right_dims[i + 1]
+        lm = Q.tensor_product(s, l, mm, self.ldf[i + 1])
+        mr = Q.tensor_product(s, mm, r, self.rdf[i])
+        if to_left:
+            new = Q.initialize(s, lm, r, wfn.lay.dq, wfn.lay.fermion, True)
+            old_f, new_f = mr, lm
+        else:
+            new = Q.initialize(s, l, mr, wfn.lay.dq, wfn.lay.fermion, True)
+            old_f, new_f = lm, mr
+        keep = []
+        args = [_sinfo(x, keep) for x in (l, mm, r, old_f, new_f)]
+        oi, ni = _cinfo(wfn.lay, keep), _cinfo(new, keep)
+        h = C.c_void_p()
+        L.check(self.lib.b2g_swap_plan(
+            self.ctx.h, s, 1 if to_left else 0, *[C.byref(a) for a in args],
+            C.byref(oi), C.byref(ni), 0, C.byref(h)))
+        out = torch.zeros(max(new.total, 1), dtype=torch.float64,
+                          device=self.dev)
+        try:
+            torch.cuda.current_stream(self.dev).synchronize()
+            L.check(self.lib.b2g_winplan_run(h, _dev(wfn.data), None,
+                                             _dev(out), 0))
+            self.ctx.synchronize()
+        finally:
+            self.lib.b2g_winplan_destroy(h)
+        return MPSTensor(new, out)
+
+    # ------------------------------------------------------ environments
+    def _schemer_left(self, i):
+        sc = self.schemer
+        return sc is not None and i == sc["left_trans_site"]
+
+    def _schemer_right(self, i):
+        sc = self.schemer
+        return sc is not None and i == sc["right_trans_site"]
+
+    def init_environments(self):
+        """Right environments for a forward sweep from the center
+        (MovingEnvironment::init_environments, right_contract_rotate)."""
+        n, c = self.n, self.center
+        self.lenv = [None] * (n + 1)
+        self.renv = [None] * (n + 1)
+        for j in range(n - 1, c + 1, -1):
+            blk = self.right_block(j, self.renv[j + 1])
+            env = self.rotate(blk, self.tensors[j], self.right_dims[j], True)
+            if self._schemer_right(j):
+                sc = self.schemer
+                env = self.transform(env, sc["right_new_operator_names"],
+                                     sc["right_new_operator_exprs"])
+            self.renv[j] = env
+        if c != 0:
+            raise NotImplementedError("starting center must be site 0")
+
+    # ------------------------------------------------------------- sweep
+    def _eff_ham(self, i, lblk, rblk, lay):
+        s, m = self.sym, self.mpo
+        strings = _products(m["middle_operator_exprs"][i]["data"][0])
+        fn = m["middle_operator_names"][i]["data"][0][3]
+        conj, ta, tb, tf = [], [], [], []
+        for f, cj, a, b in strings:
+            conj.append(cj)
+            ta.append(lblk.index[_key(a)])
+            tb.append(rblk.index[_key(b)])
+            tf.append(f / fn)
+
+        def iset(lays):
+            n = [len(x) for x in lays]
+            off = np.concatenate([[0], np.cumsum(n)[:-1]]) if n else []
+            cat = lambda k, dt: (np.concatenate([getattr(x, k) for x in lays])
+                                 .astype(dt) if lays else np.zeros(0, dt))
+            return InfoSet([x.dq for x in lays], [x.fermion for x in lays],
+                           n, off, cat("quanta", np.uint64),
+                           cat("nbra", np.uint32), cat("nket", np.uint32),
+                           cat("shift", np.uint64))
+
+        def oset(t):
+            return OpSet(t.dq, t.factor, t.info_of, t.off, None)
+
+        w = iset([lay])
+        pb = Problem(s, self.hq, 0, w, w, iset(lblk.infos),
+                     iset(rblk.infos), oset(lblk), oset(rblk), conj, ta, tb,
+                     tf, dict(site=i))
+        return EffectiveHamiltonian(pb, self.ctx, fp32=False,
+                                    device_data=(lblk.data, rblk.data))
+
+    def update_two_dot(self, i, forward, bond_dim, noise, conv):
+        s, n = self.sym, self.n
+        t0 = time.perf_counter()
+        # the two-site wavefunction at (i, i+1)
+        if self.tensors[i + 1] is not None:
+            lay = self._wfn_layout(i)
+            a, b = self.tensors[i], self.tensors[i + 1]
+            if a.lay.wfn:   # wfn x rot
+                psi = self._contract_wfn(a, True, b, lay)
+            else:           # rot x wfn
+                psi = self._contract_wfn(a, False, b, lay)
+            self.tensors[i], self.tensors[i + 1] = psi, None
+        psi = self.tensors[i]
+        lay = psi.lay
+        lblk = self.left_block(i, self.lenv[i])
+        rblk = self.right_block(i + 1, self.renv[i + 2])
+        h = self._eff_ham(i, lblk, rblk, lay)
+        x = psi.data[:lay.total].clone()
+        e, ndav, nflop, tdav = h.eigs(
+            x, conv_thrd=conv, max_iter=self.davidson_max_iter,
+            soft_max_iter=self.davidson_soft_max_iter,
+            deflation_min_size=self.davidson_def_min,
+            deflation_max_size=self.davidson_def_max, precision="f64")
+        h.deallocate()
+        energy = e + self.const_e
+        bi = T.BlockInfo(lay.quanta, lay.nbra, lay.nket, lay.shift, lay.dq,
+                         int(lay.fermion))
+        sp = T.split_density_matrix(
+            self.ctx, s, bi, x, int(bond_dim), self.cutoff, forward, True,
+            T.TRUNC_PHYSICAL, 1.0, None, noise if noise else 0.0,
+            self.noise_type, 0, rng=self.rng if noise else None)
+        (linfo, ldata), (rinfo, rdata) = sp.left, sp.right
+        if forward:
+            left = MPSTensor(_layout_of_split(linfo, False), ldata)
+            right = MPSTensor(_layout_of_split(rinfo, True), rdata)
+            self.left_dims[i + 1] = Q.extract_state_info(s, left.lay, True)
+            env = self.rotate(lblk, left, self.left_dims[i + 1], False)
+            if self._schemer_left(i):
+                sc = self.schemer
+                env = self.transform(env, sc["left_new_operator_names"],
+                                     sc["left_new_operator_exprs"])
+            self.lenv[i + 1] = env
+            self.tensors[i], self.tensors[i + 1] = left, right
+            if i + 1 != n - 1:
+                self.tensors[i + 1] = self._swap(right, i + 1, True)
+        else:
+            left = MPSTensor(_layout_of_split(linfo, True), ldata)
+            right = MPSTensor(_layout_of_split(rinfo, False), rdata)
+            self.right_dims[i + 1] = Q.extract_state_info(s, right.lay, False)
+            env = self.rotate(rblk, right, self.right_dims[i + 1], True)
+            if self._schemer_right(i + 1):
+                sc = self.schemer
+                env = self.transform(env, sc["right_new_operator_names"],
+                                     sc["right_new_operator_exprs"])
+            self.renv[i + 1] = env
+            self.tensors[i], self.tensors[i + 1] = left, right
+            if i != 0:
+                self.tensors[i] = self._swap(left, i, False)
+        return energy, sp.error, ndav, time.perf_counter() - t0
+
+    def sweep(self, forward, bond_dim, noise, conv):
+        n = self.n
+        rng = range(self.center, n - 1) if forward else \
+            range(self.center, -1, -1)
+        es, errs = [], []
+        for i in rng:
+            e, err, nd, dt = self.update_two_dot(i, forward, bond_dim, noise,
+                                                 conv)
+            es.append(e)
+            errs.append(err)
+            self.steps.append((self.isweep, len(es) - 1, e, err, nd, dt))
+            self.center = i
+        if forward:
+            self.center = n - 2
+        else:
+            self.center = 0
+        return min(es), max(errs)
+
+    def solve(self, n_sweeps, forward=True, tol=1e-6, davidson_conv=None):
+        """DMRG::solve: bond dimensions, noises and Davidson thresholds
+        extended to n_sweeps as block2 extends them; stops when the energy
+        change is below tol with the final noise and bond dimension."""
+        bd = self.bond_dims + [self.bond_dims[-1]] * max(
+            0, n_sweeps - len(self.bond_dims))
+        ns = self.noises + [self.noises[-1] if self.noises else 0.0] * max(
+            0, n_sweeps - len(self.noises))
+        dc = list(davidson_conv or [])
+        for i in range(len(dc), len(ns)):
+            dc.append((ns[i] if ns[i] != 0 else (tol if tol != 0 else 1e-9))
+                      * 0.1)
+        self.davidson_conv_thrds = dc
+        self.energies, self.discarded_weights, self.sweep_time = [], [], []
+        self.steps = []
+        self.init_environments()
+        for iw in range(n_sweeps):
+            self.isweep = iw
+            t0 = time.perf_counter()
+            e, dw = self.sweep(forward, bd[iw], ns[iw], dc[iw])
+            torch.cuda.synchronize(self.dev)
+            self.sweep_time.append(time.perf_counter() - t0)
+            self.energies.append(e)
+            self.discarded_weights.append(dw)
+            conv = (len(self.energies) >= 2 and tol > 0 and
+                    abs(self.energies[-1] - self.energies[-2]) < tol and
+                    ns[iw] == ns[-1] and bd[iw] == bd[-1])
+            forward = not forward
+            if conv:
+                break
+        return self.energies[-1]

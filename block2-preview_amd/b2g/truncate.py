@@ -129,10 +129,28 @@ class Split:
         self.spectra = spectra
 
 
+class Rng:
+    """block2's random generator (std::mt19937 as Random::rand_seed(seed)
+    leaves it) held by the library; noise drawn through it continues one
+    stream across calls, as block2's generator does across a sweep."""
+
+    def __init__(self, seed):
+        self.lib = L.lib()
+        h = C.c_void_p()
+        L.check(self.lib.b2g_rng_create(int(seed), C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.b2g_rng_destroy(self.h)
+            self.h = None
+
+
 def split_density_matrix(ctx, symmetry, info, psi, k, cutoff=0.0,
                          trace_right=True, normalize=True,
                          trunc_type=TRUNC_PHYSICAL, scale=1.0, weights=None,
-                         noise=0.0, noise_type=NOISE_WAVEFUNCTION, seed=0):
+                         noise=0.0, noise_type=NOISE_WAVEFUNCTION, seed=0,
+                         rng=None):
     """Truncate the wavefunction psi (device FP64 tensor with the layout of
     `info`) to k states (-1: all) through its density matrix.  A 2-d psi
     holds state-averaged roots (one per row, density matrix weighted by
@@ -142,13 +160,25 @@ def split_density_matrix(ctx, symmetry, info, psi, k, cutoff=0.0,
     std::mt19937 seeded with `seed` as block2's Random after
     Random::rand_seed(seed).  Each call starts a new generator: with noise
     the seed must be nonzero (block2 takes 0 as a clock seed) and a sweep
-    passes a new one at each site."""
+    passes a new one at each site; or pass `rng` (an Rng) to draw from one
+    continuing stream."""
     lib = ctx.lib
     ci = info.to_c()
     psi = psi.contiguous()
     torch.cuda.synchronize(ctx.device)
     h = C.c_void_p()
-    if noise:
+    if noise and rng is not None:
+        nr = 1 if psi.dim() == 1 else psi.shape[0]
+        w = np.ascontiguousarray(
+            np.full(nr, 1.0 / nr) if weights is None else weights,
+            dtype=np.float64)
+        assert w.size == nr
+        L.check(lib.b2g_split_density_matrix_noise_rng(
+            ctx.h, int(symmetry), C.byref(ci), C.c_void_p(psi.data_ptr()),
+            nr, L.ptr(w, C.c_double), float(noise), int(noise_type),
+            rng.h, int(k), float(cutoff), int(bool(trace_right)),
+            int(bool(normalize)), int(trunc_type), float(scale), C.byref(h)))
+    elif noise:
         nr = 1 if psi.dim() == 1 else psi.shape[0]
         w = np.ascontiguousarray(
             np.full(nr, 1.0 / nr) if weights is None else weights,

@@ -889,6 +889,12 @@ extern "C" int b2g_syev_batched(b2g_ctx ctx, int n_mats, const uint32_t *n,
 }
 
 // ------------------------------------------------------------------ split
+// block2's generator (Random::rng, src/core/utils.hpp:196-252) as a
+// handle whose stream continues across calls
+struct b2g_rng_s {
+    std::mt19937 gen;
+};
+
 struct b2g_split_s {
     b2g_ctx ctx = nullptr;
     int n_roots = 1; // the wavefunction part holds n_roots consecutive roots
@@ -974,7 +980,7 @@ namespace b2g {
 static int add_noise(b2g_ctx ctx, const b2g_sparse_info *wfn, uint64_t psi_len,
                      int n_roots, const b2g_split_s::Part &dm,
                      const std::vector<int> &wfn_dm, const TaskSet &ts,
-                     double noise, int noise_type, unsigned seed) {
+                     double noise, int noise_type, std::mt19937 &gen) {
     (void)wfn;
     const bool wav = noise_type == B2G_NOISE_WAVEFUNCTION;
     // one draw per root from the continuing stream (every root's
@@ -984,7 +990,6 @@ static int add_noise(b2g_ctx ctx, const b2g_sparse_info *wfn, uint64_t psi_len,
         return B2G_OK;
     std::vector<double> T(len * n_roots), scal(n_roots);
     {
-        std::mt19937 gen(seed);
         std::uniform_real_distribution<double> distr(wav ? -0.5 : 0.0,
                                                      wav ? 0.5 : 1.0);
         for (uint64_t i = 0; i < len * n_roots; i++)
@@ -1064,7 +1069,7 @@ static int split_impl(b2g_ctx ctx, int symmetry, const b2g_sparse_info *wfn,
                       const double *psi, int n_roots, const double *weights,
                       int k, double cutoff, int trace_right, int normalize,
                       unsigned trunc_type, double scale, double noise,
-                      int noise_type, unsigned seed, b2g_split *out) {
+                      int noise_type, std::mt19937 *gen, b2g_split *out) {
     B2G_REQUIRE(ctx && wfn && out && (psi || wfn->n == 0), B2G_ERR_ARG,
                 "NULL arg");
     B2G_REQUIRE(n_roots >= 1 && weights != nullptr, B2G_ERR_ARG,
@@ -1174,8 +1179,9 @@ static int split_impl(b2g_ctx ctx, int symmetry, const b2g_sparse_info *wfn,
             return r;
         B2G_HIP(hipStreamSynchronize(ctx->stream));
         if (noise != 0.0) {
+            B2G_REQUIRE(gen != nullptr, B2G_ERR_ARG, "noise without a generator");
             r = add_noise(ctx, wfn, psi_len, n_roots, dm, wfn_dm, ts, noise,
-                          noise_type, seed);
+                          noise_type, *gen);
             if (r != B2G_OK)
                 return r;
         }
@@ -1406,8 +1412,8 @@ extern "C" int b2g_split_density_matrix(b2g_ctx ctx, int symmetry,
                                         double scale, b2g_split *out) {
     const double one = 1.0;
     return split_impl(ctx, symmetry, wfn, psi, 1, &one, k, cutoff,
-                      trace_right, normalize, trunc_type, scale, 0.0, 0, 0u,
-                      out);
+                      trace_right, normalize, trunc_type, scale, 0.0, 0,
+                      nullptr, out);
 }
 
 extern "C" int b2g_split_density_matrix_noise(
@@ -1421,9 +1427,35 @@ extern "C" int b2g_split_density_matrix_noise(
     B2G_REQUIRE(noise == 0.0 || seed != 0u, B2G_ERR_ARG,
                 "b2g_split_density_matrix_noise: seed 0 (block2 takes 0 as a "
                 "clock seed); pass a nonzero seed, a new one per site");
+    std::mt19937 gen(seed);
     return split_impl(ctx, symmetry, wfn, psi, n_roots, weights, k, cutoff,
                       trace_right, normalize, trunc_type, scale, noise,
-                      noise_type, seed, out);
+                      noise_type, &gen, out);
+}
+
+extern "C" int b2g_rng_create(unsigned seed, b2g_rng *out) {
+    B2G_REQUIRE(out != nullptr, B2G_ERR_ARG, "NULL arg");
+    B2G_REQUIRE(seed != 0u, B2G_ERR_ARG,
+                "b2g_rng_create: seed 0 (block2 takes 0 as a clock seed)");
+    *out = new b2g_rng_s{std::mt19937(seed)};
+    return B2G_OK;
+}
+
+extern "C" int b2g_rng_destroy(b2g_rng rng) {
+    delete rng;
+    return B2G_OK;
+}
+
+extern "C" int b2g_split_density_matrix_noise_rng(
+    b2g_ctx ctx, int symmetry, const b2g_sparse_info *wfn, const double *psi,
+    int n_roots, const double *weights, double noise, int noise_type,
+    b2g_rng rng, int k, double cutoff, int trace_right, int normalize,
+    unsigned trunc_type, double scale, b2g_split *out) {
+    B2G_REQUIRE(noise == 0.0 || rng != nullptr, B2G_ERR_ARG,
+                "b2g_split_density_matrix_noise_rng: NULL generator");
+    return split_impl(ctx, symmetry, wfn, psi, n_roots, weights, k, cutoff,
+                      trace_right, normalize, trunc_type, scale, noise,
+                      noise_type, rng ? &rng->gen : nullptr, out);
 }
 
 extern "C" int b2g_split_density_matrix_multi(
@@ -1431,8 +1463,8 @@ extern "C" int b2g_split_density_matrix_multi(
     int n_roots, const double *weights, int k, double cutoff, int trace_right,
     int normalize, unsigned trunc_type, double scale, b2g_split *out) {
     return split_impl(ctx, symmetry, wfn, psi, n_roots, weights, k, cutoff,
-                      trace_right, normalize, trunc_type, scale, 0.0, 0, 0u,
-                      out);
+                      trace_right, normalize, trunc_type, scale, 0.0, 0,
+                      nullptr, out);
 }
 
 extern "C" int b2g_split_n_roots(b2g_split s, int *n) {

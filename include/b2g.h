@@ -526,6 +526,13 @@ int b2g_split_density_matrix_multi(b2g_ctx ctx, int symmetry,
                                    int normalize, unsigned trunc_type,
                                    double scale, b2g_split *out);
 int b2g_split_n_roots(b2g_split s, int *n);
+/* block2's random generator as a handle (std::mt19937 seeded like
+ * Random::rand_seed(seed), src/core/utils.hpp:196-252; seed 0, a clock
+ * seed in block2, is refused).  Every draw continues its stream, as
+ * block2's one generator continues across the sites of a sweep.          */
+typedef struct b2g_rng_s *b2g_rng;
+int b2g_rng_create(unsigned seed, b2g_rng *out);
+int b2g_rng_destroy(b2g_rng rng);
 /* With noise (MovingEnvironment::density_matrix with noise > 0 and
  * NoiseTypes Wavefunction or DensityMatrix, src/dmrg/moving_environment.hpp:
  * 3554-3577, through OperatorFunctions::trans_product,
@@ -550,6 +557,17 @@ int b2g_split_density_matrix_noise(b2g_ctx ctx, int symmetry,
                                    double cutoff, int trace_right,
                                    int normalize, unsigned trunc_type,
                                    double scale, b2g_split *out);
+/* The same, drawing the noise from `rng` (b2g_rng_create): a sweep that
+ * seeds one generator before its first site reproduces block2's noise at
+ * every site.                                                            */
+int b2g_split_density_matrix_noise_rng(b2g_ctx ctx, int symmetry,
+                                       const b2g_sparse_info *wfn,
+                                       const double *psi, int n_roots,
+                                       const double *weights, double noise,
+                                       int noise_type, b2g_rng rng, int k,
+                                       double cutoff, int trace_right,
+                                       int normalize, unsigned trunc_type,
+                                       double scale, b2g_split *out);
 /* Result parts: which 0 = left factor, 1 = right factor, 2 = the density
  * matrix (before diagonalisation).  The info arrays are owned by the
  * split; data is device memory owned by the split (the wavefunction

@@ -328,7 +328,60 @@ template <typename S> struct Dumper : CallbackKernel {
         string d = ss.str();
         mkdir(d.c_str(), 0755);
         dump(h_eff, d, isite);
+        if (G.save_sweep)
+            dump_environment(h_eff, d, isite);
         ndumped++;
+    }
+    // The environment step around a dumped site, in block2's own formats
+    // (OperatorTensor::save_data, src/core/operator_tensor.hpp:122-148;
+    // SparseMatrix::save_data with its info): the left / right environments
+    // the effective Hamiltonian was blocked from (envs[center]->left /
+    // right, src/dmrg/moving_environment.hpp:1745-2068), the blocked
+    // operators of the effective Hamiltonian (lopt, ropt) and the MPS
+    // tensors left of the center and right of the two sites (the rotations
+    // that formed the environments, left_contract_rotate :226-,
+    // right_contract_rotate :460-).
+    void dump_environment(const shared_ptr<EffectiveHamiltonian<S, double>> &h,
+                          const string &d, int isite) const {
+        auto me = dmrg->me;
+        auto save_t = [](const shared_ptr<OperatorTensor<S, double>> &t,
+                         const string &f) {
+            if (t == nullptr)
+                return;
+            ofstream ofs(f, ios::binary);
+            t->save_data(ofs);
+        };
+        // the environments live in partition files loaded into frame 1 when
+        // used (the right one's load replaced the left one's): reload each
+        // before writing it
+        if (me->envs[isite]->left != nullptr) {
+            if (isite != 0)
+                frame_<double>()->load_data(
+                    1, me->get_left_partition_filename(isite));
+            save_t(me->envs[isite]->left, d + "/lenv.bin");
+        }
+        if (me->envs[isite]->right != nullptr) {
+            frame_<double>()->load_data(
+                1, me->get_right_partition_filename(isite));
+            save_t(me->envs[isite]->right, d + "/renv.bin");
+        }
+        frame_<double>()->activate(0);
+        save_t(h->op->lopt, d + "/lblk.bin");
+        save_t(h->op->ropt, d + "/rblk.bin");
+        if (isite > 0) {
+            me->ket->load_tensor(isite - 1);
+            me->ket->tensors[isite - 1]->save_data(d + "/mps_left.bin", true);
+            me->ket->unload_tensor(isite - 1);
+        }
+        if (isite + 2 < me->n_sites) {
+            me->ket->load_tensor(isite + 2);
+            me->ket->tensors[isite + 2]->save_data(d + "/mps_right.bin",
+                                                   true);
+            me->ket->unload_tensor(isite + 2);
+        }
+        ofstream m(d + "/env.txt");
+        m << isite << " " << (me->envs[isite]->left != nullptr) << " "
+          << (me->envs[isite]->right != nullptr) << "\n";
     }
     // Density-matrix truncation of the Davidson vector by the reference
     // (MovingEnvironment::density_matrix with noise 0 and
