@@ -1,0 +1,94 @@
+"""Sweep fixtures: block2's own sweep inputs and results.
+
+Runs the reference harness oracle/_ref/b2ref (built by `make -C oracle
+ref`; needs the reference sources, so it runs in the build container only)
+and copies into tests/golden/sweep_<case>/:
+
+  mpo.bin, mps_info.bin, mps_<i>.bin, mps.txt   the MPO and the starting MPS
+      in block2's file formats (MPO::save_data, MPSInfo::save_data,
+      SparseMatrix::save_data with info), written before the first sweep
+  steps.npy     per sweep step: sweep, step, Davidson energy, discarded
+                weight (block2's DMRG::sweep, a run without dumps so that
+                the noise stream is block2's own)
+  settings.txt  bond dimensions, noises, Davidson thresholds, tol, seeds
+  env_s<sweep>_<site>/   from separate runs with dumps at those sites:
+      lenv.bin / renv.bin   the environments the effective Hamiltonian was
+                            blocked from (OperatorTensor::save_data)
+      lblk.bin / rblk.bin   the blocked operators of the effective
+                            Hamiltonian
+      mps_left.bin / mps_right.bin   the MPS tensors left of the center and
+                            right of the two sites
+
+The random initial MPS is seeded (--seed) and the generator re-seeded
+before the sweeps (--noise-seed), so block2's density-matrix noise can be
+reproduced.  Inputs: FCIDUMPs shipped in the reference's data/ directory.
+
+    python tests/golden/make_sweep_fixtures.py
+"""
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+REF = "/root/reference"
+B2REF = os.path.join(REPO, "oracle", "_ref", "b2ref")
+
+CASES = {
+    # name: fcidump, sym, bdims, noises, sweeps, dump runs (sweep, sites)
+    "hub8_sz": ("HUBBARD-L8.FCIDUMP", "sz", [100], [1e-6, 0], 6,
+                [(0, [0, 1, 2, 3]), (1, [6, 5, 4])]),
+    "hub8_su2": ("HUBBARD-L8.FCIDUMP", "su2", [100], [1e-6, 0], 6,
+                 [(0, [0, 1, 2, 3]), (1, [6, 5, 4])]),
+    "n2_su2": ("N2.STO3G.FCIDUMP", "su2", [60], [1e-6, 0], 6,
+               [(0, [0, 1, 2, 3, 4, 5]), (1, [8, 7, 6, 5, 4, 3])]),
+}
+FILES = ["lenv.bin", "renv.bin", "lblk.bin", "rblk.bin", "mps_left.bin",
+         "mps_right.bin"]
+
+
+def run(fcidump, sym, bdims, noises, nsweeps, out, dump=None):
+    cmd = [B2REF, "--fcidump", fcidump, "--sym", sym, "--out", out,
+           "--bdims", ",".join(map(str, bdims)),
+           "--noises", ",".join(map(str, noises)),
+           "--nsweeps", str(nsweeps), "--threads", "8", "--dav-conv", "1e-8",
+           "--save-sweep", "1", "--seed", "1234", "--noise-seed", "4321"]
+    if dump is not None:
+        cmd += ["--dump-sweep", str(dump[0]),
+                "--dump-sites", ",".join(map(str, dump[1]))]
+    subprocess.run(cmd, check=True, stdout=subprocess.DEVNULL,
+                   stderr=subprocess.DEVNULL)
+
+
+def main():
+    names = sys.argv[1:] or list(CASES)
+    for name in names:
+        fd, sym, bdims, noises, ns, dumps = CASES[name]
+        fcidump = os.path.join(REF, "data", fd)
+        dst = os.path.join(HERE, "sweep_" + name)
+        shutil.rmtree(dst, ignore_errors=True)
+        os.makedirs(dst)
+        with tempfile.TemporaryDirectory(prefix="b2g_sweep_") as tmp:
+            run(fcidump, sym, bdims, noises, ns, os.path.join(tmp, "clean"))
+            src = os.path.join(tmp, "clean", "sweep")
+            for f in sorted(os.listdir(src)):
+                shutil.copy(os.path.join(src, f), os.path.join(dst, f))
+            for k, dump in enumerate(dumps):
+                out = os.path.join(tmp, "dump%d" % k)
+                run(fcidump, sym, bdims, noises, ns, out, dump)
+                for site in dump[1]:
+                    d = os.path.join(out, "site%d_sw%d" % (site, dump[0]))
+                    e = os.path.join(dst, "env_s%d_%d" % (dump[0], site))
+                    os.makedirs(e)
+                    for f in FILES:
+                        if os.path.exists(os.path.join(d, f)):
+                            shutil.copy(os.path.join(d, f),
+                                        os.path.join(e, f))
+        print(name, "->", dst)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,103 @@
+"""Shared helpers of the environment and sweep tests: block2's dumped
+operator tensors (OperatorTensor::save_data files in the sweep fixtures)
+as oracle operators, and the contraction strings of an MPO site."""
+import os
+
+import numpy as np
+
+from conftest import REPO
+
+from b2g import effprob as EP
+from b2g import mpo as M
+
+GOLDEN = os.path.join(REPO, "tests", "golden")
+SWEEP_CASES = {"hub8_sz": 0, "hub8_su2": 1, "n2_su2": 1}
+
+
+def case_dir(case):
+    return os.path.join(GOLDEN, "sweep_" + case)
+
+
+def key(e):
+    return (e[1], e[2], e[4])
+
+
+def load_ops(path):
+    """{key: oracle operator} of a dumped OperatorTensor, in file order."""
+    t = EP.load_operator_tensor(path)
+    out = {}
+    for k, m in t["ops"]:
+        inf = m["info"]
+        q = [int(x) for x in inf["quanta"]]
+        nb = [int(x) for x in inf["nbra"]]
+        nk = [int(x) for x in inf["nket"]]
+        sh = [int(x) for x in inf["shift"]]
+        out[key(k)] = dict(dq=int(inf["dq"]), factor=float(m["factor"]),
+                           info=dict(quanta=q, nbra=nb, nket=nk, shift=sh,
+                                     dq=int(inf["dq"]),
+                                     total=sum(a * b for a, b in zip(nb, nk))),
+                           data=(None if m["data"] is None else
+                                 np.asarray(m["data"], np.float64)))
+    return out
+
+
+def load_rot(path):
+    m = M.load_sparse_matrix(path)
+    i = m["info"]
+    return dict(dq=int(i["dq"]),
+                info=dict(quanta=[int(x) for x in i["quanta"]],
+                          nbra=[int(x) for x in i["nbra"]],
+                          nket=[int(x) for x in i["nket"]],
+                          shift=[int(x) for x in i["shift"]], dq=int(i["dq"])),
+                data=m["data"] * m["factor"])
+
+
+def basis_of(ops, env_oracle, sym, bra=True):
+    """Row (bra) or column basis of an operator set, from its blocks."""
+    qs = {}
+    for op in ops.values():
+        inf = op["info"]
+        for q, nb, nk in zip(inf["quanta"], inf["nbra"], inf["nket"]):
+            if bra:
+                qs[env_oracle.bra_of(sym, q, op["dq"])] = nb
+            else:
+                qs[env_oracle.ket_of(sym, q)] = nk
+    k = sorted(qs)
+    return (k, [qs[x] for x in k])
+
+
+def strings(e):
+    """(factor, conj, a, b) of a sum of products (None for other forms)."""
+    if e[0] == "exprref":
+        e = e[2]
+    if e[0] == "zero":
+        return []
+    if e[0] == "prod":
+        return [(e[1], e[2], e[3], e[4])]
+    if e[0] == "sum" and all(x[0] == "prod" for x in e[1]):
+        return [(x[1], x[2], x[3], x[4]) for x in e[1]]
+    return None
+
+
+def contraction_terms(mpo, site, left, env_keys, site_keys, blk_keys):
+    """Terms (c, a, b, conj, factor) of MPO site `site`'s left (or right)
+    contraction, indices into the given key lists; None if the expressions
+    hold strings other than products."""
+    names = mpo["left_operator_names" if left else
+                "right_operator_names"][site]["data"]
+    exprs = mpo["left_operator_exprs" if left else
+                "right_operator_exprs"][site]["data"]
+    terms = []
+    for j, nm in enumerate(names):
+        st = strings(exprs[j])
+        if st is None:
+            return None
+        c = blk_keys.index(key(nm))
+        for f, cj, a, b in st:
+            if left:
+                terms.append((c, env_keys.index(key(a)),
+                              site_keys.index(key(b)), cj, f / nm[3]))
+            else:
+                terms.append((c, site_keys.index(key(a)),
+                              env_keys.index(key(b)), cj, f / nm[3]))
+    return terms
