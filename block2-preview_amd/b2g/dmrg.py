@@ -138,7 +138,8 @@ class OpTensor:
                              self.info_of[k], 0, self.off[k])
               for k in range(len(self.keys))])
         keep += [inf, ops]
-        return len(self.infos), dq, inf, len(self.keys), ops
+        return (len(self.infos), L.ptr(dq, C.c_uint64), inf, len(self.keys),
+                ops)
 
     def subset(self, keys):
         """The operators `keys` of this tensor (same data, new index)."""
@@ -199,7 +200,8 @@ class SiteOps:
             *[L.b2g_operator(self.dq[k], self.factor[k], self.info_of[k], 0,
                              self.off[k]) for k in range(len(self.keys))])
         keep += [inf, ops]
-        return len(self.infos), dq, inf, len(self.keys), ops
+        return (len(self.infos), L.ptr(dq, C.c_uint64), inf, len(self.keys),
+                ops)
 
     def block_of(self, k, q):
         """host copy of site operator k's block with composite quanta q"""
@@ -347,17 +349,15 @@ class DMRG:
         e.data.fill_(1.0)
         return e
 
-    def _names(self, sym_entry):
-        return [x for x in sym_entry["data"]]
-
     def left_block(self, i, env):
         """Operators of (left environment of site i) (x) site i on the fused
         basis (MovingEnvironment::left_contract; site 0: left_assign)."""
         s, m = self.sym, self.mpo
         fused = self._fused_left(i)
-        names = [x for x in m["left_operator_names"][i]["data"]]
-        keys = [_key(x) for x in names]
-        Cs = OpTensor(s, fused, fused, keys, [x[4] for x in names], self.dev)
+        names = m["left_operator_names"][i]["data"]
+        live = [x for x in names if x[0] != "zero"]
+        Cs = OpTensor(s, fused, fused, [_key(x) for x in live],
+                      [x[4] for x in live], self.dev)
         site = self.site[i]
         vac = Q.StateInfo([self.vacuum], [1])
         if i == 0:
@@ -366,20 +366,23 @@ class DMRG:
             A = self._vacuum_env()
             lm = site.lmat["data"]
             terms = []
-            for j, x in enumerate(lm):
+            for x in lm:
                 if x[0] == "zero":
                     continue
-                terms.append((j, 0, site.index[_key(x)], 0, 1.0))
+                terms.append((Cs.index[_key(x)], 0, site.index[_key(x)], 0,
+                              1.0))
             self._run_contract(A, site, Cs, vac, vac, self.basis[i],
                                self.basis[i], terms)
             return Cs
         exprs = m["left_operator_exprs"][i]["data"]
         terms = []
-        for j, e in enumerate(exprs):
-            fn = names[j][3]
+        for nm, e in zip(names, exprs):
+            if nm[0] == "zero":
+                continue
+            c = Cs.index[_key(nm)]
             for f, cj, a, b in _products(e):
-                terms.append((j, env.index[_key(a)], site.index[_key(b)], cj,
-                              f / fn))
+                terms.append((c, env.index[_key(a)], site.index[_key(b)], cj,
+                              f / nm[3]))
         self._run_contract(env, site, Cs, env.bra, env.ket, self.basis[i],
                            self.basis[i], terms)
         return Cs
@@ -389,29 +392,33 @@ class DMRG:
         fused basis (right_contract; last site: right_assign)."""
         s, m = self.sym, self.mpo
         fused = self._fused_right(i)
-        names = [x for x in m["right_operator_names"][i]["data"]]
-        keys = [_key(x) for x in names]
-        Cs = OpTensor(s, fused, fused, keys, [x[4] for x in names], self.dev)
+        names = m["right_operator_names"][i]["data"]
+        live = [x for x in names if x[0] != "zero"]
+        Cs = OpTensor(s, fused, fused, [_key(x) for x in live],
+                      [x[4] for x in live], self.dev)
         site = self.site[i]
         vac = Q.StateInfo([self.vacuum], [1])
         if i == self.n - 1:
             B = self._vacuum_env()
             rm = site.rmat["data"]
             terms = []
-            for j, x in enumerate(rm):
+            for x in rm:
                 if x[0] == "zero":
                     continue
-                terms.append((j, site.index[_key(x)], 0, 0, 1.0))
+                terms.append((Cs.index[_key(x)], site.index[_key(x)], 0, 0,
+                              1.0))
             self._run_contract(site, B, Cs, self.basis[i], self.basis[i], vac,
                                vac, terms)
             return Cs
         exprs = m["right_operator_exprs"][i]["data"]
         terms = []
-        for j, e in enumerate(exprs):
-            fn = names[j][3]
+        for nm, e in zip(names, exprs):
+            if nm[0] == "zero":
+                continue
+            c = Cs.index[_key(nm)]
             for f, cj, a, b in _products(e):
-                terms.append((j, site.index[_key(a)], env.index[_key(b)], cj,
-                              f / fn))
+                terms.append((c, site.index[_key(a)], env.index[_key(b)], cj,
+                              f / nm[3]))
         self._run_contract(site, env, Cs, self.basis[i], self.basis[i],
                            env.bra, env.ket, terms)
         return Cs
@@ -434,8 +441,8 @@ class DMRG:
         try:
             ri = _cinfo(rot.lay, keep)
             L.check(self.lib.b2g_rotate_plan(
-                self.ctx.h, s, 1 if trans else 0, C.byref(ri), rbuf, na, L.ptr(adq, C.c_uint64),
-                ainf, nc, L.ptr(cdq, C.c_uint64), cinf, nop, aops, cops,
+                self.ctx.h, s, 1 if trans else 0, C.byref(ri), rbuf, na, adq,
+                ainf, nc, cdq, cinf, nop, aops, cops,
                 blk.total, out.total, 0, C.byref(plan)))
             torch.cuda.current_stream(self.dev).synchronize()
             L.check(self.lib.b2g_hpsi(self.ctx.h, plan, _dev(blk.data),
@@ -453,9 +460,11 @@ class DMRG:
         the operators `names` of the environment, new ones formed from
         existing ones by `exprs`."""
         s = self.sym
-        keys = [_key(x) for x in names["data"]]
+        live = [(x, e) for x, e in zip(names["data"], exprs["data"])
+                if x[0] != "zero"]
+        keys = [_key(x) for x, _ in live]
         new = [k for k in keys if k not in env.index]
-        dqs = {k: x[4] for k, x in zip(keys, names["data"])}
+        dqs = {k: x[4] for k, (x, _) in zip(keys, live)}
         if new:
             # an environment holding both sets (the new operators zero)
             allk = env.keys + new
@@ -469,11 +478,11 @@ class DMRG:
                     env.data[env.off[a]:env.off[a] + n]
             env = out
         terms = []
-        for j, e in enumerate(exprs["data"]):
+        for j, (nm, e) in enumerate(live):
             e = _unref(e)
             if e[0] == "zero":
                 continue
-            fn = names["data"][j][3]
+            fn = nm[3]
             if e[0] != "sum":
                 raise NotImplementedError("transform expression %s" % e[0])
             for x in e[1]:
@@ -489,7 +498,7 @@ class DMRG:
                   for (c, a, cj, f) in terms])
             h = C.c_void_p()
             L.check(self.lib.b2g_transform_plan(
-                self.ctx.h, s, ni, L.ptr(dq, C.c_uint64), inf, nop, ops,
+                self.ctx.h, s, ni, dq, inf, nop, ops,
                 len(terms), tm, 0, C.byref(h)))
             try:
                 torch.cuda.current_stream(self.dev).synchronize()
@@ -666,6 +675,8 @@ class DMRG:
             self.tensors[i], self.tensors[i + 1] = left, right
             if i != 0:
                 self.tensors[i] = self._swap(left, i, False)
+        self.bonds.append(self.left_dims[i + 1] if forward else
+                          self.right_dims[i + 1])
         return energy, sp.error, ndav, time.perf_counter() - t0
 
     def sweep(self, forward, bond_dim, noise, conv):
@@ -700,7 +711,7 @@ class DMRG:
                       * 0.1)
         self.davidson_conv_thrds = dc
         self.energies, self.discarded_weights, self.sweep_time = [], [], []
-        self.steps = []
+        self.steps, self.bonds = [], []
         self.init_environments()
         for iw in range(n_sweeps):
             self.isweep = iw

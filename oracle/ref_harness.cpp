@@ -118,6 +118,9 @@ struct Settings {
     // --noise-seed re-seeds it right before the sweeps (0: no re-seed)
     bool save_sweep = false;
     unsigned seed = 0, noise_seed = 0;
+    // --dav-thrd: one Davidson threshold for every sweep (0: block2's
+    // default, derived from the noises)
+    double dav_thrd = 0.0;
 };
 static Settings G;
 
@@ -304,7 +307,8 @@ template <typename S> struct Dumper : CallbackKernel {
     DMRG<S, double, double> *dmrg = nullptr;
     mutable int ndumped = 0;
     // one record per sweep step (DMRG::sweep::iter.end): sweep, step,
-    // the step's Davidson energy and discarded weight
+    // the step's Davidson energy, discarded weight, the generator's next
+    // output, and the bond basis the step left (total, checksum)
     mutable vector<double> steps;
     void compute(const string &name, int iprint) const override {
         if (name == "DMRG::sweep::iter.end" && dmrg != nullptr &&
@@ -313,6 +317,33 @@ template <typename S> struct Dumper : CallbackKernel {
             steps.push_back((double)(dmrg->sweep_energies.size() - 1));
             steps.push_back((double)dmrg->sweep_energies.back().back());
             steps.push_back((double)dmrg->sweep_discarded_weights.back());
+            // the generator's next output, read from a copy (the stream
+            // itself is left untouched): locates block2's position in its
+            // noise stream after the step
+            mt19937 peek = Random::rng();
+            steps.push_back((double)peek());
+            // the bond basis the step left behind: total states and a
+            // checksum over (quantum, count)
+            // the step's site: sweeps alternate starting forward from the
+            // first site
+            auto mi = dmrg->me->ket->info;
+            const int st = (int)dmrg->sweep_energies.size() - 1;
+            const bool fw = (dmrg->isweep % 2) == 0;
+            const int ic = fw ? st : dmrg->me->n_sites - 2 - st;
+            shared_ptr<StateInfo<S>> bs;
+            if (fw) {
+                mi->load_left_dims(ic + 1);
+                bs = mi->left_dims[ic + 1];
+            } else {
+                mi->load_right_dims(ic + 1);
+                bs = mi->right_dims[ic + 1];
+            }
+            double cks = 0;
+            for (int q = 0; q < bs->n; q++)
+                cks += (double)(bs->quanta[q].data % 1000003ULL) *
+                       bs->n_states[q];
+            steps.push_back((double)bs->n_states_total);
+            steps.push_back(cks);
             return;
         }
         if (name != "DMRG::sweep::iter.eff_ham" || dmrg == nullptr)
@@ -389,7 +420,11 @@ template <typename S> struct Dumper : CallbackKernel {
     // src/dmrg/sweep_algorithm.hpp:430-433), in three cases:
     //   sR: trace_right (forward), Physical, k = 2/3 of all states;
     //   sL: trace_left (backward), Physical, k = 1/2;
-    //   sK: trace_left, Reduced | KeepOne * 1, k = 1/3.
+    //   sK: trace_left, Reduced | KeepOne * 1, k = 1/3;
+    //   sW: trace_right, wavefunction noise 1e-6, k = 2/3;
+    //   sD: trace_left, density-matrix noise 1e-6, k = 2/3;
+    //   sE / sF: trace_right / trace_left, density-matrix noise 1e-6,
+    //       every state kept (k = all).
     // Written per case: the density matrix (info + data, before
     // diagonalisation), the two factors (info + data), the error, the
     // wavefunction spectra and the settings.
@@ -404,7 +439,7 @@ template <typename S> struct Dumper : CallbackKernel {
             NoiseTypes nt;
         };
         // noisy cases draw from block2's generator freshly seeded with 4321
-        const Case cases[5] = {
+        const Case cases[7] = {
             {"sR", true, 2, 3, TruncationTypes::Physical, 0.0, NoiseTypes::None},
             {"sL", false, 1, 2, TruncationTypes::Physical, 0.0, NoiseTypes::None},
             {"sK", false, 1, 3,
@@ -413,6 +448,13 @@ template <typename S> struct Dumper : CallbackKernel {
             {"sW", true, 2, 3, TruncationTypes::Physical, 1E-6,
              NoiseTypes::Wavefunction},
             {"sD", false, 2, 3, TruncationTypes::Physical, 1E-6,
+             NoiseTypes::DensityMatrix},
+            // density-matrix noise with every state kept (k = all), forward
+            // and backward: the states dropped are the ones the noise
+            // leaves below the cutoff
+            {"sE", true, 1, 1, TruncationTypes::Physical, 1E-6,
+             NoiseTypes::DensityMatrix},
+            {"sF", false, 1, 1, TruncationTypes::Physical, 1E-6,
              NoiseTypes::DensityMatrix}};
         const double cutoff = 1E-14;
         for (const Case &c : cases) {
@@ -875,6 +917,8 @@ template <typename S> static int run() {
     callback_() = dumper;
     if (G.noise_seed != 0)
         Random::rand_seed(G.noise_seed);
+    if (G.dav_thrd > 0)
+        dmrg->davidson_conv_thrds = vector<double>(G.nsweeps, G.dav_thrd);
     double energy = dmrg->solve(G.nsweeps, mps->center == 0, G.dav_conv * 0.1);
     callback_() = make_shared<CallbackKernel>();
     ofstream ofs(G.outdir + "/energies.txt");
@@ -1140,6 +1184,8 @@ int main(int argc, char **argv) {
             G.seed = (unsigned)stoul(v);
         else if (k == "--noise-seed")
             G.noise_seed = (unsigned)stoul(v);
+        else if (k == "--dav-thrd")
+            G.dav_thrd = stod(v);
         else {
             cerr << "unknown option " << k << endl;
             return 2;

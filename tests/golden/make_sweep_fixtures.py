@@ -7,9 +7,11 @@ and copies into tests/golden/sweep_<case>/:
   mpo.bin, mps_info.bin, mps_<i>.bin, mps.txt   the MPO and the starting MPS
       in block2's file formats (MPO::save_data, MPSInfo::save_data,
       SparseMatrix::save_data with info), written before the first sweep
-  steps.npy     per sweep step: sweep, step, Davidson energy, discarded
-                weight (block2's DMRG::sweep, a run without dumps so that
-                the noise stream is block2's own)
+  steps.npy     per sweep step (7 columns): sweep, step, Davidson energy,
+                discarded weight, the generator's next 32-bit output, the
+                new bond basis's total states and a checksum of its
+                (quantum, count) pairs (block2's DMRG::sweep, a run without
+                dumps so that the noise stream is block2's own)
   settings.txt  bond dimensions, noises, Davidson thresholds, tol, seeds
   env_s<sweep>_<site>/   from separate runs with dumps at those sites:
       lenv.bin / renv.bin   the environments the effective Hamiltonian was
@@ -18,6 +20,10 @@ and copies into tests/golden/sweep_<case>/:
                             Hamiltonian
       mps_left.bin / mps_right.bin   the MPS tensors left of the center and
                             right of the two sites
+      split.npz             block2's noisy truncations of the site's
+                            Davidson vector (harness cases sW, sD, sE, sF:
+                            density matrix, factors, error, settings) with
+                            the vector and its layout
 
 The random initial MPS is seeded (--seed) and the generator re-seeded
 before the sweeps (--noise-seed), so block2's density-matrix noise can be
@@ -27,6 +33,8 @@ reproduced.  Inputs: FCIDUMPs shipped in the reference's data/ directory.
 """
 import os
 import shutil
+
+import numpy as np
 import subprocess
 import sys
 import tempfile
@@ -42,6 +50,11 @@ CASES = {
                 [(0, [0, 1, 2, 3]), (1, [6, 5, 4])]),
     "hub8_su2": ("HUBBARD-L8.FCIDUMP", "su2", [100], [1e-6, 0], 6,
                  [(0, [0, 1, 2, 3]), (1, [6, 5, 4])]),
+    # the same sweeps without noise: every step is then fixed by the
+    # Davidson threshold alone (with noise, the noise is drawn in a basis
+    # whose eigenvector signs are the eigensolver's own choice)
+    "hub8_sz_quiet": ("HUBBARD-L8.FCIDUMP", "sz", [100], [0], 4, []),
+    "hub8_su2_quiet": ("HUBBARD-L8.FCIDUMP", "su2", [100], [0], 4, []),
     "n2_su2": ("N2.STO3G.FCIDUMP", "su2", [60], [1e-6, 0], 6,
                [(0, [0, 1, 2, 3, 4, 5]), (1, [8, 7, 6, 5, 4, 3])]),
 }
@@ -49,12 +62,14 @@ FILES = ["lenv.bin", "renv.bin", "lblk.bin", "rblk.bin", "mps_left.bin",
          "mps_right.bin"]
 
 
-def run(fcidump, sym, bdims, noises, nsweeps, out, dump=None):
+def run(fcidump, sym, bdims, noises, nsweeps, out, dump=None, thrd=0.0):
     cmd = [B2REF, "--fcidump", fcidump, "--sym", sym, "--out", out,
            "--bdims", ",".join(map(str, bdims)),
            "--noises", ",".join(map(str, noises)),
            "--nsweeps", str(nsweeps), "--threads", "8", "--dav-conv", "1e-8",
            "--save-sweep", "1", "--seed", "1234", "--noise-seed", "4321"]
+    if thrd:
+        cmd += ["--dav-thrd", repr(thrd)]
     if dump is not None:
         cmd += ["--dump-sweep", str(dump[0]),
                 "--dump-sites", ",".join(map(str, dump[1]))]
@@ -65,19 +80,21 @@ def run(fcidump, sym, bdims, noises, nsweeps, out, dump=None):
 def main():
     names = sys.argv[1:] or list(CASES)
     for name in names:
-        fd, sym, bdims, noises, ns, dumps = CASES[name]
+        fd, sym, bdims, noises, ns, dumps = CASES[name][:6]
+        thrd = CASES[name][6] if len(CASES[name]) > 6 else 0.0
         fcidump = os.path.join(REF, "data", fd)
         dst = os.path.join(HERE, "sweep_" + name)
         shutil.rmtree(dst, ignore_errors=True)
         os.makedirs(dst)
         with tempfile.TemporaryDirectory(prefix="b2g_sweep_") as tmp:
-            run(fcidump, sym, bdims, noises, ns, os.path.join(tmp, "clean"))
+            run(fcidump, sym, bdims, noises, ns, os.path.join(tmp, "clean"),
+                thrd=thrd)
             src = os.path.join(tmp, "clean", "sweep")
             for f in sorted(os.listdir(src)):
                 shutil.copy(os.path.join(src, f), os.path.join(dst, f))
             for k, dump in enumerate(dumps):
                 out = os.path.join(tmp, "dump%d" % k)
-                run(fcidump, sym, bdims, noises, ns, out, dump)
+                run(fcidump, sym, bdims, noises, ns, out, dump, thrd)
                 for site in dump[1]:
                     d = os.path.join(out, "site%d_sw%d" % (site, dump[0]))
                     e = os.path.join(dst, "env_s%d_%d" % (dump[0], site))
@@ -86,6 +103,13 @@ def main():
                         if os.path.exists(os.path.join(d, f)):
                             shutil.copy(os.path.join(d, f),
                                         os.path.join(e, f))
+                    arrs = {}
+                    for f in sorted(os.listdir(d)):
+                        if f.endswith(".npy") and (
+                                f.startswith("ket_") or f == "psi_dav.npy" or
+                                f[:3] in ("sW_", "sD_", "sE_", "sF_")):
+                            arrs[f[:-4]] = np.load(os.path.join(d, f))
+                    np.savez_compressed(os.path.join(e, "split.npz"), **arrs)
         print(name, "->", dst)
     return 0
 

@@ -11,11 +11,41 @@ from b2g import effprob as EP
 from b2g import mpo as M
 
 GOLDEN = os.path.join(REPO, "tests", "golden")
-SWEEP_CASES = {"hub8_sz": 0, "hub8_su2": 1, "n2_su2": 1}
+SWEEP_CASES = {"hub8_sz": 0, "hub8_su2": 1, "hub8_sz_quiet": 0,
+               "hub8_su2_quiet": 1, "n2_su2": 1}
+
+
+# the truncation cases dumped at the sweep sites (split.npz): wavefunction
+# noise trace_right (sW), density-matrix noise trace_left (sD), and
+# density-matrix noise with every state kept, trace_right (sE) and
+# trace_left (sF); noise 1e-6 from block2's generator seeded with 4321
+SWEEP_SPLIT_CASES = ["sW", "sD", "sE", "sF"]
 
 
 def case_dir(case):
     return os.path.join(GOLDEN, "sweep_" + case)
+
+
+def sweep_split_fixtures():
+    out = []
+    for case in SWEEP_CASES:
+        d = case_dir(case)
+        if not os.path.isdir(d):
+            continue
+        for e in sorted(os.listdir(d)):
+            if os.path.exists(os.path.join(d, e, "split.npz")):
+                out.append((case, e))
+    return out
+
+
+def load_sweep_split(case, env):
+    """(arrays, symmetry 0 = SZ / 1 = SU(2), wavefunction layout dict) of a
+    sweep site's truncation dump"""
+    z = np.load(os.path.join(case_dir(case), env, "split.npz"))
+    info = dict(quanta=z["ket_quanta"], nbra=z["ket_nbra"],
+                nket=z["ket_nket"], shift=z["ket_shift"],
+                dq=int(z["ket_dq"][0]))
+    return z, SWEEP_CASES[case], info
 
 
 def key(e):

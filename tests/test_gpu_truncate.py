@@ -10,7 +10,9 @@ to the oracle's signed ones to 1e-9.
 
 Split: on every golden fixture and truncation case (three without noise,
 two with block2's wavefunction / density-matrix noise drawn from its
-generator seeded with 4321), against the reference's own dumps (the oracle is pinned to the same dumps on the CPU):
+generator seeded with 4321), and at every site dumped along block2's sweeps
+(wavefunction noise, density-matrix noise, and density-matrix noise with
+every state kept in both directions), against the reference's own dumps (the oracle is pinned to the same dumps on the CPU):
 density matrix to 1e-13, kept states per sector exactly (up to states
 whose eigenvalue lies within 1e-13 of the cutoff, where FP64 eigensolvers
 may fall on either side; then the structure and reconstruction
@@ -39,6 +41,8 @@ import truncate_oracle as T
 from conftest import FIXTURES, DELAYED_FIXTURES, fixture_path
 
 from b2g import truncate as TR
+from sweep_common import (SWEEP_SPLIT_CASES, load_sweep_split,
+                          sweep_split_fixtures)
 
 pytestmark = pytest.mark.gpu
 
@@ -187,6 +191,19 @@ def test_split_matches_reference(gpu_ctx, name, tag):
     z, sym, info = load(name)
     if tag + "_settings" not in z.files:
         pytest.skip("fixture without truncation dump")
+    check_device_split(gpu_ctx, z, sym, info, name, tag)
+
+
+@pytest.mark.parametrize("case,env", sweep_split_fixtures())
+@pytest.mark.parametrize("tag", SWEEP_SPLIT_CASES)
+def test_split_matches_sweep_dumps(gpu_ctx, case, env, tag):
+    z, sym, inf = load_sweep_split(case, env)
+    info = TR.BlockInfo(inf["quanta"], inf["nbra"], inf["nket"],
+                        inf["shift"], inf["dq"])
+    check_device_split(gpu_ctx, z, sym, info, case + "/" + env, tag)
+
+
+def check_device_split(gpu_ctx, z, sym, info, name, tag):
     err_ref, k, cutoff, tt, tr, noise, nt, seed = settings(z, tag)
     print("case", name, tag, flush=True)
     s = TR.split_density_matrix(gpu_ctx, sym, info, dev(z["psi_dav"]),

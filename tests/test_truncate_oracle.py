@@ -6,9 +6,11 @@ results (oracle/ref_harness.cpp dump_split) in five cases: trace_right
 Physical (sR), trace_left Physical (sL), trace_left Reduced | KeepOne (sK),
 and with noise 1e-6 drawn from block2's generator seeded with 4321:
 wavefunction noise, trace_right (sW), density-matrix noise, trace_left
-(sD).  The noisy density matrices pin the oracle's restatement of block2's
-random stream (std::mt19937 through uniform_real_distribution) and of where
-the noise enters.
+(sD).  The sweep fixtures add, at the sites dumped along block2's sweeps,
+density-matrix noise with every state kept, trace_right (sE) and
+trace_left (sF).  The noisy density matrices pin the oracle's restatement
+of block2's random stream (std::mt19937 through
+uniform_real_distribution) and of where the noise enters.
 Checked here, on the CPU:
   * density-matrix structure exact and data to 1e-13 (absolute; the
     density matrix of a normalised vector has trace 1);
@@ -34,6 +36,8 @@ import pytest
 
 import truncate_oracle as T
 from conftest import FIXTURES, DELAYED_FIXTURES, fixture_path
+from sweep_common import (SWEEP_SPLIT_CASES, load_sweep_split,
+                          sweep_split_fixtures)
 
 CASES = ["sR", "sL", "sK", "sW", "sD"]
 
@@ -95,6 +99,20 @@ def test_oracle_split_matches_reference(name, tag):
     z, sym, info = load(name)
     if tag + "_settings" not in z.files:
         pytest.skip("fixture without truncation dump")
+    check_oracle_split(z, sym, info, name, tag)
+
+
+@pytest.mark.parametrize("case,env", sweep_split_fixtures())
+@pytest.mark.parametrize("tag", SWEEP_SPLIT_CASES)
+def test_oracle_split_matches_sweep_dumps(case, env, tag):
+    """The noisy truncations dumped along block2's sweeps (every state
+    kept, sE / sF: the states dropped are those the noise leaves below the
+    cutoff)."""
+    z, sym, info = load_sweep_split(case, env)
+    check_oracle_split(z, sym, info, case + "/" + env, tag)
+
+
+def check_oracle_split(z, sym, info, name, tag):
     err_ref, k, cutoff, tt, tr, noise, nt, seed = settings(z, tag)
     psi = z["psi_dav"]
     res = T.split_density_matrix(sym, info, psi, int(k), cutoff, bool(tr),
