@@ -39,6 +39,8 @@ import subprocess
 import sys
 import tempfile
 
+from make_fixtures import write_hubbard
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(os.path.dirname(HERE))
 REF = "/root/reference"
@@ -55,6 +57,11 @@ CASES = {
     # whose eigenvector signs are the eigensolver's own choice)
     "hub8_sz_quiet": ("HUBBARD-L8.FCIDUMP", "sz", [100], [0], 4, []),
     "hub8_su2_quiet": ("HUBBARD-L8.FCIDUMP", "su2", [100], [0], 4, []),
+    # the L=16 chain at U=4, bond dimension 200, noise 1e-5 for four sweeps
+    # then none, ten sweeps (the FCIDUMP is written here: the shipped
+    # HUBBARD-L16 has U=2)
+    "hub16_sz": ("HUBBARD-L16-U4.FCIDUMP", "sz", [200],
+                 [1e-5, 1e-5, 1e-5, 1e-5, 0], 10, []),
     "n2_su2": ("N2.STO3G.FCIDUMP", "su2", [60], [1e-6, 0], 6,
                [(0, [0, 1, 2, 3, 4, 5]), (1, [8, 7, 6, 5, 4, 3])]),
 }
@@ -82,11 +89,15 @@ def main():
     for name in names:
         fd, sym, bdims, noises, ns, dumps = CASES[name][:6]
         thrd = CASES[name][6] if len(CASES[name]) > 6 else 0.0
-        fcidump = os.path.join(REF, "data", fd)
         dst = os.path.join(HERE, "sweep_" + name)
         shutil.rmtree(dst, ignore_errors=True)
         os.makedirs(dst)
         with tempfile.TemporaryDirectory(prefix="b2g_sweep_") as tmp:
+            if fd == "HUBBARD-L16-U4.FCIDUMP":
+                fcidump = os.path.join(tmp, fd)
+                write_hubbard(fcidump, 16, 4.0, 1.0)
+            else:
+                fcidump = os.path.join(REF, "data", fd)
             run(fcidump, sym, bdims, noises, ns, os.path.join(tmp, "clean"),
                 thrd=thrd)
             src = os.path.join(tmp, "clean", "sweep")
