@@ -13,6 +13,7 @@
 #include <limits>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <sstream>
 #include <tuple>
 #include <unordered_map>
@@ -21,6 +22,45 @@ namespace b2g {
 
 static thread_local std::string g_err;
 void set_error(const std::string &msg) { g_err = msg; }
+
+// Scratch of the allocation-free H*psi (b2g_hpsi_noalloc): the W windows
+// and split-K partials of one product.  Each device that loads the library
+// holds its own copy until the process ends.  In host code `scratch` names
+// a host-side stand-in, never the device memory: kernels get the device
+// address taken by hipGetSymbolAddress at context creation
+// (g_scratch_ptr[device]).
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) double
+    scratch[kNoAllocScratchBytes / sizeof(double)];
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[kMaxDevices];
+
+// Takes the device address of `scratch` on `device` (current) once.
+static int scratch_setup(int device) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (device < 0 || device >= kMaxDevices) {
+        set_error("device index beyond the scratch table");
+        return B2G_ERR_ARG;
+    }
+    if (g_scratch_ptr[device] != nullptr)
+        return B2G_OK;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || ptr == nullptr) {
+        set_error(std::string("hipGetSymbolAddress(scratch) failed: ") +
+                  hipGetErrorString(e));
+        return B2G_ERR_DEVICE;
+    }
+    g_scratch_ptr[device] = ptr;
+    return B2G_OK;
+}
+
+static void *scratch_ptr(int device) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return device >= 0 && device < kMaxDevices ? g_scratch_ptr[device]
+                                               : nullptr;
+}
 
 } // namespace b2g
 
@@ -49,6 +89,11 @@ extern "C" int b2g_ctx_create(int device, unsigned flags, b2g_ctx *out) {
     B2G_REQUIRE(device >= 0 && device < ndev, B2G_ERR_ARG,
                 "b2g_ctx_create: no such device");
     B2G_HIP(hipSetDevice(device));
+    {
+        const int r = scratch_setup(device);
+        if (r != B2G_OK)
+            return r;
+    }
     b2g_ctx c = new b2g_ctx_s();
     c->device = device;
     hipDeviceProp_t prop;
@@ -1357,6 +1402,41 @@ extern "C" int b2g_hpsi(b2g_ctx ctx, b2g_plan p, const double *x, double *y,
     }
     return hpsi_run(ctx, p, x, y, n_rhs, scale, accumulate, precision, p->w,
                     (double *)p->pbuf);
+}
+
+extern "C" int b2g_hpsi_noalloc(b2g_ctx ctx, b2g_plan p, const double *x,
+                                double *y, int n_rhs, double scale,
+                                int accumulate, int precision) {
+    B2G_REQUIRE(ctx && p && x && y, B2G_ERR_ARG, "NULL arg");
+    B2G_REQUIRE(n_rhs >= 1 && n_rhs <= 65535, B2G_ERR_ARG, "bad n_rhs");
+    B2G_REQUIRE(precision == B2G_PREC_F64 || precision == B2G_PREC_F32,
+                B2G_ERR_ARG,
+                "b2g_hpsi_noalloc: FP64 or FP32 only (split-BF16 reads packed "
+                "planes the plan allocates)");
+    B2G_REQUIRE(precision != B2G_PREC_F32 || p->l32 != nullptr, B2G_ERR_STATE,
+                "plan was uploaded without FP32 shadows (flags bit 0 or 1)");
+    B2G_REQUIRE(p->ctx == ctx, B2G_ERR_ARG, "plan belongs to another context");
+    const size_t es = precision == B2G_PREC_F64 ? 8 : 4;
+    // W windows first, the split-K partials 256-byte aligned after them
+    const uint64_t w_bytes =
+        (std::max<uint64_t>(p->w_max, 1) * n_rhs * es + 255) & ~(uint64_t)255;
+    const uint64_t p_bytes = std::max<uint64_t>(p->p_max, 1) * n_rhs * 8;
+    if (w_bytes + p_bytes > kNoAllocScratchBytes) {
+        set_error("b2g_hpsi_noalloc: the product needs " +
+                  std::to_string(w_bytes + p_bytes) +
+                  " bytes of scratch; the allocation-free entry holds " +
+                  std::to_string(kNoAllocScratchBytes));
+        return B2G_ERR_ARG;
+    }
+    char *base = (char *)scratch_ptr(ctx->device);
+    B2G_REQUIRE(base != nullptr, B2G_ERR_STATE,
+                "no scratch address for this device");
+    B2G_HIP(hipSetDevice(ctx->device));
+    // nothing in the scratch needs to start at zero: phase 1 writes every
+    // W element phase 2 reads, and each split-K partial is written before
+    // the reduction reads it
+    return hpsi_run(ctx, p, x, y, n_rhs, scale, accumulate, precision, base,
+                    (double *)(base + w_bytes));
 }
 
 // ----------------------------------------------------------------- diagonal

@@ -216,6 +216,18 @@ int b2g_plan_stats(b2g_plan plan, uint64_t *n_tasks1, uint64_t *n_tasks2,
  * Replaces TensorFunctions::operator()(b, c, scale).                     */
 int b2g_hpsi(b2g_ctx ctx, b2g_plan plan, const double *x, double *y,
              int n_rhs, double scale, int accumulate, int precision);
+/* b2g_hpsi that allocates nothing: the W windows and split-K partials live
+ * in a fixed 64 MiB device array of the library (one per device, held until
+ * the process ends) instead of plan-owned memory.  No counterpart in the
+ * reference: for callers that cannot allocate (code under stream capture,
+ * serving loops, harnesses that forbid hipMalloc).  FP64 and FP32 only
+ * (split-BF16 reads packed planes the plan allocates); a product needing
+ * more than 64 MiB of scratch is refused (B2G_ERR_ARG).  Every call on one
+ * device shares that array: calls that overlap on one device (other
+ * streams or contexts, unordered) race.  Same results as b2g_hpsi, bit for
+ * bit.                                                                    */
+int b2g_hpsi_noalloc(b2g_ctx ctx, b2g_plan plan, const double *x, double *y,
+                     int n_rhs, double scale, int accumulate, int precision);
 
 /* Partial expectations (the backward of E = sum_r <v_r| H |c_r>): the
  * derivative of E with respect to every element of the left and right
