@@ -110,6 +110,110 @@ def cpu_reference(pb, ldata, rdata, y_gpu_fn, threads):
                 gpu_fp64_vs_block2_rel=rel)
 
 
+C1_DIR = os.path.join(REPO, "tests", "golden", "sweep_hub16_sz")
+
+
+def _hubbard_fcidump(path, n, u, t):
+    """FCIDUMP of the open 1D Hubbard chain (hopping -t, on-site U)."""
+    with open(path, "w") as f:
+        f.write(" &FCI NORB=%3d,NELEC=%3d,MS2=  0,\n  ORBSYM=%s\n  ISYM=1,\n"
+                " /\n" % (n, n, "1," * n))
+        for i in range(n):
+            f.write(" %22.16f %3d %3d %3d %3d\n" % (u, i + 1, i + 1, i + 1,
+                                                    i + 1))
+        for i in range(n - 1):
+            f.write(" %22.16f %3d %3d %3d %3d\n" % (-t, i + 2, i + 1, 0, 0))
+        f.write(" %22.16f %3d %3d %3d %3d\n" % (0.0, 0, 0, 0, 0))
+
+
+def sweep_leg(ctx, threads, run_cpu):
+    """Whole two-site sweeps of the L=16, U=4 Hubbard chain (SZ, M=200,
+    noise 1e-5 for four sweeps then none, Davidson thresholds as block2
+    sets them) from block2's own MPO and starting MPS, through the sweep
+    driver on the device: per-sweep seconds and the final energy against
+    block2's.  Beside it, block2 itself (oracle/_ref/b2ref, MKL, `threads`
+    OpenMP threads) runs the same sweeps from the same seeded MPS on the
+    host."""
+    import numpy as np
+    from b2g import dmrg as D
+    st = {}
+    with open(os.path.join(C1_DIR, "settings.txt")) as f:
+        for line in f:
+            p = line.split()
+            st[p[0]] = p[1:]
+    args = (int(st["nsweeps"][0]), True, float(st["tol"][0]),
+            [float(x) for x in st["dav_conv"]])
+
+    def make():
+        return D.DMRG(ctx, "sz", os.path.join(C1_DIR, "mpo.bin"), C1_DIR,
+                      [int(x) for x in st["bdims"]],
+                      [float(x) for x in st["noises"]],
+                      noise_type=st["noise_type"][0],
+                      noise_seed=int(st["noise_seed"][0]))
+    # a first solve loads the code objects and grows the caching allocator;
+    # the second, from the same starting MPS, is the measured one
+    make().solve(*args)
+    dm = make()
+    t0 = time.perf_counter()
+    e = dm.solve(*args)
+    wall = time.perf_counter() - t0
+    ref = float(st["final"][0])
+    steps = np.array([s[5] for s in dm.steps])
+    out = dict(workload="two-site DMRG sweeps, 1D Hubbard L=16, U=4, t=1, "
+                        "half filling, SZ, M=200, noise 1e-5 x4 then 0, "
+                        "block2's MPO and starting MPS",
+               sweeps=len(dm.sweep_time),
+               s_per_sweep=[round(t, 4) for t in dm.sweep_time],
+               mean_s_per_sweep=round(float(np.mean(dm.sweep_time)), 4),
+               total_s=round(wall, 4), final_energy=e,
+               block2_final_energy=ref, abs_dE_vs_block2=abs(e - ref),
+               step_s_mean=round(float(steps.mean()), 5),
+               step_s_max=round(float(steps.max()), 5))
+    log("sweeps (L=16 Hubbard, M=200): %d sweeps, %.3f s/sweep mean, "
+        "E %.12f (block2 %.12f)" % (out["sweeps"], out["mean_s_per_sweep"],
+                                     e, ref))
+    if run_cpu:
+        import shutil
+        import subprocess
+        import tempfile
+        b2ref = os.path.join(REPO, "oracle", "_ref", "b2ref")
+        if os.path.exists(b2ref):
+            d = tempfile.mkdtemp(prefix="b2g_c1_")
+            try:
+                fd = os.path.join(d, "H16.FCIDUMP")
+                _hubbard_fcidump(fd, 16, 4.0, 1.0)
+                cmd = [b2ref, "--fcidump", fd, "--sym", "sz", "--out",
+                       os.path.join(d, "out"), "--bdims", st["bdims"][0],
+                       "--noises", ",".join(st["noises"]), "--nsweeps",
+                       st["nsweeps"][0], "--threads", str(threads),
+                       "--dav-conv", "1e-8", "--seed", "1234",
+                       "--noise-seed", st["noise_seed"][0]]
+                t0 = time.perf_counter()
+                subprocess.run(cmd, check=True, stdout=subprocess.DEVNULL,
+                               stderr=subprocess.DEVNULL, timeout=600)
+                cwall = time.perf_counter() - t0
+                en = {}
+                with open(os.path.join(d, "out", "energies.txt")) as f:
+                    for line in f:
+                        p = line.split()
+                        en[p[0]] = p[1:]
+                ss = [float(x) for x in en.get("sweep_s", [])]
+                out["block2_cpu"] = dict(
+                    cores=threads, sweeps=len(ss),
+                    s_per_sweep=[round(t, 4) for t in ss],
+                    mean_s_per_sweep=round(float(np.mean(ss)), 4) if ss
+                    else None, process_s=round(cwall, 3),
+                    final_energy=float(en["final"][0]))
+                if ss:
+                    out["speedup_vs_block2_cpu"] = round(
+                        float(np.sum(ss)) / float(np.sum(dm.sweep_time)), 3)
+                log("block2 on %d host cores: %s s/sweep" % (
+                    threads, out["block2_cpu"]["mean_s_per_sweep"]))
+            finally:
+                shutil.rmtree(d, ignore_errors=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -124,6 +228,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-davidson", action="store_true")
+    ap.add_argument("--no-sweep", action="store_true",
+                    help="skip the whole-sweep leg (L=16 Hubbard, M=200)")
     ap.add_argument("--rates", default=None,
                     help="9 comma-separated tile rates instead of autotuning "
                          "(profiling runs: keeps the tuner's launches out of "
@@ -500,6 +606,9 @@ def main():
         "cpu_baseline": cpu,
     }
     out.update(extra)
+    if not args.no_sweep:
+        out["sweep"] = sweep_leg(ctx, min(16, os.cpu_count() or 1),
+                                 not args.no_cpu)
     print(json.dumps(out), flush=True)
     h.deallocate()
     ctx.close()

@@ -30,6 +30,7 @@
 #include "b2g_block2.hpp"
 #endif
 
+#include <chrono>
 #include <cstdio>
 #include <fstream>
 #include <sstream>
@@ -310,7 +311,20 @@ template <typename S> struct Dumper : CallbackKernel {
     // the step's Davidson energy, discarded weight, the generator's next
     // output, and the bond basis the step left (total, checksum)
     mutable vector<double> steps;
+    // wall-clock seconds of each sweep (DMRG::sweep.start .. sweep.end)
+    mutable vector<double> sweep_s;
+    mutable chrono::steady_clock::time_point sweep_t0;
     void compute(const string &name, int iprint) const override {
+        if (name == "DMRG::sweep.start") {
+            sweep_t0 = chrono::steady_clock::now();
+            return;
+        }
+        if (name == "DMRG::sweep.end") {
+            sweep_s.push_back(chrono::duration<double>(
+                                  chrono::steady_clock::now() - sweep_t0)
+                                  .count());
+            return;
+        }
         if (name == "DMRG::sweep::iter.end" && dmrg != nullptr &&
             dmrg->sweep_energies.size() != 0) {
             steps.push_back((double)dmrg->isweep);
@@ -930,6 +944,10 @@ template <typename S> static int run() {
                 << (i < dmrg->sweep_time.size() ? dmrg->sweep_time[i] : 0.0)
                 << "\n";
     ofs << "dumped " << dumper->ndumped << "\n";
+    ofs << "sweep_s";
+    for (double t : dumper->sweep_s)
+        ofs << " " << t;
+    ofs << "\nthreads " << G.threads << "\n";
     if (G.save_sweep) {
         write_npy(G.outdir + "/sweep/steps.npy", dumper->steps);
         ofstream sf(G.outdir + "/sweep/settings.txt");
