@@ -57,23 +57,19 @@ def _unref(e):
     return e[2] if e[0] == "exprref" else e
 
 
-def _products(e):
-    """(factor, conj, a, b) of every string of an expression that is a sum
-    of two-operator products."""
+def _strings(e):
+    """The strings of an expression: products and SumProds."""
     e = _unref(e)
     if e[0] == "zero":
         return []
-    if e[0] == "prod":
-        return [(e[1], e[2], e[3], e[4])]
+    if e[0] in ("prod", "sumprod"):
+        return [e]
     if e[0] == "sum":
-        out = []
         for x in e[1]:
-            if x[0] != "prod":
-                raise NotImplementedError(
-                    "expression string of type %s (intermediate or delayed "
-                    "operators) is not supported" % x[0])
-            out.append((x[1], x[2], x[3], x[4]))
-        return out
+            if x[0] not in ("prod", "sumprod"):
+                raise NotImplementedError("expression string of type %s"
+                                          % x[0])
+        return list(e[1])
     raise NotImplementedError("expression of type %s" % e[0])
 
 
@@ -100,6 +96,22 @@ def _cinfo(lay, keep):
 
 def _dev(t):
     return C.c_void_p(t.data_ptr())
+
+
+def _copy_ops(src, dst):
+    """dst's copies of src's operators (same keys and shapes), one gather."""
+    si, di = [], []
+    for k in src.keys:
+        a, b = src.index[k], dst.index[k]
+        n = src.layout(a).total
+        if n:
+            si.append(np.arange(src.off[a], src.off[a] + n))
+            di.append(np.arange(dst.off[b], dst.off[b] + n))
+    if si:
+        dev = dst.data.device
+        s_ = torch.from_numpy(np.concatenate(si)).to(dev)
+        d_ = torch.from_numpy(np.concatenate(di)).to(dev)
+        dst.data.index_copy_(0, d_, src.data.index_select(0, s_))
 
 
 class OpTensor:
@@ -375,17 +387,63 @@ class DMRG:
                                self.basis[i], terms)
             return Cs
         exprs = m["left_operator_exprs"][i]["data"]
-        terms = []
+        env, terms = self._contraction_terms(env, site, Cs, names, exprs,
+                                             True)
+        self._run_contract(env, site, Cs, env.bra, env.ket, self.basis[i],
+                           self.basis[i], terms)
+        return Cs
+
+    def _contraction_terms(self, env, site, Cs, names, exprs, left):
+        """Window-sum terms (c, a, b, conj, factor) of a left (right)
+        contraction, environment (x) site (site (x) environment).  A
+        SumProd string multiplies a sum of environment operators: that sum
+        is formed first as an intermediate operator of the first summand's
+        shape (OperatorFunctions::iadd, as TensorFunctions::tensor_product
+        does, src/core/tensor_functions.hpp:2220-2271), appended to the
+        environment, which is returned with the terms."""
+        rows, inters = [], []
         for nm, e in zip(names, exprs):
             if nm[0] == "zero":
                 continue
             c = Cs.index[_key(nm)]
-            for f, cj, a, b in _products(e):
-                terms.append((c, env.index[_key(a)], site.index[_key(b)], cj,
-                              f / nm[3]))
-        self._run_contract(env, site, Cs, env.bra, env.ket, self.basis[i],
-                           self.basis[i], terms)
-        return Cs
+            for x in _strings(e):
+                f, cj = x[1], x[2]
+                if x[0] == "prod":
+                    ek = _key(x[3] if left else x[4])
+                    sk = _key(x[4] if left else x[3])
+                else:
+                    a, b, ops, conjs = x[3], x[4], x[6], x[7]
+                    if (a is None) != left or (b is None) == left:
+                        raise NotImplementedError(
+                            "SumProd summing site operators")
+                    sk = _key(b if left else a)
+                    ek = ("sum", len(inters))
+                    inters.append((ops[0], [(_key(o), int(bool(q)), o[3])
+                                            for o, q in zip(ops, conjs)]))
+                rows.append((c, ek, site.index[sk], cj, f / nm[3]))
+        if inters:
+            env = self._intermediates(env, inters)
+        terms = [(c, env.index[ek], sb, cj, f) if left else
+                 (c, sb, env.index[ek], cj, f)
+                 for c, ek, sb, cj, f in rows]
+        return env, terms
+
+    def _intermediates(self, env, inters):
+        """env with the intermediate operators appended: operator j =
+        sum of f op(env operator) over inters[j][1] (conj: transposed), in
+        the shape of the element inters[j][0] (its quantum)."""
+        keys = env.keys + [("sum", j) for j in range(len(inters))]
+        out = OpTensor(self.sym, env.bra, env.ket, keys,
+                       env.dq + [int(o[4]) for o, _ in inters], self.dev)
+        out.factor[:len(env.keys)] = env.factor
+        _copy_ops(env, out)
+        terms = []
+        for j, (_, parts) in enumerate(inters):
+            c = out.index[("sum", j)]
+            for k, cj, f in parts:
+                terms.append((c, env.index[k], cj, f))
+        self._run_transform(out, terms)
+        return out
 
     def right_block(self, i, env):
         """Operators of site i (x) (right environment of site i) on the
@@ -411,14 +469,8 @@ class DMRG:
                                vac, terms)
             return Cs
         exprs = m["right_operator_exprs"][i]["data"]
-        terms = []
-        for nm, e in zip(names, exprs):
-            if nm[0] == "zero":
-                continue
-            c = Cs.index[_key(nm)]
-            for f, cj, a, b in _products(e):
-                terms.append((c, site.index[_key(a)], env.index[_key(b)], cj,
-                              f / nm[3]))
+        env, terms = self._contraction_terms(env, site, Cs, names, exprs,
+                                             False)
         self._run_contract(site, env, Cs, self.basis[i], self.basis[i],
                            env.bra, env.ket, terms)
         return Cs
@@ -471,11 +523,7 @@ class DMRG:
             out = OpTensor(s, env.bra, env.ket, allk,
                            env.dq + [dqs[k] for k in new], self.dev)
             out.factor[:len(env.keys)] = env.factor
-            for k in env.keys:
-                a, b = env.index[k], out.index[k]
-                n = env.layout(a).total
-                out.data[out.off[b]:out.off[b] + n] = \
-                    env.data[env.off[a]:env.off[a] + n]
+            _copy_ops(env, out)
             env = out
         terms = []
         for j, (nm, e) in enumerate(live):
@@ -490,24 +538,30 @@ class DMRG:
                     raise NotImplementedError("transform string")
                 terms.append((env.index[keys[j]], env.index[_key(x[3])],
                               x[2] & 1, x[1] / fn))
-        if terms:
-            keep = []
-            ni, dq, inf, nop, ops = env.c_side(keep)
-            tm = (L.b2g_transform_term * len(terms))(
-                *[L.b2g_transform_term(c, a, cj, 0, f)
-                  for (c, a, cj, f) in terms])
-            h = C.c_void_p()
-            L.check(self.lib.b2g_transform_plan(
-                self.ctx.h, s, ni, dq, inf, nop, ops,
-                len(terms), tm, 0, C.byref(h)))
-            try:
-                torch.cuda.current_stream(self.dev).synchronize()
-                L.check(self.lib.b2g_winplan_run(h, _dev(env.data), None,
-                                                 _dev(env.data), 1))
-                self.ctx.synchronize()
-            finally:
-                self.lib.b2g_winplan_destroy(h)
+        self._run_transform(env, terms)
         return env.subset(keys)
+
+    def _run_transform(self, env, terms):
+        """env.c += f op(env.a) for the terms (c, a, conj, f), in place
+        (OperatorFunctions::iadd on the device)."""
+        if not terms:
+            return
+        keep = []
+        ni, dq, inf, nop, ops = env.c_side(keep)
+        tm = (L.b2g_transform_term * len(terms))(
+            *[L.b2g_transform_term(c, a, cj, 0, f)
+              for (c, a, cj, f) in terms])
+        h = C.c_void_p()
+        L.check(self.lib.b2g_transform_plan(
+            self.ctx.h, self.sym, ni, dq, inf, nop, ops,
+            len(terms), tm, 0, C.byref(h)))
+        try:
+            torch.cuda.current_stream(self.dev).synchronize()
+            L.check(self.lib.b2g_winplan_run(h, _dev(env.data), None,
+                                             _dev(env.data), 1))
+            self.ctx.synchronize()
+        finally:
+            self.lib.b2g_winplan_destroy(h)
 
     # ------------------------------------------------- wavefunction moves
     def _contract_wfn(self, lmat, l_is_wfn, rmat, out_lay):
@@ -588,10 +642,15 @@ class DMRG:
     # ------------------------------------------------------------- sweep
     def _eff_ham(self, i, lblk, rblk, lay):
         s, m = self.sym, self.mpo
-        strings = _products(m["middle_operator_exprs"][i]["data"][0])
+        strings = _strings(m["middle_operator_exprs"][i]["data"][0])
         fn = m["middle_operator_names"][i]["data"][0][3]
         conj, ta, tb, tf = [], [], [], []
-        for f, cj, a, b in strings:
+        for x in strings:
+            if x[0] != "prod":
+                # SumProd strings of the Hamiltonian belong to delayed
+                # operator tensors, which this driver does not form
+                raise NotImplementedError("Hamiltonian string %s" % x[0])
+            f, cj, a, b = x[1], x[2], x[3], x[4]
             conj.append(cj)
             ta.append(lblk.index[_key(a)])
             tb.append(rblk.index[_key(b)])

@@ -109,25 +109,50 @@ def strings(e):
     return None
 
 
+def _all_strings(e):
+    if e[0] == "exprref":
+        e = e[2]
+    if e[0] == "zero":
+        return []
+    if e[0] in ("prod", "sumprod"):
+        return [e]
+    if e[0] == "sum":
+        return list(e[1])
+    raise ValueError("expression %s" % e[0])
+
+
 def contraction_terms(mpo, site, left, env_keys, site_keys, blk_keys):
     """Terms (c, a, b, conj, factor) of MPO site `site`'s left (or right)
-    contraction, indices into the given key lists; None if the expressions
-    hold strings other than products."""
+    contraction, indices into the given key lists, and the intermediate
+    operators of its SumProd strings: [(dq, [(env index, conj, factor)])],
+    the sum of environment operators a SumProd string multiplies
+    (TensorFunctions::tensor_product, src/core/tensor_functions.hpp:
+    2220-2271); a term's environment index len(env_keys) + j names
+    intermediate j."""
     names = mpo["left_operator_names" if left else
                 "right_operator_names"][site]["data"]
     exprs = mpo["left_operator_exprs" if left else
                 "right_operator_exprs"][site]["data"]
-    terms = []
+    terms, inters = [], []
     for j, nm in enumerate(names):
-        st = strings(exprs[j])
-        if st is None:
-            return None
+        if nm[0] == "zero":
+            continue
         c = blk_keys.index(key(nm))
-        for f, cj, a, b in st:
-            if left:
-                terms.append((c, env_keys.index(key(a)),
-                              site_keys.index(key(b)), cj, f / nm[3]))
+        for x in _all_strings(exprs[j]):
+            f, cj = x[1], x[2]
+            if x[0] == "prod":
+                a, b = x[3], x[4]
+                ea = env_keys.index(key(a if left else b))
+                sb = site_keys.index(key(b if left else a))
             else:
-                terms.append((c, site_keys.index(key(a)),
-                              env_keys.index(key(b)), cj, f / nm[3]))
-    return terms
+                a, b, ops, conjs = x[3], x[4], x[6], x[7]
+                # the summed operators sit on the environment's side
+                assert (a is None) == left and (b is None) == (not left)
+                sb = site_keys.index(key(b if left else a))
+                ea = len(env_keys) + len(inters)
+                inters.append((ops[0][4], [(env_keys.index(key(o)), bool(cc),
+                                            o[3]) for o, cc in
+                                           zip(ops, conjs)]))
+            terms.append((c, ea, sb, cj, f / nm[3]) if left else
+                         (c, sb, ea, cj, f / nm[3]))
+    return terms, inters

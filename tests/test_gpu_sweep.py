@@ -41,9 +41,9 @@ from sweep_common import (SWEEP_CASES, basis_of, case_dir, key, load_ops,
 
 pytestmark = pytest.mark.gpu
 
-PLAIN = ["hub8_sz", "hub8_su2"]
+PLAIN = ["hub8_sz", "hub8_su2", "n2_su2"]
 QUIET = ["hub8_sz_quiet", "hub8_su2_quiet"]
-NOISY = ["hub8_sz", "hub8_su2", "hub16_sz"]
+NOISY = ["hub8_sz", "hub8_su2", "hub16_sz", "n2_su2"]
 # the L=16, U=4 Hubbard chain at bond dimension 200, ten sweeps: block2's
 # final energy of the same settings in its own fixture run
 C1_ENERGY = -8.818767451952382
