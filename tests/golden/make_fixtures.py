@@ -40,6 +40,50 @@ def write_hubbard(path, n, u, t):
         f.write(" %22.16f %3d %3d %3d %3d\n" % (0.0, 0, 0, 0, 0))
 
 
+def abinitio_integrals(n, seed):
+    """The seeded synthetic ab initio Hamiltonian: orbital energies e_i
+    sorted uniform in [-2, 1]; h = diag(e) + symmetric Gaussian
+    off-diagonals (sigma 0.05); (ij|kl) = sum_P B^P_ij B^P_kl over 3n
+    symmetric Gaussian B^P scaled so that E[(ii|ii)] = 0.5 (eight-fold
+    symmetric, positive semi-definite).  numpy default_rng(seed), drawn in
+    this order: e, the off-diagonals, the B^P."""
+    rng = np.random.default_rng(seed)
+    e = np.sort(rng.uniform(-2.0, 1.0, n))
+    g = np.triu(rng.normal(0.0, 0.05, (n, n)), 1)
+    h = np.diag(e) + g + g.T
+    npp = 3 * n
+    # B = (X + X^T) / 2 with X ~ N(0, s^2): diagonal variance s^2, so
+    # E[(ii|ii)] = 3n s^2 = 0.5
+    sd = np.sqrt(0.5 / npp)
+    x = rng.normal(0.0, sd, (npp, n, n))
+    b = 0.5 * (x + x.transpose(0, 2, 1))
+    eri = np.einsum("pij,pkl->ijkl", b, b)
+    return h, eri
+
+
+def write_abinitio(path, n, nelec, seed):
+    """FCIDUMP of abinitio_integrals(n, seed): singlet, no point group, the
+    unique integrals of the eight-fold symmetry (i >= j, k >= l, ij >= kl)
+    and the one-electron ones (i >= j), core energy 0."""
+    h, eri = abinitio_integrals(n, seed)
+    with open(path, "w") as f:
+        f.write(" &FCI NORB=%3d,NELEC=%3d,MS2=  0,\n  ORBSYM=%s\n  ISYM=1,\n"
+                " /\n" % (n, nelec, "1," * n))
+        for i in range(n):
+            for j in range(i + 1):
+                for k in range(n):
+                    for l in range(k + 1):
+                        if i * (i + 1) // 2 + j < k * (k + 1) // 2 + l:
+                            continue
+                        f.write(" %22.16e %3d %3d %3d %3d\n" % (
+                            eri[i, j, k, l], i + 1, j + 1, k + 1, l + 1))
+        for i in range(n):
+            for j in range(i + 1):
+                f.write(" %22.16e %3d %3d %3d %3d\n" % (h[i, j], i + 1,
+                                                        j + 1, 0, 0))
+        f.write(" %22.16e %3d %3d %3d %3d\n" % (0.0, 0, 0, 0, 0))
+
+
 def run(name, fcidump, sym, bdims, noises, nsweeps, dump_sweep, dump_site,
         work, dav_conv="1e-9", delayed=False, names=None):
     """One reference DMRG run; dumps site `dump_site` (an int, or a list of

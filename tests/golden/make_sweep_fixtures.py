@@ -39,7 +39,7 @@ import subprocess
 import sys
 import tempfile
 
-from make_fixtures import write_hubbard
+from make_fixtures import write_abinitio, write_hubbard
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(os.path.dirname(HERE))
@@ -62,6 +62,11 @@ CASES = {
     # HUBBARD-L16 has U=2)
     "hub16_sz": ("HUBBARD-L16-U4.FCIDUMP", "sz", [200],
                  [1e-5, 1e-5, 1e-5, 1e-5, 0], 10, []),
+    # the seeded synthetic ab initio Hamiltonian at n=26 orbitals, 26
+    # electrons, singlet (make_fixtures.write_abinitio, seed 2026), without
+    # noise: bond dimension 50, then 100, then 250
+    "c2_su2": ("C2-N26-SEED2026.FCIDUMP", "su2", [50, 100, 250], [0], 3,
+               []),
     "n2_su2": ("N2.STO3G.FCIDUMP", "su2", [60], [1e-6, 0], 6,
                [(0, [0, 1, 2, 3, 4, 5]), (1, [8, 7, 6, 5, 4, 3])]),
 }
@@ -96,6 +101,9 @@ def main():
             if fd == "HUBBARD-L16-U4.FCIDUMP":
                 fcidump = os.path.join(tmp, fd)
                 write_hubbard(fcidump, 16, 4.0, 1.0)
+            elif fd == "C2-N26-SEED2026.FCIDUMP":
+                fcidump = os.path.join(tmp, fd)
+                write_abinitio(fcidump, 26, 26, 2026)
             else:
                 fcidump = os.path.join(REF, "data", fd)
             run(fcidump, sym, bdims, noises, ns, os.path.join(tmp, "clean"),
