@@ -14,6 +14,7 @@ Serialisations followed:
 files: 4 with _LARGE_BOND (total_bond_t is then 64-bit), 2 by default, 1
 with _SMALL_BOND.  Only uncompressed storage is read.
 """
+import gzip
 import os
 
 import numpy as np
@@ -38,6 +39,16 @@ def _state_info(r):
     return dict(quanta=q, n_states=ns)
 
 
+def _read_file(path):
+    """The bytes of `path`, or of its gzip-compressed copy `path`.gz when
+    only that exists."""
+    if not os.path.exists(path) and os.path.exists(path + ".gz"):
+        with gzip.open(path + ".gz", "rb") as f:
+            return f.read()
+    with open(path, "rb") as f:
+        return f.read()
+
+
 def load_mpo(path, ubond=4):
     """An MPO written by MPO::save_data (uncompressed operators).
     Returns a dict: n_sites, tag, const_e, left_vacuum, sparse_form, op (the
@@ -46,8 +57,7 @@ def load_mpo(path, ubond=4):
     (lmat / rmat symbolics and the site operators with their data), basis[i]
     (StateInfo), left/right/middle operator names and exprs (lists of
     symbolics, empty where the MPO has none)."""
-    with open(path, "rb") as f:
-        r = _Reader(f.read(), ubond)
+    r = _Reader(_read_file(path), ubond)
     m = {}
     n = m["n_sites"] = r.i32()
     m["tag"] = bytes(r.take(r.i32())).decode()

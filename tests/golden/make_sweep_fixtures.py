@@ -4,8 +4,8 @@ Runs the reference harness oracle/_ref/b2ref (built by `make -C oracle
 ref`; needs the reference sources, so it runs in the build container only)
 and copies into tests/golden/sweep_<case>/:
 
-  mpo.bin, mps_info.bin, mps_<i>.bin, mps.txt   the MPO and the starting MPS
-      in block2's file formats (MPO::save_data, MPSInfo::save_data,
+  mpo.bin.gz, mps_info.bin, mps_<i>.bin, mps.txt   the MPO (gzip) and the
+      starting MPS in block2's file formats (MPO::save_data, MPSInfo::save_data,
       SparseMatrix::save_data with info), written before the first sweep
   steps.npy     per sweep step (7 columns): sweep, step, Davidson energy,
                 discarded weight, the generator's next 32-bit output, the
@@ -31,6 +31,7 @@ reproduced.  Inputs: FCIDUMPs shipped in the reference's data/ directory.
 
     python tests/golden/make_sweep_fixtures.py
 """
+import gzip
 import os
 import shutil
 
@@ -110,6 +111,13 @@ def main():
                 thrd=thrd)
             src = os.path.join(tmp, "clean", "sweep")
             for f in sorted(os.listdir(src)):
+                if f == "mpo.bin":
+                    # kept gzip-compressed (the readers take either form)
+                    with open(os.path.join(src, f), "rb") as fi:
+                        raw = fi.read()
+                    with open(os.path.join(dst, f + ".gz"), "wb") as fo:
+                        fo.write(gzip.compress(raw, 9, mtime=0))
+                    continue
                 shutil.copy(os.path.join(src, f), os.path.join(dst, f))
             for k, dump in enumerate(dumps):
                 out = os.path.join(tmp, "dump%d" % k)
