@@ -98,22 +98,6 @@ def _dev(t):
     return C.c_void_p(t.data_ptr())
 
 
-def _copy_ops(src, dst):
-    """dst's copies of src's operators (same keys and shapes), one gather."""
-    si, di = [], []
-    for k in src.keys:
-        a, b = src.index[k], dst.index[k]
-        n = src.layout(a).total
-        if n:
-            si.append(np.arange(src.off[a], src.off[a] + n))
-            di.append(np.arange(dst.off[b], dst.off[b] + n))
-    if si:
-        dev = dst.data.device
-        s_ = torch.from_numpy(np.concatenate(si)).to(dev)
-        d_ = torch.from_numpy(np.concatenate(di)).to(dev)
-        dst.data.index_copy_(0, d_, src.data.index_select(0, s_))
-
-
 class OpTensor:
     """The operators of one block (an OperatorTensor's ops): keys in order,
     one block layout per delta quantum (sorted), device FP64 data."""
@@ -152,6 +136,35 @@ class OpTensor:
         keep += [inf, ops]
         return (len(self.infos), L.ptr(dq, C.c_uint64), inf, len(self.keys),
                 ops)
+
+    def extend(self, keys, dqs):
+        """This tensor with zero operators `keys` (quanta `dqs`) appended
+        after its data; its own operators keep their offsets."""
+        r = object.__new__(OpTensor)
+        r.sym, r.bra, r.ket = self.sym, self.bra, self.ket
+        r.keys = self.keys + list(keys)
+        r.index = {k: i for i, k in enumerate(r.keys)}
+        r.dq = self.dq + [int(d) for d in dqs]
+        new = sorted(set(r.dq) - set(self.info_dq))
+        r.info_dq = sorted(set(self.info_dq) | set(new))
+        lay = {d: x for d, x in zip(self.info_dq, self.infos)}
+        for d in new:
+            lay[d] = Q.initialize(self.sym, self.bra, self.ket, d,
+                                  Q.q_is_fermion(d))
+        r.infos = [lay[d] for d in r.info_dq]
+        r.info_of = [r.info_dq.index(d) for d in r.dq]
+        r.factor = np.concatenate([self.factor, np.ones(len(keys))])
+        r.off = list(self.off)
+        t = self.total
+        for k in range(len(self.keys), len(r.keys)):
+            r.off.append(t)
+            t += r.infos[r.info_of[k]].total
+        r.total = t
+        r.data = torch.zeros(max(t, 1), dtype=torch.float64,
+                             device=self.data.device)
+        if self.total:
+            r.data[:self.total] = self.data[:self.total]
+        return r
 
     def subset(self, keys):
         """The operators `keys` of this tensor (same data, new index)."""
@@ -244,10 +257,21 @@ class DMRG:
     solve(n_sweeps, forward, tol) runs the sweeps and returns the final
     energy; sweep_energies / energies / discarded_weights / sweep_time hold
     the per-sweep results and `steps` one record per sweep step (sweep,
-    step, energy, discarded weight, Davidson iterations, seconds)."""
+    step, energy, discarded weight, Davidson iterations, seconds);
+    phase_time the seconds spent per phase of the steps (wavefunction moves,
+    blocking, setup, davidson, truncation, renormalisation).
+
+    precision: the Davidson's H*psi, "f64", or "f32" / "bf16x3" (FP32 MFMA
+    or split-BF16 products inside the Davidson, then an FP64 refinement of
+    the converged vector; Gram-Schmidt, the Ritz problem, residuals and the
+    truncation stay FP64)."""
 
     def __init__(self, ctx, symmetry, mpo_file, mps_dir, bond_dims, noises,
-                 ubond=4, noise_type="dm", noise_seed=0):
+                 ubond=4, noise_type="dm", noise_seed=0, precision="f64"):
+        if precision not in ("f64", "f32", "bf16x3"):
+            raise ValueError("precision: f64, f32 or bf16x3")
+        self.precision = precision
+        self.phase_time, self.davidson_macs = {}, 0
         self.ctx = ctx
         self.dev = ctx.device
         self.lib = L.lib()
@@ -345,13 +369,16 @@ class DMRG:
         keep.append(tm)
         d.n_terms, d.terms = len(terms), tm
         h = C.c_void_p()
+        t = time.perf_counter()
         L.check(self.lib.b2g_contract_plan(self.ctx.h, C.byref(d), 0,
                                            C.byref(h)))
+        t = self._tick("blocking_plan", t)
         try:
             torch.cuda.current_stream(self.dev).synchronize()
             L.check(self.lib.b2g_winplan_run(h, _dev(A.data), _dev(B.data),
                                              _dev(Cs.data), 0))
             self.ctx.synchronize()
+            self._tick("blocking_run", t)
         finally:
             self.lib.b2g_winplan_destroy(h)
 
@@ -432,11 +459,8 @@ class DMRG:
         """env with the intermediate operators appended: operator j =
         sum of f op(env operator) over inters[j][1] (conj: transposed), in
         the shape of the element inters[j][0] (its quantum)."""
-        keys = env.keys + [("sum", j) for j in range(len(inters))]
-        out = OpTensor(self.sym, env.bra, env.ket, keys,
-                       env.dq + [int(o[4]) for o, _ in inters], self.dev)
-        out.factor[:len(env.keys)] = env.factor
-        _copy_ops(env, out)
+        out = env.extend([("sum", j) for j in range(len(inters))],
+                         [int(o[4]) for o, _ in inters])
         terms = []
         for j, (_, parts) in enumerate(inters):
             c = out.index[("sum", j)]
@@ -492,15 +516,18 @@ class DMRG:
         plan = C.c_void_p()
         try:
             ri = _cinfo(rot.lay, keep)
+            t = time.perf_counter()
             L.check(self.lib.b2g_rotate_plan(
                 self.ctx.h, s, 1 if trans else 0, C.byref(ri), rbuf, na, adq,
                 ainf, nc, cdq, cinf, nop, aops, cops,
                 blk.total, out.total, 0, C.byref(plan)))
+            t = self._tick("rotation_plan", t)
             torch.cuda.current_stream(self.dev).synchronize()
             L.check(self.lib.b2g_hpsi(self.ctx.h, plan, _dev(blk.data),
                                       _dev(out.data), 1, 1.0, 0,
                                       L.PREC["f64"]))
             self.ctx.synchronize()
+            self._tick("rotation_run", t)
         finally:
             if plan:
                 self.lib.b2g_plan_destroy(plan)
@@ -519,12 +546,7 @@ class DMRG:
         dqs = {k: x[4] for k, (x, _) in zip(keys, live)}
         if new:
             # an environment holding both sets (the new operators zero)
-            allk = env.keys + new
-            out = OpTensor(s, env.bra, env.ket, allk,
-                           env.dq + [dqs[k] for k in new], self.dev)
-            out.factor[:len(env.keys)] = env.factor
-            _copy_ops(env, out)
-            env = out
+            env = env.extend(new, [dqs[k] for k in new])
         terms = []
         for j, (nm, e) in enumerate(live):
             e = _unref(e)
@@ -552,14 +574,17 @@ class DMRG:
             *[L.b2g_transform_term(c, a, cj, 0, f)
               for (c, a, cj, f) in terms])
         h = C.c_void_p()
+        t = time.perf_counter()
         L.check(self.lib.b2g_transform_plan(
             self.ctx.h, self.sym, ni, dq, inf, nop, ops,
             len(terms), tm, 0, C.byref(h)))
+        t = self._tick("transform_plan", t)
         try:
             torch.cuda.current_stream(self.dev).synchronize()
             L.check(self.lib.b2g_winplan_run(h, _dev(env.data), None,
                                              _dev(env.data), 1))
             self.ctx.synchronize()
+            self._tick("transform_run", t)
         finally:
             self.lib.b2g_winplan_destroy(h)
 
@@ -673,12 +698,19 @@ class DMRG:
         pb = Problem(s, self.hq, 0, w, w, iset(lblk.infos),
                      iset(rblk.infos), oset(lblk), oset(rblk), conj, ta, tb,
                      tf, dict(site=i))
-        return EffectiveHamiltonian(pb, self.ctx, fp32=False,
+        return EffectiveHamiltonian(pb, self.ctx,
+                                    fp32=self.precision == "f32",
                                     device_data=(lblk.data, rblk.data))
+
+    def _tick(self, name, t):
+        """Adds the seconds since t to phase `name`; returns now."""
+        now = time.perf_counter()
+        self.phase_time[name] = self.phase_time.get(name, 0.0) + now - t
+        return now
 
     def update_two_dot(self, i, forward, bond_dim, noise, conv):
         s, n = self.sym, self.n
-        t0 = time.perf_counter()
+        t0 = tp = time.perf_counter()
         # the two-site wavefunction at (i, i+1)
         if self.tensors[i + 1] is not None:
             lay = self._wfn_layout(i)
@@ -690,16 +722,22 @@ class DMRG:
             self.tensors[i], self.tensors[i + 1] = psi, None
         psi = self.tensors[i]
         lay = psi.lay
+        tp = self._tick("wavefunction", tp)
         lblk = self.left_block(i, self.lenv[i])
         rblk = self.right_block(i + 1, self.renv[i + 2])
+        tp = self._tick("blocking", tp)
         h = self._eff_ham(i, lblk, rblk, lay)
+        tp = self._tick("setup", tp)
         x = psi.data[:lay.total].clone()
         e, ndav, nflop, tdav = h.eigs(
             x, conv_thrd=conv, max_iter=self.davidson_max_iter,
             soft_max_iter=self.davidson_soft_max_iter,
             deflation_min_size=self.davidson_def_min,
-            deflation_max_size=self.davidson_def_max, precision="f64")
+            deflation_max_size=self.davidson_def_max,
+            precision=self.precision, refine=True)
         h.deallocate()
+        tp = self._tick("davidson", tp)
+        self.davidson_macs += nflop
         energy = e + self.const_e
         bi = T.BlockInfo(lay.quanta, lay.nbra, lay.nket, lay.shift, lay.dq,
                          int(lay.fermion))
@@ -707,6 +745,7 @@ class DMRG:
             self.ctx, s, bi, x, int(bond_dim), self.cutoff, forward, True,
             T.TRUNC_PHYSICAL, 1.0, None, noise if noise else 0.0,
             self.noise_type, 0, rng=self.rng if noise else None)
+        tp = self._tick("truncation", tp)
         (linfo, ldata), (rinfo, rdata) = sp.left, sp.right
         if forward:
             left = MPSTensor(_layout_of_split(linfo, False), ldata)
@@ -718,6 +757,7 @@ class DMRG:
                 env = self.transform(env, sc["left_new_operator_names"],
                                      sc["left_new_operator_exprs"])
             self.lenv[i + 1] = env
+            tp = self._tick("renormalisation", tp)
             self.tensors[i], self.tensors[i + 1] = left, right
             if i + 1 != n - 1:
                 self.tensors[i + 1] = self._swap(right, i + 1, True)
@@ -731,9 +771,11 @@ class DMRG:
                 env = self.transform(env, sc["right_new_operator_names"],
                                      sc["right_new_operator_exprs"])
             self.renv[i + 1] = env
+            tp = self._tick("renormalisation", tp)
             self.tensors[i], self.tensors[i + 1] = left, right
             if i != 0:
                 self.tensors[i] = self._swap(left, i, False)
+        self._tick("wavefunction", tp)
         self.bonds.append(self.left_dims[i + 1] if forward else
                           self.right_dims[i + 1])
         return energy, sp.error, ndav, time.perf_counter() - t0
@@ -771,6 +813,9 @@ class DMRG:
         self.davidson_conv_thrds = dc
         self.energies, self.discarded_weights, self.sweep_time = [], [], []
         self.steps, self.bonds = [], []
+        # seconds per phase of the sweep steps (host clock; the phases end
+        # in synchronisations), and the Davidson's multiply-adds
+        self.phase_time, self.davidson_macs = {}, 0
         self.init_environments()
         for iw in range(n_sweeps):
             self.isweep = iw
