@@ -214,6 +214,67 @@ def sweep_leg(ctx, threads, run_cpu):
     return out
 
 
+def davidson_profile(args):
+    """One FP64 Davidson solve (one root, the bench's start vector) on the
+    bench site, for a kernel-trace profile of the bandwidth-bound kernels:
+    prints the algorithmic HBM bytes of each Davidson step kernel summed
+    over the solve (step t of the solve works on a subspace of m = t
+    vectors of |psi| doubles; the last step, converged, appends nothing)
+    and of the diagonal (its output written once, the operator diagonals
+    read once), to be divided by the summed kernel durations the profiler
+    reports."""
+    import ctypes as C
+    import numpy as np
+    import torch
+    from b2g import _lib as L
+    from b2g import synth
+    from b2g.effham import Context, EffectiveHamiltonian
+    pb = synth.middle_site(n_orb=args.n_orb, m=args.m, max_dim=args.max_dim,
+                           seed=args.seed, pair_scale=args.pair_scale,
+                           data="device")
+    ctx = Context(0)
+    ldev, rdev = synth.fill_device(pb, ctx.device)
+    torch.cuda.synchronize()
+    h = EffectiveHamiltonian(pb, ctx, fp32=False, device_data=(ldev, rdev))
+    n = h.ket_len
+    nt, we, re_ = C.c_uint64(), C.c_uint64(), C.c_uint64()
+    L.check(L.lib().b2g_effham_diag_stats(h.h, C.byref(nt), C.byref(we),
+                                          C.byref(re_)))
+    # the bench's guess: the lowest diagonal state plus seeded noise
+    g = torch.Generator(device=ctx.device).manual_seed(args.seed)
+    x = 0.3 * torch.randn(n, generator=g, dtype=torch.float64,
+                          device=ctx.device) / float(np.sqrt(n))
+    x[int(torch.argmin(h.diag))] += 1.0
+    x /= torch.linalg.norm(x)
+    e, nd, nflop, tdav = h.eigs(x, conv_thrd=1e-9, precision="f64")
+    steps = list(range(1, nd + 1))
+    grow = steps[:-1]            # steps that append a vector
+    b = 8.0
+    per = {
+        "put_sigma_kernel": b * 2 * n * len(grow),
+        "gram_mfma_kernel": sum(b * 2 * m * n for m in steps),
+        "rotate_mfma_kernel": sum(b * 4 * m * n for m in steps),
+        "residual_kernel": b * 3 * n * len(steps),
+        "olsen1_kernel": b * 5 * n * len(steps),
+        "olsen2_kernel": b * 3 * n * len(steps),
+        "gs_dots_kernel": sum(2 * b * (m + 1) * n for m in grow),
+        "gs_update_kernel": sum(2 * b * (m + 2) * n for m in grow),
+        "norm_kernel": b * n * len(grow),
+        "append_kernel": b * 3 * n * len(grow),
+        "diag_kernel": b * (n + re_.value),
+    }
+    out = {"workload": "one FP64 Davidson solve (1 root, conv 1e-9) on the "
+                       "n=%d, M=%d SU(2) middle site, seed %d" % (
+                           args.n_orb, args.m, args.seed),
+           "psi_len": n, "davidson_steps": nd, "energy": e,
+           "diag_terms": nt.value, "diag_window_elems": we.value,
+           "diag_flops": 2 * we.value, "algorithmic_bytes_total": per}
+    print(json.dumps(out), flush=True)
+    h.deallocate()
+    ctx.close()
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -241,7 +302,13 @@ def main():
                     help="only run 5 device-to-device copies of MIB MiB "
                          "(reference kernel of known traffic for the HBM "
                          "byte counters) and exit")
+    ap.add_argument("--davidson-profile", action="store_true",
+                    help="only run one FP64 Davidson solve on the site and "
+                         "print the algorithmic bytes of its bandwidth-bound "
+                         "kernels (for a kernel-trace profile) and exit")
     args = ap.parse_args()
+    if args.davidson_profile:
+        return davidson_profile(args)
     if args.calibrate_copy:
         import torch
         n = args.calibrate_copy * (1 << 20) // 8

@@ -198,6 +198,7 @@ _SIGS = {
     "b2g_plan_stats": (C.c_int, [_vp, u64p, u64p, u64p]),
     "b2g_hpsi": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_double, C.c_int,
                            C.c_int]),
+    "b2g_effham_diag_stats": (C.c_int, [_vp, u64p, u64p, u64p]),
     "b2g_hpsi_noalloc": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_double,
                                    C.c_int, C.c_int]),
     "b2g_diag": (C.c_int, [_vp, _vp, _vp, C.POINTER(b2g_diag_term),

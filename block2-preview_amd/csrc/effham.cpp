@@ -475,6 +475,24 @@ extern "C" int b2g_effham_destroy(b2g_effham h) {
     return B2G_OK;
 }
 
+extern "C" int b2g_effham_diag_stats(b2g_effham h, uint64_t *n_terms,
+                                     uint64_t *window_elems,
+                                     uint64_t *read_elems) {
+    B2G_REQUIRE(h, B2G_ERR_ARG, "NULL effham");
+    uint64_t w = 0, r = 0;
+    for (const DiagRec &q : h->drecs) {
+        w += (uint64_t)q.a_n * q.b_n;
+        r += (uint64_t)q.a_n + q.b_n;
+    }
+    if (n_terms)
+        *n_terms = h->drecs.size();
+    if (window_elems)
+        *window_elems = w;
+    if (read_elems)
+        *read_elems = r;
+    return B2G_OK;
+}
+
 extern "C" int b2g_effham_info(b2g_effham h, uint64_t *ket_len,
                                uint64_t *bra_len, uint64_t *n_pairs,
                                uint64_t *macs) {

@@ -369,6 +369,11 @@ int b2g_effham_destroy(b2g_effham h);
 /* sizes: |psi| (ket total), |sigma| (bra total), pairs, reference MACs    */
 int b2g_effham_info(b2g_effham h, uint64_t *ket_len, uint64_t *bra_len,
                     uint64_t *n_pairs, uint64_t *macs);
+/* The diagonal's terms: their number, the summed term windows (multiply-
+ * adds of one diagonal) and the diagonal elements of the operator blocks
+ * they read.  For accounting only.                                       */
+int b2g_effham_diag_stats(b2g_effham h, uint64_t *n_terms,
+                          uint64_t *window_elems, uint64_t *read_elems);
 /* Host copies of the connection lists the builder produced (for checking
  * against ConnectionInfo): which = 0 wfn list, 1 diagonal list.  Arrays of
  * length *n_conn (queried with NULL pointers first).                      */
