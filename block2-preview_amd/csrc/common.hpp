@@ -148,6 +148,10 @@ struct b2g_ctx_s {
     // upload waits for its copy before returning)
     char *stage = nullptr;
     size_t stage_len = 0;
+    // device blocks of destroyed window plans, kept for the next ones
+    // (sweeps create and destroy several plans per step); freed with the
+    // context
+    std::vector<std::pair<size_t, void *>> win_pool;
 };
 
 struct b2g_buf_s {

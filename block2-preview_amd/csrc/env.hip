@@ -201,6 +201,7 @@ struct b2g_winplan_s {
     bool host_only = false;
     bool b_is_one = false; // B is the plan's own 1.0 (transforms, swaps)
     void *dev = nullptr;
+    size_t dev_len = 0; // bytes of the block (from the context's pool)
     b2g::WinGroup *d_groups = nullptr;
     b2g::WinRec *d_recs = nullptr;
     b2g::WinChunk *d_chunks = nullptr;
@@ -210,10 +211,37 @@ struct b2g_winplan_s {
 
 namespace b2g {
 
+// A device block of at least `bytes` from the context's pool of released
+// window-plan blocks (smallest that fits), else a new allocation.
+static int win_block(b2g_ctx ctx, size_t bytes, void **out, size_t *len) {
+    auto &pool = ctx->win_pool;
+    size_t best = pool.size();
+    for (size_t i = 0; i < pool.size(); i++)
+        if (pool[i].first >= bytes &&
+            (best == pool.size() || pool[i].first < pool[best].first))
+            best = i;
+    if (best != pool.size()) {
+        *len = pool[best].first;
+        *out = pool[best].second;
+        pool.erase(pool.begin() + best);
+        return B2G_OK;
+    }
+    const size_t mib = (size_t)1 << 20;
+    const size_t n = (bytes + mib - 1) / mib * mib;
+    B2G_HIP(hipMalloc(out, n));
+    *len = n;
+    return B2G_OK;
+}
+
 static int winplan_finish(b2g_winplan w,
                           std::vector<std::pair<WinGroup, WinRec>> &items) {
     std::string err;
+    SetupTimer tm("window plan");
     B2G_REQUIRE(w->plan.finish(items, err), B2G_ERR_ARG, err);
+    tm.mark("group");
+    if (getenv("B2G_TIME_SETUP") != nullptr)
+        fprintf(stderr, "b2g window plan: %zu records, %zu windows\n",
+                items.size(), w->plan.groups.size());
     for (uint32_t g = 0; g < w->plan.groups.size(); g++) {
         const uint64_t ne = (uint64_t)w->plan.groups[g].rows *
                             w->plan.groups[g].cols;
@@ -229,24 +257,38 @@ static int winplan_finish(b2g_winplan w,
     const size_t bytes = ng * sizeof(WinGroup) + nr * sizeof(WinRec) +
                          nc * sizeof(WinChunk) + 64;
     B2G_HIP(hipSetDevice(w->ctx->device));
-    B2G_HIP(hipMalloc(&w->dev, bytes));
+    {
+        const int r = win_block(w->ctx, bytes, &w->dev, &w->dev_len);
+        if (r != B2G_OK)
+            return r;
+    }
     char *p = (char *)w->dev;
     w->d_one = (double *)p;
     w->d_groups = (WinGroup *)(p + 64);
     w->d_recs = (WinRec *)((char *)w->d_groups + ng * sizeof(WinGroup));
     w->d_chunks = (WinChunk *)((char *)w->d_recs + nr * sizeof(WinRec));
-    std::vector<char> host(bytes, 0);
+    // staged through the context's pinned buffer
+    char *host = nullptr;
+    {
+        const int r = ctx_stage(w->ctx, bytes, &host);
+        if (r != B2G_OK)
+            return r;
+    }
+    std::memset(host, 0, 64);
     const double one = 1.0;
-    std::memcpy(host.data(), &one, sizeof(double));
+    std::memcpy(host, &one, sizeof(double));
     if (ng)
-        std::memcpy(host.data() + 64, w->plan.groups.data(), ng * sizeof(WinGroup));
+        std::memcpy(host + 64, w->plan.groups.data(), ng * sizeof(WinGroup));
     if (nr)
-        std::memcpy(host.data() + 64 + ng * sizeof(WinGroup), w->plan.recs.data(),
-               nr * sizeof(WinRec));
+        std::memcpy(host + 64 + ng * sizeof(WinGroup), w->plan.recs.data(),
+                    nr * sizeof(WinRec));
     if (nc)
-        std::memcpy(host.data() + 64 + ng * sizeof(WinGroup) + nr * sizeof(WinRec),
-               w->chunks.data(), nc * sizeof(WinChunk));
-    B2G_HIP(hipMemcpy(w->dev, host.data(), bytes, hipMemcpyHostToDevice));
+        std::memcpy(host + 64 + ng * sizeof(WinGroup) + nr * sizeof(WinRec),
+                    w->chunks.data(), nc * sizeof(WinChunk));
+    B2G_HIP(hipMemcpyAsync(w->dev, host, bytes, hipMemcpyHostToDevice,
+                           w->ctx->stream));
+    B2G_HIP(hipStreamSynchronize(w->ctx->stream));
+    tm.mark("upload");
     return B2G_OK;
 }
 
@@ -295,6 +337,7 @@ extern "C" int b2g_contract_plan(b2g_ctx ctx, const b2g_contract *c,
                    std::make_pair(&c->ket, "ket")})
         if ((r = check_states(*x.first, x.second)) != B2G_OK)
             return r;
+    SetupTimer tm("contract plan");
     OpSide A, B, Cs;
     if ((r = to_side(c->n_a_infos, c->a_info_dq, c->a_infos, c->n_a_ops,
                      c->a_ops, "A operators", A)) != B2G_OK ||
@@ -384,6 +427,7 @@ extern "C" int b2g_contract_plan(b2g_ctx ctx, const b2g_contract *c,
             items.emplace_back(g, rec);
         }
     }
+    tm.mark("connections and records");
     if ((r = winplan_finish(w.get(), items)) != B2G_OK)
         return r;
     *out = w.release();
@@ -437,11 +481,13 @@ extern "C" int b2g_transform_plan(b2g_ctx ctx, int symmetry, int n_infos,
     B2G_REQUIRE(n_terms >= 0 && (n_terms == 0 || terms), B2G_ERR_ARG,
                 "bad terms");
     const Symm s = symm_of(symmetry);
+    SetupTimer tm("transform plan");
     OpSide O;
     int r;
     if ((r = to_side(n_infos, info_dq, infos, n_ops, ops, "operators", O)) !=
         B2G_OK)
         return r;
+    tm.mark("operators");
     SU2Coupling cg;
     std::unique_ptr<b2g_winplan_s> w(new b2g_winplan_s());
     w->ctx = ctx;
@@ -449,6 +495,44 @@ extern "C" int b2g_transform_plan(b2g_ctx ctx, int symmetry, int n_infos,
     w->b_is_one = true;
     std::vector<std::pair<WinGroup, WinRec>> items;
     std::vector<uint8_t> written(n_ops, 0), read(n_ops, 0);
+    // per (c info, a info, conj): the a block of each c block (-1: none)
+    // and its transpose factor, shared by every term with those infos
+    struct BlockMap {
+        std::vector<int> xa;
+        std::vector<long double> cgf;
+    };
+    std::map<std::tuple<int, int, int>, BlockMap> maps;
+    auto block_map = [&](const Operator &co, const Operator &ao,
+                         int conj) -> const BlockMap & {
+        auto key = std::make_tuple(co.info, ao.info, conj);
+        auto it = maps.find(key);
+        if (it != maps.end())
+            return it->second;
+        const BlockInfo &ci = O.infos[co.info], &ai = O.infos[ao.info];
+        BlockMap m;
+        m.xa.assign(ci.n(), -1);
+        m.cgf.assign(ci.n(), 1.0L);
+        for (int xc = 0; xc < ci.n(); xc++) {
+            const uint64_t bra = q_get_bra(s, ci.quanta[xc], ci.dq);
+            const uint64_t ket = q_get_ket(s, ci.quanta[xc]);
+            const uint64_t aq = conj ? q_combine(s, ao.dq, ket, bra)
+                                     : q_combine(s, ao.dq, bra, ket);
+            if (aq == QINVALID)
+                continue;
+            m.xa[xc] = ai.find_state(aq);
+            if (m.xa[xc] >= 0 && conj && s == Symm::SU2)
+                m.cgf[xc] = cg.transpose(q_twos(s, ao.dq), q_twos(s, bra),
+                                         q_twos(s, ket));
+        }
+        return maps.emplace(key, std::move(m)).first->second;
+    };
+    // terms by destination operator (destinations in first-appearance
+    // order, each one's terms in term order), so that the records of a
+    // window come out consecutive: block by block of the destination, the
+    // window's terms in term order (OperatorFunctions::iadd(c, a, factor,
+    // conj) per term)
+    std::vector<int> dest_order, dest_of(n_ops, -1);
+    std::vector<std::vector<int>> by_dest;
     for (int it = 0; it < n_terms; it++) {
         const b2g_transform_term &t = terms[it];
         B2G_REQUIRE(t.c >= 0 && t.c < n_ops && t.a >= 0 && t.a < n_ops &&
@@ -456,45 +540,81 @@ extern "C" int b2g_transform_plan(b2g_ctx ctx, int symmetry, int n_infos,
                     B2G_ERR_ARG, "transform term out of range");
         written[t.c] = 1;
         read[t.a] = 1;
-        const Operator &co = O.ops[t.c], &ao = O.ops[t.a];
-        const BlockInfo &ci = O.infos[co.info], &ai = O.infos[ao.info];
-        // OperatorFunctions::iadd(c, a, factor, conj)
-        if (std::abs(ao.factor * t.factor) < 1E-20)
+        if (std::abs(O.ops[t.a].factor * t.factor) < 1E-20)
             continue;
-        for (int xc = 0; xc < ci.n(); xc++) {
-            const uint64_t bra = q_get_bra(s, ci.quanta[xc], ci.dq);
-            const uint64_t ket = q_get_ket(s, ci.quanta[xc]);
-            const uint64_t aq = t.conj ? q_combine(s, ao.dq, ket, bra)
-                                       : q_combine(s, ao.dq, bra, ket);
-            if (aq == QINVALID)
-                continue;
-            const int xa = ai.find_state(aq);
-            if (xa < 0)
-                continue;
-            double f = t.factor * ao.factor;
-            if (t.conj && s == Symm::SU2)
-                f = (double)((long double)f *
-                             cg.transpose(q_twos(s, ao.dq), q_twos(s, bra),
-                                          q_twos(s, ket)));
-            WinRec rec{};
-            rec.a_off = ao.data_off + ai.shift[xa];
-            rec.am = ai.nbra[xa], rec.an = ai.nket[xa], rec.lda = rec.an;
-            rec.b_off = 0, rec.bm = rec.bn = rec.ldb = 1;
-            rec.conj = t.conj;
-            rec.scale = f;
-            WinGroup g{};
-            g.c_off = co.data_off + ci.shift[xc];
-            g.ldc = ci.nket[xc];
-            g.rows = t.conj ? rec.an : rec.am;
-            g.cols = t.conj ? rec.am : rec.an;
-            B2G_REQUIRE(g.rows == ci.nbra[xc] && g.cols == ci.nket[xc],
-                        B2G_ERR_ARG, "transform block shapes differ");
-            items.emplace_back(g, rec);
+        if (dest_of[t.c] < 0) {
+            dest_of[t.c] = (int)by_dest.size();
+            dest_order.push_back(t.c);
+            by_dest.emplace_back();
         }
+        by_dest[dest_of[t.c]].push_back(it);
+    }
+    for (size_t d = 0; d < dest_order.size(); d++) {
+        const Operator &co = O.ops[dest_order[d]];
+        const BlockInfo &ci = O.infos[co.info];
+        // every term of this destination untransposed and of its layout:
+        // the sum is one axpy over the whole operator (the reference's
+        // iadd does the same when the infos agree), one record per term
+        bool whole = ci.total() < (1ull << 31);
+        for (int it : by_dest[d])
+            whole = whole && terms[it].conj == 0 &&
+                    O.ops[terms[it].a].info == co.info;
+        if (whole) {
+            const uint32_t len = (uint32_t)ci.total();
+            for (int it : by_dest[d]) {
+                const b2g_transform_term &t = terms[it];
+                const Operator &ao = O.ops[t.a];
+                WinRec rec{};
+                rec.a_off = ao.data_off;
+                rec.am = len == 0 ? 0 : 1, rec.an = len, rec.lda = len;
+                rec.b_off = 0, rec.bm = rec.bn = rec.ldb = 1;
+                rec.conj = 0;
+                rec.scale = t.factor * ao.factor;
+                WinGroup g{};
+                g.c_off = co.data_off;
+                g.ldc = len;
+                g.rows = rec.am, g.cols = len;
+                if (len != 0)
+                    items.emplace_back(g, rec);
+            }
+            continue;
+        }
+        std::vector<const BlockMap *> bm(by_dest[d].size());
+        for (size_t q = 0; q < by_dest[d].size(); q++) {
+            const b2g_transform_term &t = terms[by_dest[d][q]];
+            bm[q] = &block_map(co, O.ops[t.a], t.conj);
+        }
+        for (int xc = 0; xc < ci.n(); xc++)
+            for (size_t q = 0; q < by_dest[d].size(); q++) {
+                const int xa = bm[q]->xa[xc];
+                if (xa < 0)
+                    continue;
+                const b2g_transform_term &t = terms[by_dest[d][q]];
+                const Operator &ao = O.ops[t.a];
+                const BlockInfo &ai = O.infos[ao.info];
+                double f = t.factor * ao.factor;
+                if (t.conj && s == Symm::SU2)
+                    f = (double)((long double)f * bm[q]->cgf[xc]);
+                WinRec rec{};
+                rec.a_off = ao.data_off + ai.shift[xa];
+                rec.am = ai.nbra[xa], rec.an = ai.nket[xa], rec.lda = rec.an;
+                rec.b_off = 0, rec.bm = rec.bn = rec.ldb = 1;
+                rec.conj = t.conj;
+                rec.scale = f;
+                WinGroup g{};
+                g.c_off = co.data_off + ci.shift[xc];
+                g.ldc = ci.nket[xc];
+                g.rows = t.conj ? rec.an : rec.am;
+                g.cols = t.conj ? rec.am : rec.an;
+                B2G_REQUIRE(g.rows == ci.nbra[xc] && g.cols == ci.nket[xc],
+                            B2G_ERR_ARG, "transform block shapes differ");
+                items.emplace_back(g, rec);
+            }
     }
     for (int i = 0; i < n_ops; i++)
         B2G_REQUIRE(!(written[i] && read[i]), B2G_ERR_ARG,
                     "a transform reads an operator it writes");
+    tm.mark("records");
     if ((r = winplan_finish(w.get(), items)) != B2G_OK)
         return r;
     *out = w.release();
@@ -721,9 +841,10 @@ extern "C" int b2g_winplan_destroy(b2g_winplan w) {
     if (w == nullptr)
         return B2G_OK;
     if (w->dev) {
+        // the plan's launches are done before its block serves another
         hipSetDevice(w->ctx->device);
         hipStreamSynchronize(w->ctx->stream);
-        hipFree(w->dev);
+        w->ctx->win_pool.emplace_back(w->dev_len, w->dev);
     }
     delete w;
     return B2G_OK;

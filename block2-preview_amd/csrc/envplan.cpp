@@ -226,49 +226,65 @@ struct WinKeyHash {
 
 bool WinPlan::finish(std::vector<std::pair<WinGroup, WinRec>> &items,
                      std::string &err) {
-    FlatMap<WinKey, uint32_t, WinKeyHash> idx(items.size());
-    std::vector<std::vector<uint32_t>> members;
+    // group id of every item (windows in first-appearance order), then a
+    // counting sort of the records by group, stable within a group
+    // sized for the windows, not the records (records come in runs)
+    FlatMap<WinKey, uint32_t, WinKeyHash> idx(1024);
+    std::vector<uint32_t> gid(items.size());
     std::vector<WinGroup> gs;
+    std::vector<uint32_t> cnt;
     for (uint32_t i = 0; i < items.size(); i++) {
         const WinGroup &g = items[i].first;
+        const WinGroup *p = i > 0 ? &items[i - 1].first : nullptr;
+        if (p != nullptr && p->c_off == g.c_off && p->rows == g.rows &&
+            p->cols == g.cols && p->ldc == g.ldc) {
+            // a run of records of one window (no lookup)
+            gid[i] = gid[i - 1];
+            cnt[gid[i]]++;
+            continue;
+        }
         WinKey k{g.c_off, g.rows, g.cols, g.ldc};
         auto r = idx.emplace(k, (uint32_t)gs.size());
         if (r.second) {
             gs.push_back(g);
-            members.emplace_back();
+            cnt.push_back(0);
         }
-        members[*r.first].push_back(i);
+        gid[i] = *r.first;
+        cnt[*r.first]++;
     }
     // windows starting at the same element must be the same window
-    std::vector<std::pair<uint64_t, uint32_t>> starts;
+    std::vector<std::pair<uint64_t, uint32_t>> starts(gs.size());
     for (uint32_t g = 0; g < gs.size(); g++)
-        starts.emplace_back(gs[g].c_off, g);
+        starts[g] = std::make_pair(gs[g].c_off, g);
     std::sort(starts.begin(), starts.end());
     for (size_t i = 1; i < starts.size(); i++)
         if (starts[i].first == starts[i - 1].first) {
             err = "two windows of different shapes start at one element";
             return false;
         }
-    groups.clear();
-    recs.clear();
+    std::vector<uint32_t> pos(gs.size() + 1, 0);
+    for (uint32_t g = 0; g < gs.size(); g++)
+        pos[g + 1] = pos[g] + cnt[g];
+    groups.assign(gs.begin(), gs.end());
+    recs.resize(items.size());
     src_elems = dst_elems = 0;
     for (uint32_t g = 0; g < gs.size(); g++) {
-        WinGroup w = gs[g];
-        w.rec_begin = (uint32_t)recs.size();
-        for (uint32_t i : members[g]) {
-            const WinRec &r = items[i].second;
-            recs.push_back(r);
-            src_elems += (uint64_t)r.am * r.an + (uint64_t)r.bm * r.bn;
-            a_need = std::max(a_need, r.a_off + (r.am == 0 ? 0 :
-                              (uint64_t)(r.am - 1) * r.lda + r.an));
-            b_need = std::max(b_need, r.b_off + (r.bm == 0 ? 0 :
-                              (uint64_t)(r.bm - 1) * r.ldb + r.bn));
-        }
-        w.rec_end = (uint32_t)recs.size();
+        groups[g].rec_begin = pos[g];
+        groups[g].rec_end = pos[g + 1];
+    }
+    for (uint32_t i = 0; i < items.size(); i++) {
+        const WinRec &r = items[i].second;
+        recs[pos[gid[i]]++] = r;
+        src_elems += (uint64_t)r.am * r.an + (uint64_t)r.bm * r.bn;
+        a_need = std::max(a_need, r.a_off + (r.am == 0 ? 0 :
+                          (uint64_t)(r.am - 1) * r.lda + r.an));
+        b_need = std::max(b_need, r.b_off + (r.bm == 0 ? 0 :
+                          (uint64_t)(r.bm - 1) * r.ldb + r.bn));
+    }
+    for (const WinGroup &w : groups) {
         dst_elems += (uint64_t)w.rows * w.cols;
         c_len = std::max(c_len, w.c_off + (w.rows == 0 ? 0 :
                          (uint64_t)(w.rows - 1) * w.ldc + w.cols));
-        groups.push_back(w);
     }
     return true;
 }

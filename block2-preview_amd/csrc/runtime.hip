@@ -178,6 +178,8 @@ extern "C" int b2g_ctx_destroy(b2g_ctx ctx) {
     hipHostFree(ctx->red_host);
     if (ctx->stage)
         hipHostFree(ctx->stage);
+    for (auto &b : ctx->win_pool)
+        hipFree(b.second);
     hipStreamDestroy(ctx->stream);
     delete ctx;
     return B2G_OK;
