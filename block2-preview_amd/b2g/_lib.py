@@ -254,6 +254,10 @@ _SIGS = {
         _vp, C.c_int, C.POINTER(b2g_sparse_info), _vp, C.c_int, f64p,
         C.c_int, C.c_double, C.c_int, C.c_int, C.c_uint, C.c_double,
         C.POINTER(_vp)]),
+    "b2g_split_density_matrix_extra": (C.c_int, [
+        _vp, C.c_int, C.POINTER(b2g_sparse_info), _vp, C.c_int,
+        C.POINTER(b2g_sparse_info), u64p, C.c_int, C.c_double, C.c_int,
+        C.c_int, C.c_uint, C.c_double, C.POINTER(_vp)]),
     "b2g_split_n_roots": (C.c_int, [_vp, C.POINTER(C.c_int)]),
     "b2g_split_density_matrix_noise": (C.c_int, [
         _vp, C.c_int, C.POINTER(b2g_sparse_info), _vp, C.c_int, f64p,

@@ -98,6 +98,22 @@ def _dev(t):
     return C.c_void_p(t.data_ptr())
 
 
+def _info_set(lays):
+    """An InfoSet of the engine's Problem from block layouts."""
+    n = [len(x) for x in lays]
+    off = np.concatenate([[0], np.cumsum(n)[:-1]]) if n else []
+    cat = lambda k, dt: (np.concatenate([getattr(x, k) for x in lays])
+                         .astype(dt) if lays else np.zeros(0, dt))
+    return InfoSet([x.dq for x in lays], [x.fermion for x in lays],
+                   n, off, cat("quanta", np.uint64),
+                   cat("nbra", np.uint32), cat("nket", np.uint32),
+                   cat("shift", np.uint64))
+
+
+def _op_set(t):
+    return OpSet(t.dq, t.factor, t.info_of, t.off, None)
+
+
 class OpTensor:
     """The operators of one block (an OperatorTensor's ops): keys in order,
     one block layout per delta quantum (sorted), device FP64 data."""
@@ -296,8 +312,11 @@ class DMRG:
         self.schemer = m["schemer"]
         self.bond_dims = list(bond_dims)
         self.noises = list(noises)
+        # "pert": ReducedPerturbativeCollected, the Python driver's default
+        self.perturbative = noise_type == "pert"
         self.noise_type = {"dm": T.NOISE_DENSITY_MATRIX,
-                           "wfn": T.NOISE_WAVEFUNCTION}[noise_type]
+                           "wfn": T.NOISE_WAVEFUNCTION,
+                           "pert": 0}[noise_type]
         self.rng = T.Rng(noise_seed) if noise_seed else None
         self.davidson_max_iter = 5000
         self.davidson_soft_max_iter = 4000
@@ -681,19 +700,7 @@ class DMRG:
             tb.append(rblk.index[_key(b)])
             tf.append(f / fn)
 
-        def iset(lays):
-            n = [len(x) for x in lays]
-            off = np.concatenate([[0], np.cumsum(n)[:-1]]) if n else []
-            cat = lambda k, dt: (np.concatenate([getattr(x, k) for x in lays])
-                                 .astype(dt) if lays else np.zeros(0, dt))
-            return InfoSet([x.dq for x in lays], [x.fermion for x in lays],
-                           n, off, cat("quanta", np.uint64),
-                           cat("nbra", np.uint32), cat("nket", np.uint32),
-                           cat("shift", np.uint64))
-
-        def oset(t):
-            return OpSet(t.dq, t.factor, t.info_of, t.off, None)
-
+        iset, oset = _info_set, _op_set
         w = iset([lay])
         pb = Problem(s, self.hq, 0, w, w, iset(lblk.infos),
                      iset(rblk.infos), oset(lblk), oset(rblk), conj, ta, tb,
@@ -707,6 +714,90 @@ class DMRG:
         now = time.perf_counter()
         self.phase_time[name] = self.phase_time.get(name, 0.0) + now - t
         return now
+
+    def _perturbative_noise(self, i, lblk, rblk, lay, x, forward, noise):
+        """EffectiveHamiltonian::perturbative_noise for ReducedPerturbative
+        noise (src/dmrg/effective_hamiltonian.hpp:263-433) and its scaling
+        (MovingEnvironment::scale_perturbative_noise and density_matrix,
+        src/dmrg/moving_environment.hpp:3554-3577, 3698-3713).
+
+        Forward (the density matrix traces the right side), every string
+        f (a (x) b) of the Hamiltonian contributes f op(a) psi with the
+        right side's identity in place of b
+        (TensorFunctions::tensor_product_partial_multiply,
+        src/core/tensor_functions.hpp:366-804, TraceTypes::Right); backward
+        the mirror, psi op(b)^T with the left identity.  The contributions
+        are summed per quantum q of the result (q = psi's + op(a)'s), one
+        perturbed wavefunction per q, each an H*psi of the engine with the
+        identity as the other operator.  Each is normalised, all are scaled
+        to a total squared norm of `noise`, and all but the one of the
+        lowest q are returned, as (layout, data), for the density
+        matrix."""
+        s, m = self.sym, self.mpo
+        ident = (2, 0, self.vacuum)        # OpNames::I, no site, vacuum
+        side = rblk if forward else lblk
+        if ident not in side.index:
+            return []                      # nothing to perturb here
+        iid = side.index[ident]
+        strings = _strings(m["middle_operator_exprs"][i]["data"][0])
+        fn = m["middle_operator_names"][i]["data"][0][3]
+        groups = {}
+        for st in strings:
+            if st[0] != "prod":
+                raise NotImplementedError("Hamiltonian string %s" % st[0])
+            f, cj, a, b = st[1], st[2], st[3], st[4]
+            if forward:
+                c1, op = cj & 1, a
+                term = (c1, lblk.index[_key(a)], iid)
+            else:
+                c1, op = (cj >> 1) & 1, b
+                term = (c1 << 1, iid, rblk.index[_key(b)])
+            opdq = Q.q_neg(s, int(op[4])) if c1 else int(op[4])
+            groups.setdefault((c1, opdq), []).append(term + (f / fn,))
+        vdqs = {}
+        for (c1, opdq) in groups:
+            pk = Q.q_add(s, int(lay.dq), opdq)
+            vdqs[(c1, opdq)] = [Q.q_at(s, pk, k)
+                                for k in range(Q.q_count(s, pk))]
+        labels = sorted(set(v for vs in vdqs.values() for v in vs))
+        fl, fr = self._fused_left(i), self._fused_right(i + 1)
+        lays = {q: Q.initialize(s, fl, fr, q, False, True) for q in labels}
+        vecs = {q: torch.zeros(max(lays[q].total, 1), dtype=torch.float64,
+                               device=self.dev) for q in labels}
+        ket = _info_set([lay])
+        li, ri = _info_set(lblk.infos), _info_set(rblk.infos)
+        lo, ro = _op_set(lblk), _op_set(rblk)
+        for (c1, opdq), terms in groups.items():
+            for q in vdqs[(c1, opdq)]:
+                if lays[q].total == 0:
+                    continue
+                pb = Problem(s, opdq, 0, ket, _info_set([lays[q]]), li, ri,
+                             lo, ro, [t[0] for t in terms],
+                             [t[1] for t in terms], [t[2] for t in terms],
+                             [t[3] for t in terms], dict(site=i))
+                h = EffectiveHamiltonian(pb, self.ctx, fp32=False,
+                                         device_data=(lblk.data, rblk.data))
+                try:
+                    h(x, vecs[q][:lays[q].total])
+                finally:
+                    h.deallocate()
+        # scale_perturbative_noise: each normalised, then all together to a
+        # squared norm of `noise`
+        for q in labels:
+            nrm = float(torch.linalg.norm(vecs[q]))
+            if nrm > 1e-20:
+                vecs[q] /= nrm
+        tot = float(torch.sqrt(sum(torch.dot(vecs[q], vecs[q])
+                                   for q in labels))) if labels else 0.0
+        if tot > 1e-20:
+            for q in labels:
+                vecs[q] *= np.sqrt(noise) / tot
+        out = []
+        for q in labels[1:]:             # density_matrix adds pkets 1..n-1
+            ly = lays[q]
+            out.append((T.BlockInfo(ly.quanta, ly.nbra, ly.nket, ly.shift,
+                                    ly.dq, 0), vecs[q][:ly.total]))
+        return out
 
     def update_two_dot(self, i, forward, bond_dim, noise, conv):
         s, n = self.sym, self.n
@@ -741,10 +832,19 @@ class DMRG:
         energy = e + self.const_e
         bi = T.BlockInfo(lay.quanta, lay.nbra, lay.nket, lay.shift, lay.dq,
                          int(lay.fermion))
-        sp = T.split_density_matrix(
-            self.ctx, s, bi, x, int(bond_dim), self.cutoff, forward, True,
-            T.TRUNC_PHYSICAL, 1.0, None, noise if noise else 0.0,
-            self.noise_type, 0, rng=self.rng if noise else None)
+        if self.perturbative and noise:
+            extra = self._perturbative_noise(i, lblk, rblk, lay, x, forward,
+                                             noise)
+            tp = self._tick("perturbative", tp)
+            sp = T.split_density_matrix(
+                self.ctx, s, bi, x, int(bond_dim), self.cutoff, forward,
+                True, T.TRUNC_PHYSICAL, 1.0, extra=extra)
+        else:
+            sp = T.split_density_matrix(
+                self.ctx, s, bi, x, int(bond_dim), self.cutoff, forward,
+                True, T.TRUNC_PHYSICAL, 1.0, None,
+                noise if noise and not self.perturbative else 0.0,
+                self.noise_type, 0, rng=self.rng if noise else None)
         tp = self._tick("truncation", tp)
         (linfo, ldata), (rinfo, rdata) = sp.left, sp.right
         if forward:

@@ -150,7 +150,7 @@ def split_density_matrix(ctx, symmetry, info, psi, k, cutoff=0.0,
                          trace_right=True, normalize=True,
                          trunc_type=TRUNC_PHYSICAL, scale=1.0, weights=None,
                          noise=0.0, noise_type=NOISE_WAVEFUNCTION, seed=0,
-                         rng=None):
+                         rng=None, extra=None):
     """Truncate the wavefunction psi (device FP64 tensor with the layout of
     `info`) to k states (-1: all) through its density matrix.  A 2-d psi
     holds state-averaged roots (one per row, density matrix weighted by
@@ -161,13 +161,31 @@ def split_density_matrix(ctx, symmetry, info, psi, k, cutoff=0.0,
     Random::rand_seed(seed).  Each call starts a new generator: with noise
     the seed must be nonzero (block2 takes 0 as a clock seed) and a sweep
     passes a new one at each site; or pass `rng` (an Rng) to draw from one
-    continuing stream."""
+    continuing stream.  extra: [(BlockInfo, device vector)] added to the
+    density matrix as v v^T (v^T v backward), the perturbed wavefunctions
+    of perturbative noise."""
     lib = ctx.lib
     ci = info.to_c()
     psi = psi.contiguous()
     torch.cuda.synchronize(ctx.device)
     h = C.c_void_p()
-    if noise and rng is not None:
+    if extra:
+        if noise or psi.dim() != 1:
+            raise ValueError("extra terms with one root and no other noise")
+        parts, offs, t = [psi], [], psi.numel()
+        for _, v in extra:
+            offs.append(t)
+            parts.append(v.reshape(-1))
+            t += v.numel()
+        buf = torch.cat(parts)
+        xinf = (L.b2g_sparse_info * len(extra))(*[x.to_c() for x, _ in extra])
+        xoff = np.ascontiguousarray(offs, dtype=np.uint64)
+        L.check(lib.b2g_split_density_matrix_extra(
+            ctx.h, int(symmetry), C.byref(ci), C.c_void_p(buf.data_ptr()),
+            len(extra), xinf, L.ptr(xoff, C.c_uint64), int(k), float(cutoff),
+            int(bool(trace_right)), int(bool(normalize)), int(trunc_type),
+            float(scale), C.byref(h)))
+    elif noise and rng is not None:
         nr = 1 if psi.dim() == 1 else psi.shape[0]
         w = np.ascontiguousarray(
             np.full(nr, 1.0 / nr) if weights is None else weights,

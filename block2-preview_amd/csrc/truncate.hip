@@ -1065,11 +1065,21 @@ static int add_noise(b2g_ctx ctx, const b2g_sparse_info *wfn, uint64_t psi_len,
     return B2G_OK;
 }
 
+// Extra density-matrix terms: vectors of their own block layouts, in the
+// same device buffer as the wavefunction (at `offs`), each adding
+// v v^T (trace_right) or v^T v to the density-matrix sectors of its blocks.
+struct ExtraTerms {
+    int n = 0;
+    const b2g_sparse_info *infos = nullptr;
+    const uint64_t *offs = nullptr;
+};
+
 static int split_impl(b2g_ctx ctx, int symmetry, const b2g_sparse_info *wfn,
                       const double *psi, int n_roots, const double *weights,
                       int k, double cutoff, int trace_right, int normalize,
                       unsigned trunc_type, double scale, double noise,
-                      int noise_type, std::mt19937 *gen, b2g_split *out) {
+                      int noise_type, std::mt19937 *gen, b2g_split *out,
+                      const ExtraTerms &extra = ExtraTerms()) {
     B2G_REQUIRE(ctx && wfn && out && (psi || wfn->n == 0), B2G_ERR_ARG,
                 "NULL arg");
     B2G_REQUIRE(n_roots >= 1 && weights != nullptr, B2G_ERR_ARG,
@@ -1144,6 +1154,30 @@ static int split_impl(b2g_ctx ctx, int symmetry, const b2g_sparse_info *wfn,
         std::vector<std::vector<int>> by(nd);
         for (int i = 0; i < nw; i++)
             by[wfn_dm[i]].push_back(i);
+        // extra terms' blocks by density-matrix sector: (term, block)
+        std::vector<std::vector<std::pair<int, int>>> xby(nd);
+        for (int e = 0; e < extra.n; e++) {
+            const b2g_sparse_info &xi = extra.infos[e];
+            B2G_REQUIRE(xi.n >= 0 && (xi.n == 0 || (xi.quanta &&
+                                                    xi.n_states_bra &&
+                                                    xi.n_states_ket &&
+                                                    xi.block_shifts)),
+                        B2G_ERR_ARG, "bad extra term info");
+            for (int j = 0; j < xi.n; j++) {
+                const uint64_t q =
+                    trace_right
+                        ? q_get_bra(sy, xi.quanta[j], xi.delta_quantum)
+                        : q_neg(sy, q_get_ket(sy, xi.quanta[j]));
+                const int b = find_q(dm.quanta, q);
+                if (b < 0) // TraceProduct skips sectors the DM lacks
+                    continue;
+                const uint32_t d = trace_right ? xi.n_states_bra[j]
+                                               : xi.n_states_ket[j];
+                B2G_REQUIRE(d == dm.nb[b], B2G_ERR_ARG,
+                            "extra term block of another sector dimension");
+                xby[b].emplace_back(e, j);
+            }
+        }
         for (int b = 0; b < nd; b++) {
             const uint32_t sb = (uint32_t)ts.s.size();
             uint64_t K = 0;
@@ -1161,6 +1195,23 @@ static int split_impl(b2g_ctx ctx, int symmetry, const b2g_sparse_info *wfn,
                 } else { // psi^T psi: K = bra dimension
                     s.lda = nkt, s.ldb = nkt, s.K = nbr, s.flags = 1;
                 }
+                if (s.K == 0)
+                    continue;
+                K += s.K;
+                ts.s.push_back(s);
+            }
+            for (const auto &ej : xby[b]) {
+                const b2g_sparse_info &xi = extra.infos[ej.first];
+                const uint32_t nbr = xi.n_states_bra[ej.second],
+                               nkt = xi.n_states_ket[ej.second];
+                Seg s;
+                s.a_off = extra.offs[ej.first] + xi.block_shifts[ej.second];
+                s.b_off = s.a_off;
+                s.alpha = 1.0;
+                if (trace_right)
+                    s.lda = nkt, s.ldb = nkt, s.K = nkt, s.flags = 2;
+                else
+                    s.lda = nkt, s.ldb = nkt, s.K = nbr, s.flags = 1;
                 if (s.K == 0)
                     continue;
                 K += s.K;
@@ -1465,6 +1516,22 @@ extern "C" int b2g_split_density_matrix_multi(
     return split_impl(ctx, symmetry, wfn, psi, n_roots, weights, k, cutoff,
                       trace_right, normalize, trunc_type, scale, 0.0, 0,
                       nullptr, out);
+}
+
+extern "C" int b2g_split_density_matrix_extra(
+    b2g_ctx ctx, int symmetry, const b2g_sparse_info *wfn, const double *buf,
+    int n_extra, const b2g_sparse_info *extra_infos,
+    const uint64_t *extra_offsets, int k, double cutoff, int trace_right,
+    int normalize, unsigned trunc_type, double scale, b2g_split *out) {
+    B2G_REQUIRE(n_extra >= 0 && (n_extra == 0 || (extra_infos &&
+                                                   extra_offsets)),
+                B2G_ERR_ARG, "bad extra terms");
+    ExtraTerms ex;
+    ex.n = n_extra, ex.infos = extra_infos, ex.offs = extra_offsets;
+    const double one = 1.0;
+    return split_impl(ctx, symmetry, wfn, buf, 1, &one, k, cutoff,
+                      trace_right, normalize, trunc_type, scale, 0.0, 0,
+                      nullptr, out, ex);
 }
 
 extern "C" int b2g_split_n_roots(b2g_split s, int *n) {

@@ -528,6 +528,24 @@ int b2g_split_density_matrix(b2g_ctx ctx, int symmetry,
                              int k, double cutoff, int trace_right,
                              int normalize, unsigned trunc_type, double scale,
                              b2g_split *out);
+/* The split with perturbative noise (MovingEnvironment::density_matrix
+ * with pkets, src/dmrg/moving_environment.hpp:3554-3577; the perturbed
+ * wavefunctions of EffectiveHamiltonian::perturbative_noise,
+ * src/dmrg/effective_hamiltonian.hpp:263-433, already scaled as
+ * scale_perturbative_noise, :3698-3713, leaves them): the density matrix of
+ * the wavefunction at offset 0 of `buf` (times `scale`) plus, for each of
+ * the n_extra vectors of layout extra_infos[e] at buf + extra_offsets[e],
+ * v v^T (trace_right) or v^T v on the density-matrix sectors it shares with
+ * the wavefunction (OperatorFunctions::trans_product, operator_functions.hpp
+ * :776-835, skips the others); then as b2g_split_density_matrix.         */
+int b2g_split_density_matrix_extra(b2g_ctx ctx, int symmetry,
+                                   const b2g_sparse_info *wfn,
+                                   const double *buf, int n_extra,
+                                   const b2g_sparse_info *extra_infos,
+                                   const uint64_t *extra_offsets, int k,
+                                   double cutoff, int trace_right,
+                                   int normalize, unsigned trunc_type,
+                                   double scale, b2g_split *out);
 /* State-averaged form (MovingEnvironment::density_matrix_with_multi_target
  * + multi_split_density_matrix, src/dmrg/moving_environment.hpp:3582-3608,
  * 4347-4462): n_roots wavefunctions of the one structure `wfn`, stored one

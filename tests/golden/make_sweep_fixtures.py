@@ -68,6 +68,14 @@ CASES = {
     # noise: bond dimension 50, then 100, then 250
     "c2_su2": ("C2-N26-SEED2026.FCIDUMP", "su2", [50, 100, 250], [0], 3,
                []),
+    # perturbative noise (ReducedPerturbativeCollected, the Python
+    # driver's default): no random numbers, so noisy steps are comparable
+    "hub8_sz_pert": ("HUBBARD-L8.FCIDUMP", "sz", [100], [1e-5, 0], 4, [],
+                     0.0, "pert"),
+    "hub8_su2_pert": ("HUBBARD-L8.FCIDUMP", "su2", [100], [1e-5, 0], 4, [],
+                      0.0, "pert"),
+    "n2_su2_pert": ("N2.STO3G.FCIDUMP", "su2", [60], [1e-5, 0], 4, [],
+                    0.0, "pert"),
     "n2_su2": ("N2.STO3G.FCIDUMP", "su2", [60], [1e-6, 0], 6,
                [(0, [0, 1, 2, 3, 4, 5]), (1, [8, 7, 6, 5, 4, 3])]),
 }
@@ -75,7 +83,8 @@ FILES = ["lenv.bin", "renv.bin", "lblk.bin", "rblk.bin", "mps_left.bin",
          "mps_right.bin"]
 
 
-def run(fcidump, sym, bdims, noises, nsweeps, out, dump=None, thrd=0.0):
+def run(fcidump, sym, bdims, noises, nsweeps, out, dump=None, thrd=0.0,
+        noise_type="dm"):
     cmd = [B2REF, "--fcidump", fcidump, "--sym", sym, "--out", out,
            "--bdims", ",".join(map(str, bdims)),
            "--noises", ",".join(map(str, noises)),
@@ -83,6 +92,8 @@ def run(fcidump, sym, bdims, noises, nsweeps, out, dump=None, thrd=0.0):
            "--save-sweep", "1", "--seed", "1234", "--noise-seed", "4321"]
     if thrd:
         cmd += ["--dav-thrd", repr(thrd)]
+    if noise_type != "dm":
+        cmd += ["--noise-type", noise_type]
     if dump is not None:
         cmd += ["--dump-sweep", str(dump[0]),
                 "--dump-sites", ",".join(map(str, dump[1]))]
@@ -95,6 +106,7 @@ def main():
     for name in names:
         fd, sym, bdims, noises, ns, dumps = CASES[name][:6]
         thrd = CASES[name][6] if len(CASES[name]) > 6 else 0.0
+        ntype = CASES[name][7] if len(CASES[name]) > 7 else "dm"
         dst = os.path.join(HERE, "sweep_" + name)
         shutil.rmtree(dst, ignore_errors=True)
         os.makedirs(dst)
@@ -108,7 +120,7 @@ def main():
             else:
                 fcidump = os.path.join(REF, "data", fd)
             run(fcidump, sym, bdims, noises, ns, os.path.join(tmp, "clean"),
-                thrd=thrd)
+                thrd=thrd, noise_type=ntype)
             src = os.path.join(tmp, "clean", "sweep")
             for f in sorted(os.listdir(src)):
                 if f == "mpo.bin":
@@ -121,7 +133,7 @@ def main():
                 shutil.copy(os.path.join(src, f), os.path.join(dst, f))
             for k, dump in enumerate(dumps):
                 out = os.path.join(tmp, "dump%d" % k)
-                run(fcidump, sym, bdims, noises, ns, out, dump, thrd)
+                run(fcidump, sym, bdims, noises, ns, out, dump, thrd, ntype)
                 for site in dump[1]:
                     d = os.path.join(out, "site%d_sw%d" % (site, dump[0]))
                     e = os.path.join(dst, "env_s%d_%d" % (dump[0], site))
