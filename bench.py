@@ -110,7 +110,7 @@ def cpu_reference(pb, ldata, rdata, y_gpu_fn, threads):
                 gpu_fp64_vs_block2_rel=rel)
 
 
-C1_DIR = os.path.join(REPO, "tests", "golden", "sweep_hub16_sz")
+C1_DIR = os.path.join(REPO, "tests", "golden", "sweep_hub16_sz_pert")
 
 
 def _hubbard_fcidump(path, n, u, t):
@@ -128,7 +128,8 @@ def _hubbard_fcidump(path, n, u, t):
 
 def sweep_leg(ctx, threads, run_cpu):
     """Whole two-site sweeps of the L=16, U=4 Hubbard chain (SZ, M=200,
-    noise 1e-5 for four sweeps then none, Davidson thresholds as block2
+    noise 1e-5 for four sweeps then none, block2's default
+    ReducedPerturbativeCollected noise, Davidson thresholds as block2
     sets them) from block2's own MPO and starting MPS, through the sweep
     driver on the device: per-sweep seconds and the final energy against
     block2's.  Beside it, block2 itself (oracle/_ref/b2ref, MKL, `threads`
@@ -160,8 +161,8 @@ def sweep_leg(ctx, threads, run_cpu):
     ref = float(st["final"][0])
     steps = np.array([s[5] for s in dm.steps])
     out = dict(workload="two-site DMRG sweeps, 1D Hubbard L=16, U=4, t=1, "
-                        "half filling, SZ, M=200, noise 1e-5 x4 then 0, "
-                        "block2's MPO and starting MPS",
+                        "half filling, SZ, M=200, perturbative noise 1e-5 "
+                        "x4 then 0, block2's MPO and starting MPS",
                sweeps=len(dm.sweep_time),
                s_per_sweep=[round(t, 4) for t in dm.sweep_time],
                mean_s_per_sweep=round(float(np.mean(dm.sweep_time)), 4),
@@ -188,7 +189,8 @@ def sweep_leg(ctx, threads, run_cpu):
                        "--noises", ",".join(st["noises"]), "--nsweeps",
                        st["nsweeps"][0], "--threads", str(threads),
                        "--dav-conv", "1e-8", "--seed", "1234",
-                       "--noise-seed", st["noise_seed"][0]]
+                       "--noise-seed", st["noise_seed"][0],
+                       "--noise-type", st["noise_type"][0]]
                 t0 = time.perf_counter()
                 subprocess.run(cmd, check=True, stdout=subprocess.DEVNULL,
                                stderr=subprocess.DEVNULL, timeout=600)

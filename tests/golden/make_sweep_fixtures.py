@@ -63,6 +63,10 @@ CASES = {
     # HUBBARD-L16 has U=2)
     "hub16_sz": ("HUBBARD-L16-U4.FCIDUMP", "sz", [200],
                  [1e-5, 1e-5, 1e-5, 1e-5, 0], 10, []),
+    # the same with the Python driver's default noise
+    # (ReducedPerturbativeCollected)
+    "hub16_sz_pert": ("HUBBARD-L16-U4.FCIDUMP", "sz", [200],
+                      [1e-5, 1e-5, 1e-5, 1e-5, 0], 10, [], 0.0, "pert"),
     # the seeded synthetic ab initio Hamiltonian at n=26 orbitals, 26
     # electrons, singlet (make_fixtures.write_abinitio, seed 2026), without
     # noise: bond dimension 50, then 100, then 250

@@ -14,7 +14,7 @@ GOLDEN = os.path.join(REPO, "tests", "golden")
 SWEEP_CASES = {"hub8_sz": 0, "hub8_su2": 1, "hub8_sz_quiet": 0,
                "hub8_su2_quiet": 1, "hub16_sz": 0, "n2_su2": 1,
                "c2_su2": 1, "hub8_sz_pert": 0, "hub8_su2_pert": 1,
-               "n2_su2_pert": 1}
+               "n2_su2_pert": 1, "hub16_sz_pert": 0}
 
 
 # the truncation cases dumped at the sweep sites (split.npz): wavefunction

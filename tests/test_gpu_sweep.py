@@ -59,7 +59,7 @@ pytestmark = pytest.mark.gpu
 PLAIN = ["hub8_sz", "hub8_su2", "n2_su2"]
 QUIET = ["hub8_sz_quiet", "hub8_su2_quiet", "c2_su2"]
 # perturbative noise draws no random numbers: steps comparable one by one
-PERT = ["hub8_sz_pert", "hub8_su2_pert", "n2_su2_pert"]
+PERT = ["hub8_sz_pert", "hub8_su2_pert", "n2_su2_pert", "hub16_sz_pert"]
 NOISY = ["hub8_sz", "hub8_su2", "hub16_sz", "n2_su2"]
 # the L=16, U=4 Hubbard chain at bond dimension 200, ten sweeps: block2's
 # final energy of the same settings in its own fixture run
