@@ -182,6 +182,33 @@ class OpTensor:
             r.data[:self.total] = self.data[:self.total]
         return r
 
+    def signature(self):
+        """The structure a plan depends on: operators, layouts, offsets,
+        factors and bases."""
+        return (tuple(self.keys), tuple(self.dq), tuple(self.off),
+                np.asarray(self.factor, np.float64).tobytes(),
+                self.bra.q.tobytes(), self.bra.n.tobytes(),
+                self.ket.q.tobytes(), self.ket.n.tobytes(), self.total,
+                self.data.numel())
+
+    def structure(self):
+        """This tensor's structure without its data (a template)."""
+        r = object.__new__(OpTensor)
+        r.__dict__.update(self.__dict__)
+        r._device = self.data.device
+        r.data = None
+        return r
+
+    def empty_like(self):
+        """A tensor of this structure with zero data (of a fresh tensor's
+        length)."""
+        r = object.__new__(OpTensor)
+        r.__dict__.update(self.__dict__)
+        dev = self.data.device if self.data is not None else self._device
+        r.data = torch.zeros(max(self.total, 1), dtype=torch.float64,
+                             device=dev)
+        return r
+
     def subset(self, keys):
         """The operators `keys` of this tensor (same data, new index)."""
         r = object.__new__(OpTensor)
@@ -288,6 +315,7 @@ class DMRG:
             raise ValueError("precision: f64, f32 or bf16x3")
         self.precision = precision
         self.phase_time, self.davidson_macs = {}, 0
+        self._plans = {}
         self.ctx = ctx
         self.dev = ctx.device
         self.lib = L.lib()
@@ -369,7 +397,8 @@ class DMRG:
                             True)
 
     # ------------------------------------------------------ contractions
-    def _run_contract(self, A, B, Cs, bra_a, ket_a, bra_b, ket_b, terms):
+    def _run_contract(self, A, B, Cs, bra_a, ket_a, bra_b, ket_b, terms,
+                      keep_plan=False):
         keep = []
         d = L.b2g_contract()
         d.symmetry = self.sym
@@ -391,15 +420,96 @@ class DMRG:
         t = time.perf_counter()
         L.check(self.lib.b2g_contract_plan(self.ctx.h, C.byref(d), 0,
                                            C.byref(h)))
-        t = self._tick("blocking_plan", t)
+        self._tick("blocking_plan", t)
+        if keep_plan:
+            self._exec_winplan(h, A.data, B.data, Cs.data, 0, "blocking_run")
+            return h
         try:
-            torch.cuda.current_stream(self.dev).synchronize()
-            L.check(self.lib.b2g_winplan_run(h, _dev(A.data), _dev(B.data),
-                                             _dev(Cs.data), 0))
-            self.ctx.synchronize()
-            self._tick("blocking_run", t)
+            self._exec_winplan(h, A.data, B.data, Cs.data, 0, "blocking_run")
         finally:
             self.lib.b2g_winplan_destroy(h)
+
+    def _exec_winplan(self, h, a, b, c, accumulate, phase):
+        t = time.perf_counter()
+        torch.cuda.current_stream(self.dev).synchronize()
+        L.check(self.lib.b2g_winplan_run(h, _dev(a), None if b is None else
+                                         _dev(b), _dev(c), accumulate))
+        self.ctx.synchronize()
+        self._tick(phase, t)
+
+    def _cached(self, slot, sig):
+        ent = self._plans.get(slot)
+        return ent if ent is not None and ent["sig"] == sig else None
+
+    def _store(self, slot, ent):
+        old = self._plans.pop(slot, None)
+        if old is not None:
+            self._free_entry(old)
+        self._plans[slot] = ent
+
+    def _free_entry(self, ent):
+        for k in ("contract", "inter"):
+            if ent.get(k):
+                self.lib.b2g_winplan_destroy(ent[k])
+                ent[k] = None
+
+    def close(self):
+        """Releases the plans kept across sweeps."""
+        for ent in getattr(self, "_plans", {}).values():
+            self._free_entry(ent)
+        self._plans = {}
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _block_general(self, i, env, left):
+        """The contraction of environment and site i (left: env (x) site,
+        else site (x) env), its plans kept per side and site while the
+        environment's structure repeats (a sweep at a steady bond
+        dimension meets the same structures again)."""
+        s, m = self.sym, self.mpo
+        slot = ("L" if left else "R", i)
+        sig = env.signature()
+        site = self.site[i]
+        ent = self._cached(slot, sig)
+        if ent is not None:
+            Cs = ent["Cs"].empty_like()
+            src = env
+            if ent["inter"] is not None:
+                src = env.extend(ent["inter_keys"], ent["inter_dqs"])
+                self._exec_winplan(ent["inter"], src.data, None, src.data, 1,
+                                   "transform_run")
+            a, b = (src, site) if left else (site, src)
+            self._exec_winplan(ent["contract"], a.data, b.data, Cs.data, 0,
+                               "blocking_run")
+            return Cs
+        fused = self._fused_left(i) if left else self._fused_right(i)
+        side = "left" if left else "right"
+        names = m[side + "_operator_names"][i]["data"]
+        exprs = m[side + "_operator_exprs"][i]["data"]
+        live = [x for x in names if x[0] != "zero"]
+        Cs = OpTensor(s, fused, fused, [_key(x) for x in live],
+                      [x[4] for x in live], self.dev)
+        src, terms, inter = self._contraction_terms(env, site, Cs, names,
+                                                    exprs, left)
+        if left:
+            h = self._run_contract(src, site, Cs, src.bra, src.ket,
+                                   self.basis[i], self.basis[i], terms,
+                                   keep_plan=True)
+        else:
+            h = self._run_contract(site, src, Cs, self.basis[i],
+                                   self.basis[i], src.bra, src.ket, terms,
+                                   keep_plan=True)
+        ent = dict(sig=sig, Cs=Cs.structure(), contract=h, inter=None,
+                   inter_keys=None, inter_dqs=None)
+        if inter is not None:
+            ent.update(inter=inter[0], inter_keys=inter[1],
+                       inter_dqs=inter[2])
+        self._store(slot, ent)
+        return Cs
 
     def _vacuum_env(self):
         vac = Q.StateInfo([self.vacuum], [1])
@@ -410,6 +520,8 @@ class DMRG:
     def left_block(self, i, env):
         """Operators of (left environment of site i) (x) site i on the fused
         basis (MovingEnvironment::left_contract; site 0: left_assign)."""
+        if i != 0:
+            return self._block_general(i, env, True)
         s, m = self.sym, self.mpo
         fused = self._fused_left(i)
         names = m["left_operator_names"][i]["data"]
@@ -432,12 +544,7 @@ class DMRG:
             self._run_contract(A, site, Cs, vac, vac, self.basis[i],
                                self.basis[i], terms)
             return Cs
-        exprs = m["left_operator_exprs"][i]["data"]
-        env, terms = self._contraction_terms(env, site, Cs, names, exprs,
-                                             True)
-        self._run_contract(env, site, Cs, env.bra, env.ket, self.basis[i],
-                           self.basis[i], terms)
-        return Cs
+        return self._block_general(i, env, True)
 
     def _contraction_terms(self, env, site, Cs, names, exprs, left):
         """Window-sum terms (c, a, b, conj, factor) of a left (right)
@@ -467,30 +574,35 @@ class DMRG:
                     inters.append((ops[0], [(_key(o), int(bool(q)), o[3])
                                             for o, q in zip(ops, conjs)]))
                 rows.append((c, ek, site.index[sk], cj, f / nm[3]))
+        inter = None
         if inters:
-            env = self._intermediates(env, inters)
+            env, inter = self._intermediates(env, inters)
         terms = [(c, env.index[ek], sb, cj, f) if left else
                  (c, sb, env.index[ek], cj, f)
                  for c, ek, sb, cj, f in rows]
-        return env, terms
+        return env, terms, inter
 
     def _intermediates(self, env, inters):
         """env with the intermediate operators appended: operator j =
         sum of f op(env operator) over inters[j][1] (conj: transposed), in
-        the shape of the element inters[j][0] (its quantum)."""
-        out = env.extend([("sum", j) for j in range(len(inters))],
-                         [int(o[4]) for o, _ in inters])
+        the shape of the element inters[j][0] (its quantum); and the kept
+        plan with the appended keys and quanta."""
+        keys = [("sum", j) for j in range(len(inters))]
+        dqs = [int(o[4]) for o, _ in inters]
+        out = env.extend(keys, dqs)
         terms = []
         for j, (_, parts) in enumerate(inters):
             c = out.index[("sum", j)]
             for k, cj, f in parts:
                 terms.append((c, env.index[k], cj, f))
-        self._run_transform(out, terms)
-        return out
+        h = self._run_transform(out, terms, keep_plan=True)
+        return out, (h, keys, dqs)
 
     def right_block(self, i, env):
         """Operators of site i (x) (right environment of site i) on the
         fused basis (right_contract; last site: right_assign)."""
+        if i != self.n - 1:
+            return self._block_general(i, env, False)
         s, m = self.sym, self.mpo
         fused = self._fused_right(i)
         names = m["right_operator_names"][i]["data"]
@@ -511,12 +623,7 @@ class DMRG:
             self._run_contract(site, B, Cs, self.basis[i], self.basis[i], vac,
                                vac, terms)
             return Cs
-        exprs = m["right_operator_exprs"][i]["data"]
-        env, terms = self._contraction_terms(env, site, Cs, names, exprs,
-                                             False)
-        self._run_contract(site, env, Cs, self.basis[i], self.basis[i],
-                           env.bra, env.ket, terms)
-        return Cs
+        return self._block_general(i, env, False)
 
     def rotate(self, blk, rot, bond, trans):
         """Renormalised operators on `bond` (left_rotate: trans False,
@@ -582,11 +689,12 @@ class DMRG:
         self._run_transform(env, terms)
         return env.subset(keys)
 
-    def _run_transform(self, env, terms):
+    def _run_transform(self, env, terms, keep_plan=False):
         """env.c += f op(env.a) for the terms (c, a, conj, f), in place
-        (OperatorFunctions::iadd on the device)."""
+        (OperatorFunctions::iadd on the device); keep_plan: returns the
+        plan instead of releasing it."""
         if not terms:
-            return
+            return None
         keep = []
         ni, dq, inf, nop, ops = env.c_side(keep)
         tm = (L.b2g_transform_term * len(terms))(
@@ -597,15 +705,17 @@ class DMRG:
         L.check(self.lib.b2g_transform_plan(
             self.ctx.h, self.sym, ni, dq, inf, nop, ops,
             len(terms), tm, 0, C.byref(h)))
-        t = self._tick("transform_plan", t)
+        self._tick("transform_plan", t)
+        if keep_plan:
+            self._exec_winplan(h, env.data, None, env.data, 1,
+                               "transform_run")
+            return h
         try:
-            torch.cuda.current_stream(self.dev).synchronize()
-            L.check(self.lib.b2g_winplan_run(h, _dev(env.data), None,
-                                             _dev(env.data), 1))
-            self.ctx.synchronize()
-            self._tick("transform_run", t)
+            self._exec_winplan(h, env.data, None, env.data, 1,
+                               "transform_run")
         finally:
             self.lib.b2g_winplan_destroy(h)
+        return None
 
     # ------------------------------------------------- wavefunction moves
     def _contract_wfn(self, lmat, l_is_wfn, rmat, out_lay):

@@ -59,8 +59,12 @@ pytestmark = pytest.mark.gpu
 PLAIN = ["hub8_sz", "hub8_su2", "n2_su2"]
 QUIET = ["hub8_sz_quiet", "hub8_su2_quiet", "c2_su2"]
 # perturbative noise draws no random numbers: steps comparable one by one
-PERT = ["hub8_sz_pert", "hub8_su2_pert", "n2_su2_pert", "hub16_sz_pert"]
-NOISY = ["hub8_sz", "hub8_su2", "hub16_sz", "n2_su2"]
+PERT = ["hub8_sz_pert", "hub8_su2_pert", "n2_su2_pert"]
+# C1 with perturbative noise as well: at M = 200 the noise leaves nearly
+# degenerate density-matrix eigenvalues at the bond-dimension cut, so which
+# of them is kept is decided at the rounding level and the middle of the
+# trajectory parts (the final energy does not)
+NOISY = ["hub8_sz", "hub8_su2", "hub16_sz", "n2_su2", "hub16_sz_pert"]
 # the L=16, U=4 Hubbard chain at bond dimension 200, ten sweeps: block2's
 # final energy of the same settings in its own fixture run
 C1_ENERGY = -8.818767451952382
@@ -267,7 +271,7 @@ def test_noisy_sweeps_reach_block2_energy(gpu_ctx, case):
                                  for w in np.unique(got[:, 0])], flush=True)
     assert de[last].max() < 1e-8, de[last]
     assert abs(e - final) < 1e-8, (e, final)
-    if case == "hub16_sz":
+    if case.startswith("hub16_sz"):
         assert abs(e - C1_ENERGY) < 1e-8, e
     print("s/sweep", ["%.3f" % t for t in dm.sweep_time], flush=True)
 
