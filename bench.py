@@ -398,22 +398,50 @@ def main():
     eh.destroy(hp)
     rep = autotune.representative_dim(np.ctypeslib.as_array(parr))
     t_tune = time.time()
-    if args.rates:
-        rates = [float(r) for r in args.rates.split(",")]
-        L.check(L.lib().b2g_set_tile_rates((C.c_double * 9)(*rates), 9))
-    else:
-        rates = autotune.tune(ctx, dim=min(max(rep, 64), 512))
-    # split-BF16 tiles: the library's table measured on this workload's own
-    # sector histogram (uniform-block rates mispredict its small tiles)
-    rates_bf = None
-    if args.rates_bf16:
-        rates_bf = [float(r) for r in args.rates_bf16.split(",")]
-        L.check(L.lib().b2g_set_tile_rates_for(
-            L.PREC["bf16x3"], (C.c_double * 9)(*rates_bf), 9))
-    log("autotune at block dim %d (%.1fs): FP64 %s" % (
-        rep, time.time() - t_tune, " ".join("%.1f" % r for r in rates)))
     ldev, rdev = synth.fill_device(pb, dev)
     torch.cuda.synchronize()
+    gx = torch.Generator(device=dev).manual_seed(args.seed + 1)
+    xt = torch.randn((args.rhs, pb.ket_len), generator=gx,
+                     dtype=torch.float64, device=dev)
+    tuning = {}
+
+    def tables_for(prec, fixed):
+        """Rate tables to choose from for `prec`: the library's current
+        one, the uniform-block measurement (FP64) and the per-shape
+        measurement on this problem; the fastest product is installed."""
+        cur = (C.c_double * 9)()
+        L.check(L.lib().b2g_get_tile_rates_for(L.PREC[prec], cur, 9))
+        tabs = {"library": list(cur)}
+        if fixed is not None:
+            L.check(L.lib().b2g_set_tile_rates_for(
+                L.PREC[prec], (C.c_double * 9)(*fixed), 9))
+            return fixed, {"given": fixed}, None
+        if prec == "f64":
+            tabs["uniform_blocks"] = autotune.tune(
+                ctx, dim=min(max(rep, 64), 512), install=False)
+        tabs["this_problem"] = autotune.tune_on_problem(
+            ctx, pb, (ldev, rdev), prec, x=xt, install=False)
+        names = list(tabs)
+        best, times = autotune.choose(ctx, pb, (ldev, rdev),
+                                      [tabs[k] for k in names], prec, x=xt)
+        ms = {k: round(t, 4) for k, t in zip(names, times)}
+        return tabs[names[best]], tabs, dict(chosen=names[best],
+                                              ms_per_product=ms)
+
+    fixed = ([float(r) for r in args.rates.split(",")] if args.rates
+             else None)
+    rates, tabs64, ch64 = tables_for("f64", fixed)
+    fixed_bf = ([float(r) for r in args.rates_bf16.split(",")]
+                if args.rates_bf16 else None)
+    rates_bf, tabsbf, chbf = tables_for("bf16x3", fixed_bf)
+    tuning = {"f64": dict(tables={k: [round(r, 2) for r in v]
+                                  for k, v in tabs64.items()}, choice=ch64),
+              "bf16x3": dict(tables={k: [round(r, 2) for r in v]
+                                     for k, v in tabsbf.items()},
+                             choice=chbf)}
+    log("tiling (%.1fs): FP64 %s, split-BF16 %s" % (
+        time.time() - t_tune, ch64, chbf))
+    del xt
     h = EffectiveHamiltonian(pb, ctx, fp32=True, device_data=(ldev, rdev))
     log("setup %.1fs: |psi| %d, pairs %d, MACs/H*psi %.4g, terms %d" % (
         time.time() - t0, h.ket_len, h.n_pairs, h.macs, len(pb.term_conj)))
@@ -703,8 +731,8 @@ def main():
         "kernels_fp64": kernels,
         "autotune": {"block_dim": rep, "tile_rates_tflops":
                      [round(r, 2) for r in rates],
-                     "tile_rates_bf16x3": (None if rates_bf is None else
-                                           [round(r, 2) for r in rates_bf]),
+                     "tile_rates_bf16x3": [round(r, 2) for r in rates_bf],
+                     "measured": tuning,
                      "fixed_64x64_tflops": round(tflops_fixed, 4),
                      "speedup_vs_fixed": round(tflops / tflops_fixed, 3)},
         "fp32": {"value": round(tflops32, 4), "unit": "TFLOP/s",

@@ -180,6 +180,7 @@ _SIGS = {
     "b2g_effham_plan": (_vp, [_vp]),
     "b2g_set_tile_rates": (C.c_int, [f64p, C.c_int]),
     "b2g_set_tile_rates_for": (C.c_int, [C.c_int, f64p, C.c_int]),
+    "b2g_get_tile_rates_for": (C.c_int, [C.c_int, f64p, C.c_int]),
     "b2g_ctx_num_cu": (C.c_int, [_vp, C.POINTER(C.c_int)]),
     "b2g_buf_create": (C.c_int, [_vp, C.c_size_t, C.c_int, C.POINTER(_vp)]),
     "b2g_buf_destroy": (C.c_int, [_vp]),

@@ -1,13 +1,23 @@
 """Tile autotuner for the grouped GEMM.
 
 Measures the throughput of the nine workgroup tile shapes (TM, TN in
-{32, 64, 128}) of the FP64 kernels, or of the split-BF16 kernels, on uniform
-block problems whose block size is the multiply-add-weighted median
-output-block size of the problem to be solved (its sector-size histogram),
-and installs the rates with b2g_set_tile_rates_for, so that the tiling of
-every plan uploaded afterwards (FP64 / FP32 kernels), or of every packed
-split-BF16 plan built afterwards, minimises its estimated time.  Measurement goes through the thin C ABI with
-HIP events on the library's stream.
+{32, 64, 128}) of the FP64 kernels, or of the split-BF16 kernels, and
+installs the rates with b2g_set_tile_rates_for, so that the tiling of every
+plan uploaded afterwards (FP64 / FP32 kernels), or of every packed
+split-BF16 plan built afterwards, minimises its estimated time.  Two
+measurements:
+
+* `tune`: uniform block problems whose block size is the
+  multiply-add-weighted median output-block size of the problem to be
+  solved;
+* `tune_on_problem`: the problem itself, tiled with one shape at a time,
+  each shape's launches timed with per-kernel HIP events: the rate a shape
+  reaches on this sector-size histogram (shallow segments and ragged edges
+  included).
+
+`choose` times the problem's product under each candidate rate table and
+installs the faster.  Measurement goes through the C ABI with HIP events
+on the library's stream.
 """
 import ctypes as C
 import os
@@ -97,3 +107,113 @@ def tune(ctx, dim=256, install=True, precision="f64"):
         arr = (C.c_double * 9)(*rates)
         L.check(L.lib().b2g_set_tile_rates_for(prec, arr, 9))
     return rates
+
+
+def _forced(shape):
+    """Context manager: B2G_TILE forced to `shape` (None: unset)."""
+    class _F:
+        def __enter__(self):
+            self.old = os.environ.get("B2G_TILE")
+            if shape is None:
+                os.environ.pop("B2G_TILE", None)
+            else:
+                os.environ["B2G_TILE"] = shape
+            return self
+
+        def __exit__(self, *a):
+            if self.old is None:
+                os.environ.pop("B2G_TILE", None)
+            else:
+                os.environ["B2G_TILE"] = self.old
+    return _F()
+
+
+def _product_ms(h, x, precision, reps):
+    """Average ms of `reps` products after one warm-up (which builds the
+    packed split-BF16 plan)."""
+    lib = L.lib()
+    stream = torch.cuda.ExternalStream(lib.b2g_ctx_stream(h.ctx.h))
+    h.multiply(x, precision=precision)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    y = torch.empty((x.shape[0], h.bra_len), dtype=torch.float64,
+                    device=x.device)
+    e0.record(stream)
+    for _ in range(reps):
+        L.check(lib.b2g_effham_hpsi(h.h, x.data_ptr(), y.data_ptr(),
+                                    x.shape[0], 1.0, 0, L.PREC[precision]))
+    e1.record(stream)
+    e1.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def tune_on_problem(ctx, problem, device_data, precision="f64", x=None,
+                    reps=2, install=True):
+    """Per-shape throughput (TFLOP/s, FP64-equivalent) of the nine tile
+    shapes on `problem` itself: the product tiled with one shape at a time,
+    the shape's phase-1 and phase-2 launches timed with per-kernel events;
+    rate = their algorithmic FLOPs / their time.  x: (n_rhs, |psi|) device
+    vectors (default one seeded vector)."""
+    from .effham import EffectiveHamiltonian
+    rates = []
+    for cfg in range(9):
+        shape = "%dx%d" % (32 << (cfg // 3), 32 << (cfg % 3))
+        with _forced(shape):
+            h = EffectiveHamiltonian(problem, ctx,
+                                     fp32=(precision == "f32"),
+                                     device_data=device_data)
+            try:
+                if x is None:
+                    g = torch.Generator(device=ctx.device).manual_seed(7)
+                    x = torch.randn((1, h.ket_len), generator=g,
+                                    dtype=torch.float64, device=ctx.device)
+                h.multiply(x, precision=precision)   # packs while forced
+                h.set_timing(True)
+                for _ in range(reps):
+                    h.multiply(x, precision=precision)
+                ms, n = h.timing()
+                h.set_timing(False)
+                fl = h.kernel_flops(precision)
+            finally:
+                h.deallocate()
+        k1, k2 = L.K_PHASE1 + cfg, L.K_PHASE2 + cfg
+        t = (ms[k1] + ms[k2]) * 1e-3
+        f = (fl[k1] + fl[k2]) * x.shape[0] * reps
+        rates.append(f / t / 1e12 if t > 0 and f > 0 else 0.0)
+    # a shape without work on this histogram gets the slowest measured rate
+    lo = min([r for r in rates if r > 0] or [1.0])
+    rates = [r if r > 0 else lo for r in rates]
+    if install:
+        L.check(L.lib().b2g_set_tile_rates_for(
+            L.PREC[precision], (C.c_double * 9)(*rates), 9))
+    return rates
+
+
+def choose(ctx, problem, device_data, tables, precision="f64", x=None,
+           reps=3):
+    """Times the product of `problem` tiled with each rate table of
+    `tables` and installs the fastest; returns (index, ms per product of
+    every table)."""
+    from .effham import EffectiveHamiltonian
+    lib = L.lib()
+    times = []
+    for rates in tables:
+        L.check(lib.b2g_set_tile_rates_for(L.PREC[precision],
+                                           (C.c_double * 9)(*rates), 9))
+        with _forced(None):
+            h = EffectiveHamiltonian(problem, ctx,
+                                     fp32=(precision == "f32"),
+                                     device_data=device_data)
+            try:
+                if x is None:
+                    g = torch.Generator(device=ctx.device).manual_seed(7)
+                    x = torch.randn((1, h.ket_len), generator=g,
+                                    dtype=torch.float64, device=ctx.device)
+                times.append(_product_ms(h, x, precision, reps))
+            finally:
+                h.deallocate()
+    best = min(range(len(tables)), key=lambda i: times[i])
+    L.check(lib.b2g_set_tile_rates_for(L.PREC[precision],
+                                       (C.c_double * 9)(*tables[best]), 9))
+    return best, times
+

@@ -1074,6 +1074,18 @@ extern "C" int b2g_plan_destroy(b2g_plan p) {
     return B2G_OK;
 }
 
+extern "C" int b2g_get_tile_rates_for(int precision, double *rates, int n) {
+    B2G_REQUIRE(rates != nullptr && n == kTileConfigs, B2G_ERR_ARG,
+                "b2g_get_tile_rates_for: expects room for 9 rates");
+    B2G_REQUIRE(precision == B2G_PREC_F64 || precision == B2G_PREC_F32 ||
+                    precision == B2G_PREC_BF16X3,
+                B2G_ERR_ARG, "unsupported precision");
+    const int t = precision == B2G_PREC_BF16X3 ? 1 : 0;
+    for (int i = 0; i < n; i++)
+        rates[i] = g_rates[t][i];
+    return B2G_OK;
+}
+
 extern "C" int b2g_set_tile_rates_for(int precision, const double *rates,
                                       int n) {
     B2G_REQUIRE(rates != nullptr && n == kTileConfigs, B2G_ERR_ARG,
@@ -1084,6 +1096,11 @@ extern "C" int b2g_set_tile_rates_for(int precision, const double *rates,
     for (int i = 0; i < n; i++)
         B2G_REQUIRE(rates[i] > 0, B2G_ERR_ARG, "rates must be positive");
     const int t = precision == B2G_PREC_BF16X3 ? 1 : 0;
+    bool same = true;
+    for (int i = 0; i < n; i++)
+        same = same && g_rates[t][i] == rates[i];
+    if (same) // the tilings memoised for this table stay valid
+        return B2G_OK;
     for (int i = 0; i < n; i++)
         g_rates[t][i] = rates[i];
     g_rates_gen++;

@@ -204,6 +204,8 @@ int b2g_set_tile_rates(const double *rates, int n);
  * share the table above; B2G_PREC_BF16X3 has its own, with which the packed
  * split-BF16 plan is tiled when it is built.                              */
 int b2g_set_tile_rates_for(int precision, const double *rates, int n);
+/* The rate table currently installed for the kernels of `precision`.     */
+int b2g_get_tile_rates_for(int precision, double *rates, int n);
 /* Number of kernel tasks per phase and scratch size, for reporting.      */
 int b2g_plan_stats(b2g_plan plan, uint64_t *n_tasks1, uint64_t *n_tasks2,
                    uint64_t *scratch_elems);
