@@ -3,7 +3,6 @@
 #include "kernels.hpp"
 #include "launch.hpp"
 
-#include <type_traits>
 
 namespace b2g {
 
@@ -43,19 +42,23 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void *p) {
 }
 constexpr uint32_t kMasked = 0x80000000u; // offset past the record count
 
+// buffer load at byte offset off + soff (off past the record count: zero)
 template <typename T>
-__device__ __forceinline__ T buf_load(__amdgpu_buffer_rsrc_t r, uint32_t off);
+__device__ __forceinline__ T buf_load_s(__amdgpu_buffer_rsrc_t r, uint32_t off,
+                                        uint32_t soff);
 template <>
-__device__ __forceinline__ double buf_load<double>(__amdgpu_buffer_rsrc_t r,
-                                                   uint32_t off) {
-    return __builtin_bit_cast(double,
-                              __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+__device__ __forceinline__ double buf_load_s<double>(__amdgpu_buffer_rsrc_t r,
+                                                     uint32_t off,
+                                                     uint32_t soff) {
+    return __builtin_bit_cast(
+        double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, (int)soff, 0));
 }
 template <>
-__device__ __forceinline__ float buf_load<float>(__amdgpu_buffer_rsrc_t r,
-                                                 uint32_t off) {
-    return __builtin_bit_cast(float,
-                              __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+__device__ __forceinline__ float buf_load_s<float>(__amdgpu_buffer_rsrc_t r,
+                                                   uint32_t off,
+                                                   uint32_t soff) {
+    return __builtin_bit_cast(
+        float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, (int)soff, 0));
 }
 
 // One workgroup = 4 waves in a 2x2 arrangement; the workgroup tile is
@@ -63,84 +66,20 @@ __device__ __forceinline__ float buf_load<float>(__amdgpu_buffer_rsrc_t r,
 // K is staged through two LDS buffers in chunks of TK: while the MFMAs run on
 // one buffer, the next chunk's global loads are in flight in registers and
 // are then written to the other buffer, so one barrier per chunk suffices.
-// Loads are buffer loads at 32-bit byte offsets from the segment's blocks;
-// elements outside the block (rows past M, columns past N, k past K) use an
-// offset past the buffer's record count and read as zero, so nothing is
-// masked when the chunk is written to LDS (where the per-segment alpha is
-// applied).  Element q of thread tid is e = tid + 256 q; since 256 is a
-// multiple of TK, TM and TN, one coordinate of e is the same for all q and
-// the other advances by a constant, so offsets advance by a uniform step.
-// Tasks are strided over the grid (persistent launch); blockIdx.y selects
-// the right-hand side.
+// Element q of thread tid is e = tid + 256 q; since 256 is a multiple of TK,
+// TM and TN, one coordinate of e is the same for all q and the other
+// advances by a constant.  Tasks are strided over the grid (persistent
+// launch); blockIdx.y selects the right-hand side.
+//
+// The registers of one chunk and their store to LDS ([k][m] and [k][n]
+// layouts, one padding column), the per-segment alpha applied there.
 template <typename TA, typename TB, typename TC, int TM, int TN, int TK>
-struct TileLoaderB {
+struct TileRegs {
     static constexpr int NA = TM * TK / 256, NB = TN * TK / 256;
     static_assert(256 % TK == 0 && 256 % TM == 0 && 256 % TN == 0,
                   "tile shape");
     TA ra[NA];
     TB rb[NB];
-    __device__ __forceinline__ void load(const TA *__restrict__ A,
-                                         const TB *__restrict__ B,
-                                         const Seg &sg, uint32_t k0,
-                                         const Task &task, int tid) {
-        const bool ta = sg.flags & 1, tb = (sg.flags >> 1) & 1;
-        const __amdgpu_buffer_rsrc_t rA = buf_rsrc(A + sg.a_off);
-        const __amdgpu_buffer_rsrc_t rB = buf_rsrc(B + sg.b_off);
-        if (!ta) { // A stored M x K: element (m, k) at m * lda + k
-            constexpr uint32_t DM = 256 / TK;
-            const uint32_t k = k0 + tid % TK;
-            uint32_t r = task.row0 + tid / TK;
-            uint32_t off = (r * sg.lda + k) * (uint32_t)sizeof(TA);
-            const uint32_t step = DM * sg.lda * (uint32_t)sizeof(TA);
-            const bool kok = k < sg.K;
-#pragma unroll
-            for (int q = 0; q < NA; q++) {
-                ra[q] = buf_load<TA>(rA, (kok && r < task.M) ? off : kMasked);
-                r += DM;
-                off += step;
-            }
-        } else { // A stored K x M: element (m, k) at k * lda + m
-            constexpr uint32_t DK = 256 / TM;
-            const uint32_t m = task.row0 + tid % TM;
-            uint32_t k = k0 + tid / TM;
-            uint32_t off = (k * sg.lda + m) * (uint32_t)sizeof(TA);
-            const uint32_t step = DK * sg.lda * (uint32_t)sizeof(TA);
-            const bool mok = m < task.M;
-#pragma unroll
-            for (int q = 0; q < NA; q++) {
-                ra[q] = buf_load<TA>(rA, (mok && k < sg.K) ? off : kMasked);
-                k += DK;
-                off += step;
-            }
-        }
-        if (!tb) { // B stored K x N: element (k, n) at k * ldb + n
-            constexpr uint32_t DK = 256 / TN;
-            const uint32_t n = task.col0 + tid % TN;
-            uint32_t k = k0 + tid / TN;
-            uint32_t off = (k * sg.ldb + n) * (uint32_t)sizeof(TB);
-            const uint32_t step = DK * sg.ldb * (uint32_t)sizeof(TB);
-            const bool nok = n < task.N;
-#pragma unroll
-            for (int q = 0; q < NB; q++) {
-                rb[q] = buf_load<TB>(rB, (nok && k < sg.K) ? off : kMasked);
-                k += DK;
-                off += step;
-            }
-        } else { // B stored N x K: element (k, n) at n * ldb + k
-            constexpr uint32_t DN = 256 / TK;
-            const uint32_t k = k0 + tid % TK;
-            uint32_t c = task.col0 + tid / TK;
-            uint32_t off = (c * sg.ldb + k) * (uint32_t)sizeof(TB);
-            const uint32_t step = DN * sg.ldb * (uint32_t)sizeof(TB);
-            const bool kok = k < sg.K;
-#pragma unroll
-            for (int q = 0; q < NB; q++) {
-                rb[q] = buf_load<TB>(rB, (kok && c < task.N) ? off : kMasked);
-                c += DN;
-                off += step;
-            }
-        }
-    }
     // registers -> LDS buffer ([k][m] and [k][n] layouts), alpha applied;
     // per layout, element q sits at a per-thread base plus a constant
     __device__ __forceinline__ void store(TC *As, TC *Bs, const Seg &sg,
@@ -172,85 +111,108 @@ struct TileLoaderB {
     }
 };
 
-// The earlier loader: 64-bit global addresses, out-of-block elements read
-// the block's first element and are zeroed through a bit mask when written
-// to LDS.  Measured faster than the buffer loader for 128-row A tiles,
-// where the buffer loader's offsets share registers with the previous
-// chunk's loads and wait on them; slower for the smaller tiles.
+// Buffer loads at 32-bit byte offsets from the segment's blocks, with
+// segment-invariant offsets: at the first chunk of a segment every
+// element's byte offset at k0 = 0 is computed once (rows or columns outside
+// the block get an offset past the buffer's record count and read as zero);
+// each chunk then passes its k0 as the instruction's scalar offset, so a
+// full chunk costs no vector arithmetic before its loads.  Elements at
+// k >= K (the last, partial chunk of a segment) get the masked offset per
+// element.  Measured on the n=40 M=3000 middle site against 64-bit
+// addresses recomputed per chunk: FP64 H*psi 54.3 -> 48.0 ms per 4-RHS step,
+// its 128 x 128 phase-2 launch 17.5 -> 14.9 ms.
 template <typename TA, typename TB, typename TC, int TM, int TN, int TK>
-struct TileLoaderG {
-    static constexpr int NA = TM * TK / 256, NB = TN * TK / 256;
-    static_assert(NA + NB <= 32, "mask bits");
-    TA ra[NA];
-    TB rb[NB];
-    uint32_t okmask; // bit q: element q in range (applied at store time)
+struct TileLoaderS : TileRegs<TA, TB, TC, TM, TN, TK> {
+    using Base = TileRegs<TA, TB, TC, TM, TN, TK>;
+    static constexpr int NA = Base::NA, NB = Base::NB;
+    uint32_t va[NA], vb[NB]; // byte offsets at k0 = 0 (kMasked: outside)
+    uint32_t sa, sb;         // bytes per unit of k0
+    __amdgpu_buffer_rsrc_t rA, rB;
+    __device__ __forceinline__ void setup(const TA *__restrict__ A,
+                                          const TB *__restrict__ B,
+                                          const Seg &sg, const Task &task,
+                                          int tid) {
+        const bool ta = sg.flags & 1, tb = (sg.flags >> 1) & 1;
+        rA = buf_rsrc(A + sg.a_off);
+        rB = buf_rsrc(B + sg.b_off);
+        if (!ta) { // (m, k) at m * lda + k; k = tid % TK
+            const uint32_t k = tid % TK;
+            uint32_t r = task.row0 + tid / TK;
+#pragma unroll
+            for (int q = 0; q < NA; q++) {
+                va[q] = r < task.M ? (r * sg.lda + k) * (uint32_t)sizeof(TA)
+                                   : kMasked;
+                r += 256 / TK;
+            }
+            sa = (uint32_t)sizeof(TA);
+        } else { // (m, k) at k * lda + m; m = tid % TM
+            const uint32_t m = task.row0 + tid % TM;
+            uint32_t k = tid / TM;
+#pragma unroll
+            for (int q = 0; q < NA; q++) {
+                va[q] = m < task.M ? (k * sg.lda + m) * (uint32_t)sizeof(TA)
+                                   : kMasked;
+                k += 256 / TM;
+            }
+            sa = sg.lda * (uint32_t)sizeof(TA);
+        }
+        if (!tb) { // (k, n) at k * ldb + n; n = tid % TN
+            const uint32_t n = task.col0 + tid % TN;
+            uint32_t k = tid / TN;
+#pragma unroll
+            for (int q = 0; q < NB; q++) {
+                vb[q] = n < task.N ? (k * sg.ldb + n) * (uint32_t)sizeof(TB)
+                                   : kMasked;
+                k += 256 / TN;
+            }
+            sb = sg.ldb * (uint32_t)sizeof(TB);
+        } else { // (k, n) at n * ldb + k; k = tid % TK
+            const uint32_t k = tid % TK;
+            uint32_t c = task.col0 + tid / TK;
+#pragma unroll
+            for (int q = 0; q < NB; q++) {
+                vb[q] = c < task.N ? (c * sg.ldb + k) * (uint32_t)sizeof(TB)
+                                   : kMasked;
+                c += 256 / TK;
+            }
+            sb = (uint32_t)sizeof(TB);
+        }
+        sa = __builtin_amdgcn_readfirstlane(sa);
+        sb = __builtin_amdgcn_readfirstlane(sb);
+    }
     __device__ __forceinline__ void load(const TA *__restrict__ A,
                                          const TB *__restrict__ B,
                                          const Seg &sg, uint32_t k0,
                                          const Task &task, int tid) {
+        if (k0 == 0)
+            setup(A, B, sg, task, tid);
+        const uint32_t soa = __builtin_amdgcn_readfirstlane(k0 * sa);
+        const uint32_t sob = __builtin_amdgcn_readfirstlane(k0 * sb);
+        if (k0 + TK <= sg.K) {
+#pragma unroll
+            for (int q = 0; q < NA; q++)
+                this->ra[q] = buf_load_s<TA>(rA, va[q], soa);
+#pragma unroll
+            for (int q = 0; q < NB; q++)
+                this->rb[q] = buf_load_s<TB>(rB, vb[q], sob);
+            return;
+        }
+        // partial chunk: the k of element q, per storage order
         const bool ta = sg.flags & 1, tb = (sg.flags >> 1) & 1;
-        uint32_t mask = 0;
 #pragma unroll
         for (int q = 0; q < NA; q++) {
-            const int e = tid + q * 256;
-            // (m, k) of this element for either storage order of A
-            const int m = ta ? e % TM : e / TK, k = ta ? e / TM : e % TK;
-            const uint32_t r = task.row0 + m, kk = k0 + k;
-            const bool ok = r < task.M && kk < sg.K;
-            const uint64_t idx =
-                ok ? sg.a_off + (ta ? (uint64_t)kk * sg.lda + r
-                                    : (uint64_t)r * sg.lda + kk)
-                   : sg.a_off;
-            ra[q] = A[idx];
-            mask |= (uint32_t)ok << q;
+            const uint32_t k =
+                k0 + (ta ? tid / TM + q * (256 / TM) : tid % TK);
+            this->ra[q] = buf_load_s<TA>(rA, k < sg.K ? va[q] : kMasked, soa);
         }
 #pragma unroll
         for (int q = 0; q < NB; q++) {
-            const int e = tid + q * 256;
-            const int n = tb ? e / TK : e % TN, k = tb ? e % TK : e / TN;
-            const uint32_t c = task.col0 + n, kk = k0 + k;
-            const bool ok = c < task.N && kk < sg.K;
-            const uint64_t idx =
-                ok ? sg.b_off + (tb ? (uint64_t)c * sg.ldb + kk
-                                    : (uint64_t)kk * sg.ldb + c)
-                   : sg.b_off;
-            rb[q] = B[idx];
-            mask |= (uint32_t)ok << (NA + q);
-        }
-        okmask = mask;
-    }
-    // registers -> LDS buffer ([k][m] and [k][n] layouts), alpha applied
-    __device__ __forceinline__ void store(TC *As, TC *Bs, const Seg &sg,
-                                          int tid) {
-        const bool ta = sg.flags & 1, tb = (sg.flags >> 1) & 1;
-        const TC alpha = (TC)sg.alpha;
-#pragma unroll
-        for (int q = 0; q < NA; q++) {
-            const int e = tid + q * 256;
-            const int m = ta ? e % TM : e / TK, k = ta ? e / TM : e % TK;
-            As[k * (TM + 1) + m] =
-                ((okmask >> q) & 1) ? (TC)ra[q] * alpha : (TC)0;
-        }
-#pragma unroll
-        for (int q = 0; q < NB; q++) {
-            const int e = tid + q * 256;
-            const int n = tb ? e / TK : e % TN, k = tb ? e % TK : e / TN;
-            Bs[k * (TN + 1) + n] =
-                ((okmask >> (NA + q)) & 1) ? (TC)rb[q] : (TC)0;
+            const uint32_t k =
+                k0 + (tb ? tid % TK : tid / TN + q * (256 / TN));
+            this->rb[q] = buf_load_s<TB>(rB, k < sg.K ? vb[q] : kMasked, sob);
         }
     }
 };
-
-// the 64-bit loader for the 128-row tiles, except the 128 x 32 and 128 x 64
-// tiles of phase 1 (W = C op(B)), which run faster on the buffer loader
-// (measured on the n=40 M=3000 middle site: 1.31 -> 1.19 and 2.37 -> 2.09 ms
-// per launch; the same tiles of phase 2 run slower on it)
-template <typename TA, typename TB, typename TC, int TM, int TN, int TK,
-          int PHASE>
-using TileLoader = typename std::conditional<
-    TM == 128 && (TN == 128 || PHASE != 1),
-    TileLoaderG<TA, TB, TC, TM, TN, TK>,
-    TileLoaderB<TA, TB, TC, TM, TN, TK>>::type;
 
 template <typename TA, typename TB, typename TD, typename TC, int TM, int TN,
           int TK, int PHASE>
@@ -272,7 +234,7 @@ __global__ __launch_bounds__(256) void seg_gemm_kernel(
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
     const int lr = lane & 15, lk = lane >> 4;
-    TileLoader<TA, TB, TC, TM, TN, TK, PHASE> ld;
+    TileLoaderS<TA, TB, TC, TM, TN, TK> ld;
     int t0 = blockIdx.x, tend = ntasks, tstep = gridDim.x;
     if (xs.off[8] != 0) {
         const int x = blockIdx.x & 7;
