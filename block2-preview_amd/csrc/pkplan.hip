@@ -18,7 +18,9 @@
 #include "replay.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <memory>
 #include <thread>
@@ -151,8 +153,6 @@ static int pk_build_into(b2g_plan p, PkPlan *k) {
     for (size_t ic = 0; ic < p->chunks.size(); ic++) {
         const Chunk &ch = p->chunks[ic];
         HostChunk &h = hc[ic];
-        h.s1.reserve(ch.s1.size());
-        h.s2.reserve(ch.s2.size());
         // rows / columns of each segment's output block
         std::vector<uint32_t> m1(ch.s1.size(), 0), n1(ch.s1.size(), 0);
         std::vector<uint64_t> w1(ch.s1.size(), 0);
@@ -189,79 +189,91 @@ static int pk_build_into(b2g_plan p, PkPlan *k) {
                 std::make_pair(wlen, ldw);
             wlen += (uint64_t)n1[si] * ldw;
         }
-        // The phase-1 operand lists (psi windows and right operators
-        // packed) and the phase-2 ones (left operators) share only the W
-        // layout above, read-only: phase 1's run on a second host thread
-        // while this one builds phase 2's and tiles both phases (the tiling
-        // memo is per host thread and stays warm on this one).
-        const char *err1 = nullptr;
-        std::thread th1([&]() {
+        // Five independent parts on the host workers: the packed lists of
+        // the wavefunction windows, of the right operators (phase 1) and of
+        // the left operators (phase 2), and the tiles of either phase.  They
+        // share only read-only inputs; each packer is used by one part, so
+        // the packed layouts are those of a serial pass.  The tiles depend
+        // on the segments' depths only, which packing keeps.
+        std::vector<int64_t> xa(ch.s1.size()), rb(ch.s1.size());
+        const char *err[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+        std::vector<std::function<void()>> jobs;
+        jobs.emplace_back([&]() {
             for (size_t si = 0; si < ch.s1.size(); si++) {
                 const Seg &s = ch.s1[si];
-                const int64_t a = X.add(s.a_off, s.lda, false, m1[si], s.K);
-                const int64_t b =
-                    R.add(s.b_off, s.ldb, !(s.flags & 2), n1[si], s.K);
-                if (a < 0 || b < 0) {
-                    err1 = "packed plan: phase-1 operand outside its buffer";
+                xa[si] = X.add(s.a_off, s.lda, false, m1[si], s.K);
+                if (xa[si] < 0) {
+                    err[0] = "packed plan: phase-1 operand outside its buffer";
                     return;
                 }
-                Seg q = s;
-                q.a_off = (uint64_t)a, q.lda = (uint32_t)round8(s.K);
-                q.b_off = (uint64_t)b, q.ldb = (uint32_t)round8(s.K);
-                q.flags = 0, q.alpha = 1.0;
-                h.s1.push_back(q);
             }
         });
-        auto phase2 = [&]() -> const char * {
+        jobs.emplace_back([&]() {
+            for (size_t si = 0; si < ch.s1.size(); si++) {
+                const Seg &s = ch.s1[si];
+                rb[si] = R.add(s.b_off, s.ldb, !(s.flags & 2), n1[si], s.K);
+                if (rb[si] < 0) {
+                    err[1] = "packed plan: phase-1 operand outside its buffer";
+                    return;
+                }
+            }
+        });
+        jobs.emplace_back([&]() {
+            h.s2.reserve(ch.s2.size());
             for (size_t si = 0; si < ch.s2.size(); si++) {
                 const Seg &s = ch.s2[si];
                 const std::pair<uint64_t, uint32_t> *wp = wmap.find(s.b_off);
-                if (wp == nullptr || wp->second != round8(s.K))
-                    return "packed plan: phase-2 W not found";
-                if (m2[si] == 0)
-                    return "packed plan: phase-2 segment without a task";
+                if (wp == nullptr || wp->second != round8(s.K)) {
+                    err[2] = "packed plan: phase-2 W not found";
+                    return;
+                }
+                if (m2[si] == 0) {
+                    err[2] = "packed plan: phase-2 segment without a task";
+                    return;
+                }
                 const int64_t a =
                     L.add(s.a_off, s.lda, s.flags & 1, m2[si], s.K);
-                if (a < 0)
-                    return "packed plan: phase-2 operand outside its buffer";
+                if (a < 0) {
+                    err[2] = "packed plan: phase-2 operand outside its buffer";
+                    return;
+                }
                 Seg q = s;
                 q.a_off = (uint64_t)a, q.lda = (uint32_t)round8(s.K);
                 q.b_off = wp->first, q.ldb = wp->second;
                 q.flags = 0;
                 h.s2.push_back(q);
             }
-            // one W^T block per phase-1 segment, tiled with the split-BF16
-            // rates (segment K from the plan's own segments)
+        });
+        // one W^T block per phase-1 segment, tiled with the split-BF16
+        // rates (segment K from the plan's own segments)
+        jobs.emplace_back([&]() {
+            tile_env_refresh();
             for (size_t si = 0; si < ch.s1.size(); si++)
                 for (const Tile &t :
                      tile_cover(m1[si], n1[si], ch.s1[si].K, 1))
                     h.t1[cfg_of(t)].push_back(make_task(
                         wpk[si], (uint32_t)round8(m1[si]), m1[si], n1[si], t,
                         (uint32_t)si, (uint32_t)si + 1, 0));
-            // output blocks re-tiled likewise, with split-K partial slots
-            // and reductions of their own
-            for (const BlockJob &j : ch.jobs2)
-                add_split_tiles(h.s2, j, 1, h.t2, h.red, h.p_len);
-            return nullptr;
-        };
-        const char *err2 = phase2();
-        th1.join();
-        B2G_REQUIRE(err1 == nullptr, B2G_ERR_ARG, err1 ? err1 : "");
-        B2G_REQUIRE(err2 == nullptr, B2G_ERR_STATE, err2 ? err2 : "");
-        tm.mark("pack lists and tiles (two threads)");
-        // task order, the two phases on two threads
-        std::thread th2([&]() {
-            for (int c = 0; c < kTileConfigs; c++)
-                h.x1[c] = xcd_order(h.t1[c], h.s1, true, group_b);
         });
-        for (int c = 0; c < kTileConfigs; c++)
-            h.x2[c] = xcd_order(h.t2[c], h.s2, true);
-        th2.join();
-        tm.mark("task order (two threads)");
-        // (task order: tiles of one output block (one segment list) share
-        // their operand panels and stay on one XCD, back to back, where its
-        // L2 serves the re-reads; the reductions address partial slots by
-        // task offsets, which reordering keeps)
+        // output blocks re-tiled likewise, with split-K partial slots and
+        // reductions of their own
+        jobs.emplace_back([&]() {
+            tile_env_refresh();
+            for (const BlockJob &j : ch.jobs2)
+                add_split_tiles(ch.s2, j, 1, h.t2, h.red, h.p_len);
+        });
+        run_host_jobs(jobs);
+        for (const char *e : err)
+            B2G_REQUIRE(e == nullptr, B2G_ERR_ARG, e ? e : "");
+        h.s1.reserve(ch.s1.size());
+        for (size_t si = 0; si < ch.s1.size(); si++) {
+            Seg q = ch.s1[si];
+            q.a_off = (uint64_t)xa[si], q.lda = (uint32_t)round8(q.K);
+            q.b_off = (uint64_t)rb[si], q.ldb = (uint32_t)round8(q.K);
+            q.flags = 0, q.alpha = 1.0;
+            h.s1.push_back(q);
+        }
+        tm.mark("pack lists and tiles (host workers)");
         k->w_max = std::max(k->w_max, wlen);
         k->p_max = std::max(k->p_max, h.p_len);
     }
@@ -274,15 +286,18 @@ static int pk_build_into(b2g_plan p, PkPlan *k) {
                 (unsigned long long)L.len, (unsigned long long)R.len,
                 (unsigned long long)X.len, (size_t)L.src_len,
                 (size_t)R.src_len);
-    // one allocation for every task, segment and pack-tile array
-    size_t bytes = 0;
+    // one allocation for the pack lists (first: uploaded and used by the
+    // operator packing while the host orders the tasks) and every task,
+    // segment and reduction array
+    const size_t mat_bytes =
+        (L.mats.size() + R.mats.size() + X.mats.size()) * sizeof(PackMat);
+    size_t bytes = mat_bytes;
     for (auto &h : hc) {
         for (int c = 0; c < kTileConfigs; c++)
             bytes += (h.t1[c].size() + h.t2[c].size()) * sizeof(Task);
         bytes += (h.s1.size() + h.s2.size()) * sizeof(Seg);
         bytes += h.red.size() * sizeof(RedTask);
     }
-    bytes += (L.mats.size() + R.mats.size() + X.mats.size()) * sizeof(PackMat);
     B2G_REQUIRE(L.ngroups < (1ull << 40) && R.ngroups < (1ull << 40) &&
                     X.ngroups < (1ull << 40) && L.mats.size() < (1u << 31) &&
                     R.mats.size() < (1u << 31) && X.mats.size() < (1u << 31),
@@ -300,6 +315,68 @@ static int pk_build_into(b2g_plan p, PkPlan *k) {
         off += nb;
         return (char *)k->dev + o;
     };
+    k->d_lmats = (PackMat *)put(L.mats.data(), L.mats.size() * sizeof(PackMat));
+    k->d_rmats = (PackMat *)put(R.mats.data(), R.mats.size() * sizeof(PackMat));
+    k->d_xmats = (PackMat *)put(X.mats.data(), X.mats.size() * sizeof(PackMat));
+    k->n_lmats = (int)L.mats.size();
+    k->n_rmats = (int)R.mats.size();
+    k->n_xmats = (int)X.mats.size();
+    k->n_lgroups = L.ngroups;
+    k->n_rgroups = R.ngroups;
+    k->n_xgroups = X.ngroups;
+    if (mat_bytes)
+        B2G_HIP(hipMemcpyAsync(k->dev, host, mat_bytes, hipMemcpyHostToDevice,
+                               p->ctx->stream));
+    B2G_HIP(hipMalloc(&k->op, std::max<uint64_t>(2 * (k->l_len + k->r_len), 8) *
+                                  sizeof(uint16_t)));
+    tm.mark("pack lists uploaded, planes allocated");
+    {
+        const int r = pk_pack_ops(p, k);
+        if (r != B2G_OK) {
+            hipStreamSynchronize(p->ctx->stream);
+            return r;
+        }
+    }
+    // task order while the device packs the operators: every (chunk,
+    // phase, configuration) launch is one item, taken largest first by the
+    // host workers
+    {
+        struct Item {
+            size_t n;
+            HostChunk *h;
+            int phase, c;
+        };
+        std::vector<Item> items;
+        for (auto &h : hc)
+            for (int c = 0; c < kTileConfigs; c++) {
+                if (!h.t1[c].empty())
+                    items.push_back(Item{h.t1[c].size(), &h, 1, c});
+                if (!h.t2[c].empty())
+                    items.push_back(Item{h.t2[c].size(), &h, 2, c});
+            }
+        std::stable_sort(items.begin(), items.end(),
+                         [](const Item &a, const Item &b) { return a.n > b.n; });
+        std::atomic<size_t> next{0};
+        // (task order: tiles of one output block (one segment list) share
+        // their operand panels and stay on one XCD, back to back, where its
+        // L2 serves the re-reads; the reductions address partial slots by
+        // task offsets, which reordering keeps)
+        auto worker = [&]() {
+            for (size_t i = next++; i < items.size(); i = next++) {
+                const Item &it = items[i];
+                HostChunk &h = *it.h;
+                if (it.phase == 1)
+                    h.x1[it.c] = xcd_order(h.t1[it.c], h.s1, true, group_b);
+                else
+                    h.x2[it.c] = xcd_order(h.t2[it.c], h.s2, true);
+            }
+        };
+        const size_t nw = std::min<size_t>(items.size(), 4);
+        std::vector<std::function<void()>> jobs(std::max<size_t>(nw, 1),
+                                                worker);
+        run_host_jobs(jobs);
+    }
+    tm.mark("task order (host workers)");
     for (auto &h : hc) {
         PkChunk pc;
         for (int c = 0; c < kTileConfigs; c++) {
@@ -320,27 +397,12 @@ static int pk_build_into(b2g_plan p, PkPlan *k) {
         pc.d_s2 = (Seg *)put(h.s2.data(), h.s2.size() * sizeof(Seg));
         k->chunks.push_back(pc);
     }
-    k->d_lmats = (PackMat *)put(L.mats.data(), L.mats.size() * sizeof(PackMat));
-    k->d_rmats = (PackMat *)put(R.mats.data(), R.mats.size() * sizeof(PackMat));
-    k->d_xmats = (PackMat *)put(X.mats.data(), X.mats.size() * sizeof(PackMat));
-    k->n_lmats = (int)L.mats.size();
-    k->n_rmats = (int)R.mats.size();
-    k->n_xmats = (int)X.mats.size();
-    k->n_lgroups = L.ngroups;
-    k->n_rgroups = R.ngroups;
-    k->n_xgroups = X.ngroups;
-    B2G_HIP(hipMemcpyAsync(k->dev, host, bytes, hipMemcpyHostToDevice,
-                           p->ctx->stream));
+    if (bytes > mat_bytes)
+        B2G_HIP(hipMemcpyAsync((char *)k->dev + mat_bytes, host + mat_bytes,
+                               bytes - mat_bytes, hipMemcpyHostToDevice,
+                               p->ctx->stream));
     B2G_HIP(hipStreamSynchronize(p->ctx->stream));
-    tm.mark("upload");
-    B2G_HIP(hipMalloc(&k->op, std::max<uint64_t>(2 * (k->l_len + k->r_len), 8) *
-                                  sizeof(uint16_t)));
-    tm.mark("planes allocated");
-    const int r = pk_pack_ops(p, k);
-    if (r != B2G_OK)
-        return r;
-    B2G_HIP(hipStreamSynchronize(p->ctx->stream));
-    tm.mark("operators packed");
+    tm.mark("tasks uploaded, operators packed");
     return B2G_OK;
 }
 

@@ -6,6 +6,8 @@
 
 #include <algorithm>
 #include <atomic>
+#include <condition_variable>
+#include <deque>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -15,8 +17,10 @@
 #include <memory>
 #include <mutex>
 #include <sstream>
+#include <thread>
 #include <tuple>
 #include <unordered_map>
+#include <unistd.h>
 
 namespace b2g {
 
@@ -675,6 +679,82 @@ XcdSplit xcd_order(std::vector<Task> &tasks, const std::vector<Seg> &segs,
     xs.off[8] = (uint32_t)acc;
     tasks.swap(out);
     return xs;
+}
+
+// host workers fed from one queue; never destroyed (the threads are
+// detached and wait on the queue until the process ends)
+namespace {
+class HostPool {
+  public:
+    explicit HostPool(int n) {
+        for (int i = 0; i < n; i++)
+            std::thread([this] { loop(); }).detach();
+    }
+    void submit(std::function<void()> f) {
+        {
+            std::lock_guard<std::mutex> g(m_);
+            q_.push_back(std::move(f));
+        }
+        cv_.notify_one();
+    }
+
+  private:
+    void loop() {
+        for (;;) {
+            std::function<void()> f;
+            {
+                std::unique_lock<std::mutex> g(m_);
+                cv_.wait(g, [this] { return !q_.empty(); });
+                f = std::move(q_.front());
+                q_.pop_front();
+            }
+            f();
+        }
+    }
+    std::mutex m_;
+    std::condition_variable cv_;
+    std::deque<std::function<void()>> q_;
+};
+
+// one pool per process: a child created by fork has none of its parent's
+// threads, so it starts a pool of its own
+HostPool *host_pool() {
+    static std::mutex m;
+    static HostPool *pool = nullptr;
+    static pid_t owner = 0;
+    std::lock_guard<std::mutex> g(m);
+    if (pool == nullptr || owner != getpid()) {
+        const unsigned hw = std::thread::hardware_concurrency();
+        const int n = (int)std::min(7u, std::max(2u, hw) - 1);
+        pool = new HostPool(n);
+        owner = getpid();
+    }
+    return pool;
+}
+} // namespace
+
+void run_host_jobs(std::vector<std::function<void()>> &jobs) {
+    if (jobs.empty())
+        return;
+    struct Wait {
+        std::mutex m;
+        std::condition_variable cv;
+        size_t left = 0;
+    } w;
+    w.left = jobs.size() - 1;
+    if (w.left > 0) {
+        HostPool *pool = host_pool();
+        for (size_t i = 1; i < jobs.size(); i++)
+            pool->submit([&w, &jobs, i] {
+                jobs[i]();
+                std::lock_guard<std::mutex> g(w.m);
+                if (--w.left == 0)
+                    w.cv.notify_one();
+            });
+    }
+    jobs[0]();
+    std::unique_lock<std::mutex> g(w.m);
+    w.cv.wait(g, [&w] { return w.left == 0; });
 }
 
 uint64_t w_cap_elems() {
