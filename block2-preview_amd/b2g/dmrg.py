@@ -909,8 +909,20 @@ class DMRG:
                                     ly.dq, 0), vecs[q][:ly.total]))
         return out
 
+    def _relieve_memory(self):
+        """Under device-memory pressure (less than a quarter of the device
+        free) drops the plans kept across sweeps (rebuilt on demand) and
+        hands torch's cached blocks back to the device, so that library
+        allocations can use them."""
+        free, total = torch.cuda.mem_get_info(self.dev)
+        if free >= total // 4:
+            return
+        self.close()
+        torch.cuda.empty_cache()
+
     def update_two_dot(self, i, forward, bond_dim, noise, conv):
         s, n = self.sym, self.n
+        self._relieve_memory()
         t0 = tp = time.perf_counter()
         # the two-site wavefunction at (i, i+1)
         if self.tensors[i + 1] is not None:
@@ -986,6 +998,13 @@ class DMRG:
             if i != 0:
                 self.tensors[i] = self._swap(left, i, False)
         self._tick("wavefunction", tp)
+        # the environment on the far side of the step is rebuilt by the next
+        # sweep before it is read again: released now, so that one set of
+        # environments is held on the device rather than two
+        if forward:
+            self.renv[i + 2] = None
+        else:
+            self.lenv[i] = None
         self.bonds.append(self.left_dims[i + 1] if forward else
                           self.right_dims[i + 1])
         return energy, sp.error, ndav, time.perf_counter() - t0

@@ -228,10 +228,24 @@ static int win_block(b2g_ctx ctx, size_t bytes, void **out, size_t *len) {
     }
     const size_t mib = (size_t)1 << 20;
     const size_t n = (bytes + mib - 1) / mib * mib;
-    B2G_HIP(hipMalloc(out, n));
+    hipError_t e = hipMalloc(out, n);
+    if (e != hipSuccess && !pool.empty()) {
+        // none of the released blocks fits: give them back and retry
+        (void)hipGetLastError();
+        for (auto &b : pool)
+            hipFree(b.second);
+        pool.clear();
+        e = hipMalloc(out, n);
+    }
+    B2G_HIP(e);
     *len = n;
     return B2G_OK;
 }
+
+// bytes of released window-plan blocks a context keeps for reuse; beyond
+// it the smallest are freed (a sweep whose bond dimension grows releases
+// plans smaller than every later request)
+constexpr size_t kWinPoolCap = (size_t)8 << 30;
 
 static int winplan_finish(b2g_winplan w,
                           std::vector<std::pair<WinGroup, WinRec>> &items) {
@@ -844,7 +858,17 @@ extern "C" int b2g_winplan_destroy(b2g_winplan w) {
         // the plan's launches are done before its block serves another
         hipSetDevice(w->ctx->device);
         hipStreamSynchronize(w->ctx->stream);
-        w->ctx->win_pool.emplace_back(w->dev_len, w->dev);
+        auto &pool = w->ctx->win_pool;
+        pool.emplace_back(w->dev_len, w->dev);
+        size_t total = 0;
+        for (const auto &b : pool)
+            total += b.first;
+        while (total > kWinPoolCap) {
+            auto it = std::min_element(pool.begin(), pool.end());
+            total -= it->first;
+            hipFree(it->second);
+            pool.erase(it);
+        }
     }
     delete w;
     return B2G_OK;
