@@ -281,39 +281,57 @@ def davidson_profile(args):
 C2_DIR = os.path.join(REPO, "tests", "golden", "sweep_c2_su2")
 
 
-def abinitio_leg(ctx, bond_dims):
+def abinitio_run(prec, bond_dims):
+    """One trajectory of the n=26 sweeps (see abinitio_leg) in this
+    process: per-sweep seconds, energies and phases."""
+    import torch
+    from b2g import dmrg as D
+    from b2g.effham import Context
+    ctx = Context(0)
+    dm = D.DMRG(ctx, "su2", os.path.join(C2_DIR, "mpo.bin"), C2_DIR,
+                list(bond_dims), [0.0], precision=prec)
+    t0 = time.perf_counter()
+    dm.solve(len(bond_dims), True, 0.0, None)
+    wall = time.perf_counter() - t0
+    dav = dm.phase_time.get("davidson", 0.0)
+    res = dict(
+        energies=dm.energies,
+        s_per_sweep=[round(t, 3) for t in dm.sweep_time],
+        total_s=round(wall, 3),
+        phase_s={k: round(v, 3) for k, v in dm.phase_time.items()},
+        davidson_tflops=round(2.0 * dm.davidson_macs / dav / 1e12, 3)
+        if dav else None)
+    dm.close()
+    del dm
+    torch.cuda.synchronize()
+    ctx.close()
+    return res
+
+
+def abinitio_leg(bond_dims):
     """Sweeps on the seeded n=26 ab initio Hamiltonian (SU(2), 26 electrons,
     singlet; block2's MPO and starting MPS) with the bond dimension growing
     over `bond_dims`, noise 0: per-sweep seconds, energies and the phases
     of the steps, with FP64 products and with split-BF16 products plus the
-    FP64 refinement."""
-    import gc
-    import torch
-    from b2g import dmrg as D
+    FP64 refinement.  Each trajectory runs in a child process of its own,
+    so that the second starts on a device the first has fully released."""
+    import subprocess
     out = {"workload": "two-site DMRG sweeps, synthetic ab initio "
                        "Hamiltonian n=26, SU(2), bond dimension %s, no noise"
                        % "/".join(map(str, bond_dims)),
            "bond_dims": bond_dims}
     for prec in ("f64", "bf16x3"):
-        dm = D.DMRG(ctx, "su2", os.path.join(C2_DIR, "mpo.bin"), C2_DIR,
-                    list(bond_dims), [0.0], precision=prec)
-        t0 = time.perf_counter()
-        dm.solve(len(bond_dims), True, 0.0, None)
-        wall = time.perf_counter() - t0
-        dav = dm.phase_time.get("davidson", 0.0)
-        out[prec] = dict(
-            energies=dm.energies,
-            s_per_sweep=[round(t, 3) for t in dm.sweep_time],
-            total_s=round(wall, 3),
-            phase_s={k: round(v, 3) for k, v in dm.phase_time.items()},
-            davidson_tflops=round(2.0 * dm.davidson_macs / dav / 1e12, 3)
-            if dav else None)
+        r = subprocess.run(
+            [sys.executable, os.path.abspath(__file__), "--abinitio-run",
+             prec, "--abinitio-bdims", ",".join(map(str, bond_dims))],
+            stdout=subprocess.PIPE, text=True, timeout=900)
+        if r.returncode != 0:
+            raise RuntimeError("n=26 sweeps (%s) failed: exit %d" % (
+                prec, r.returncode))
+        out[prec] = json.loads(r.stdout.strip().splitlines()[-1])
         log("n=26 sweeps %s: %s s/sweep, E %s" % (
             prec, out[prec]["s_per_sweep"],
-            ", ".join("%.8f" % e for e in dm.energies)))
-        del dm
-        gc.collect()
-        torch.cuda.empty_cache()
+            ", ".join("%.8f" % e for e in out[prec]["energies"])))
     out["dE_per_sweep_bf16x3_vs_fp64"] = [
         a - b for a, b in zip(out["bf16x3"]["energies"],
                               out["f64"]["energies"])]
@@ -352,11 +370,20 @@ def main():
                     help="only run 5 device-to-device copies of MIB MiB "
                          "(reference kernel of known traffic for the HBM "
                          "byte counters) and exit")
+    ap.add_argument("--abinitio-run", default=None, metavar="PREC",
+                    help=argparse.SUPPRESS)
+    ap.add_argument("--abinitio-bdims", default="50,100,250,500,1000",
+                    help=argparse.SUPPRESS)
     ap.add_argument("--davidson-profile", action="store_true",
                     help="only run one FP64 Davidson solve on the site and "
                          "print the algorithmic bytes of its bandwidth-bound "
                          "kernels (for a kernel-trace profile) and exit")
     args = ap.parse_args()
+    if args.abinitio_run:
+        print(json.dumps(abinitio_run(
+            args.abinitio_run,
+            [int(b) for b in args.abinitio_bdims.split(",")])), flush=True)
+        return 0
     if args.davidson_profile:
         return davidson_profile(args)
     if args.calibrate_copy:
@@ -754,7 +781,7 @@ def main():
     if not args.no_sweep:
         out["sweep"] = sweep_leg(ctx, min(16, os.cpu_count() or 1),
                                  not args.no_cpu)
-        out["sweep_abinitio"] = abinitio_leg(ctx, [50, 100, 250, 500, 1000])
+        out["sweep_abinitio"] = abinitio_leg([50, 100, 250, 500, 1000])
     print(json.dumps(out), flush=True)
     h.deallocate()
     ctx.close()
