@@ -11,6 +11,8 @@ generated on the device).
 * Size-independent properties of the full product: <x|H y> = <y|H x>
   (H is symmetric) to 1e-12 relative, and the 4-RHS product bitwise equal to
   four 1-RHS products (outputs are tile-owned, reductions in fixed order).
+* The split-BF16 product of effective Hamiltonians rebuilt from the same
+  problem (packed plan derived again on the host workers) bitwise equal.
 * The diagonal against the oracle's, 1e-12.
 * Four state-averaged roots, graph-captured Davidson: mixed precision
   (FP32 and split-BF16 products + FP64 refinement) against FP64, |dE| <=
@@ -46,6 +48,7 @@ def site(gpu_ctx):
     h = EffectiveHamiltonian(pb, gpu_ctx, fp32=True,
                              device_data=(ldev, rdev))
     assert h.n_pairs == 47204 and h.ket_len == 2341066
+    h.device_data = (ldev, rdev)
     pb.lops.data = ldev.cpu().numpy()
     pb.rops.data = rdev.cpu().numpy()
     orc = O.Oracle(pb)
@@ -103,6 +106,22 @@ def test_multi_rhs_equals_single_rhs_bitwise(site):
     for r in range(4):
         y1 = h.multiply(x[r]).cpu().numpy()
         assert np.array_equal(y1, y4[r]), r
+
+
+def test_packed_plan_rebuild_bitwise(site, gpu_ctx):
+    """The split-BF16 packed plan is derived by host worker threads (lists,
+    tiles and task order as separate jobs): a second effective Hamiltonian
+    built from the same problem, and a third after the workers have built
+    other plans, give bitwise the same split-BF16 product."""
+    pb, h, orc, x, sel, mask, ref = site
+    y0 = h.multiply(x[:2], precision="bf16x3").cpu().numpy()
+    ldata, rdata = h.device_data
+    for _ in range(2):
+        h2 = EffectiveHamiltonian(pb, gpu_ctx, fp32=False,
+                                  device_data=(ldata, rdata))
+        y2 = h2.multiply(x[:2], precision="bf16x3").cpu().numpy()
+        h2.deallocate()
+        assert np.array_equal(y0, y2)
 
 
 def test_diagonal_against_oracle(site):
