@@ -255,6 +255,32 @@ def test_sweeps_match_block2_per_step(gpu_ctx, case, monkeypatch):
     assert de.max() < 1e-6, de
 
 
+def test_sweeps_under_memory_pressure_match_block2(gpu_ctx, monkeypatch):
+    """The driver's relief path on every step (device reported nearly full):
+    the plan cache dropped and rebuilt at each site, torch's cached blocks
+    returned, environments released behind the sweep; the steps are still
+    block2's (the quiet SU(2) Hubbard case, energies per step to 1e-8)."""
+    total = torch.cuda.mem_get_info()[1]
+    monkeypatch.setattr(D.torch.cuda, "mem_get_info",
+                        lambda dev=None: (total // 10, total))
+    closes = []
+    close = D.DMRG.close
+
+    def counted(self):
+        closes.append(1)
+        return close(self)
+    monkeypatch.setattr(D.DMRG, "close", counted)
+    dm, e, got, ref, final = _sweeps(gpu_ctx, "hub8_su2_quiet")
+    assert len(closes) >= len(got)
+    assert abs(e - final) < 1e-8
+    assert np.abs(got[:, 2] - ref[:, 2]).max() < 1e-8
+    # the last sweep released every environment behind it
+    if dm.center == 0:
+        assert all(x is None for x in dm.lenv[:dm.n - 1])
+    else:
+        assert all(x is None for x in dm.renv[2:])
+
+
 @pytest.mark.parametrize("case", NOISY)
 def test_noisy_sweeps_reach_block2_energy(gpu_ctx, case):
     dm, e, got, ref, final = _sweeps(gpu_ctx, case)
