@@ -1258,15 +1258,30 @@ extern "C" int b2g_effham_eigs_ex(b2g_effham h, double *x, int k,
                        h->kernel == nullptr && o.n_ortho == 0 &&
                        !(o.type & (B2G_DAV_GREATER_THAN | B2G_DAV_LESS_THAN |
                                    B2G_DAV_CLOSE_TO));
-    auto solve = [&](int prec, int &ndv, uint64_t &nm, bool &sf) {
+    auto solve = [&](const DavOpts &oo, int prec, int &ndv, uint64_t &nm,
+                     bool &sf) {
         if (exact)
-            return exact_diag(h, x, k, o, prec, ev, ndv, nm);
-        return graph ? davidson_graph(h, x, k, o, prec, ev, ndv, nm, sf)
-                     : davidson(h, x, k, o, prec, ev, ndv, nm, sf);
+            return exact_diag(h, x, k, oo, prec, ev, ndv, nm);
+        return graph ? davidson_graph(h, x, k, oo, prec, ev, ndv, nm, sf)
+                     : davidson(h, x, k, oo, prec, ev, ndv, nm, sf);
     };
-    int r = solve(precision, nd, nmv, soft);
+    // A reduced-precision phase followed by the FP64 refinement stops at the
+    // residual its products resolve: with split-BF16 (FP32) products the
+    // computed H x carries a relative error of a few 1e-6 (1e-7), so its
+    // iterations past |r| ~ 1e-5 |lambda| (4e-7 |lambda|) converge to the
+    // eigenvector of the rounded operator, which the refinement has to undo
+    // anyway.  B2G_MIXED_REL overrides the relative threshold (0: the
+    // caller's thresholds throughout).
+    DavOpts o1 = o;
+    if (precision != B2G_PREC_F64 && opts->refine) {
+        double kappa = precision == B2G_PREC_BF16X3 ? 1e-5 : 4e-7;
+        if (const char *e = getenv("B2G_MIXED_REL"))
+            kappa = atof(e);
+        o1.rel = std::max(o.rel, kappa);
+    }
+    int r = solve(o1, precision, nd, nmv, soft);
     if (r == B2G_OK && precision != B2G_PREC_F64 && opts->refine && !soft)
-        r = solve(B2G_PREC_F64, nd2, nmv2, soft2);
+        r = solve(o, B2G_PREC_F64, nd2, nmv2, soft2);
     auto t1 = std::chrono::steady_clock::now();
     for (int i = 0; i < k && i < (int)ev.size(); i++)
         energies[i] = ev[i];
