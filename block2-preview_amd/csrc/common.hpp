@@ -9,6 +9,7 @@
 #include <cstdlib>
 #include <string>
 #include <utility>
+#include <functional>
 #include <vector>
 
 #include "../../include/b2g.h"
@@ -109,6 +110,15 @@ struct U64Hash {
 // Wall time of the stages of a host-side setup (effective Hamiltonian,
 // plan, packed plan): with B2G_TIME_SETUP set, mark() prints the time since
 // the previous mark to stderr.
+// Runs jobs[0] on the calling thread and the other jobs on persistent host
+// worker threads (created on first use; their thread-local tiling memos stay
+// warm from one plan build to the next) and returns when all have finished.
+// Jobs report errors through their captures and must not throw.
+void run_host_jobs(std::vector<std::function<void()>> &jobs);
+// memcpy of a large host range split over the host workers (staging copies
+// of plan uploads: one thread moves about 6-10 GB/s)
+void parallel_memcpy(void *dst, const void *src, size_t bytes);
+
 class SetupTimer {
   public:
     explicit SetupTimer(const char *what)

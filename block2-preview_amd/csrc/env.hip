@@ -21,6 +21,7 @@
 #include "replay.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -294,8 +295,8 @@ static int winplan_finish(b2g_winplan w,
     if (ng)
         std::memcpy(host + 64, w->plan.groups.data(), ng * sizeof(WinGroup));
     if (nr)
-        std::memcpy(host + 64 + ng * sizeof(WinGroup), w->plan.recs.data(),
-                    nr * sizeof(WinRec));
+        parallel_memcpy(host + 64 + ng * sizeof(WinGroup),
+                        w->plan.recs.data(), nr * sizeof(WinRec));
     if (nc)
         std::memcpy(host + 64 + ng * sizeof(WinGroup) + nr * sizeof(WinRec),
                     w->chunks.data(), nc * sizeof(WinChunk));
@@ -563,68 +564,114 @@ extern "C" int b2g_transform_plan(b2g_ctx ctx, int symmetry, int n_infos,
         }
         by_dest[dest_of[t.c]].push_back(it);
     }
-    for (size_t d = 0; d < dest_order.size(); d++) {
+    // Per destination: whether its terms sum as whole operators, its block
+    // maps (built here, on this thread) and its record count; then the
+    // records of the destinations are written on the host workers into
+    // their slots of one array, in the serial order.
+    const size_t nd = dest_order.size();
+    std::vector<uint8_t> whole(nd, 0);
+    std::vector<std::vector<const BlockMap *>> bms(nd);
+    std::vector<uint64_t> at(nd + 1, 0);
+    for (size_t d = 0; d < nd; d++) {
         const Operator &co = O.ops[dest_order[d]];
         const BlockInfo &ci = O.infos[co.info];
         // every term of this destination untransposed and of its layout:
         // the sum is one axpy over the whole operator (the reference's
         // iadd does the same when the infos agree), one record per term
-        bool whole = ci.total() < (1ull << 31);
+        bool wh = ci.total() < (1ull << 31);
         for (int it : by_dest[d])
-            whole = whole && terms[it].conj == 0 &&
-                    O.ops[terms[it].a].info == co.info;
-        if (whole) {
-            const uint32_t len = (uint32_t)ci.total();
-            for (int it : by_dest[d]) {
-                const b2g_transform_term &t = terms[it];
-                const Operator &ao = O.ops[t.a];
-                WinRec rec{};
-                rec.a_off = ao.data_off;
-                rec.am = len == 0 ? 0 : 1, rec.an = len, rec.lda = len;
-                rec.b_off = 0, rec.bm = rec.bn = rec.ldb = 1;
-                rec.conj = 0;
-                rec.scale = t.factor * ao.factor;
-                WinGroup g{};
-                g.c_off = co.data_off;
-                g.ldc = len;
-                g.rows = rec.am, g.cols = len;
-                if (len != 0)
-                    items.emplace_back(g, rec);
-            }
-            continue;
-        }
-        std::vector<const BlockMap *> bm(by_dest[d].size());
-        for (size_t q = 0; q < by_dest[d].size(); q++) {
-            const b2g_transform_term &t = terms[by_dest[d][q]];
-            bm[q] = &block_map(co, O.ops[t.a], t.conj);
-        }
-        for (int xc = 0; xc < ci.n(); xc++)
+            wh = wh && terms[it].conj == 0 && O.ops[terms[it].a].info == co.info;
+        whole[d] = wh;
+        uint64_t n = 0;
+        if (wh) {
+            n = ci.total() != 0 ? by_dest[d].size() : 0;
+        } else {
+            bms[d].resize(by_dest[d].size());
             for (size_t q = 0; q < by_dest[d].size(); q++) {
-                const int xa = bm[q]->xa[xc];
-                if (xa < 0)
-                    continue;
                 const b2g_transform_term &t = terms[by_dest[d][q]];
-                const Operator &ao = O.ops[t.a];
-                const BlockInfo &ai = O.infos[ao.info];
-                double f = t.factor * ao.factor;
-                if (t.conj && s == Symm::SU2)
-                    f = (double)((long double)f * bm[q]->cgf[xc]);
-                WinRec rec{};
-                rec.a_off = ao.data_off + ai.shift[xa];
-                rec.am = ai.nbra[xa], rec.an = ai.nket[xa], rec.lda = rec.an;
-                rec.b_off = 0, rec.bm = rec.bn = rec.ldb = 1;
-                rec.conj = t.conj;
-                rec.scale = f;
-                WinGroup g{};
-                g.c_off = co.data_off + ci.shift[xc];
-                g.ldc = ci.nket[xc];
-                g.rows = t.conj ? rec.an : rec.am;
-                g.cols = t.conj ? rec.am : rec.an;
-                B2G_REQUIRE(g.rows == ci.nbra[xc] && g.cols == ci.nket[xc],
-                            B2G_ERR_ARG, "transform block shapes differ");
-                items.emplace_back(g, rec);
+                bms[d][q] = &block_map(co, O.ops[t.a], t.conj);
             }
+            for (int xc = 0; xc < ci.n(); xc++)
+                for (size_t q = 0; q < by_dest[d].size(); q++)
+                    n += bms[d][q]->xa[xc] >= 0;
+        }
+        at[d + 1] = at[d] + n;
     }
+    items.resize(at[nd]);
+    std::atomic<const char *> shape_err{nullptr};
+    auto fill = [&](size_t d0, size_t d1) {
+        for (size_t d = d0; d < d1; d++) {
+            const Operator &co = O.ops[dest_order[d]];
+            const BlockInfo &ci = O.infos[co.info];
+            std::pair<WinGroup, WinRec> *out = items.data() + at[d];
+            if (whole[d]) {
+                const uint32_t len = (uint32_t)ci.total();
+                if (len == 0)
+                    continue;
+                for (int it : by_dest[d]) {
+                    const b2g_transform_term &t = terms[it];
+                    const Operator &ao = O.ops[t.a];
+                    WinRec rec{};
+                    rec.a_off = ao.data_off;
+                    rec.am = 1, rec.an = len, rec.lda = len;
+                    rec.b_off = 0, rec.bm = rec.bn = rec.ldb = 1;
+                    rec.conj = 0;
+                    rec.scale = t.factor * ao.factor;
+                    WinGroup g{};
+                    g.c_off = co.data_off;
+                    g.ldc = len;
+                    g.rows = rec.am, g.cols = len;
+                    *out++ = std::make_pair(g, rec);
+                }
+                continue;
+            }
+            const std::vector<const BlockMap *> &bm = bms[d];
+            for (int xc = 0; xc < ci.n(); xc++)
+                for (size_t q = 0; q < by_dest[d].size(); q++) {
+                    const int xa = bm[q]->xa[xc];
+                    if (xa < 0)
+                        continue;
+                    const b2g_transform_term &t = terms[by_dest[d][q]];
+                    const Operator &ao = O.ops[t.a];
+                    const BlockInfo &ai = O.infos[ao.info];
+                    double f = t.factor * ao.factor;
+                    if (t.conj && s == Symm::SU2)
+                        f = (double)((long double)f * bm[q]->cgf[xc]);
+                    WinRec rec{};
+                    rec.a_off = ao.data_off + ai.shift[xa];
+                    rec.am = ai.nbra[xa], rec.an = ai.nket[xa];
+                    rec.lda = rec.an;
+                    rec.b_off = 0, rec.bm = rec.bn = rec.ldb = 1;
+                    rec.conj = t.conj;
+                    rec.scale = f;
+                    WinGroup g{};
+                    g.c_off = co.data_off + ci.shift[xc];
+                    g.ldc = ci.nket[xc];
+                    g.rows = t.conj ? rec.an : rec.am;
+                    g.cols = t.conj ? rec.am : rec.an;
+                    if (g.rows != ci.nbra[xc] || g.cols != ci.nket[xc])
+                        shape_err = "transform block shapes differ";
+                    *out++ = std::make_pair(g, rec);
+                }
+        }
+    };
+    {
+        // destinations split into jobs of about equal record counts
+        const size_t nj = std::min<size_t>(8, at[nd] / (1u << 20) + 1);
+        std::vector<std::function<void()>> jobs;
+        size_t d0 = 0;
+        for (size_t j = 0; j < nj; j++) {
+            const uint64_t target = at[nd] * (j + 1) / nj;
+            size_t d1 = d0;
+            while (d1 < nd && (j + 1 == nj || at[d1] < target))
+                d1++;
+            jobs.emplace_back([&fill, d0, d1]() { fill(d0, d1); });
+            d0 = d1;
+        }
+        run_host_jobs(jobs);
+    }
+    const char *serr = shape_err.load();
+    B2G_REQUIRE(serr == nullptr, B2G_ERR_ARG, serr ? serr : "");
     for (int i = 0; i < n_ops; i++)
         B2G_REQUIRE(!(written[i] && read[i]), B2G_ERR_ARG,
                     "a transform reads an operator it writes");

@@ -3,6 +3,8 @@
 #include "envplan.hpp"
 
 #include <algorithm>
+#include <atomic>
+#include <functional>
 #include <cmath>
 #include <limits>
 #include <map>
@@ -226,60 +228,175 @@ struct WinKeyHash {
 
 bool WinPlan::finish(std::vector<std::pair<WinGroup, WinRec>> &items,
                      std::string &err) {
-    // group id of every item (windows in first-appearance order), then a
-    // counting sort of the records by group, stable within a group
-    // sized for the windows, not the records (records come in runs)
-    FlatMap<WinKey, uint32_t, WinKeyHash> idx(1024);
-    std::vector<uint32_t> gid(items.size());
+    // runs of consecutive records with one window, each run's group
+    // (windows in first-appearance order), then the records of every group
+    // in item order: a run's records land at its group's start plus the
+    // records of the group's earlier runs, so runs are copied independently
+    // (on the host workers) into the order of a stable sort by group
+    SetupTimer tm("window plan finish");
+    // run starts, found on the host workers over slices of the items
+    const size_t N = items.size();
+    const size_t nj = std::min<size_t>(8, N / (1u << 20) + 1);
+    std::vector<std::vector<uint32_t>> slice_runs(nj);
+    {
+        std::vector<std::function<void()>> jobs;
+        for (size_t j = 0; j < nj; j++)
+            jobs.emplace_back([&, j]() {
+                const size_t lo = N * j / nj, hi = N * (j + 1) / nj;
+                for (size_t i = lo; i < hi; i++) {
+                    const WinGroup &g = items[i].first;
+                    if (i > 0) {
+                        const WinGroup &p = items[i - 1].first;
+                        if (p.c_off == g.c_off && p.rows == g.rows &&
+                            p.cols == g.cols && p.ldc == g.ldc)
+                            continue;
+                    }
+                    slice_runs[j].push_back((uint32_t)i);
+                }
+            });
+        run_host_jobs(jobs);
+    }
+    std::vector<uint32_t> run_at, run_gid; // first item and group of a run
+    {
+        size_t nr = 0;
+        for (const auto &v : slice_runs)
+            nr += v.size();
+        run_at.reserve(nr);
+        for (auto &v : slice_runs) {
+            run_at.insert(run_at.end(), v.begin(), v.end());
+            std::vector<uint32_t>().swap(v);
+        }
+    }
+    // Windows by their start element, checked on the host workers (job j
+    // takes the starts whose hash falls in its share): two runs starting at
+    // one element are either one window met again (then the runs are
+    // grouped through a map, below) or windows of different shapes (an
+    // error: windows must be disjoint).  When every run is a window of its
+    // own, the groups are the runs in order.
+    const size_t nrun0 = run_at.size();
+    // the runs' windows, gathered into one compact array on the workers
+    std::vector<WinGroup> rw(nrun0);
+    {
+        const size_t nc = std::min<size_t>(8, nrun0 / (1u << 18) + 1);
+        std::vector<std::function<void()>> jobs;
+        for (size_t j = 0; j < nc; j++)
+            jobs.emplace_back([&, j, nc]() {
+                for (size_t q = nrun0 * j / nc; q < nrun0 * (j + 1) / nc; q++)
+                    rw[q] = items[run_at[q]].first;
+            });
+        run_host_jobs(jobs);
+    }
+    std::atomic<int> repeated{0}, clash{0};
+    {
+        const size_t nh = std::min<size_t>(8, nrun0 / (1u << 18) + 1);
+        std::vector<std::function<void()>> jobs;
+        for (size_t j = 0; j < nh; j++)
+            jobs.emplace_back([&, j, nh]() {
+                FlatMap<uint64_t, uint32_t, U64Hash> seen(nrun0 / nh + 16);
+                for (size_t q = 0; q < nrun0; q++) {
+                    const WinGroup &g = rw[q];
+                    const uint64_t h = g.c_off * 0x9E3779B97F4A7C15ull;
+                    if ((h >> 40) % nh != j)
+                        continue;
+                    auto r = seen.emplace(g.c_off, (uint32_t)q);
+                    if (r.second)
+                        continue;
+                    const WinGroup &o = rw[*r.first];
+                    if (o.rows == g.rows && o.cols == g.cols && o.ldc == g.ldc)
+                        repeated = 1;
+                    else
+                        clash = 1;
+                }
+            });
+        run_host_jobs(jobs);
+    }
+    if (clash) {
+        err = "two windows of different shapes start at one element";
+        return false;
+    }
     std::vector<WinGroup> gs;
     std::vector<uint32_t> cnt;
-    for (uint32_t i = 0; i < items.size(); i++) {
-        const WinGroup &g = items[i].first;
-        const WinGroup *p = i > 0 ? &items[i - 1].first : nullptr;
-        if (p != nullptr && p->c_off == g.c_off && p->rows == g.rows &&
-            p->cols == g.cols && p->ldc == g.ldc) {
-            // a run of records of one window (no lookup)
-            gid[i] = gid[i - 1];
-            cnt[gid[i]]++;
-            continue;
+    run_gid.resize(nrun0);
+    if (!repeated) {
+        gs.swap(rw);
+        cnt.resize(nrun0);
+        for (size_t q = 0; q < nrun0; q++) {
+            run_gid[q] = (uint32_t)q;
+            cnt[q] = (q + 1 < nrun0 ? run_at[q + 1] : (uint32_t)N) - run_at[q];
         }
-        WinKey k{g.c_off, g.rows, g.cols, g.ldc};
-        auto r = idx.emplace(k, (uint32_t)gs.size());
-        if (r.second) {
-            gs.push_back(g);
-            cnt.push_back(0);
+    } else {
+        FlatMap<WinKey, uint32_t, WinKeyHash> idx(1024);
+        for (size_t q = 0; q < nrun0; q++) {
+            const WinGroup &g = rw[q];
+            WinKey k{g.c_off, g.rows, g.cols, g.ldc};
+            auto r = idx.emplace(k, (uint32_t)gs.size());
+            if (r.second) {
+                gs.push_back(g);
+                cnt.push_back(0);
+            }
+            run_gid[q] = *r.first;
+            cnt[*r.first] +=
+                (q + 1 < nrun0 ? run_at[q + 1] : (uint32_t)N) - run_at[q];
         }
-        gid[i] = *r.first;
-        cnt[*r.first]++;
     }
-    // windows starting at the same element must be the same window
-    std::vector<std::pair<uint64_t, uint32_t>> starts(gs.size());
-    for (uint32_t g = 0; g < gs.size(); g++)
-        starts[g] = std::make_pair(gs[g].c_off, g);
-    std::sort(starts.begin(), starts.end());
-    for (size_t i = 1; i < starts.size(); i++)
-        if (starts[i].first == starts[i - 1].first) {
-            err = "two windows of different shapes start at one element";
-            return false;
-        }
+    tm.mark("runs and windows");
     std::vector<uint32_t> pos(gs.size() + 1, 0);
     for (uint32_t g = 0; g < gs.size(); g++)
         pos[g + 1] = pos[g] + cnt[g];
     groups.assign(gs.begin(), gs.end());
-    recs.resize(items.size());
-    src_elems = dst_elems = 0;
     for (uint32_t g = 0; g < gs.size(); g++) {
         groups[g].rec_begin = pos[g];
         groups[g].rec_end = pos[g + 1];
     }
-    for (uint32_t i = 0; i < items.size(); i++) {
-        const WinRec &r = items[i].second;
-        recs[pos[gid[i]]++] = r;
-        src_elems += (uint64_t)r.am * r.an + (uint64_t)r.bm * r.bn;
-        a_need = std::max(a_need, r.a_off + (r.am == 0 ? 0 :
-                          (uint64_t)(r.am - 1) * r.lda + r.an));
-        b_need = std::max(b_need, r.b_off + (r.bm == 0 ? 0 :
-                          (uint64_t)(r.bm - 1) * r.ldb + r.bn));
+    const size_t nrun = run_at.size();
+    std::vector<uint32_t> run_dst(nrun);
+    for (size_t q = 0; q < nrun; q++) {
+        const uint32_t end =
+            q + 1 < nrun ? run_at[q + 1] : (uint32_t)items.size();
+        run_dst[q] = pos[run_gid[q]];
+        pos[run_gid[q]] += end - run_at[q];
+    }
+    recs.resize(items.size());
+    tm.mark("offsets and allocation");
+    // runs split into jobs of about equal record counts; per job the
+    // algorithmic element counts and buffer extents, combined in job order
+    struct Part {
+        uint64_t src = 0, a = 0, b = 0;
+    };
+    std::vector<Part> part(nj);
+    std::vector<std::function<void()>> jobs;
+    size_t q0 = 0;
+    for (size_t j = 0; j < nj; j++) {
+        const uint64_t target = (uint64_t)items.size() * (j + 1) / nj;
+        size_t q1 = q0;
+        while (q1 < nrun && (j + 1 == nj || run_at[q1] < target))
+            q1++;
+        jobs.emplace_back([&, j, q0, q1]() {
+            Part &pt = part[j];
+            for (size_t q = q0; q < q1; q++) {
+                const uint32_t end =
+                    q + 1 < nrun ? run_at[q + 1] : (uint32_t)items.size();
+                WinRec *d = recs.data() + run_dst[q];
+                for (uint32_t i = run_at[q]; i < end; i++) {
+                    const WinRec &r = items[i].second;
+                    *d++ = r;
+                    pt.src += (uint64_t)r.am * r.an + (uint64_t)r.bm * r.bn;
+                    pt.a = std::max(pt.a, r.a_off + (r.am == 0 ? 0 :
+                                    (uint64_t)(r.am - 1) * r.lda + r.an));
+                    pt.b = std::max(pt.b, r.b_off + (r.bm == 0 ? 0 :
+                                    (uint64_t)(r.bm - 1) * r.ldb + r.bn));
+                }
+            }
+        });
+        q0 = q1;
+    }
+    run_host_jobs(jobs);
+    tm.mark("copy");
+    src_elems = dst_elems = 0;
+    for (const Part &pt : part) {
+        src_elems += pt.src;
+        a_need = std::max(a_need, pt.a);
+        b_need = std::max(b_need, pt.b);
     }
     for (const WinGroup &w : groups) {
         dst_elems += (uint64_t)w.rows * w.cols;

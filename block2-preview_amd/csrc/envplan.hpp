@@ -7,6 +7,7 @@
 #include "plan.hpp"
 
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <utility>
 #include <vector>
@@ -82,9 +83,27 @@ struct WinGroup {
     uint32_t pad;
 };
 
+// allocator whose resize leaves trivially constructible elements
+// uninitialised (record arrays of hundreds of millions of entries are
+// written right after sizing; zero-filling them first doubles the traffic)
+template <typename T> struct NoInitAlloc : std::allocator<T> {
+    template <typename U> struct rebind {
+        using other = NoInitAlloc<U>;
+    };
+    NoInitAlloc() = default;
+    template <typename U> NoInitAlloc(const NoInitAlloc<U> &) noexcept {}
+    template <typename U> void construct(U *p) noexcept {
+        ::new ((void *)p) U;
+    }
+    template <typename U, typename... Args>
+    void construct(U *p, Args &&...args) {
+        ::new ((void *)p) U(std::forward<Args>(args)...);
+    }
+};
+
 struct WinPlan {
     std::vector<WinGroup> groups;
-    std::vector<WinRec> recs;
+    std::vector<WinRec, NoInitAlloc<WinRec>> recs;
     uint64_t c_len = 0;     // elements of the C buffer the plan writes into
     uint64_t a_need = 0;    // minimum element counts of the A / B buffers
     uint64_t b_need = 0;

@@ -5,7 +5,6 @@
 #include "common.hpp"
 #include "launch.hpp"
 
-#include <functional>
 #include <vector>
 
 namespace b2g {
@@ -112,11 +111,6 @@ void sort_tasks(std::vector<Task> &tasks, const std::vector<Seg> &segs);
 // segment; or sorted by cost (empty ranges)
 XcdSplit xcd_order(std::vector<Task> &tasks, const std::vector<Seg> &segs,
                    bool grouped, bool by_b = false);
-// Runs jobs[0] on the calling thread and the other jobs on persistent host
-// worker threads (created on first use; their thread-local tiling memos stay
-// warm from one plan build to the next) and returns when all have finished.
-// Jobs report errors through their captures and must not throw.
-void run_host_jobs(std::vector<std::function<void()>> &jobs);
 // scratch elements per chunk and right-hand side (B2G_W_CAP)
 uint64_t w_cap_elems();
 // allocate the H*psi scratch for n_rhs at precision ahead of a capture

@@ -757,6 +757,25 @@ void run_host_jobs(std::vector<std::function<void()>> &jobs) {
     w.cv.wait(g, [&w] { return w.left == 0; });
 }
 
+void parallel_memcpy(void *dst, const void *src, size_t bytes) {
+    const size_t piece = (size_t)64 << 20;
+    const size_t n = std::min<size_t>(8, (bytes + piece - 1) / piece);
+    if (n <= 1) {
+        if (bytes)
+            std::memcpy(dst, src, bytes);
+        return;
+    }
+    const size_t step = (bytes / n + 4095) & ~(size_t)4095;
+    std::vector<std::function<void()>> jobs;
+    for (size_t o = 0; o < bytes; o += step) {
+        const size_t len = std::min(step, bytes - o);
+        jobs.emplace_back([=]() {
+            std::memcpy((char *)dst + o, (const char *)src + o, len);
+        });
+    }
+    run_host_jobs(jobs);
+}
+
 uint64_t w_cap_elems() {
     const char *e = getenv("B2G_W_CAP");
     if (e != nullptr) {
