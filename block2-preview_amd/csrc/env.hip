@@ -395,17 +395,22 @@ extern "C" int b2g_contract_plan(b2g_ctx ctx, const b2g_contract *c,
                              bra_a, bra_b, ket_a, ket_b, bc, kc, A.refs,
                              B.refs, Cs.infos[i]);
     }
+    tm.mark("connections");
     // records: every string, every connection of its class
-    // (OperatorFunctions::tensor_product)
-    std::vector<std::pair<WinGroup, WinRec>> items;
-    for (int it = 0; it < c->n_terms; it++) {
+    // (OperatorFunctions::tensor_product).  Each term's class range is
+    // found here; the records are then written on the host workers into
+    // their slots of one array, in term order.
+    const int nt = c->n_terms;
+    std::vector<int> cls_b(nt, 0), cls_e(nt, 0);
+    std::vector<uint64_t> at(nt + 1, 0);
+    for (int it = 0; it < nt; it++) {
         const b2g_contract_term &t = c->terms[it];
         const Operator &co = Cs.ops[t.c], &ao = A.ops[t.a], &bo = B.ops[t.b];
-        const BlockInfo &ci = Cs.infos[co.info], &ai = A.infos[ao.info],
-                        &bi = B.infos[bo.info];
         const double scale = t.factor * ao.factor * bo.factor;
+        at[it + 1] = at[it];
         if (std::abs(scale) < 1E-20)
             continue;
+        const BlockInfo &ci = Cs.infos[co.info];
         const uint64_t abdq = q_combine(
             s, ci.dq, (t.conj & 1) ? q_neg(s, ao.dq) : ao.dq,
             (t.conj & 2) ? bo.dq : q_neg(s, bo.dq));
@@ -414,35 +419,65 @@ extern "C" int b2g_contract_plan(b2g_ctx ctx, const b2g_contract *c,
                         w->tp[co.info].find_class((uint8_t)t.conj, abdq, ib,
                                                   ie),
                     B2G_ERR_ARG, "contraction string without its class");
-        const Connections &cn = w->tp[co.info];
-        for (int il = ib; il < ie; il++) {
-            const uint32_t xa = cn.ia[il], xb = cn.ib[il], xc = cn.ic[il];
-            WinRec rec{};
-            rec.a_off = ao.data_off + ai.shift[xa];
-            rec.am = ai.nbra[xa], rec.an = ai.nket[xa], rec.lda = rec.an;
-            rec.b_off = bo.data_off + bi.shift[xb];
-            rec.bm = bi.nbra[xb], rec.bn = bi.nket[xb], rec.ldb = rec.bn;
-            rec.conj = t.conj;
-            rec.scale = scale * cn.factor[il];
-            const uint32_t amo = (t.conj & 1) ? rec.an : rec.am,
-                           ano = (t.conj & 1) ? rec.am : rec.an;
-            const uint32_t bmo = (t.conj & 2) ? rec.bn : rec.bm,
-                           bno = (t.conj & 2) ? rec.bm : rec.bn;
-            WinGroup g{};
-            g.c_off = co.data_off + ci.shift[xc] + cn.stride[il];
-            g.ldc = ci.nket[xc];
-            g.rows = amo * bmo;
-            g.cols = ano * bno;
-            const uint64_t sr = cn.stride[il] / ci.nket[xc],
-                           sc = cn.stride[il] % ci.nket[xc];
-            B2G_REQUIRE(sr + g.rows <= ci.nbra[xc] &&
-                            sc + g.cols <= ci.nket[xc],
-                        B2G_ERR_ARG,
-                        "tensor-product window outside its block");
-            items.emplace_back(g, rec);
-        }
+        cls_b[it] = ib, cls_e[it] = ie;
+        at[it + 1] += (uint64_t)(ie - ib);
     }
-    tm.mark("connections and records");
+    std::vector<std::pair<WinGroup, WinRec>> items(at[nt]);
+    std::atomic<int> outside{0};
+    auto fill = [&](int t0, int t1) {
+        for (int it = t0; it < t1; it++) {
+            const b2g_contract_term &t = c->terms[it];
+            const Operator &co = Cs.ops[t.c], &ao = A.ops[t.a],
+                           &bo = B.ops[t.b];
+            const BlockInfo &ci = Cs.infos[co.info], &ai = A.infos[ao.info],
+                            &bi = B.infos[bo.info];
+            const double scale = t.factor * ao.factor * bo.factor;
+            const Connections &cn = w->tp[co.info];
+            std::pair<WinGroup, WinRec> *out = items.data() + at[it];
+            for (int il = cls_b[it]; il < cls_e[it]; il++) {
+                const uint32_t xa = cn.ia[il], xb = cn.ib[il],
+                               xc = cn.ic[il];
+                WinRec rec{};
+                rec.a_off = ao.data_off + ai.shift[xa];
+                rec.am = ai.nbra[xa], rec.an = ai.nket[xa], rec.lda = rec.an;
+                rec.b_off = bo.data_off + bi.shift[xb];
+                rec.bm = bi.nbra[xb], rec.bn = bi.nket[xb], rec.ldb = rec.bn;
+                rec.conj = t.conj;
+                rec.scale = scale * cn.factor[il];
+                const uint32_t amo = (t.conj & 1) ? rec.an : rec.am,
+                               ano = (t.conj & 1) ? rec.am : rec.an;
+                const uint32_t bmo = (t.conj & 2) ? rec.bn : rec.bm,
+                               bno = (t.conj & 2) ? rec.bm : rec.bn;
+                WinGroup g{};
+                g.c_off = co.data_off + ci.shift[xc] + cn.stride[il];
+                g.ldc = ci.nket[xc];
+                g.rows = amo * bmo;
+                g.cols = ano * bno;
+                const uint64_t sr = cn.stride[il] / ci.nket[xc],
+                               sc = cn.stride[il] % ci.nket[xc];
+                if (sr + g.rows > ci.nbra[xc] || sc + g.cols > ci.nket[xc])
+                    outside = 1;
+                *out++ = std::make_pair(g, rec);
+            }
+        }
+    };
+    {
+        // terms split into jobs of about equal record counts
+        const size_t nj = std::min<size_t>(8, at[nt] / (1u << 20) + 1);
+        std::vector<std::function<void()>> jobs;
+        int t0 = 0;
+        for (size_t j = 0; j < nj; j++) {
+            const uint64_t target = at[nt] * (j + 1) / nj;
+            int t1 = t0;
+            while (t1 < nt && (j + 1 == nj || at[t1] < target))
+                t1++;
+            jobs.emplace_back([&fill, t0, t1]() { fill(t0, t1); });
+            t0 = t1;
+        }
+        run_host_jobs(jobs);
+    }
+    B2G_REQUIRE(!outside, B2G_ERR_ARG, "tensor-product window outside its block");
+    tm.mark("records");
     if ((r = winplan_finish(w.get(), items)) != B2G_OK)
         return r;
     *out = w.release();
