@@ -210,22 +210,6 @@ void build_tp_connections(Connections &ci, Symm s, const SU2Coupling &cg,
         ci.quanta[i] = subdq[i].second;
 }
 
-namespace {
-struct WinKey {
-    uint64_t off;
-    uint32_t rows, cols, ld;
-    bool operator==(const WinKey &o) const {
-        return off == o.off && rows == o.rows && cols == o.cols && ld == o.ld;
-    }
-};
-struct WinKeyHash {
-    size_t operator()(const WinKey &k) const {
-        return (size_t)(k.off * 0x9E3779B97F4A7C15ull ^
-                        ((uint64_t)k.rows << 32 | k.cols) ^ k.ld);
-    }
-};
-} // namespace
-
 bool WinPlan::finish(std::vector<std::pair<WinGroup, WinRec>> &items,
                      std::string &err) {
     // runs of consecutive records with one window, each run's group
@@ -256,6 +240,7 @@ bool WinPlan::finish(std::vector<std::pair<WinGroup, WinRec>> &items,
             });
         run_host_jobs(jobs);
     }
+    tm.mark("run scan");
     std::vector<uint32_t> run_at, run_gid; // first item and group of a run
     {
         size_t nr = 0;
@@ -286,7 +271,10 @@ bool WinPlan::finish(std::vector<std::pair<WinGroup, WinRec>> &items,
             });
         run_host_jobs(jobs);
     }
+    tm.mark("run windows gathered");
     std::atomic<int> repeated{0}, clash{0};
+    // the first run of each run's window
+    std::vector<uint32_t> run_first(nrun0);
     {
         const size_t nh = std::min<size_t>(8, nrun0 / (1u << 18) + 1);
         std::vector<std::function<void()>> jobs;
@@ -299,6 +287,7 @@ bool WinPlan::finish(std::vector<std::pair<WinGroup, WinRec>> &items,
                     if ((h >> 40) % nh != j)
                         continue;
                     auto r = seen.emplace(g.c_off, (uint32_t)q);
+                    run_first[q] = *r.first;
                     if (r.second)
                         continue;
                     const WinGroup &o = rw[*r.first];
@@ -310,6 +299,7 @@ bool WinPlan::finish(std::vector<std::pair<WinGroup, WinRec>> &items,
             });
         run_host_jobs(jobs);
     }
+    tm.mark("window check");
     if (clash) {
         err = "two windows of different shapes start at one element";
         return false;
@@ -325,17 +315,18 @@ bool WinPlan::finish(std::vector<std::pair<WinGroup, WinRec>> &items,
             cnt[q] = (q + 1 < nrun0 ? run_at[q + 1] : (uint32_t)N) - run_at[q];
         }
     } else {
-        FlatMap<WinKey, uint32_t, WinKeyHash> idx(1024);
+        // groups in first-appearance order: a run that starts its window
+        // opens a group, a later run of it joins that group
         for (size_t q = 0; q < nrun0; q++) {
-            const WinGroup &g = rw[q];
-            WinKey k{g.c_off, g.rows, g.cols, g.ldc};
-            auto r = idx.emplace(k, (uint32_t)gs.size());
-            if (r.second) {
-                gs.push_back(g);
+            const uint32_t f = run_first[q];
+            if (f == q) {
+                run_gid[q] = (uint32_t)gs.size();
+                gs.push_back(rw[q]);
                 cnt.push_back(0);
+            } else {
+                run_gid[q] = run_gid[f];
             }
-            run_gid[q] = *r.first;
-            cnt[*r.first] +=
+            cnt[run_gid[q]] +=
                 (q + 1 < nrun0 ? run_at[q + 1] : (uint32_t)N) - run_at[q];
         }
     }
