@@ -249,7 +249,7 @@ static int win_block(b2g_ctx ctx, size_t bytes, void **out, size_t *len) {
 constexpr size_t kWinPoolCap = (size_t)8 << 30;
 
 static int winplan_finish(b2g_winplan w,
-                          std::vector<std::pair<WinGroup, WinRec>> &items) {
+                          WinItems &items) {
     std::string err;
     SetupTimer tm("window plan");
     B2G_REQUIRE(w->plan.finish(items, err), B2G_ERR_ARG, err);
@@ -422,7 +422,7 @@ extern "C" int b2g_contract_plan(b2g_ctx ctx, const b2g_contract *c,
         cls_b[it] = ib, cls_e[it] = ie;
         at[it + 1] += (uint64_t)(ie - ib);
     }
-    std::vector<std::pair<WinGroup, WinRec>> items(at[nt]);
+    WinItems items(at[nt]);
     std::atomic<int> outside{0};
     auto fill = [&](int t0, int t1) {
         for (int it = t0; it < t1; it++) {
@@ -433,7 +433,7 @@ extern "C" int b2g_contract_plan(b2g_ctx ctx, const b2g_contract *c,
                             &bi = B.infos[bo.info];
             const double scale = t.factor * ao.factor * bo.factor;
             const Connections &cn = w->tp[co.info];
-            std::pair<WinGroup, WinRec> *out = items.data() + at[it];
+            WinItem *out = items.data() + at[it];
             for (int il = cls_b[it]; il < cls_e[it]; il++) {
                 const uint32_t xa = cn.ia[il], xb = cn.ib[il],
                                xc = cn.ic[il];
@@ -457,7 +457,7 @@ extern "C" int b2g_contract_plan(b2g_ctx ctx, const b2g_contract *c,
                                sc = cn.stride[il] % ci.nket[xc];
                 if (sr + g.rows > ci.nbra[xc] || sc + g.cols > ci.nket[xc])
                     outside = 1;
-                *out++ = std::make_pair(g, rec);
+                *out++ = WinItem(g, rec);
             }
         }
     };
@@ -543,7 +543,7 @@ extern "C" int b2g_transform_plan(b2g_ctx ctx, int symmetry, int n_infos,
     w->ctx = ctx;
     w->host_only = host_only;
     w->b_is_one = true;
-    std::vector<std::pair<WinGroup, WinRec>> items;
+    WinItems items;
     std::vector<uint8_t> written(n_ops, 0), read(n_ops, 0);
     // per (c info, a info, conj): the a block of each c block (-1: none)
     // and its transpose factor, shared by every term with those infos
@@ -599,6 +599,7 @@ extern "C" int b2g_transform_plan(b2g_ctx ctx, int symmetry, int n_infos,
         }
         by_dest[dest_of[t.c]].push_back(it);
     }
+    tm.mark("destinations");
     // Per destination: whether its terms sum as whole operators, its block
     // maps (built here, on this thread) and its record count; then the
     // records of the destinations are written on the host workers into
@@ -632,13 +633,14 @@ extern "C" int b2g_transform_plan(b2g_ctx ctx, int symmetry, int n_infos,
         }
         at[d + 1] = at[d] + n;
     }
+    tm.mark("block maps and counts");
     items.resize(at[nd]);
     std::atomic<const char *> shape_err{nullptr};
     auto fill = [&](size_t d0, size_t d1) {
         for (size_t d = d0; d < d1; d++) {
             const Operator &co = O.ops[dest_order[d]];
             const BlockInfo &ci = O.infos[co.info];
-            std::pair<WinGroup, WinRec> *out = items.data() + at[d];
+            WinItem *out = items.data() + at[d];
             if (whole[d]) {
                 const uint32_t len = (uint32_t)ci.total();
                 if (len == 0)
@@ -656,7 +658,7 @@ extern "C" int b2g_transform_plan(b2g_ctx ctx, int symmetry, int n_infos,
                     g.c_off = co.data_off;
                     g.ldc = len;
                     g.rows = rec.am, g.cols = len;
-                    *out++ = std::make_pair(g, rec);
+                    *out++ = WinItem(g, rec);
                 }
                 continue;
             }
@@ -686,7 +688,7 @@ extern "C" int b2g_transform_plan(b2g_ctx ctx, int symmetry, int n_infos,
                     g.cols = t.conj ? rec.am : rec.an;
                     if (g.rows != ci.nbra[xc] || g.cols != ci.nket[xc])
                         shape_err = "transform block shapes differ";
-                    *out++ = std::make_pair(g, rec);
+                    *out++ = WinItem(g, rec);
                 }
         }
     };
@@ -769,7 +771,7 @@ extern "C" int b2g_swap_plan(b2g_ctx ctx, int symmetry, int to_left,
     w->ctx = ctx;
     w->host_only = host_only;
     w->b_is_one = true;
-    std::vector<std::pair<WinGroup, WinRec>> items;
+    WinItems items;
     if (to_left) {
         // [l x (m r)] -> [(l m) x r]
         std::map<std::pair<uint32_t, uint32_t>,

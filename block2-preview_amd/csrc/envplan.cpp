@@ -210,7 +210,7 @@ void build_tp_connections(Connections &ci, Symm s, const SU2Coupling &cg,
         ci.quanta[i] = subdq[i].second;
 }
 
-bool WinPlan::finish(std::vector<std::pair<WinGroup, WinRec>> &items,
+bool WinPlan::finish(WinItems &items,
                      std::string &err) {
     // runs of consecutive records with one window, each run's group
     // (windows in first-appearance order), then the records of every group

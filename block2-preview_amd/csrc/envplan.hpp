@@ -9,6 +9,7 @@
 #include <cstdint>
 #include <memory>
 #include <string>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -101,6 +102,18 @@ template <typename T> struct NoInitAlloc : std::allocator<T> {
     }
 };
 
+// one record with its destination window, as the plan builders emit them
+// (trivially default-constructible: sized arrays of them are not zeroed)
+struct WinItem {
+    WinGroup first;
+    WinRec second;
+    WinItem() = default;
+    WinItem(const WinGroup &g, const WinRec &r) : first(g), second(r) {}
+};
+static_assert(std::is_trivially_default_constructible<WinItem>::value,
+              "window items are sized without zero-filling");
+using WinItems = std::vector<WinItem, NoInitAlloc<WinItem>>;
+
 struct WinPlan {
     std::vector<WinGroup> groups;
     std::vector<WinRec, NoInitAlloc<WinRec>> recs;
@@ -112,7 +125,7 @@ struct WinPlan {
     // group the records by destination window (equal c_off and shape), in
     // first-appearance order of the windows and record order within one;
     // fails if two different windows overlap
-    bool finish(std::vector<std::pair<WinGroup, WinRec>> &items,
+    bool finish(WinItems &items,
                 std::string &err);
 };
 
