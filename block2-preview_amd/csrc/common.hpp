@@ -118,6 +118,10 @@ void run_host_jobs(std::vector<std::function<void()>> &jobs);
 // memcpy of a large host range split over the host workers (staging copies
 // of plan uploads: one thread moves about 6-10 GB/s)
 void parallel_memcpy(void *dst, const void *src, size_t bytes);
+// number of host jobs (at most 8) for `items` units of plan-building work
+// of about `per_job` units each (B2G_PLAN_JOB_ITEMS divides per_job down to
+// that many items, for small plans split on purpose)
+size_t plan_jobs(uint64_t items, uint64_t per_job);
 
 class SetupTimer {
   public:

@@ -463,7 +463,7 @@ extern "C" int b2g_contract_plan(b2g_ctx ctx, const b2g_contract *c,
     };
     {
         // terms split into jobs of about equal record counts
-        const size_t nj = std::min<size_t>(8, at[nt] / (1u << 20) + 1);
+        const size_t nj = plan_jobs(at[nt], 1u << 20);
         std::vector<std::function<void()>> jobs;
         int t0 = 0;
         for (size_t j = 0; j < nj; j++) {
@@ -694,7 +694,7 @@ extern "C" int b2g_transform_plan(b2g_ctx ctx, int symmetry, int n_infos,
     };
     {
         // destinations split into jobs of about equal record counts
-        const size_t nj = std::min<size_t>(8, at[nd] / (1u << 20) + 1);
+        const size_t nj = plan_jobs(at[nd], 1u << 20);
         std::vector<std::function<void()>> jobs;
         size_t d0 = 0;
         for (size_t j = 0; j < nj; j++) {

@@ -220,7 +220,7 @@ bool WinPlan::finish(WinItems &items,
     SetupTimer tm("window plan finish");
     // run starts, found on the host workers over slices of the items
     const size_t N = items.size();
-    const size_t nj = std::min<size_t>(8, N / (1u << 20) + 1);
+    const size_t nj = plan_jobs(N, 1u << 20);
     std::vector<std::vector<uint32_t>> slice_runs(nj);
     {
         std::vector<std::function<void()>> jobs;
@@ -262,7 +262,7 @@ bool WinPlan::finish(WinItems &items,
     // the runs' windows, gathered into one compact array on the workers
     std::vector<WinGroup> rw(nrun0);
     {
-        const size_t nc = std::min<size_t>(8, nrun0 / (1u << 18) + 1);
+        const size_t nc = plan_jobs(nrun0, 1u << 18);
         std::vector<std::function<void()>> jobs;
         for (size_t j = 0; j < nc; j++)
             jobs.emplace_back([&, j, nc]() {
@@ -276,7 +276,7 @@ bool WinPlan::finish(WinItems &items,
     // the first run of each run's window
     std::vector<uint32_t> run_first(nrun0);
     {
-        const size_t nh = std::min<size_t>(8, nrun0 / (1u << 18) + 1);
+        const size_t nh = plan_jobs(nrun0, 1u << 18);
         std::vector<std::function<void()>> jobs;
         for (size_t j = 0; j < nh; j++)
             jobs.emplace_back([&, j, nh]() {

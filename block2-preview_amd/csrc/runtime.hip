@@ -757,6 +757,15 @@ void run_host_jobs(std::vector<std::function<void()>> &jobs) {
     w.cv.wait(g, [&w] { return w.left == 0; });
 }
 
+size_t plan_jobs(uint64_t items, uint64_t per_job) {
+    if (const char *e = getenv("B2G_PLAN_JOB_ITEMS")) {
+        const long long v = atoll(e);
+        if (v > 0)
+            per_job = std::min<uint64_t>(per_job, (uint64_t)v);
+    }
+    return (size_t)std::min<uint64_t>(8, items / std::max<uint64_t>(per_job, 1) + 1);
+}
+
 void parallel_memcpy(void *dst, const void *src, size_t bytes) {
     const size_t piece = (size_t)64 << 20;
     const size_t n = std::min<size_t>(8, (bytes + piece - 1) / piece);
