@@ -113,7 +113,8 @@ struct U64Hash {
 // Runs jobs[0] on the calling thread and the other jobs on persistent host
 // worker threads (created on first use; their thread-local tiling memos stay
 // warm from one plan build to the next) and returns when all have finished.
-// Jobs report errors through their captures and must not throw.
+// Jobs report errors through their captures and must not throw, and must
+// not call run_host_jobs themselves (waiting jobs could hold every worker).
 void run_host_jobs(std::vector<std::function<void()>> &jobs);
 // memcpy of a large host range split over the host workers (staging copies
 // of plan uploads: one thread moves about 6-10 GB/s)
