@@ -343,6 +343,40 @@ def abinitio_leg(bond_dims):
     return out
 
 
+def winsum_profile(args):
+    """n=26 sweeps (M = 50 ... 1000, FP64) with the algorithmic bytes of
+    every window-sum launch (one per window-plan run with windows, in launch
+    order: 8 bytes x (elements read + elements written)) printed for a
+    kernel-trace profile of winsum_kernel."""
+    import ctypes as C
+    from b2g import _lib as L
+    from b2g import dmrg as D
+    from b2g.effham import Context
+    ctx = Context(0)
+    rec = []
+    lib = L.lib()
+    run = lib.b2g_winplan_run
+
+    def counted(h, a, b, c, accumulate):
+        v = [C.c_uint64() for _ in range(5)]
+        L.check(lib.b2g_winplan_stats(h, *[C.byref(x) for x in v]))
+        if v[0].value > 0:
+            rec.append(("run", 8 * (v[2].value + v[3].value)))
+        return run(h, a, b, c, accumulate)
+    lib.b2g_winplan_run = counted
+    dm = D.DMRG(ctx, "su2", os.path.join(C2_DIR, "mpo.bin"), C2_DIR,
+                [50, 100, 250, 500, 1000], [0.0])
+    dm.solve(5, True, 0.0, None)
+    print(json.dumps({"workload": "n=26 SU(2) sweeps, M = 50/100/250/500/"
+                                  "1000, FP64",
+                      "winsum_launches": len(rec),
+                      "phases": [p for p, _ in rec],
+                      "algorithmic_bytes": [b for _, b in rec]}))
+    dm.close()
+    ctx.close()
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -374,6 +408,10 @@ def main():
                     help=argparse.SUPPRESS)
     ap.add_argument("--abinitio-bdims", default="50,100,250,500,1000",
                     help=argparse.SUPPRESS)
+    ap.add_argument("--winsum-profile", action="store_true",
+                    help="only run the n=26 sweeps in FP64 and print the "
+                         "algorithmic bytes of every window-sum launch (for "
+                         "a kernel-trace profile) and exit")
     ap.add_argument("--davidson-profile", action="store_true",
                     help="only run one FP64 Davidson solve on the site and "
                          "print the algorithmic bytes of its bandwidth-bound "
@@ -384,6 +422,8 @@ def main():
             args.abinitio_run,
             [int(b) for b in args.abinitio_bdims.split(",")])), flush=True)
         return 0
+    if args.winsum_profile:
+        return winsum_profile(args)
     if args.davidson_profile:
         return davidson_profile(args)
     if args.calibrate_copy:
